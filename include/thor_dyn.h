@@ -1,0 +1,180 @@
+/*
+ * thor_dyn.h — C ABI of the MI355X-native HE-VI dynamical core.
+ *
+ * Drop-in boundary for THOR's dynamics step. A caller (the project's ESP
+ * driver, a THOR build, or a Python/ctypes harness) owns the model state and
+ * hands it over in THOR's own column-contiguous layout; the library keeps a
+ * level-planar copy on the GPU and runs the step with hand-written gfx950
+ * kernels. All functions return a thor_status (0 = OK); thor_last_error()
+ * describes the most recent failure on the calling thread. The library never
+ * calls exit().
+ *
+ * Reference interfaces replaced (paths relative to exoclime/THOR):
+ *   thor_dyn_thor          <- ESP::Thor(const SimulationSetup&, kernel_diagnostics&)
+ *                             src/headers/esp.h:466, src/ESP/thor_driver.cu:70-1466
+ *   thor_dyn_profx         <- ESP::ProfX(...) with core_benchmark = HeldSuarez
+ *                             src/headers/esp.h:468, src/ESP/profx_driver.cu:70-474
+ *   thor_dyn_globdiag      <- ESP::globdiag(const SimulationSetup&)
+ *                             src/ESP/profx_driver.cu:476-569
+ *   thor_dyn_set_state /
+ *   thor_dyn_get_state     <- the device arrays Mh_d, Wh_d, W_d, Rho_d, pressure_d owned by
+ *                             class ESP (src/headers/esp.h:224-371) and their host copies
+ *                             (ESP::copy_to_host, src/ESP/esp_initial.cu)
+ *   thor_grid_build        <- Icogrid::Icogrid(...) src/ESP/grid.cu:85-302
+ *   thor_ic_rest           <- ESP::initial_values(...) rest branch, src/ESP/esp_initial.cu:502-1139
+ *   thor_dyn_create        <- ESP::ESP(...) + ESP::alloc_data + the Kd set-up of
+ *                             ESP::initial_values (src/ESP/esp_initial.cu:68-483, 1292-1392)
+ */
+#ifndef THOR_DYN_H
+#define THOR_DYN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int thor_status;
+enum {
+    THOR_OK = 0,
+    THOR_ERR_BAD_ARG = -1,
+    THOR_ERR_HIP = -2,
+    THOR_ERR_NAN = -3,
+    THOR_ERR_NO_DEVICE = -4,
+    THOR_ERR_IO = -5,
+    THOR_ERR_UNSUPPORTED = -6
+};
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *thor_last_error(void);
+
+/* ------------------------------------------------------------------ grid */
+
+typedef struct thor_grid thor_grid;
+
+/* Geometry in THOR's per-point layout (src/headers/grid.h:63-129):
+ * point_local[P][6], maps[nr][18][18], div/grad[P][7][3], areas[P][6][3],
+ * areasTr[P][6], areasT[P], nvecoa/nvecti/nvecte[P][6][3], func_r[P][3],
+ * lonlat[P][2], Altitude[nv], Altitudeh[nv+1]. Poles are points P-2, P-1. */
+typedef struct thor_grid_desc {
+    int point_num, nv, nr, nl_region, glevel;
+    int pent_ind[12];
+    const int *point_local;
+    const int *maps;
+    const double *div, *grad, *areas, *areasTr, *areasT;
+    const double *nvecoa, *nvecti, *nvecte, *func_r, *lonlat;
+    const double *Altitude, *Altitudeh;
+    const double *xyz, *xyzq;
+} thor_grid_desc;
+
+/* Builds the icosahedral grid (spring dynamics optional) and its operators.
+ * If cache_dir is not NULL, the horizontal construction is read from / written
+ * to cache_dir/thor_grid_g<level>_<spring>_<beta>.bin. */
+thor_status thor_grid_build(int glevel, int nv, double radius, double top_altitude, int spring_dynamics,
+                            double spring_beta, const char *cache_dir, thor_grid **out);
+thor_status thor_grid_describe(const thor_grid *g, thor_grid_desc *out);
+/* Iterations the spring dynamics took (0 if disabled or loaded from cache). */
+int thor_grid_spring_iterations(const thor_grid *g);
+void thor_grid_destroy(thor_grid *g);
+
+/* ------------------------------------------------------------ parameters */
+
+enum { THOR_PRECISION_FP64 = 0, THOR_PRECISION_MIXED = 1 };
+
+/* Scalars of SimulationSetup (src/headers/simulation_setup.h:54-119) and of
+ * the ESP object that the dynamics step reads, plus the acoustic substep count
+ * (the reference hard-codes 6, thor_driver.cu:91-92) and the precision mode. */
+typedef struct thor_params {
+    double A;        /* planet radius [m] */
+    double Omega;    /* rotation rate [1/s] */
+    double Gravit;   /* gravity [m/s2] */
+    double P_Ref;    /* reference pressure [Pa] */
+    double Rd;       /* gas constant [J/kg/K] */
+    double Cp;       /* heat capacity [J/kg/K] */
+    double Tmean;    /* initial isothermal temperature [K] */
+    double Diffc;    /* hyperdiffusion coefficient */
+    double DivDampc; /* divergence-damping coefficient */
+    double timestep; /* large time step [s] */
+    int glevel;
+    int HyDiff, HyDiffOrder, DivDampP;
+    int DeepModel, NonHydro, GravHeightVar;
+    int n_substeps; /* acoustic substeps per large step (reference: 6) */
+    int precision;  /* THOR_PRECISION_FP64 or THOR_PRECISION_MIXED */
+} thor_params;
+
+/* Fills p with THOR's defaults for the Earth Held-Suarez case (ifile/earth_hstest.thr). */
+void thor_params_default_earth(thor_params *p);
+
+/* Rest initial condition: each column is the isothermal hydrostatic profile
+ * of the reference's Newton solve (esp_initial.cu:520-587); then, if
+ * temperature_noise > 0, T += U(-a, a) per cell at fixed pressure with rho
+ * recomputed (precedent esp_initial.cu:1064-1071), and if wind_noise > 0 each
+ * velocity component gets U(-w, w) with the radial part removed, Mh = rho*v.
+ * Noise comes from std::mt19937_64(seed), u = (x >> 11) * 2^-53.
+ * Outputs are host arrays in THOR's layout: Mh[P][nv][3], Wh[P][nv+1],
+ * W[P][nv], Rho[P][nv], pressure[P][nv]. */
+thor_status thor_ic_rest(const thor_grid_desc *g, const thor_params *p, unsigned long long seed,
+                         double temperature_noise, double wind_noise, double *Mh, double *Wh, double *W,
+                         double *Rho, double *pressure);
+
+/* ------------------------------------------------------ dynamical core */
+
+typedef struct thor_dyn thor_dyn;
+
+/* Model state in THOR's layout (see thor_ic_rest). on_device = 0: host
+ * pointers; 1: device pointers (HIP, same device as the handle). */
+typedef struct thor_state {
+    int on_device;
+    double *Mh, *Wh, *W, *Rho, *pressure;
+} thor_state;
+
+/* Uploads the geometry, allocates state and scratch on the current HIP
+ * device and computes the Kdh4/Kdhz profiles (esp_initial.cu:1292-1321). */
+thor_status thor_dyn_create(const thor_grid_desc *g, const thor_params *p, thor_dyn **out);
+void thor_dyn_destroy(thor_dyn *h);
+
+thor_status thor_dyn_set_state(thor_dyn *h, const thor_state *s);
+thor_status thor_dyn_get_state(thor_dyn *h, const thor_state *s);
+
+/* One large step of the dynamical core (ESP::Thor). Asynchronous. */
+thor_status thor_dyn_thor(thor_dyn *h);
+
+enum { THOR_FORCING_NONE = 0, THOR_FORCING_HELD_SUAREZ = 1 };
+/* ESP::ProfX before the dynamics: W from Wh, T and theta from p and rho, the
+ * benchmark forcing, then p = rho*Rd*T. Asynchronous. */
+thor_status thor_dyn_profx(thor_dyn *h, int forcing);
+
+/* nsteps x (ProfX + Thor) without host synchronisation. */
+thor_status thor_dyn_run(thor_dyn *h, int nsteps, int forcing);
+
+/* Waits for the handle's stream; reports THOR_ERR_NAN if the pressure NaN
+ * flag was raised since the last sync (the reference exits instead,
+ * thor_driver.cu:1390-1396). */
+thor_status thor_dyn_sync(thor_dyn *h);
+
+/* Global mass, total energy and angular momentum x, y, z (FP64 on the GPU,
+ * per-cell values summed in the reference's tree order). Synchronous. */
+thor_status thor_dyn_globdiag(thor_dyn *h, double out[5]);
+
+/* Copies a named field to / from the host in THOR's layout. Names: the state
+ * ("Mh","Wh","W","Rho","pressure"), diagnostics of the last step
+ * ("temperature","pt","h","hh","pth","gtil","gtilh"), forcing inputs
+ * ("profx_Qheat","profx_dMh","profx_dWh"). n is the element count. */
+thor_status thor_dyn_get_field(thor_dyn *h, const char *name, double *dst, long n);
+thor_status thor_dyn_set_field(thor_dyn *h, const char *name, const double *src, long n);
+
+/* Per-kernel timing with HIP events recorded on the handle's stream.
+ * enable = 1 starts collection (resets counters). */
+thor_status thor_dyn_kernel_timing(thor_dyn *h, int enable);
+/* Number of distinct kernels timed so far; index i gives its name, launch
+ * count, total milliseconds and algorithmic bytes per launch. */
+int thor_dyn_kernel_stat_count(thor_dyn *h);
+thor_status thor_dyn_kernel_stat(thor_dyn *h, int i, const char **name, long *launches, double *total_ms,
+                                 double *bytes_per_launch);
+
+/* Milliseconds of device time for n steps of thor_dyn_run, measured with HIP
+ * events around the whole sequence on the handle's stream. */
+thor_status thor_dyn_time_run(thor_dyn *h, int nsteps, int forcing, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* THOR_DYN_H */

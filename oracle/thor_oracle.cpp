@@ -1,0 +1,1507 @@
+// ORACLE — test infrastructure only (see thor_oracle.h).
+//
+// Each routine below restates one THOR kernel as a serial loop over the same
+// (block, thread) indices the CUDA launch uses, reading neighbours through the
+// rhombus maps exactly as the kernel's shared-memory halo load does
+// (src/headers/kernel_halo_helpers.h:47-128). Build with -ffp-contract=off so
+// every product and sum is rounded as written.
+#include "thor_oracle.h"
+#include "../thor_amd/csrc/portable_math.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace {
+
+const int NL = 16, NHL = 18, NHL2 = NHL * NHL;
+
+// Shared-memory positions a 16x16 block loads: its interior, the four halo
+// sides and the two corners (-1,-1), (16,16) (kernel_halo_helpers.h:73-120).
+std::vector<int> make_load_positions() {
+    std::vector<int> pos;
+    for (int y = 0; y < NL; y++)
+        for (int x = 0; x < NL; x++) pos.push_back((y + 1) * NHL + x + 1);
+    for (int x = 0; x < NL; x++) {
+        pos.push_back((x + 1) * NHL);                 // y==0 threads: left column
+        pos.push_back((x + 1) * NHL + NHL - 3 + 2);   // y==3 threads: right column
+        pos.push_back(x + 1);                         // y==7 threads: bottom row
+        pos.push_back((NHL - 3 + 2) * NHL + (x + 1)); // y==11 threads: top row
+    }
+    pos.push_back(0);
+    pos.push_back((NHL - 3 + 2) * NHL + (NHL - 3 + 2));
+    return pos;
+}
+const std::vector<int> kLoad = make_load_positions();
+
+struct Oracle {
+    oracle_grid g;
+    oracle_params p;
+    int P, nv, nvi, nr;
+    std::map<std::string, std::vector<double>> arr;
+    std::vector<double> Kdh4, Kdhz;
+
+    double *A_(const char *n) { return arr[n].data(); }
+};
+
+// Fills a shared tile: f(global index) for loaded positions whose map entry is
+// valid, 0.0 where the map holds -1 (missing pentagon corner).
+template <class F> inline void fill(double *sh, const int *m, F f) {
+    for (int pos : kLoad) {
+        int gi = m[pos];
+        sh[pos] = gi >= 0 ? f(gi) : 0.0;
+    }
+}
+
+struct Stencil {
+    int ir, id, iri, pt[7]; // pt[0] = ir, pt[1..6] = pt1..pt6
+    bool pent;
+};
+inline Stencil stencil(const int *m, int x, int y) {
+    Stencil s;
+    s.ir = (y + 1) * NHL + x + 1;
+    s.id = m[s.ir];
+    s.iri = y * NL + x;
+    s.pent = (x == 0 && y == 0 && m[0] == -1);
+    s.pt[0] = s.ir;
+    s.pt[1] = (y + 2) * NHL + x + 1;
+    s.pt[2] = (y + 2) * NHL + x + 2;
+    s.pt[3] = (y + 1) * NHL + x + 2;
+    s.pt[4] = (y)*NHL + x + 1;
+    s.pt[5] = s.pent ? ((y + 1) * NHL + x) : ((y)*NHL + x);
+    s.pt[6] = (y + 1) * NHL + x;
+    return s;
+}
+// The three ring neighbours of triangle j used by Diffusion_Op / DivM_Op
+// (thor_diff.h:257-289, thor_div.h:174-205).
+inline void tri_pts(int x, int y, int j, bool pent, int &p1, int &p2, int &p3) {
+    const int n1 = (y + 2) * NHL + x + 1, n2 = (y + 2) * NHL + x + 2, n3 = (y + 1) * NHL + x + 2;
+    const int n4 = (y)*NHL + x + 1, n5 = (y)*NHL + x, n6 = (y + 1) * NHL + x;
+    if (j == 0) { p1 = n1; p2 = n2; p3 = n3; }
+    else if (j == 1) { p1 = n2; p2 = n3; p3 = n4; }
+    else if (j == 2) { p1 = n3; p2 = n4; p3 = pent ? n6 : n5; }
+    else if (j == 3) { p1 = n4; p2 = pent ? n6 : n5; p3 = pent ? n1 : n6; }
+    else if (j == 4) { p1 = pent ? n6 : n5; p2 = pent ? n1 : n6; p3 = pent ? n2 : n1; }
+    else { p1 = n6; p2 = n1; p3 = n2; }
+}
+
+inline double rscale_of(const Oracle &o, int lev) {
+    return o.p.DeepModel ? o.p.A / (o.g.Altitude[lev] + o.p.A) : 1.0;
+}
+
+// ---------------------------------------------------------------- advection
+// Compute_Advec_Cori1 (thor_adv_cor.h:51-235).
+void advec_cori1(Oracle &o) {
+    const int nv = o.nv;
+    double *Adv = o.A_("Adv"), *v = o.A_("v"), *Mh = o.A_("Mhk"), *W = o.A_("Wk"), *Rho = o.A_("Rhok");
+    const double *div = o.g.div, *fr = o.g.func_r;
+    double Ms[3][NHL2], vs[3][NHL2];
+    for (int ib = 0; ib < o.nr; ib++) {
+        const int *m = o.g.maps + ib * NHL2;
+        for (int lev = 0; lev < nv; lev++) {
+            for (int pos : kLoad) {
+                int gi = m[pos];
+                if (gi >= 0) {
+                    double rho = Rho[gi * nv + lev], w = W[gi * nv + lev];
+                    for (int k = 0; k < 3; k++) Ms[k][pos] = Mh[(gi * nv + lev) * 3 + k];
+                    for (int k = 0; k < 3; k++) vs[k][pos] = Ms[k][pos] / rho + (w / rho) * fr[gi * 3 + k];
+                } else {
+                    for (int k = 0; k < 3; k++) Ms[k][pos] = vs[k][pos] = 0.0;
+                }
+            }
+            double rs = rscale_of(o, lev);
+            for (int y = 0; y < NL; y++)
+                for (int x = 0; x < NL; x++) {
+                    Stencil s = stencil(m, x, y);
+                    const double *d = div + s.id * 21;
+                    double nf[3] = {0.0, 0.0, 0.0};
+                    for (int kk = 0; kk < 3; kk++)     // divergence component
+                        for (int c = 0; c < 3; c++) {  // advected component
+                            double sum = d[0 * 3 + kk] * vs[c][s.pt[0]] * Ms[kk][s.pt[0]];
+                            for (int j = 1; j < 7; j++)
+                                sum = sum + d[j * 3 + kk] * vs[c][s.pt[j]] * Ms[kk][s.pt[j]];
+                            nf[c] += rs * sum;
+                        }
+                    for (int c = 0; c < 3; c++) {
+                        Adv[(s.id * nv + lev) * 3 + c] = nf[c];
+                        v[(s.id * nv + lev) * 3 + c] = vs[c][s.ir];
+                    }
+                }
+        }
+    }
+}
+
+// Compute_Advec_Cori_Poles (thor_adv_cor.h:241-362).
+void advec_poles(Oracle &o) {
+    const int nv = o.nv, P = o.P;
+    double *Adv = o.A_("Adv"), *v = o.A_("v"), *Mh = o.A_("Mhk"), *W = o.A_("Wk"), *Rho = o.A_("Rhok");
+    const double *div = o.g.div, *fr = o.g.func_r;
+    for (int id = P - 2; id < P; id++) {
+        int loc[6];
+        loc[0] = id;
+        for (int i = 0; i < 5; i++) loc[i + 1] = o.g.point_local[id * 6 + i];
+        for (int lev = 0; lev < nv; lev++) {
+            double vp[6][3], Mp[6][3];
+            for (int i = 0; i < 6; i++) {
+                int gi = loc[i];
+                double rho = Rho[gi * nv + lev], w = W[gi * nv + lev];
+                for (int k = 0; k < 3; k++) {
+                    Mp[i][k] = Mh[(gi * nv + lev) * 3 + k];
+                    vp[i][k] = Mp[i][k] / rho + (w / rho) * fr[gi * 3 + k];
+                }
+            }
+            double rs = rscale_of(o, lev);
+            double nf[3] = {0.0, 0.0, 0.0};
+            for (int k = 0; k < 3; k++)
+                for (int c = 0; c < 3; c++) {
+                    double sum = div[id * 21 + 0 * 3 + k] * vp[0][c] * Mp[0][k];
+                    for (int i = 1; i < 6; i++) sum = sum + div[id * 21 + i * 3 + k] * vp[i][c] * Mp[i][k];
+                    nf[c] += rs * sum;
+                }
+            for (int c = 0; c < 3; c++) {
+                Adv[(id * nv + lev) * 3 + c] = nf[c];
+                v[(id * nv + lev) * 3 + c] = vp[0][c];
+            }
+        }
+    }
+}
+
+// Compute_Advec_Cori2 (thor_adv_cor.h:366-536).
+void advec_cori2(Oracle &o) {
+    const int nv = o.nv;
+    double *Adv = o.A_("Adv"), *v = o.A_("v"), *Wh = o.A_("Whk"), *Rho = o.A_("Rhok");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *fr = o.g.func_r;
+    const double A = o.p.A, Omega = o.p.Omega;
+    for (int id = 0; id < o.P; id++) {
+        double altht = 0, althl = 0, altt = 0, alt = 0, altl = 0;
+        double wht = 0, whl = 0, vxl = 0, vx = 0, vxt = 0, vyl = 0, vy = 0, vyt = 0, vzl = 0, vz = 0, vzt = 0;
+        double dvwxl, dvwyl, dvwzl, dvwxt, dvwyt, dvwzt, intt, intl, xi, xim, xip;
+        for (int lev = 0; lev < nv; lev++) {
+            if (lev == 0) {
+                dvwxl = 0.0; dvwyl = 0.0; dvwzl = 0.0;
+                altht = Alth[lev + 1]; althl = Alth[lev]; alt = Alt[lev]; altt = Alt[lev + 1];
+                vx = v[(id * nv + lev) * 3 + 0]; vxt = v[(id * nv + lev + 1) * 3 + 0];
+                vy = v[(id * nv + lev) * 3 + 1]; vyt = v[(id * nv + lev + 1) * 3 + 1];
+                vz = v[(id * nv + lev) * 3 + 2]; vzt = v[(id * nv + lev + 1) * 3 + 2];
+                wht = Wh[id * (nv + 1) + lev + 1];
+                xi = altht; xim = alt; xip = altt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                dvwxt = (vx * intt + vxt * intl) * wht;
+                dvwyt = (vy * intt + vyt * intl) * wht;
+                dvwzt = (vz * intt + vzt * intl) * wht;
+            } else if (lev == nv - 1) {
+                dvwxt = 0.0; dvwyt = 0.0; dvwzt = 0.0;
+                xi = althl; xim = altl; xip = alt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                dvwxl = (vxl * intt + vx * intl) * whl;
+                dvwyl = (vyl * intt + vy * intl) * whl;
+                dvwzl = (vzl * intt + vz * intl) * whl;
+            } else {
+                xi = althl; xim = altl; xip = alt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                dvwxl = (vxl * intt + vx * intl) * whl;
+                dvwyl = (vyl * intt + vy * intl) * whl;
+                dvwzl = (vzl * intt + vz * intl) * whl;
+                xi = altht; xim = alt; xip = altt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                dvwxt = (vx * intt + vxt * intl) * wht;
+                dvwyt = (vy * intt + vyt * intl) * wht;
+                dvwzt = (vz * intt + vzt * intl) * wht;
+            }
+            double r2p, r2m, r2l;
+            if (o.p.DeepModel) {
+                r2p = MPOW(altht + A, 2.0);
+                r2m = MPOW(alt + A, 2.0);
+                r2l = MPOW(althl + A, 2.0);
+            } else {
+                r2p = r2m = r2l = 1.0;
+            }
+            double dz = 1.0 / ((althl - altht) * r2m);
+            double davx = (dvwxl * r2l - dvwxt * r2p) * dz;
+            double davy = (dvwyl * r2l - dvwyt * r2p) * dz;
+            double davz = (dvwzl * r2l - dvwzt * r2p) * dz;
+            double rho = Rho[id * nv + lev];
+            double Cx, Cy, Cz;
+            if (o.p.DeepModel) {
+                Cx = -2.0 * Omega * vy * rho;
+                Cy = 2.0 * Omega * vx * rho;
+                Cz = 0.0;
+            } else {
+                double f0 = fr[id * 3 + 0], f1 = fr[id * 3 + 1], f2 = fr[id * 3 + 2];
+                Cx = 2 * Omega * (vz * f2 * f1 - vy * MPOW(f2, 2)) * rho;
+                Cy = -2 * Omega * (vz * f2 * f0 - vx * MPOW(f2, 2)) * rho;
+                Cz = 2 * Omega * (vy * f2 * f0 - vx * f2 * f1) * rho;
+            }
+            Adv[(id * nv + lev) * 3 + 0] += davx + Cx;
+            Adv[(id * nv + lev) * 3 + 1] += davy + Cy;
+            Adv[(id * nv + lev) * 3 + 2] += davz + Cz;
+            if (lev < nv - 1) {
+                althl = altht;
+                altht = Alth[lev + 2];
+                altl = alt;
+                alt = altt;
+                if (lev < nv - 2) altt = Alt[lev + 2];
+                whl = wht;
+                if (lev < nv - 2) wht = Wh[id * (nv + 1) + lev + 2];
+                vxl = vx; vx = vxt;
+                if (lev < nv - 2) vxt = v[(id * nv + lev + 2) * 3 + 0];
+                vyl = vy; vy = vyt;
+                if (lev < nv - 2) vyt = v[(id * nv + lev + 2) * 3 + 1];
+                vzl = vz; vz = vzt;
+                if (lev < nv - 2) vzt = v[(id * nv + lev + 2) * 3 + 2];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------ thermodynamic diagnostics
+// Compute_Temperature_H_Pt_Geff, entropy form, calcT = true (thor_auxiliary.h:55-309).
+void temperature_h_pt_geff(Oracle &o) {
+    const int nv = o.nv;
+    double *T = o.A_("temperature"), *pk = o.A_("pressurek"), *Rho = o.A_("Rhok"), *h_d = o.A_("h"),
+           *hh = o.A_("hh"), *pt_d = o.A_("pt"), *pth = o.A_("pth"), *gtil = o.A_("gtil"),
+           *gtilh_d = o.A_("gtilh"), *Wh = o.A_("Whk"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    const double P_Ref = o.p.P_Ref, Gravit = o.p.Gravit, A = o.p.A;
+    const bool ghv = o.p.GravHeightVar;
+    for (int id = 0; id < o.P; id++) {
+        double temperature, rho = 0, rhoh, rhol = 0, rhop, pressure = 0, h = 0, hl = 0, pt = 0, ptl = 0,
+                                     pl = 0, pp, dpdz;
+        double gtilh = 0, gtilht, dz, dp_dz, Cv, CvoCp, xi, xim, xip, intt, intl, extr;
+        double alt = 0, altl = 0, alth = 0, altht = 0;
+        for (int lev = 0; lev < nv + 1; lev++) {
+            if (lev < nv) {
+                Cv = Cp[id * nv + lev] - Rd[id * nv + lev];
+                CvoCp = Cv / Cp[id * nv + lev];
+                if (lev > 0) {
+                    pl = pressure; rhol = rho; hl = h; ptl = pt; altl = alt;
+                }
+                pressure = pk[id * nv + lev];
+                rho = Rho[id * nv + lev];
+                temperature = pressure / (rho * Rd[id * nv + lev]);
+                T[id * nv + lev] = temperature;
+                alt = Alt[lev];
+                alth = Alth[lev];
+                altht = Alth[lev + 1];
+                h = Cv * temperature + pressure / rho;
+                h_d[id * nv + lev] = h;
+                pt = (P_Ref / (Rd[id * nv + lev] * rho)) * MPOW(pressure / P_Ref, CvoCp);
+                pt_d[id * nv + lev] = pt;
+            }
+            if (lev == 0) {
+                extr = (-alt - Alt[lev + 1]) / (Alt[lev + 1] - alt);
+                hh[id * (nv + 1) + 0] = h;
+                if (ghv)
+                    pl = pk[id * nv + 1] - rho * Gravit * MPOW(A / (A + Alt[0]), 2) * (-alt - Alt[1]);
+                else
+                    pl = pk[id * nv + 1] - rho * Gravit * (-alt - Alt[1]);
+                rhol = Rho[id * nv + 1] + (Rho[id * nv + 1] - rho) * extr;
+                rhoh = 0.5 * (rho + rhol);
+                dz = 2.0 * Alt[0];
+                dpdz = (pk[id * nv + 0] - pl) / dz;
+                gtilh_d[id * (nv + 1) + 0] = -(1.0 / rhoh) * dpdz;
+                gtilh = 0.0;
+            } else if (lev == nv) {
+                extr = (2 * Alth[nv] - alt - Alt[nv - 2]) / (Alt[nv - 1] - Alt[nv - 2]);
+                Cv = Cp[id * nv + lev - 1] - Rd[id * nv + lev - 1];
+                CvoCp = Cv / Cp[id * nv + lev - 1];
+                hh[id * (nv + 1) + nv] = h;
+                if (ghv)
+                    pp = pk[id * nv + nv - 2]
+                         - rho * Gravit * MPOW(A / (A + Alt[lev]), 2) * (2 * Alth[nv] - alt - Alt[nv - 2]);
+                else
+                    pp = pk[id * nv + nv - 2] - rho * Gravit * (2 * Alth[nv] - alt - Alt[nv - 2]);
+                if (pp < 0) pp = 0;
+                rhop = Rho[id * nv + nv - 2] + (Rho[id * nv + nv - 1] - Rho[id * nv + nv - 2]) * extr;
+                rhoh = 0.5 * (rho + rhop);
+                pth[id * (nv + 1) + nv] =
+                    pt_d[id * nv + nv - 2] + (pt_d[id * nv + nv - 1] - pt_d[id * nv + nv - 2]) * extr;
+                pth[id * (nv + 1) + 0] = pt_d[id * nv + 1]
+                                         + (pt_d[id * nv + 1] - pt_d[id * nv + 0]) * (Alth[0] - Alt[1])
+                                               / (Alt[1] - Alt[0]);
+                dz = 2.0 * (Alth[nv] - Alt[nv - 1]);
+                if (ghv)
+                    pp = pk[id * nv + nv - 2]
+                         - Rho[id * nv + nv - 1] * Gravit * MPOW(A / (A + Alt[nv - 1]), 2)
+                               * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+                else
+                    pp = pk[id * nv + nv - 2]
+                         - Rho[id * nv + nv - 1] * Gravit * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+                dpdz = (pp - pk[id * nv + nv - 1]) / dz;
+                gtilh_d[id * (nv + 1) + nv] = -(1.0 / rhoh) * dpdz;
+                gtilht = 0.0;
+                xi = alt; xim = alth; xip = altht;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                gtil[id * nv + nv - 1] = gtilh * intt + gtilht * intl;
+            } else {
+                dz = alt - altl;
+                dp_dz = (pressure - pl) / dz;
+                xi = alth; xim = altl; xip = alt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                hh[id * (nv + 1) + lev] = hl * intt + h * intl;
+                pth[id * (nv + 1) + lev] = ptl * intt + pt * intl;
+                rhoh = rhol * intt + rho * intl;
+                gtilht = -(Wh[id * (nv + 1) + lev] / rhoh) * dp_dz;
+                xi = alt; xim = alth; xip = altht;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                gtil[id * nv + lev - 1] = gtilh * intt + gtilht * intl;
+                gtilh = gtilht;
+                gtilh_d[id * (nv + 1) + lev] = -(1.0 / rhoh) * dp_dz;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- diffusion
+// One Laplacian around the control volume (thor_diff.h:359-494): returns the
+// accumulated lap for the six triangles (five for pentagons).
+template <class GetA>
+inline double laplacian(const Oracle &o, int id, double rscale, int n_faces, GetA a, const int ring[6],
+                        int aself) {
+    (void)aself;
+    const double *areas = o.g.areas + id * 18, *areasTr = o.g.areasTr + id * 6;
+    const double *nti = o.g.nvecti + id * 18, *nte = o.g.nvecte + id * 18, *noa = o.g.nvecoa + id * 18;
+    double lap = 0.0, lap1 = 0, lap2 = 0, lap3 = 0, lap4, lap5, lap6;
+    double lapx1 = 0, lapx2 = 0, lapy1 = 0, lapy2 = 0, lapz1 = 0, lapz2 = 0, AT1 = 0, AT2, astar;
+    const double a0 = a(-1);
+    for (int j = 0; j < 6; j++) {
+        int jp1 = (j + 1) % n_faces, jp2 = (j + 2) % n_faces;
+        double a1 = a(ring[j % n_faces]), a2 = a(ring[jp1]), a3 = a(ring[jp2]);
+        double vdiff = 0.5 * rscale;
+        if (j == 0) {
+            AT1 = rscale / areasTr[j];
+            astar = (areas[j * 3 + 0] * a0 + areas[j * 3 + 1] * a1 + areas[j * 3 + 2] * a2) / areasTr[j];
+            lap1 = (0.5 * (a1 + a0) - astar);
+            lap2 = (0.5 * (a1 + a2) - astar);
+            lap3 = (0.5 * (a0 + a2) - astar);
+        }
+        AT2 = rscale / areasTr[jp1];
+        astar = (areas[jp1 * 3 + 0] * a0 + areas[jp1 * 3 + 1] * a2 + areas[jp1 * 3 + 2] * a3) / areasTr[jp1];
+        lap4 = (0.5 * (a0 + a2) - astar);
+        lap5 = (0.5 * (a2 + a3) - astar);
+        lap6 = (0.5 * (a0 + a3) - astar);
+        if (j == 0) {
+            lapx1 = (-lap1 * nti[j * 3 + 0] + lap2 * nte[j * 3 + 0] + lap3 * nti[jp1 * 3 + 0]) * AT1;
+            lapx2 = (-lap4 * nti[jp1 * 3 + 0] + lap5 * nte[jp1 * 3 + 0] + lap6 * nti[jp2 * 3 + 0]) * AT2;
+            lapy1 = (-lap1 * nti[j * 3 + 1] + lap2 * nte[j * 3 + 1] + lap3 * nti[jp1 * 3 + 1]) * AT1;
+            lapy2 = (-lap4 * nti[jp1 * 3 + 1] + lap5 * nte[jp1 * 3 + 1] + lap6 * nti[jp2 * 3 + 1]) * AT2;
+            lapz1 = (-lap1 * nti[j * 3 + 2] + lap2 * nte[j * 3 + 2] + lap3 * nti[jp1 * 3 + 2]) * AT1;
+            lapz2 = (-lap4 * nti[jp1 * 3 + 2] + lap5 * nte[jp1 * 3 + 2] + lap6 * nti[jp2 * 3 + 2]) * AT2;
+        } else {
+            lapx1 = lapx2;
+            lapx2 = (-lap4 * nti[jp1 * 3 + 0] + lap5 * nte[jp1 * 3 + 0] + lap6 * nti[jp2 * 3 + 0]) * AT2;
+            lapy1 = lapy2;
+            lapy2 = (-lap4 * nti[jp1 * 3 + 1] + lap5 * nte[jp1 * 3 + 1] + lap6 * nti[jp2 * 3 + 1]) * AT2;
+            lapz1 = lapz2;
+            lapz2 = (-lap4 * nti[jp1 * 3 + 2] + lap5 * nte[jp1 * 3 + 2] + lap6 * nti[jp2 * 3 + 2]) * AT2;
+        }
+        lap += ((lapx1 + lapx2) * noa[j * 3 + 0] + (lapy1 + lapy2) * noa[j * 3 + 1]
+                + (lapz1 + lapz2) * noa[j * 3 + 2])
+               * vdiff;
+        if (n_faces == 5 && j == 4) break;
+    }
+    return lap;
+}
+
+// Diffusion_Op + Diffusion_Op_Poles for one pass (thor_diff.h:54-880).
+// stepnum == 0: argument is the fluid property; otherwise the previous result
+// (times rho on the last pass for var > 0). Reads a frozen copy of diff.
+void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
+    const int nv = o.nv, P = o.P;
+    double *diff = o.A_("diff"), *Mh = o.A_("Mhk"), *Rho = o.A_("Rhok"), *T = o.A_("temperature"),
+           *W = o.A_("Wk"), *Rd = o.A_("Rd");
+    double *diffmh = o.A_("diffmh"), *diffw = o.A_("diffw"), *diffrh = o.A_("diffrh"), *diffpr = o.A_("diffpr");
+    std::vector<double> diff_in(diff, diff + (size_t)6 * P * nv);
+    auto arg = [&](int gi, int lev, int var) -> double {
+        double rho = Rho[gi * nv + lev];
+        if (stepnum == 0) {
+            double a;
+            if (var == 0) a = rho;
+            else if (var <= 3) a = Mh[(gi * nv + lev) * 3 + var - 1];
+            else if (var == 4) a = W[gi * nv + lev];
+            else a = Rd[gi * nv + lev] * T[gi * nv + lev];
+            if (var >= 1 && var <= 4) a = a / rho;
+            return a;
+        } else if (laststep) {
+            if (var == 0) return diff_in[(gi * nv + lev) * 6 + var];
+            return diff_in[(gi * nv + lev) * 6 + var] * rho;
+        }
+        return diff_in[(gi * nv + lev) * 6 + var];
+    };
+    const double sign = std::pow(-1.0, o.p.HyDiffOrder / 2);
+    auto store = [&](int id, int lev, int var, double lap) {
+        if (laststep) {
+            if (var == 0) diffrh[id * nv + lev] = lap;
+            else if (var <= 3) diffmh[(id * nv + lev) * 3 + var - 1] = lap;
+            else if (var == 4) diffw[id * nv + lev] = lap;
+            else diffpr[id * nv + lev] = lap;
+        } else {
+            diff[(id * nv + lev) * 6 + var] = lap;
+        }
+    };
+    double as[NHL2];
+    for (int var = 0; var < 6; var++)
+        for (int ib = 0; ib < o.nr; ib++) {
+            const int *m = o.g.maps + ib * NHL2;
+            for (int lev = 0; lev < nv; lev++) {
+                fill(as, m, [&](int gi) { return arg(gi, lev, var); });
+                double rs = rscale_of(o, lev);
+                double sdiff = -o.Kdh4[lev] * sign;
+                for (int y = 0; y < NL; y++)
+                    for (int x = 0; x < NL; x++) {
+                        Stencil s = stencil(m, x, y);
+                        int nf = s.pent ? 5 : 6;
+                        int ring[6];
+                        for (int j = 0; j < 6; j++) {
+                            int p1, p2, p3;
+                            tri_pts(x, y, j, s.pent, p1, p2, p3);
+                            ring[j] = p1;
+                        }
+                        double lap = laplacian(o, s.id, rs, nf,
+                                               [&](int hp) { return hp < 0 ? as[s.ir] : as[hp]; }, ring, s.ir);
+                        if (laststep) lap *= sdiff;
+                        store(s.id, lev, var, lap);
+                    }
+            }
+        }
+    // Poles.
+    for (int var = 0; var < 6; var++)
+        for (int id = P - 2; id < P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                int ring[6];
+                for (int i = 0; i < 5; i++) ring[i] = o.g.point_local[id * 6 + i];
+                ring[5] = ring[4];
+                double rs = rscale_of(o, lev);
+                double sdiff = -o.Kdh4[lev] * sign;
+                double lap = laplacian(o, id, rs, 5,
+                                       [&](int gi) { return gi < 0 ? arg(id, lev, var) : arg(gi, lev, var); },
+                                       ring, id);
+                if (laststep) lap = sdiff * lap;
+                store(id, lev, var, lap);
+            }
+}
+
+// Correct_Horizontal (thor_diff.h:882-915); diffmv is identically zero here.
+void correct_horizontal(Oracle &o) {
+    const int nv = o.nv;
+    double *dm = o.A_("diffmh");
+    const double *fr = o.g.func_r;
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            double *d = dm + (id * nv + lev) * 3;
+            double r = fr[id * 3 + 0] * d[0] + fr[id * 3 + 1] * d[1] + fr[id * 3 + 2] * d[2];
+            d[0] += -fr[id * 3 + 0] * r;
+            d[1] += -fr[id * 3 + 1] * r;
+            d[2] += -fr[id * 3 + 2] * r;
+        }
+}
+
+// ------------------------------------------------------- divergence damping
+// DivM_Op + DivM_Op_Poles (thor_div.h:49-535). M/Wh are Mhk/Whk in the slow
+// part and Mhs/Whs in the acoustic substeps.
+void divm_pass(Oracle &o, const char *Mname, const char *Whname, bool laststep) {
+    const int nv = o.nv, P = o.P;
+    double *DivM = o.A_("DivM"), *divg = o.A_("divg_Mh"), *M = o.A_(Mname), *Wh = o.A_(Whname);
+    const double *areas = o.g.areas, *areasTr = o.g.areasTr, *nti = o.g.nvecti, *nte = o.g.nvecte,
+                 *noa = o.g.nvecoa, *fr = o.g.func_r, *Alth = o.g.Altitudeh;
+    double as[3][NHL2], wl[NHL2], wt[NHL2];
+    std::vector<double> divg_in;
+    if (laststep) divg_in.assign(divg, divg + (size_t)3 * P * nv);
+
+    // Core of the per-point computation for one ring (pentagon-aware).
+    auto point = [&](int id, int lev, int nfaces, bool pole, auto get_a, auto get_wl, auto get_wt, double out[3]) {
+        double rs = rscale_of(o, lev);
+        double AT0 = 0.5 * rs, lapx = 0.0, lapy = 0.0, lapz = 0.0;
+        double dz = laststep ? 0.0 : 1.0 / (Alth[lev] - Alth[lev + 1]);
+        double a01x = 0, a01y = 0, a01z = 0, a12x = 0, a12y = 0, a12z = 0, AT1 = 0, lap1 = 0, lap2 = 0;
+        const double *ar = areas + id * 18, *aT = areasTr + id * 6, *ti = nti + id * 18, *te = nte + id * 18,
+                     *oa = noa + id * 18;
+        for (int j = 0; j < nfaces; j++) {
+            int jp1 = (j + 1) % nfaces, jp2 = (j + 2) % nfaces;
+            // ring slots: 0 = centre, j+1.. = ring neighbours
+            double dwdz, dwdz2;
+            if (!laststep) {
+                double w1 = pole ? ar[j * 3 + 1] : ar[j * 3 + 2];
+                double w2 = pole ? ar[jp1 * 3 + 1] : ar[jp1 * 3 + 2];
+                double meanwl = (ar[j * 3 + 0] * get_wl(0) + ar[j * 3 + 1] * get_wl(j + 1) + w1 * get_wl(jp1 + 1)) / aT[j];
+                double meanwt = (ar[j * 3 + 0] * get_wt(0) + ar[j * 3 + 1] * get_wt(j + 1) + w1 * get_wt(jp1 + 1)) / aT[j];
+                double meanwl2 = (ar[jp1 * 3 + 0] * get_wl(0) + ar[jp1 * 3 + 1] * get_wl(jp1 + 1) + w2 * get_wl(jp2 + 1)) / aT[jp1];
+                double meanwt2 = (ar[jp1 * 3 + 0] * get_wt(0) + ar[jp1 * 3 + 1] * get_wt(jp1 + 1) + w2 * get_wt(jp2 + 1)) / aT[jp1];
+                dwdz = (meanwl - meanwt) * dz;
+                dwdz2 = (meanwl2 - meanwt2) * dz;
+            } else {
+                dwdz = 0.0;
+                dwdz2 = 0.0;
+            }
+            if (j == 0) {
+                a01x = get_a(0, 0) + get_a(j + 1, 0);
+                a01y = get_a(0, 1) + get_a(j + 1, 1);
+                a01z = get_a(0, 2) + get_a(j + 1, 2);
+                a12x = get_a(j + 1, 0) + get_a(jp1 + 1, 0);
+                a12y = get_a(j + 1, 1) + get_a(jp1 + 1, 1);
+                a12z = get_a(j + 1, 2) + get_a(jp1 + 1, 2);
+                AT1 = 0.5 * rs / aT[j];
+            }
+            double AT2 = 0.5 * rs / aT[jp1];
+            double a02x = get_a(0, 0) + get_a(jp1 + 1, 0), a02y = get_a(0, 1) + get_a(jp1 + 1, 1),
+                   a02z = get_a(0, 2) + get_a(jp1 + 1, 2);
+            double a23x = get_a(jp1 + 1, 0) + get_a(jp2 + 1, 0), a23y = get_a(jp1 + 1, 1) + get_a(jp2 + 1, 1),
+                   a23z = get_a(jp1 + 1, 2) + get_a(jp2 + 1, 2);
+            double a03x = get_a(0, 0) + get_a(jp2 + 1, 0), a03y = get_a(0, 1) + get_a(jp2 + 1, 1),
+                   a03z = get_a(0, 2) + get_a(jp2 + 1, 2);
+            if (j == 0) {
+                lap1 = (-a01x * ti[j * 3 + 0] - a01y * ti[j * 3 + 1] - a01z * ti[j * 3 + 2] + a12x * te[j * 3 + 0]
+                        + a12y * te[j * 3 + 1] + a12z * te[j * 3 + 2] + a02x * ti[jp1 * 3 + 0]
+                        + a02y * ti[jp1 * 3 + 1] + a02z * ti[jp1 * 3 + 2])
+                       * AT1;
+            } else {
+                lap1 = lap2;
+            }
+            lap2 = (-a02x * ti[jp1 * 3 + 0] - a02y * ti[jp1 * 3 + 1] - a02z * ti[jp1 * 3 + 2]
+                    + a23x * te[jp1 * 3 + 0] + a23y * te[jp1 * 3 + 1] + a23z * te[jp1 * 3 + 2]
+                    + a03x * ti[jp2 * 3 + 0] + a03y * ti[jp2 * 3 + 1] + a03z * ti[jp2 * 3 + 2])
+                   * AT2;
+            lapx += (lap1 + dwdz + lap2 + dwdz2) * oa[j * 3 + 0] * AT0;
+            lapy += (lap1 + dwdz + lap2 + dwdz2) * oa[j * 3 + 1] * AT0;
+            lapz += (lap1 + dwdz + lap2 + dwdz2) * oa[j * 3 + 2] * AT0;
+        }
+        out[0] = lapx; out[1] = lapy; out[2] = lapz;
+    };
+    auto finish = [&](int id, int lev, double l[3]) {
+        if (laststep) {
+            double sdiff = o.Kdhz[lev];
+            double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
+            double lapr = fx * l[0] + fy * l[1] + fz * l[2];
+            double lx = l[0] - fx * lapr, ly = l[1] - fy * lapr, lz = l[2] - fz * lapr;
+            DivM[(id * nv + lev) * 3 + 0] = -lx * sdiff;
+            DivM[(id * nv + lev) * 3 + 1] = -ly * sdiff;
+            DivM[(id * nv + lev) * 3 + 2] = -lz * sdiff;
+        } else {
+            for (int k = 0; k < 3; k++) divg[(id * nv + lev) * 3 + k] = l[k];
+        }
+    };
+    for (int ib = 0; ib < o.nr; ib++) {
+        const int *m = o.g.maps + ib * NHL2;
+        for (int lev = 0; lev < nv; lev++) {
+            for (int pos : kLoad) {
+                int gi = m[pos];
+                for (int k = 0; k < 3; k++)
+                    as[k][pos] = gi < 0 ? 0.0 : (laststep ? divg_in[(gi * nv + lev) * 3 + k] : M[(gi * nv + lev) * 3 + k]);
+                wl[pos] = (gi < 0 || laststep) ? 0.0 : Wh[gi * (nv + 1) + lev];
+                wt[pos] = (gi < 0 || laststep) ? 0.0 : Wh[gi * (nv + 1) + lev + 1];
+            }
+            for (int y = 0; y < NL; y++)
+                for (int x = 0; x < NL; x++) {
+                    Stencil s = stencil(m, x, y);
+                    int nfc = s.pent ? 5 : 6;
+                    int slot[7];
+                    slot[0] = s.ir;
+                    for (int j = 0; j < 6; j++) {
+                        int p1, p2, p3;
+                        tri_pts(x, y, j, s.pent, p1, p2, p3);
+                        slot[j + 1] = p1;
+                    }
+                    double l[3];
+                    point(s.id, lev, nfc, false, [&](int sl, int k) { return as[k][slot[sl]]; },
+                          [&](int sl) { return wl[slot[sl]]; }, [&](int sl) { return wt[slot[sl]]; }, l);
+                    finish(s.id, lev, l);
+                }
+        }
+    }
+    for (int id = P - 2; id < P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            int gslot[6];
+            gslot[0] = id;
+            for (int i = 0; i < 5; i++) gslot[i + 1] = o.g.point_local[id * 6 + i];
+            double l[3];
+            point(id, lev, 5, true,
+                  [&](int sl, int k) {
+                      int gi = gslot[sl];
+                      return laststep ? divg_in[(gi * nv + lev) * 3 + k] : M[(gi * nv + lev) * 3 + k];
+                  },
+                  [&](int sl) { return Wh[gslot[sl] * (nv + 1) + lev]; },
+                  [&](int sl) { return Wh[gslot[sl] * (nv + 1) + lev + 1]; }, l);
+            finish(id, lev, l);
+        }
+}
+
+// -------------------------------------------------------------- slow modes
+// Compute_Slow_Modes + _Poles (thor_slowmodes.h:48-683), entropy form; the
+// vertical-diffusion terms diffmv/diffwv/diffrv/diffprv are identically zero.
+void slow_modes(Oracle &o) {
+    const int nv = o.nv, P = o.P;
+    double *SlowMh = o.A_("SlowMh"), *SlowWh = o.A_("SlowWh"), *SlowRho = o.A_("SlowRho"),
+           *Slowp = o.A_("Slowpressure"), *Mh = o.A_("Mhk"), *Wh = o.A_("Whk"), *Rho = o.A_("Rhok"),
+           *Adv = o.A_("Adv"), *DivM = o.A_("DivM"), *diffmh = o.A_("diffmh"), *diffw = o.A_("diffw"),
+           *diffrh = o.A_("diffrh"), *diffpr = o.A_("diffpr"), *pk = o.A_("pressurek"), *h_d = o.A_("h"),
+           *hh = o.A_("hh"), *gtil = o.A_("gtil"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd"),
+           *dMh = o.A_("profx_dMh"), *dWh = o.A_("profx_dWh"), *Qh = o.A_("profx_Qheat");
+    const double *fr = o.g.func_r, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *div = o.g.div, *grad = o.g.grad;
+    const double A = o.p.A, Gravit = o.p.Gravit;
+
+    // Per-point body shared by tiles and poles; nb(k) gives the global index of
+    // stencil slot k (0 = self). pole selects the pole variant of dw/dz.
+    auto body = [&](int id, int lev, int nslots, bool pole, auto nb) {
+        double Cv = Cp[id * nv + lev] - Rd[id * nv + lev];
+        double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
+        double advx = Adv[(id * nv + lev) * 3 + 0], advy = Adv[(id * nv + lev) * 3 + 1],
+               advz = Adv[(id * nv + lev) * 3 + 2];
+        double rho = Rho[id * nv + lev];
+        double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
+        if (lev != 0) {
+            double ax = Adv[(id * nv + lev - 1) * 3 + 0], ay = Adv[(id * nv + lev - 1) * 3 + 1],
+                   az = Adv[(id * nv + lev - 1) * 3 + 2];
+            advrl = ax * fx + ay * fy + az * fz;
+            rhol = Rho[id * nv + lev - 1];
+            altl = Alt[lev - 1];
+            pressurel = pk[id * nv + lev - 1];
+        }
+        double advr = advx * fx + advy * fy + advz * fz;
+        advx += -advr * fx;
+        advy += -advr * fy;
+        advz += -advr * fz;
+        double altht = Alth[lev + 1], althl = Alth[lev], alt = Alt[lev];
+        if (!o.p.NonHydro) {
+            const double *M0 = Mh + (id * nv + lev) * 3;
+            advr = (MPOW(M0[0], 2) + MPOW(M0[1], 2) + MPOW(M0[2], 2)) / (rho * (alt + A));
+            if (lev != 0) {
+                const double *M1 = Mh + (id * nv + lev - 1) * 3;
+                advrl = (MPOW(M1[0], 2) + MPOW(M1[1], 2) + MPOW(M1[2], 2)) / (rhol * (altl + A));
+            }
+        }
+        double dz = altht - althl, r2p, r2m, r2l, rscale, dr2dz;
+        if (o.p.DeepModel) {
+            r2p = MPOW(altht + A, 2.0);
+            r2m = MPOW(alt + A, 2.0);
+            r2l = MPOW(althl + A, 2.0);
+            rscale = A / (alt + A);
+            dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3;
+        } else {
+            r2p = r2m = r2l = 1.0;
+            rscale = 1.0;
+            dr2dz = r2m * dz;
+        }
+        double nflxv[3], nflxr = 0.0, nflxp = 0.0;
+        for (int k = 0; k < 3; k++) {
+            double sg = grad[id * 21 + 0 * 3 + k] * pk[nb(0) * nv + lev];
+            double sr = div[id * 21 + 0 * 3 + k] * Mh[(nb(0) * nv + lev) * 3 + k];
+            double sp = div[id * 21 + 0 * 3 + k] * Mh[(nb(0) * nv + lev) * 3 + k] * h_d[nb(0) * nv + lev];
+            for (int j = 1; j < nslots; j++) {
+                int gi = nb(j);
+                sg = sg + grad[id * 21 + j * 3 + k] * pk[gi * nv + lev];
+                sr = sr + div[id * 21 + j * 3 + k] * Mh[(gi * nv + lev) * 3 + k];
+                sp = sp + div[id * 21 + j * 3 + k] * Mh[(gi * nv + lev) * 3 + k] * h_d[gi * nv + lev];
+            }
+            nflxv[k] = rscale * sg;
+            nflxr += rscale * sr;
+            nflxp += rscale * sp;
+        }
+        double whl, wht, hhl, hht;
+        if (lev == 0) {
+            whl = 0.0; wht = Wh[id * (nv + 1) + lev + 1]; hhl = 0.0; hht = hh[id * (nv + 1) + lev + 1];
+        } else {
+            whl = Wh[id * (nv + 1) + lev]; wht = Wh[id * (nv + 1) + lev + 1];
+            hhl = hh[id * (nv + 1) + lev]; hht = hh[id * (nv + 1) + lev + 1];
+        }
+        double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
+        double dwhdz = (wht * r2p * hht - whl * r2l * hhl) / (dz * r2m);
+        double dpr = nflxv[0] * fx + nflxv[1] * fy + nflxv[2] * fz;
+        const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
+        for (int k = 0; k < 3; k++) {
+            int e = (id * nv + lev) * 3 + k;
+            SlowMh[e] = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[e] + diffmh[e] + 0.0 + dMh[e];
+        }
+        if (lev == 0) {
+            SlowWh[id * (nv + 1) + 0] = 0.0;
+            SlowWh[id * (nv + 1) + nv] = 0.0;
+        } else {
+            double xi = althl, xim = altl, xip = alt;
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            double swr;
+            if (o.p.DeepModel) {
+                if (o.p.GravHeightVar)
+                    swr = (advrl + rhol * Gravit * MPOW(A / (A + altl), 2)) * intt
+                          + (advr + rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
+                else
+                    swr = (advrl + rhol * Gravit) * intt + (advr + rho * Gravit) * intl;
+            } else {
+                if (o.p.GravHeightVar)
+                    swr = (rhol * Gravit * MPOW(A / (A + altl), 2)) * intt
+                          + (rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
+                else
+                    swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
+            }
+            if (o.p.NonHydro)
+                swr += (-diffw[id * nv + lev - 1] - 0.0) * intt + (-diffw[id * nv + lev] - 0.0) * intl;
+            double dzl = alt - altl;
+            double dpdz = (pk[id * nv + lev] - pressurel) / dzl;
+            swr += dpdz;
+            swr -= dWh[id * (nv + 1) + lev];
+            SlowWh[id * (nv + 1) + lev] = -swr;
+        }
+        nflxr += dwdz;
+        SlowRho[id * nv + lev] = -nflxr + diffrh[id * nv + lev] + 0.0;
+        double r = 1.0 / rho;
+        const double *M0 = Mh + (id * nv + lev) * 3;
+        double vgp = (nflxv[0] - dpr * fx) * M0[0] * r + (nflxv[1] - dpr * fy) * M0[1] * r
+                     + (nflxv[2] - dpr * fz) * M0[2] * r;
+        Slowp[id * nv + lev] = (Rd[id * nv + lev] / Cv)
+                               * (-nflxp - gtil[id * nv + lev] - dwhdz + vgp + Qh[id * nv + lev]);
+        Slowp[id * nv + lev] += diffpr[id * nv + lev] + 0.0;
+    };
+    for (int ib = 0; ib < o.nr; ib++) {
+        const int *m = o.g.maps + ib * NHL2;
+        for (int lev = 0; lev < nv; lev++)
+            for (int y = 0; y < NL; y++)
+                for (int x = 0; x < NL; x++) {
+                    Stencil s = stencil(m, x, y);
+                    // A missing pentagon corner reads zeros through the halo; its
+                    // coefficient div6/grad6 is zero and pt5 is remapped, so the
+                    // -1 entry is never dereferenced.
+                    body(s.id, lev, 7, false, [&](int k) { return m[s.pt[k]]; });
+                }
+    }
+    for (int id = P - 2; id < P; id++) {
+        int gslot[6];
+        gslot[0] = id;
+        for (int i = 0; i < 5; i++) gslot[i + 1] = o.g.point_local[id * 6 + i];
+        for (int lev = 0; lev < nv; lev++) body(id, lev, 6, true, [&](int k) { return gslot[k]; });
+    }
+}
+
+// ------------------------------------------------------- RK bookkeeping
+// UpdateRK (thor_auxiliary.h:311-418): deviations = y(n) - y(k).
+void update_rk(Oracle &o) {
+    const int nv = o.nv;
+    double *Ms = o.A_("Mhs"), *Mk = o.A_("Mhk"), *Mi = o.A_("Mh"), *Whs = o.A_("Whs"), *Whk = o.A_("Whk"),
+           *Whi = o.A_("Wh"), *Ws = o.A_("Ws"), *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"), *Ri = o.A_("Rho"),
+           *ps = o.A_("pressures"), *pk = o.A_("pressurek"), *pi = o.A_("pressure");
+    const double *fr = o.g.func_r, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    for (int id = 0; id < o.P; id++) {
+        double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
+        for (int lev = 0; lev < nv; lev++) {
+            int e = (id * nv + lev) * 3;
+            double Mx = Mi[e + 0] - Mk[e + 0], My = Mi[e + 1] - Mk[e + 1], Mz = Mi[e + 2] - Mk[e + 2];
+            double aux = Mx * fx + My * fy + Mz * fz;
+            Ms[e + 0] = Mx - aux * fx;
+            Ms[e + 1] = My - aux * fy;
+            Ms[e + 2] = Mz - aux * fz;
+            Rs[id * nv + lev] = Ri[id * nv + lev] - Rk[id * nv + lev];
+            ps[id * nv + lev] = pi[id * nv + lev] - pk[id * nv + lev];
+        }
+        double altht = 0, althl, wht = 0, whl;
+        for (int lev = 0; lev < nv + 1; lev++) {
+            if (lev == 0) {
+                Whs[id * (nv + 1) + 0] = 0.0;
+                altht = Alth[lev];
+                wht = 0.0;
+            } else {
+                whl = wht;
+                if (lev == nv) wht = 0.0;
+                else wht = Whi[id * (nv + 1) + lev] - Whk[id * (nv + 1) + lev];
+                Whs[id * (nv + 1) + lev] = wht;
+                althl = altht;
+                altht = Alth[lev];
+                double xi = Alt[lev - 1], xim = althl, xip = altht;
+                double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+                Ws[id * nv + lev - 1] = whl * intt + wht * intl;
+            }
+        }
+    }
+}
+
+// UpdateRK2 (thor_auxiliary.h:420-529): y(k) += deviation.
+void update_rk2(Oracle &o) {
+    const int nv = o.nv;
+    double *Ms = o.A_("Mhs"), *Mk = o.A_("Mhk"), *Whs = o.A_("Whs"), *Whk = o.A_("Whk"), *Wk = o.A_("Wk"),
+           *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"), *ps = o.A_("pressures"), *pk = o.A_("pressurek");
+    const double *fr = o.g.func_r, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    for (int id = 0; id < o.P; id++) {
+        double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
+        for (int lev = 0; lev < nv; lev++) {
+            int e = (id * nv + lev) * 3;
+            double Mx = Ms[e + 0] + Mk[e + 0], My = Ms[e + 1] + Mk[e + 1], Mz = Ms[e + 2] + Mk[e + 2];
+            double aux = Mx * fx + My * fy + Mz * fz;
+            Mk[e + 0] = Mx - aux * fx;
+            Mk[e + 1] = My - aux * fy;
+            Mk[e + 2] = Mz - aux * fz;
+            Rk[id * nv + lev] += Rs[id * nv + lev];
+            pk[id * nv + lev] += ps[id * nv + lev];
+        }
+        for (int lev = 0; lev < nv + 1; lev++) {
+            if (lev == 0 || lev == nv) Whk[id * (nv + 1) + lev] = 0.0;
+            else Whk[id * (nv + 1) + lev] += Whs[id * (nv + 1) + lev];
+        }
+        double althl = 0, altht = 0, whl = 0, wht = 0;
+        for (int lev = 0; lev < nv; lev++) {
+            if (lev == 0) {
+                althl = 0;
+                altht = Alth[1];
+                whl = 0.0;
+                wht = Whk[id * (nv + 1) + 1];
+            }
+            double xi = Alt[lev], xim = althl, xip = altht;
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            Wk[id * nv + lev] = whl * intt + wht * intl;
+            if (lev < nv - 1) {
+                althl = altht;
+                altht = Alth[lev + 2];
+                whl = wht;
+                wht = Whk[id * (nv + 1) + lev + 2];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ fast modes
+// Momentum_Eq + _Poles (thor_fastmodes.h:54-253).
+void momentum_eq(Oracle &o, double dt) {
+    const int nv = o.nv, P = o.P;
+    double *M = o.A_("Mhs"), *ps = o.A_("pressures"), *SlowMh = o.A_("SlowMh"), *DivM = o.A_("DivM");
+    const double *grad = o.g.grad, *fr = o.g.func_r;
+    auto body = [&](int id, int lev, int nslots, auto psv) {
+        double rs = rscale_of(o, lev);
+        double nf[3];
+        for (int k = 0; k < 3; k++) {
+            double s = grad[id * 21 + 0 * 3 + k] * psv(0);
+            for (int j = 1; j < nslots; j++) s = s + grad[id * 21 + j * 3 + k] * psv(j);
+            nf[k] = rs * s;
+        }
+        int e = (id * nv + lev) * 3;
+        double Mx = (-nf[0] + SlowMh[e + 0] + DivM[e + 0]) * dt;
+        double My = (-nf[1] + SlowMh[e + 1] + DivM[e + 1]) * dt;
+        double Mz = (-nf[2] + SlowMh[e + 2] + DivM[e + 2]) * dt;
+        double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
+        double vr = Mx * fx + My * fy + Mz * fz;
+        Mx += -vr * fx;
+        My += -vr * fy;
+        Mz += -vr * fz;
+        M[e + 0] += Mx;
+        M[e + 1] += My;
+        M[e + 2] += Mz;
+    };
+    double sh[NHL2];
+    for (int ib = 0; ib < o.nr; ib++) {
+        const int *m = o.g.maps + ib * NHL2;
+        for (int lev = 0; lev < nv; lev++) {
+            fill(sh, m, [&](int gi) { return ps[gi * nv + lev]; });
+            for (int y = 0; y < NL; y++)
+                for (int x = 0; x < NL; x++) {
+                    Stencil s = stencil(m, x, y);
+                    body(s.id, lev, 7, [&](int k) { return sh[s.pt[k]]; });
+                }
+        }
+    }
+    for (int id = P - 2; id < P; id++) {
+        int gslot[6];
+        gslot[0] = id;
+        for (int i = 0; i < 5; i++) gslot[i + 1] = o.g.point_local[id * 6 + i];
+        for (int lev = 0; lev < nv; lev++) body(id, lev, 6, [&](int k) { return ps[gslot[k] * nv + lev]; });
+    }
+}
+
+// Prepare_Implicit_Vertical + _Poles (thor_vertical_int.h:566-771).
+void prepare_implicit_vertical(Oracle &o) {
+    const int nv = o.nv, P = o.P;
+    double *M = o.A_("Mhs"), *h_d = o.A_("h"), *Slowp = o.A_("Slowpressure"), *SlowRho = o.A_("SlowRho"),
+           *Sp = o.A_("Sp"), *Sd = o.A_("Sd"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+    const double *div = o.g.div;
+    auto body = [&](int id, int lev, int nslots, auto Mv, auto hv) {
+        double RoC = Rd[id * nv + lev] / (Cp[id * nv + lev] - Rd[id * nv + lev]);
+        double rs = rscale_of(o, lev);
+        double nflxr = 0.0, nflxp = 0.0;
+        for (int k = 0; k < 3; k++) {
+            double sr = div[id * 21 + 0 * 3 + k] * Mv(0, k);
+            double sp = div[id * 21 + 0 * 3 + k] * Mv(0, k) * hv(0);
+            for (int j = 1; j < nslots; j++) {
+                sr = sr + div[id * 21 + j * 3 + k] * Mv(j, k);
+                sp = sp + div[id * 21 + j * 3 + k] * Mv(j, k) * hv(j);
+            }
+            nflxr -= rs * sr;
+            nflxp -= (RoC)*rs * sp;
+        }
+        Sp[id * nv + lev] = nflxp + Slowp[id * nv + lev];
+        Sd[id * nv + lev] = nflxr + SlowRho[id * nv + lev];
+    };
+    double vs[3][NHL2], hs[NHL2];
+    for (int ib = 0; ib < o.nr; ib++) {
+        const int *m = o.g.maps + ib * NHL2;
+        for (int lev = 0; lev < nv; lev++) {
+            for (int k = 0; k < 3; k++) fill(vs[k], m, [&](int gi) { return M[(gi * nv + lev) * 3 + k]; });
+            fill(hs, m, [&](int gi) { return h_d[gi * nv + lev]; });
+            for (int y = 0; y < NL; y++)
+                for (int x = 0; x < NL; x++) {
+                    Stencil s = stencil(m, x, y);
+                    body(s.id, lev, 7, [&](int j, int k) { return vs[k][s.pt[j]]; },
+                         [&](int j) { return hs[s.pt[j]]; });
+                }
+        }
+    }
+    for (int id = P - 2; id < P; id++) {
+        int gslot[6];
+        gslot[0] = id;
+        for (int i = 0; i < 5; i++) gslot[i + 1] = o.g.point_local[id * 6 + i];
+        for (int lev = 0; lev < nv; lev++)
+            body(id, lev, 6, [&](int j, int k) { return M[(gslot[j] * nv + lev) * 3 + k]; },
+                 [&](int j) { return h_d[gslot[j] * nv + lev]; });
+    }
+}
+
+// Vertical_Eq (thor_vertical_int.h:52-563): per-column tridiagonal solve for
+// Whs at interfaces 1..nv-1, then Ws at layer centres.
+void vertical_eq(Oracle &o, double deltat) {
+    const int nv = o.nv, nvi = nv + 1;
+    double *Whs = o.A_("Whs"), *Ws = o.A_("Ws"), *ps = o.A_("pressures"), *hh = o.A_("hh"),
+           *Rhos = o.A_("Rhos"), *gtilh = o.A_("gtilh"), *Sp_d = o.A_("Sp"), *Sd_d = o.A_("Sd"),
+           *Srh = o.A_("SlowWh"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    const double A = o.p.A, Gravit = o.p.Gravit;
+    const bool deep = o.p.DeepModel, ghv = o.p.GravHeightVar, nonh = o.p.NonHydro;
+    std::vector<double> cc(nvi), dd(nvi);
+    for (int id = 0; id < o.P; id++) {
+        double altht = 0, althl = 0, alth = 0, alt = 0, altl = 0, hp = 0, h = 0, hm = 0, gp = 0, g = 0, gm = 0;
+        double Sp = 0, Spl = 0, p = 0, pl = 0, r = 0, rl = 0, Sdl = 0, Sd = 0;
+        double CRddl = 0, CRddu = 0, GCoRl = 0, GCoRu = 0;
+        for (int lev = 1; lev < nv; lev++) {
+            if (lev == 1) {
+                altht = Alth[lev + 1]; althl = Alth[lev - 1]; alth = Alth[lev];
+                alt = Alt[lev]; altl = Alt[lev - 1];
+                hp = hh[id * nvi + lev + 1]; h = hh[id * nvi + lev]; hm = hh[id * nvi + lev - 1];
+                gp = gtilh[id * nvi + lev + 1]; g = gtilh[id * nvi + lev]; gm = gtilh[id * nvi + lev - 1];
+                Sp = Sp_d[id * nv + lev]; Spl = Sp_d[id * nv + lev - 1];
+                p = ps[id * nv + lev]; pl = ps[id * nv + lev - 1];
+                rl = Rhos[id * nv + lev - 1]; r = Rhos[id * nv + lev];
+                Sdl = Sd_d[id * nv + lev - 1]; Sd = Sd_d[id * nv + lev];
+                CRddl = (Cp[id * nv + lev - 1] - Rd[id * nv + lev - 1]) / (Rd[id * nv + lev - 1] * deltat * deltat);
+                CRddu = (Cp[id * nv + lev] - Rd[id * nv + lev]) / (Rd[id * nv + lev] * deltat * deltat);
+                if (ghv) {
+                    GCoRl = Gravit * MPOW(A / (A + altl), 2) * (Cp[id * nv + lev - 1] - Rd[id * nv + lev - 1])
+                            / Rd[id * nv + lev - 1];
+                    GCoRu = Gravit * MPOW(A / (A + alt), 2) * (Cp[id * nv + lev] - Rd[id * nv + lev])
+                            / Rd[id * nv + lev];
+                } else {
+                    GCoRl = Gravit * (Cp[id * nv + lev - 1] - Rd[id * nv + lev - 1]) / Rd[id * nv + lev - 1];
+                    GCoRu = Gravit * (Cp[id * nv + lev] - Rd[id * nv + lev]) / Rd[id * nv + lev];
+                }
+            }
+            double aa, bb, C0, intt, intl, inttm, xi, xim, xip;
+            double dzp = 1.0 / (altht - alth), dzh = 1.0 / (alt - altl), dzm = 1.0 / (alth - althl);
+            double dzph = dzp * dzh, dzmh = dzm * dzh;
+            if (deep) {
+                hp = hp * MPOW(altht + A, 2);
+                h = h * MPOW(alth + A, 2);
+                hm = hm * MPOW(althl + A, 2);
+                double or2 = 1 / MPOW(alth + A, 2);
+                double tor3 = 2 / MPOW(alth + A, 3);
+                xi = alt; xim = alth; xip = altht;
+                intl = (xi - xim) * dzp * dzh;
+                xi = altl; xim = althl; xip = alth;
+                inttm = -(xi - xip) * dzm * dzh;
+                double CRdd = (CRddl * (alt - alth) + CRddu * (alth - altl)) / (alt - altl);
+                double GCoR = (GCoRl * (alt - alth) + GCoRu * (alth - altl)) / (alt - altl);
+                cc[lev] = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
+                if (nonh)
+                    bb = CRdd + (dzph + dzmh) * or2 * h + (intl - inttm) * (g + GCoR - tor3 * h);
+                else
+                    bb = (dzph + dzmh) * or2 * h + (intl - inttm) * (g + GCoR - tor3 * h);
+                aa = -dzmh * or2 * hm + intl * (gm + GCoR - tor3 * hm);
+                double dSpdz = (Sp - Spl) * dzh, dPdz = (p - pl) * dzh;
+                xi = alth; xim = altl; xip = alt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                double rhohs = rl * intt + r * intl;
+                double Sdh = Sdl * intt + Sd * intl;
+                double gh = ghv ? Gravit * MPOW(A / (A + alt), 2) : Gravit;
+                if (!nonh) {
+                    if (ghv)
+                        C0 = (MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                              + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz
+                              - deltat * Srh[id * nvi + lev]) * (CRdd);
+                    else
+                        C0 = (MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * Sdh
+                              + Gravit * deltat * rhohs + deltat * dPdz - deltat * Srh[id * nvi + lev]) * (CRdd);
+                } else {
+                    if (ghv)
+                        C0 = (-Whs[id * nvi + lev] + MPOW(deltat, 2.0) * dSpdz
+                              + MPOW(deltat, 2.0) * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                              + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz
+                              - deltat * Srh[id * nvi + lev]) * (CRdd);
+                    else
+                        C0 = (-Whs[id * nvi + lev] + MPOW(deltat, 2.0) * dSpdz
+                              + MPOW(deltat, 2.0) * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                              - deltat * Srh[id * nvi + lev]) * (CRdd);
+                }
+                (void)gh;
+                if (lev < nv - 1) {
+                    althl = alth; alth = altht; altht = Alth[lev + 2];
+                    altl = alt; alt = Alt[lev + 1];
+                    hm = hh[id * nvi + lev]; h = hh[id * nvi + lev + 1]; hp = hh[id * nvi + lev + 2];
+                    gm = g; g = gp; gp = gtilh[id * nvi + lev + 2];
+                    Spl = Sp; Sp = Sp_d[id * nv + lev + 1];
+                    pl = p; p = ps[id * nv + lev + 1];
+                    rl = r; r = Rhos[id * nv + lev + 1];
+                    Sdl = Sd; Sd = Sd_d[id * nv + lev + 1];
+                    CRddl = CRddu;
+                    CRddu = (Cp[id * nv + lev + 1] - Rd[id * nv + lev + 1]) / (Rd[id * nv + lev + 1] * deltat * deltat);
+                    GCoRl = GCoRu;
+                    if (ghv)
+                        GCoRu = Gravit * MPOW(A / (A + alt), 2) * (Cp[id * nv + lev + 1] - Rd[id * nv + lev + 1])
+                                / Rd[id * nv + lev + 1];
+                    else
+                        GCoRu = Gravit * (Cp[id * nv + lev + 1] - Rd[id * nv + lev + 1]) / Rd[id * nv + lev + 1];
+                }
+            } else {
+                xi = alt; xim = alth; xip = altht;
+                intl = (xi - xim) * dzp * dzh;
+                xi = altl; xim = althl; xip = alth;
+                inttm = -(xi - xip) * dzm * dzh;
+                double CRdd = (CRddl * (alt - alth) + CRddu * (alth - altl)) / (alt - altl);
+                double GCoR = (GCoRl * (alt - alth) + GCoRu * (alth - altl)) / (alt - altl);
+                cc[lev] = -dzph * hp - inttm * (gp + GCoR);
+                if (nonh) bb = CRdd + (dzph + dzmh) * h + (intl - inttm) * (g + GCoR);
+                else bb = (dzph + dzmh) * h + (intl - inttm) * (g + GCoR);
+                aa = -dzmh * hm + intl * (gm + GCoR);
+                double dSpdz = (Sp - Spl) * dzh, dPdz = (p - pl) * dzh;
+                xi = alth; xim = altl; xip = alt;
+                intt = (xi - xip) / (xim - xip);
+                intl = (xi - xim) / (xip - xim);
+                double rhohs = rl * intt + r * intl;
+                double Sdh = Sdl * intt + Sd * intl;
+                if (!nonh) {
+                    if (ghv)
+                        C0 = (MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                              + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz
+                              - deltat * Srh[id * nvi + lev]) * (CRdd);
+                    else
+                        C0 = (MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * Sdh
+                              + Gravit * deltat * rhohs + deltat * dPdz - deltat * Srh[id * nvi + lev]) * (CRdd);
+                } else {
+                    if (ghv)
+                        C0 = (-Whs[id * nvi + lev] + MPOW(deltat, 2.0) * dSpdz
+                              + MPOW(deltat, 2.0) * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                              + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz
+                              - deltat * Srh[id * nvi + lev]) * (CRdd);
+                    else
+                        C0 = (-Whs[id * nvi + lev] + MPOW(deltat, 2.0) * dSpdz
+                              + MPOW(deltat, 2.0) * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                              - deltat * Srh[id * nvi + lev]) * (CRdd);
+                }
+                if (lev < nv - 1) {
+                    althl = alth; alth = altht; altht = Alth[lev + 2];
+                    altl = alt; alt = Alt[lev + 1];
+                    hm = h; h = hp; hp = hh[id * nvi + lev + 2];
+                    gm = g; g = gp; gp = gtilh[id * nvi + lev + 2];
+                    Spl = Sp; Sp = Sp_d[id * nv + lev + 1];
+                    pl = p; p = ps[id * nv + lev + 1];
+                    rl = r; r = Rhos[id * nv + lev + 1];
+                    Sdl = Sd; Sd = Sd_d[id * nv + lev + 1];
+                    CRddl = CRddu;
+                    CRddu = (Cp[id * nv + lev + 1] - Rd[id * nv + lev + 1]) / (Rd[id * nv + lev + 1] * deltat * deltat);
+                    GCoRl = GCoRu;
+                    if (ghv)
+                        GCoRu = Gravit * MPOW(A / (A + alt), 2) * (Cp[id * nv + lev + 1] - Rd[id * nv + lev + 1])
+                                / Rd[id * nv + lev + 1];
+                    else
+                        GCoRu = Gravit * (Cp[id * nv + lev + 1] - Rd[id * nv + lev + 1]) / Rd[id * nv + lev + 1];
+                }
+            }
+            dd[lev] = -C0;
+            if (lev == 1) {
+                cc[1] = cc[1] / bb;
+                dd[1] = dd[1] / bb;
+            } else {
+                double t = 1.0 / (bb - cc[lev - 1] * aa);
+                cc[lev] *= t;
+                dd[lev] = (dd[lev] - dd[lev - 1] * aa) * t;
+            }
+        }
+        Whs[id * nvi + nv] = 0.0;
+        Whs[id * nvi] = 0.0;
+        Whs[id * nvi + nv - 1] = dd[nv - 1];
+        for (int lev = nvi - 2; lev > 0; lev--)
+            Whs[id * nvi + lev] = (-cc[lev] * Whs[id * nvi + lev + 1] + dd[lev]);
+        double althl2 = 0, altht2 = 0, whl = 0, wht = 0;
+        for (int lev = 0; lev < nv; lev++) {
+            if (lev == 0) {
+                althl2 = 0; altht2 = Alth[1]; whl = 0.0; wht = Whs[id * nvi + 1];
+            }
+            double xi = Alt[lev], xim = althl2, xip = altht2;
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            Ws[id * nv + lev] = whl * intt + wht * intl;
+            if (lev < nv - 1) {
+                althl2 = altht2; altht2 = Alth[lev + 2]; whl = wht; wht = Whs[id * nvi + lev + 2];
+            }
+        }
+    }
+}
+
+// Density_Pressure_Eqs + _Poles, entropy form (thor_fastmodes.h:255-847).
+void density_pressure(Oracle &o, double dt) {
+    const int nv = o.nv, P = o.P;
+    double *ps = o.A_("pressures"), *pk = o.A_("pressurek"), *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"),
+           *Ms = o.A_("Mhs"), *Mk = o.A_("Mhk"), *Whs = o.A_("Whs"), *Whk = o.A_("Whk"), *pt = o.A_("pt"),
+           *pth = o.A_("pth"), *SlowRho = o.A_("SlowRho"), *Qh = o.A_("profx_Qheat"), *diffpr = o.A_("diffpr"),
+           *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+    const double *div = o.g.div, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    const double A = o.p.A, P_Ref = o.p.P_Ref;
+    auto body = [&](int id, int lev, int nslots, bool pole, auto vv, auto v1v, auto ptv) {
+        double Cv = Cp[id * nv + lev] - Rd[id * nv + lev];
+        double altht = Alth[lev + 1], althl = Alth[lev], dz = altht - althl;
+        double r2p, r2m, r2l, rscale, dr2dz;
+        if (o.p.DeepModel) {
+            double alt = Alt[lev];
+            r2p = MPOW(altht + A, 2.0);
+            r2m = MPOW(alt + A, 2.0);
+            r2l = MPOW(althl + A, 2.0);
+            rscale = A / (alt + A);
+            dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3.0;
+        } else {
+            r2p = r2m = r2l = 1.0;
+            rscale = 1.0;
+            dr2dz = r2m * dz;
+        }
+        double nflxr = 0.0, nflxpt = 0.0;
+        for (int k = 0; k < 3; k++) {
+            double sr = div[id * 21 + 0 * 3 + k] * vv(0, k);
+            double sp = div[id * 21 + 0 * 3 + k] * v1v(0, k) * ptv(0);
+            for (int j = 1; j < nslots; j++) {
+                sr = sr + div[id * 21 + j * 3 + k] * vv(j, k);
+                sp = sp + div[id * 21 + j * 3 + k] * v1v(j, k) * ptv(j);
+            }
+            nflxr += rscale * sr;
+            nflxpt += rscale * sp;
+        }
+        double whl, wht, whl2, wht2, phl, pht;
+        if (lev == 0) {
+            whl = 0.0;
+            wht = Whs[id * (nv + 1) + lev + 1];
+            whl2 = 0.0;
+            wht2 = Whs[id * (nv + 1) + lev + 1] + Whk[id * (nv + 1) + lev + 1];
+            phl = 0.0;
+            pht = pth[id * (nv + 1) + lev + 1];
+        } else {
+            whl = Whs[id * (nv + 1) + lev];
+            wht = Whs[id * (nv + 1) + lev + 1];
+            whl2 = Whs[id * (nv + 1) + lev] + Whk[id * (nv + 1) + lev];
+            wht2 = Whs[id * (nv + 1) + lev + 1] + Whk[id * (nv + 1) + lev + 1];
+            phl = pth[id * (nv + 1) + lev];
+            pht = pth[id * (nv + 1) + lev + 1];
+        }
+        double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
+        double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
+        double aux = -(nflxpt + dwptdz) * dt;
+        double r = Rk[id * nv + lev] + Rs[id * nv + lev];
+        nflxr += dwdz;
+        Rs[id * nv + lev] += (SlowRho[id * nv + lev] - nflxr) * dt;
+        double ptc = (P_Ref / (Rd[id * nv + lev] * r))
+                     * MPOW((ps[id * nv + lev] + pk[id * nv + lev]) / P_Ref, Cv / Cp[id * nv + lev]);
+        aux += ptc * r;
+        double p = P_Ref * MPOW(Rd[id * nv + lev] * aux / P_Ref, Cp[id * nv + lev] / Cv);
+        ps[id * nv + lev] = p - pk[id * nv + lev] + (diffpr[id * nv + lev] + 0.0) * dt
+                            + Rd[id * nv + lev] / Cv * Qh[id * nv + lev] * dt;
+    };
+    double vs[3][NHL2], v1s[3][NHL2], pts[NHL2];
+    for (int ib = 0; ib < o.nr; ib++) {
+        const int *m = o.g.maps + ib * NHL2;
+        for (int lev = 0; lev < nv; lev++) {
+            for (int pos : kLoad) {
+                int gi = m[pos];
+                for (int k = 0; k < 3; k++) {
+                    if (gi >= 0) {
+                        vs[k][pos] = Ms[(gi * nv + lev) * 3 + k];
+                        v1s[k][pos] = vs[k][pos] + Mk[(gi * nv + lev) * 3 + k];
+                    } else {
+                        vs[k][pos] = v1s[k][pos] = 0.0;
+                    }
+                }
+                pts[pos] = gi >= 0 ? pt[gi * nv + lev] : 0.0;
+            }
+            for (int y = 0; y < NL; y++)
+                for (int x = 0; x < NL; x++) {
+                    Stencil s = stencil(m, x, y);
+                    body(s.id, lev, 7, false, [&](int j, int k) { return vs[k][s.pt[j]]; },
+                         [&](int j, int k) { return v1s[k][s.pt[j]]; }, [&](int j) { return pts[s.pt[j]]; });
+                }
+        }
+    }
+    for (int id = P - 2; id < P; id++) {
+        int gs[6];
+        gs[0] = id;
+        for (int i = 0; i < 5; i++) gs[i + 1] = o.g.point_local[id * 6 + i];
+        for (int lev = 0; lev < nv; lev++)
+            body(id, lev, 6, true, [&](int j, int k) { return Ms[(gs[j] * nv + lev) * 3 + k]; },
+                 [&](int j, int k) { return Ms[(gs[j] * nv + lev) * 3 + k] + Mk[(gs[j] * nv + lev) * 3 + k]; },
+                 [&](int j) { return pt[gs[j] * nv + lev]; });
+    }
+}
+
+void zero(Oracle &o, const char *n) { std::fill(o.arr[n].begin(), o.arr[n].end(), 0.0); }
+void copy(Oracle &o, const char *dst, const char *src) { o.arr[dst] = o.arr[src]; }
+
+// ESP::Thor (thor_driver.cu:70-1466) for the configuration space used here.
+void thor(Oracle &o) {
+    const int ns_total = o.p.n_substeps;
+    copy(o, "Mhk", "Mh");
+    copy(o, "Whk", "Wh");
+    copy(o, "Wk", "W");
+    copy(o, "Rhok", "Rho");
+    copy(o, "pressurek", "pressure");
+    zero(o, "Mhs");
+    zero(o, "Rhos");
+    zero(o, "Whs");
+    zero(o, "Ws");
+    zero(o, "pressures");
+    for (int rk = 0; rk < 3; rk++) {
+        int ns_it = rk == 0 ? 1 : (rk == 1 ? ns_total / 2 : ns_total);
+        double times = rk == 0 ? o.p.timestep / 3.0 : o.p.timestep / (double)ns_total;
+        zero(o, "Adv");
+        advec_cori1(o);
+        advec_poles(o);
+        advec_cori2(o);
+        temperature_h_pt_geff(o);
+        zero(o, "SlowMh");
+        zero(o, "SlowWh");
+        zero(o, "SlowRho");
+        zero(o, "Slowpressure");
+        if (o.p.HyDiff) {
+            zero(o, "diff");
+            int stepnum = 0;
+            for (int ihyp = 0; ihyp < o.p.HyDiffOrder / 2 - 1; ihyp++) {
+                diffusion_pass(o, stepnum, false);
+                stepnum += 1;
+            }
+            zero(o, "diffrh");
+            diffusion_pass(o, stepnum, true);
+            correct_horizontal(o);
+        }
+        zero(o, "DivM");
+        zero(o, "divg_Mh");
+        if (o.p.DivDampP) {
+            divm_pass(o, "Mhk", "Whk", false);
+            divm_pass(o, "Mhk", "Whk", true);
+        }
+        slow_modes(o);
+        if (rk > 0) update_rk(o);
+        for (int ns = 0; ns < ns_it; ns++) {
+            zero(o, "DivM");
+            zero(o, "divg_Mh");
+            if (o.p.DivDampP) {
+                divm_pass(o, "Mhs", "Whs", false);
+                divm_pass(o, "Mhs", "Whs", true);
+            }
+            momentum_eq(o, times);
+            prepare_implicit_vertical(o);
+            vertical_eq(o, times);
+            density_pressure(o, times);
+        }
+        update_rk2(o);
+    }
+    copy(o, "Mh", "Mhk");
+    copy(o, "Wh", "Whk");
+    copy(o, "W", "Wk");
+    copy(o, "Rho", "Rhok");
+    copy(o, "pressure", "pressurek");
+}
+
+// ESP::ProfX for the Held-Suarez benchmark (profx_driver.cu:70-474).
+void profx_hs(Oracle &o) {
+    const int nv = o.nv;
+    double *W = o.A_("W"), *Wh = o.A_("Wh"), *T = o.A_("temperature"), *pt = o.A_("pt"), *p = o.A_("pressure"),
+           *Rho = o.A_("Rho"), *Rd = o.A_("Rd"), *Cp = o.A_("Cp"), *Mh = o.A_("Mh");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat;
+    zero(o, "profx_Qheat");
+    // Recompute_W (profx_auxiliary.h:110-130)
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
+            double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            W[id * nv + lev] = whl * intt + wht * intl;
+        }
+    // Compute_temperature (profx_auxiliary.h:64-93)
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            int e = id * nv + lev;
+            double Cv = Cp[e] - Rd[e], CvoCp = Cv / Cp[e];
+            T[e] = p[e] / (Rd[e] * Rho[e]);
+            pt[e] = (o.p.P_Ref / (Rd[e] * Rho[e])) * MPOW(p[e] / o.p.P_Ref, CvoCp);
+        }
+    // held_suarez (profx_held_suarez.h:51-117)
+    const double Gravit = o.p.Gravit, kappa = o.p.Rd / o.p.Cp, dt = o.p.timestep;
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            double sigmab = 0.7, ka = (1.0 / 40.0) * (1.0 / 86400.0), kf = 1.0 / 86400.0,
+                   ks = (1.0 / 4.0) * (1.0 / 86400.0), dTy = 60.0, dthetaz = 10.0, p0 = 100000.0;
+            double lat = ll[id * 2 + 1];
+            double psm1 = p[id * nv + 1] - Rho[id * nv + 0] * Gravit * (-Alt[0] - Alt[1]);
+            double ps = 0.5 * (p[id * nv + 0] + psm1);
+            double pre = p[id * nv + lev];
+            double sigma = (pre / ps), sigma0 = (pre / p0);
+            double Teq = std::max(200.0, (315.0 - dTy * MPOW(MSIN(lat), 2.0)
+                                          - dthetaz * MLOG(sigma0) * MPOW(MCOS(lat), 2.0))
+                                             * MPOW(sigma0, kappa));
+            double kt = ka + (ks - ka) * std::max(0.0, (sigma - sigmab) / (1.0 - sigmab)) * MPOW(MCOS(lat), 4.0);
+            double kv = kf * std::max(0.0, (sigma - sigmab) / (1.0 - sigmab));
+            for (int k = 0; k < 3; k++) Mh[(id * nv + lev) * 3 + k] = Mh[(id * nv + lev) * 3 + k] / (1.0 + kv * dt);
+            T[id * nv + lev] -= kt * dt * (T[id * nv + lev] - Teq);
+        }
+    // Compute_pressure (profx_auxiliary.h:95-108)
+    for (int e = 0; e < o.P * nv; e++) p[e] = T[e] * Rd[e] * Rho[e];
+}
+
+void make_kd(Oracle &o) {
+    // Horizontal diffusion / divergence-damping strengths (esp_initial.cu:1292-1321).
+    o.Kdh4.assign(o.nv, 0.0);
+    o.Kdhz.assign(o.nv, 0.0);
+    for (int lev = 0; lev < o.nv; lev++) {
+        double dbar = std::sqrt(2 * M_PI / 5) * o.p.A / (std::pow(2, o.p.glevel));
+        o.Kdh4[lev] = (o.p.Diffc) * std::pow(dbar, 1.0 * o.p.HyDiffOrder) / o.p.timestep;
+        o.Kdhz[lev] = (o.p.DivDampc) * std::pow(dbar, 4.) / o.p.timestep;
+    }
+}
+
+} // namespace
+
+double oracle_tree_sum(const double *d, long n) {
+    const long B = 1024;
+    long nb = (long)std::ceil(double(n) / double(2 * B));
+    double out = 0.0;
+    std::vector<double> buf(2 * B);
+    for (long b = 0; b < nb; b++) {
+        for (long j = 0; j < 2 * B; j++) {
+            long idx = b * 2 * B + j;
+            buf[j] = idx < n ? d[idx] : 0.0;
+        }
+        for (long stride = B; stride > 0; stride /= 2)
+            for (long t = 0; t < stride; t++) buf[t] += buf[t + stride];
+        out += buf[0];
+    }
+    return out;
+}
+
+struct oracle : Oracle {};
+
+// Elementary functions as the restatement calls them (portable in the parity
+// build, glibc otherwise): fn 0 = pow(x, y), 1 = log(x), 2 = sin(x), 3 = cos(x).
+extern "C" double oracle_math(int fn, double x, double y) {
+    switch (fn) {
+    case 0: return MPOW(x, y);
+    case 1: return MLOG(x);
+    case 2: return MSIN(x);
+    default: return MCOS(x);
+    }
+}
+
+extern "C" {
+
+oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
+    oracle *o = new oracle();
+    o->g = *g;
+    o->p = *p;
+    o->P = g->point_num;
+    o->nv = g->nv;
+    o->nvi = g->nv + 1;
+    o->nr = g->nr;
+    const long N = (long)o->P * o->nv, Ni = (long)o->P * o->nvi;
+    const char *sc[] = {"W", "Rho", "pressure", "temperature", "Rd", "Cp", "profx_Qheat", "Wk", "Rhok",
+                        "pressurek", "Ws", "Rhos", "pressures", "h", "pt", "gtil", "SlowRho", "Slowpressure",
+                        "diffw", "diffrh", "diffpr", "Sp", "Sd"};
+    const char *it[] = {"Wh", "profx_dWh", "Whk", "Whs", "hh", "pth", "gtilh", "SlowWh"};
+    const char *vc[] = {"Mh", "profx_dMh", "Mhk", "Mhs", "Adv", "v", "SlowMh", "diffmh", "DivM", "divg_Mh"};
+    for (auto n : sc) o->arr[n].assign(N, 0.0);
+    for (auto n : it) o->arr[n].assign(Ni, 0.0);
+    for (auto n : vc) o->arr[n].assign(3 * N, 0.0);
+    o->arr["diff"].assign(6 * N, 0.0);
+    for (long e = 0; e < N; e++) {
+        o->arr["Rd"][e] = p->Rd;
+        o->arr["Cp"][e] = p->Cp;
+    }
+    make_kd(*o);
+    return o;
+}
+
+void oracle_destroy(oracle *o) { delete o; }
+
+long oracle_array_size(oracle *o, const char *name) {
+    auto it = o->arr.find(name);
+    return it == o->arr.end() ? -1 : (long)it->second.size();
+}
+int oracle_set(oracle *o, const char *name, const double *src, long n) {
+    auto it = o->arr.find(name);
+    if (it == o->arr.end() || (long)it->second.size() != n) return -1;
+    std::memcpy(it->second.data(), src, n * sizeof(double));
+    return 0;
+}
+int oracle_get(oracle *o, const char *name, double *dst, long n) {
+    auto it = o->arr.find(name);
+    if (it == o->arr.end() || (long)it->second.size() != n) return -1;
+    std::memcpy(dst, it->second.data(), n * sizeof(double));
+    return 0;
+}
+void oracle_thor(oracle *o) { thor(*o); }
+void oracle_profx_held_suarez(oracle *o) { profx_hs(*o); }
+void oracle_steps(oracle *o, int n) {
+    for (int i = 0; i < n; i++) {
+        profx_hs(*o);
+        thor(*o);
+    }
+}
+
+// CalcMass / CalcTotEnergy / CalcAngMom (profx_globdiag.cu:45-212) + tree sums.
+void oracle_globdiag(oracle *o, double out[5]) {
+    const int nv = o->nv, P = o->P;
+    const long N = (long)P * nv;
+    std::vector<double> mass(N), en(N), amx(N), amy(N), amz(N);
+    double *Rho = o->A_("Rho"), *Mh = o->A_("Mh"), *W = o->A_("W"), *T = o->A_("temperature"), *Cp = o->A_("Cp"),
+           *Rd = o->A_("Rd");
+    const double *Alt = o->g.Altitude, *Alth = o->g.Altitudeh, *ll = o->g.lonlat, *aT = o->g.areasT,
+                 *fr = o->g.func_r;
+    const double A = o->p.A, Omega = o->p.Omega, Gravit = o->p.Gravit;
+    for (int id = 0; id < P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            long e = (long)id * nv + lev;
+            double zup = Alth[lev + 1] + A, zlow = Alth[lev] + A, Vol;
+            if (o->p.DeepModel) Vol = aT[id] / MPOW(A, 2) * (MPOW(zup, 3) - MPOW(zlow, 3)) / 3;
+            else Vol = aT[id] * (zup - zlow);
+            mass[e] = Rho[e] * Vol;
+            double lat = ll[id * 2 + 1], lon = ll[id * 2];
+            double wx = W[e] * MCOS(lat) * MCOS(lon), wy = W[e] * MCOS(lat) * MSIN(lon),
+                   wz = W[e] * MSIN(lat);
+            double Cv = Cp[e] - Rd[e];
+            double Ek = 0.5
+                        * ((Mh[e * 3 + 0] + wx) * (Mh[e * 3 + 0] + wx) + (Mh[e * 3 + 1] + wy) * (Mh[e * 3 + 1] + wy)
+                           + (Mh[e * 3 + 2] + wz) * (Mh[e * 3 + 2] + wz))
+                        / Rho[e];
+            double Eint = Cv * T[e] * Rho[e];
+            double Eg = Rho[e] * Gravit * Alt[lev];
+            en[e] = (Ek + Eint + Eg) * Vol;
+            double r = (A + Alt[lev]);
+            double rx = r * fr[id * 3 + 0], ry = r * fr[id * 3 + 1], rz = r * fr[id * 3 + 2];
+            double AMx = ry * Mh[e * 3 + 2] - rz * Mh[e * 3 + 1]
+                         - Rho[e] * Omega * r * rz * MCOS(lat) * MCOS(lon);
+            double AMy = -rx * Mh[e * 3 + 2] + rz * Mh[e * 3 + 0]
+                         - Rho[e] * Omega * r * rz * MCOS(lat) * MSIN(lon);
+            double AMz = rx * Mh[e * 3 + 1] - ry * Mh[e * 3 + 0]
+                         + Rho[e] * Omega * r * r * MCOS(lat) * MCOS(lat);
+            amx[e] = AMx * Vol;
+            amy[e] = AMy * Vol;
+            amz[e] = AMz * Vol;
+        }
+    out[0] = oracle_tree_sum(mass.data(), N);
+    out[1] = oracle_tree_sum(en.data(), N);
+    out[2] = oracle_tree_sum(amx.data(), N);
+    out[3] = oracle_tree_sum(amy.data(), N);
+    out[4] = oracle_tree_sum(amz.data(), N);
+}
+
+} // extern "C"

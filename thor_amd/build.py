@@ -1,0 +1,66 @@
+"""Builds libthor_amd.so in-tree: host C++ with g++, HIP kernels with hipcc for gfx950.
+
+Usage: python -m thor_amd.build
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libthor_amd.so")
+# Parity build: identical sources with the portable elementary functions shared
+# with the CPU oracle (bit-for-bit comparison of the restated arithmetic).
+LIB_PM = os.path.join(HERE, "libthor_amd_pm.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = "gfx950"
+
+HOST_SRC = ["host.cpp", "grid.cpp"]
+HIP_SRC = ["dyn.hip"]
+
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+# The FP64 path must round every product and sum as written (no FMA
+# contraction); packed FP32 instructions are disabled for all HIP code.
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-result", "-Wno-unused-value"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def _link(objs, out):
+    tmp = out + ".tmp"
+    _run(["g++", "-shared", "-fopenmp", "-o", tmp, *objs, f"-L{ROCM}/lib", "-lamdhip64",
+          f"-Wl,-rpath,{ROCM}/lib"])
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = True) -> str:
+    """Compiles both variants of the library (production and parity build)."""
+    os.makedirs(OBJ, exist_ok=True)
+    hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
+    host = []
+    for src in HOST_SRC:
+        o = os.path.join(OBJ, src + ".o")
+        _run(["g++", *CXXFLAGS, "-I", os.path.join(ROCM, "include"), "-c", os.path.join(CSRC, src), "-o", o])
+        host.append(o)
+    for lib, extra, tag in ((LIB, [], ""), (LIB_PM, ["-DTHOR_PORTABLE_MATH"], "_pm")):
+        objs = list(host)
+        for src in HIP_SRC:
+            o = os.path.join(OBJ, src + tag + ".o")
+            _run([hipcc, *HIPFLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", o])
+            objs.append(o)
+        _link(objs, lib)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

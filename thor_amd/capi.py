@@ -1,0 +1,286 @@
+"""ctypes binding of the C ABI declared in include/thor_dyn.h.
+
+The host-side mirror of THOR's driver objects: ``Grid`` (Icogrid,
+src/ESP/grid.cu:85-302), ``Params`` (SimulationSetup,
+src/headers/simulation_setup.h:54-119) and ``Dyn`` (the ESP object's dynamics
+state and its Thor/ProfX/globdiag methods, src/headers/esp.h:466-470).
+Errors raise ``ThorError`` carrying the library's status code and message,
+where the reference would log and exit (thor_driver.cu:1277-1286).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libthor_amd.so")
+# Parity build of the same sources with portable elementary functions.
+LIB_PATH_PM = os.path.join(_HERE, "libthor_amd_pm.so")
+
+THOR_OK = 0
+THOR_ERR_BAD_ARG = -1
+THOR_ERR_HIP = -2
+THOR_ERR_NAN = -3
+THOR_ERR_NO_DEVICE = -4
+THOR_ERR_IO = -5
+THOR_ERR_UNSUPPORTED = -6
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+FORCING_NONE = 0
+FORCING_HELD_SUAREZ = 1
+
+
+class ThorError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"thor status {code}: {msg}")
+        self.code = code
+
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("point_num", C.c_int), ("nv", C.c_int), ("nr", C.c_int), ("nl_region", C.c_int),
+        ("glevel", C.c_int), ("pent_ind", C.c_int * 12),
+        ("point_local", _ip), ("maps", _ip),
+        ("div", _dp), ("grad", _dp), ("areas", _dp), ("areasTr", _dp), ("areasT", _dp),
+        ("nvecoa", _dp), ("nvecti", _dp), ("nvecte", _dp), ("func_r", _dp), ("lonlat", _dp),
+        ("Altitude", _dp), ("Altitudeh", _dp), ("xyz", _dp), ("xyzq", _dp),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("A", C.c_double), ("Omega", C.c_double), ("Gravit", C.c_double), ("P_Ref", C.c_double),
+        ("Rd", C.c_double), ("Cp", C.c_double), ("Tmean", C.c_double), ("Diffc", C.c_double),
+        ("DivDampc", C.c_double), ("timestep", C.c_double),
+        ("glevel", C.c_int), ("HyDiff", C.c_int), ("HyDiffOrder", C.c_int), ("DivDampP", C.c_int),
+        ("DeepModel", C.c_int), ("NonHydro", C.c_int), ("GravHeightVar", C.c_int),
+        ("n_substeps", C.c_int), ("precision", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class State(C.Structure):
+    _fields_ = [("on_device", C.c_int), ("Mh", _dp), ("Wh", _dp), ("W", _dp), ("Rho", _dp),
+                ("pressure", _dp)]
+
+
+_libs: dict = {}
+
+
+def lib(variant: str = "") -> C.CDLL:
+    """Loads libthor_amd.so (variant "") or the parity build libthor_amd_pm.so
+    (variant "pm"), both built by thor_amd.build. Raises if it is missing."""
+    if variant not in _libs:
+        path = LIB_PATH_PM if variant == "pm" else LIB_PATH
+        if not os.path.exists(path):
+            raise ThorError(THOR_ERR_IO, f"{path} not built; run python -m thor_amd.build")
+        L = C.CDLL(path)
+        L.thor_last_error.restype = C.c_char_p
+        L.thor_grid_build.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
+                                      C.c_char_p, C.POINTER(C.c_void_p)]
+        L.thor_grid_describe.argtypes = [C.c_void_p, C.POINTER(GridDesc)]
+        L.thor_grid_spring_iterations.argtypes = [C.c_void_p]
+        L.thor_grid_destroy.argtypes = [C.c_void_p]
+        L.thor_grid_destroy.restype = None
+        L.thor_params_default_earth.argtypes = [C.POINTER(Params)]
+        L.thor_params_default_earth.restype = None
+        L.thor_ic_rest.argtypes = [C.POINTER(GridDesc), C.POINTER(Params), C.c_ulonglong, C.c_double,
+                                   C.c_double, _dp, _dp, _dp, _dp, _dp]
+        if hasattr(L, "thor_dyn_create"):
+            L.thor_dyn_create.argtypes = [C.POINTER(GridDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
+            L.thor_dyn_destroy.argtypes = [C.c_void_p]
+            L.thor_dyn_destroy.restype = None
+            for f in ("thor_dyn_set_state", "thor_dyn_get_state"):
+                getattr(L, f).argtypes = [C.c_void_p, C.POINTER(State)]
+            L.thor_dyn_thor.argtypes = [C.c_void_p]
+            L.thor_dyn_profx.argtypes = [C.c_void_p, C.c_int]
+            L.thor_dyn_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+            L.thor_dyn_sync.argtypes = [C.c_void_p]
+            L.thor_dyn_globdiag.argtypes = [C.c_void_p, _dp]
+            L.thor_dyn_get_field.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
+            L.thor_dyn_set_field.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
+            L.thor_dyn_kernel_timing.argtypes = [C.c_void_p, C.c_int]
+            L.thor_dyn_kernel_stat_count.argtypes = [C.c_void_p]
+            L.thor_dyn_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
+                                               C.POINTER(C.c_long), _dp, _dp]
+            L.thor_dyn_time_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        _libs[variant] = L
+    return _libs[variant]
+
+
+def _check(code: int, variant: str = ""):
+    if code != THOR_OK:
+        raise ThorError(code, lib(variant).thor_last_error().decode())
+
+
+def _ptr(a: np.ndarray):
+    assert a.flags.c_contiguous
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    raise TypeError(a.dtype)
+
+
+def default_params(**overrides) -> Params:
+    p = Params()
+    lib().thor_params_default_earth(C.byref(p))
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Grid:
+    """Icosahedral grid + operators in THOR's per-point layout."""
+
+    def __init__(self, glevel: int, nv: int, radius: float, top: float, spring: bool = True,
+                 beta: float = 1.15, cache_dir: str | None = None):
+        h = C.c_void_p()
+        cd = cache_dir.encode() if cache_dir else None
+        _check(lib().thor_grid_build(glevel, nv, radius, top, int(spring), beta, cd, C.byref(h)))
+        self._h = h
+        self.desc = GridDesc()
+        _check(lib().thor_grid_describe(h, C.byref(self.desc)))
+        d = self.desc
+        self.P, self.nv, self.nr, self.glevel = d.point_num, d.nv, d.nr, d.glevel
+        self.spring_iterations = lib().thor_grid_spring_iterations(h)
+        P = self.P
+        shapes = {
+            "point_local": (P, 6), "maps": (self.nr, 18, 18), "div": (P, 7, 3), "grad": (P, 7, 3),
+            "areas": (P, 6, 3), "areasTr": (P, 6), "areasT": (P,), "nvecoa": (P, 6, 3),
+            "nvecti": (P, 6, 3), "nvecte": (P, 6, 3), "func_r": (P, 3), "lonlat": (P, 2),
+            "Altitude": (self.nv,), "Altitudeh": (self.nv + 1,), "xyz": (P, 3), "xyzq": (P, 6, 3),
+        }
+        for name, shp in shapes.items():
+            ptr = getattr(d, name)
+            n = int(np.prod(shp))
+            arr = np.ctypeslib.as_array(ptr, shape=(n,)).reshape(shp)
+            setattr(self, name, arr)
+        self.pent_ind = np.array(list(d.pent_ind), dtype=np.int64)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and "" in _libs:
+            _libs[""].thor_grid_destroy(self._h)
+            self._h = None
+
+
+def ic_rest(grid: Grid, params: Params, seed: int = 1000, temperature_noise: float = 0.0,
+            wind_noise: float = 0.0) -> dict:
+    """Rest initial condition in THOR's layout (host numpy arrays)."""
+    P, nv = grid.P, grid.nv
+    st = {
+        "Mh": np.zeros((P, nv, 3)), "Wh": np.zeros((P, nv + 1)), "W": np.zeros((P, nv)),
+        "Rho": np.zeros((P, nv)), "pressure": np.zeros((P, nv)),
+    }
+    _check(lib().thor_ic_rest(C.byref(grid.desc), C.byref(params), seed, temperature_noise, wind_noise,
+                              _ptr(st["Mh"]), _ptr(st["Wh"]), _ptr(st["W"]), _ptr(st["Rho"]),
+                              _ptr(st["pressure"])))
+    return st
+
+
+FIELD_SHAPES = {
+    "Mh": "v", "W": "s", "Rho": "s", "pressure": "s", "Wh": "i", "temperature": "s", "pt": "s",
+    "h": "s", "hh": "i", "pth": "i", "gtil": "s", "gtilh": "i", "profx_Qheat": "s",
+    "profx_dMh": "v", "profx_dWh": "i",
+}
+
+
+class Dyn:
+    """Dynamical-core handle on the current HIP device (ESP::Thor drop-in)."""
+
+    def __init__(self, grid: Grid, params: Params, variant: str = ""):
+        L = lib(variant)
+        self.variant = variant
+        self.L = L
+        self.grid = grid
+        self.params = params
+        h = C.c_void_p()
+        self._chk(L.thor_dyn_create(C.byref(grid.desc), C.byref(params), C.byref(h)))
+        self._h = h
+
+    def _chk(self, code):
+        _check(code, self.variant)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.thor_dyn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _shape(self, kind):
+        P, nv = self.grid.P, self.grid.nv
+        return {"s": (P, nv), "i": (P, nv + 1), "v": (P, nv, 3)}[kind]
+
+    def set_state(self, st: dict):
+        arrs = {k: np.ascontiguousarray(st[k], dtype=np.float64) for k in ("Mh", "Wh", "W", "Rho", "pressure")}
+        s = State(0, *(_ptr(arrs[k]) for k in ("Mh", "Wh", "W", "Rho", "pressure")))
+        self._chk(self.L.thor_dyn_set_state(self._h, C.byref(s)))
+
+    def get_state(self) -> dict:
+        out = {k: np.zeros(self._shape(FIELD_SHAPES[k])) for k in ("Mh", "Wh", "W", "Rho", "pressure")}
+        s = State(0, *(_ptr(out[k]) for k in ("Mh", "Wh", "W", "Rho", "pressure")))
+        self._chk(self.L.thor_dyn_get_state(self._h, C.byref(s)))
+        return out
+
+    def get_field(self, name: str) -> np.ndarray:
+        a = np.zeros(self._shape(FIELD_SHAPES[name]))
+        self._chk(self.L.thor_dyn_get_field(self._h, name.encode(), _ptr(a), a.size))
+        return a
+
+    def set_field(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self._chk(self.L.thor_dyn_set_field(self._h, name.encode(), _ptr(a), a.size))
+
+    def thor(self):
+        self._chk(self.L.thor_dyn_thor(self._h))
+
+    def profx(self, forcing=FORCING_HELD_SUAREZ):
+        self._chk(self.L.thor_dyn_profx(self._h, forcing))
+
+    def run(self, nsteps: int, forcing=FORCING_HELD_SUAREZ):
+        self._chk(self.L.thor_dyn_run(self._h, nsteps, forcing))
+
+    def sync(self):
+        self._chk(self.L.thor_dyn_sync(self._h))
+
+    def globdiag(self) -> np.ndarray:
+        out = np.zeros(5)
+        self._chk(self.L.thor_dyn_globdiag(self._h, _ptr(out)))
+        return out
+
+    def kernel_timing(self, enable: bool):
+        self._chk(self.L.thor_dyn_kernel_timing(self._h, int(enable)))
+
+    def kernel_stats(self) -> list:
+        L = self.L
+        res = []
+        for i in range(L.thor_dyn_kernel_stat_count(self._h)):
+            name = C.c_char_p()
+            n = C.c_long()
+            ms = C.c_double()
+            by = C.c_double()
+            self._chk(L.thor_dyn_kernel_stat(self._h, i, C.byref(name), C.byref(n), C.byref(ms), C.byref(by)))
+            res.append({"name": name.value.decode(), "launches": n.value, "total_ms": ms.value,
+                        "bytes_per_launch": by.value})
+        return res
+
+    def time_run(self, nsteps: int, forcing=FORCING_HELD_SUAREZ) -> float:
+        ms = C.c_double()
+        self._chk(self.L.thor_dyn_time_run(self._h, nsteps, forcing, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,543 @@
+// Host side of the dynamical core: handle life cycle, layout conversion,
+// launch sequence of one large step (ESP::Thor, thor_driver.cu:70-1466),
+// ProfX, global diagnostics and per-kernel timing.
+#include "../../include/thor_dyn.h"
+#include "common.h"
+#include "dyn_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace thor {
+
+#define HIP_TRY(expr)                                                                                \
+    do {                                                                                             \
+        hipError_t _e = (expr);                                                                      \
+        if (_e != hipSuccess)                                                                        \
+            return fail(THOR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e));             \
+    } while (0)
+
+constexpr int kLC = 8;  // levels per thread in the stencil kernels
+
+struct KernelStat {
+    long launches = 0;
+    double total_ms = 0.0;
+    double bytes = 0.0;
+};
+
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct Dyn {
+    Geo g{};
+    thor_params prm{};
+    int P = 0, S = 0, nv = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    // state y(n) and step buffers (see Fields)
+    double *Mh, *Wh, *W, *Rho, *p;
+    double *Mhk, *WhkA, *WhkB, *Wk, *Rhok, *pk;
+    Fields f{};
+    double *partials = nullptr;
+    int npart = 0;
+    bool timing = false;
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    std::map<std::string, KernelStat> stats;
+    std::vector<std::string> stat_order;
+    double diff_sign = 1.0;
+};
+
+static hipError_t dalloc(Dyn &d, double **ptr, size_t n) {
+    hipError_t e = hipMalloc((void **)ptr, n * sizeof(double));
+    if (e == hipSuccess) {
+        d.allocs.push_back(*ptr);
+        e = hipMemset(*ptr, 0, n * sizeof(double));
+    }
+    return e;
+}
+
+static hipEvent_t get_event(Dyn &d) {
+    if (!d.free_events.empty()) {
+        hipEvent_t e = d.free_events.back();
+        d.free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+static void drain_timing(Dyn &d) {
+    if (d.pending.empty()) return;
+    hipStreamSynchronize(d.stream);
+    for (auto &pd : d.pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, pd.a, pd.b);
+        d.stats[pd.name].total_ms += ms;
+        d.free_events.push_back(pd.a);
+        d.free_events.push_back(pd.b);
+    }
+    d.pending.clear();
+}
+
+// Launches a kernel through fn(); when timing is on, brackets it with events
+// on the handle's stream and books bytes_per_launch of algorithmic traffic.
+template <class Fn> static void launch(Dyn &d, const char *name, double bytes, Fn fn) {
+    if (d.timing) {
+        auto &st = d.stats[name];
+        if (st.launches == 0 && std::find(d.stat_order.begin(), d.stat_order.end(), name) == d.stat_order.end())
+            d.stat_order.push_back(name);
+        st.launches++;
+        st.bytes = bytes;
+        Pending pd{name, get_event(d), get_event(d)};
+        hipEventRecord(pd.a, d.stream);
+        fn();
+        hipEventRecord(pd.b, d.stream);
+        d.pending.push_back(pd);
+        if (d.pending.size() > 2048) drain_timing(d);
+    } else {
+        fn();
+    }
+}
+
+static Fields stage_fields(Dyn &d, int rk) {
+    Fields f = d.f;
+    f.Mh = d.Mh;
+    f.Wh = d.Wh;
+    f.W = d.W;
+    f.Rho = d.Rho;
+    f.p = d.p;
+    if (rk == 0) {
+        f.Mhk = d.Mh;
+        f.Whk = d.Wh;
+        f.Wk = d.W;
+        f.Rhok = d.Rho;
+        f.pk = d.p;
+        f.Whk_out = d.WhkA;
+    } else {
+        f.Mhk = d.Mhk;
+        f.Whk = rk == 1 ? d.WhkA : d.WhkB;
+        f.Wk = d.Wk;
+        f.Rhok = d.Rhok;
+        f.pk = d.pk;
+        f.Whk_out = rk == 1 ? d.WhkB : d.WhkA;
+    }
+    f.Mhk_out = d.Mhk;
+    f.Wk_out = d.Wk;
+    f.Rhok_out = d.Rhok;
+    f.pk_out = d.pk;
+    return f;
+}
+
+static int thor_step(Dyn &d) {
+    const double N = (double)d.P * d.nv, B = 8.0 * N;
+    const int ns_total = d.prm.n_substeps;
+    const int hydiff = d.prm.HyDiff, divdamp = d.prm.DivDampP;
+    const dim3 blk(kTile), grid_s((d.P + kTile - 1) / kTile, (d.nv + kLC - 1) / kLC);
+    const dim3 grid_c((d.P + kTile - 1) / kTile);
+    const dim3 grid_t((d.P + kThomasBlock - 1) / kThomasBlock);
+    const size_t lds_t = (size_t)2 * kThomasBlock * (d.nv + 1) * sizeof(double);
+    hipStream_t st = d.stream;
+    for (int rk = 0; rk < 3; rk++) {
+        const int ns_it = rk == 0 ? 1 : (rk == 1 ? ns_total / 2 : ns_total);
+        const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
+        Fields f = stage_fields(d, rk);
+        const Geo g = d.g;
+        launch(d, "column_prep", 17 * B, [&] { k_column_prep<<<grid_c, blk, 0, st>>>(g, f); });
+        launch(d, "adv_diff1", (hydiff ? 18 : 9) * B, [&] { k_adv_diff1<kLC><<<grid_s, blk, 0, st>>>(g, f, hydiff); });
+        if (hydiff || divdamp)
+            launch(d, "diff2_divm0", ((hydiff ? 16 : 0) + (divdamp ? 7 : 0)) * B, [&] {
+                k_diff2_divm0<kLC><<<grid_s, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
+            });
+        launch(d, "slow_modes", ((divdamp ? 3 : 0) + 29 + (rk ? 12 : 6)) * B,
+               [&] { k_slow<kLC><<<grid_s, blk, 0, st>>>(g, f, divdamp, rk); });
+        if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<kLC><<<grid_s, blk, 0, st>>>(g, f); });
+        for (int ns = 0; ns < ns_it; ns++) {
+            const int next = divdamp && ns < ns_it - 1;
+            launch(d, "momentum", ((divdamp ? 3 : 0) + 10) * B,
+                   [&] { k_momentum<kLC><<<grid_s, blk, 0, st>>>(g, f, divdamp, times); });
+            launch(d, "thomas", 13 * B, [&] { k_thomas<<<grid_t, kThomasBlock, lds_t, st>>>(g, f, times); });
+            launch(d, "density_pressure", (19 + (next ? 3 : 0)) * B,
+                   [&] { k_density_pressure<kLC><<<grid_s, blk, 0, st>>>(g, f, times, next); });
+        }
+        launch(d, "update_rk2", 19 * B, [&] { k_update_rk2<kLC><<<grid_s, blk, 0, st>>>(g, f); });
+    }
+    std::swap(d.Mh, d.Mhk);
+    std::swap(d.Wh, d.WhkA);
+    std::swap(d.W, d.Wk);
+    std::swap(d.Rho, d.Rhok);
+    std::swap(d.p, d.pk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("thor step launch: ") + hipGetErrorString(e));
+    return THOR_OK;
+}
+
+static int profx(Dyn &d, int forcing) {
+    if (forcing == THOR_FORCING_NONE) return THOR_OK;
+    if (forcing != THOR_FORCING_HELD_SUAREZ) return fail(THOR_ERR_BAD_ARG, "unknown forcing");
+    Fields f = stage_fields(d, 0);
+    const Geo g = d.g;
+    const dim3 blk(kTile), grid_c((d.P + kTile - 1) / kTile);
+    const double B = 8.0 * d.P * d.nv;
+    launch(d, "profx_held_suarez", 12 * B,
+           [&] { k_profx_hs<<<grid_c, blk, 0, d.stream>>>(g, f, d.prm.timestep, 1); });
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("profx launch: ") + hipGetErrorString(e));
+    return THOR_OK;
+}
+
+// Resolves a field name to (device pointer, levels, components).
+static bool field_ptr(Dyn &d, const std::string &n, double *&ptr, int &nlev, int &ncomp) {
+    const int nv = d.nv;
+    ncomp = 1;
+    nlev = nv;
+    if (n == "Mh") { ptr = d.Mh; ncomp = 3; }
+    else if (n == "Wh") { ptr = d.Wh; nlev = nv + 1; }
+    else if (n == "W") ptr = d.W;
+    else if (n == "Rho") ptr = d.Rho;
+    else if (n == "pressure") ptr = d.p;
+    else if (n == "temperature") ptr = d.f.T;
+    else if (n == "pt") ptr = d.f.pt;
+    else if (n == "h") ptr = d.f.h;
+    else if (n == "gtil") ptr = d.f.gtil;
+    else if (n == "hh") { ptr = d.f.hh; nlev = nv + 1; }
+    else if (n == "pth") { ptr = d.f.pth; nlev = nv + 1; }
+    else if (n == "gtilh") { ptr = d.f.gtilh; nlev = nv + 1; }
+    else if (n == "profx_Qheat") ptr = d.f.Qheat;
+    else if (n == "profx_dMh") { ptr = d.f.dMh; ncomp = 3; }
+    else if (n == "profx_dWh") { ptr = d.f.dWh; nlev = nv + 1; }
+    else return false;
+    return true;
+}
+
+static int copy_field(Dyn &d, double *dev, int nlev, int ncomp, double *user, bool to_device, bool user_on_device) {
+    const long n = (long)d.P * nlev * ncomp;
+    const size_t plane = (size_t)ncomp * nlev * d.S;
+    if (user_on_device) {
+        const int tb = 256;
+        const long nb = (n + tb - 1) / tb;
+        if (to_device) k_aos_to_soa<<<nb, tb, 0, d.stream>>>(user, dev, d.P, d.S, nlev, ncomp);
+        else k_soa_to_aos<<<nb, tb, 0, d.stream>>>(dev, user, d.P, d.S, nlev, ncomp);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        return THOR_OK;
+    }
+    std::vector<double> tmp(plane, 0.0);
+    if (to_device) {
+        for (int p = 0; p < d.P; p++)
+            for (int lev = 0; lev < nlev; lev++)
+                for (int c = 0; c < ncomp; c++)
+                    tmp[((size_t)c * nlev + lev) * d.S + p] = user[((size_t)p * nlev + lev) * ncomp + c];
+        HIP_TRY(hipMemcpyAsync(dev, tmp.data(), plane * sizeof(double), hipMemcpyHostToDevice, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+    } else {
+        HIP_TRY(hipMemcpyAsync(tmp.data(), dev, plane * sizeof(double), hipMemcpyDeviceToHost, d.stream));
+        HIP_TRY(hipStreamSynchronize(d.stream));
+        for (int p = 0; p < d.P; p++)
+            for (int lev = 0; lev < nlev; lev++)
+                for (int c = 0; c < ncomp; c++)
+                    user[((size_t)p * nlev + lev) * ncomp + c] = tmp[((size_t)c * nlev + lev) * d.S + p];
+    }
+    return THOR_OK;
+}
+
+} // namespace thor
+
+using thor::Dyn;
+struct thor_dyn : Dyn {};
+
+extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_params *prm, thor_dyn **out) {
+    using namespace thor;
+    if (!gd || !prm || !out) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: null argument");
+    if (prm->HyDiff && prm->HyDiffOrder != 4)
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: only HyDiffOrder = 4 is supported (the reference's "
+                                          "intermediate hyperdiffusion passes race, thor_diff.h:214-217)");
+    if (prm->n_substeps < 2 || prm->n_substeps % 2)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: n_substeps must be even and >= 2");
+    if (prm->precision != THOR_PRECISION_FP64)
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: precision mode not built");
+    if (gd->nv < 3 || gd->point_num < 12) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: grid too small");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(THOR_ERR_NO_DEVICE, "thor_dyn_create: no HIP device");
+    thor_dyn *h = new thor_dyn();
+    Dyn &d = *h;
+    d.prm = *prm;
+    d.P = gd->point_num;
+    d.nv = gd->nv;
+    d.S = (d.P + kAlign - 1) / kAlign * kAlign;
+    const int P = d.P, S = d.S, nv = d.nv;
+    auto cleanup_fail = [&](int code) {
+        thor_dyn_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: stream"));
+
+    // Geometry in level-planar layout.
+    std::vector<int> ring((size_t)6 * S, 0), nfc(S, 0);
+    std::vector<char> pent(P, 0);
+    for (int k = 0; k < 12; k++)
+        if (gd->pent_ind[k] >= 0 && gd->pent_ind[k] < P) pent[gd->pent_ind[k]] = 1;
+    for (int p = 0; p < P; p++) {
+        nfc[p] = pent[p] ? 5 : 6;
+        for (int m = 0; m < 6; m++) {
+            int q = gd->point_local[p * 6 + m];
+            if (pent[p] && m == 5) q = gd->point_local[p * 6 + 4];
+            if (q < 0 || q >= P) return cleanup_fail(fail(THOR_ERR_BAD_ARG, "thor_dyn_create: bad point_local"));
+            ring[(size_t)m * S + p] = q;
+        }
+    }
+    auto planar = [&](const double *src, int ncoef) {
+        std::vector<double> v((size_t)ncoef * S, 0.0);
+        for (int p = 0; p < P; p++)
+            for (int c = 0; c < ncoef; c++) v[(size_t)c * S + p] = src[(size_t)p * ncoef + c];
+        return v;
+    };
+    std::vector<double> Kdh4(nv), Kdhz(nv);
+    for (int lev = 0; lev < nv; lev++) {
+        double dbar = std::sqrt(2 * M_PI / 5) * prm->A / (std::pow(2, prm->glevel));
+        Kdh4[lev] = (prm->Diffc) * std::pow(dbar, 1.0 * prm->HyDiffOrder) / prm->timestep;
+        Kdhz[lev] = (prm->DivDampc) * std::pow(dbar, 4.) / prm->timestep;
+    }
+    d.diff_sign = std::pow(-1.0, prm->HyDiffOrder / 2);
+
+    auto up_d = [&](const std::vector<double> &v, const double **dst) -> bool {
+        double *ptr;
+        if (hipMalloc((void **)&ptr, v.size() * sizeof(double)) != hipSuccess) return false;
+        d.allocs.push_back(ptr);
+        if (hipMemcpy(ptr, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return false;
+        *dst = ptr;
+        return true;
+    };
+    auto up_i = [&](const std::vector<int> &v, const int **dst) -> bool {
+        int *ptr;
+        if (hipMalloc((void **)&ptr, v.size() * sizeof(int)) != hipSuccess) return false;
+        d.allocs.push_back(ptr);
+        if (hipMemcpy(ptr, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return false;
+        *dst = ptr;
+        return true;
+    };
+    Geo &g = d.g;
+    g.P = P;
+    g.S = S;
+    g.nv = nv;
+    g.nr = gd->nr;
+    g.A = prm->A;
+    g.Omega = prm->Omega;
+    g.Gravit = prm->Gravit;
+    g.P_Ref = prm->P_Ref;
+    g.Rd = prm->Rd;
+    g.Cp = prm->Cp;
+    g.deep = prm->DeepModel;
+    g.nonhydro = prm->NonHydro;
+    g.ghv = prm->GravHeightVar;
+    bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_d(planar(gd->div, 21), &g.div)
+              && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
+              && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)
+              && up_d(planar(gd->nvecte, 18), &g.nte) && up_d(planar(gd->areasTr, 6), &g.areasTr)
+              && up_d(planar(gd->func_r, 3), &g.fr) && up_d(planar(gd->lonlat, 2), &g.lonlat)
+              && up_d(planar(gd->areasT, 1), &g.areasT)
+              && up_d(std::vector<double>(gd->Altitude, gd->Altitude + nv), &g.Alt)
+              && up_d(std::vector<double>(gd->Altitudeh, gd->Altitudeh + nv + 1), &g.Alth)
+              && up_d(Kdh4, &g.Kdh4) && up_d(Kdhz, &g.Kdhz);
+    if (!ok) return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: geometry upload"));
+
+    const size_t n1 = (size_t)nv * S, ni = (size_t)(nv + 1) * S, n3 = 3 * n1;
+    Fields &f = d.f;
+    hipError_t e = hipSuccess;
+    auto A = [&](double **ptr, size_t n) {
+        if (e == hipSuccess) e = dalloc(d, ptr, n);
+    };
+    A(&d.Mh, n3); A(&d.Wh, ni); A(&d.W, n1); A(&d.Rho, n1); A(&d.p, n1);
+    A(&d.Mhk, n3); A(&d.WhkA, ni); A(&d.WhkB, ni); A(&d.Wk, n1); A(&d.Rhok, n1); A(&d.pk, n1);
+    A(&f.Mhs, n3); A(&f.Whs, ni); A(&f.Rhos, n1); A(&f.ps, n1);
+    A(&f.T, n1); A(&f.h, n1); A(&f.pt, n1); A(&f.gtil, n1);
+    A(&f.hh, ni); A(&f.pth, ni); A(&f.gtilh, ni);
+    A(&f.Adv, n3); A(&f.diff, 6 * n1); A(&f.diffmh, n3);
+    A(&f.diffw, n1); A(&f.diffrh, n1); A(&f.diffpr, n1); A(&f.divg, n3);
+    A(&f.SlowMh, n3); A(&f.SlowWh, ni); A(&f.SlowRho, n1); A(&f.Slowp, n1);
+    A(&f.Qheat, n1); A(&f.dMh, n3); A(&f.dWh, ni);
+    const long ncell = (long)P * nv;
+    d.npart = (int)((ncell + 2047) / 2048);
+    A(&d.partials, (size_t)5 * d.npart);
+    if (e == hipSuccess) {
+        e = hipMalloc((void **)&f.flags, sizeof(unsigned int));
+        if (e == hipSuccess) {
+            d.allocs.push_back(f.flags);
+            e = hipMemset(f.flags, 0, sizeof(unsigned int));
+        }
+    }
+    if (e != hipSuccess) return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: alloc: ") + hipGetErrorString(e)));
+    *out = h;
+    return THOR_OK;
+}
+
+extern "C" void thor_dyn_destroy(thor_dyn *h) {
+    if (!h) return;
+    if (h->stream) hipStreamSynchronize(h->stream);
+    for (auto &pd : h->pending) {
+        hipEventDestroy(pd.a);
+        hipEventDestroy(pd.b);
+    }
+    for (auto ev : h->free_events) hipEventDestroy(ev);
+    for (void *ptr : h->allocs) hipFree(ptr);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+static thor_status state_io(thor_dyn *h, const thor_state *s, bool to_device) {
+    using namespace thor;
+    if (!h || !s) return fail(THOR_ERR_BAD_ARG, "state: null argument");
+    double *user[5] = {s->Mh, s->Wh, s->W, s->Rho, s->pressure};
+    const char *names[5] = {"Mh", "Wh", "W", "Rho", "pressure"};
+    for (int i = 0; i < 5; i++) {
+        if (!user[i]) return fail(THOR_ERR_BAD_ARG, "state: null field pointer");
+        double *dev;
+        int nlev, ncomp;
+        field_ptr(*h, names[i], dev, nlev, ncomp);
+        int rc = copy_field(*h, dev, nlev, ncomp, user[i], to_device, s->on_device != 0);
+        if (rc != THOR_OK) return rc;
+    }
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_set_state(thor_dyn *h, const thor_state *s) { return state_io(h, s, true); }
+extern "C" thor_status thor_dyn_get_state(thor_dyn *h, const thor_state *s) { return state_io(h, s, false); }
+
+extern "C" thor_status thor_dyn_thor(thor_dyn *h) {
+    if (!h) return thor::fail(THOR_ERR_BAD_ARG, "thor_dyn_thor: null handle");
+    return thor::thor_step(*h);
+}
+
+extern "C" thor_status thor_dyn_profx(thor_dyn *h, int forcing) {
+    if (!h) return thor::fail(THOR_ERR_BAD_ARG, "thor_dyn_profx: null handle");
+    return thor::profx(*h, forcing);
+}
+
+extern "C" thor_status thor_dyn_run(thor_dyn *h, int nsteps, int forcing) {
+    if (!h || nsteps < 0) return thor::fail(THOR_ERR_BAD_ARG, "thor_dyn_run: bad argument");
+    for (int i = 0; i < nsteps; i++) {
+        int rc = thor::profx(*h, forcing);
+        if (rc != THOR_OK) return rc;
+        rc = thor::thor_step(*h);
+        if (rc != THOR_OK) return rc;
+    }
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_sync(thor_dyn *h) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "thor_dyn_sync: null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    drain_timing(*h);
+    unsigned int flags = 0;
+    HIP_TRY(hipMemcpy(&flags, h->f.flags, sizeof(unsigned int), hipMemcpyDeviceToHost));
+    if (flags) {
+        HIP_TRY(hipMemset(h->f.flags, 0, sizeof(unsigned int)));
+        if (flags & FLAG_NAN_T) return fail(THOR_ERR_NAN, "NaN in temperature after ProfX (profx_driver.cu:389-396)");
+        return fail(THOR_ERR_NAN, "NaN in pressure in Density_Pressure_Eqs (thor_fastmodes.h:562-569)");
+    }
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_globdiag(thor_dyn *h, double out[5]) {
+    using namespace thor;
+    if (!h || !out) return fail(THOR_ERR_BAD_ARG, "thor_dyn_globdiag: bad argument");
+    Fields f = stage_fields(*h, 0);
+    const long n = (long)h->P * h->nv;
+    k_globdiag<<<h->npart, 1024, 0, h->stream>>>(h->g, f, h->partials, n);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part((size_t)5 * h->npart);
+    HIP_TRY(hipMemcpyAsync(part.data(), h->partials, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < 5; q++) {
+        double s = 0.0;
+        for (int b = 0; b < h->npart; b++) s += part[(size_t)q * h->npart + b];
+        out[q] = s;
+    }
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_get_field(thor_dyn *h, const char *name, double *dst, long n) {
+    using namespace thor;
+    if (!h || !name || !dst) return fail(THOR_ERR_BAD_ARG, "get_field: null argument");
+    double *dev;
+    int nlev, ncomp;
+    if (!field_ptr(*h, name, dev, nlev, ncomp)) return fail(THOR_ERR_BAD_ARG, std::string("get_field: unknown ") + name);
+    if (n != (long)h->P * nlev * ncomp) return fail(THOR_ERR_BAD_ARG, "get_field: size mismatch");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return copy_field(*h, dev, nlev, ncomp, dst, false, false);
+}
+
+extern "C" thor_status thor_dyn_set_field(thor_dyn *h, const char *name, const double *src, long n) {
+    using namespace thor;
+    if (!h || !name || !src) return fail(THOR_ERR_BAD_ARG, "set_field: null argument");
+    double *dev;
+    int nlev, ncomp;
+    if (!field_ptr(*h, name, dev, nlev, ncomp)) return fail(THOR_ERR_BAD_ARG, std::string("set_field: unknown ") + name);
+    if (n != (long)h->P * nlev * ncomp) return fail(THOR_ERR_BAD_ARG, "set_field: size mismatch");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return copy_field(*h, dev, nlev, ncomp, const_cast<double *>(src), true, false);
+}
+
+extern "C" thor_status thor_dyn_kernel_timing(thor_dyn *h, int enable) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "kernel_timing: null handle");
+    drain_timing(*h);
+    h->timing = enable != 0;
+    if (enable) {
+        h->stats.clear();
+        h->stat_order.clear();
+    }
+    return THOR_OK;
+}
+
+extern "C" int thor_dyn_kernel_stat_count(thor_dyn *h) {
+    if (!h) return 0;
+    thor::drain_timing(*h);
+    return (int)h->stat_order.size();
+}
+
+extern "C" thor_status thor_dyn_kernel_stat(thor_dyn *h, int i, const char **name, long *launches, double *total_ms,
+                                            double *bytes_per_launch) {
+    using namespace thor;
+    if (!h || i < 0 || i >= (int)h->stat_order.size()) return fail(THOR_ERR_BAD_ARG, "kernel_stat: bad index");
+    drain_timing(*h);
+    const std::string &n = h->stat_order[i];
+    const KernelStat &s = h->stats[n];
+    if (name) *name = n.c_str();
+    if (launches) *launches = s.launches;
+    if (total_ms) *total_ms = s.total_ms;
+    if (bytes_per_launch) *bytes_per_launch = s.bytes;
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_time_run(thor_dyn *h, int nsteps, int forcing, double *ms) {
+    using namespace thor;
+    if (!h || !ms || nsteps < 1) return fail(THOR_ERR_BAD_ARG, "time_run: bad argument");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    HIP_TRY(hipEventRecord(a, h->stream));
+    int rc = thor_dyn_run(h, nsteps, forcing);
+    HIP_TRY(hipEventRecord(b, h->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    *ms = f;
+    drain_timing(*h);
+    return rc;
+}
