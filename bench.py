@@ -1,0 +1,141 @@
+"""Benchmark: simulated days per wall-day of the HE-VI dynamical-core step.
+
+Workload (the configuration the headline metric is quoted on): Held-Suarez
+Earth, icosahedral glevel 6 (40 962 columns), 48 levels, 36 km top, dt = 300 s,
+8 acoustic substeps, 4th-order hyperdiffusion and 3D divergence damping on.
+One "step" = ProfX (Held-Suarez forcing) + Thor (one large RK3 step), the same
+sequence as THOR's main loop (esp.cu:1393-1398). Synthetic input: the rest
+state plus seeded +-1e-3 K temperature noise; all state resident in HBM.
+
+Prints ONE JSON line. Extra objects:
+  roofline     - dominant kernel: algorithmic bytes per launch / average launch
+                 time from HIP events on the library's stream, against 8 TB/s;
+  cpu_baseline - the serial FP64 oracle (THOR's kernels as a host loop) timed
+                 on one host core for one step of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Sim-days/wall-day at g6 L48, mixed+MLP vs FP64 at stated err; HBM GB/s % peak"
+HBM_PEAK_GBS = 8000.0
+CACHE = os.path.join(ROOT, ".thor_cache")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--glevel", type=int, default=6)
+    ap.add_argument("--nv", type=int, default=48)
+    ap.add_argument("--dt", type=float, default=300.0)
+    ap.add_argument("--substeps", type=int, default=8)
+    ap.add_argument("--variant", default="", help='"" production build, "pm" parity build')
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (rhombus-sharded) "
+                                   "path is a design only, so --gpus must be 1"}))
+        sys.exit(2)
+
+    import thor_amd.capi as capi
+
+    os.makedirs(CACHE, exist_ok=True)
+    t0 = time.perf_counter()
+    g = capi.Grid(args.glevel, args.nv, 6.371e6, 36000.0, spring=True, beta=1.15, cache_dir=CACHE)
+    log(f"grid g{args.glevel} L{args.nv}: P={g.P} built in {time.perf_counter() - t0:.1f}s")
+    p = capi.default_params(glevel=args.glevel, timestep=args.dt, n_substeps=args.substeps)
+    st = capi.ic_rest(g, p, seed=1002, temperature_noise=1e-3)
+
+    d = capi.Dyn(g, p, variant=args.variant)
+    d.set_state(st)
+    # warm-up (untimed)
+    d.run(args.warmup)
+    d.sync()
+    # timed region: exactly K steps, synchronised on both sides
+    d.sync()
+    t1 = time.perf_counter()
+    d.run(args.steps)
+    d.sync()
+    wall = time.perf_counter() - t1
+    ms_step = 1000.0 * wall / args.steps
+    sdpd = args.dt * args.steps / wall  # simulated seconds per wall second = sim-days per wall-day
+
+    # per-kernel device times (HIP events on the library's stream) over a few steps
+    nprof = max(3, min(20, args.steps // 5))
+    d.kernel_timing(True)
+    d.run(nprof)
+    d.sync()
+    stats = d.kernel_stats()
+    d.kernel_timing(False)
+    tot = sum(s["total_ms"] for s in stats)
+    for s in sorted(stats, key=lambda s: -s["total_ms"]):
+        avg = s["total_ms"] / s["launches"]
+        gbs = s["bytes_per_launch"] / (avg * 1e-3) / 1e9
+        log(f"  {s['name']:20s} {s['launches'] / nprof:6.1f}/step {avg * 1e3:9.1f} us "
+            f"{100 * s['total_ms'] / tot:5.1f}%  {gbs:7.0f} GB/s alg")
+    dom = max(stats, key=lambda s: s["total_ms"])
+    dom_avg_ms = dom["total_ms"] / dom["launches"]
+    achieved = dom["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
+    step_bytes = sum(s["bytes_per_launch"] * s["launches"] for s in stats) / nprof
+    step_dev_ms = tot / nprof
+    roofline = {
+        "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "avg_launch_us": round(dom_avg_ms * 1e3, 2),
+        "step_alg_GBs": round(step_bytes / (step_dev_ms * 1e-3) / 1e9, 1),
+        "step_alg_frac": round(step_bytes / (step_dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(g, p, variant=args.variant)
+        o.set_state(st)
+        tc = time.perf_counter()
+        o.steps(1)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": round(args.dt / cpu_s, 3), "unit": "sim-days/wall-day", "cores": 1, "kind": "port",
+               "sample": f"1 step (ProfX + Thor) of the same g{args.glevel} L{args.nv} workload, serial FP64 "
+                         f"oracle on 1 host core: {cpu_s:.2f} s"}
+        # parity of the same first step on the GPU
+        d2 = capi.Dyn(g, p, variant=args.variant)
+        d2.set_state(st)
+        d2.run(1)
+        d2.sync()
+        gs, os_ = d2.get_state(), o.get_state()
+        errs = {k: float(np.abs(gs[k] - os_[k]).max() / max(np.abs(os_[k]).max(), 1e-300))
+                for k in ("Mh", "Wh", "Rho", "pressure")}
+        cpu["first_step_max_rel_err"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        d2.close()
+
+    out = {
+        "metric": METRIC, "value": round(sdpd, 1), "unit": "sim-days/wall-day", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (rest state + seeded 1e-3 K noise)",
+        "config": {"workload": f"Held-Suarez Earth g{args.glevel} L{args.nv} ({g.P} columns), dt={args.dt:g} s, "
+                               f"{args.substeps} acoustic substeps, HyDiff 4 + DivDamp, FP64",
+                   "glevel": args.glevel, "nv": args.nv, "dt": args.dt, "n_substeps": args.substeps,
+                   "precision": "fp64", "build": args.variant or "production"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
