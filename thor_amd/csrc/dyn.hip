@@ -144,9 +144,9 @@ static int thor_step(Dyn &d) {
     const int ns_total = d.prm.n_substeps;
     const int hydiff = d.prm.HyDiff, divdamp = d.prm.DivDampP;
     const dim3 blk(kTile), grid_s((d.P + kTile - 1) / kTile, (d.nv + kLC - 1) / kLC);
+    const dim3 grid_t2(8 * d.g.tiles_per_xcd * d.nv);  // LDS-tiled stencils: one tile x one level per block
     const dim3 grid_c((d.P + kTile - 1) / kTile);
     const dim3 grid_t((d.P + kThomasBlock - 1) / kThomasBlock);
-    const size_t lds_t = (size_t)2 * kThomasBlock * (d.nv + 1) * sizeof(double);
     hipStream_t st = d.stream;
     for (int rk = 0; rk < 3; rk++) {
         const int ns_it = rk == 0 ? 1 : (rk == 1 ? ns_total / 2 : ns_total);
@@ -154,21 +154,23 @@ static int thor_step(Dyn &d) {
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
         launch(d, "column_prep", 17 * B, [&] { k_column_prep<<<grid_c, blk, 0, st>>>(g, f); });
-        launch(d, "adv_diff1", (hydiff ? 18 : 9) * B, [&] { k_adv_diff1<kLC><<<grid_s, blk, 0, st>>>(g, f, hydiff); });
+        launch(d, "adv_diff1", (hydiff ? 18 : 9) * B, [&] { k_adv_diff1<<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
         if (hydiff || divdamp)
             launch(d, "diff2_divm0", ((hydiff ? 16 : 0) + (divdamp ? 7 : 0)) * B, [&] {
-                k_diff2_divm0<kLC><<<grid_s, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
+                k_diff2_divm0<<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
             });
         launch(d, "slow_modes", ((divdamp ? 3 : 0) + 29 + (rk ? 12 : 6)) * B,
-               [&] { k_slow<kLC><<<grid_s, blk, 0, st>>>(g, f, divdamp, rk); });
-        if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<kLC><<<grid_s, blk, 0, st>>>(g, f); });
+               [&] { k_slow<<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
+        if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<<<grid_t2, blk, 0, st>>>(g, f); });
+        launch(d, "thomas_factor", 5 * B, [&] { k_thomas_factor<<<grid_t, kThomasBlock, 0, st>>>(g, f, times); });
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
             launch(d, "momentum", ((divdamp ? 3 : 0) + 10) * B,
-                   [&] { k_momentum<kLC><<<grid_s, blk, 0, st>>>(g, f, divdamp, times); });
-            launch(d, "thomas", 13 * B, [&] { k_thomas<<<grid_t, kThomasBlock, lds_t, st>>>(g, f, times); });
+                   [&] { k_momentum<<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
+            launch(d, "prepare", 8 * B, [&] { k_prepare<<<grid_t2, blk, 0, st>>>(g, f); });
+            launch(d, "thomas_solve", 12 * B, [&] { k_thomas_solve<<<grid_t, kThomasBlock, 0, st>>>(g, f, times); });
             launch(d, "density_pressure", (19 + (next ? 3 : 0)) * B,
-                   [&] { k_density_pressure<kLC><<<grid_s, blk, 0, st>>>(g, f, times, next); });
+                   [&] { k_density_pressure<<<grid_t2, blk, 0, st>>>(g, f, times, next); });
         }
         launch(d, "update_rk2", 19 * B, [&] { k_update_rk2<kLC><<<grid_s, blk, 0, st>>>(g, f); });
     }
@@ -333,6 +335,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     g.S = S;
     g.nv = nv;
     g.nr = gd->nr;
+    g.tiles_per_xcd = (gd->nr + 1 + 7) / 8;
     g.A = prm->A;
     g.Omega = prm->Omega;
     g.Gravit = prm->Gravit;
@@ -342,7 +345,11 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     g.deep = prm->DeepModel;
     g.nonhydro = prm->NonHydro;
     g.ghv = prm->GravHeightVar;
-    bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_d(planar(gd->div, 21), &g.div)
+    std::vector<int> halo((size_t)gd->nr * NHALO);
+    for (int r = 0; r < gd->nr; r++)
+        for (int t = 0; t < NHALO; t++) halo[(size_t)r * NHALO + t] = gd->maps[(size_t)r * NHL2 + halo_pos(t)];
+    if (gd->nr * kTile + 2 != P) return cleanup_fail(fail(THOR_ERR_BAD_ARG, "thor_dyn_create: P != 256 nr + 2"));
+    bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
               && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)
               && up_d(planar(gd->nvecte, 18), &g.nte) && up_d(planar(gd->areasTr, 6), &g.areasTr)
@@ -367,6 +374,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     A(&f.Adv, n3); A(&f.diff, 6 * n1); A(&f.diffmh, n3);
     A(&f.diffw, n1); A(&f.diffrh, n1); A(&f.diffpr, n1); A(&f.divg, n3);
     A(&f.SlowMh, n3); A(&f.SlowWh, ni); A(&f.SlowRho, n1); A(&f.Slowp, n1);
+    A(&f.Sp, n1); A(&f.Sd, n1); A(&f.tAA, n1); A(&f.tCC, n1); A(&f.tTT, n1);
     A(&f.Qheat, n1); A(&f.dMh, n3); A(&f.dWh, ni);
     const long ncell = (long)P * nv;
     d.npart = (int)((ncell + 2047) / 2048);

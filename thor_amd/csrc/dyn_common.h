@@ -20,8 +20,10 @@ constexpr int kAlign = 64;
 
 struct Geo {
     int P, S, nv, nr;
+    int tiles_per_xcd;  // ceil((nr + 1) / 8): tiles (incl. the pole block) per XCD
     const int *ring;  // [6][S]: pt1..pt6 (pentagons/poles: ring[4] = ring[5])
     const int *nfc;   // [S]: 6 hexagon, 5 pentagon or pole, 0 padding
+    const int *halo;  // [nr][66]: global index of each tile's halo points (-1: missing corner)
     const double *div, *grad;                  // [21][S]  (j * 3 + k)
     const double *areas, *noa, *nti, *nte;     // [18][S]
     const double *areasTr;                     // [6][S]
@@ -49,6 +51,8 @@ struct Fields {
     double *diffw, *diffrh, *diffpr;      // [nv]
     double *divg;                         // [3][nv]
     double *SlowMh, *SlowWh, *SlowRho, *Slowp;
+    double *Sp, *Sd;                      // implicit-solve sources [nv]
+    double *tAA, *tCC, *tTT;              // Thomas factorisation of the stage [nv]
     double *Qheat, *dMh, *dWh;            // ProfX tendencies
     unsigned int *flags;                  // bit 0: NaN pressure, bit 1: NaN temperature
 };

@@ -3,15 +3,28 @@
 // Each kernel fuses one or more of THOR's kernels; inside a fused kernel the
 // per-cell arithmetic is the reference's, operation for operation (compiled
 // with -ffp-contract=off), so the FP64 path reproduces THOR's own results.
-// Horizontal stencils read the six ring neighbours through a per-point index
-// table instead of THOR's shared-memory halo tiles; stencil kernels map one
-// thread to one point and loop over a chunk of LC levels so the per-point
-// geometry stays in L1/L2 across the levels of a chunk.
+// Horizontal stencils run one 256-thread block per (rhombus tile, level): the
+// tile and its one-point halo are staged in LDS, one thread per point; the two
+// poles are one extra block that gathers its neighbours directly. Column
+// kernels (vertical sweeps, the Thomas solve) map one thread to one column.
 #pragma once
 #include "dyn_common.h"
 #include "portable_math.h"
 
 namespace thor {
+
+// Minimum resident 256-thread blocks per CU requested from the compiler for
+// the LDS-tiled stencil kernels (bounds their register use).
+#ifndef THOR_STENCIL_MIN_BLOCKS
+#define THOR_STENCIL_MIN_BLOCKS 2
+#endif
+#define STENCIL_BOUNDS __launch_bounds__(256, THOR_STENCIL_MIN_BLOCKS)
+// Unroll factor of the 7-slot operator sums (register use vs load latency).
+#ifndef THOR_SLOT_UNROLL
+#define THOR_SLOT_UNROLL 1
+#endif
+#define PRAGMA_STR(x) _Pragma(#x)
+#define SLOT_UNROLL(n) PRAGMA_STR(unroll n)
 
 #define LV(arr, lev) (arr)[(size_t)(lev) * S + p]
 #define LVP(arr, lev, q) (arr)[(size_t)(lev) * S + (q)]
@@ -29,142 +42,230 @@ __device__ __forceinline__ void load_slots(const Geo &g, int p, int gi[7]) {
     for (int m = 0; m < 6; m++) gi[1 + m] = g.ring[(size_t)m * S + p];
 }
 
-// Laplacian of a[] around point p (Diffusion_Op, thor_diff.h:359-494):
-// a[0] is the centre, a[1 + m] the value at ring neighbour m.
-template <int NF>
-__device__ __forceinline__ double laplacian(const Geo &g, int p, double rscale, const double a[7]) {
-    const size_t S = g.S;
-    double lap = 0.0, lap1 = 0, lap2 = 0, lap3 = 0, lap4, lap5, lap6;
-    double lapx1 = 0, lapx2 = 0, lapy1 = 0, lapy2 = 0, lapz1 = 0, lapz2 = 0, AT1 = 0, AT2, astar;
-    const double a0 = a[0];
-#pragma unroll
-    for (int j = 0; j < NF; j++) {
-        const int jp1 = (j + 1) % NF, jp2 = (j + 2) % NF;
-        const double a1 = a[1 + j], a2 = a[1 + jp1], a3 = a[1 + jp2];
-        const double vdiff = 0.5 * rscale;
-        if (j == 0) {
-            const double aTr = g.areasTr[(size_t)j * S + p];
-            AT1 = rscale / aTr;
-            astar = (g.areas[(size_t)(j * 3 + 0) * S + p] * a0 + g.areas[(size_t)(j * 3 + 1) * S + p] * a1
-                     + g.areas[(size_t)(j * 3 + 2) * S + p] * a2)
-                    / aTr;
-            lap1 = (0.5 * (a1 + a0) - astar);
-            lap2 = (0.5 * (a1 + a2) - astar);
-            lap3 = (0.5 * (a0 + a2) - astar);
-        }
-        const double aTr2 = g.areasTr[(size_t)jp1 * S + p];
-        AT2 = rscale / aTr2;
-        astar = (g.areas[(size_t)(jp1 * 3 + 0) * S + p] * a0 + g.areas[(size_t)(jp1 * 3 + 1) * S + p] * a2
-                 + g.areas[(size_t)(jp1 * 3 + 2) * S + p] * a3)
-                / aTr2;
-        lap4 = (0.5 * (a0 + a2) - astar);
-        lap5 = (0.5 * (a2 + a3) - astar);
-        lap6 = (0.5 * (a0 + a3) - astar);
-        const double *ti = g.nti, *te = g.nte;
-        if (j == 0) {
-            lapx1 = (-lap1 * ti[(size_t)(j * 3 + 0) * S + p] + lap2 * te[(size_t)(j * 3 + 0) * S + p]
-                     + lap3 * ti[(size_t)(jp1 * 3 + 0) * S + p])
-                    * AT1;
-            lapy1 = (-lap1 * ti[(size_t)(j * 3 + 1) * S + p] + lap2 * te[(size_t)(j * 3 + 1) * S + p]
-                     + lap3 * ti[(size_t)(jp1 * 3 + 1) * S + p])
-                    * AT1;
-            lapz1 = (-lap1 * ti[(size_t)(j * 3 + 2) * S + p] + lap2 * te[(size_t)(j * 3 + 2) * S + p]
-                     + lap3 * ti[(size_t)(jp1 * 3 + 2) * S + p])
-                    * AT1;
-        } else {
-            lapx1 = lapx2;
-            lapy1 = lapy2;
-            lapz1 = lapz2;
-        }
-        lapx2 = (-lap4 * ti[(size_t)(jp1 * 3 + 0) * S + p] + lap5 * te[(size_t)(jp1 * 3 + 0) * S + p]
-                 + lap6 * ti[(size_t)(jp2 * 3 + 0) * S + p])
-                * AT2;
-        lapy2 = (-lap4 * ti[(size_t)(jp1 * 3 + 1) * S + p] + lap5 * te[(size_t)(jp1 * 3 + 1) * S + p]
-                 + lap6 * ti[(size_t)(jp2 * 3 + 1) * S + p])
-                * AT2;
-        lapz2 = (-lap4 * ti[(size_t)(jp1 * 3 + 2) * S + p] + lap5 * te[(size_t)(jp1 * 3 + 2) * S + p]
-                 + lap6 * ti[(size_t)(jp2 * 3 + 2) * S + p])
-                * AT2;
-        lap += ((lapx1 + lapx2) * g.noa[(size_t)(j * 3 + 0) * S + p]
-                + (lapy1 + lapy2) * g.noa[(size_t)(j * 3 + 1) * S + p]
-                + (lapz1 + lapz2) * g.noa[(size_t)(j * 3 + 2) * S + p])
-               * vdiff;
-    }
-    return lap;
+// ---------------------------------------------------------------- LDS tiles
+// A stencil block is one 16x16 rhombus tile at one level. Its values and the
+// one-point halo are staged in LDS in THOR's 18x18 map order (point (x, y) at
+// (y + 1) * 18 + x + 1), so every field value is read from HBM once per block.
+// The two poles are handled by one extra block with direct gathers.
+constexpr int NHL = 18, NHL2 = NHL * NHL, NHALO = 66;
+
+// LDS position of halo entry t (left, right, bottom, top sides, then the
+// (-1,-1) and (16,16) corners; the other two corners are never read).
+__host__ __device__ __forceinline__ int halo_pos(int t) {
+    if (t < 16) return (t + 1) * NHL;
+    if (t < 32) return (t - 16 + 1) * NHL + 17;
+    if (t < 48) return t - 32 + 1;
+    if (t < 64) return 17 * NHL + (t - 48) + 1;
+    if (t == 64) return 0;
+    return 17 * NHL + 17;
 }
 
-__device__ __forceinline__ double laplacian_nf(const Geo &g, int p, int nf, double rscale, const double a[7]) {
-    return nf == 6 ? laplacian<6>(g, p, rscale, a) : laplacian<5>(g, p, rscale, a);
+// Block -> (tile, level) mapping. Workgroups are dispatched round-robin over
+// the 8 XCDs (block i runs on XCD i % 8); each XCD is given a contiguous range
+// of tiles and walks all levels of a tile consecutively, so a tile's geometry
+// and its neighbours' halo lines stay in that XCD's L2 across levels.
+__device__ __forceinline__ bool block_tile(const Geo &g, int &r, int &lev) {
+    const int i = blockIdx.x, xcd = i & 7, idx = i >> 3;
+    r = xcd * g.tiles_per_xcd + idx / g.nv;
+    lev = idx % g.nv;
+    return r <= g.nr;
+}
+
+struct TileThread {
+    int p, nf, ir;
+    // LDS position of stencil slot s: 0 = self, 1..6 = pt1..pt6 (a pentagon's pt5
+    // is its pt6 position, thor_fastmodes.h:126).
+    __device__ __forceinline__ int pos(int s) const {
+        // byte s of kOff: 0, +18, +19, +1, -18, -19, -1 (two's complement)
+        constexpr unsigned long long kOff = 0x00FFEDEE01131200ull;
+        int o = (int)(signed char)((kOff >> (8 * s)) & 0xff);
+        if (s == 5 && nf == 5) o = -1;
+        return ir + o;
+    }
+};
+
+__device__ __forceinline__ TileThread tile_thread(const Geo &g, int r) {
+    TileThread T;
+    const int t = threadIdx.x, x = t & 15, y = t >> 4;
+    T.p = r * kTile + t;
+    T.nf = g.nfc[T.p];
+    T.ir = (y + 1) * NHL + x + 1;
+    return T;
+}
+
+// put(pos, q) stores the values of global point q (q < 0: zeros, the missing
+// pentagon corner) at LDS position pos; called for the thread's own point and,
+// for threads 0..65, one halo point.
+template <class Put> __device__ __forceinline__ void stage_tile(const Geo &g, int r, const TileThread &T, Put put) {
+    put(T.ir, T.p);
+    const int t = threadIdx.x;
+    if (t < NHALO) put(halo_pos(t), g.halo[r * NHALO + t]);
+}
+
+// Laplacians of NV fields around point p, all in one pass over the triangles
+// so each geometry coefficient is read once (Diffusion_Op, thor_diff.h:359-494;
+// poles :524-880). a(v, 0) is field v at the centre, a(v, 1 + m) at ring
+// neighbour m; nf = 6 (hexagon) or 5 (pentagon, pole). The arithmetic of each
+// field is exactly the reference's.
+template <int NV, class GetA>
+__device__ __forceinline__ void laplacians(const Geo &g, int p, int nf, double rscale, GetA a, double out[NV]) {
+    const size_t S = g.S;
+    double a0[NV], a1[NV], a2[NV], lap[NV], lx[NV], ly[NV], lz[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        a0[v] = a(v, 0);
+        a1[v] = a(v, 1);
+        a2[v] = a(v, 2);
+        lap[v] = 0.0;
+    }
+    const double vdiff = 0.5 * rscale;
+#pragma unroll 1
+    for (int j = 0; j < nf; j++) {
+        const int jp1 = j + 1 == nf ? 0 : j + 1;
+        const int jp2 = jp1 + 1 == nf ? 0 : jp1 + 1;
+        const double aTr2 = g.areasTr[(size_t)jp1 * S + p];
+        const double AT2 = rscale / aTr2;
+        const double b0 = g.areas[(size_t)(jp1 * 3 + 0) * S + p], b1 = g.areas[(size_t)(jp1 * 3 + 1) * S + p],
+                     b2 = g.areas[(size_t)(jp1 * 3 + 2) * S + p];
+        const double ti1x = g.nti[(size_t)(jp1 * 3 + 0) * S + p], ti1y = g.nti[(size_t)(jp1 * 3 + 1) * S + p],
+                     ti1z = g.nti[(size_t)(jp1 * 3 + 2) * S + p];
+        const double te1x = g.nte[(size_t)(jp1 * 3 + 0) * S + p], te1y = g.nte[(size_t)(jp1 * 3 + 1) * S + p],
+                     te1z = g.nte[(size_t)(jp1 * 3 + 2) * S + p];
+        const double ti2x = g.nti[(size_t)(jp2 * 3 + 0) * S + p], ti2y = g.nti[(size_t)(jp2 * 3 + 1) * S + p],
+                     ti2z = g.nti[(size_t)(jp2 * 3 + 2) * S + p];
+        const double oax = g.noa[(size_t)(j * 3 + 0) * S + p], oay = g.noa[(size_t)(j * 3 + 1) * S + p],
+                     oaz = g.noa[(size_t)(j * 3 + 2) * S + p];
+        if (j == 0) {
+            const double aTr = g.areasTr[p];
+            const double AT1 = rscale / aTr;
+            const double c0 = g.areas[p], c1 = g.areas[S + p], c2 = g.areas[2 * S + p];
+            const double ti0x = g.nti[p], ti0y = g.nti[S + p], ti0z = g.nti[2 * S + p];
+            const double te0x = g.nte[p], te0y = g.nte[S + p], te0z = g.nte[2 * S + p];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                const double astar = (c0 * a0[v] + c1 * a1[v] + c2 * a2[v]) / aTr;
+                const double lap1 = (0.5 * (a1[v] + a0[v]) - astar);
+                const double lap2 = (0.5 * (a1[v] + a2[v]) - astar);
+                const double lap3 = (0.5 * (a0[v] + a2[v]) - astar);
+                lx[v] = (-lap1 * ti0x + lap2 * te0x + lap3 * ti1x) * AT1;
+                ly[v] = (-lap1 * ti0y + lap2 * te0y + lap3 * ti1y) * AT1;
+                lz[v] = (-lap1 * ti0z + lap2 * te0z + lap3 * ti1z) * AT1;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double a3 = a(v, 1 + jp2);
+            const double astar = (b0 * a0[v] + b1 * a2[v] + b2 * a3) / aTr2;
+            const double lap4 = (0.5 * (a0[v] + a2[v]) - astar);
+            const double lap5 = (0.5 * (a2[v] + a3) - astar);
+            const double lap6 = (0.5 * (a0[v] + a3) - astar);
+            const double x2 = (-lap4 * ti1x + lap5 * te1x + lap6 * ti2x) * AT2;
+            const double y2 = (-lap4 * ti1y + lap5 * te1y + lap6 * ti2y) * AT2;
+            const double z2 = (-lap4 * ti1z + lap5 * te1z + lap6 * ti2z) * AT2;
+            lap[v] += ((lx[v] + x2) * oax + (ly[v] + y2) * oay + (lz[v] + z2) * oaz) * vdiff;
+            lx[v] = x2;
+            ly[v] = y2;
+            lz[v] = z2;
+            a2[v] = a3;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) out[v] = lap[v];
 }
 
 // Divergence-damping operator at point p (DivM_Op, thor_div.h:49-315 and the
 // pole variant :317-535, whose vertical means weight the third corner with
-// areas[.][1]). a[s][k]: momentum (pass 0) or divg (pass 1) at slot s;
-// wl/wt: Wh at the layer's lower/upper interface (pass 0 only).
-template <int NF>
-__device__ __forceinline__ void divm_op(const Geo &g, int p, int lev, bool pole, bool laststep,
-                                        const double a[7][3], const double wl[7], const double wt[7],
+// areas[.][1]). a(s, k): momentum (pass 0) or divg (pass 1) at slot s;
+// wl(s)/wt(s): Wh at the layer's lower/upper interface (pass 0 only).
+template <bool LAST, class GetA, class GetWl, class GetWt>
+__device__ __forceinline__ void divm_nf(const Geo &g, int p, int lev, int nf, bool pole, GetA a, GetWl wl, GetWt wt,
                                         double out[3]) {
+    // Triangle j's quantities (vertical mean, dw/dz, divergence lap1) are the
+    // same expressions as the previous iteration's second-triangle ones
+    // (thor_div.h:207-262), so they are carried instead of recomputed, along
+    // with the geometry of the shared corner.
     const size_t S = g.S;
     const double rs = rscale_at(g, lev);
     const double AT0 = 0.5 * rs;
     double lapx = 0.0, lapy = 0.0, lapz = 0.0;
-    const double dz = laststep ? 0.0 : 1.0 / (g.Alth[lev] - g.Alth[lev + 1]);
-    double a01x = 0, a01y = 0, a01z = 0, a12x = 0, a12y = 0, a12z = 0, AT1 = 0, lap1 = 0, lap2 = 0;
+    const double dz = LAST ? 0.0 : 1.0 / (g.Alth[lev] - g.Alth[lev + 1]);
+    const double a0x = a(0, 0), a0y = a(0, 1), a0z = a(0, 2);
+    const double wl0 = LAST ? 0.0 : wl(0), wt0 = LAST ? 0.0 : wt(0);
 #define AR(j, k) g.areas[(size_t)((j)*3 + (k)) * S + p]
 #define TI(j, k) g.nti[(size_t)((j)*3 + (k)) * S + p]
 #define TE(j, k) g.nte[(size_t)((j)*3 + (k)) * S + p]
-#pragma unroll
-    for (int j = 0; j < NF; j++) {
-        const int jp1 = (j + 1) % NF, jp2 = (j + 2) % NF;
-        const int s1 = 1 + j, s2 = 1 + jp1, s3 = 1 + jp2;
-        const double aT1 = g.areasTr[(size_t)j * S + p], aT2 = g.areasTr[(size_t)jp1 * S + p];
-        double dwdz, dwdz2;
-        if (!laststep) {
-            const double w1 = pole ? AR(j, 1) : AR(j, 2);
-            const double w2 = pole ? AR(jp1, 1) : AR(jp1, 2);
-            const double meanwl = (AR(j, 0) * wl[0] + AR(j, 1) * wl[s1] + w1 * wl[s2]) / aT1;
-            const double meanwt = (AR(j, 0) * wt[0] + AR(j, 1) * wt[s1] + w1 * wt[s2]) / aT1;
-            const double meanwl2 = (AR(jp1, 0) * wl[0] + AR(jp1, 1) * wl[s2] + w2 * wl[s3]) / aT2;
-            const double meanwt2 = (AR(jp1, 0) * wt[0] + AR(jp1, 1) * wt[s2] + w2 * wt[s3]) / aT2;
-            dwdz = (meanwl - meanwt) * dz;
-            dwdz2 = (meanwl2 - meanwt2) * dz;
-        } else {
-            dwdz = 0.0;
-            dwdz2 = 0.0;
+    // corner slot 1 + jp1 values, carried to the next triangle
+    double a2x = a(1, 0), a2y = a(1, 1), a2z = a(1, 2), wl2 = 0.0, wt2 = 0.0;
+    double lap2 = 0.0, dwdz2 = 0.0;
+    double ti1x = TI(0, 0), ti1y = TI(0, 1), ti1z = TI(0, 2);  // nvecti of the current first edge
+    double aT1 = g.areasTr[p];
+    if (!LAST) {
+        wl2 = wl(1);
+        wt2 = wt(1);
+    }
+#pragma unroll 1
+    for (int j = 0; j < nf; j++) {
+        const int jp1 = j + 1 == nf ? 0 : j + 1;
+        const int jp2 = jp1 + 1 == nf ? 0 : jp1 + 1;
+        const int s2 = 1 + jp1, s3 = 1 + jp2;
+        const double a1x = a2x, a1y = a2y, a1z = a2z, wl1 = wl2, wt1 = wt2;
+        a2x = a(s2, 0);
+        a2y = a(s2, 1);
+        a2z = a(s2, 2);
+        if (!LAST) {
+            wl2 = wl(s2);
+            wt2 = wt(s2);
         }
-        if (j == 0) {
-            a01x = a[0][0] + a[s1][0];
-            a01y = a[0][1] + a[s1][1];
-            a01z = a[0][2] + a[s1][2];
-            a12x = a[s1][0] + a[s2][0];
-            a12y = a[s1][1] + a[s2][1];
-            a12z = a[s1][2] + a[s2][2];
-            AT1 = 0.5 * rs / aT1;
+        const double aT2 = g.areasTr[(size_t)jp1 * S + p];
+        const double a3x = a(s3, 0), a3y = a(s3, 1), a3z = a(s3, 2);
+        double dwdz = 0.0;
+        if (!LAST) {
+            if (j == 0) {
+                const double w1 = pole ? AR(j, 1) : AR(j, 2);
+                const double meanwl = (AR(j, 0) * wl0 + AR(j, 1) * wl1 + w1 * wl2) / aT1;
+                const double meanwt = (AR(j, 0) * wt0 + AR(j, 1) * wt1 + w1 * wt2) / aT1;
+                dwdz = (meanwl - meanwt) * dz;
+            } else {
+                dwdz = dwdz2;
+            }
+            const double wl3 = wl(s3), wt3 = wt(s3);
+            const double r0 = AR(jp1, 0), r1 = AR(jp1, 1);
+            const double w2 = pole ? r1 : AR(jp1, 2);
+            const double meanwl2 = (r0 * wl0 + r1 * wl2 + w2 * wl3) / aT2;
+            const double meanwt2 = (r0 * wt0 + r1 * wt2 + w2 * wt3) / aT2;
+            dwdz2 = (meanwl2 - meanwt2) * dz;
         }
         const double AT2 = 0.5 * rs / aT2;
-        const double a02x = a[0][0] + a[s2][0], a02y = a[0][1] + a[s2][1], a02z = a[0][2] + a[s2][2];
-        const double a23x = a[s2][0] + a[s3][0], a23y = a[s2][1] + a[s3][1], a23z = a[s2][2] + a[s3][2];
-        const double a03x = a[0][0] + a[s3][0], a03y = a[0][1] + a[s3][1], a03z = a[0][2] + a[s3][2];
+        const double ti2x = TI(jp1, 0), ti2y = TI(jp1, 1), ti2z = TI(jp1, 2);
+        const double a02x = a0x + a2x, a02y = a0y + a2y, a02z = a0z + a2z;
+        double lap1;
         if (j == 0) {
-            lap1 = (-a01x * TI(j, 0) - a01y * TI(j, 1) - a01z * TI(j, 2) + a12x * TE(j, 0) + a12y * TE(j, 1)
-                    + a12z * TE(j, 2) + a02x * TI(jp1, 0) + a02y * TI(jp1, 1) + a02z * TI(jp1, 2))
+            const double a01x = a0x + a1x, a01y = a0y + a1y, a01z = a0z + a1z;
+            const double a12x = a1x + a2x, a12y = a1y + a2y, a12z = a1z + a2z;
+            const double AT1 = 0.5 * rs / aT1;
+            lap1 = (-a01x * ti1x - a01y * ti1y - a01z * ti1z + a12x * TE(j, 0) + a12y * TE(j, 1) + a12z * TE(j, 2)
+                    + a02x * ti2x + a02y * ti2y + a02z * ti2z)
                    * AT1;
         } else {
             lap1 = lap2;
         }
-        lap2 = (-a02x * TI(jp1, 0) - a02y * TI(jp1, 1) - a02z * TI(jp1, 2) + a23x * TE(jp1, 0)
-                + a23y * TE(jp1, 1) + a23z * TE(jp1, 2) + a03x * TI(jp2, 0) + a03y * TI(jp2, 1)
-                + a03z * TI(jp2, 2))
+        const double a23x = a2x + a3x, a23y = a2y + a3y, a23z = a2z + a3z;
+        const double a03x = a0x + a3x, a03y = a0y + a3y, a03z = a0z + a3z;
+        lap2 = (-a02x * ti2x - a02y * ti2y - a02z * ti2z + a23x * TE(jp1, 0) + a23y * TE(jp1, 1) + a23z * TE(jp1, 2)
+                + a03x * TI(jp2, 0) + a03y * TI(jp2, 1) + a03z * TI(jp2, 2))
                * AT2;
         lapx += (lap1 + dwdz + lap2 + dwdz2) * g.noa[(size_t)(j * 3 + 0) * S + p] * AT0;
         lapy += (lap1 + dwdz + lap2 + dwdz2) * g.noa[(size_t)(j * 3 + 1) * S + p] * AT0;
         lapz += (lap1 + dwdz + lap2 + dwdz2) * g.noa[(size_t)(j * 3 + 2) * S + p] * AT0;
+        ti1x = ti2x;
+        ti1y = ti2y;
+        ti1z = ti2z;
+        aT1 = aT2;
     }
 #undef AR
 #undef TI
 #undef TE
-    if (laststep) {
+    if (LAST) {
         const double sdiff = g.Kdhz[lev];
         const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
         const double lapr = fx * lapx + fy * lapy + fz * lapz;
@@ -181,42 +282,21 @@ __device__ __forceinline__ void divm_op(const Geo &g, int p, int lev, bool pole,
     }
 }
 
-// DivM pass 0 (divergence) on momentum M and vertical momentum Wh at one cell.
-__device__ __forceinline__ void divm_pass0(const Geo &g, int p, int lev, int nf, const int gi[7],
-                                           const double *M, const double *Wh, double out[3]) {
-    const size_t S = g.S;
-    const int nv = g.nv;
-    double a[7][3], wl[7], wt[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) {
-        const int q = gi[s];
-#pragma unroll
-        for (int k = 0; k < 3; k++) a[s][k] = VEC(M, k, lev, q);
-        wl[s] = LVP(Wh, lev, q);
-        wt[s] = LVP(Wh, lev + 1, q);
-    }
-    const bool pole = p >= g.P - 2;
-    if (nf == 6) divm_op<6>(g, p, lev, pole, false, a, wl, wt, out);
-    else divm_op<5>(g, p, lev, pole, false, a, wl, wt, out);
-}
-
-// DivM pass 1 (gradient of the divergence, tangential, times Kdhz) at one cell.
-__device__ __forceinline__ void divm_pass1(const Geo &g, int p, int lev, int nf, const int gi[7],
-                                           const double *divg, double out[3]) {
-    const size_t S = g.S;
-    const int nv = g.nv;
-    double a[7][3], wl[7], wt[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) {
-        const int q = gi[s];
-#pragma unroll
-        for (int k = 0; k < 3; k++) a[s][k] = VEC(divg, k, lev, q);
-        wl[s] = 0.0;
-        wt[s] = 0.0;
-    }
-    const bool pole = p >= g.P - 2;
-    if (nf == 6) divm_op<6>(g, p, lev, pole, true, a, wl, wt, out);
-    else divm_op<5>(g, p, lev, pole, true, a, wl, wt, out);
+// Pole point of a stencil kernel: slot s is read from global memory through
+// the ring (the pole's five neighbours; slot 6 repeats slot 5 with a zero
+// coefficient).
+struct PoleCtx {
+    int p;
+    const int *ring;
+    size_t S;
+    __device__ __forceinline__ int gi(int s) const { return s == 0 ? p : ring[(size_t)(s - 1) * S + p]; }
+};
+__device__ __forceinline__ bool pole_ctx(const Geo &g, PoleCtx &c) {
+    if (threadIdx.x >= 2) return false;
+    c.p = g.P - 2 + threadIdx.x;
+    c.ring = g.ring;
+    c.S = g.S;
+    return true;
 }
 
 // ===================================================================== K_A
@@ -417,114 +497,160 @@ __global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
 // added to the vertical part, and the first Laplacian pass of the 4th-order
 // hyperdiffusion (Diffusion_Op stepnum 0, thor_diff.h:54-522) of rho, u, v,
 // w = M/rho, W/rho and Rd*T.
-template <int LC>
-__global__ __launch_bounds__(256) void k_adv_diff1(Geo g, Fields f, int hydiff) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+template <class GetM, class GetV, class GetR, class GetW, class GetT>
+__device__ __forceinline__ void adv_diff1_point(const Geo &g, const Fields &f, int p, int lev, int nf, int hydiff,
+                                                GetM M, GetV v, GetR rho, GetW W, GetT T) {
     const size_t S = g.S;
     const int nv = g.nv;
-    const int l0 = blockIdx.y * LC, l1 = min(l0 + LC, nv);
-    const int nf = g.nfc[p];
-    int gi[7];
-    load_slots(g, p, gi);
-    for (int lev = l0; lev < l1; lev++) {
-        const double rs = rscale_at(g, lev);
-        double M[7][3], v[7][3], rho[7], W[7];
+    const double rs = rscale_at(g, lev);
+    double nf3[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s = 0; s < 7; s++) {
-            const int q = gi[s];
-            rho[s] = LVP(f.Rhok, lev, q);
-            W[s] = LVP(f.Wk, lev, q);
+    for (int kk = 0; kk < 3; kk++) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                M[s][k] = VEC(f.Mhk, k, lev, q);
-                v[s][k] = M[s][k] / rho[s] + (W[s] / rho[s]) * g.fr[(size_t)k * S + q];
-            }
-        }
-        double nf3[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < 3; kk++) {
-            double d[7];
-#pragma unroll
-            for (int s = 0; s < 7; s++) d[s] = g.div[(size_t)(s * 3 + kk) * S + p];
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                double sum = d[0] * v[0][c] * M[0][kk];
-#pragma unroll
-                for (int s = 1; s < 7; s++) sum = sum + d[s] * v[s][c] * M[s][kk];
-                nf3[c] += rs * sum;
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < 3; c++) VEC(f.Adv, c, lev, p) = nf3[c] + VEC(f.Adv, c, lev, p);
-        if (hydiff) {
-            double a[7];
-#pragma unroll
-            for (int var = 0; var < 6; var++) {
-#pragma unroll
-                for (int s = 0; s < 7; s++) {
-                    if (var == 0) a[s] = rho[s];
-                    else if (var <= 3) a[s] = M[s][var - 1] / rho[s];
-                    else if (var == 4) a[s] = W[s] / rho[s];
-                    else a[s] = g.Rd * LVP(f.T, lev, gi[s]);
-                }
-                VEC(f.diff, var, lev, p) = laplacian_nf(g, p, nf, rs, a);
-            }
+        for (int c = 0; c < 3; c++) {
+            double sum = g.div[(size_t)(0 * 3 + kk) * S + p] * v(0, c) * M(0, kk);
+SLOT_UNROLL(THOR_SLOT_UNROLL)
+            for (int s = 1; s < 7; s++) sum = sum + g.div[(size_t)(s * 3 + kk) * S + p] * v(s, c) * M(s, kk);
+            nf3[c] += rs * sum;
         }
     }
+#pragma unroll
+    for (int c = 0; c < 3; c++) VEC(f.Adv, c, lev, p) = nf3[c] + VEC(f.Adv, c, lev, p);
+    if (!hydiff) return;
+    const double Rd = g.Rd;
+    auto a = [&](int var, int s) -> double {
+        if (var == 0) return rho(s);
+        if (var <= 3) return M(s, var - 1) / rho(s);
+        if (var == 4) return W(s) / rho(s);
+        return Rd * T(s);
+    };
+    double lap[6];
+    laplacians<6>(g, p, nf, rs, a, lap);
+#pragma unroll
+    for (int var = 0; var < 6; var++) VEC(f.diff, var, lev, p) = lap[var];
+}
+
+__global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, int hydiff) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    auto vel = [&](int q, int c) {
+        const double rho = LVP(f.Rhok, lev, q), w = LVP(f.Wk, lev, q);
+        return VEC(f.Mhk, c, lev, q) / rho + (w / rho) * g.fr[(size_t)c * S + q];
+    };
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        adv_diff1_point(
+            g, f, c.p, lev, 5, hydiff, [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); },
+            [&](int s, int k) { return vel(c.gi(s), k); }, [&](int s) { return LVP(f.Rhok, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.Wk, lev, c.gi(s)); }, [&](int s) { return LVP(f.T, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ double sM[3][NHL2], sV[3][NHL2], sR[NHL2], sW[NHL2], sT[NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 3; k++) sM[k][pos] = sV[k][pos] = 0.0;
+            sR[pos] = sW[pos] = sT[pos] = 0.0;
+            return;
+        }
+        const double rho = LVP(f.Rhok, lev, q), w = LVP(f.Wk, lev, q);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double m = VEC(f.Mhk, k, lev, q);
+            sM[k][pos] = m;
+            sV[k][pos] = m / rho + (w / rho) * g.fr[(size_t)k * S + q];
+        }
+        sR[pos] = rho;
+        sW[pos] = w;
+        sT[pos] = LVP(f.T, lev, q);
+    });
+    __syncthreads();
+    adv_diff1_point(
+        g, f, T.p, lev, T.nf, hydiff, [&](int s, int k) { return sM[k][T.pos(s)]; },
+        [&](int s, int k) { return sV[k][T.pos(s)]; }, [&](int s) { return sR[T.pos(s)]; },
+        [&](int s) { return sW[T.pos(s)]; }, [&](int s) { return sT[T.pos(s)]; });
 }
 
 // ===================================================================== K_C
 // Last hyperdiffusion pass (Diffusion_Op laststep) + Correct_Horizontal
 // (thor_diff.h:882-915), and divergence-damping pass 0 on Mhk/Whk
 // (DivM_Op laststep = false).
-template <int LC>
-__global__ __launch_bounds__(256) void k_diff2_divm0(Geo g, Fields f, int hydiff, int divdamp, double sign) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+template <class GetD, class GetS, class GetM, class GetWl, class GetWt>
+__device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
+                                                  int hydiff, int divdamp, double sign, GetD dif, GetS rho, GetM M,
+                                                  GetWl wl, GetWt wt) {
     const size_t S = g.S;
     const int nv = g.nv;
-    const int l0 = blockIdx.y * LC, l1 = min(l0 + LC, nv);
-    const int nf = g.nfc[p];
-    const bool pole = p >= g.P - 2;
-    int gi[7];
-    load_slots(g, p, gi);
-    for (int lev = l0; lev < l1; lev++) {
+    if (hydiff) {
+        const double rs = rscale_at(g, lev);
+        const double sdiff = -g.Kdh4[lev] * sign;
+        double out[6];
+        auto a = [&](int var, int s) -> double { return var == 0 ? dif(s, 0) : dif(s, var) * rho(s); };
+        laplacians<6>(g, p, nf, rs, a, out);
+#pragma unroll
+        for (int var = 0; var < 6; var++) out[var] = pole ? sdiff * out[var] : out[var] * sdiff;
+        LV(f.diffrh, lev) = out[0];
+        LV(f.diffw, lev) = out[4];
+        LV(f.diffpr, lev) = out[5];
+        const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+        const double dmhr = fx * out[1] + fy * out[2] + fz * out[3];
+        VEC(f.diffmh, 0, lev, p) = out[1] + -fx * dmhr;
+        VEC(f.diffmh, 1, lev, p) = out[2] + -fy * dmhr;
+        VEC(f.diffmh, 2, lev, p) = out[3] + -fz * dmhr;
+    }
+    if (divdamp) {
+        double o3[3];
+        divm_nf<false>(g, p, lev, nf, pole, M, wl, wt, o3);
+#pragma unroll
+        for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
+    }
+}
+
+__global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int divdamp, double sign) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        diff2_divm0_point(
+            g, f, c.p, lev, 5, true, hydiff, divdamp, sign,
+            [&](int s, int var) { return VEC(f.diff, var, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.Rhok, lev, c.gi(s)); },
+            [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.Whk, lev, c.gi(s)); }, [&](int s) { return LVP(f.Whk, lev + 1, c.gi(s)); });
+        return;
+    }
+    __shared__ double sD[6][NHL2], sR[NHL2], sM[3][NHL2], sWl[NHL2], sWt[NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 6; k++) sD[k][pos] = 0.0;
+            for (int k = 0; k < 3; k++) sM[k][pos] = 0.0;
+            sR[pos] = sWl[pos] = sWt[pos] = 0.0;
+            return;
+        }
         if (hydiff) {
-            const double rs = rscale_at(g, lev);
-            const double sdiff = -g.Kdh4[lev] * sign;
-            double rho[7];
 #pragma unroll
-            for (int s = 0; s < 7; s++) rho[s] = LVP(f.Rhok, lev, gi[s]);
-            double out[6];
-#pragma unroll
-            for (int var = 0; var < 6; var++) {
-                double a[7];
-#pragma unroll
-                for (int s = 0; s < 7; s++) {
-                    const double dv = VEC(f.diff, var, lev, gi[s]);
-                    a[s] = var == 0 ? dv : dv * rho[s];
-                }
-                double lap = laplacian_nf(g, p, nf, rs, a);
-                lap = pole ? sdiff * lap : lap * sdiff;
-                out[var] = lap;
-            }
-            LV(f.diffrh, lev) = out[0];
-            LV(f.diffw, lev) = out[4];
-            LV(f.diffpr, lev) = out[5];
-            const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
-            const double dmhr = fx * out[1] + fy * out[2] + fz * out[3];
-            VEC(f.diffmh, 0, lev, p) = out[1] + -fx * dmhr;
-            VEC(f.diffmh, 1, lev, p) = out[2] + -fy * dmhr;
-            VEC(f.diffmh, 2, lev, p) = out[3] + -fz * dmhr;
+            for (int k = 0; k < 6; k++) sD[k][pos] = VEC(f.diff, k, lev, q);
+            sR[pos] = LVP(f.Rhok, lev, q);
         }
         if (divdamp) {
-            double o3[3];
-            divm_pass0(g, p, lev, nf, gi, f.Mhk, f.Whk, o3);
 #pragma unroll
-            for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
+            for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhk, k, lev, q);
+            sWl[pos] = LVP(f.Whk, lev, q);
+            sWt[pos] = LVP(f.Whk, lev + 1, q);
         }
-    }
+    });
+    __syncthreads();
+    diff2_divm0_point(
+        g, f, T.p, lev, T.nf, false, hydiff, divdamp, sign, [&](int s, int var) { return sD[var][T.pos(s)]; },
+        [&](int s) { return sR[T.pos(s)]; }, [&](int s, int k) { return sM[k][T.pos(s)]; },
+        [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; });
 }
 
 // ===================================================================== K_D
@@ -532,225 +658,294 @@ __global__ __launch_bounds__(256) void k_diff2_divm0(Geo g, Fields f, int hydiff
 // (thor_slowmodes.h:48-683, entropy form) + UpdateRK (thor_auxiliary.h:311-418)
 // for rk > 0, or the zero start of the deviations for rk = 0
 // (thor_driver.cu:104-108).
-template <int LC>
-__global__ __launch_bounds__(256) void k_slow(Geo g, Fields f, int divdamp, int rk) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+template <class GetG, class GetP, class GetH, class GetM>
+__device__ __forceinline__ void slow_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
+                                           int divdamp, int rk, GetG dg, GetP pv, GetH hv, GetM mv) {
     const size_t S = g.S;
     const int nv = g.nv;
-    const int l0 = blockIdx.y * LC, l1 = min(l0 + LC, nv);
-    const int nf = g.nfc[p];
-    const bool pole = p >= g.P - 2;
-    int gi[7];
-    load_slots(g, p, gi);
     const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cv = g.Cp - g.Rd;
     const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
     const double *Alt = g.Alt, *Alth = g.Alth;
-    for (int lev = l0; lev < l1; lev++) {
-        double DivM[3] = {0.0, 0.0, 0.0};
-        if (divdamp) divm_pass1(g, p, lev, nf, gi, f.divg, DivM);
-        double advx = VEC(f.Adv, 0, lev, p), advy = VEC(f.Adv, 1, lev, p), advz = VEC(f.Adv, 2, lev, p);
-        const double rho = LV(f.Rhok, lev);
-        double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
+    double DivM[3] = {0.0, 0.0, 0.0};
+    if (divdamp) {
+        auto none = [&](int) { return 0.0; };
+        divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
+    }
+    double advx = VEC(f.Adv, 0, lev, p), advy = VEC(f.Adv, 1, lev, p), advz = VEC(f.Adv, 2, lev, p);
+    const double rho = LV(f.Rhok, lev);
+    double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
+    if (lev != 0) {
+        const double ax = VEC(f.Adv, 0, lev - 1, p), ay = VEC(f.Adv, 1, lev - 1, p), az = VEC(f.Adv, 2, lev - 1, p);
+        advrl = ax * fx + ay * fy + az * fz;
+        rhol = LV(f.Rhok, lev - 1);
+        altl = Alt[lev - 1];
+        pressurel = LV(f.pk, lev - 1);
+    }
+    double advr = advx * fx + advy * fy + advz * fz;
+    advx += -advr * fx;
+    advy += -advr * fy;
+    advz += -advr * fz;
+    const double altht = Alth[lev + 1], althl = Alth[lev], alt = Alt[lev];
+    const double M0x = mv(0, 0), M0y = mv(0, 1), M0z = mv(0, 2);
+    if (!g.nonhydro) {
+        advr = (MPOW(M0x, 2) + MPOW(M0y, 2) + MPOW(M0z, 2)) / (rho * (alt + A));
         if (lev != 0) {
-            const double ax = VEC(f.Adv, 0, lev - 1, p), ay = VEC(f.Adv, 1, lev - 1, p),
-                         az = VEC(f.Adv, 2, lev - 1, p);
-            advrl = ax * fx + ay * fy + az * fz;
-            rhol = LV(f.Rhok, lev - 1);
-            altl = Alt[lev - 1];
-            pressurel = LV(f.pk, lev - 1);
-        }
-        double advr = advx * fx + advy * fy + advz * fz;
-        advx += -advr * fx;
-        advy += -advr * fy;
-        advz += -advr * fz;
-        const double altht = Alth[lev + 1], althl = Alth[lev], alt = Alt[lev];
-        const double M0x = VEC(f.Mhk, 0, lev, p), M0y = VEC(f.Mhk, 1, lev, p), M0z = VEC(f.Mhk, 2, lev, p);
-        if (!g.nonhydro) {
-            advr = (MPOW(M0x, 2) + MPOW(M0y, 2) + MPOW(M0z, 2)) / (rho * (alt + A));
-            if (lev != 0) {
-                const double m1x = VEC(f.Mhk, 0, lev - 1, p), m1y = VEC(f.Mhk, 1, lev - 1, p),
-                             m1z = VEC(f.Mhk, 2, lev - 1, p);
-                advrl = (MPOW(m1x, 2) + MPOW(m1y, 2) + MPOW(m1z, 2)) / (rhol * (altl + A));
-            }
-        }
-        const double dz = altht - althl;
-        double r2p, r2m, r2l, rscale, dr2dz;
-        if (g.deep) {
-            r2p = MPOW(altht + A, 2.0);
-            r2m = MPOW(alt + A, 2.0);
-            r2l = MPOW(althl + A, 2.0);
-            rscale = A / (alt + A);
-            dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3;
-        } else {
-            r2p = r2m = r2l = 1.0;
-            rscale = 1.0;
-            dr2dz = r2m * dz;
-        }
-        double nflxv[3], nflxr = 0.0, nflxp = 0.0;
-        double pv[7], hv[7];
-#pragma unroll
-        for (int s = 0; s < 7; s++) {
-            pv[s] = LVP(f.pk, lev, gi[s]);
-            hv[s] = LVP(f.h, lev, gi[s]);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double mv[7];
-#pragma unroll
-            for (int s = 0; s < 7; s++) mv[s] = VEC(f.Mhk, k, lev, gi[s]);
-            double sg = g.grad[(size_t)(0 * 3 + k) * S + p] * pv[0];
-            double sr = g.div[(size_t)(0 * 3 + k) * S + p] * mv[0];
-            double sp = g.div[(size_t)(0 * 3 + k) * S + p] * mv[0] * hv[0];
-#pragma unroll
-            for (int s = 1; s < 7; s++) {
-                const double gd = g.grad[(size_t)(s * 3 + k) * S + p], dd = g.div[(size_t)(s * 3 + k) * S + p];
-                sg = sg + gd * pv[s];
-                sr = sr + dd * mv[s];
-                sp = sp + dd * mv[s] * hv[s];
-            }
-            nflxv[k] = rscale * sg;
-            nflxr += rscale * sr;
-            nflxp += rscale * sp;
-        }
-        double whl, wht, hhl, hht;
-        if (lev == 0) {
-            whl = 0.0;
-            wht = LV(f.Whk, lev + 1);
-            hhl = 0.0;
-            hht = LV(f.hh, lev + 1);
-        } else {
-            whl = LV(f.Whk, lev);
-            wht = LV(f.Whk, lev + 1);
-            hhl = LV(f.hh, lev);
-            hht = LV(f.hh, lev + 1);
-        }
-        const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
-        const double dwhdz = (wht * r2p * hht - whl * r2l * hhl) / (dz * r2m);
-        const double dpr = nflxv[0] * fx + nflxv[1] * fy + nflxv[2] * fz;
-        const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
-#pragma unroll
-        for (int k = 0; k < 3; k++)
-            VEC(f.SlowMh, k, lev, p) = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[k] + VEC(f.diffmh, k, lev, p) + 0.0
-                                       + VEC(f.dMh, k, lev, p);
-        if (lev == 0) {
-            LV(f.SlowWh, 0) = 0.0;
-            LV(f.SlowWh, nv) = 0.0;
-        } else {
-            const double xi = althl, xim = altl, xip = alt;
-            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            double swr;
-            if (g.deep) {
-                if (g.ghv)
-                    swr = (advrl + rhol * Gravit * MPOW(A / (A + altl), 2)) * intt
-                          + (advr + rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
-                else
-                    swr = (advrl + rhol * Gravit) * intt + (advr + rho * Gravit) * intl;
-            } else {
-                if (g.ghv)
-                    swr = (rhol * Gravit * MPOW(A / (A + altl), 2)) * intt + (rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
-                else
-                    swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
-            }
-            if (g.nonhydro) swr += (-LV(f.diffw, lev - 1) - 0.0) * intt + (-LV(f.diffw, lev) - 0.0) * intl;
-            const double dzl = alt - altl;
-            const double dpdz = (pv[0] - pressurel) / dzl;
-            swr += dpdz;
-            swr -= LV(f.dWh, lev);
-            LV(f.SlowWh, lev) = -swr;
-        }
-        nflxr += dwdz;
-        LV(f.SlowRho, lev) = -nflxr + LV(f.diffrh, lev) + 0.0;
-        const double r = 1.0 / rho;
-        const double vgp = (nflxv[0] - dpr * fx) * M0x * r + (nflxv[1] - dpr * fy) * M0y * r
-                           + (nflxv[2] - dpr * fz) * M0z * r;
-        double sp = (Rd / Cv) * (-nflxp - LV(f.gtil, lev) - dwhdz + vgp + LV(f.Qheat, lev));
-        sp += LV(f.diffpr, lev) + 0.0;
-        LV(f.Slowp, lev) = sp;
-
-        // Deviations at the start of the stage.
-        if (rk == 0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) VEC(f.Mhs, k, lev, p) = 0.0;
-            LV(f.Rhos, lev) = 0.0;
-            LV(f.ps, lev) = 0.0;
-            LV(f.Whs, lev) = 0.0;
-            if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
-        } else {
-            const double Mx = VEC(f.Mh, 0, lev, p) - M0x, My = VEC(f.Mh, 1, lev, p) - M0y,
-                         Mz = VEC(f.Mh, 2, lev, p) - M0z;
-            const double aux = Mx * fx + My * fy + Mz * fz;
-            VEC(f.Mhs, 0, lev, p) = Mx - aux * fx;
-            VEC(f.Mhs, 1, lev, p) = My - aux * fy;
-            VEC(f.Mhs, 2, lev, p) = Mz - aux * fz;
-            LV(f.Rhos, lev) = LV(f.Rho, lev) - rho;
-            LV(f.ps, lev) = LV(f.p, lev) - pv[0];
-            LV(f.Whs, lev) = lev == 0 ? 0.0 : LV(f.Wh, lev) - LV(f.Whk, lev);
-            if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+            const double m1x = VEC(f.Mhk, 0, lev - 1, p), m1y = VEC(f.Mhk, 1, lev - 1, p),
+                         m1z = VEC(f.Mhk, 2, lev - 1, p);
+            advrl = (MPOW(m1x, 2) + MPOW(m1y, 2) + MPOW(m1z, 2)) / (rhol * (altl + A));
         }
     }
+    const double dz = altht - althl;
+    double r2p, r2m, r2l, rscale, dr2dz;
+    if (g.deep) {
+        r2p = MPOW(altht + A, 2.0);
+        r2m = MPOW(alt + A, 2.0);
+        r2l = MPOW(althl + A, 2.0);
+        rscale = A / (alt + A);
+        dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3;
+    } else {
+        r2p = r2m = r2l = 1.0;
+        rscale = 1.0;
+        dr2dz = r2m * dz;
+    }
+    double nflxv[3], nflxr = 0.0, nflxp = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double sg = g.grad[(size_t)(0 * 3 + k) * S + p] * pv(0);
+        double sr = g.div[(size_t)(0 * 3 + k) * S + p] * mv(0, k);
+        double sp = g.div[(size_t)(0 * 3 + k) * S + p] * mv(0, k) * hv(0);
+SLOT_UNROLL(THOR_SLOT_UNROLL)
+        for (int s = 1; s < 7; s++) {
+            const double gd = g.grad[(size_t)(s * 3 + k) * S + p], dd = g.div[(size_t)(s * 3 + k) * S + p];
+            sg = sg + gd * pv(s);
+            sr = sr + dd * mv(s, k);
+            sp = sp + dd * mv(s, k) * hv(s);
+        }
+        nflxv[k] = rscale * sg;
+        nflxr += rscale * sr;
+        nflxp += rscale * sp;
+    }
+    double whl, wht, hhl, hht;
+    if (lev == 0) {
+        whl = 0.0;
+        wht = LV(f.Whk, lev + 1);
+        hhl = 0.0;
+        hht = LV(f.hh, lev + 1);
+    } else {
+        whl = LV(f.Whk, lev);
+        wht = LV(f.Whk, lev + 1);
+        hhl = LV(f.hh, lev);
+        hht = LV(f.hh, lev + 1);
+    }
+    const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
+    const double dwhdz = (wht * r2p * hht - whl * r2l * hhl) / (dz * r2m);
+    const double dpr = nflxv[0] * fx + nflxv[1] * fy + nflxv[2] * fz;
+    const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        VEC(f.SlowMh, k, lev, p) = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[k] + VEC(f.diffmh, k, lev, p) + 0.0
+                                   + VEC(f.dMh, k, lev, p);
+    if (lev == 0) {
+        LV(f.SlowWh, 0) = 0.0;
+        LV(f.SlowWh, nv) = 0.0;
+    } else {
+        const double xi = althl, xim = altl, xip = alt;
+        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+        double swr;
+        if (g.deep) {
+            if (g.ghv)
+                swr = (advrl + rhol * Gravit * MPOW(A / (A + altl), 2)) * intt
+                      + (advr + rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
+            else
+                swr = (advrl + rhol * Gravit) * intt + (advr + rho * Gravit) * intl;
+        } else {
+            if (g.ghv)
+                swr = (rhol * Gravit * MPOW(A / (A + altl), 2)) * intt + (rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
+            else
+                swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
+        }
+        if (g.nonhydro) swr += (-LV(f.diffw, lev - 1) - 0.0) * intt + (-LV(f.diffw, lev) - 0.0) * intl;
+        const double dzl = alt - altl;
+        const double dpdz = (pv(0) - pressurel) / dzl;
+        swr += dpdz;
+        swr -= LV(f.dWh, lev);
+        LV(f.SlowWh, lev) = -swr;
+    }
+    nflxr += dwdz;
+    LV(f.SlowRho, lev) = -nflxr + LV(f.diffrh, lev) + 0.0;
+    const double r = 1.0 / rho;
+    const double vgp =
+        (nflxv[0] - dpr * fx) * M0x * r + (nflxv[1] - dpr * fy) * M0y * r + (nflxv[2] - dpr * fz) * M0z * r;
+    double sp = (Rd / Cv) * (-nflxp - LV(f.gtil, lev) - dwhdz + vgp + LV(f.Qheat, lev));
+    sp += LV(f.diffpr, lev) + 0.0;
+    LV(f.Slowp, lev) = sp;
+
+    // Deviations at the start of the stage.
+    if (rk == 0) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) VEC(f.Mhs, k, lev, p) = 0.0;
+        LV(f.Rhos, lev) = 0.0;
+        LV(f.ps, lev) = 0.0;
+        LV(f.Whs, lev) = 0.0;
+        if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+    } else {
+        const double Mx = VEC(f.Mh, 0, lev, p) - M0x, My = VEC(f.Mh, 1, lev, p) - M0y, Mz = VEC(f.Mh, 2, lev, p) - M0z;
+        const double aux = Mx * fx + My * fy + Mz * fz;
+        VEC(f.Mhs, 0, lev, p) = Mx - aux * fx;
+        VEC(f.Mhs, 1, lev, p) = My - aux * fy;
+        VEC(f.Mhs, 2, lev, p) = Mz - aux * fz;
+        LV(f.Rhos, lev) = LV(f.Rho, lev) - rho;
+        LV(f.ps, lev) = LV(f.p, lev) - pv(0);
+        LV(f.Whs, lev) = lev == 0 ? 0.0 : LV(f.Wh, lev) - LV(f.Whk, lev);
+        if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+    }
+}
+
+__global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        slow_point(
+            g, f, c.p, lev, 5, true, divdamp, rk, [&](int s, int k) { return VEC(f.divg, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.pk, lev, c.gi(s)); }, [&](int s) { return LVP(f.h, lev, c.gi(s)); },
+            [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ double sG[3][NHL2], sP[NHL2], sH[NHL2], sM[3][NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 3; k++) sG[k][pos] = sM[k][pos] = 0.0;
+            sP[pos] = sH[pos] = 0.0;
+            return;
+        }
+        if (divdamp) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sG[k][pos] = VEC(f.divg, k, lev, q);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhk, k, lev, q);
+        sP[pos] = LVP(f.pk, lev, q);
+        sH[pos] = LVP(f.h, lev, q);
+    });
+    __syncthreads();
+    slow_point(
+        g, f, T.p, lev, T.nf, false, divdamp, rk, [&](int s, int k) { return sG[k][T.pos(s)]; },
+        [&](int s) { return sP[T.pos(s)]; }, [&](int s) { return sH[T.pos(s)]; },
+        [&](int s, int k) { return sM[k][T.pos(s)]; });
 }
 
 // ===================================================================== K_E
 // Divergence-damping pass 0 on the deviations Mhs / Whs.
-template <int LC>
-__global__ __launch_bounds__(256) void k_divm0(Geo g, Fields f) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+__global__ STENCIL_BOUNDS void k_divm0(Geo g, Fields f) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
     const int nv = g.nv;
-    const int l0 = blockIdx.y * LC, l1 = min(l0 + LC, nv);
-    const int nf = g.nfc[p];
-    int gi[7];
-    load_slots(g, p, gi);
-    for (int lev = l0; lev < l1; lev++) {
-        double o3[3];
-        divm_pass0(g, p, lev, nf, gi, f.Mhs, f.Whs, o3);
+    double o3[3];
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        divm_nf<false>(
+            g, c.p, lev, 5, true, [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.Whs, lev, c.gi(s)); }, [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); }, o3);
+        const int p = c.p;
 #pragma unroll
         for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
+        return;
     }
+    __shared__ double sM[3][NHL2], sWl[NHL2], sWt[NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 3; k++) sM[k][pos] = 0.0;
+            sWl[pos] = sWt[pos] = 0.0;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhs, k, lev, q);
+        sWl[pos] = LVP(f.Whs, lev, q);
+        sWt[pos] = LVP(f.Whs, lev + 1, q);
+    });
+    __syncthreads();
+    divm_nf<false>(
+        g, T.p, lev, T.nf, false, [&](int s, int k) { return sM[k][T.pos(s)]; }, [&](int s) { return sWl[T.pos(s)]; },
+        [&](int s) { return sWt[T.pos(s)]; }, o3);
+    const int p = T.p;
+#pragma unroll
+    for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
 }
 
 // ===================================================================== K_F
 // Divergence-damping pass 1 on the deviations + Momentum_Eq / _Poles
 // (thor_fastmodes.h:54-253).
-template <int LC>
-__global__ __launch_bounds__(256) void k_momentum(Geo g, Fields f, int divdamp, double dt) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+template <class GetG, class GetS>
+__device__ __forceinline__ void momentum_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
+                                               int divdamp, double dt, GetG dg, GetS pv) {
     const size_t S = g.S;
     const int nv = g.nv;
-    const int l0 = blockIdx.y * LC, l1 = min(l0 + LC, nv);
-    const int nf = g.nfc[p];
-    int gi[7];
-    load_slots(g, p, gi);
     const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
-    for (int lev = l0; lev < l1; lev++) {
-        double DivM[3] = {0.0, 0.0, 0.0};
-        if (divdamp) divm_pass1(g, p, lev, nf, gi, f.divg, DivM);
-        const double rs = rscale_at(g, lev);
-        double pv[7];
-#pragma unroll
-        for (int s = 0; s < 7; s++) pv[s] = LVP(f.ps, lev, gi[s]);
-        double nf3[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double s0 = g.grad[(size_t)(0 * 3 + k) * S + p] * pv[0];
-#pragma unroll
-            for (int s = 1; s < 7; s++) s0 = s0 + g.grad[(size_t)(s * 3 + k) * S + p] * pv[s];
-            nf3[k] = rs * s0;
-        }
-        double Mx = (-nf3[0] + VEC(f.SlowMh, 0, lev, p) + DivM[0]) * dt;
-        double My = (-nf3[1] + VEC(f.SlowMh, 1, lev, p) + DivM[1]) * dt;
-        double Mz = (-nf3[2] + VEC(f.SlowMh, 2, lev, p) + DivM[2]) * dt;
-        const double vr = Mx * fx + My * fy + Mz * fz;
-        Mx += -vr * fx;
-        My += -vr * fy;
-        Mz += -vr * fz;
-        VEC(f.Mhs, 0, lev, p) += Mx;
-        VEC(f.Mhs, 1, lev, p) += My;
-        VEC(f.Mhs, 2, lev, p) += Mz;
+    double DivM[3] = {0.0, 0.0, 0.0};
+    if (divdamp) {
+        auto none = [&](int) { return 0.0; };
+        divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
     }
+    const double rs = rscale_at(g, lev);
+    double nf3[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double s0 = g.grad[(size_t)(0 * 3 + k) * S + p] * pv(0);
+SLOT_UNROLL(THOR_SLOT_UNROLL)
+        for (int s = 1; s < 7; s++) s0 = s0 + g.grad[(size_t)(s * 3 + k) * S + p] * pv(s);
+        nf3[k] = rs * s0;
+    }
+    double Mx = (-nf3[0] + VEC(f.SlowMh, 0, lev, p) + DivM[0]) * dt;
+    double My = (-nf3[1] + VEC(f.SlowMh, 1, lev, p) + DivM[1]) * dt;
+    double Mz = (-nf3[2] + VEC(f.SlowMh, 2, lev, p) + DivM[2]) * dt;
+    const double vr = Mx * fx + My * fy + Mz * fz;
+    Mx += -vr * fx;
+    My += -vr * fy;
+    Mz += -vr * fz;
+    VEC(f.Mhs, 0, lev, p) += Mx;
+    VEC(f.Mhs, 1, lev, p) += My;
+    VEC(f.Mhs, 2, lev, p) += Mz;
+}
+
+__global__ STENCIL_BOUNDS void k_momentum(Geo g, Fields f, int divdamp, double dt) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        momentum_point(
+            g, f, c.p, lev, 5, true, divdamp, dt, [&](int s, int k) { return VEC(f.divg, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.ps, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ double sG[3][NHL2], sP[NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 3; k++) sG[k][pos] = 0.0;
+            sP[pos] = 0.0;
+            return;
+        }
+        if (divdamp) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sG[k][pos] = VEC(f.divg, k, lev, q);
+        }
+        sP[pos] = LVP(f.ps, lev, q);
+    });
+    __syncthreads();
+    momentum_point(
+        g, f, T.p, lev, T.nf, false, divdamp, dt, [&](int s, int k) { return sG[k][T.pos(s)]; },
+        [&](int s) { return sP[T.pos(s)]; });
 }
 
 // ===================================================================== K_G1
@@ -760,188 +955,195 @@ __global__ __launch_bounds__(256) void k_momentum(Geo g, Fields f, int divdamp, 
 // Thomas algorithm; cc/dd live in LDS.
 constexpr int kThomasBlock = 64;
 
-__device__ __forceinline__ void sources_at(const Geo &g, const Fields &f, int p, int lev, const int gi[7],
-                                           double &Sp, double &Sd) {
+// Prepare_Implicit_Vertical + _Poles (thor_vertical_int.h:566-771): source
+// terms Sp = -(Rd/Cv) div(Mhs h) + Slowpressure, Sd = -div(Mhs) + SlowRho.
+template <class GetM, class GetH>
+__device__ __forceinline__ void prepare_point(const Geo &g, const Fields &f, int p, int lev, GetM mv, GetH hv) {
     const size_t S = g.S;
-    const int nv = g.nv;
     const double RoC = g.Rd / (g.Cp - g.Rd);
     const double rs = rscale_at(g, lev);
-    double nflxr = 0.0, nflxp = 0.0, hv[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) hv[s] = LVP(f.h, lev, gi[s]);
+    double nflxr = 0.0, nflxp = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        double mv[7];
-#pragma unroll
-        for (int s = 0; s < 7; s++) mv[s] = VEC(f.Mhs, k, lev, gi[s]);
-        double sr = g.div[(size_t)(0 * 3 + k) * S + p] * mv[0];
-        double sp = g.div[(size_t)(0 * 3 + k) * S + p] * mv[0] * hv[0];
-#pragma unroll
+        double sr = g.div[(size_t)(0 * 3 + k) * S + p] * mv(0, k);
+        double sp = g.div[(size_t)(0 * 3 + k) * S + p] * mv(0, k) * hv(0);
+SLOT_UNROLL(THOR_SLOT_UNROLL)
         for (int s = 1; s < 7; s++) {
             const double dd = g.div[(size_t)(s * 3 + k) * S + p];
-            sr = sr + dd * mv[s];
-            sp = sp + dd * mv[s] * hv[s];
+            sr = sr + dd * mv(s, k);
+            sp = sp + dd * mv(s, k) * hv(s);
         }
         nflxr -= rs * sr;
         nflxp -= (RoC)*rs * sp;
     }
-    Sp = nflxp + LV(f.Slowp, lev);
-    Sd = nflxr + LV(f.SlowRho, lev);
+    LV(f.Sp, lev) = nflxp + LV(f.Slowp, lev);
+    LV(f.Sd, lev) = nflxr + LV(f.SlowRho, lev);
 }
 
-__global__ __launch_bounds__(kThomasBlock) void k_thomas(Geo g, Fields f, double deltat) {
-    extern __shared__ double lds[];
+__global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        prepare_point(g, f, c.p, lev, [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)); },
+                      [&](int s) { return LVP(f.h, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ double sM[3][NHL2], sH[NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 3; k++) sM[k][pos] = 0.0;
+            sH[pos] = 0.0;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhs, k, lev, q);
+        sH[pos] = LVP(f.h, lev, q);
+    });
+    __syncthreads();
+    prepare_point(g, f, T.p, lev, [&](int s, int k) { return sM[k][T.pos(s)]; },
+                  [&](int s) { return sH[T.pos(s)]; });
+}
+
+// Vertical_Eq (thor_vertical_int.h:52-563): per column, the tridiagonal solve
+// for Whs at interfaces 1..nv-1 with the modify-in-place Thomas algorithm;
+// cc/dd live in LDS.
+// The tridiagonal matrix depends only on stage-start quantities (hh, gtilh)
+// and the substep length, so its Thomas factorisation - the modified upper
+// diagonal cc' and the pivots t = 1 / (bb - cc'_{k-1} aa) of the reference's
+// modify-in-place algorithm - is computed once per RK stage and reused by every
+// acoustic substep; the values are those the reference recomputes each substep.
+struct Plane {
+    double *b;
+    size_t S;
+    int p;
+    __device__ __forceinline__ double &operator[](int lev) const { return b[(size_t)lev * S + p]; }
+};
+
+__global__ __launch_bounds__(kThomasBlock) void k_thomas_factor(Geo g, Fields f, double deltat) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.P) return;
     const size_t S = g.S;
-    const int nv = g.nv, nvi = nv + 1;
-    double *cc = lds + (size_t)threadIdx.x * nvi;
-    double *dd = lds + (size_t)kThomasBlock * nvi + (size_t)threadIdx.x * nvi;
-    int gi[7];
-    load_slots(g, p, gi);
+    const int nv = g.nv;
+    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
     const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
     const double *Alt = g.Alt, *Alth = g.Alth;
     const bool ghv = g.ghv, nonh = g.nonhydro;
-    double altht = 0, althl = 0, alth = 0, alt = 0, altl = 0, hp = 0, h = 0, hm = 0, gp = 0, gg = 0, gm = 0;
-    double Sp = 0, Spl = 0, pr = 0, pl = 0, r = 0, rl = 0, Sdl = 0, Sd = 0;
-    double CRddl = 0, CRddu = 0, GCoRl = 0, GCoRu = 0;
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
     auto gcor = [&](double altx) {
         return ghv ? Gravit * MPOW(A / (A + altx), 2) * (Cp - Rd) / Rd : Gravit * (Cp - Rd) / Rd;
     };
+    double ccp = 0.0;
+#pragma unroll 2
     for (int lev = 1; lev < nv; lev++) {
-        if (lev == 1) {
-            altht = Alth[lev + 1];
-            althl = Alth[lev - 1];
-            alth = Alth[lev];
-            alt = Alt[lev];
-            altl = Alt[lev - 1];
-            hp = LV(f.hh, lev + 1);
-            h = LV(f.hh, lev);
-            hm = LV(f.hh, lev - 1);
-            gp = LV(f.gtilh, lev + 1);
-            gg = LV(f.gtilh, lev);
-            gm = LV(f.gtilh, lev - 1);
-            sources_at(g, f, p, lev - 1, gi, Spl, Sdl);
-            sources_at(g, f, p, lev, gi, Sp, Sd);
-            pr = LV(f.ps, lev);
-            pl = LV(f.ps, lev - 1);
-            rl = LV(f.Rhos, lev - 1);
-            r = LV(f.Rhos, lev);
-            CRddl = CRdd_c;
-            CRddu = CRdd_c;
-            GCoRl = gcor(altl);
-            GCoRu = gcor(alt);
-        }
-        double aa, bb, C0, intt, intl, inttm, xi, xim, xip;
+        const double altht = Alth[lev + 1], alth = Alth[lev], althl = Alth[lev - 1];
+        const double alt = Alt[lev], altl = Alt[lev - 1];
+        double hp = LV(f.hh, lev + 1), h = LV(f.hh, lev), hm = LV(f.hh, lev - 1);
+        const double gp = LV(f.gtilh, lev + 1), gg = LV(f.gtilh, lev), gm = LV(f.gtilh, lev - 1);
+        const double GCoRl = gcor(altl), GCoRu = gcor(alt);
         const double dzp = 1.0 / (altht - alth), dzh = 1.0 / (alt - altl), dzm = 1.0 / (alth - althl);
         const double dzph = dzp * dzh, dzmh = dzm * dzh;
-        const double CRdd = (CRddl * (alt - alth) + CRddu * (alth - altl)) / (alt - altl);
+        const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
         const double GCoR = (GCoRl * (alt - alth) + GCoRu * (alth - altl)) / (alt - altl);
+        const double intl = (alt - alth) * dzp * dzh;
+        const double inttm = -(altl - alth) * dzm * dzh;
+        double aa, bb, cc;
         if (g.deep) {
             hp = hp * MPOW(altht + A, 2);
             h = h * MPOW(alth + A, 2);
             hm = hm * MPOW(althl + A, 2);
             const double or2 = 1 / MPOW(alth + A, 2);
             const double tor3 = 2 / MPOW(alth + A, 3);
-            xi = alt;
-            xim = alth;
-            intl = (xi - xim) * dzp * dzh;
-            xi = altl;
-            xip = alth;
-            inttm = -(xi - xip) * dzm * dzh;
-            cc[lev] = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
+            cc = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
             if (nonh) bb = CRdd + (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
             else bb = (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
             aa = -dzmh * or2 * hm + intl * (gm + GCoR - tor3 * hm);
         } else {
-            xi = alt;
-            xim = alth;
-            intl = (xi - xim) * dzp * dzh;
-            xi = altl;
-            xip = alth;
-            inttm = -(xi - xip) * dzm * dzh;
-            cc[lev] = -dzph * hp - inttm * (gp + GCoR);
+            cc = -dzph * hp - inttm * (gp + GCoR);
             if (nonh) bb = CRdd + (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
             else bb = (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
             aa = -dzmh * hm + intl * (gm + GCoR);
         }
+        if (lev == 1) {
+            ccp = cc / bb;
+            TT[1] = bb;
+        } else {
+            const double t = 1.0 / (bb - ccp * aa);
+            ccp = cc * t;
+            TT[lev] = t;
+            AA[lev] = aa;
+        }
+        CC[lev] = ccp;
+    }
+}
+
+// Vertical_Eq (thor_vertical_int.h:52-563) with the stage's factorisation:
+// right-hand side C0 from the substep's sources, forward elimination into Whs
+// (as the reference's dd), back substitution Whs_k = -cc'_k Whs_{k+1} + dd_k,
+// and Whs = 0 at the bottom and top interfaces.
+__global__ __launch_bounds__(kThomasBlock) void k_thomas_solve(Geo g, Fields f, double deltat) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p}, W{f.Whs, S, p};
+    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const bool ghv = g.ghv, nonh = g.nonhydro;
+    const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
+    const double dt2 = MPOW(deltat, 2.0);
+    double Spl = LV(f.Sp, 0), Sdl = LV(f.Sd, 0), pl = LV(f.ps, 0), rl = LV(f.Rhos, 0);
+    double ddp = 0.0;
+#pragma unroll 2
+    for (int lev = 1; lev < nv; lev++) {
+        const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
+        const double Sp = LV(f.Sp, lev), Sd = LV(f.Sd, lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
+        const double dzh = 1.0 / (alt - altl);
+        const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
         const double dSpdz = (Sp - Spl) * dzh, dPdz = (pr - pl) * dzh;
-        xi = alth;
-        xim = altl;
-        xip = alt;
-        intt = (xi - xip) / (xim - xip);
-        intl = (xi - xim) / (xip - xim);
+        const double intt = (alth - alt) / (altl - alt), intl = (alth - altl) / (alt - altl);
         const double rhohs = rl * intt + r * intl;
         const double Sdh = Sdl * intt + Sd * intl;
         const double srh = LV(f.SlowWh, lev);
+        double C0;
         if (!nonh) {
             if (ghv)
-                C0 = (MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                C0 = (dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
                       + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
                      * (CRdd);
             else
-                C0 = (MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * Sdh + Gravit * deltat * rhohs
-                      + deltat * dPdz - deltat * srh)
+                C0 = (dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz - deltat * srh)
                      * (CRdd);
         } else {
-            const double whs = LV(f.Whs, lev);
+            const double whs = W[lev];
             if (ghv)
-                C0 = (-whs + MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
                       + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
                      * (CRdd);
             else
-                C0 = (-whs + MPOW(deltat, 2.0) * dSpdz + MPOW(deltat, 2.0) * Gravit * Sdh
-                      + Gravit * deltat * rhohs + deltat * dPdz - deltat * srh)
+                C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                      - deltat * srh)
                      * (CRdd);
         }
-        if (lev < nv - 1) {
-            althl = alth;
-            alth = altht;
-            altht = Alth[lev + 2];
-            altl = alt;
-            alt = Alt[lev + 1];
-            if (g.deep) {
-                hm = LV(f.hh, lev);
-                h = LV(f.hh, lev + 1);
-            } else {
-                hm = h;
-                h = hp;
-            }
-            hp = LV(f.hh, lev + 2);
-            gm = gg;
-            gg = gp;
-            gp = LV(f.gtilh, lev + 2);
-            Spl = Sp;
-            Sdl = Sd;
-            sources_at(g, f, p, lev + 1, gi, Sp, Sd);
-            pl = pr;
-            pr = LV(f.ps, lev + 1);
-            rl = r;
-            r = LV(f.Rhos, lev + 1);
-            CRddl = CRddu;
-            CRddu = CRdd_c;
-            GCoRl = GCoRu;
-            GCoRu = gcor(alt);
-        }
-        dd[lev] = -C0;
-        if (lev == 1) {
-            cc[1] = cc[1] / bb;
-            dd[1] = dd[1] / bb;
-        } else {
-            const double t = 1.0 / (bb - cc[lev - 1] * aa);
-            cc[lev] *= t;
-            dd[lev] = (dd[lev] - dd[lev - 1] * aa) * t;
-        }
+        if (lev == 1) ddp = (-C0) / TT[1];
+        else ddp = (-C0 - ddp * AA[lev]) * TT[lev];
+        W[lev] = ddp;
+        Spl = Sp;
+        Sdl = Sd;
+        pl = pr;
+        rl = r;
     }
-    LV(f.Whs, nv) = 0.0;
-    LV(f.Whs, 0) = 0.0;
-    double wn = dd[nv - 1];
-    LV(f.Whs, nv - 1) = wn;
+    W[nv] = 0.0;
+    W[0] = 0.0;
     double wup = 0.0;
-    for (int lev = nvi - 2; lev > 0; lev--) {
-        const double w = (-cc[lev] * wup + dd[lev]);
-        LV(f.Whs, lev) = w;
+#pragma unroll 2
+    for (int lev = nv - 1; lev > 0; lev--) {
+        const double w = (-CC[lev] * wup + W[lev]);
+        W[lev] = w;
         wup = w;
     }
 }
@@ -950,90 +1152,127 @@ __global__ __launch_bounds__(kThomasBlock) void k_thomas(Geo g, Fields f, double
 // Density_Pressure_Eqs / _Poles (thor_fastmodes.h:255-847, entropy form) and,
 // unless this is the stage's last substep, divergence-damping pass 0 on the
 // updated deviations for the next substep.
-template <int LC>
-__global__ __launch_bounds__(256) void k_density_pressure(Geo g, Fields f, double dt, int next_divm) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+template <class GetM, class GetV1, class GetS>
+__device__ __forceinline__ void dp_point(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
+                                         GetM vv, GetV1 v1, GetS ptv) {
     const size_t S = g.S;
     const int nv = g.nv;
-    const int l0 = blockIdx.y * LC, l1 = min(l0 + LC, nv);
-    const int nf = g.nfc[p];
-    const bool pole = p >= g.P - 2;
-    int gi[7];
-    load_slots(g, p, gi);
     const double A = g.A, P_Ref = g.P_Ref, Rd = g.Rd, Cp = g.Cp, Cv = Cp - Rd;
-    for (int lev = l0; lev < l1; lev++) {
-        const double altht = g.Alth[lev + 1], althl = g.Alth[lev], dz = altht - althl;
-        double r2p, r2m, r2l, rscale, dr2dz;
-        if (g.deep) {
-            const double alt = g.Alt[lev];
-            r2p = MPOW(altht + A, 2.0);
-            r2m = MPOW(alt + A, 2.0);
-            r2l = MPOW(althl + A, 2.0);
-            rscale = A / (alt + A);
-            dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3.0;
-        } else {
-            r2p = r2m = r2l = 1.0;
-            rscale = 1.0;
-            dr2dz = r2m * dz;
+    const double altht = g.Alth[lev + 1], althl = g.Alth[lev], dz = altht - althl;
+    double r2p, r2m, r2l, rscale, dr2dz;
+    if (g.deep) {
+        const double alt = g.Alt[lev];
+        r2p = MPOW(altht + A, 2.0);
+        r2m = MPOW(alt + A, 2.0);
+        r2l = MPOW(althl + A, 2.0);
+        rscale = A / (alt + A);
+        dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3.0;
+    } else {
+        r2p = r2m = r2l = 1.0;
+        rscale = 1.0;
+        dr2dz = r2m * dz;
+    }
+    double nflxr = 0.0, nflxpt = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        double sr = g.div[(size_t)(0 * 3 + k) * S + p] * vv(0, k);
+        double sp = g.div[(size_t)(0 * 3 + k) * S + p] * v1(0, k) * ptv(0);
+SLOT_UNROLL(THOR_SLOT_UNROLL)
+        for (int s = 1; s < 7; s++) {
+            const double dd = g.div[(size_t)(s * 3 + k) * S + p];
+            sr = sr + dd * vv(s, k);
+            sp = sp + dd * v1(s, k) * ptv(s);
         }
-        double nflxr = 0.0, nflxpt = 0.0, ptv[7];
-#pragma unroll
-        for (int s = 0; s < 7; s++) ptv[s] = LVP(f.pt, lev, gi[s]);
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            double vv[7], v1[7];
-#pragma unroll
-            for (int s = 0; s < 7; s++) {
-                vv[s] = VEC(f.Mhs, k, lev, gi[s]);
-                v1[s] = vv[s] + VEC(f.Mhk, k, lev, gi[s]);
-            }
-            double sr = g.div[(size_t)(0 * 3 + k) * S + p] * vv[0];
-            double sp = g.div[(size_t)(0 * 3 + k) * S + p] * v1[0] * ptv[0];
-#pragma unroll
-            for (int s = 1; s < 7; s++) {
-                const double dd = g.div[(size_t)(s * 3 + k) * S + p];
-                sr = sr + dd * vv[s];
-                sp = sp + dd * v1[s] * ptv[s];
-            }
-            nflxr += rscale * sr;
-            nflxpt += rscale * sp;
-        }
-        double whl, wht, whl2, wht2, phl, pht;
-        if (lev == 0) {
-            whl = 0.0;
-            wht = LV(f.Whs, lev + 1);
-            whl2 = 0.0;
-            wht2 = LV(f.Whs, lev + 1) + LV(f.Whk, lev + 1);
-            phl = 0.0;
-            pht = LV(f.pth, lev + 1);
-        } else {
-            whl = LV(f.Whs, lev);
-            wht = LV(f.Whs, lev + 1);
-            whl2 = LV(f.Whs, lev) + LV(f.Whk, lev);
-            wht2 = LV(f.Whs, lev + 1) + LV(f.Whk, lev + 1);
-            phl = LV(f.pth, lev);
-            pht = LV(f.pth, lev + 1);
-        }
-        const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
-        const double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
-        double aux = -(nflxpt + dwptdz) * dt;
-        const double rhos = LV(f.Rhos, lev), pss = LV(f.ps, lev), pkv = LV(f.pk, lev);
-        const double r = LV(f.Rhok, lev) + rhos;
-        nflxr += dwdz;
-        LV(f.Rhos, lev) = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
-        const double ptc = (P_Ref / (Rd * r)) * MPOW((pss + pkv) / P_Ref, Cv / Cp);
-        aux += ptc * r;
-        const double pnew = P_Ref * MPOW(Rd * aux / P_Ref, Cp / Cv);
-        const double psn = pnew - pkv + (LV(f.diffpr, lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
-        LV(f.ps, lev) = psn;
-        if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
+        nflxr += rscale * sr;
+        nflxpt += rscale * sp;
+    }
+    double whl, wht, whl2, wht2, phl, pht;
+    if (lev == 0) {
+        whl = 0.0;
+        wht = LV(f.Whs, lev + 1);
+        whl2 = 0.0;
+        wht2 = LV(f.Whs, lev + 1) + LV(f.Whk, lev + 1);
+        phl = 0.0;
+        pht = LV(f.pth, lev + 1);
+    } else {
+        whl = LV(f.Whs, lev);
+        wht = LV(f.Whs, lev + 1);
+        whl2 = LV(f.Whs, lev) + LV(f.Whk, lev);
+        wht2 = LV(f.Whs, lev + 1) + LV(f.Whk, lev + 1);
+        phl = LV(f.pth, lev);
+        pht = LV(f.pth, lev + 1);
+    }
+    const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
+    const double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
+    double aux = -(nflxpt + dwptdz) * dt;
+    const double rhos = LV(f.Rhos, lev), pss = LV(f.ps, lev), pkv = LV(f.pk, lev);
+    const double r = LV(f.Rhok, lev) + rhos;
+    nflxr += dwdz;
+    LV(f.Rhos, lev) = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
+    const double ptc = (P_Ref / (Rd * r)) * MPOW((pss + pkv) / P_Ref, Cv / Cp);
+    aux += ptc * r;
+    const double pnew = P_Ref * MPOW(Rd * aux / P_Ref, Cp / Cv);
+    const double psn = pnew - pkv + (LV(f.diffpr, lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
+    LV(f.ps, lev) = psn;
+    if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
+}
+
+__global__ STENCIL_BOUNDS void k_density_pressure(Geo g, Fields f, double dt, int next_divm) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        auto Ms = [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)); };
+        dp_point(
+            g, f, c.p, lev, true, dt, Ms,
+            [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
         if (next_divm) {
             double o3[3];
-            divm_pass0(g, p, lev, nf, gi, f.Mhs, f.Whs, o3);
+            divm_nf<false>(
+                g, c.p, lev, 5, true, Ms, [&](int s) { return LVP(f.Whs, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); }, o3);
+            const int p = c.p;
 #pragma unroll
             for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
         }
+        return;
+    }
+    __shared__ double sM[3][NHL2], sV1[3][NHL2], sPt[NHL2], sWl[NHL2], sWt[NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+        if (q < 0) {
+            for (int k = 0; k < 3; k++) sM[k][pos] = sV1[k][pos] = 0.0;
+            sPt[pos] = sWl[pos] = sWt[pos] = 0.0;
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double m = VEC(f.Mhs, k, lev, q);
+            sM[k][pos] = m;
+            sV1[k][pos] = m + VEC(f.Mhk, k, lev, q);
+        }
+        sPt[pos] = LVP(f.pt, lev, q);
+        if (next_divm) {
+            sWl[pos] = LVP(f.Whs, lev, q);
+            sWt[pos] = LVP(f.Whs, lev + 1, q);
+        }
+    });
+    __syncthreads();
+    auto Ms = [&](int s, int k) { return sM[k][T.pos(s)]; };
+    dp_point(g, f, T.p, lev, false, dt, Ms, [&](int s, int k) { return sV1[k][T.pos(s)]; },
+             [&](int s) { return sPt[T.pos(s)]; });
+    if (next_divm) {
+        double o3[3];
+        divm_nf<false>(
+            g, T.p, lev, T.nf, false, Ms, [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; },
+            o3);
+        const int p = T.p;
+#pragma unroll
+        for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
     }
 }
 
