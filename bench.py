@@ -8,8 +8,12 @@ sequence as THOR's main loop (esp.cu:1393-1398). Synthetic input: the rest
 state plus seeded +-1e-3 K temperature noise; all state resident in HBM.
 
 Prints ONE JSON line. Extra objects:
-  roofline     - dominant kernel: algorithmic bytes per launch / average launch
-                 time from HIP events on the library's stream, against 8 TB/s;
+  roofline     - dominant kernel: algorithmic bytes per launch (per-cell doubles
+                 of the reference kernels it fuses x P x nv x 8 B) / average
+                 launch time from HIP events on the library's stream, against
+                 8 TB/s; "traffic" = HBM bytes per launch from the committed
+                 rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_*.json);
+                 "step" = the whole large step's algorithmic bytes / wall time;
   cpu_baseline - the serial FP64 oracle (THOR's kernels as a host loop) timed
                  on one host core for one step of the same workload.
 """
@@ -35,6 +39,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_probe(mib: int) -> None:
+    """Plain copy kernel of known size: reads and writes mib MiB per launch."""
+    import torch
+    a = torch.ones((mib << 20) // 8, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {mib} MiB device copy")
+    del a, b
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter runs
+    (profiles/hbm_traffic_<workload>.json: FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the same counter reads for a copy of known size), or None when the file
+    does not cover this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    if t.get("workload") != workload or k is None:
+        return None
+    return k["hbm_bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -46,6 +79,9 @@ def main():
     ap.add_argument("--substeps", type=int, default=8)
     ap.add_argument("--variant", default="", help='"" production build, "pm" parity build')
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run a plain device-to-device copy of this many MiB (three times); "
+                         "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (rhombus-sharded) "
@@ -54,6 +90,8 @@ def main():
 
     import thor_amd.capi as capi
 
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     os.makedirs(CACHE, exist_ok=True)
     t0 = time.perf_counter()
     g = capi.Grid(args.glevel, args.nv, 6.371e6, 36000.0, spring=True, beta=1.15, cache_dir=CACHE)
@@ -91,14 +129,20 @@ def main():
     dom = max(stats, key=lambda s: s["total_ms"])
     dom_avg_ms = dom["total_ms"] / dom["launches"]
     achieved = dom["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
-    step_bytes = sum(s["bytes_per_launch"] * s["launches"] for s in stats) / nprof
-    step_dev_ms = tot / nprof
+    # whole step: the reference decomposition's algorithmic bytes per large step,
+    # N * 8 B * (470 + 70 * n_sub + 24) with n_sub = 1 + ns/2 + ns acoustic
+    # substeps (ProfX included), over the timed wall time per step
+    n_sub = 1 + args.substeps // 2 + args.substeps
+    step_bytes = g.P * args.nv * 8.0 * (470 + 70 * n_sub + 24)
+    step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
     roofline = {
         "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(dom["name"], f"g{args.glevel}L{args.nv}"),
+        "alg_bytes_per_launch": round(dom["bytes_per_launch"]),
         "avg_launch_us": round(dom_avg_ms * 1e3, 2),
-        "step_alg_GBs": round(step_bytes / (step_dev_ms * 1e-3) / 1e9, 1),
-        "step_alg_frac": round(step_bytes / (step_dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "step": {"alg_bytes": round(step_bytes), "achieved": round(step_gbs, 1),
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }
 
     cpu = None

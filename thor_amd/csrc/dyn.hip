@@ -29,7 +29,7 @@ constexpr int kLC = 8;  // levels per thread in the stencil kernels
 struct KernelStat {
     long launches = 0;
     double total_ms = 0.0;
-    double bytes = 0.0;
+    double bytes = 0.0;  // sum over launches of algorithmic bytes
 };
 
 struct Pending {
@@ -98,7 +98,7 @@ template <class Fn> static void launch(Dyn &d, const char *name, double bytes, F
         if (st.launches == 0 && std::find(d.stat_order.begin(), d.stat_order.end(), name) == d.stat_order.end())
             d.stat_order.push_back(name);
         st.launches++;
-        st.bytes = bytes;
+        st.bytes += bytes;
         Pending pd{name, get_event(d), get_event(d)};
         hipEventRecord(pd.a, d.stream);
         fn();
@@ -139,6 +139,16 @@ static Fields stage_fields(Dyn &d, int rk) {
     return f;
 }
 
+// Algorithmic bytes booked per launch: the per-cell doubles of the reference
+// kernels that each fused kernel replaces (each field array read once and
+// written once per reference kernel; geometry excluded), times P * nv cells
+// * 8 B. Reference kernels (doubles per cell): Compute_Advec_Cori1 11,
+// Compute_Advec_Cori2 11, Compute_Temperature_H_Pt_Geff 12, Diffusion_Op 13
+// per pass, Correct_Horizontal 12, DivM_Op 7 (pass 0) + 6 (pass 1),
+// Compute_Slow_Modes 40, UpdateRK 19, UpdateRK2 19, Momentum_Eq 13,
+// Prepare_Implicit_Vertical 10, Vertical_Eq 12, Density_Pressure_Eqs 22,
+// ProfX (Held-Suarez) 24. The per-stage Thomas factorisation has no reference
+// counterpart and books its own traffic (hh, gtilh in; three planes out).
 static int thor_step(Dyn &d) {
     const double N = (double)d.P * d.nv, B = 8.0 * N;
     const int ns_total = d.prm.n_substeps;
@@ -153,23 +163,23 @@ static int thor_step(Dyn &d) {
         const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
-        launch(d, "column_prep", 17 * B, [&] { k_column_prep<<<grid_c, blk, 0, st>>>(g, f); });
-        launch(d, "adv_diff1", (hydiff ? 18 : 9) * B, [&] { k_adv_diff1<<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
+        launch(d, "column_prep", (12 + 11) * B, [&] { k_column_prep<<<grid_c, blk, 0, st>>>(g, f); });
+        launch(d, "adv_diff1", (11 + (hydiff ? 13 : 0)) * B, [&] { k_adv_diff1<<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
         if (hydiff || divdamp)
-            launch(d, "diff2_divm0", ((hydiff ? 16 : 0) + (divdamp ? 7 : 0)) * B, [&] {
+            launch(d, "diff2_divm0", ((hydiff ? 13 + 12 : 0) + (divdamp ? 7 : 0)) * B, [&] {
                 k_diff2_divm0<<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
             });
-        launch(d, "slow_modes", ((divdamp ? 3 : 0) + 29 + (rk ? 12 : 6)) * B,
+        launch(d, "slow_modes", ((divdamp ? 6 : 0) + 40 + (rk ? 19 : 0)) * B,
                [&] { k_slow<<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
         if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<<<grid_t2, blk, 0, st>>>(g, f); });
         launch(d, "thomas_factor", 5 * B, [&] { k_thomas_factor<<<grid_t, kThomasBlock, 0, st>>>(g, f, times); });
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
-            launch(d, "momentum", ((divdamp ? 3 : 0) + 10) * B,
+            launch(d, "momentum", ((divdamp ? 6 : 0) + 13) * B,
                    [&] { k_momentum<<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
-            launch(d, "prepare", 8 * B, [&] { k_prepare<<<grid_t2, blk, 0, st>>>(g, f); });
+            launch(d, "prepare", 10 * B, [&] { k_prepare<<<grid_t2, blk, 0, st>>>(g, f); });
             launch(d, "thomas_solve", 12 * B, [&] { k_thomas_solve<<<grid_t, kThomasBlock, 0, st>>>(g, f, times); });
-            launch(d, "density_pressure", (19 + (next ? 3 : 0)) * B,
+            launch(d, "density_pressure", (22 + (next ? 7 : 0)) * B,
                    [&] { k_density_pressure<<<grid_t2, blk, 0, st>>>(g, f, times, next); });
         }
         launch(d, "update_rk2", 19 * B, [&] { k_update_rk2<kLC><<<grid_s, blk, 0, st>>>(g, f); });
@@ -191,7 +201,7 @@ static int profx(Dyn &d, int forcing) {
     const Geo g = d.g;
     const dim3 blk(kTile), grid_c((d.P + kTile - 1) / kTile);
     const double B = 8.0 * d.P * d.nv;
-    launch(d, "profx_held_suarez", 12 * B,
+    launch(d, "profx_held_suarez", 24 * B,
            [&] { k_profx_hs<<<grid_c, blk, 0, d.stream>>>(g, f, d.prm.timestep, 1); });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("profx launch: ") + hipGetErrorString(e));
@@ -527,7 +537,7 @@ extern "C" thor_status thor_dyn_kernel_stat(thor_dyn *h, int i, const char **nam
     if (name) *name = n.c_str();
     if (launches) *launches = s.launches;
     if (total_ms) *total_ms = s.total_ms;
-    if (bytes_per_launch) *bytes_per_launch = s.bytes;
+    if (bytes_per_launch) *bytes_per_launch = s.launches ? s.bytes / s.launches : 0.0;
     return THOR_OK;
 }
 
