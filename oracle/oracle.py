@@ -67,6 +67,8 @@ def lib(variant: str = ""):
         L.oracle_tree_sum.restype = C.c_double
         L.oracle_math.argtypes = [C.c_int, C.c_double, C.c_double]
         L.oracle_math.restype = C.c_double
+        L.oracle_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
+        L.oracle_kernel.restype = C.c_int
         _libs[variant] = L
     return _libs[variant]
 
@@ -81,21 +83,26 @@ def math(fn: str, x: float, y: float = 0.0, variant: str = "") -> float:
     return lib(variant).oracle_math({"pow": 0, "log": 1, "sin": 2, "cos": 3}[fn], x, y)
 
 
+def _structs(grid, params):
+    g = _Grid(grid.P, grid.nv, grid.nr, 16, *(
+        np.ascontiguousarray(getattr(grid, n)).ctypes.data_as(_ip if n in ("point_local", "maps") else _dp)
+        for n in ("point_local", "maps", "div", "grad", "areas", "areasTr", "areasT", "nvecoa",
+                  "nvecti", "nvecte", "func_r", "lonlat", "Altitude", "Altitudeh")))
+    p = _Params(params.A, params.Omega, params.Gravit, params.P_Ref, params.Rd, params.Cp,
+                params.Diffc, params.DivDampc, params.timestep, params.glevel, params.HyDiffOrder,
+                params.n_substeps, params.DeepModel, params.NonHydro, params.HyDiff, params.DivDampP,
+                params.GravHeightVar)
+    return g, p
+
+
 class Oracle:
     """Serial host loop of THOR's kernels on a grid given in THOR's layout."""
 
     def __init__(self, grid, params, variant: str = ""):
         self.grid = grid  # keeps the geometry arrays alive
         self.L = lib(variant)
-        g = _Grid(grid.P, grid.nv, grid.nr, 16, *(
-            np.ascontiguousarray(getattr(grid, n)).ctypes.data_as(_ip if n in ("point_local", "maps") else _dp)
-            for n in ("point_local", "maps", "div", "grad", "areas", "areasTr", "areasT", "nvecoa",
-                      "nvecti", "nvecte", "func_r", "lonlat", "Altitude", "Altitudeh")))
+        g, p = _structs(grid, params)
         self._g = g
-        p = _Params(params.A, params.Omega, params.Gravit, params.P_Ref, params.Rd, params.Cp,
-                    params.Diffc, params.DivDampc, params.timestep, params.glevel, params.HyDiffOrder,
-                    params.n_substeps, params.DeepModel, params.NonHydro, params.HyDiff, params.DivDampP,
-                    params.GravHeightVar)
         self._h = C.c_void_p(self.L.oracle_create(C.byref(g), C.byref(p)))
 
     def __del__(self):
@@ -138,3 +145,70 @@ class Oracle:
         out = np.zeros(5)
         self.L.oracle_globdiag(self._h, out.ctypes.data_as(_dp))
         return out
+
+    def kernel(self, name: str, a0: int = 0, a1: int = 0):
+        """One restated kernel (see oracle_kernel in thor_oracle.h)."""
+        if self.L.oracle_kernel(self._h, name.encode(), a0, a1) != 0:
+            raise KeyError(name)
+
+
+REF_LIB = os.path.join(_HERE, "_ref", "libthor_ref.so")
+
+
+def ref_available() -> bool:
+    """True when THOR's own kernels have been compiled into oracle/_ref/."""
+    return os.path.exists(REF_LIB)
+
+
+class RefKernels:
+    """THOR's own kernels (compiled from the reference tree by `make -C oracle
+    ref`) on device copies of named arrays in THOR's layout; same names and
+    kernel arguments as Oracle.kernel. Needs a GPU."""
+
+    def __init__(self, grid, params):
+        L = C.CDLL(REF_LIB)
+        L.ref_create.restype = C.c_void_p
+        L.ref_create.argtypes = [C.POINTER(_Grid), C.POINTER(_Params)]
+        L.ref_destroy.argtypes = [C.c_void_p]
+        L.ref_destroy.restype = None
+        L.ref_array_size.argtypes = [C.c_void_p, C.c_char_p]
+        L.ref_array_size.restype = C.c_long
+        for f in (L.ref_set, L.ref_get):
+            f.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
+            f.restype = C.c_int
+        L.ref_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
+        L.ref_kernel.restype = C.c_int
+        self.L, self.grid = L, grid
+        g, p = _structs(grid, params)
+        self._g = g
+        h = L.ref_create(C.byref(g), C.byref(p))
+        if not h:
+            raise RuntimeError("ref_create failed (no GPU or out of memory)")
+        self._h = C.c_void_p(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self.L.ref_destroy(self._h)
+            self._h = None
+
+    def has(self, name: str) -> bool:
+        return self.L.ref_array_size(self._h, name.encode()) >= 0
+
+    def set(self, name: str, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if self.L.ref_set(self._h, name.encode(), a.ctypes.data_as(_dp), a.size) != 0:
+            raise KeyError(name)
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.L.ref_array_size(self._h, name.encode())
+        if n < 0:
+            raise KeyError(name)
+        a = np.zeros(n)
+        if self.L.ref_get(self._h, name.encode(), a.ctypes.data_as(_dp), n) != 0:
+            raise RuntimeError(f"ref_get {name}")
+        return a
+
+    def kernel(self, name: str, a0: int = 0, a1: int = 0):
+        rc = self.L.ref_kernel(self._h, name.encode(), a0, a1)
+        if rc != 0:
+            raise RuntimeError(f"ref_kernel {name}: {rc}")

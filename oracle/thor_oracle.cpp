@@ -1313,12 +1313,14 @@ void thor(Oracle &o) {
     copy(o, "pressure", "pressurek");
 }
 
+void held_suarez(Oracle &o);
+
 // ESP::ProfX for the Held-Suarez benchmark (profx_driver.cu:70-474).
 void profx_hs(Oracle &o) {
     const int nv = o.nv;
     double *W = o.A_("W"), *Wh = o.A_("Wh"), *T = o.A_("temperature"), *pt = o.A_("pt"), *p = o.A_("pressure"),
-           *Rho = o.A_("Rho"), *Rd = o.A_("Rd"), *Cp = o.A_("Cp"), *Mh = o.A_("Mh");
-    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat;
+           *Rho = o.A_("Rho"), *Rd = o.A_("Rd"), *Cp = o.A_("Cp");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     zero(o, "profx_Qheat");
     // Recompute_W (profx_auxiliary.h:110-130)
     for (int id = 0; id < o.P; id++)
@@ -1336,7 +1338,17 @@ void profx_hs(Oracle &o) {
             T[e] = p[e] / (Rd[e] * Rho[e]);
             pt[e] = (o.p.P_Ref / (Rd[e] * Rho[e])) * MPOW(p[e] / o.p.P_Ref, CvoCp);
         }
-    // held_suarez (profx_held_suarez.h:51-117)
+    held_suarez(o);
+    // Compute_pressure (profx_auxiliary.h:95-108)
+    for (int e = 0; e < o.P * nv; e++) p[e] = T[e] * Rd[e] * Rho[e];
+}
+
+// held_suarez (profx_held_suarez.h:51-117): Rayleigh drag on Mh and Newtonian
+// relaxation of temperature, with dt = the large time step.
+void held_suarez(Oracle &o) {
+    const int nv = o.nv;
+    double *T = o.A_("temperature"), *p = o.A_("pressure"), *Rho = o.A_("Rho"), *Mh = o.A_("Mh");
+    const double *Alt = o.g.Altitude, *ll = o.g.lonlat;
     const double Gravit = o.p.Gravit, kappa = o.p.Rd / o.p.Cp, dt = o.p.timestep;
     for (int id = 0; id < o.P; id++)
         for (int lev = 0; lev < nv; lev++) {
@@ -1355,8 +1367,6 @@ void profx_hs(Oracle &o) {
             for (int k = 0; k < 3; k++) Mh[(id * nv + lev) * 3 + k] = Mh[(id * nv + lev) * 3 + k] / (1.0 + kv * dt);
             T[id * nv + lev] -= kt * dt * (T[id * nv + lev] - Teq);
         }
-    // Compute_pressure (profx_auxiliary.h:95-108)
-    for (int e = 0; e < o.P * nv; e++) p[e] = T[e] * Rd[e] * Rho[e];
 }
 
 void make_kd(Oracle &o) {
@@ -1427,6 +1437,8 @@ oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
         o->arr["Cp"][e] = p->Cp;
     }
     make_kd(*o);
+    o->arr["Kdh4"] = o->Kdh4;  // read-only copies for callers
+    o->arr["Kdhz"] = o->Kdhz;
     return o;
 }
 
@@ -1450,6 +1462,28 @@ int oracle_get(oracle *o, const char *name, double *dst, long n) {
 }
 void oracle_thor(oracle *o) { thor(*o); }
 void oracle_profx_held_suarez(oracle *o) { profx_hs(*o); }
+int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
+    const std::string n(name);
+    if (n == "advec_cori1") {
+        advec_cori1(*o);
+        advec_poles(*o);
+    } else if (n == "advec_cori2") {
+        advec_cori2(*o);
+    } else if (n == "diffusion") {
+        diffusion_pass(*o, a0, a1 != 0);
+    } else if (n == "correct_horizontal") {
+        correct_horizontal(*o);
+    } else if (n == "divm") {
+        divm_pass(*o, "Mhk", "Whk", a0 != 0);
+    } else if (n == "slow_modes") {
+        slow_modes(*o);
+    } else if (n == "held_suarez") {
+        held_suarez(*o);
+    } else {
+        return -1;
+    }
+    return 0;
+}
 void oracle_steps(oracle *o, int n) {
     for (int i = 0; i < n; i++) {
         profx_hs(*o);

@@ -44,6 +44,17 @@ int oracle_get(oracle *o, const char *name, double *dst, long n);
 void oracle_thor(oracle *o);
 // ESP::ProfX with the Held-Suarez benchmark (no physics modules).
 void oracle_profx_held_suarez(oracle *o);
+// One restated kernel (with its pole kernel, where THOR has one) on the
+// current arrays, for kernel-by-kernel comparison with THOR's own kernels:
+//   "advec_cori1"  Compute_Advec_Cori1 + _Poles   (thor_adv_cor.h:51-362)
+//   "advec_cori2"  Compute_Advec_Cori2            (thor_adv_cor.h:366-536)
+//   "diffusion"    Diffusion_Op + _Poles, a0 = stepnum, a1 = laststep (thor_diff.h:54-880)
+//   "correct_horizontal"                          (thor_diff.h:882-915)
+//   "divm"         DivM_Op + _Poles on Mhk/Whk, a0 = laststep (thor_div.h:49-535)
+//   "slow_modes"   Compute_Slow_Modes + _Poles    (thor_slowmodes.h:48-683)
+//   "held_suarez"  held_suarez                    (profx_held_suarez.h:51-117)
+// Returns 0, or -1 for an unknown name. "Kdh4"/"Kdhz" are readable arrays.
+int oracle_kernel(oracle *o, const char *name, int a0, int a1);
 // ProfX + Thor, n times.
 void oracle_steps(oracle *o, int n);
 // Global mass, total energy, angular momentum (x, y, z): per-cell values summed
