@@ -155,8 +155,10 @@ static int thor_step(Dyn &d) {
     const int hydiff = d.prm.HyDiff, divdamp = d.prm.DivDampP;
     const dim3 blk(kTile), grid_s((d.P + kTile - 1) / kTile, (d.nv + kLC - 1) / kLC);
     const dim3 grid_t2(8 * d.g.tiles_per_xcd * d.nv);  // LDS-tiled stencils: one tile x one level per block
-    const dim3 grid_c((d.P + kTile - 1) / kTile);
-    const dim3 grid_t((d.P + kThomasBlock - 1) / kThomasBlock);
+    const dim3 grid_c((d.P + kTile - 1) / kTile, d.nv + 1);  // column_prep: (column, level 0..nv)
+    const int cb = kThomasCB(d.nv);
+    const dim3 grid_t((d.P + cb - 1) / cb);
+    const size_t lds_t = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
     hipStream_t st = d.stream;
     for (int rk = 0; rk < 3; rk++) {
         const int ns_it = rk == 0 ? 1 : (rk == 1 ? ns_total / 2 : ns_total);
@@ -172,13 +174,19 @@ static int thor_step(Dyn &d) {
         launch(d, "slow_modes", ((divdamp ? 6 : 0) + 40 + (rk ? 19 : 0)) * B,
                [&] { k_slow<<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
         if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<<<grid_t2, blk, 0, st>>>(g, f); });
-        launch(d, "thomas_factor", 5 * B, [&] { k_thomas_factor<<<grid_t, kThomasBlock, 0, st>>>(g, f, times); });
+        launch(d, "thomas_factor", 5 * B, [&] {
+            if (cb == 32) k_thomas_factor<32><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            else k_thomas_factor<16><<<grid_t, 256, lds_t, st>>>(g, f, times);
+        });
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
             launch(d, "momentum", ((divdamp ? 6 : 0) + 13) * B,
                    [&] { k_momentum<<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
             launch(d, "prepare", 10 * B, [&] { k_prepare<<<grid_t2, blk, 0, st>>>(g, f); });
-            launch(d, "thomas_solve", 12 * B, [&] { k_thomas_solve<<<grid_t, kThomasBlock, 0, st>>>(g, f, times); });
+            launch(d, "thomas_solve", 12 * B, [&] {
+                if (cb == 32) k_thomas_solve<32><<<grid_t, 256, lds_t, st>>>(g, f, times);
+                else k_thomas_solve<16><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            });
             launch(d, "density_pressure", (22 + (next ? 7 : 0)) * B,
                    [&] { k_density_pressure<<<grid_t2, blk, 0, st>>>(g, f, times, next); });
         }
@@ -199,7 +207,7 @@ static int profx(Dyn &d, int forcing) {
     if (forcing != THOR_FORCING_HELD_SUAREZ) return fail(THOR_ERR_BAD_ARG, "unknown forcing");
     Fields f = stage_fields(d, 0);
     const Geo g = d.g;
-    const dim3 blk(kTile), grid_c((d.P + kTile - 1) / kTile);
+    const dim3 blk(kTile), grid_c((d.P + 63) / 64);  // 64 columns x 4 level rows per block
     const double B = 8.0 * d.P * d.nv;
     launch(d, "profx_held_suarez", 24 * B,
            [&] { k_profx_hs<<<grid_c, blk, 0, d.stream>>>(g, f, d.prm.timestep, 1); });
@@ -279,6 +287,9 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     if (prm->precision != THOR_PRECISION_FP64)
         return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: precision mode not built");
     if (gd->nv < 3 || gd->point_num < 12) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: grid too small");
+    if (4 * (gd->nv + 1) * kThomasCB(gd->nv) * 8 > 65536)
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: at most 127 levels (the vertical solve stages 16 "
+                                          "columns x (nv + 1) levels x 4 planes in 64 KiB of LDS)");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(THOR_ERR_NO_DEVICE, "thor_dyn_create: no HIP device");

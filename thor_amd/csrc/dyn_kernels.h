@@ -300,196 +300,168 @@ __device__ __forceinline__ bool pole_ctx(const Geo &g, PoleCtx &c) {
 }
 
 // ===================================================================== K_A
-// Per column: Compute_Temperature_H_Pt_Geff (thor_auxiliary.h:55-309, entropy
-// form) and the vertical advection + Coriolis part of Compute_Advec_Cori2
-// (thor_adv_cor.h:366-536), written to Adv; the horizontal part is added by
-// k_adv_diff1.
+// Compute_Temperature_H_Pt_Geff (thor_auxiliary.h:55-309, entropy form) and
+// the vertical advection + Coriolis part of Compute_Advec_Cori2
+// (thor_adv_cor.h:366-536), written to Adv; k_adv_diff1 adds the horizontal
+// part. The reference runs both as one serial sweep per column; here one
+// thread takes one (column, level) pair, lev in [0, nv], and recomputes the
+// values the sweep carries from the level below from their own inputs, so
+// every output is the same number the sweep produces.
 __global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lev = blockIdx.y;  // 0..nv: centre lev (if < nv) and interface lev
     if (p >= g.P) return;
     const size_t S = g.S;
     const int nv = g.nv;
     const double Rd = g.Rd, Cp = g.Cp, P_Ref = g.P_Ref, Gravit = g.Gravit, A = g.A;
+    const double Cv = Cp - Rd, CvoCp = Cv / Cp;
     const double *Alt = g.Alt, *Alth = g.Alth;
     const double *pk = f.pk, *Rho = f.Rhok, *Wh = f.Whk;
-    {
-        double rho = 0, rhoh, rhol = 0, rhop, pressure = 0, h = 0, hl = 0, pt = 0, ptl = 0, pl = 0, pp, dpdz;
-        double gtilh = 0, gtilht, dz, dp_dz, Cv, xi, xim, xip, intt, intl, extr;
-        double alt = 0, altl = 0, alth = 0, altht = 0;
-        for (int lev = 0; lev < nv + 1; lev++) {
-            if (lev < nv) {
-                Cv = Cp - Rd;
-                const double CvoCp = Cv / Cp;
-                if (lev > 0) {
-                    pl = pressure;
-                    rhol = rho;
-                    hl = h;
-                    ptl = pt;
-                    altl = alt;
-                }
-                pressure = LV(pk, lev);
-                rho = LV(Rho, lev);
-                const double temperature = pressure / (rho * Rd);
-                LV(f.T, lev) = temperature;
-                alt = Alt[lev];
-                alth = Alth[lev];
-                altht = Alth[lev + 1];
-                h = Cv * temperature + pressure / rho;
-                LV(f.h, lev) = h;
-                pt = (P_Ref / (Rd * rho)) * MPOW(pressure / P_Ref, CvoCp);
-                LV(f.pt, lev) = pt;
-            }
-            if (lev == 0) {
-                extr = (-alt - Alt[lev + 1]) / (Alt[lev + 1] - alt);
-                LV(f.hh, 0) = h;
-                if (g.ghv)
-                    pl = LV(pk, 1) - rho * Gravit * MPOW(A / (A + Alt[0]), 2) * (-alt - Alt[1]);
-                else
-                    pl = LV(pk, 1) - rho * Gravit * (-alt - Alt[1]);
-                rhol = LV(Rho, 1) + (LV(Rho, 1) - rho) * extr;
-                rhoh = 0.5 * (rho + rhol);
-                dz = 2.0 * Alt[0];
-                dpdz = (LV(pk, 0) - pl) / dz;
-                LV(f.gtilh, 0) = -(1.0 / rhoh) * dpdz;
-                gtilh = 0.0;
-            } else if (lev == nv) {
-                extr = (2 * Alth[nv] - alt - Alt[nv - 2]) / (Alt[nv - 1] - Alt[nv - 2]);
-                LV(f.hh, nv) = h;
-                rhop = LV(Rho, nv - 2) + (LV(Rho, nv - 1) - LV(Rho, nv - 2)) * extr;
-                rhoh = 0.5 * (rho + rhop);
-                LV(f.pth, nv) = LV(f.pt, nv - 2) + (LV(f.pt, nv - 1) - LV(f.pt, nv - 2)) * extr;
-                LV(f.pth, 0) = LV(f.pt, 1) + (LV(f.pt, 1) - LV(f.pt, 0)) * (Alth[0] - Alt[1]) / (Alt[1] - Alt[0]);
-                dz = 2.0 * (Alth[nv] - Alt[nv - 1]);
-                if (g.ghv)
-                    pp = LV(pk, nv - 2)
-                         - LV(Rho, nv - 1) * Gravit * MPOW(A / (A + Alt[nv - 1]), 2)
-                               * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
-                else
-                    pp = LV(pk, nv - 2) - LV(Rho, nv - 1) * Gravit * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
-                dpdz = (pp - LV(pk, nv - 1)) / dz;
-                LV(f.gtilh, nv) = -(1.0 / rhoh) * dpdz;
-                gtilht = 0.0;
-                xi = alt;
-                xim = alth;
-                xip = altht;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                LV(f.gtil, nv - 1) = gtilh * intt + gtilht * intl;
-            } else {
-                dz = alt - altl;
-                dp_dz = (pressure - pl) / dz;
-                xi = alth;
-                xim = altl;
-                xip = alt;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                LV(f.hh, lev) = hl * intt + h * intl;
-                LV(f.pth, lev) = ptl * intt + pt * intl;
-                rhoh = rhol * intt + rho * intl;
-                gtilht = -(LV(Wh, lev) / rhoh) * dp_dz;
-                xi = alt;
-                xim = alth;
-                xip = altht;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                LV(f.gtil, lev - 1) = gtilh * intt + gtilht * intl;
-                gtilh = gtilht;
-                LV(f.gtilh, lev) = -(1.0 / rhoh) * dp_dz;
-            }
+    auto h_at = [&](int l) {
+        const double pr = LV(pk, l), r = LV(Rho, l);
+        const double temperature = pr / (r * Rd);
+        return Cv * temperature + pr / r;
+    };
+    auto pt_at = [&](int l) {
+        const double pr = LV(pk, l), r = LV(Rho, l);
+        return (P_Ref / (Rd * r)) * MPOW(pr / P_Ref, CvoCp);
+    };
+    // interpolation weights of the point xi between xim and xip
+    auto wts = [](double xi, double xim, double xip, double &intt, double &intl) {
+        intt = (xi - xip) / (xim - xip);
+        intl = (xi - xim) / (xip - xim);
+    };
+    // -(Wh/rho_h) dp/dz at interior interface k (the sweep's gtilht), 0 at k = 0
+    auto gtil_tilde = [&](int k) {
+        if (k == 0) return 0.0;
+        const double dp_dz = (LV(pk, k) - LV(pk, k - 1)) / (Alt[k] - Alt[k - 1]);
+        double intt, intl;
+        wts(Alth[k], Alt[k - 1], Alt[k], intt, intl);
+        const double rhoh = LV(Rho, k - 1) * intt + LV(Rho, k) * intl;
+        return -(LV(Wh, k) / rhoh) * dp_dz;
+    };
+
+    // ---- centre values
+    if (lev < nv) {
+        const double pressure = LV(pk, lev), rho = LV(Rho, lev);
+        const double temperature = pressure / (rho * Rd);
+        LV(f.T, lev) = temperature;
+        LV(f.h, lev) = Cv * temperature + pressure / rho;
+        LV(f.pt, lev) = (P_Ref / (Rd * rho)) * MPOW(pressure / P_Ref, CvoCp);
+        // gtil at this centre: the sweep forms it at interface lev + 1 from the
+        // interface values below and above, weighted at (Alt, Alth, Alth+1) of
+        // that sweep step
+        double intt, intl;
+        if (lev < nv - 1) {
+            wts(Alt[lev + 1], Alth[lev + 1], Alth[lev + 2], intt, intl);
+            LV(f.gtil, lev) = gtil_tilde(lev) * intt + gtil_tilde(lev + 1) * intl;
+        } else {
+            wts(Alt[nv - 1], Alth[nv - 1], Alth[nv], intt, intl);
+            const double gtilht = 0.0;
+            LV(f.gtil, nv - 1) = gtil_tilde(nv - 1) * intt + gtilht * intl;
         }
     }
-    // Vertical advection of momentum and Coriolis (Compute_Advec_Cori2).
-    {
-        const double fr0 = g.fr[p], fr1 = g.fr[S + p], fr2 = g.fr[2 * S + p];
-        const double Omega = g.Omega;
-        auto vel = [&](int lev, double v[3]) {
-            const double rho = LV(Rho, lev), w = LV(f.Wk, lev);
-            const double fr[3] = {fr0, fr1, fr2};
+    // ---- interface values
+    if (lev == 0) {
+        const double rho = LV(Rho, 0), alt = Alt[0];
+        const double extr = (-alt - Alt[1]) / (Alt[1] - alt);
+        LV(f.hh, 0) = h_at(0);
+        double pl;
+        if (g.ghv) pl = LV(pk, 1) - rho * Gravit * MPOW(A / (A + Alt[0]), 2) * (-alt - Alt[1]);
+        else pl = LV(pk, 1) - rho * Gravit * (-alt - Alt[1]);
+        const double rhol = LV(Rho, 1) + (LV(Rho, 1) - rho) * extr;
+        const double rhoh = 0.5 * (rho + rhol);
+        const double dz = 2.0 * Alt[0];
+        const double dpdz = (LV(pk, 0) - pl) / dz;
+        LV(f.gtilh, 0) = -(1.0 / rhoh) * dpdz;
+        const double pt0 = pt_at(0), pt1 = pt_at(1);
+        LV(f.pth, 0) = pt1 + (pt1 - pt0) * (Alth[0] - Alt[1]) / (Alt[1] - Alt[0]);
+    } else if (lev == nv) {
+        const double alt = Alt[nv - 1], rho = LV(Rho, nv - 1);
+        const double extr = (2 * Alth[nv] - alt - Alt[nv - 2]) / (Alt[nv - 1] - Alt[nv - 2]);
+        LV(f.hh, nv) = h_at(nv - 1);
+        const double rhop = LV(Rho, nv - 2) + (LV(Rho, nv - 1) - LV(Rho, nv - 2)) * extr;
+        const double rhoh = 0.5 * (rho + rhop);
+        const double ptm = pt_at(nv - 2), ptc = pt_at(nv - 1);
+        LV(f.pth, nv) = ptm + (ptc - ptm) * extr;
+        const double dz = 2.0 * (Alth[nv] - Alt[nv - 1]);
+        double pp;
+        if (g.ghv)
+            pp = LV(pk, nv - 2)
+                 - LV(Rho, nv - 1) * Gravit * MPOW(A / (A + Alt[nv - 1]), 2) * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+        else
+            pp = LV(pk, nv - 2) - LV(Rho, nv - 1) * Gravit * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+        const double dpdz = (pp - LV(pk, nv - 1)) / dz;
+        LV(f.gtilh, nv) = -(1.0 / rhoh) * dpdz;
+    } else {
+        const double dz = Alt[lev] - Alt[lev - 1];
+        const double dp_dz = (LV(pk, lev) - LV(pk, lev - 1)) / dz;
+        double intt, intl;
+        wts(Alth[lev], Alt[lev - 1], Alt[lev], intt, intl);
+        LV(f.hh, lev) = h_at(lev - 1) * intt + h_at(lev) * intl;
+        LV(f.pth, lev) = pt_at(lev - 1) * intt + pt_at(lev) * intl;
+        const double rhoh = LV(Rho, lev - 1) * intt + LV(Rho, lev) * intl;
+        LV(f.gtilh, lev) = -(1.0 / rhoh) * dp_dz;
+    }
+    if (lev == nv) return;
+
+    // ---- vertical advection of momentum and Coriolis (Compute_Advec_Cori2)
+    const double fr[3] = {g.fr[p], g.fr[S + p], g.fr[2 * S + p]};
+    const double Omega = g.Omega;
+    auto vel = [&](int l, double v[3]) {
+        const double rho = LV(Rho, l), w = LV(f.Wk, l);
 #pragma unroll
-            for (int c = 0; c < 3; c++) v[c] = VEC(f.Mhk, c, lev, p) / rho + (w / rho) * fr[c];
-        };
-        double altht = 0, althl = 0, altt = 0, alt = 0, altl = 0, wht = 0, whl = 0;
-        double vl[3] = {0, 0, 0}, vc[3] = {0, 0, 0}, vt[3] = {0, 0, 0};
-        double dvl[3], dvt[3], intt, intl, xi, xim, xip;
-        for (int lev = 0; lev < nv; lev++) {
-            if (lev == 0) {
-                dvl[0] = dvl[1] = dvl[2] = 0.0;
-                altht = Alth[lev + 1];
-                althl = Alth[lev];
-                alt = Alt[lev];
-                altt = Alt[lev + 1];
-                vel(lev, vc);
-                vel(lev + 1, vt);
-                wht = LV(Wh, lev + 1);
-                xi = altht;
-                xim = alt;
-                xip = altt;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                for (int c = 0; c < 3; c++) dvt[c] = (vc[c] * intt + vt[c] * intl) * wht;
-            } else if (lev == nv - 1) {
-                dvt[0] = dvt[1] = dvt[2] = 0.0;
-                xi = althl;
-                xim = altl;
-                xip = alt;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                for (int c = 0; c < 3; c++) dvl[c] = (vl[c] * intt + vc[c] * intl) * whl;
-            } else {
-                xi = althl;
-                xim = altl;
-                xip = alt;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                for (int c = 0; c < 3; c++) dvl[c] = (vl[c] * intt + vc[c] * intl) * whl;
-                xi = altht;
-                xim = alt;
-                xip = altt;
-                intt = (xi - xip) / (xim - xip);
-                intl = (xi - xim) / (xip - xim);
-                for (int c = 0; c < 3; c++) dvt[c] = (vc[c] * intt + vt[c] * intl) * wht;
-            }
-            double r2p, r2m, r2l;
-            if (g.deep) {
-                r2p = MPOW(altht + A, 2.0);
-                r2m = MPOW(alt + A, 2.0);
-                r2l = MPOW(althl + A, 2.0);
-            } else {
-                r2p = r2m = r2l = 1.0;
-            }
-            const double dzi = 1.0 / ((althl - altht) * r2m);
-            double dav[3];
-            for (int c = 0; c < 3; c++) dav[c] = (dvl[c] * r2l - dvt[c] * r2p) * dzi;
-            const double rho = LV(Rho, lev);
-            double Cx, Cy, Cz;
-            if (g.deep) {
-                Cx = -2.0 * Omega * vc[1] * rho;
-                Cy = 2.0 * Omega * vc[0] * rho;
-                Cz = 0.0;
-            } else {
-                Cx = 2 * Omega * (vc[2] * fr2 * fr1 - vc[1] * MPOW(fr2, 2)) * rho;
-                Cy = -2 * Omega * (vc[2] * fr2 * fr0 - vc[0] * MPOW(fr2, 2)) * rho;
-                Cz = 2 * Omega * (vc[1] * fr2 * fr0 - vc[0] * fr2 * fr1) * rho;
-            }
-            VEC(f.Adv, 0, lev, p) = dav[0] + Cx;
-            VEC(f.Adv, 1, lev, p) = dav[1] + Cy;
-            VEC(f.Adv, 2, lev, p) = dav[2] + Cz;
-            if (lev < nv - 1) {
-                althl = altht;
-                altht = Alth[lev + 2];
-                altl = alt;
-                alt = altt;
-                if (lev < nv - 2) altt = Alt[lev + 2];
-                whl = wht;
-                if (lev < nv - 2) wht = LV(Wh, lev + 2);
-                for (int c = 0; c < 3; c++) {
-                    vl[c] = vc[c];
-                    vc[c] = vt[c];
-                }
-                if (lev < nv - 2) vel(lev + 2, vt);
-            }
-        }
+        for (int c = 0; c < 3; c++) v[c] = VEC(f.Mhk, c, l, p) / rho + (w / rho) * fr[c];
+    };
+    const double althl = Alth[lev], altht = Alth[lev + 1], alt = Alt[lev];
+    double vc[3], dvl[3], dvt[3], intt, intl;
+    vel(lev, vc);
+    if (lev == 0) {
+        dvl[0] = dvl[1] = dvl[2] = 0.0;
+    } else {
+        double vl[3];
+        vel(lev - 1, vl);
+        const double whl = LV(Wh, lev);
+        wts(althl, Alt[lev - 1], alt, intt, intl);
+#pragma unroll
+        for (int c = 0; c < 3; c++) dvl[c] = (vl[c] * intt + vc[c] * intl) * whl;
     }
+    if (lev == nv - 1) {
+        dvt[0] = dvt[1] = dvt[2] = 0.0;
+    } else {
+        double vt[3];
+        vel(lev + 1, vt);
+        const double wht = LV(Wh, lev + 1);
+        wts(altht, alt, Alt[lev + 1], intt, intl);
+#pragma unroll
+        for (int c = 0; c < 3; c++) dvt[c] = (vc[c] * intt + vt[c] * intl) * wht;
+    }
+    double r2p, r2m, r2l;
+    if (g.deep) {
+        r2p = MPOW(altht + A, 2.0);
+        r2m = MPOW(alt + A, 2.0);
+        r2l = MPOW(althl + A, 2.0);
+    } else {
+        r2p = r2m = r2l = 1.0;
+    }
+    const double dzi = 1.0 / ((althl - altht) * r2m);
+    double dav[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) dav[c] = (dvl[c] * r2l - dvt[c] * r2p) * dzi;
+    const double rho = LV(Rho, lev);
+    double Cx, Cy, Cz;
+    if (g.deep) {
+        Cx = -2.0 * Omega * vc[1] * rho;
+        Cy = 2.0 * Omega * vc[0] * rho;
+        Cz = 0.0;
+    } else {
+        Cx = 2 * Omega * (vc[2] * fr[2] * fr[1] - vc[1] * MPOW(fr[2], 2)) * rho;
+        Cy = -2 * Omega * (vc[2] * fr[2] * fr[0] - vc[0] * MPOW(fr[2], 2)) * rho;
+        Cz = 2 * Omega * (vc[1] * fr[2] * fr[0] - vc[0] * fr[2] * fr[1]) * rho;
+    }
+    VEC(f.Adv, 0, lev, p) = dav[0] + Cx;
+    VEC(f.Adv, 1, lev, p) = dav[1] + Cy;
+    VEC(f.Adv, 2, lev, p) = dav[2] + Cz;
 }
 
 // ===================================================================== K_B
@@ -953,7 +925,8 @@ __global__ STENCIL_BOUNDS void k_momentum(Geo g, Fields f, int divdamp, double d
 // source terms Sp, Sd and Vertical_Eq (thor_vertical_int.h:52-563): the
 // tridiagonal solve for Whs at interfaces 1..nv-1 with the modify-in-place
 // Thomas algorithm; cc/dd live in LDS.
-constexpr int kThomasBlock = 64;
+// Columns per Thomas block: 4 LDS planes of (nv + 1) x CB doubles must fit 64 KiB.
+__host__ __device__ constexpr int kThomasCB(int nv) { return 4 * (nv + 1) * 32 * 8 <= 65536 ? 32 : 16; }
 
 // Prepare_Implicit_Vertical + _Poles (thor_vertical_int.h:566-771): source
 // terms Sp = -(Rd/Cv) div(Mhs h) + Slowpressure, Sd = -div(Mhs) + SlowRho.
@@ -1024,12 +997,20 @@ struct Plane {
     __device__ __forceinline__ double &operator[](int lev) const { return b[(size_t)lev * S + p]; }
 };
 
-__global__ __launch_bounds__(kThomasBlock) void k_thomas_factor(Geo g, Fields f, double deltat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+// Thomas kernels: a block of 256 threads serves kThomasCB(nv) columns. First
+// every thread computes per-level terms (coefficients or right-hand sides)
+// for a share of the levels of its column into LDS, all loads in flight at
+// once; then one lane per column runs the serial recurrence out of LDS. The
+// arithmetic of each term and of the recurrence is the reference's.
+// LDS: 4 planes of (nv + 1) x CB doubles.
+template <int CB>
+__global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double deltat) {
+    extern __shared__ double sm[];
+    const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
+    constexpr int NROW = 256 / CB;
+    const int p = blockIdx.x * CB + c;
     const size_t S = g.S;
-    const int nv = g.nv;
-    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+    double *sA = sm, *sB = sm + (nv + 1) * CB, *sC = sm + 2 * (nv + 1) * CB;
     const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
     const double *Alt = g.Alt, *Alth = g.Alth;
     const bool ghv = g.ghv, nonh = g.nonhydro;
@@ -1037,44 +1018,54 @@ __global__ __launch_bounds__(kThomasBlock) void k_thomas_factor(Geo g, Fields f,
     auto gcor = [&](double altx) {
         return ghv ? Gravit * MPOW(A / (A + altx), 2) * (Cp - Rd) / Rd : Gravit * (Cp - Rd) / Rd;
     };
-    double ccp = 0.0;
-#pragma unroll 2
-    for (int lev = 1; lev < nv; lev++) {
-        const double altht = Alth[lev + 1], alth = Alth[lev], althl = Alth[lev - 1];
-        const double alt = Alt[lev], altl = Alt[lev - 1];
-        double hp = LV(f.hh, lev + 1), h = LV(f.hh, lev), hm = LV(f.hh, lev - 1);
-        const double gp = LV(f.gtilh, lev + 1), gg = LV(f.gtilh, lev), gm = LV(f.gtilh, lev - 1);
-        const double GCoRl = gcor(altl), GCoRu = gcor(alt);
-        const double dzp = 1.0 / (altht - alth), dzh = 1.0 / (alt - altl), dzm = 1.0 / (alth - althl);
-        const double dzph = dzp * dzh, dzmh = dzm * dzh;
-        const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
-        const double GCoR = (GCoRl * (alt - alth) + GCoRu * (alth - altl)) / (alt - altl);
-        const double intl = (alt - alth) * dzp * dzh;
-        const double inttm = -(altl - alth) * dzm * dzh;
-        double aa, bb, cc;
-        if (g.deep) {
-            hp = hp * MPOW(altht + A, 2);
-            h = h * MPOW(alth + A, 2);
-            hm = hm * MPOW(althl + A, 2);
-            const double or2 = 1 / MPOW(alth + A, 2);
-            const double tor3 = 2 / MPOW(alth + A, 3);
-            cc = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
-            if (nonh) bb = CRdd + (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
-            else bb = (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
-            aa = -dzmh * or2 * hm + intl * (gm + GCoR - tor3 * hm);
-        } else {
-            cc = -dzph * hp - inttm * (gp + GCoR);
-            if (nonh) bb = CRdd + (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
-            else bb = (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
-            aa = -dzmh * hm + intl * (gm + GCoR);
+    if (p < g.P) {
+        for (int lev = 1 + row; lev < nv; lev += NROW) {
+            const double altht = Alth[lev + 1], alth = Alth[lev], althl = Alth[lev - 1];
+            const double alt = Alt[lev], altl = Alt[lev - 1];
+            double hp = LV(f.hh, lev + 1), h = LV(f.hh, lev), hm = LV(f.hh, lev - 1);
+            const double gp = LV(f.gtilh, lev + 1), gg = LV(f.gtilh, lev), gm = LV(f.gtilh, lev - 1);
+            const double GCoRl = gcor(altl), GCoRu = gcor(alt);
+            const double dzp = 1.0 / (altht - alth), dzh = 1.0 / (alt - altl), dzm = 1.0 / (alth - althl);
+            const double dzph = dzp * dzh, dzmh = dzm * dzh;
+            const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
+            const double GCoR = (GCoRl * (alt - alth) + GCoRu * (alth - altl)) / (alt - altl);
+            const double intl = (alt - alth) * dzp * dzh;
+            const double inttm = -(altl - alth) * dzm * dzh;
+            double aa, bb, cc;
+            if (g.deep) {
+                hp = hp * MPOW(altht + A, 2);
+                h = h * MPOW(alth + A, 2);
+                hm = hm * MPOW(althl + A, 2);
+                const double or2 = 1 / MPOW(alth + A, 2);
+                const double tor3 = 2 / MPOW(alth + A, 3);
+                cc = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
+                if (nonh) bb = CRdd + (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
+                else bb = (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
+                aa = -dzmh * or2 * hm + intl * (gm + GCoR - tor3 * hm);
+            } else {
+                cc = -dzph * hp - inttm * (gp + GCoR);
+                if (nonh) bb = CRdd + (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
+                else bb = (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
+                aa = -dzmh * hm + intl * (gm + GCoR);
+            }
+            sA[lev * CB + c] = aa;
+            sB[lev * CB + c] = bb;
+            sC[lev * CB + c] = cc;
         }
+    }
+    __syncthreads();
+    if (row != 0 || p >= g.P) return;
+    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+    double ccp = 0.0;
+    for (int lev = 1; lev < nv; lev++) {
+        const double aa = sA[lev * CB + c], bb = sB[lev * CB + c], cc = sC[lev * CB + c];
         if (lev == 1) {
             ccp = cc / bb;
             TT[1] = bb;
         } else {
-            const double t = 1.0 / (bb - ccp * aa);
-            ccp = cc * t;
-            TT[lev] = t;
+            const double tt = 1.0 / (bb - ccp * aa);
+            ccp = cc * tt;
+            TT[lev] = tt;
             AA[lev] = aa;
         }
         CC[lev] = ccp;
@@ -1085,64 +1076,75 @@ __global__ __launch_bounds__(kThomasBlock) void k_thomas_factor(Geo g, Fields f,
 // right-hand side C0 from the substep's sources, forward elimination into Whs
 // (as the reference's dd), back substitution Whs_k = -cc'_k Whs_{k+1} + dd_k,
 // and Whs = 0 at the bottom and top interfaces.
-__global__ __launch_bounds__(kThomasBlock) void k_thomas_solve(Geo g, Fields f, double deltat) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+template <int CB>
+__global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double deltat) {
+    extern __shared__ double sm[];
+    const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
+    constexpr int NROW = 256 / CB;
+    const int p = blockIdx.x * CB + c;
     const size_t S = g.S;
-    const int nv = g.nv;
-    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p}, W{f.Whs, S, p};
+    double *sD = sm, *sA = sm + (nv + 1) * CB, *sT = sm + 2 * (nv + 1) * CB, *sC = sm + 3 * (nv + 1) * CB;
     const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
     const double *Alt = g.Alt, *Alth = g.Alth;
     const bool ghv = g.ghv, nonh = g.nonhydro;
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
     const double dt2 = MPOW(deltat, 2.0);
-    double Spl = LV(f.Sp, 0), Sdl = LV(f.Sd, 0), pl = LV(f.ps, 0), rl = LV(f.Rhos, 0);
-    double ddp = 0.0;
-#pragma unroll 2
-    for (int lev = 1; lev < nv; lev++) {
-        const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
-        const double Sp = LV(f.Sp, lev), Sd = LV(f.Sd, lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
-        const double dzh = 1.0 / (alt - altl);
-        const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
-        const double dSpdz = (Sp - Spl) * dzh, dPdz = (pr - pl) * dzh;
-        const double intt = (alth - alt) / (altl - alt), intl = (alth - altl) / (alt - altl);
-        const double rhohs = rl * intt + r * intl;
-        const double Sdh = Sdl * intt + Sd * intl;
-        const double srh = LV(f.SlowWh, lev);
-        double C0;
-        if (!nonh) {
-            if (ghv)
-                C0 = (dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
-                      + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
-                     * (CRdd);
-            else
-                C0 = (dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz - deltat * srh)
-                     * (CRdd);
-        } else {
-            const double whs = W[lev];
-            if (ghv)
-                C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
-                      + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
-                     * (CRdd);
-            else
-                C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
-                      - deltat * srh)
-                     * (CRdd);
+    if (p < g.P) {
+        const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+        for (int lev = 1 + row; lev < nv; lev += NROW) {
+            const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
+            const double Sp = LV(f.Sp, lev), Sd = LV(f.Sd, lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
+            const double Spl = LV(f.Sp, lev - 1), Sdl = LV(f.Sd, lev - 1), pl = LV(f.ps, lev - 1),
+                         rl = LV(f.Rhos, lev - 1);
+            const double dzh = 1.0 / (alt - altl);
+            const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
+            const double dSpdz = (Sp - Spl) * dzh, dPdz = (pr - pl) * dzh;
+            const double intt = (alth - alt) / (altl - alt), intl = (alth - altl) / (alt - altl);
+            const double rhohs = rl * intt + r * intl;
+            const double Sdh = Sdl * intt + Sd * intl;
+            const double srh = LV(f.SlowWh, lev);
+            double C0;
+            if (!nonh) {
+                if (ghv)
+                    C0 = (dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                          + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
+                         * (CRdd);
+                else
+                    C0 = (dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                          - deltat * srh)
+                         * (CRdd);
+            } else {
+                const double whs = LV(f.Whs, lev);
+                if (ghv)
+                    C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
+                          + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
+                         * (CRdd);
+                else
+                    C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                          - deltat * srh)
+                         * (CRdd);
+            }
+            sD[lev * CB + c] = C0;
+            sA[lev * CB + c] = lev > 1 ? AA[lev] : 0.0;  // AA[1] is never formed
+            sT[lev * CB + c] = TT[lev];
+            sC[lev * CB + c] = CC[lev];
         }
-        if (lev == 1) ddp = (-C0) / TT[1];
-        else ddp = (-C0 - ddp * AA[lev]) * TT[lev];
-        W[lev] = ddp;
-        Spl = Sp;
-        Sdl = Sd;
-        pl = pr;
-        rl = r;
+    }
+    __syncthreads();
+    if (row != 0 || p >= g.P) return;
+    const Plane W{f.Whs, S, p};
+    double ddp = 0.0;
+    for (int lev = 1; lev < nv; lev++) {
+        const double C0 = sD[lev * CB + c];
+        if (lev == 1) ddp = (-C0) / sT[CB + c];
+        else ddp = (-C0 - ddp * sA[lev * CB + c]) * sT[lev * CB + c];
+        sD[lev * CB + c] = ddp;
     }
     W[nv] = 0.0;
     W[0] = 0.0;
     double wup = 0.0;
-#pragma unroll 2
     for (int lev = nv - 1; lev > 0; lev--) {
-        const double w = (-CC[lev] * wup + W[lev]);
+        const double w = (-sC[lev * CB + c] * wup + sD[lev * CB + c]);
         W[lev] = w;
         wup = w;
     }
@@ -1316,9 +1318,14 @@ __global__ __launch_bounds__(256) void k_update_rk2(Geo g, Fields f) {
 // ESP::ProfX for the Held-Suarez benchmark, per column: Recompute_W
 // (profx_auxiliary.h:110-130), Compute_temperature (:64-93), held_suarez
 // (profx_held_suarez.h:51-117) and Compute_pressure (:95-108).
+// Block of 64 columns x 4 level rows; the surface pressure of each column is
+// read before any level of it is overwritten (one barrier), then every
+// (column, level) is independent.
 __global__ __launch_bounds__(256) void k_profx_hs(Geo g, Fields f, double time_step, int zero_q) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+    constexpr int CB = 64, NROW = 4;
+    const int c = threadIdx.x % CB, row = threadIdx.x / CB;
+    const int p = blockIdx.x * CB + c;
+    const bool valid = p < g.P;
     const size_t S = g.S;
     const int nv = g.nv;
     const double *Alt = g.Alt, *Alth = g.Alth;
@@ -1326,11 +1333,16 @@ __global__ __launch_bounds__(256) void k_profx_hs(Geo g, Fields f, double time_s
     const double kappa = Rd / Cp;
     const double sigmab = 0.7, ka = (1.0 / 40.0) * (1.0 / 86400.0), kf = 1.0 / 86400.0,
                  ks = (1.0 / 4.0) * (1.0 / 86400.0), dTy = 60.0, dthetaz = 10.0, p0 = 100000.0;
-    const double lat = g.lonlat[S + p];
-    const double psm1 = LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
-    const double ps = 0.5 * (LV(f.p, 0) + psm1);
+    double ps = 0.0, lat = 0.0;
+    if (valid) {
+        lat = g.lonlat[S + p];
+        const double psm1 = LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
+        ps = 0.5 * (LV(f.p, 0) + psm1);
+    }
+    __syncthreads();
+    if (!valid) return;
     bool nan_t = false;
-    for (int lev = 0; lev < nv; lev++) {
+    for (int lev = row; lev < nv; lev += NROW) {
         {
             const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
             const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
