@@ -78,6 +78,7 @@ def main():
     ap.add_argument("--dt", type=float, default=300.0)
     ap.add_argument("--substeps", type=int, default=8)
     ap.add_argument("--variant", default="", help='"" production build, "pm" parity build')
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (three times); "
@@ -90,13 +91,16 @@ def main():
 
     import thor_amd.capi as capi
 
+    mixed = args.precision == "mixed"
+
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     os.makedirs(CACHE, exist_ok=True)
     t0 = time.perf_counter()
     g = capi.Grid(args.glevel, args.nv, 6.371e6, 36000.0, spring=True, beta=1.15, cache_dir=CACHE)
     log(f"grid g{args.glevel} L{args.nv}: P={g.P} built in {time.perf_counter() - t0:.1f}s")
-    p = capi.default_params(glevel=args.glevel, timestep=args.dt, n_substeps=args.substeps)
+    p = capi.default_params(glevel=args.glevel, timestep=args.dt, n_substeps=args.substeps,
+                            precision=1 if args.precision == "mixed" else 0)
     st = capi.ic_rest(g, p, seed=1002, temperature_noise=1e-3)
 
     d = capi.Dyn(g, p, variant=args.variant)
@@ -138,7 +142,7 @@ def main():
     roofline = {
         "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(dom["name"], f"g{args.glevel}L{args.nv}"),
+        "traffic": measured_traffic(dom["name"], f"g{args.glevel}L{args.nv}" + ("_mixed" if mixed else "")),
         "alg_bytes_per_launch": round(dom["bytes_per_launch"]),
         "avg_launch_us": round(dom_avg_ms * 1e3, 2),
         "step": {"alg_bytes": round(step_bytes), "achieved": round(step_gbs, 1),
@@ -170,12 +174,15 @@ def main():
     out = {
         "metric": METRIC, "value": round(sdpd, 1), "unit": "sim-days/wall-day", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (rest state + seeded 1e-3 K noise)",
         "config": {"workload": f"Held-Suarez Earth g{args.glevel} L{args.nv} ({g.P} columns), dt={args.dt:g} s, "
-                               f"{args.substeps} acoustic substeps, HyDiff 4 + DivDamp, FP64",
+                               f"{args.substeps} acoustic substeps, HyDiff 4 + DivDamp, "
+                               + ("mixed precision (FP32 damping operators and momentum, FP64 rest)" if mixed
+                                  else "FP64"),
                    "glevel": args.glevel, "nv": args.nv, "dt": args.dt, "n_substeps": args.substeps,
-                   "precision": "fp64", "build": args.variant or "production"},
+                   "precision": args.precision, "build": args.variant or "production"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,0 +1,108 @@
+"""Mixed-precision path: operator weights and one step against FP64.
+
+The mixed path applies the hyperdiffusion Laplacian and the divergence-damping
+operator as FP32 per-point weights derived on the GPU from the FP64 operators.
+Applied in FP64 with numpy, those weights must reproduce the oracle's own
+kernels (which are pinned bit for bit to THOR's Diffusion_Op and DivM_Op) to
+the weights' FP32 rounding. Then a full step in mixed precision must stay
+within FP32-sized distances of the FP64 step.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from common import earth_params, grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _ring(g):
+    """Stencil slots (0 = self, 1..6 = ring) per point; a pentagon's or pole's
+    sixth slot repeats the fifth (its weight is zero)."""
+    P = g.P
+    ring = np.asarray(g.point_local).reshape(P, 6).copy()
+    for p in g.pent_ind:
+        ring[p, 5] = ring[p, 4]
+    return np.concatenate([np.arange(P)[:, None], ring], axis=1)
+
+
+@pytest.fixture(scope="module")
+def setup(built):
+    capi = built
+    g = grid(capi, 4, 16)
+    p = earth_params(capi, 4, dt=600.0, precision=1)
+    d = capi.Dyn(g, p)
+    w = {n: d.get_field(n) for n in ("wlap", "wdd", "udd")}
+    d.close()
+    return capi, g, p, w
+
+
+def test_laplacian_weights_reproduce_oracle_diffusion(setup):
+    from oracle.oracle import Oracle
+    capi, g, p, w = setup
+    P, nv = g.P, g.nv
+    rng = np.random.default_rng(11)
+    rho = 1.0 + 0.1 * rng.standard_normal((P, nv))
+    o = Oracle(g, p)
+    o.set("Rhok", rho)
+    o.kernel("diffusion", 0, 0)
+    want = o.get("diff").reshape(P, nv, 6)[:, :, 0]  # var 0: Laplacian of rho
+    slots = _ring(g)
+    wl = w["wlap"].reshape(P, 7)
+    rs = p.A / (p.A + np.asarray(g.Altitude))
+    got = np.einsum("ps,psl->pl", wl[:, 1:], rho[slots[:, 1:], :] - rho[:, None, :]) * rs[None, :] ** 2
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"Laplacian from weights vs oracle: max rel diff {err:.2e}")
+    assert err < 2e-6
+
+
+def test_divergence_damping_weights_reproduce_oracle_divm(setup):
+    from oracle.oracle import Oracle
+    capi, g, p, w = setup
+    P, nv = g.P, g.nv
+    rng = np.random.default_rng(12)
+    M = rng.standard_normal((P, nv, 3))
+    Wh = 0.1 * rng.standard_normal((P, nv + 1))
+    o = Oracle(g, p)
+    o.set("Mhk", M)
+    o.set("Whk", Wh)
+    o.kernel("divm", 0, 0)
+    want = o.get("divg_Mh").reshape(P, nv, 3)
+    slots = _ring(g)
+    W = w["wdd"].reshape(P, 3, 7, 3)
+    U = w["udd"].reshape(P, 3, 7)
+    alt, alth = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    rs = p.A / (p.A + alt)
+    dz = 1.0 / (alth[:-1] - alth[1:])
+    Ms = M[slots]                                  # [P, 7, nv, 3]
+    dw = (Wh[:, :-1] - Wh[:, 1:])[slots]           # [P, 7, nv]
+    got = np.einsum("pcsk,pslk->plc", W, Ms) * (rs ** 2)[None, :, None] \
+        + np.einsum("pcs,psl->plc", U, dw) * (rs * dz)[None, :, None]
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"divergence damping from weights vs oracle: max rel diff {err:.2e}")
+    assert err < 2e-6
+
+
+def test_mixed_step_stays_close_to_fp64(setup):
+    capi, g, p64, _ = setup
+    st = capi.ic_rest(g, p64, seed=1001, temperature_noise=1e-3, wind_noise=1.0)
+    out = {}
+    for prec in (0, 1):
+        p = earth_params(capi, 4, dt=600.0, precision=prec)
+        d = capi.Dyn(g, p)
+        d.set_state(st)
+        d.run(3)
+        d.sync()
+        out[prec] = d.get_state()
+        d.close()
+    # FP32 storage of the momentum deviation and FP32 damping operators move
+    # the state by FP32-sized amounts; density and pressure stay FP64-exact
+    # to that level too.
+    tol = {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}
+    for k, t in tol.items():
+        a, b = out[1][k], out[0][k]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"3 steps mixed vs FP64 {k}: max rel diff {err:.2e}")
+        assert np.isfinite(a).all()
+        assert err < t, k

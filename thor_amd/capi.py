@@ -195,6 +195,8 @@ FIELD_SHAPES = {
     "Mh": "v", "W": "s", "Rho": "s", "pressure": "s", "Wh": "i", "temperature": "s", "pt": "s",
     "h": "s", "hh": "i", "pth": "i", "gtil": "s", "gtilh": "i", "profx_Qheat": "s",
     "profx_dMh": "v", "profx_dWh": "i",
+    # mixed precision only: operator weights per point (see Geo in dyn_common.h)
+    "wlap": 7, "wdd": 63, "udd": 21,
 }
 
 
@@ -227,6 +229,8 @@ class Dyn:
 
     def _shape(self, kind):
         P, nv = self.grid.P, self.grid.nv
+        if isinstance(kind, int):
+            return (P, kind)
         return {"s": (P, nv), "i": (P, nv + 1), "v": (P, nv, 3)}[kind]
 
     def set_state(self, st: dict):

@@ -55,6 +55,7 @@ struct Dyn {
     std::map<std::string, KernelStat> stats;
     std::vector<std::string> stat_order;
     double diff_sign = 1.0;
+    bool mixed = false;  // THOR_PRECISION_MIXED
 };
 
 static hipError_t dalloc(Dyn &d, double **ptr, size_t n) {
@@ -62,6 +63,15 @@ static hipError_t dalloc(Dyn &d, double **ptr, size_t n) {
     if (e == hipSuccess) {
         d.allocs.push_back(*ptr);
         e = hipMemset(*ptr, 0, n * sizeof(double));
+    }
+    return e;
+}
+
+static hipError_t falloc(Dyn &d, float **ptr, size_t n) {
+    hipError_t e = hipMalloc((void **)ptr, n * sizeof(float));
+    if (e == hipSuccess) {
+        d.allocs.push_back(*ptr);
+        e = hipMemset(*ptr, 0, n * sizeof(float));
     }
     return e;
 }
@@ -149,7 +159,7 @@ static Fields stage_fields(Dyn &d, int rk) {
 // Prepare_Implicit_Vertical 10, Vertical_Eq 12, Density_Pressure_Eqs 22,
 // ProfX (Held-Suarez) 24. The per-stage Thomas factorisation has no reference
 // counterpart and books its own traffic (hh, gtilh in; three planes out).
-static int thor_step(Dyn &d) {
+template <bool MX> static int thor_step_t(Dyn &d) {
     const double N = (double)d.P * d.nv, B = 8.0 * N;
     const int ns_total = d.prm.n_substeps;
     const int hydiff = d.prm.HyDiff, divdamp = d.prm.DivDampP;
@@ -165,15 +175,16 @@ static int thor_step(Dyn &d) {
         const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
-        launch(d, "column_prep", (12 + 11) * B, [&] { k_column_prep<<<grid_c, blk, 0, st>>>(g, f); });
-        launch(d, "adv_diff1", (11 + (hydiff ? 13 : 0)) * B, [&] { k_adv_diff1<<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
+        launch(d, "column_prep", (12 + 11) * B, [&] { k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f); });
+        launch(d, "adv_diff1", (11 + (hydiff ? 13 : 0)) * B,
+               [&] { k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
         if (hydiff || divdamp)
             launch(d, "diff2_divm0", ((hydiff ? 13 + 12 : 0) + (divdamp ? 7 : 0)) * B, [&] {
-                k_diff2_divm0<<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
+                k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
             });
         launch(d, "slow_modes", ((divdamp ? 6 : 0) + 40 + (rk ? 19 : 0)) * B,
-               [&] { k_slow<<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
-        if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<<<grid_t2, blk, 0, st>>>(g, f); });
+               [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
+        if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f); });
         launch(d, "thomas_factor", 5 * B, [&] {
             if (cb == 32) k_thomas_factor<32><<<grid_t, 256, lds_t, st>>>(g, f, times);
             else k_thomas_factor<16><<<grid_t, 256, lds_t, st>>>(g, f, times);
@@ -181,16 +192,16 @@ static int thor_step(Dyn &d) {
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
             launch(d, "momentum", ((divdamp ? 6 : 0) + 13) * B,
-                   [&] { k_momentum<<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
-            launch(d, "prepare", 10 * B, [&] { k_prepare<<<grid_t2, blk, 0, st>>>(g, f); });
+                   [&] { k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
+            launch(d, "prepare", 10 * B, [&] { k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f); });
             launch(d, "thomas_solve", 12 * B, [&] {
                 if (cb == 32) k_thomas_solve<32><<<grid_t, 256, lds_t, st>>>(g, f, times);
                 else k_thomas_solve<16><<<grid_t, 256, lds_t, st>>>(g, f, times);
             });
             launch(d, "density_pressure", (22 + (next ? 7 : 0)) * B,
-                   [&] { k_density_pressure<<<grid_t2, blk, 0, st>>>(g, f, times, next); });
+                   [&] { k_density_pressure<MX><<<grid_t2, blk, 0, st>>>(g, f, times, next); });
         }
-        launch(d, "update_rk2", 19 * B, [&] { k_update_rk2<kLC><<<grid_s, blk, 0, st>>>(g, f); });
+        launch(d, "update_rk2", 19 * B, [&] { k_update_rk2<kLC, MX><<<grid_s, blk, 0, st>>>(g, f); });
     }
     std::swap(d.Mh, d.Mhk);
     std::swap(d.Wh, d.WhkA);
@@ -201,6 +212,8 @@ static int thor_step(Dyn &d) {
     if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("thor step launch: ") + hipGetErrorString(e));
     return THOR_OK;
 }
+
+static int thor_step(Dyn &d) { return d.mixed ? thor_step_t<true>(d) : thor_step_t<false>(d); }
 
 static int profx(Dyn &d, int forcing) {
     if (forcing == THOR_FORCING_NONE) return THOR_OK;
@@ -284,8 +297,8 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
                                           "intermediate hyperdiffusion passes race, thor_diff.h:214-217)");
     if (prm->n_substeps < 2 || prm->n_substeps % 2)
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: n_substeps must be even and >= 2");
-    if (prm->precision != THOR_PRECISION_FP64)
-        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: precision mode not built");
+    if (prm->precision != THOR_PRECISION_FP64 && prm->precision != THOR_PRECISION_MIXED)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: unknown precision mode");
     if (gd->nv < 3 || gd->point_num < 12) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: grid too small");
     if (4 * (gd->nv + 1) * kThomasCB(gd->nv) * 8 > 65536)
         return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: at most 127 levels (the vertical solve stages 16 "
@@ -296,6 +309,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     thor_dyn *h = new thor_dyn();
     Dyn &d = *h;
     d.prm = *prm;
+    d.mixed = prm->precision == THOR_PRECISION_MIXED;
     d.P = gd->point_num;
     d.nv = gd->nv;
     d.S = (d.P + kAlign - 1) / kAlign * kAlign;
@@ -408,6 +422,27 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         }
     }
     if (e != hipSuccess) return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: alloc: ") + hipGetErrorString(e)));
+    if (d.mixed) {
+        auto F = [&](float **ptr, size_t n) {
+            if (e == hipSuccess) e = falloc(d, ptr, n);
+        };
+        F(&f.mMhs, n3); F(&f.mAdv, n3); F(&f.mdiff, 6 * n1); F(&f.mdiffmh, n3);
+        F(&f.mdiffw, n1); F(&f.mdiffrh, n1); F(&f.mdiffpr, n1); F(&f.mdivg, n3); F(&f.mSlowMh, n3);
+        float *wlap = nullptr, *wdd = nullptr, *udd = nullptr;
+        F(&wlap, (size_t)7 * S); F(&wdd, (size_t)63 * S); F(&udd, (size_t)21 * S);
+        if (e == hipSuccess) {
+            Geo gu = g;
+            gu.deep = 0;  // rscale = 1
+            k_probe_weights<<<(P + 255) / 256, 256, 0, d.stream>>>(gu, wlap, wdd, udd);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipStreamSynchronize(d.stream);
+            g.wlap = wlap;
+            g.wdd = wdd;
+            g.udd = udd;
+        }
+        if (e != hipSuccess)
+            return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: mixed-precision set-up: ") + hipGetErrorString(e)));
+    }
     *out = h;
     return THOR_OK;
 }
@@ -501,6 +536,25 @@ extern "C" thor_status thor_dyn_globdiag(thor_dyn *h, double out[5]) {
 extern "C" thor_status thor_dyn_get_field(thor_dyn *h, const char *name, double *dst, long n) {
     using namespace thor;
     if (!h || !name || !dst) return fail(THOR_ERR_BAD_ARG, "get_field: null argument");
+    {
+        // mixed-precision operator weights, [P][ncoef] per point
+        const std::string n_(name);
+        const float *w = nullptr;
+        int nc = 0;
+        if (n_ == "wlap") { w = h->g.wlap; nc = 7; }
+        else if (n_ == "wdd") { w = h->g.wdd; nc = 63; }
+        else if (n_ == "udd") { w = h->g.udd; nc = 21; }
+        if (nc) {
+            if (!h->mixed) return fail(THOR_ERR_BAD_ARG, "get_field: operator weights exist in mixed precision only");
+            if (n != (long)h->P * nc) return fail(THOR_ERR_BAD_ARG, "get_field: size mismatch");
+            std::vector<float> tmp((size_t)nc * h->S);
+            HIP_TRY(hipStreamSynchronize(h->stream));
+            HIP_TRY(hipMemcpy(tmp.data(), w, tmp.size() * sizeof(float), hipMemcpyDeviceToHost));
+            for (int p = 0; p < h->P; p++)
+                for (int c = 0; c < nc; c++) dst[(size_t)p * nc + c] = tmp[(size_t)c * h->S + p];
+            return THOR_OK;
+        }
+    }
     double *dev;
     int nlev, ncomp;
     if (!field_ptr(*h, name, dev, nlev, ncomp)) return fail(THOR_ERR_BAD_ARG, std::string("get_field: unknown ") + name);

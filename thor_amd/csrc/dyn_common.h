@@ -31,6 +31,15 @@ struct Geo {
     const double *lonlat;                      // [2][S]
     const double *areasT;                      // [S]
     const double *Alt, *Alth, *Kdh4, *Kdhz;    // [nv], [nv+1], [nv], [nv]
+    // Mixed precision only: the hyperdiffusion Laplacian and the divergence-
+    // damping operator as per-point weights on the 7 stencil slots (0 = self,
+    // 1..6 = ring), for rscale = 1:
+    //   L(a)       = rs^2 sum_s wlap[s] (a_s - a_0)
+    //   DivM(a)_c  = rs^2 sum_{s,k} wdd[c*21 + s*3 + k] a_{s,k}
+    //                + rs dz sum_s udd[c*7 + s] (wl_s - wt_s)
+    const float *wlap;  // [7][S]
+    const float *wdd;   // [63][S]
+    const float *udd;   // [21][S]
     double A, Omega, Gravit, P_Ref, Rd, Cp;
     int deep, nonhydro, ghv;
 };
@@ -55,6 +64,38 @@ struct Fields {
     double *tAA, *tCC, *tTT;              // Thomas factorisation of the stage [nv]
     double *Qheat, *dMh, *dWh;            // ProfX tendencies
     unsigned int *flags;                  // bit 0: NaN pressure, bit 1: NaN temperature
+    // Mixed precision: FP32 storage of the momentum deviation, the advection,
+    // hyperdiffusion and divergence-damping tendencies and the slow momentum
+    // tendency (same layouts as their FP64 counterparts above).
+    float *mMhs, *mAdv, *mdiff, *mdiffmh, *mdiffw, *mdiffrh, *mdiffpr, *mdivg, *mSlowMh;
+};
+
+// Storage of the mode-dependent fields: FP64 in the FP64 path, FP32 in the
+// mixed-precision path.
+template <bool MX> struct FS;
+template <> struct FS<false> {
+    using R = double;
+    static __host__ __device__ double *Mhs(const Fields &f) { return f.Mhs; }
+    static __host__ __device__ double *Adv(const Fields &f) { return f.Adv; }
+    static __host__ __device__ double *diff(const Fields &f) { return f.diff; }
+    static __host__ __device__ double *diffmh(const Fields &f) { return f.diffmh; }
+    static __host__ __device__ double *diffw(const Fields &f) { return f.diffw; }
+    static __host__ __device__ double *diffrh(const Fields &f) { return f.diffrh; }
+    static __host__ __device__ double *diffpr(const Fields &f) { return f.diffpr; }
+    static __host__ __device__ double *divg(const Fields &f) { return f.divg; }
+    static __host__ __device__ double *SlowMh(const Fields &f) { return f.SlowMh; }
+};
+template <> struct FS<true> {
+    using R = float;
+    static __host__ __device__ float *Mhs(const Fields &f) { return f.mMhs; }
+    static __host__ __device__ float *Adv(const Fields &f) { return f.mAdv; }
+    static __host__ __device__ float *diff(const Fields &f) { return f.mdiff; }
+    static __host__ __device__ float *diffmh(const Fields &f) { return f.mdiffmh; }
+    static __host__ __device__ float *diffw(const Fields &f) { return f.mdiffw; }
+    static __host__ __device__ float *diffrh(const Fields &f) { return f.mdiffrh; }
+    static __host__ __device__ float *diffpr(const Fields &f) { return f.mdiffpr; }
+    static __host__ __device__ float *divg(const Fields &f) { return f.mdivg; }
+    static __host__ __device__ float *SlowMh(const Fields &f) { return f.mSlowMh; }
 };
 
 enum : unsigned { FLAG_NAN_P = 1u, FLAG_NAN_T = 2u };

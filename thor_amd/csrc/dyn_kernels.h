@@ -299,6 +299,109 @@ __device__ __forceinline__ bool pole_ctx(const Geo &g, PoleCtx &c) {
     return true;
 }
 
+// ------------------------------------------------ mixed-precision operators
+// The hyperdiffusion Laplacian and the divergence-damping operator above are
+// linear in their inputs, with coefficients fixed by the geometry and the
+// level's rscale. The mixed-precision path applies them as FP32 per-point
+// weights (Geo::wlap, wdd, udd), which k_probe_weights derives from the FP64
+// operators themselves by feeding them unit inputs.
+
+// Laplacians of NV fields: rs^2 sum_{s>=1} wlap[s] (a_s - a_0). The weights
+// of a Laplacian sum to zero, so the differences form keeps FP32 precision
+// for fields with a large mean (R*T).
+template <int NV, class GetA>
+__device__ __forceinline__ void laplacians_w(const Geo &g, int p, float rs2, GetA a, float out[NV]) {
+    const size_t S = g.S;
+    float w[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) w[s] = g.wlap[(size_t)(s + 1) * S + p];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const float a0 = a(v, 0);
+        float sum = 0.f;
+#pragma unroll
+        for (int s = 0; s < 6; s++) sum += w[s] * (a(v, s + 1) - a0);
+        out[v] = rs2 * sum;
+    }
+}
+
+// Divergence-damping pass (DivM_Op): pass 0 (LAST = false) from the momentum
+// a and the vertical-velocity differences dw(s) = wl(s) - wt(s); pass 1
+// (LAST = true) from pass 0's output, with the radial part removed and
+// -Kdhz applied as in the FP64 operator.
+template <bool LAST, class GetA, class GetDW>
+__device__ __forceinline__ void divm_w(const Geo &g, int p, int lev, GetA a, GetDW dw, float out[3]) {
+    const size_t S = g.S;
+    const double rsd = rscale_at(g, lev);
+    const float rs = (float)rsd, rs2 = (float)(rsd * rsd);
+    float av[7][3];
+#pragma unroll
+    for (int s = 0; s < 7; s++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) av[s][k] = a(s, k);
+    float d[7];
+    if (!LAST) {
+#pragma unroll
+        for (int s = 0; s < 7; s++) d[s] = dw(s);
+    }
+    const float dz = LAST ? 0.f : (float)(1.0 / (g.Alth[lev] - g.Alth[lev + 1]));
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float sum = 0.f;
+#pragma unroll
+        for (int s = 0; s < 7; s++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) sum += g.wdd[(size_t)(c * 21 + s * 3 + k) * S + p] * av[s][k];
+        out[c] = rs2 * sum;
+        if (!LAST) {
+            float su = 0.f;
+#pragma unroll
+            for (int s = 0; s < 7; s++) su += g.udd[(size_t)(c * 7 + s) * S + p] * d[s];
+            out[c] += rs * dz * su;
+        }
+    }
+    if (LAST) {
+        const float sdiff = (float)g.Kdhz[lev];
+        const float fx = (float)g.fr[p], fy = (float)g.fr[S + p], fz = (float)g.fr[2 * S + p];
+        const float lapr = fx * out[0] + fy * out[1] + fz * out[2];
+        out[0] = -(out[0] - fx * lapr) * sdiff;
+        out[1] = -(out[1] - fy * lapr) * sdiff;
+        out[2] = -(out[2] - fz * lapr) * sdiff;
+    }
+}
+
+// Derives the mixed-precision weights from the FP64 operators: for every
+// point, the Laplacian of the 7 unit slot fields and the divergence-damping
+// pass 0 of the 21 unit momentum fields and the 7 unit vertical-velocity
+// fields, all at rscale = 1 (gu is the grid with DeepModel off) and, for the
+// vertical part, divided by level 0's 1/dz. One thread per point.
+__global__ __launch_bounds__(256) void k_probe_weights(Geo gu, float *wlap, float *wdd, float *udd) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= gu.P) return;
+    const size_t S = gu.S;
+    const bool pole = p >= gu.P - 2;
+    const int nf = pole ? 5 : gu.nfc[p];
+    {
+        double out[7];
+        laplacians<7>(gu, p, nf, 1.0, [&](int v, int s) { return v == s ? 1.0 : 0.0; }, out);
+        for (int s = 0; s < 7; s++) wlap[(size_t)s * S + p] = (float)out[s];
+    }
+    auto zero = [&](int) { return 0.0; };
+    for (int s0 = 0; s0 < 7; s0++) {
+        for (int k0 = 0; k0 < 3; k0++) {
+            double o3[3];
+            divm_nf<false>(
+                gu, p, 0, nf, pole, [&](int s, int k) { return (s == s0 && k == k0) ? 1.0 : 0.0; }, zero, zero, o3);
+            for (int c = 0; c < 3; c++) wdd[(size_t)(c * 21 + s0 * 3 + k0) * S + p] = (float)o3[c];
+        }
+        double o3[3];
+        divm_nf<false>(
+            gu, p, 0, nf, pole, [&](int, int) { return 0.0; }, [&](int s) { return s == s0 ? 1.0 : 0.0; }, zero, o3);
+        const double dz0 = 1.0 / (gu.Alth[0] - gu.Alth[1]);
+        for (int c = 0; c < 3; c++) udd[(size_t)(c * 7 + s0) * S + p] = (float)(o3[c] / dz0);
+    }
+}
+
 // ===================================================================== K_A
 // Compute_Temperature_H_Pt_Geff (thor_auxiliary.h:55-309, entropy form) and
 // the vertical advection + Coriolis part of Compute_Advec_Cori2
@@ -307,7 +410,7 @@ __device__ __forceinline__ bool pole_ctx(const Geo &g, PoleCtx &c) {
 // thread takes one (column, level) pair, lev in [0, nv], and recomputes the
 // values the sweep carries from the level below from their own inputs, so
 // every output is the same number the sweep produces.
-__global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
+template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     const int lev = blockIdx.y;  // 0..nv: centre lev (if < nv) and interface lev
     if (p >= g.P) return;
@@ -459,9 +562,10 @@ __global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
         Cy = -2 * Omega * (vc[2] * fr[2] * fr[0] - vc[0] * MPOW(fr[2], 2)) * rho;
         Cz = 2 * Omega * (vc[1] * fr[2] * fr[0] - vc[0] * fr[2] * fr[1]) * rho;
     }
-    VEC(f.Adv, 0, lev, p) = dav[0] + Cx;
-    VEC(f.Adv, 1, lev, p) = dav[1] + Cy;
-    VEC(f.Adv, 2, lev, p) = dav[2] + Cz;
+    auto *Adv = FS<MX>::Adv(f);
+    VEC(Adv, 0, lev, p) = dav[0] + Cx;
+    VEC(Adv, 1, lev, p) = dav[1] + Cy;
+    VEC(Adv, 2, lev, p) = dav[2] + Cz;
 }
 
 // ===================================================================== K_B
@@ -469,7 +573,7 @@ __global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
 // added to the vertical part, and the first Laplacian pass of the 4th-order
 // hyperdiffusion (Diffusion_Op stepnum 0, thor_diff.h:54-522) of rho, u, v,
 // w = M/rho, W/rho and Rd*T.
-template <class GetM, class GetV, class GetR, class GetW, class GetT>
+template <bool MX, class GetM, class GetV, class GetR, class GetW, class GetT>
 __device__ __forceinline__ void adv_diff1_point(const Geo &g, const Fields &f, int p, int lev, int nf, int hydiff,
                                                 GetM M, GetV v, GetR rho, GetW W, GetT T) {
     const size_t S = g.S;
@@ -486,8 +590,9 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
             nf3[c] += rs * sum;
         }
     }
+    auto *Adv = FS<MX>::Adv(f);
 #pragma unroll
-    for (int c = 0; c < 3; c++) VEC(f.Adv, c, lev, p) = nf3[c] + VEC(f.Adv, c, lev, p);
+    for (int c = 0; c < 3; c++) VEC(Adv, c, lev, p) = nf3[c] + VEC(Adv, c, lev, p);
     if (!hydiff) return;
     const double Rd = g.Rd;
     auto a = [&](int var, int s) -> double {
@@ -496,13 +601,21 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         if (var == 4) return W(s) / rho(s);
         return Rd * T(s);
     };
-    double lap[6];
-    laplacians<6>(g, p, nf, rs, a, lap);
+    auto *diff = FS<MX>::diff(f);
+    if constexpr (MX) {
+        float lap[6];
+        laplacians_w<6>(g, p, (float)(rs * rs), [&](int var, int s) { return (float)a(var, s); }, lap);
 #pragma unroll
-    for (int var = 0; var < 6; var++) VEC(f.diff, var, lev, p) = lap[var];
+        for (int var = 0; var < 6; var++) VEC(diff, var, lev, p) = lap[var];
+    } else {
+        double lap[6];
+        laplacians<6>(g, p, nf, rs, a, lap);
+#pragma unroll
+        for (int var = 0; var < 6; var++) VEC(diff, var, lev, p) = lap[var];
+    }
 }
 
-__global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, int hydiff) {
+template <bool MX> __global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, int hydiff) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
@@ -514,7 +627,7 @@ __global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, int hydiff) {
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        adv_diff1_point(
+        adv_diff1_point<MX>(
             g, f, c.p, lev, 5, hydiff, [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); },
             [&](int s, int k) { return vel(c.gi(s), k); }, [&](int s) { return LVP(f.Rhok, lev, c.gi(s)); },
             [&](int s) { return LVP(f.Wk, lev, c.gi(s)); }, [&](int s) { return LVP(f.T, lev, c.gi(s)); });
@@ -540,7 +653,7 @@ __global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, int hydiff) {
         sT[pos] = LVP(f.T, lev, q);
     });
     __syncthreads();
-    adv_diff1_point(
+    adv_diff1_point<MX>(
         g, f, T.p, lev, T.nf, hydiff, [&](int s, int k) { return sM[k][T.pos(s)]; },
         [&](int s, int k) { return sV[k][T.pos(s)]; }, [&](int s) { return sR[T.pos(s)]; },
         [&](int s) { return sW[T.pos(s)]; }, [&](int s) { return sT[T.pos(s)]; });
@@ -550,7 +663,7 @@ __global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, int hydiff) {
 // Last hyperdiffusion pass (Diffusion_Op laststep) + Correct_Horizontal
 // (thor_diff.h:882-915), and divergence-damping pass 0 on Mhk/Whk
 // (DivM_Op laststep = false).
-template <class GetD, class GetS, class GetM, class GetWl, class GetWt>
+template <bool MX, class GetD, class GetS, class GetM, class GetWl, class GetWt>
 __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
                                                   int hydiff, int divdamp, double sign, GetD dif, GetS rho, GetM M,
                                                   GetWl wl, GetWt wt) {
@@ -560,27 +673,47 @@ __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f,
         const double rs = rscale_at(g, lev);
         const double sdiff = -g.Kdh4[lev] * sign;
         double out[6];
-        auto a = [&](int var, int s) -> double { return var == 0 ? dif(s, 0) : dif(s, var) * rho(s); };
-        laplacians<6>(g, p, nf, rs, a, out);
+        if constexpr (MX) {
+            float o[6];
+            laplacians_w<6>(g, p, (float)(rs * rs),
+                            [&](int var, int s) { return var == 0 ? (float)dif(s, 0) : (float)(dif(s, var) * rho(s)); },
+                            o);
+#pragma unroll
+            for (int var = 0; var < 6; var++) out[var] = o[var];
+        } else {
+            auto a = [&](int var, int s) -> double { return var == 0 ? dif(s, 0) : dif(s, var) * rho(s); };
+            laplacians<6>(g, p, nf, rs, a, out);
+        }
 #pragma unroll
         for (int var = 0; var < 6; var++) out[var] = pole ? sdiff * out[var] : out[var] * sdiff;
-        LV(f.diffrh, lev) = out[0];
-        LV(f.diffw, lev) = out[4];
-        LV(f.diffpr, lev) = out[5];
+        LV(FS<MX>::diffrh(f), lev) = out[0];
+        LV(FS<MX>::diffw(f), lev) = out[4];
+        LV(FS<MX>::diffpr(f), lev) = out[5];
         const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
         const double dmhr = fx * out[1] + fy * out[2] + fz * out[3];
-        VEC(f.diffmh, 0, lev, p) = out[1] + -fx * dmhr;
-        VEC(f.diffmh, 1, lev, p) = out[2] + -fy * dmhr;
-        VEC(f.diffmh, 2, lev, p) = out[3] + -fz * dmhr;
+        auto *diffmh = FS<MX>::diffmh(f);
+        VEC(diffmh, 0, lev, p) = out[1] + -fx * dmhr;
+        VEC(diffmh, 1, lev, p) = out[2] + -fy * dmhr;
+        VEC(diffmh, 2, lev, p) = out[3] + -fz * dmhr;
     }
     if (divdamp) {
-        double o3[3];
-        divm_nf<false>(g, p, lev, nf, pole, M, wl, wt, o3);
+        auto *divg = FS<MX>::divg(f);
+        if constexpr (MX) {
+            float o3[3];
+            divm_w<false>(g, p, lev, [&](int s, int k) { return (float)M(s, k); },
+                          [&](int s) { return (float)(wl(s) - wt(s)); }, o3);
 #pragma unroll
-        for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
+        } else {
+            double o3[3];
+            divm_nf<false>(g, p, lev, nf, pole, M, wl, wt, o3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
+        }
     }
 }
 
+template <bool MX>
 __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int divdamp, double sign) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
@@ -589,16 +722,20 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int di
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        diff2_divm0_point(
+        auto *diff = FS<MX>::diff(f);
+        diff2_divm0_point<MX>(
             g, f, c.p, lev, 5, true, hydiff, divdamp, sign,
-            [&](int s, int var) { return VEC(f.diff, var, lev, c.gi(s)); },
+            [&](int s, int var) { return VEC(diff, var, lev, c.gi(s)); },
             [&](int s) { return LVP(f.Rhok, lev, c.gi(s)); },
             [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); },
             [&](int s) { return LVP(f.Whk, lev, c.gi(s)); }, [&](int s) { return LVP(f.Whk, lev + 1, c.gi(s)); });
         return;
     }
-    __shared__ double sD[6][NHL2], sR[NHL2], sM[3][NHL2], sWl[NHL2], sWt[NHL2];
+    using R = typename FS<MX>::R;
+    __shared__ R sD[6][NHL2];
+    __shared__ double sR[NHL2], sM[3][NHL2], sWl[NHL2], sWt[NHL2];
     const TileThread T = tile_thread(g, r);
+    auto *diff = FS<MX>::diff(f);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
             for (int k = 0; k < 6; k++) sD[k][pos] = 0.0;
@@ -608,7 +745,7 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int di
         }
         if (hydiff) {
 #pragma unroll
-            for (int k = 0; k < 6; k++) sD[k][pos] = VEC(f.diff, k, lev, q);
+            for (int k = 0; k < 6; k++) sD[k][pos] = VEC(diff, k, lev, q);
             sR[pos] = LVP(f.Rhok, lev, q);
         }
         if (divdamp) {
@@ -619,7 +756,7 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int di
         }
     });
     __syncthreads();
-    diff2_divm0_point(
+    diff2_divm0_point<MX>(
         g, f, T.p, lev, T.nf, false, hydiff, divdamp, sign, [&](int s, int var) { return sD[var][T.pos(s)]; },
         [&](int s) { return sR[T.pos(s)]; }, [&](int s, int k) { return sM[k][T.pos(s)]; },
         [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; });
@@ -630,7 +767,7 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int di
 // (thor_slowmodes.h:48-683, entropy form) + UpdateRK (thor_auxiliary.h:311-418)
 // for rk > 0, or the zero start of the deviations for rk = 0
 // (thor_driver.cu:104-108).
-template <class GetG, class GetP, class GetH, class GetM>
+template <bool MX, class GetG, class GetP, class GetH, class GetM>
 __device__ __forceinline__ void slow_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
                                            int divdamp, int rk, GetG dg, GetP pv, GetH hv, GetM mv) {
     const size_t S = g.S;
@@ -640,14 +777,21 @@ __device__ __forceinline__ void slow_point(const Geo &g, const Fields &f, int p,
     const double *Alt = g.Alt, *Alth = g.Alth;
     double DivM[3] = {0.0, 0.0, 0.0};
     if (divdamp) {
-        auto none = [&](int) { return 0.0; };
-        divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
+        if constexpr (MX) {
+            float o3[3];
+            divm_w<true>(g, p, lev, dg, [](int) { return 0.f; }, o3);
+            for (int k = 0; k < 3; k++) DivM[k] = o3[k];
+        } else {
+            auto none = [&](int) { return 0.0; };
+            divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
+        }
     }
-    double advx = VEC(f.Adv, 0, lev, p), advy = VEC(f.Adv, 1, lev, p), advz = VEC(f.Adv, 2, lev, p);
+    const auto *Adv = FS<MX>::Adv(f);
+    double advx = VEC(Adv, 0, lev, p), advy = VEC(Adv, 1, lev, p), advz = VEC(Adv, 2, lev, p);
     const double rho = LV(f.Rhok, lev);
     double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
     if (lev != 0) {
-        const double ax = VEC(f.Adv, 0, lev - 1, p), ay = VEC(f.Adv, 1, lev - 1, p), az = VEC(f.Adv, 2, lev - 1, p);
+        const double ax = VEC(Adv, 0, lev - 1, p), ay = VEC(Adv, 1, lev - 1, p), az = VEC(Adv, 2, lev - 1, p);
         advrl = ax * fx + ay * fy + az * fz;
         rhol = LV(f.Rhok, lev - 1);
         altl = Alt[lev - 1];
@@ -713,10 +857,12 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     const double dwhdz = (wht * r2p * hht - whl * r2l * hhl) / (dz * r2m);
     const double dpr = nflxv[0] * fx + nflxv[1] * fy + nflxv[2] * fz;
     const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
+    auto *SlowMh = FS<MX>::SlowMh(f);
+    const auto *diffmh = FS<MX>::diffmh(f);
 #pragma unroll
     for (int k = 0; k < 3; k++)
-        VEC(f.SlowMh, k, lev, p) = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[k] + VEC(f.diffmh, k, lev, p) + 0.0
-                                   + VEC(f.dMh, k, lev, p);
+        VEC(SlowMh, k, lev, p) = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[k] + VEC(diffmh, k, lev, p) + 0.0
+                                 + VEC(f.dMh, k, lev, p);
     if (lev == 0) {
         LV(f.SlowWh, 0) = 0.0;
         LV(f.SlowWh, nv) = 0.0;
@@ -736,7 +882,8 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
             else
                 swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
         }
-        if (g.nonhydro) swr += (-LV(f.diffw, lev - 1) - 0.0) * intt + (-LV(f.diffw, lev) - 0.0) * intl;
+        const auto *diffw = FS<MX>::diffw(f);
+        if (g.nonhydro) swr += (-LV(diffw, lev - 1) - 0.0) * intt + (-LV(diffw, lev) - 0.0) * intl;
         const double dzl = alt - altl;
         const double dpdz = (pv(0) - pressurel) / dzl;
         swr += dpdz;
@@ -744,18 +891,19 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         LV(f.SlowWh, lev) = -swr;
     }
     nflxr += dwdz;
-    LV(f.SlowRho, lev) = -nflxr + LV(f.diffrh, lev) + 0.0;
+    LV(f.SlowRho, lev) = -nflxr + LV(FS<MX>::diffrh(f), lev) + 0.0;
     const double r = 1.0 / rho;
     const double vgp =
         (nflxv[0] - dpr * fx) * M0x * r + (nflxv[1] - dpr * fy) * M0y * r + (nflxv[2] - dpr * fz) * M0z * r;
     double sp = (Rd / Cv) * (-nflxp - LV(f.gtil, lev) - dwhdz + vgp + LV(f.Qheat, lev));
-    sp += LV(f.diffpr, lev) + 0.0;
+    sp += LV(FS<MX>::diffpr(f), lev) + 0.0;
     LV(f.Slowp, lev) = sp;
 
     // Deviations at the start of the stage.
+    auto *Mhs = FS<MX>::Mhs(f);
     if (rk == 0) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) VEC(f.Mhs, k, lev, p) = 0.0;
+        for (int k = 0; k < 3; k++) VEC(Mhs, k, lev, p) = 0.0;
         LV(f.Rhos, lev) = 0.0;
         LV(f.ps, lev) = 0.0;
         LV(f.Whs, lev) = 0.0;
@@ -763,9 +911,9 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     } else {
         const double Mx = VEC(f.Mh, 0, lev, p) - M0x, My = VEC(f.Mh, 1, lev, p) - M0y, Mz = VEC(f.Mh, 2, lev, p) - M0z;
         const double aux = Mx * fx + My * fy + Mz * fz;
-        VEC(f.Mhs, 0, lev, p) = Mx - aux * fx;
-        VEC(f.Mhs, 1, lev, p) = My - aux * fy;
-        VEC(f.Mhs, 2, lev, p) = Mz - aux * fz;
+        VEC(Mhs, 0, lev, p) = Mx - aux * fx;
+        VEC(Mhs, 1, lev, p) = My - aux * fy;
+        VEC(Mhs, 2, lev, p) = Mz - aux * fz;
         LV(f.Rhos, lev) = LV(f.Rho, lev) - rho;
         LV(f.ps, lev) = LV(f.p, lev) - pv(0);
         LV(f.Whs, lev) = lev == 0 ? 0.0 : LV(f.Wh, lev) - LV(f.Whk, lev);
@@ -773,7 +921,7 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     }
 }
 
-__global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
+template <bool MX> __global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
@@ -781,14 +929,18 @@ __global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        slow_point(
-            g, f, c.p, lev, 5, true, divdamp, rk, [&](int s, int k) { return VEC(f.divg, k, lev, c.gi(s)); },
+        const auto *divg = FS<MX>::divg(f);
+        slow_point<MX>(
+            g, f, c.p, lev, 5, true, divdamp, rk, [&](int s, int k) { return VEC(divg, k, lev, c.gi(s)); },
             [&](int s) { return LVP(f.pk, lev, c.gi(s)); }, [&](int s) { return LVP(f.h, lev, c.gi(s)); },
             [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); });
         return;
     }
-    __shared__ double sG[3][NHL2], sP[NHL2], sH[NHL2], sM[3][NHL2];
+    using R = typename FS<MX>::R;
+    __shared__ R sG[3][NHL2];
+    __shared__ double sP[NHL2], sH[NHL2], sM[3][NHL2];
     const TileThread T = tile_thread(g, r);
+    const auto *divg = FS<MX>::divg(f);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
             for (int k = 0; k < 3; k++) sG[k][pos] = sM[k][pos] = 0.0;
@@ -797,7 +949,7 @@ __global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
         }
         if (divdamp) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) sG[k][pos] = VEC(f.divg, k, lev, q);
+            for (int k = 0; k < 3; k++) sG[k][pos] = VEC(divg, k, lev, q);
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhk, k, lev, q);
@@ -805,7 +957,7 @@ __global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
         sH[pos] = LVP(f.h, lev, q);
     });
     __syncthreads();
-    slow_point(
+    slow_point<MX>(
         g, f, T.p, lev, T.nf, false, divdamp, rk, [&](int s, int k) { return sG[k][T.pos(s)]; },
         [&](int s) { return sP[T.pos(s)]; }, [&](int s) { return sH[T.pos(s)]; },
         [&](int s, int k) { return sM[k][T.pos(s)]; });
@@ -813,24 +965,37 @@ __global__ STENCIL_BOUNDS void k_slow(Geo g, Fields f, int divdamp, int rk) {
 
 // ===================================================================== K_E
 // Divergence-damping pass 0 on the deviations Mhs / Whs.
-__global__ STENCIL_BOUNDS void k_divm0(Geo g, Fields f) {
+template <bool MX> __global__ STENCIL_BOUNDS void k_divm0(Geo g, Fields f) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
     const int nv = g.nv;
-    double o3[3];
+    using R = typename FS<MX>::R;
+    const R *Mhs = FS<MX>::Mhs(f);
+    R *divg = FS<MX>::divg(f);
+    // pass 0 of one point from slot getters, written to divg
+    auto run = [&](int p, int nf, bool pole, auto a, auto wl, auto wt) {
+        if constexpr (MX) {
+            float o3[3];
+            divm_w<false>(g, p, lev, a, [&](int s) { return (float)(wl(s) - wt(s)); }, o3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
+        } else {
+            double o3[3];
+            divm_nf<false>(g, p, lev, nf, pole, a, wl, wt, o3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
+        }
+    };
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        divm_nf<false>(
-            g, c.p, lev, 5, true, [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)); },
-            [&](int s) { return LVP(f.Whs, lev, c.gi(s)); }, [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); }, o3);
-        const int p = c.p;
-#pragma unroll
-        for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
+        run(c.p, 5, true, [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.Whs, lev, c.gi(s)); }, [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); });
         return;
     }
-    __shared__ double sM[3][NHL2], sWl[NHL2], sWt[NHL2];
+    __shared__ R sM[3][NHL2];
+    __shared__ double sWl[NHL2], sWt[NHL2];
     const TileThread T = tile_thread(g, r);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
@@ -839,23 +1004,19 @@ __global__ STENCIL_BOUNDS void k_divm0(Geo g, Fields f) {
             return;
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhs, k, lev, q);
+        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(Mhs, k, lev, q);
         sWl[pos] = LVP(f.Whs, lev, q);
         sWt[pos] = LVP(f.Whs, lev + 1, q);
     });
     __syncthreads();
-    divm_nf<false>(
-        g, T.p, lev, T.nf, false, [&](int s, int k) { return sM[k][T.pos(s)]; }, [&](int s) { return sWl[T.pos(s)]; },
-        [&](int s) { return sWt[T.pos(s)]; }, o3);
-    const int p = T.p;
-#pragma unroll
-    for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
+    run(T.p, T.nf, false, [&](int s, int k) { return sM[k][T.pos(s)]; }, [&](int s) { return sWl[T.pos(s)]; },
+        [&](int s) { return sWt[T.pos(s)]; });
 }
 
 // ===================================================================== K_F
 // Divergence-damping pass 1 on the deviations + Momentum_Eq / _Poles
 // (thor_fastmodes.h:54-253).
-template <class GetG, class GetS>
+template <bool MX, class GetG, class GetS>
 __device__ __forceinline__ void momentum_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
                                                int divdamp, double dt, GetG dg, GetS pv) {
     const size_t S = g.S;
@@ -863,8 +1024,14 @@ __device__ __forceinline__ void momentum_point(const Geo &g, const Fields &f, in
     const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
     double DivM[3] = {0.0, 0.0, 0.0};
     if (divdamp) {
-        auto none = [&](int) { return 0.0; };
-        divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
+        if constexpr (MX) {
+            float o3[3];
+            divm_w<true>(g, p, lev, dg, [](int) { return 0.f; }, o3);
+            for (int k = 0; k < 3; k++) DivM[k] = o3[k];
+        } else {
+            auto none = [&](int) { return 0.0; };
+            divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
+        }
     }
     const double rs = rscale_at(g, lev);
     double nf3[3];
@@ -875,19 +1042,21 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         for (int s = 1; s < 7; s++) s0 = s0 + g.grad[(size_t)(s * 3 + k) * S + p] * pv(s);
         nf3[k] = rs * s0;
     }
-    double Mx = (-nf3[0] + VEC(f.SlowMh, 0, lev, p) + DivM[0]) * dt;
-    double My = (-nf3[1] + VEC(f.SlowMh, 1, lev, p) + DivM[1]) * dt;
-    double Mz = (-nf3[2] + VEC(f.SlowMh, 2, lev, p) + DivM[2]) * dt;
+    const auto *SlowMh = FS<MX>::SlowMh(f);
+    double Mx = (-nf3[0] + VEC(SlowMh, 0, lev, p) + DivM[0]) * dt;
+    double My = (-nf3[1] + VEC(SlowMh, 1, lev, p) + DivM[1]) * dt;
+    double Mz = (-nf3[2] + VEC(SlowMh, 2, lev, p) + DivM[2]) * dt;
     const double vr = Mx * fx + My * fy + Mz * fz;
     Mx += -vr * fx;
     My += -vr * fy;
     Mz += -vr * fz;
-    VEC(f.Mhs, 0, lev, p) += Mx;
-    VEC(f.Mhs, 1, lev, p) += My;
-    VEC(f.Mhs, 2, lev, p) += Mz;
+    auto *Mhs = FS<MX>::Mhs(f);
+    VEC(Mhs, 0, lev, p) += Mx;
+    VEC(Mhs, 1, lev, p) += My;
+    VEC(Mhs, 2, lev, p) += Mz;
 }
 
-__global__ STENCIL_BOUNDS void k_momentum(Geo g, Fields f, int divdamp, double dt) {
+template <bool MX> __global__ STENCIL_BOUNDS void k_momentum(Geo g, Fields f, int divdamp, double dt) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
@@ -895,13 +1064,17 @@ __global__ STENCIL_BOUNDS void k_momentum(Geo g, Fields f, int divdamp, double d
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        momentum_point(
-            g, f, c.p, lev, 5, true, divdamp, dt, [&](int s, int k) { return VEC(f.divg, k, lev, c.gi(s)); },
+        const auto *divg = FS<MX>::divg(f);
+        momentum_point<MX>(
+            g, f, c.p, lev, 5, true, divdamp, dt, [&](int s, int k) { return VEC(divg, k, lev, c.gi(s)); },
             [&](int s) { return LVP(f.ps, lev, c.gi(s)); });
         return;
     }
-    __shared__ double sG[3][NHL2], sP[NHL2];
+    using R = typename FS<MX>::R;
+    __shared__ R sG[3][NHL2];
+    __shared__ double sP[NHL2];
     const TileThread T = tile_thread(g, r);
+    const auto *divg = FS<MX>::divg(f);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
             for (int k = 0; k < 3; k++) sG[k][pos] = 0.0;
@@ -910,12 +1083,12 @@ __global__ STENCIL_BOUNDS void k_momentum(Geo g, Fields f, int divdamp, double d
         }
         if (divdamp) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) sG[k][pos] = VEC(f.divg, k, lev, q);
+            for (int k = 0; k < 3; k++) sG[k][pos] = VEC(divg, k, lev, q);
         }
         sP[pos] = LVP(f.ps, lev, q);
     });
     __syncthreads();
-    momentum_point(
+    momentum_point<MX>(
         g, f, T.p, lev, T.nf, false, divdamp, dt, [&](int s, int k) { return sG[k][T.pos(s)]; },
         [&](int s) { return sP[T.pos(s)]; });
 }
@@ -953,7 +1126,7 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     LV(f.Sd, lev) = nflxr + LV(f.SlowRho, lev);
 }
 
-__global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
+template <bool MX> __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
@@ -961,12 +1134,16 @@ __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        prepare_point(g, f, c.p, lev, [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)); },
+        const auto *Mhs = FS<MX>::Mhs(f);
+        prepare_point(g, f, c.p, lev, [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)); },
                       [&](int s) { return LVP(f.h, lev, c.gi(s)); });
         return;
     }
-    __shared__ double sM[3][NHL2], sH[NHL2];
+    using R = typename FS<MX>::R;
+    __shared__ R sM[3][NHL2];
+    __shared__ double sH[NHL2];
     const TileThread T = tile_thread(g, r);
+    const auto *Mhs = FS<MX>::Mhs(f);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
             for (int k = 0; k < 3; k++) sM[k][pos] = 0.0;
@@ -974,11 +1151,11 @@ __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
             return;
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(f.Mhs, k, lev, q);
+        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(Mhs, k, lev, q);
         sH[pos] = LVP(f.h, lev, q);
     });
     __syncthreads();
-    prepare_point(g, f, T.p, lev, [&](int s, int k) { return sM[k][T.pos(s)]; },
+    prepare_point(g, f, T.p, lev, [&](int s, int k) { return (double)sM[k][T.pos(s)]; },
                   [&](int s) { return sH[T.pos(s)]; });
 }
 
@@ -1154,7 +1331,7 @@ __global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double de
 // Density_Pressure_Eqs / _Poles (thor_fastmodes.h:255-847, entropy form) and,
 // unless this is the stage's last substep, divergence-damping pass 0 on the
 // updated deviations for the next substep.
-template <class GetM, class GetV1, class GetS>
+template <bool MX, class GetM, class GetV1, class GetS>
 __device__ __forceinline__ void dp_point(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
                                          GetM vv, GetV1 v1, GetS ptv) {
     const size_t S = g.S;
@@ -1214,36 +1391,49 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     const double ptc = (P_Ref / (Rd * r)) * MPOW((pss + pkv) / P_Ref, Cv / Cp);
     aux += ptc * r;
     const double pnew = P_Ref * MPOW(Rd * aux / P_Ref, Cp / Cv);
-    const double psn = pnew - pkv + (LV(f.diffpr, lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
+    const double psn = pnew - pkv + (LV(FS<MX>::diffpr(f), lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
     LV(f.ps, lev) = psn;
     if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
 }
 
+template <bool MX>
 __global__ STENCIL_BOUNDS void k_density_pressure(Geo g, Fields f, double dt, int next_divm) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
     const int nv = g.nv;
+    using R = typename FS<MX>::R;
+    const R *Mhs = FS<MX>::Mhs(f);
+    R *divg = FS<MX>::divg(f);
+    // next substep's divergence-damping pass 0 of one point, written to divg
+    auto next = [&](int p, int nf, bool pole, auto a, auto wl, auto wt) {
+        if constexpr (MX) {
+            float o3[3];
+            divm_w<false>(g, p, lev, a, [&](int s) { return (float)(wl(s) - wt(s)); }, o3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
+        } else {
+            double o3[3];
+            divm_nf<false>(g, p, lev, nf, pole, a, wl, wt, o3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
+        }
+    };
     if (r == g.nr) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
-        auto Ms = [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)); };
-        dp_point(
-            g, f, c.p, lev, true, dt, Ms,
-            [&](int s, int k) { return VEC(f.Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
+        auto Ms = [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)); };
+        dp_point<MX>(
+            g, f, c.p, lev, true, dt, [&](int s, int k) { return (double)Ms(s, k); },
+            [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
             [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
-        if (next_divm) {
-            double o3[3];
-            divm_nf<false>(
-                g, c.p, lev, 5, true, Ms, [&](int s) { return LVP(f.Whs, lev, c.gi(s)); },
-                [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); }, o3);
-            const int p = c.p;
-#pragma unroll
-            for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
-        }
+        if (next_divm)
+            next(c.p, 5, true, Ms, [&](int s) { return LVP(f.Whs, lev, c.gi(s)); },
+                 [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); });
         return;
     }
-    __shared__ double sM[3][NHL2], sV1[3][NHL2], sPt[NHL2], sWl[NHL2], sWt[NHL2];
+    __shared__ R sM[3][NHL2];
+    __shared__ double sV1[3][NHL2], sPt[NHL2], sWl[NHL2], sWt[NHL2];
     const TileThread T = tile_thread(g, r);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
@@ -1253,9 +1443,9 @@ __global__ STENCIL_BOUNDS void k_density_pressure(Geo g, Fields f, double dt, in
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            const double m = VEC(f.Mhs, k, lev, q);
+            const R m = VEC(Mhs, k, lev, q);
             sM[k][pos] = m;
-            sV1[k][pos] = m + VEC(f.Mhk, k, lev, q);
+            sV1[k][pos] = (double)m + VEC(f.Mhk, k, lev, q);
         }
         sPt[pos] = LVP(f.pt, lev, q);
         if (next_divm) {
@@ -1265,24 +1455,17 @@ __global__ STENCIL_BOUNDS void k_density_pressure(Geo g, Fields f, double dt, in
     });
     __syncthreads();
     auto Ms = [&](int s, int k) { return sM[k][T.pos(s)]; };
-    dp_point(g, f, T.p, lev, false, dt, Ms, [&](int s, int k) { return sV1[k][T.pos(s)]; },
-             [&](int s) { return sPt[T.pos(s)]; });
-    if (next_divm) {
-        double o3[3];
-        divm_nf<false>(
-            g, T.p, lev, T.nf, false, Ms, [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; },
-            o3);
-        const int p = T.p;
-#pragma unroll
-        for (int k = 0; k < 3; k++) VEC(f.divg, k, lev, p) = o3[k];
-    }
+    dp_point<MX>(g, f, T.p, lev, false, dt, [&](int s, int k) { return (double)Ms(s, k); },
+             [&](int s, int k) { return sV1[k][T.pos(s)]; }, [&](int s) { return sPt[T.pos(s)]; });
+    if (next_divm)
+        next(T.p, T.nf, false, Ms, [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; });
 }
 
 // ===================================================================== K_H
 // UpdateRK2 (thor_auxiliary.h:420-529): y(k) += deviation. Whk is written to
 // a separate buffer so each level can form W at the layer centre from both of
 // its interfaces without a race.
-template <int LC>
+template <int LC, bool MX>
 __global__ __launch_bounds__(256) void k_update_rk2(Geo g, Fields f) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.P) return;
@@ -1293,10 +1476,11 @@ __global__ __launch_bounds__(256) void k_update_rk2(Geo g, Fields f) {
     auto whk_new = [&](int lev) {
         return (lev == 0 || lev == nv) ? 0.0 : LV(f.Whk, lev) + LV(f.Whs, lev);
     };
+    const auto *Mhs = FS<MX>::Mhs(f);
     for (int lev = l0; lev < l1; lev++) {
-        const double Mx = VEC(f.Mhs, 0, lev, p) + VEC(f.Mhk, 0, lev, p);
-        const double My = VEC(f.Mhs, 1, lev, p) + VEC(f.Mhk, 1, lev, p);
-        const double Mz = VEC(f.Mhs, 2, lev, p) + VEC(f.Mhk, 2, lev, p);
+        const double Mx = VEC(Mhs, 0, lev, p) + VEC(f.Mhk, 0, lev, p);
+        const double My = VEC(Mhs, 1, lev, p) + VEC(f.Mhk, 1, lev, p);
+        const double Mz = VEC(Mhs, 2, lev, p) + VEC(f.Mhk, 2, lev, p);
         const double aux = Mx * fx + My * fy + Mz * fz;
         VEC(f.Mhk_out, 0, lev, p) = Mx - aux * fx;
         VEC(f.Mhk_out, 1, lev, p) = My - aux * fy;
