@@ -137,10 +137,47 @@ thor_status thor_dyn_get_state(thor_dyn *h, const thor_state *s);
 /* One large step of the dynamical core (ESP::Thor). Asynchronous. */
 thor_status thor_dyn_thor(thor_dyn *h);
 
-enum { THOR_FORCING_NONE = 0, THOR_FORCING_HELD_SUAREZ = 1 };
+enum {
+    THOR_FORCING_NONE = 0,
+    THOR_FORCING_HELD_SUAREZ = 1, /* profx_held_suarez.h:51-117 */
+    THOR_FORCING_NEWTONIAN = 2,   /* hot-Jupiter Newtonian relaxation, see thor_newtonian_params */
+    THOR_FORCING_DOUBLE_GRAY = 3  /* double-gray two-stream RT heating, see thor_rt_params */
+};
 /* ESP::ProfX before the dynamics: W from Wh, T and theta from p and rho, the
- * benchmark forcing, then p = rho*Rd*T. Asynchronous. */
+ * benchmark forcing, then p = rho*Rd*T. With THOR_FORCING_DOUBLE_GRAY the
+ * forcing is a heating rate (profx_Qheat) that the step integrates, as
+ * THOR's physics modules do (phy_modules.h:76-86). Asynchronous. */
 thor_status thor_dyn_profx(thor_dyn *h, int forcing);
+
+/* Newtonian relaxation of temperature towards a day-night profile (no
+ * reference counterpart; THOR's shallowHJ/deepHJ benchmarks,
+ * profx_shallowHJ.h:51-117, have the same structure with other profiles):
+ *   Teq(lon, lat, p) = T_mean + dT * w(p) * cos(lon - lon_sub) * cos(lat),
+ *   w(p) = 1 for p <= p_contrast, falling linearly in ln p to 0 at p_bottom;
+ *   tau_rad(p) log-spaced from tau_top at p_top to tau_bottom at p_bottom
+ *   (clamped outside);  T <- (T + dt * Teq / tau) / (1 + dt / tau). */
+typedef struct thor_newtonian_params {
+    double T_mean, dT, lon_sub;
+    double p_top, p_bottom, p_contrast;
+    double tau_top, tau_bottom;
+} thor_newtonian_params;
+thor_status thor_dyn_set_newtonian(thor_dyn *h, const thor_newtonian_params *p);
+
+/* Double-gray two-stream radiative transfer (rtm_dual_band, computetau,
+ * radcsw, radclw, source_func_lin: src/physics/modules/inc/profx_RT.h:133-587;
+ * insolation src/ESP/insolation.cu:128-207 for a synchronous rotator on a
+ * circular orbit with zero obliquity: cos z = cos(lat) cos(lon - lon_sub),
+ * clipped at 0). incflx is the stellar flux at the planet [W/m2], i.e. THOR's
+ * incflx * r_orb^-2. No surface, latitude-independent kappa_lw. */
+typedef struct thor_rt_params {
+    double kappa_sw, kappa_lw; /* [m2/kg] */
+    double n_sw, n_lw, f_lw;
+    double diff_ang, Tint, albedo, incflx, ps0, lon_sub;
+} thor_rt_params;
+thor_status thor_dyn_set_rt(thor_dyn *h, const thor_rt_params *p);
+/* Per-column RT diagnostics of the last double-gray ProfX: insolation
+ * [W/m2], ASR and OLR [W] (reference ASR_d, OLR_d: flux x areasT). n = P. */
+thor_status thor_dyn_get_rt_column(thor_dyn *h, const char *name, double *dst, long n);
 
 /* nsteps x (ProfX + Thor) without host synchronisation. */
 thor_status thor_dyn_run(thor_dyn *h, int nsteps, int forcing);

@@ -69,6 +69,13 @@ def lib(variant: str = ""):
         L.oracle_math.restype = C.c_double
         L.oracle_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int, C.c_int]
         L.oracle_kernel.restype = C.c_int
+        for f in (L.oracle_set_newtonian, L.oracle_set_rt):
+            f.argtypes = [C.c_void_p, _dp]
+            f.restype = None
+        L.oracle_profx.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_profx.restype = C.c_int
+        L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.oracle_run.restype = C.c_int
         _libs[variant] = L
     return _libs[variant]
 
@@ -79,8 +86,8 @@ def tree_sum(a: np.ndarray) -> float:
 
 
 def math(fn: str, x: float, y: float = 0.0, variant: str = "") -> float:
-    """Elementary function as the oracle evaluates it (pow, log, sin, cos)."""
-    return lib(variant).oracle_math({"pow": 0, "log": 1, "sin": 2, "cos": 3}[fn], x, y)
+    """Elementary function as the oracle evaluates it (pow, log, sin, cos, exp)."""
+    return lib(variant).oracle_math({"pow": 0, "log": 1, "sin": 2, "cos": 3, "exp": 4}[fn], x, y)
 
 
 def _structs(grid, params):
@@ -145,6 +152,24 @@ class Oracle:
         out = np.zeros(5)
         self.L.oracle_globdiag(self._h, out.ctypes.data_as(_dp))
         return out
+
+    def set_newtonian(self, q):
+        """q: a ctypes NewtonianParams (or 8 numbers in its field order)."""
+        a = np.array([getattr(q, n) for n, _ in q._fields_] if hasattr(q, "_fields_") else q, dtype=np.float64)
+        self.L.oracle_set_newtonian(self._h, a.ctypes.data_as(_dp))
+
+    def set_rt(self, q):
+        """q: a ctypes RTParams (or 11 numbers in its field order)."""
+        a = np.array([getattr(q, n) for n, _ in q._fields_] if hasattr(q, "_fields_") else q, dtype=np.float64)
+        self.L.oracle_set_rt(self._h, a.ctypes.data_as(_dp))
+
+    def profx_forcing(self, forcing: int):
+        if self.L.oracle_profx(self._h, forcing) != 0:
+            raise ValueError(forcing)
+
+    def run(self, n: int, forcing: int):
+        if self.L.oracle_run(self._h, n, forcing) != 0:
+            raise ValueError(forcing)
 
     def kernel(self, name: str, a0: int = 0, a1: int = 0):
         """One restated kernel (see oracle_kernel in thor_oracle.h)."""

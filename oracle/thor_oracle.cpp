@@ -43,6 +43,7 @@ struct Oracle {
     int P, nv, nvi, nr;
     std::map<std::string, std::vector<double>> arr;
     std::vector<double> Kdh4, Kdhz;
+    double nrq[8] = {0}, rtq[11] = {0};  // forcing parameters
 
     double *A_(const char *n) { return arr[n].data(); }
 };
@@ -1369,6 +1370,171 @@ void held_suarez(Oracle &o) {
         }
 }
 
+// --------------------------------------------------- Newtonian relaxation
+// Hot-Jupiter forcing (no reference counterpart; structure of shallowHJ,
+// profx_shallowHJ.h:51-117): T relaxes implicitly towards
+// Teq = T_mean + dT w(p) cos(lon - lon_sub) cos(lat) with time scale tau(p);
+// w = 1 above p_contrast, linear in ln p down to 0 at p_bottom; ln tau linear
+// in ln p between (p_top, tau_top) and (p_bottom, tau_bottom), clamped.
+void profx_nr(Oracle &o) {
+    const int nv = o.nv;
+    double *W = o.A_("W"), *Wh = o.A_("Wh"), *T = o.A_("temperature"), *p = o.A_("pressure"), *Rho = o.A_("Rho");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat;
+    const double *q = o.nrq;  // T_mean, dT, lon_sub, p_top, p_bottom, p_contrast, tau_top, tau_bottom
+    const double lnp_top = std::log(q[3]), lnp_bot = std::log(q[4]), lnp_con = std::log(q[5]);
+    const double lntau_top = std::log(q[6]), lntau_bot = std::log(q[7]);
+    const double Rd = o.p.Rd, dt = o.p.timestep;
+    zero(o, "profx_Qheat");
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            const int e = id * nv + lev;
+            double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
+            double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            W[e] = whl * intt + wht * intl;
+            const double pre = p[e], rho = Rho[e];
+            double t = pre / (Rd * rho);
+            const double lnp = MLOG(pre);
+            double w = (lnp_bot - lnp) / (lnp_bot - lnp_con);
+            w = std::fmin(1.0, std::fmax(0.0, w));
+            const double Teq = q[0] + q[1] * w * MCOS(ll[id * 2] - q[2]) * MCOS(ll[id * 2 + 1]);
+            double fr = (lnp - lnp_top) / (lnp_bot - lnp_top);
+            fr = std::fmin(1.0, std::fmax(0.0, fr));
+            const double tau = MEXP(lntau_top + fr * (lntau_bot - lntau_top));
+            t = (t + dt * Teq / tau) / (1.0 + dt / tau);
+            T[e] = t;
+            p[e] = t * Rd * rho;
+        }
+}
+
+// ------------------------------------------------ double-gray radiation
+// source_func_lin (profx_RT.h:133-160).
+double rt_source(double bb, double bl, double bt, double tau, double diff_ang) {
+    double e1 = 0.0, e2 = 0.0, e = 0.0;
+    if (tau >= 1e-10) {
+        e1 = bb - bl + (bl - (diff_ang / (tau / 2.0)) * (bb - bl)) * (1.0 - MEXP(-(tau / 2.0) / diff_ang));
+        e2 = bl - bt + (bt - (diff_ang / (tau / 2.0)) * (bl - bt)) * (1.0 - MEXP(-(tau / 2.0) / diff_ang));
+        e = e1 + e2 * MEXP(-(tau / 2.0) / diff_ang);
+    } else {
+        for (int i = 0; i < 5; i++) {
+            int fac = 1;
+            for (int n1 = i + 2; n1 != 0; n1--) fac *= n1;  // factorial_num (profx_RT.h:119-131)
+            const double sgn = (i % 2 == 0) ? 1.0 : -1.0;   // pow(-1, i + 2)
+            e1 = e1 + sgn * ((bl + (i + 1) * bl) / fac) * MPOW((tau / 2.0) / diff_ang, i + 1.0);
+            e2 = e2 + sgn * ((bt + (i + 1) * bb) / fac) * MPOW((tau / 2.0) / diff_ang, i + 1.0);
+        }
+        e = e1 + e2 * MEXP(-(tau / 2.0) / diff_ang);
+    }
+    return e;
+}
+
+// ProfX with the double-gray module: Recompute_W, Compute_temperature,
+// rtm_dual_band (profx_RT.h:296-587: computetau, radcsw, radclw; no surface,
+// constant kappa_lw, Qheat_scaling 1), Compute_pressure. cos z for a
+// synchronous rotator (insolation.cu:128-207).
+void profx_rt(Oracle &o) {
+    const int nv = o.nv, nvi = nv + 1;
+    double *W = o.A_("W"), *Wh = o.A_("Wh"), *T = o.A_("temperature"), *p = o.A_("pressure"), *Rho = o.A_("Rho"),
+           *Qh = o.A_("profx_Qheat"), *insol = o.A_("insol"), *ASR = o.A_("ASR"), *OLR = o.A_("OLR");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat, *areasT = o.g.areasT;
+    // kappa_sw, kappa_lw, n_sw, n_lw, f_lw, diff_ang, Tint, albedo, incflx, ps0, lon_sub
+    const double *q = o.rtq;
+    const double kappa_sw = q[0], kappa_lw = q[1], n_sw = q[2], n_lw = q[3], f_lw = q[4], diff_ang = q[5],
+                 tint = q[6], alb = q[7], incflx = q[8], ps0 = q[9], lon_sub = q[10];
+    const double Rd = o.p.Rd, Cp = o.p.Cp, gravit = o.p.Gravit, A = o.p.A, timestep = o.p.timestep;
+    const double bc = 5.677036E-8;
+    zero(o, "profx_Qheat");
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            const int e = id * nv + lev;
+            double whl = Wh[id * nvi + lev], wht = Wh[id * nvi + lev + 1];
+            double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            W[e] = whl * intt + wht * intl;
+            T[e] = p[e] / (Rd * Rho[e]);
+        }
+    std::vector<double> phtemp(nvi), thtemp(nvi), ttemp(nv), tau(2 * nv), dtemp(nv), fsw_up(nvi), fsw_dn(nvi),
+        flw_up(nvi), flw_dn(nvi);
+    for (int id = 0; id < o.P; id++) {
+        const double *pr = p + (size_t)id * nv, *rh = Rho + (size_t)id * nv, *te = T + (size_t)id * nv;
+        for (int lev = 0; lev < nv; lev++) dtemp[lev] = 0.0;
+        for (int lev = 0; lev <= nv; lev++) {
+            if (lev == 0) {
+                double psm = pr[1] - rh[0] * gravit * (-Alt[0] - Alt[1]);
+                phtemp[0] = 0.5 * (pr[0] + psm);
+                ttemp[0] = te[0];
+                thtemp[0] = ttemp[0];
+            } else if (lev == nv) {
+                double pp = pr[nv - 2] + (pr[nv - 1] - pr[nv - 2]) / (Alt[nv - 1] - Alt[nv - 2])
+                                             * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+                if (pp < 0) pp = 0;
+                phtemp[nv] = 0.5 * (pr[nv - 1] + pp);
+                thtemp[nv] = te[nv - 1];
+            } else {
+                ttemp[lev] = te[lev];
+                double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
+                double a = (xi - xip) / (xim - xip), b = (xi - xim) / (xip - xim);
+                phtemp[lev] = pr[lev - 1] * a + pr[lev] * b;
+                thtemp[lev] = te[lev - 1] * a + ttemp[lev] * b;
+            }
+        }
+        double coszrs = MCOS(ll[id * 2 + 1]) * MCOS(ll[id * 2] - lon_sub);
+        if (coszrs < 0.0) coszrs = 0.0;
+        // computetau (profx_RT.h:267-292)
+        for (int lev = 0; lev < nv; lev++) {
+            tau[lev * 2] = n_sw * kappa_sw * MPOW(pr[lev] / ps0, n_sw - 1) * rh[lev] * (Alth[lev + 1] - Alth[lev]);
+            tau[lev * 2 + 1] = kappa_lw * (1 + n_lw * (1 - f_lw) / f_lw * MPOW(pr[lev] / ps0, n_lw - 1)) * rh[lev]
+                               * (Alth[lev + 1] - Alth[lev]);
+        }
+        for (int lev = 0; lev <= nv; lev++) fsw_up[lev] = fsw_dn[lev] = 0.0;
+        if (coszrs > 0.0) {
+            // radcsw (profx_RT.h:14-117)
+            double tt = (kappa_sw / gravit) * phtemp[nv];
+            insol[id] = incflx * coszrs;
+            double flux_top = insol[id] * (1.0 - alb);
+            fsw_dn[nv] = flux_top * MEXP(-(1.0 / coszrs) * tt);
+            for (int lev = nv; lev >= 1; lev--) fsw_dn[lev - 1] = fsw_dn[lev] * MEXP(-(1.0 / coszrs) * tau[(lev - 1) * 2]);
+            for (int lev = 0; lev <= nv; lev++) fsw_up[lev] = 0.0;
+            for (int lev = 0; lev < nv; lev++)
+                dtemp[lev] = -((fsw_up[lev] - fsw_dn[lev]) - (fsw_up[lev + 1] - fsw_dn[lev + 1]))
+                             / ((Alth[lev] - Alth[lev + 1]));
+        } else {
+            insol[id] = 0;
+        }
+        const double rscale = o.p.DeepModel ? (A + Alth[nv]) / A : 1.0;
+        ASR[id] = fsw_dn[nv] * areasT[id] * MPOW(rscale, 2);
+        // radclw (profx_RT.h:162-265)
+        for (int lev = 0; lev <= nv; lev++) flw_up[lev] = flw_dn[lev] = 0.0;
+        flw_dn[nv] = 0.0;
+        for (int lev = nv - 1; lev >= 0; lev--) {
+            if (tau[2 * lev + 1] < 0.0) tau[2 * lev + 1] = 0.0;
+            double tb = thtemp[lev], tl = ttemp[lev], tt = thtemp[lev + 1];
+            double bb = bc * tb * tb * tb * tb, bl = bc * tl * tl * tl * tl, bt = bc * tt * tt * tt * tt;
+            double ed = rt_source(bb, bl, bt, tau[2 * lev + 1], diff_ang);
+            flw_dn[lev] = ed + flw_dn[lev + 1] * MEXP(-(1. / diff_ang) * tau[2 * lev + 1]);
+        }
+        flw_up[0] = bc * tint * tint * tint * tint;
+        flw_up[0] += flw_dn[0];
+        for (int lev = 1; lev <= nv; lev++) {
+            if (tau[2 * (lev - 1) + 1] < 0.0) tau[2 * (lev - 1) + 1] = 0.0;
+            double tb = thtemp[lev - 1], tl = ttemp[lev - 1], tt = thtemp[lev];
+            double bb = bc * tb * tb * tb * tb, bl = bc * tl * tl * tl * tl, bt = bc * tt * tt * tt * tt;
+            double eu = rt_source(bt, bl, bb, tau[2 * (lev - 1) + 1], diff_ang);
+            flw_up[lev] = eu + flw_up[lev - 1] * MEXP(-(1. / diff_ang) * tau[2 * (lev - 1) + 1]);
+        }
+        for (int lev = 0; lev < nv; lev++)
+            dtemp[lev] = dtemp[lev]
+                         - ((flw_up[lev] - flw_dn[lev]) - (flw_up[lev + 1] - flw_dn[lev + 1]))
+                               / ((Alth[lev] - Alth[lev + 1]));
+        OLR[id] = flw_up[nv] * areasT[id] * MPOW(rscale, 2);
+        for (int lev = 0; lev < nv; lev++) {
+            if (pr[lev] + Rd / (Cp - Rd) * dtemp[lev] * timestep < 0) dtemp[lev] = -pr[lev] / timestep;
+            Qh[id * nv + lev] += dtemp[lev];
+        }
+    }
+    for (int e = 0; e < o.P * nv; e++) p[e] = T[e] * Rd * Rho[e];
+}
+
 void make_kd(Oracle &o) {
     // Horizontal diffusion / divergence-damping strengths (esp_initial.cu:1292-1321).
     o.Kdh4.assign(o.nv, 0.0);
@@ -1402,13 +1568,15 @@ double oracle_tree_sum(const double *d, long n) {
 struct oracle : Oracle {};
 
 // Elementary functions as the restatement calls them (portable in the parity
-// build, glibc otherwise): fn 0 = pow(x, y), 1 = log(x), 2 = sin(x), 3 = cos(x).
+// build, glibc otherwise): fn 0 = pow(x, y), 1 = log(x), 2 = sin(x), 3 = cos(x),
+// 4 = exp(x).
 extern "C" double oracle_math(int fn, double x, double y) {
     switch (fn) {
     case 0: return MPOW(x, y);
     case 1: return MLOG(x);
     case 2: return MSIN(x);
-    default: return MCOS(x);
+    case 3: return MCOS(x);
+    default: return MEXP(x);
     }
 }
 
@@ -1437,6 +1605,9 @@ oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
         o->arr["Cp"][e] = p->Cp;
     }
     make_kd(*o);
+    o->arr["insol"].assign(o->P, 0.0);
+    o->arr["ASR"].assign(o->P, 0.0);
+    o->arr["OLR"].assign(o->P, 0.0);
     o->arr["Kdh4"] = o->Kdh4;  // read-only copies for callers
     o->arr["Kdhz"] = o->Kdhz;
     return o;
@@ -1489,6 +1660,23 @@ void oracle_steps(oracle *o, int n) {
         profx_hs(*o);
         thor(*o);
     }
+}
+void oracle_set_newtonian(oracle *o, const double q[8]) { std::memcpy(o->nrq, q, sizeof(o->nrq)); }
+void oracle_set_rt(oracle *o, const double q[11]) { std::memcpy(o->rtq, q, sizeof(o->rtq)); }
+int oracle_profx(oracle *o, int forcing) {
+    if (forcing == 0) return 0;
+    if (forcing == 1) profx_hs(*o);
+    else if (forcing == 2) profx_nr(*o);
+    else if (forcing == 3) profx_rt(*o);
+    else return -1;
+    return 0;
+}
+int oracle_run(oracle *o, int n, int forcing) {
+    for (int i = 0; i < n; i++) {
+        if (oracle_profx(o, forcing) != 0) return -1;
+        thor(*o);
+    }
+    return 0;
 }
 
 // CalcMass / CalcTotEnergy / CalcAngMom (profx_globdiag.cu:45-212) + tree sums.

@@ -57,12 +57,23 @@ void oracle_profx_held_suarez(oracle *o);
 int oracle_kernel(oracle *o, const char *name, int a0, int a1);
 // ProfX + Thor, n times.
 void oracle_steps(oracle *o, int n);
+// Forcings other than Held-Suarez (forcing codes as THOR_FORCING_* of the
+// library): 2 Newtonian relaxation with q = {T_mean, dT, lon_sub, p_top,
+// p_bottom, p_contrast, tau_top, tau_bottom}; 3 double-gray RT with q =
+// {kappa_sw, kappa_lw, n_sw, n_lw, f_lw, diff_ang, Tint, albedo, incflx, ps0,
+// lon_sub} (per-column results in the arrays "insol", "ASR", "OLR").
+void oracle_set_newtonian(oracle *o, const double q[8]);
+void oracle_set_rt(oracle *o, const double q[11]);
+// ProfX with forcing 0..3; returns -1 for an unknown forcing.
+int oracle_profx(oracle *o, int forcing);
+// n x (ProfX with forcing + Thor).
+int oracle_run(oracle *o, int n, int forcing);
 // Global mass, total energy, angular momentum (x, y, z): per-cell values summed
 // with the reference's 1024-thread tree reduction order.
 void oracle_globdiag(oracle *o, double out[5]);
 // Tree-reduction sum of an arbitrary array in the reference order (reduction_add.h:59-111).
 double oracle_tree_sum(const double *d, long n);
-// Elementary functions used by the restatement (fn 0 pow, 1 log, 2 sin, 3 cos).
+// Elementary functions used by the restatement (fn 0 pow, 1 log, 2 sin, 3 cos, 4 exp).
 double oracle_math(int fn, double x, double y);
 
 #ifdef __cplusplus

@@ -28,3 +28,35 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     if den == 0:
         return float(num)
     return float(num / den)
+
+
+# Hot-Jupiter set-up of the Newtonian-relaxation and double-gray runs
+HJ_TOP = 7.4e6  # [m]: isothermal H = Rd T / g ~ 6.42e5 m, ln(1e7 / 1e2) ~ 11.5 scale heights
+
+
+def hj_params(capi, glevel, dt=300.0, ns=6, **kw):
+    """Hot Jupiter: R = 1e8 m, g = 9.42, Rd = 3779, Cp = 13226, Omega = 2.06e-5,
+    P0 = 1e7 Pa, isothermal 1600 K start."""
+    return capi.default_params(glevel=glevel, timestep=dt, n_substeps=ns, A=1e8, Omega=2.06e-5, Gravit=9.42,
+                               P_Ref=1e7, Rd=3779.0, Cp=13226.0, Tmean=1600.0, **kw)
+
+
+def hj_grid(capi, glevel, nv):
+    return grid(capi, glevel, nv, top=HJ_TOP, radius=1e8)
+
+
+def nr_params(capi):
+    """Day-night relaxation: 1600 +- 600 K above 1e5 Pa, vanishing at 1e7 Pa;
+    tau_rad 1e4 s at 1e2 Pa to 1e7 s at 1e7 Pa."""
+    return capi.NewtonianParams(T_mean=1600.0, dT=600.0, lon_sub=0.0, p_top=1e2, p_bottom=1e7, p_contrast=1e5,
+                                tau_top=1e4, tau_bottom=1e7)
+
+
+def rt_params(capi, **kw):
+    """Double-gray RT: kappa_sw 4e-4, kappa_lw 1e-3 m2/kg, Tint 100 K, incident
+    flux sigma * (2000 K)^4 at the substellar point, f_lw 0.5, n_sw 1, n_lw 2,
+    diffusivity 0.5, no albedo."""
+    q = dict(kappa_sw=4e-4, kappa_lw=1e-3, n_sw=1.0, n_lw=2.0, f_lw=0.5, diff_ang=0.5, Tint=100.0, albedo=0.0,
+             incflx=5.6704e-8 * 2000.0 ** 4, ps0=1e7, lon_sub=0.0)
+    q.update(kw)
+    return capi.RTParams(**q)

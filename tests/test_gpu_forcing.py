@@ -1,0 +1,113 @@
+"""Hot-Jupiter forcings on the GPU against the oracle.
+
+The parity build (portable elementary functions on both sides) must
+reproduce the oracle bit for bit through the Newtonian relaxation, the
+double-gray RT ProfX and the dynamics steps that follow them. The mixed-
+precision step under the same forcing must stay close to FP64.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from common import hj_grid, hj_params, nr_params, rt_params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hj(built):
+    capi = built
+    g = hj_grid(capi, 4, 20)
+    p = hj_params(capi, 4)
+    st = capi.ic_rest(g, p, seed=1001, temperature_noise=1.0, wind_noise=0.1)
+    return capi, g, p, st
+
+
+def _max_rel(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
+
+
+def test_newtonian_steps_bitwise(hj):
+    from oracle.oracle import Oracle
+    capi, g, p, st = hj
+    print("case: parity build, hot Jupiter g4 L20, 2 x (Newtonian ProfX + Thor)", flush=True)
+    d = capi.Dyn(g, p, variant="pm")
+    d.set_state(st)
+    d.set_newtonian(nr_params(capi))
+    o = Oracle(g, p, variant="pm")
+    o.set_state(st)
+    o.set_newtonian(nr_params(capi))
+    d.run(2, capi.FORCING_NEWTONIAN)
+    d.sync()
+    o.run(2, capi.FORCING_NEWTONIAN)
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("newtonian", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()
+
+
+def test_double_gray_profx_and_steps_bitwise(hj):
+    from oracle.oracle import Oracle
+    capi, g, p, st = hj
+    print("case: parity build, hot Jupiter g4 L20, double-gray ProfX, then 2 x (ProfX + Thor)", flush=True)
+    d = capi.Dyn(g, p, variant="pm")
+    d.set_state(st)
+    d.set_rt(rt_params(capi))
+    o = Oracle(g, p, variant="pm")
+    o.set_state(st)
+    o.set_rt(rt_params(capi))
+    d.profx(capi.FORCING_DOUBLE_GRAY)
+    d.sync()
+    o.profx_forcing(capi.FORCING_DOUBLE_GRAY)
+    for name in ("profx_Qheat", "temperature", "pressure"):
+        e = _max_rel(d.get_field(name).ravel(), o.get(name))
+        print(f"  {name}: {e:.2e}")
+        assert e == 0.0, name
+    for name in ("insol", "ASR", "OLR"):
+        e = _max_rel(d.rt_column(name), o.get(name))
+        print(f"  {name}: {e:.2e}")
+        assert e == 0.0, name
+    d.thor()
+    o.thor()
+    d.run(1, capi.FORCING_DOUBLE_GRAY)
+    d.sync()
+    o.run(1, capi.FORCING_DOUBLE_GRAY)
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("double gray", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()
+
+
+@pytest.mark.parametrize("forcing", ["newtonian", "double_gray"])
+def test_hot_jupiter_mixed_stays_close_to_fp64(hj, forcing):
+    capi, g, p64, st = hj
+    code = capi.FORCING_NEWTONIAN if forcing == "newtonian" else capi.FORCING_DOUBLE_GRAY
+    print(f"case: hot Jupiter g4 L20, 10 steps of {forcing}, FP64 vs mixed", flush=True)
+    out, mass = {}, {}
+    for prec in (0, 1):
+        p = hj_params(capi, 4, precision=prec)
+        d = capi.Dyn(g, p)
+        d.set_state(st)
+        if code == capi.FORCING_NEWTONIAN:
+            d.set_newtonian(nr_params(capi))
+        else:
+            d.set_rt(rt_params(capi))
+        m0 = d.globdiag()[0]
+        d.run(10, code)
+        d.sync()
+        out[prec] = d.get_state()
+        mass[prec] = (d.globdiag()[0] - m0) / m0
+        d.close()
+    for prec in (0, 1):
+        print(f"  precision {prec}: relative mass change {mass[prec]:.2e}")
+        assert abs(mass[prec]) < 1e-10
+        assert all(np.isfinite(v).all() for v in out[prec].values())
+    tol = {"Mh": 1e-3, "Rho": 1e-6, "pressure": 1e-6}
+    for k, t in tol.items():
+        e = _max_rel(out[1][k], out[0][k])
+        print(f"  mixed vs FP64 {k}: {e:.2e}")
+        assert e < t, k

@@ -56,6 +56,23 @@ def test_elementary_functions(built, variant):
         assert abs(oracle.math("cos", t, 0, variant) - np.cos(t)) < 2e-15
     for x in (1.0, 3.5, 6.371e6 + 36000.0):
         assert oracle.math("pow", x, 2.0, variant) == x * x
+    # exp over its whole range, including subnormal results, overflow and underflow
+    for x in np.concatenate([rng.uniform(-745, 709.7, 3000), [-1e6, -746.0, -745.0, -708.4, 709.78, 710.0]]):
+        want = np.exp(x)
+        got = oracle.math("exp", float(x), 0, variant)
+        if want == 0.0 or np.isinf(want):
+            assert got == want or (want == 0.0 and abs(got) < 1e-320), x
+        elif want < 2.2250738585072014e-308:
+            assert abs(got - want) <= 2 * 4.9e-324, x  # subnormal: within two of the smallest step
+        else:
+            assert abs(got / want - 1) < 4e-16, x
+    # sin / cos beyond [-pi, pi] and log at its edges
+    for t in rng.uniform(-10.0, 10.0, 500):
+        assert abs(oracle.math("sin", float(t), 0, variant) - np.sin(t)) < 4e-15
+        assert abs(oracle.math("cos", float(t), 0, variant) - np.cos(t)) < 4e-15
+    assert oracle.math("log", 0.0, 0, variant) == -np.inf
+    assert np.isnan(oracle.math("log", -1.0, 0, variant))
+    assert abs(oracle.math("log", 1e-310, 0, variant) - np.log(1e-310)) < 1e-13
 
 
 def test_rest_state_is_discrete_hydrostatic(setup):

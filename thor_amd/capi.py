@@ -31,6 +31,20 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 FORCING_NONE = 0
 FORCING_HELD_SUAREZ = 1
+FORCING_NEWTONIAN = 2
+FORCING_DOUBLE_GRAY = 3
+
+
+class NewtonianParams(C.Structure):
+    """thor_newtonian_params: hot-Jupiter day-night Newtonian relaxation."""
+    _fields_ = [(n, C.c_double) for n in ("T_mean", "dT", "lon_sub", "p_top", "p_bottom", "p_contrast",
+                                          "tau_top", "tau_bottom")]
+
+
+class RTParams(C.Structure):
+    """thor_rt_params: double-gray two-stream radiative transfer."""
+    _fields_ = [(n, C.c_double) for n in ("kappa_sw", "kappa_lw", "n_sw", "n_lw", "f_lw", "diff_ang", "Tint",
+                                          "albedo", "incflx", "ps0", "lon_sub")]
 
 
 class ThorError(RuntimeError):
@@ -115,6 +129,9 @@ def lib(variant: str = "") -> C.CDLL:
             L.thor_dyn_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
                                                C.POINTER(C.c_long), _dp, _dp]
             L.thor_dyn_time_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+            L.thor_dyn_set_newtonian.argtypes = [C.c_void_p, C.POINTER(NewtonianParams)]
+            L.thor_dyn_set_rt.argtypes = [C.c_void_p, C.POINTER(RTParams)]
+            L.thor_dyn_get_rt_column.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
         _libs[variant] = L
     return _libs[variant]
 
@@ -255,6 +272,17 @@ class Dyn:
 
     def thor(self):
         self._chk(self.L.thor_dyn_thor(self._h))
+
+    def set_newtonian(self, q: NewtonianParams):
+        self._chk(self.L.thor_dyn_set_newtonian(self._h, C.byref(q)))
+
+    def set_rt(self, q: RTParams):
+        self._chk(self.L.thor_dyn_set_rt(self._h, C.byref(q)))
+
+    def rt_column(self, name: str) -> np.ndarray:
+        a = np.zeros(self.grid.P)
+        self._chk(self.L.thor_dyn_get_rt_column(self._h, name.encode(), _ptr(a), a.size))
+        return a
 
     def profx(self, forcing=FORCING_HELD_SUAREZ):
         self._chk(self.L.thor_dyn_profx(self._h, forcing))

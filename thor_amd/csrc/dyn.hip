@@ -56,6 +56,12 @@ struct Dyn {
     std::vector<std::string> stat_order;
     double diff_sign = 1.0;
     bool mixed = false;  // THOR_PRECISION_MIXED
+    // forcings
+    bool nr_set = false, rt_set = false;
+    NRPar nr{};
+    RTPar rt{};
+    double *insol = nullptr, *asr = nullptr, *olr = nullptr;  // [P] RT column diagnostics
+    double *fdn_sw = nullptr, *fdn_lw = nullptr;              // [nv+1][S] RT scratch
 };
 
 static hipError_t dalloc(Dyn &d, double **ptr, size_t n) {
@@ -217,13 +223,31 @@ static int thor_step(Dyn &d) { return d.mixed ? thor_step_t<true>(d) : thor_step
 
 static int profx(Dyn &d, int forcing) {
     if (forcing == THOR_FORCING_NONE) return THOR_OK;
-    if (forcing != THOR_FORCING_HELD_SUAREZ) return fail(THOR_ERR_BAD_ARG, "unknown forcing");
     Fields f = stage_fields(d, 0);
     const Geo g = d.g;
     const dim3 blk(kTile), grid_c((d.P + 63) / 64);  // 64 columns x 4 level rows per block
     const double B = 8.0 * d.P * d.nv;
-    launch(d, "profx_held_suarez", 24 * B,
-           [&] { k_profx_hs<<<grid_c, blk, 0, d.stream>>>(g, f, d.prm.timestep, 1); });
+    const double dt = d.prm.timestep;
+    if (forcing == THOR_FORCING_HELD_SUAREZ) {
+        launch(d, "profx_held_suarez", 24 * B, [&] {
+            k_profx_relax<THOR_FORCING_HELD_SUAREZ><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr);
+        });
+    } else if (forcing == THOR_FORCING_NEWTONIAN) {
+        if (!d.nr_set) return fail(THOR_ERR_BAD_ARG, "profx: Newtonian forcing parameters not set");
+        // W, T, relaxation and pressure per cell: Wh 2, p, rho, Mh untouched -> 4 in, W, p, Qheat out
+        launch(d, "profx_newtonian", 7 * B, [&] {
+            k_profx_relax<THOR_FORCING_NEWTONIAN><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr);
+        });
+    } else if (forcing == THOR_FORCING_DOUBLE_GRAY) {
+        if (!d.rt_set) return fail(THOR_ERR_BAD_ARG, "profx: RT parameters not set");
+        // Wh 2, p, rho in; W, T, Qheat, p out; two flux planes written and read twice
+        launch(d, "profx_double_gray", 14 * B, [&] {
+            k_profx_rt<<<(d.P + kTile - 1) / kTile, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr,
+                                                                       d.fdn_sw, d.fdn_lw);
+        });
+    } else {
+        return fail(THOR_ERR_BAD_ARG, "unknown forcing");
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("profx launch: ") + hipGetErrorString(e));
     return THOR_OK;
@@ -497,6 +521,58 @@ extern "C" thor_status thor_dyn_run(thor_dyn *h, int nsteps, int forcing) {
         rc = thor::thor_step(*h);
         if (rc != THOR_OK) return rc;
     }
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_set_newtonian(thor_dyn *h, const thor_newtonian_params *q) {
+    using namespace thor;
+    if (!h || !q) return fail(THOR_ERR_BAD_ARG, "set_newtonian: null argument");
+    if (!(q->p_top > 0 && q->p_bottom > q->p_top && q->p_contrast > 0 && q->p_contrast < q->p_bottom &&
+          q->tau_top > 0 && q->tau_bottom > 0))
+        return fail(THOR_ERR_BAD_ARG, "set_newtonian: need 0 < p_top < p_bottom, 0 < p_contrast < p_bottom, tau > 0");
+    NRPar &n = h->nr;
+    n.T_mean = q->T_mean;
+    n.dT = q->dT;
+    n.lon_sub = q->lon_sub;
+    n.lnp_top = std::log(q->p_top);
+    n.lnp_bot = std::log(q->p_bottom);
+    n.lnp_con = std::log(q->p_contrast);
+    n.lntau_top = std::log(q->tau_top);
+    n.lntau_bot = std::log(q->tau_bottom);
+    h->nr_set = true;
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_set_rt(thor_dyn *h, const thor_rt_params *q) {
+    using namespace thor;
+    if (!h || !q) return fail(THOR_ERR_BAD_ARG, "set_rt: null argument");
+    if (!(q->kappa_sw >= 0 && q->kappa_lw >= 0 && q->f_lw > 0 && q->diff_ang > 0 && q->ps0 > 0 && q->incflx >= 0))
+        return fail(THOR_ERR_BAD_ARG, "set_rt: need kappa >= 0, f_lw > 0, diff_ang > 0, ps0 > 0, incflx >= 0");
+    if (!h->insol) {
+        const size_t ni = (size_t)(h->nv + 1) * h->S;
+        hipError_t e = dalloc(*h, &h->insol, h->S);
+        if (e == hipSuccess) e = dalloc(*h, &h->asr, h->S);
+        if (e == hipSuccess) e = dalloc(*h, &h->olr, h->S);
+        if (e == hipSuccess) e = dalloc(*h, &h->fdn_sw, ni);
+        if (e == hipSuccess) e = dalloc(*h, &h->fdn_lw, ni);
+        if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("set_rt: alloc: ") + hipGetErrorString(e));
+    }
+    h->rt = RTPar{q->kappa_sw, q->kappa_lw, q->n_sw, q->n_lw, q->f_lw, q->diff_ang,
+                  q->Tint, q->albedo, q->incflx, q->ps0, q->lon_sub};
+    h->rt_set = true;
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_get_rt_column(thor_dyn *h, const char *name, double *dst, long n) {
+    using namespace thor;
+    if (!h || !name || !dst) return fail(THOR_ERR_BAD_ARG, "get_rt_column: null argument");
+    if (!h->rt_set) return fail(THOR_ERR_BAD_ARG, "get_rt_column: RT not set");
+    if (n != h->P) return fail(THOR_ERR_BAD_ARG, "get_rt_column: n must be the point count");
+    const std::string s(name);
+    const double *src = s == "insol" ? h->insol : s == "ASR" ? h->asr : s == "OLR" ? h->olr : nullptr;
+    if (!src) return fail(THOR_ERR_BAD_ARG, "get_rt_column: name must be insol, ASR or OLR");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
     return THOR_OK;
 }
 

@@ -1502,10 +1502,34 @@ __global__ __launch_bounds__(256) void k_update_rk2(Geo g, Fields f) {
 // ESP::ProfX for the Held-Suarez benchmark, per column: Recompute_W
 // (profx_auxiliary.h:110-130), Compute_temperature (:64-93), held_suarez
 // (profx_held_suarez.h:51-117) and Compute_pressure (:95-108).
+// Newtonian-relaxation parameters in the form the kernel uses (logarithms of
+// the pressures and time scales).
+struct NRPar {
+    double T_mean, dT, lon_sub, lnp_top, lnp_bot, lnp_con, lntau_top, lntau_bot;
+};
+
+// Equilibrium temperature and relaxation time of the hot-Jupiter Newtonian
+// forcing at pressure pre (see thor_newtonian_params).
+__device__ __forceinline__ void nr_profile(const NRPar &q, double lon, double lat, double pre, double &Teq,
+                                           double &tau) {
+    const double lnp = MLOG(pre);
+    double w = (q.lnp_bot - lnp) / (q.lnp_bot - q.lnp_con);
+    w = fmin(1.0, fmax(0.0, w));
+    Teq = q.T_mean + q.dT * w * MCOS(lon - q.lon_sub) * MCOS(lat);
+    double fr = (lnp - q.lnp_top) / (q.lnp_bot - q.lnp_top);
+    fr = fmin(1.0, fmax(0.0, fr));
+    tau = MEXP(q.lntau_top + fr * (q.lntau_bot - q.lntau_top));
+}
+
+// ProfX with a temperature-relaxation forcing: Recompute_W
+// (profx_auxiliary.h:110-130), T = p / (rho Rd), the forcing, p = rho Rd T
+// (profx_auxiliary.h:95-108). F = THOR_FORCING_HELD_SUAREZ: held_suarez
+// (profx_held_suarez.h:51-117); F = THOR_FORCING_NEWTONIAN: nr_profile.
 // Block of 64 columns x 4 level rows; the surface pressure of each column is
 // read before any level of it is overwritten (one barrier), then every
 // (column, level) is independent.
-__global__ __launch_bounds__(256) void k_profx_hs(Geo g, Fields f, double time_step, int zero_q) {
+template <int F>
+__global__ __launch_bounds__(256) void k_profx_relax(Geo g, Fields f, double time_step, int zero_q, NRPar nr) {
     constexpr int CB = 64, NROW = 4;
     const int c = threadIdx.x % CB, row = threadIdx.x / CB;
     const int p = blockIdx.x * CB + c;
@@ -1517,8 +1541,9 @@ __global__ __launch_bounds__(256) void k_profx_hs(Geo g, Fields f, double time_s
     const double kappa = Rd / Cp;
     const double sigmab = 0.7, ka = (1.0 / 40.0) * (1.0 / 86400.0), kf = 1.0 / 86400.0,
                  ks = (1.0 / 4.0) * (1.0 / 86400.0), dTy = 60.0, dthetaz = 10.0, p0 = 100000.0;
-    double ps = 0.0, lat = 0.0;
+    double ps = 0.0, lat = 0.0, lon = 0.0;
     if (valid) {
+        lon = g.lonlat[p];
         lat = g.lonlat[S + p];
         const double psm1 = LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
         ps = 0.5 * (LV(f.p, 0) + psm1);
@@ -1535,19 +1560,163 @@ __global__ __launch_bounds__(256) void k_profx_hs(Geo g, Fields f, double time_s
         }
         const double pre = LV(f.p, lev), rho = LV(f.Rho, lev);
         double T = pre / (Rd * rho);
-        const double sigma = (pre / ps), sigma0 = (pre / p0);
-        const double Teq = fmax(200.0, (315.0 - dTy * MPOW(MSIN(lat), 2.0) - dthetaz * MLOG(sigma0) * MPOW(MCOS(lat), 2.0))
-                                           * MPOW(sigma0, kappa));
-        const double kt = ka + (ks - ka) * fmax(0.0, (sigma - sigmab) / (1.0 - sigmab)) * MPOW(MCOS(lat), 4.0);
-        const double kv = kf * fmax(0.0, (sigma - sigmab) / (1.0 - sigmab));
+        if constexpr (F == THOR_FORCING_HELD_SUAREZ) {
+            const double sigma = (pre / ps), sigma0 = (pre / p0);
+            const double Teq = fmax(200.0, (315.0 - dTy * MPOW(MSIN(lat), 2.0) - dthetaz * MLOG(sigma0) * MPOW(MCOS(lat), 2.0))
+                                               * MPOW(sigma0, kappa));
+            const double kt = ka + (ks - ka) * fmax(0.0, (sigma - sigmab) / (1.0 - sigmab)) * MPOW(MCOS(lat), 4.0);
+            const double kv = kf * fmax(0.0, (sigma - sigmab) / (1.0 - sigmab));
 #pragma unroll
-        for (int k = 0; k < 3; k++) VEC(f.Mh, k, lev, p) = VEC(f.Mh, k, lev, p) / (1.0 + kv * time_step);
-        T -= kt * time_step * (T - Teq);
+            for (int k = 0; k < 3; k++) VEC(f.Mh, k, lev, p) = VEC(f.Mh, k, lev, p) / (1.0 + kv * time_step);
+            T -= kt * time_step * (T - Teq);
+        } else {
+            double Teq, tau;
+            nr_profile(nr, lon, lat, pre, Teq, tau);
+            T = (T + time_step * Teq / tau) / (1.0 + time_step / tau);
+        }
         nan_t |= isnan(T);
         LV(f.p, lev) = T * Rd * rho;
         if (zero_q) LV(f.Qheat, lev) = 0.0;
     }
     if (nan_t) atomicOr(f.flags, FLAG_NAN_T);
+}
+
+// ------------------------------------------------- double-gray radiation
+struct RTPar {
+    double kappa_sw, kappa_lw, n_sw, n_lw, f_lw, diff_ang, Tint, albedo, incflx, ps0, lon_sub;
+};
+
+// source_func_lin (profx_RT.h:133-160): layer emission with a source linear
+// in optical depth between the layer's bottom, centre and top values.
+__device__ __forceinline__ double rt_source(double bb, double bl, double bt, double tau, double diff_ang) {
+    double e1 = 0.0, e2 = 0.0;
+    if (tau >= 1e-10) {
+        e1 = bb - bl + (bl - (diff_ang / (tau / 2.0)) * (bb - bl)) * (1.0 - MEXP(-(tau / 2.0) / diff_ang));
+        e2 = bl - bt + (bt - (diff_ang / (tau / 2.0)) * (bl - bt)) * (1.0 - MEXP(-(tau / 2.0) / diff_ang));
+    } else {
+        // optically thin layer: the reference's 5-term series; (-1)^(i+2) and
+        // (i+2)! are exact integers
+        double fac = 1.0;
+        for (int i = 0; i < 5; i++) {
+            fac *= (double)(i + 2);
+            const double sgn = (i % 2 == 0) ? 1.0 : -1.0;
+            const double xp = MPOW((tau / 2.0) / diff_ang, i + 1.0);
+            e1 = e1 + sgn * ((bl + (i + 1) * bl) / fac) * xp;
+            e2 = e2 + sgn * ((bt + (i + 1) * bb) / fac) * xp;
+        }
+    }
+    return e1 + e2 * MEXP(-(tau / 2.0) / diff_ang);
+}
+
+// rtm_dual_band (profx_RT.h:296-587) without surface, for one column:
+// interface pressure and temperature, optical depths (computetau), shortwave
+// down with an extra layer above the top (radcsw; no upward shortwave),
+// longwave down and up (radclw), heating = shortwave then longwave flux
+// divergence [W/m3], clipped so p stays >= 0, added to Qheat; and the
+// column's insolation, ASR and OLR. One thread per column. Shortwave and
+// longwave downward fluxes go to two interface planes (fdn_sw, fdn_lw); the
+// upward longwave flux is carried level to level.
+__global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, double time_step, double *insol,
+                                                  double *asr, double *olr, double *fdn_sw, double *fdn_lw) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const double Rd = g.Rd, Cp = g.Cp, Gravit = g.Gravit, A = g.A;
+    const double bc = 5.677036E-8;  // the reference's Stefan-Boltzmann constant (profx_RT.h:178)
+    // Recompute_W and T = p / (rho Rd), as ProfX does before its physics
+    for (int lev = 0; lev < nv; lev++) {
+        const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+        const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+        LV(f.W, lev) = whl * intt + wht * intl;
+        LV(f.T, lev) = LV(f.p, lev) / (Rd * LV(f.Rho, lev));
+    }
+    auto pint = [&](int lev) {  // interface pressure (phtemp)
+        if (lev == 0) {
+            const double psm = LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
+            return 0.5 * (LV(f.p, 0) + psm);
+        }
+        if (lev == nv) {
+            double pp = LV(f.p, nv - 2) + (LV(f.p, nv - 1) - LV(f.p, nv - 2)) / (Alt[nv - 1] - Alt[nv - 2])
+                                              * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+            if (pp < 0) pp = 0;
+            return 0.5 * (LV(f.p, nv - 1) + pp);
+        }
+        const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
+        const double a = (xi - xip) / (xim - xip), b = (xi - xim) / (xip - xim);
+        return LV(f.p, lev - 1) * a + LV(f.p, lev) * b;
+    };
+    auto tint = [&](int lev) {  // interface temperature (thtemp)
+        if (lev == 0) return LV(f.T, 0);
+        if (lev == nv) return LV(f.T, nv - 1);
+        const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
+        const double a = (xi - xip) / (xim - xip), b = (xi - xim) / (xip - xim);
+        return LV(f.T, lev - 1) * a + LV(f.T, lev) * b;
+    };
+    auto tau_sw = [&](int lev) {
+        return q.n_sw * q.kappa_sw * MPOW(LV(f.p, lev) / q.ps0, q.n_sw - 1) * LV(f.Rho, lev)
+               * (Alth[lev + 1] - Alth[lev]);
+    };
+    auto tau_lw = [&](int lev) {
+        const double t = q.kappa_lw * (1 + q.n_lw * (1 - q.f_lw) / q.f_lw * MPOW(LV(f.p, lev) / q.ps0, q.n_lw - 1))
+                         * LV(f.Rho, lev) * (Alth[lev + 1] - Alth[lev]);
+        return t < 0.0 ? 0.0 : t;
+    };
+    auto planck = [&](double t) { return bc * t * t * t * t; };
+    const double lon = g.lonlat[p], lat = g.lonlat[S + p];
+    double coszrs = MCOS(lat) * MCOS(lon - q.lon_sub);
+    if (coszrs < 0.0) coszrs = 0.0;
+    // shortwave, downward
+    double ins = 0.0;
+    const bool day = coszrs > 0.0;
+    if (day) {
+        ins = q.incflx * coszrs;
+        const double flux_top = ins * (1.0 - q.albedo);
+        const double tau = (q.kappa_sw / Gravit) * pint(nv);
+        double fd = flux_top * MEXP(-(1.0 / coszrs) * tau);
+        LV(fdn_sw, nv) = fd;
+        for (int lev = nv; lev >= 1; lev--) {
+            fd = fd * MEXP(-(1.0 / coszrs) * tau_sw(lev - 1));
+            LV(fdn_sw, lev - 1) = fd;
+        }
+    } else {
+        for (int lev = 0; lev <= nv; lev++) LV(fdn_sw, lev) = 0.0;
+    }
+    // longwave, downward
+    double fl = 0.0;
+    LV(fdn_lw, nv) = 0.0;
+    for (int lev = nv - 1; lev >= 0; lev--) {
+        const double tl_ = tau_lw(lev);
+        const double ed = rt_source(planck(tint(lev)), planck(LV(f.T, lev)), planck(tint(lev + 1)), tl_, q.diff_ang);
+        fl = ed + fl * MEXP(-(1. / q.diff_ang) * tl_);
+        LV(fdn_lw, lev) = fl;
+    }
+    // longwave, upward, and the heating of each layer below the new interface
+    double fu_lo = planck(q.Tint);
+    fu_lo += LV(fdn_lw, 0);
+    double fu = fu_lo;
+    for (int lev = 1; lev <= nv; lev++) {
+        const double tl_ = tau_lw(lev - 1);
+        const double eu = rt_source(planck(tint(lev)), planck(LV(f.T, lev - 1)), planck(tint(lev - 1)), tl_, q.diff_ang);
+        fu = eu + fu_lo * MEXP(-(1. / q.diff_ang) * tl_);
+        const int l = lev - 1;
+        double dtemp = 0.0;
+        if (day)
+            dtemp = -((0.0 - LV(fdn_sw, l)) - (0.0 - LV(fdn_sw, l + 1))) / ((Alth[l] - Alth[l + 1]));
+        dtemp = dtemp - ((fu_lo - LV(fdn_lw, l)) - (fu - LV(fdn_lw, l + 1))) / ((Alth[l] - Alth[l + 1]));
+        const double pr = LV(f.p, l);
+        if (pr + Rd / (Cp - Rd) * dtemp * time_step < 0) dtemp = -pr / time_step;
+        LV(f.Qheat, l) = dtemp;  // Qheat is cleared at the start of ProfX, then the module adds
+        fu_lo = fu;
+    }
+    // Compute_pressure (profx_auxiliary.h:95-108) closes ProfX
+    for (int lev = 0; lev < nv; lev++) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
+    insol[p] = ins;
+    const double rscale = g.deep ? (A + Alth[nv]) / A : 1.0;
+    asr[p] = LV(fdn_sw, nv) * g.areasT[p] * MPOW(rscale, 2);
+    olr[p] = fu * g.areasT[p] * MPOW(rscale, 2);
 }
 
 // ================================================================== globdiag

@@ -17,6 +17,7 @@
 #endif
 
 #include <cstdint>
+#include <cmath>
 
 namespace pm {
 
@@ -39,8 +40,15 @@ constexpr double INV_LN2 = 1.44269504088896338700e+00;
 // log m = 2 atanh(s), s = (m - 1) / (m + 1) (|s| <= 0.172), by the odd power
 // series of atanh in Horner form (terms to s^27).
 PM_FN double log(double x) {
+    if (!(x > 0.0)) return x == 0.0 ? -1.0 / 0.0 : 0.0 / 0.0;  // log 0 = -inf; negative or NaN -> NaN
+    if (x == 1.0 / 0.0) return x;
+    int scale = 0;
+    if (x < 2.2250738585072014e-308) {  // subnormal: normalise
+        x *= 18014398509481984.0;        // 2^54
+        scale = -54;
+    }
     uint64_t u = bits(x);
-    int e = (int)((u >> 52) & 0x7ff) - 1023;
+    int e = (int)((u >> 52) & 0x7ff) - 1023 + scale;
     double m = from_bits((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  // [1, 2)
     if (m > 1.4142135623730951) {
         m = m * 0.5;
@@ -70,6 +78,9 @@ PM_FN double log(double x) {
 // e^x for |x| < 700: x = n ln2 + r, |r| <= ln2 / 2, e^r by its Taylor series
 // in Horner form (terms to r^17).
 PM_FN double exp(double x) {
+    if (!(x == x)) return x;                            // NaN
+    if (x > 709.782712893384) return 1.0 / 0.0;         // overflow
+    if (x < -745.1332191019412) return 0.0;             // below the smallest subnormal
     const double t = x * INV_LN2;
     const double n = (double)(int64_t)(t < 0 ? t - 0.5 : t + 0.5);
     const double r = (x - n * LN2_HI) - n * LN2_LO;
@@ -91,6 +102,10 @@ PM_FN double exp(double x) {
     q = q * r + 0.5;
     const double sum = 1.0 + (r + r * (r * q));
     const int64_t ni = (int64_t)n;
+    if (ni > 1023)  // near the largest double: 2^ni is not representable, scale in two steps
+        return (sum * from_bits((uint64_t)(ni - 1 + 1023) << 52)) * 2.0;
+    if (ni < -1021)  // subnormal result: scale in two steps so both factors are normal
+        return (sum * from_bits((uint64_t)(ni + 54 + 1023) << 52)) * 5.551115123125783e-17;  // 2^-54
     return sum * from_bits((uint64_t)(ni + 1023) << 52);
 }
 
@@ -105,8 +120,16 @@ PM_FN double pow(double x, double y) {
     return exp(y * log(x));
 }
 
-// sin and cos for |x| <= 4 by their Taylor series.
+// Reduces x to [-pi, pi] (x unchanged when already there).
+PM_FN double reduce_2pi(double x) {
+    const double twopi = 6.283185307179586;
+    const double n = std::rint(x / twopi);
+    return n == 0.0 ? x : x - n * twopi;
+}
+
+// sin and cos by their Taylor series after reduction to [-pi, pi].
 PM_FN double sin(double x) {
+    x = reduce_2pi(x);
     const double x2 = x * x;
     double term = x, sum = x;
     for (int k = 1; k <= 14; k++) {
@@ -116,6 +139,7 @@ PM_FN double sin(double x) {
     return sum;
 }
 PM_FN double cos(double x) {
+    x = reduce_2pi(x);
     const double x2 = x * x;
     double term = 1.0, sum = 1.0;
     for (int k = 1; k <= 14; k++) {
@@ -134,9 +158,11 @@ PM_FN double cos(double x) {
 #define MLOG(x) pm::log(x)
 #define MSIN(x) pm::sin(x)
 #define MCOS(x) pm::cos(x)
+#define MEXP(x) pm::exp(x)
 #else
 #define MPOW(x, y) pow((x), (y))
 #define MLOG(x) log(x)
 #define MSIN(x) sin(x)
 #define MCOS(x) cos(x)
+#define MEXP(x) exp(x)
 #endif
