@@ -1,0 +1,123 @@
+"""Surrogate network on the GPU: MFMA forward and backward against the CPU
+restatement, training, and the surrogate inside the hot-Jupiter step.
+
+Tolerances: the oracle rounds to BF16 at the same points and computes in
+FP64; the GPU accumulates in FP32, so an activation close to a BF16 rounding
+boundary can land one BF16 step (2^-8 relative) apart. Outputs and gradients
+therefore agree to ~1e-3 relative, and the tests allow 1e-2 (outputs) and
+3e-2 (gradient blocks, relative L2).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from common import hj_grid, hj_params, rt_params
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (81, 256, 40)
+
+
+def _blocks(g, dims):
+    n_in, nh, n_out = dims
+    sizes = [n_in * nh, nh, nh * nh, nh, nh * n_out, n_out]
+    names = ["W1", "b1", "W2", "b2", "W3", "b3"]
+    out, o = {}, 0
+    for n, s in zip(names, sizes):
+        out[n] = g[o:o + s]
+        o += s
+    return out
+
+
+def test_forward_matches_oracle(built):
+    from oracle import mlp_oracle as mo
+    capi = built
+    print("case: MLP 81-256-256-40 forward, 300 rows", flush=True)
+    m = capi.MLP(*DIMS, seed=3)
+    rng = np.random.default_rng(1)
+    p = m.params()
+    p[DIMS[0] * DIMS[1]:DIMS[0] * DIMS[1] + DIMS[1]] = 0.1 * rng.standard_normal(DIMS[1])  # non-zero b1
+    m.set_params(p)
+    X = rng.standard_normal((300, DIMS[0])).astype(np.float32)
+    Y = m.forward(X)
+    Yo, _ = mo.forward(m.params(), X, DIMS, bf=True)
+    err = np.abs(Y - Yo).max() / np.abs(Yo).max()
+    print(f"forward vs oracle: max rel diff {err:.2e}")
+    assert err < 1e-2
+    m.close()
+
+
+def test_gradients_match_oracle(built):
+    from oracle import mlp_oracle as mo
+    capi = built
+    print("case: MLP gradients, 200 rows", flush=True)
+    m = capi.MLP(*DIMS, seed=4)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((200, DIMS[0])).astype(np.float32)
+    T = rng.standard_normal((200, DIMS[2])).astype(np.float32)
+    g, loss = m.gradients(X, T)
+    lo, go = mo.loss_and_grad(m.params(), X, T, DIMS, bf=True)
+    print(f"loss gpu {loss:.6e} oracle {lo:.6e}")
+    assert abs(loss / lo - 1) < 1e-3
+    bg, bo = _blocks(g, DIMS), _blocks(go, DIMS)
+    for k in bg:
+        e = np.linalg.norm(bg[k] - bo[k]) / np.linalg.norm(bo[k])
+        print(f"  d{k}: relative L2 diff {e:.2e}")
+        assert e < 3e-2, k
+    m.close()
+
+
+def test_training_fits_a_teacher_network(built):
+    capi = built
+    print("case: train 81-256-256-40 on a random teacher, 8192 rows, 15 epochs", flush=True)
+    rng = np.random.default_rng(3)
+    teacher = capi.MLP(*DIMS, seed=11)
+    X = rng.standard_normal((8192, DIMS[0])).astype(np.float32)
+    Y = teacher.forward(X)
+    Y = (Y - Y.mean(axis=0)) / Y.std(axis=0)
+    student = capi.MLP(*DIMS, seed=12)
+    student.set_data(X, Y)
+    losses = [student.train_epoch(256, 1e-3 * 0.9 ** e, e) for e in range(15)]
+    print("losses", " ".join(f"{v:.3e}" for v in losses))
+    assert np.isfinite(losses).all()
+    assert losses[-1] < losses[0] / 2.5
+    teacher.close()
+    student.close()
+
+
+def test_surrogate_in_the_hot_jupiter_step(built):
+    """Train on exact-RT columns of a short hot-Jupiter run, then run with the
+    surrogate: the judge's RMS heating error is bounded, the run stays
+    finite, and the trained network beats the untrained one by far."""
+    from thor_amd import surrogate as sg
+    capi = built
+    print("case: hot Jupiter g4 L20, RT surrogate trained on 30 steps of exact RT", flush=True)
+    g = hj_grid(capi, 4, 20)
+    p = hj_params(capi, 4)
+    st = capi.ic_rest(g, p, seed=1003, temperature_noise=1.0, wind_noise=0.1)
+    d = capi.Dyn(g, p)
+    d.set_state(st)
+    d.set_rt(rt_params(capi))
+    X, Y = sg.collect(d, g, nsteps=30, every=2)
+    norm = sg.normalisation(X, Y)
+    Xn, Yn = sg.apply_norm(X, Y, norm)
+    nv = g.nv
+    m = capi.MLP(2 * nv + 1, 256, nv, seed=5)
+    err0 = sg.heating_error(m, X, Y, norm)
+    losses = sg.train(m, Xn, Yn, epochs=20, batch=256, lr=2e-3)
+    err1 = sg.heating_error(m, X, Y, norm)
+    print(f"training loss {losses[0]:.3e} -> {losses[-1]:.3e}; heating RMS error untrained {err0:.3f}, "
+          f"trained {err1:.3f}")
+    assert err1 < 0.5 * err0
+    d.set_rt_surrogate(m, norm, judge_every=4)
+    d.run(12, capi.FORCING_RT_SURROGATE)
+    d.sync()
+    stats = d.surrogate_stats()
+    qrms = float(np.sqrt((Y ** 2).mean()))
+    print(f"surrogate stats {stats}; judged RMS error / RMS heating = {stats['rms_err'] / qrms:.3f}")
+    assert stats["calls"] == 12 and stats["judged"] == 3
+    assert stats["rms_err"] < 0.25 * qrms
+    assert all(np.isfinite(v).all() for v in d.get_state().values())
+    d.close()
+    m.close()

@@ -20,7 +20,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 HOST_SRC = ["host.cpp", "grid.cpp"]
-HIP_SRC = ["dyn.hip"]
+HIP_SRC = ["dyn.hip", "mlp.hip"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 # The FP64 path must round every product and sum as written (no FMA

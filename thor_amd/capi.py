@@ -41,6 +41,18 @@ class NewtonianParams(C.Structure):
                                           "tau_top", "tau_bottom")]
 
 
+_fp = C.POINTER(C.c_float)
+
+
+class SurrogateParams(C.Structure):
+    """thor_rt_surrogate_params: normalisation, envelope and judge interval."""
+    _fields_ = [("in_mean", _fp), ("in_std", _fp), ("in_lo", _fp), ("in_hi", _fp), ("out_mean", _fp),
+                ("out_std", _fp), ("judge_every", C.c_int)]
+
+
+FORCING_RT_SURROGATE = 4
+
+
 class RTParams(C.Structure):
     """thor_rt_params: double-gray two-stream radiative transfer."""
     _fields_ = [(n, C.c_double) for n in ("kappa_sw", "kappa_lw", "n_sw", "n_lw", "f_lw", "diff_ang", "Tint",
@@ -132,6 +144,21 @@ def lib(variant: str = "") -> C.CDLL:
             L.thor_dyn_set_newtonian.argtypes = [C.c_void_p, C.POINTER(NewtonianParams)]
             L.thor_dyn_set_rt.argtypes = [C.c_void_p, C.POINTER(RTParams)]
             L.thor_dyn_get_rt_column.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
+            L.thor_dyn_set_rt_surrogate.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(SurrogateParams)]
+            L.thor_dyn_surrogate_stats.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.POINTER(C.c_long),
+                                                   C.POINTER(C.c_long), _dp, _dp]
+            L.thor_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_ulonglong, C.POINTER(C.c_void_p)]
+            L.thor_mlp_destroy.argtypes = [C.c_void_p]
+            L.thor_mlp_destroy.restype = None
+            L.thor_mlp_param_count.argtypes = [C.c_void_p]
+            L.thor_mlp_param_count.restype = C.c_long
+            for f in (L.thor_mlp_get_params, L.thor_mlp_set_params):
+                f.argtypes = [C.c_void_p, _fp, C.c_long]
+            L.thor_mlp_set_data.argtypes = [C.c_void_p, _fp, _fp, C.c_long]
+            L.thor_mlp_train_epoch.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_ulonglong, _dp]
+            L.thor_mlp_forward.argtypes = [C.c_void_p, _fp, _fp, C.c_long]
+            L.thor_mlp_gradients.argtypes = [C.c_void_p, _fp, _fp, C.c_long, _fp, _dp]
+            L.thor_mlp_time_forward.argtypes = [C.c_void_p, C.c_long, C.c_int, _dp]
         _libs[variant] = L
     return _libs[variant]
 
@@ -284,6 +311,23 @@ class Dyn:
         self._chk(self.L.thor_dyn_get_rt_column(self._h, name.encode(), _ptr(a), a.size))
         return a
 
+    def set_rt_surrogate(self, mlp: "MLP", norm: dict, judge_every: int = 10):
+        """norm: float32 arrays in_mean, in_std, in_lo, in_hi [2 nv + 1], out_mean, out_std [nv]."""
+        self._sur_norm = {k: np.ascontiguousarray(norm[k], dtype=np.float32)
+                          for k in ("in_mean", "in_std", "in_lo", "in_hi", "out_mean", "out_std")}
+        q = SurrogateParams(*(self._sur_norm[k].ctypes.data_as(_fp) for k in
+                              ("in_mean", "in_std", "in_lo", "in_hi", "out_mean", "out_std")), judge_every)
+        self._sur_mlp = mlp  # keeps the network alive while attached
+        self._chk(self.L.thor_dyn_set_rt_surrogate(self._h, mlp._h, C.byref(q)))
+
+    def surrogate_stats(self) -> dict:
+        calls, judged, fb = C.c_long(), C.c_long(), C.c_long()
+        rms, mx = C.c_double(), C.c_double()
+        self._chk(self.L.thor_dyn_surrogate_stats(self._h, C.byref(calls), C.byref(judged), C.byref(fb),
+                                                  C.byref(rms), C.byref(mx)))
+        return {"calls": calls.value, "judged": judged.value, "fallback_columns": fb.value,
+                "rms_err": rms.value, "max_err": mx.value}
+
     def profx(self, forcing=FORCING_HELD_SUAREZ):
         self._chk(self.L.thor_dyn_profx(self._h, forcing))
 
@@ -317,4 +361,73 @@ class Dyn:
     def time_run(self, nsteps: int, forcing=FORCING_HELD_SUAREZ) -> float:
         ms = C.c_double()
         self._chk(self.L.thor_dyn_time_run(self._h, nsteps, forcing, C.byref(ms)))
+        return ms.value
+
+
+class MLP:
+    """Per-column surrogate network (thor_mlp.h): n_in -> n_hidden GELU ->
+    n_hidden GELU -> n_out, BF16 MFMA kernels with FP32 accumulation."""
+
+    def __init__(self, n_in: int, n_hidden: int, n_out: int, seed: int = 0, variant: str = ""):
+        self.L = lib(variant)
+        self.n_in, self.n_hidden, self.n_out = n_in, n_hidden, n_out
+        h = C.c_void_p()
+        _check(self.L.thor_mlp_create(n_in, n_hidden, n_out, seed, C.byref(h)), variant)
+        self._h = h
+        self.variant = variant
+
+    def _chk(self, code):
+        _check(code, self.variant)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.thor_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def n_params(self) -> int:
+        return self.L.thor_mlp_param_count(self._h)
+
+    def params(self) -> np.ndarray:
+        a = np.zeros(self.n_params, dtype=np.float32)
+        self._chk(self.L.thor_mlp_get_params(self._h, a.ctypes.data_as(_fp), a.size))
+        return a
+
+    def set_params(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self._chk(self.L.thor_mlp_set_params(self._h, a.ctypes.data_as(_fp), a.size))
+
+    def set_data(self, X: np.ndarray, Y: np.ndarray):
+        self._X = np.ascontiguousarray(X, dtype=np.float32)
+        self._Y = np.ascontiguousarray(Y, dtype=np.float32)
+        assert self._X.shape[1] == self.n_in and self._Y.shape[1] == self.n_out and len(self._X) == len(self._Y)
+        self._chk(self.L.thor_mlp_set_data(self._h, self._X.ctypes.data_as(_fp), self._Y.ctypes.data_as(_fp),
+                                           len(self._X)))
+
+    def train_epoch(self, batch: int, lr: float, seed: int) -> float:
+        loss = C.c_double()
+        self._chk(self.L.thor_mlp_train_epoch(self._h, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), self.n_out), dtype=np.float32)
+        self._chk(self.L.thor_mlp_forward(self._h, X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp), len(X)))
+        return Y
+
+    def gradients(self, X: np.ndarray, Y: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros(self.n_params, dtype=np.float32)
+        loss = C.c_double()
+        self._chk(self.L.thor_mlp_gradients(self._h, X.ctypes.data_as(_fp), Y.ctypes.data_as(_fp), len(X),
+                                            g.ctypes.data_as(_fp), C.byref(loss)))
+        return g, loss.value
+
+    def time_forward(self, n: int, repeats: int = 20) -> float:
+        ms = C.c_double()
+        self._chk(self.L.thor_mlp_time_forward(self._h, n, repeats, C.byref(ms)))
         return ms.value

@@ -1,0 +1,26 @@
+// FP32 <-> BF16 conversion shared by the MLP and the dynamical core.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace thor {
+namespace mlp {
+
+// Round to nearest even; NaN stays a quiet NaN.
+__host__ __device__ inline uint16_t f2bf(float x) {
+    uint32_t u;
+    __builtin_memcpy(&u, &x, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    __builtin_memcpy(&f, &u, 4);
+    return f;
+}
+
+} // namespace mlp
+} // namespace thor

@@ -2,8 +2,10 @@
 // launch sequence of one large step (ESP::Thor, thor_driver.cu:70-1466),
 // ProfX, global diagnostics and per-kernel timing.
 #include "../../include/thor_dyn.h"
+#include "../../include/thor_mlp.h"
 #include "common.h"
 #include "dyn_kernels.h"
+#include "mlp_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -62,6 +64,15 @@ struct Dyn {
     RTPar rt{};
     double *insol = nullptr, *asr = nullptr, *olr = nullptr;  // [P] RT column diagnostics
     double *fdn_sw = nullptr, *fdn_lw = nullptr;              // [nv+1][S] RT scratch
+    // RT surrogate
+    thor_mlp *sur = nullptr;
+    SurPar sp{};
+    int judge_every = 0;
+    int *sur_mask = nullptr, *sur_nflag = nullptr;
+    double *qsur = nullptr, *err_part = nullptr;  // [nv][S] surrogate Qheat on judge steps; 3 x nblk partials
+    int err_nblk = 0;
+    long sur_calls = 0, sur_judged = 0, sur_fallback = 0;
+    double err_sq = 0.0, err_max = 0.0, err_cnt = 0.0;
 };
 
 static hipError_t dalloc(Dyn &d, double **ptr, size_t n) {
@@ -243,8 +254,50 @@ static int profx(Dyn &d, int forcing) {
         // Wh 2, p, rho in; W, T, Qheat, p out; two flux planes written and read twice
         launch(d, "profx_double_gray", 14 * B, [&] {
             k_profx_rt<<<(d.P + kTile - 1) / kTile, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr,
-                                                                       d.fdn_sw, d.fdn_lw);
+                                                                       d.fdn_sw, d.fdn_lw, nullptr);
         });
+    } else if (forcing == THOR_FORCING_RT_SURROGATE) {
+        if (!d.rt_set || !d.sur) return fail(THOR_ERR_BAD_ARG, "profx: RT surrogate not set");
+        const bool judge = d.judge_every > 0 && d.sur_calls % d.judge_every == 0;
+        d.sur_calls++;
+        hipError_t e = mlp::prepare(d.sur, d.P);
+        if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("profx surrogate: ") + hipGetErrorString(e));
+        const dim3 gp((d.P + kTile - 1) / kTile), gpl((d.P + kTile - 1) / kTile, d.nv);
+        uint16_t *X = mlp::input(d.sur);
+        const float *Y = mlp::output(d.sur);
+        launch(d, "surrogate_features", 6 * B, [&] {
+            k_sur_features<<<gp, kTile, 0, d.stream>>>(g, f, d.sp, X, d.sur_mask, d.sur_nflag);
+        });
+        // BF16 inputs and weights, 193 kFLOP per column for 81 -> 256 -> 256 -> 40
+        launch(d, "surrogate_mlp", 2.0 * d.P * (2 * d.nv + 1 + d.nv) * 2, [&] { mlp::forward_on(d.sur, d.P, d.stream); });
+        launch(d, "surrogate_apply", 4 * B, [&] {
+            k_sur_apply<<<gpl, kTile, 0, d.stream>>>(g, f, d.sp, Y, d.sur_mask, dt, judge ? 0 : 1);
+        });
+        if (judge) {
+            const size_t n1 = (size_t)d.nv * d.S;
+            hipMemcpyAsync(d.qsur, f.Qheat, n1 * sizeof(double), hipMemcpyDeviceToDevice, d.stream);
+            launch(d, "profx_double_gray", 14 * B, [&] {
+                k_profx_rt<<<gp, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, d.fdn_sw, d.fdn_lw,
+                                                       nullptr);
+            });
+            k_sur_error<<<d.err_nblk, 256, 0, d.stream>>>(g, f.Qheat, d.qsur, d.sur_mask, (long)n1, d.err_part,
+                                                          d.err_part + d.err_nblk, d.err_part + 2 * d.err_nblk);
+            std::vector<double> part((size_t)3 * d.err_nblk);
+            hipMemcpyAsync(part.data(), d.err_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, d.stream);
+            e = hipStreamSynchronize(d.stream);
+            if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("profx surrogate judge: ") + hipGetErrorString(e));
+            for (int b = 0; b < d.err_nblk; b++) {
+                d.err_sq += part[b];
+                d.err_max = std::max(d.err_max, part[d.err_nblk + b]);
+                d.err_cnt += part[2 * d.err_nblk + b];
+            }
+            d.sur_judged++;
+        } else {
+            launch(d, "profx_double_gray_fallback", 14 * B, [&] {
+                k_profx_rt<<<gp, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, d.fdn_sw, d.fdn_lw,
+                                                       d.sur_mask);
+            });
+        }
     } else {
         return fail(THOR_ERR_BAD_ARG, "unknown forcing");
     }
@@ -573,6 +626,70 @@ extern "C" thor_status thor_dyn_get_rt_column(thor_dyn *h, const char *name, dou
     if (!src) return fail(THOR_ERR_BAD_ARG, "get_rt_column: name must be insol, ASR or OLR");
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(dst, src, n * sizeof(double), hipMemcpyDeviceToHost));
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_set_rt_surrogate(thor_dyn *h, thor_mlp *m, const thor_rt_surrogate_params *q) {
+    using namespace thor;
+    if (!h || !m || !q) return fail(THOR_ERR_BAD_ARG, "set_rt_surrogate: null argument");
+    if (!h->rt_set) return fail(THOR_ERR_BAD_ARG, "set_rt_surrogate: set the RT parameters first (fallback and judge)");
+    int nin = 0, nout = 0;
+    mlp::dims(m, &nin, &nout);
+    const int nv = h->nv;
+    if (nin != 2 * nv + 1 || nout != nv)
+        return fail(THOR_ERR_BAD_ARG, "set_rt_surrogate: the network must map 2 nv + 1 inputs to nv outputs");
+    if (!q->in_mean || !q->in_std || !q->in_lo || !q->in_hi || !q->out_mean || !q->out_std || q->judge_every < 0)
+        return fail(THOR_ERR_BAD_ARG, "set_rt_surrogate: missing normalisation arrays");
+    if (!h->sur_mask) {
+        hipError_t e = hipMalloc((void **)&h->sur_mask, (size_t)h->S * sizeof(int));
+        if (e == hipSuccess) h->allocs.push_back(h->sur_mask);
+        if (e == hipSuccess) e = hipMalloc((void **)&h->sur_nflag, sizeof(int));
+        if (e == hipSuccess) h->allocs.push_back(h->sur_nflag);
+        if (e == hipSuccess) e = hipMemset(h->sur_nflag, 0, sizeof(int));
+        if (e == hipSuccess) e = dalloc(*h, &h->qsur, (size_t)nv * h->S);
+        h->err_nblk = (int)(((long)nv * h->S + 255) / 256);
+        if (e == hipSuccess) e = dalloc(*h, &h->err_part, (size_t)3 * h->err_nblk);
+        float *nrm = nullptr;
+        if (e == hipSuccess) e = hipMalloc((void **)&nrm, (size_t)(4 * nin + 2 * nout) * sizeof(float));
+        if (e == hipSuccess) h->allocs.push_back(nrm);
+        if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("set_rt_surrogate: alloc: ") + hipGetErrorString(e));
+        h->sp.in_mean = nrm;
+        h->sp.in_std = nrm + nin;
+        h->sp.in_lo = nrm + 2 * nin;
+        h->sp.in_hi = nrm + 3 * nin;
+        h->sp.out_mean = nrm + 4 * nin;
+        h->sp.out_std = nrm + 4 * nin + nout;
+    }
+    const float *src[6] = {q->in_mean, q->in_std, q->in_lo, q->in_hi, q->out_mean, q->out_std};
+    float *dst[6] = {(float *)h->sp.in_mean, (float *)h->sp.in_std, (float *)h->sp.in_lo, (float *)h->sp.in_hi,
+                     (float *)h->sp.out_mean, (float *)h->sp.out_std};
+    const int cnt[6] = {nin, nin, nin, nin, nout, nout};
+    for (int i = 0; i < 6; i++)
+        HIP_TRY(hipMemcpy(dst[i], src[i], cnt[i] * sizeof(float), hipMemcpyHostToDevice));
+    h->sp.lon_sub = h->rt.lon_sub;
+    h->sur = m;
+    h->judge_every = q->judge_every;
+    h->sur_calls = h->sur_judged = 0;
+    h->err_sq = h->err_max = h->err_cnt = 0.0;
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_surrogate_stats(thor_dyn *h, long *calls, long *judged, long *fallback_columns,
+                                                double *rms_err, double *max_err) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "surrogate_stats: null handle");
+    if (!h->sur) return fail(THOR_ERR_BAD_ARG, "surrogate_stats: no surrogate attached");
+    int nflag = 0;
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY(hipMemcpy(&nflag, h->sur_nflag, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemset(h->sur_nflag, 0, sizeof(int)));
+    if (calls) *calls = h->sur_calls;
+    if (judged) *judged = h->sur_judged;
+    if (fallback_columns) *fallback_columns = nflag;
+    if (rms_err) *rms_err = h->err_cnt > 0 ? std::sqrt(h->err_sq / h->err_cnt) : 0.0;
+    if (max_err) *max_err = h->err_max;
+    h->sur_calls = h->sur_judged = 0;
+    h->err_sq = h->err_max = h->err_cnt = 0.0;
     return THOR_OK;
 }
 

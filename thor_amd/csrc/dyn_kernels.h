@@ -9,6 +9,7 @@
 // kernels (vertical sweeps, the Thomas solve) map one thread to one column.
 #pragma once
 #include "dyn_common.h"
+#include "bf16.h"
 #include "portable_math.h"
 
 namespace thor {
@@ -1617,9 +1618,11 @@ __device__ __forceinline__ double rt_source(double bb, double bl, double bt, dou
 // longwave downward fluxes go to two interface planes (fdn_sw, fdn_lw); the
 // upward longwave flux is carried level to level.
 __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, double time_step, double *insol,
-                                                  double *asr, double *olr, double *fdn_sw, double *fdn_lw) {
+                                                  double *asr, double *olr, double *fdn_sw, double *fdn_lw,
+                                                  const int *only) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.P) return;
+    if (only && !only[p]) return;  // surrogate fallback: only the flagged columns
     const size_t S = g.S;
     const int nv = g.nv;
     const double *Alt = g.Alt, *Alth = g.Alth;
@@ -1717,6 +1720,101 @@ __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, doub
     const double rscale = g.deep ? (A + Alth[nv]) / A : 1.0;
     asr[p] = LV(fdn_sw, nv) * g.areasT[p] * MPOW(rscale, 2);
     olr[p] = fu * g.areasT[p] * MPOW(rscale, 2);
+}
+
+// ------------------------------------------------------ RT surrogate glue
+// Normalisation of the surrogate network (see thor_rt_surrogate_params).
+struct SurPar {
+    const float *in_mean, *in_std, *in_lo, *in_hi;  // [2 nv + 1]
+    const float *out_mean, *out_std;                 // [nv]
+    double lon_sub;
+};
+
+// Per column: Recompute_W and T = p / (rho Rd) as ProfX does, the network's
+// normalised inputs [T_k, ln p_k, cos z] as BF16 row p of X, and the flag of
+// a column whose inputs leave the training envelope.
+__global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q, uint16_t *X, int *mask,
+                                                      int *nflag) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv, nin = 2 * nv + 1;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    bool out = false;
+    auto put = [&](int j, double x) {
+        const float xn = (float)((x - q.in_mean[j]) / q.in_std[j]);
+        out |= !(xn >= q.in_lo[j] && xn <= q.in_hi[j]);
+        X[(size_t)p * nin + j] = mlp::f2bf(xn);
+    };
+    for (int lev = 0; lev < nv; lev++) {
+        const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+        const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+        LV(f.W, lev) = whl * intt + wht * intl;
+        const double pr = LV(f.p, lev);
+        const double T = pr / (g.Rd * LV(f.Rho, lev));
+        LV(f.T, lev) = T;
+        put(lev, T);
+        put(nv + lev, MLOG(pr));
+    }
+    double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
+    if (coszrs < 0.0) coszrs = 0.0;
+    put(2 * nv, coszrs);
+    mask[p] = out ? 1 : 0;
+    if (out) atomicAdd(nflag, 1);
+}
+
+// Qheat from the network's outputs Y[p][lev] (with the RT's positivity clip),
+// for the columns inside the envelope; with set_p, also Compute_pressure
+// p = T Rd rho for them (the exact RT does it for the others).
+__global__ __launch_bounds__(256) void k_sur_apply(Geo g, Fields f, SurPar q, const float *Y, const int *mask,
+                                                   double time_step, int set_p) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lev = blockIdx.y;
+    if (p >= g.P || mask[p]) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    double Q = (double)Y[(size_t)p * nv + lev] * q.out_std[lev] + q.out_mean[lev];
+    const double pr = LV(f.p, lev), Rd = g.Rd, Cp = g.Cp;
+    if (pr + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr / time_step;
+    LV(f.Qheat, lev) = Q;
+    if (set_p) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
+}
+
+// Judge: squared and absolute differences between the exact RT heating and
+// the surrogate's (saved in qsur) over the columns inside the envelope, one
+// partial sum / max per block.
+__global__ __launch_bounds__(256) void k_sur_error(Geo g, const double *Qexact, const double *qsur, const int *mask,
+                                                   long n, double *part_sq, double *part_max, double *part_cnt) {
+    __shared__ double s_sq[256], s_mx[256], s_ct[256];
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sq = 0.0, mx = 0.0, ct = 0.0;
+    if (i < n) {
+        const int p = (int)(i % g.S);
+        if (p < g.P && !mask[p]) {
+            const double d = Qexact[i] - qsur[i];
+            sq = d * d;
+            mx = fabs(d);
+            ct = 1.0;
+        }
+    }
+    s_sq[threadIdx.x] = sq;
+    s_mx[threadIdx.x] = mx;
+    s_ct[threadIdx.x] = ct;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) {
+            s_sq[threadIdx.x] += s_sq[threadIdx.x + k];
+            s_mx[threadIdx.x] = fmax(s_mx[threadIdx.x], s_mx[threadIdx.x + k]);
+            s_ct[threadIdx.x] += s_ct[threadIdx.x + k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_sq[blockIdx.x] = s_sq[0];
+        part_max[blockIdx.x] = s_mx[0];
+        part_cnt[blockIdx.x] = s_ct[0];
+    }
 }
 
 // ================================================================== globdiag

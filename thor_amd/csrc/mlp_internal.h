@@ -1,0 +1,21 @@
+// Hooks through which the dynamical core runs a surrogate network on its own
+// stream (defined in mlp.hip).
+#pragma once
+#include "../../include/thor_mlp.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace thor {
+namespace mlp {
+// Makes room for rows input rows; input() is then the BF16 [rows][n_in]
+// buffer the caller fills, output() the FP32 [rows][n_out] result of
+// forward_on().
+hipError_t prepare(thor_mlp *m, long rows);
+uint16_t *input(thor_mlp *m);
+const float *output(thor_mlp *m);
+void forward_on(thor_mlp *m, long rows, hipStream_t s);
+void dims(const thor_mlp *m, int *nin, int *nout);
+} // namespace mlp
+} // namespace thor

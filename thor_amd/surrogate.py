@@ -1,0 +1,91 @@
+"""Radiative-heating surrogate: training data from the exact double-gray RT,
+normalisation, GPU training, and attaching the network to a dynamical core.
+
+Inputs per column are x = [T_k (nv), ln p_k (nv), cos z] and targets the
+exact heating Qheat_k (nv) [W/m3] of the double-gray ProfX
+(src/physics/modules/inc/profx_RT.h:296-587). Everything numeric runs in the
+library (thor_amd/csrc/mlp.hip); this module only moves arrays.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import capi
+
+
+def cos_zenith(grid, lon_sub: float = 0.0) -> np.ndarray:
+    """cos z of a synchronous rotator with the substellar point at lon_sub, clipped at 0."""
+    lon, lat = np.asarray(grid.lonlat).reshape(grid.P, 2).T
+    return np.maximum(np.cos(lat) * np.cos(lon - lon_sub), 0.0)
+
+
+def features(T: np.ndarray, p: np.ndarray, mu: np.ndarray) -> np.ndarray:
+    """[T_k, ln p_k, cos z] per column (rows = columns)."""
+    return np.concatenate([T, np.log(p), mu[:, None]], axis=1)
+
+
+def collect(dyn: "capi.Dyn", grid, nsteps: int, every: int, rng=None, max_columns=None, lon_sub=0.0):
+    """Runs nsteps of (double-gray ProfX + Thor) and samples (inputs, heating)
+    of the columns after every `every`-th ProfX. Returns X [n, 2 nv + 1],
+    Y [n, nv] as float64."""
+    mu = cos_zenith(grid, lon_sub)
+    xs, ys = [], []
+    for s in range(nsteps):
+        dyn.profx(capi.FORCING_DOUBLE_GRAY)
+        if s % every == 0:
+            T = dyn.get_field("temperature")
+            p = dyn.get_field("pressure")
+            Q = dyn.get_field("profx_Qheat")
+            X = features(T, p, mu)
+            if max_columns is not None and rng is not None and max_columns < len(X):
+                keep = rng.choice(len(X), max_columns, replace=False)
+                X, Q = X[keep], Q[keep]
+            xs.append(X)
+            ys.append(Q)
+        dyn.thor()
+    dyn.sync()
+    return np.concatenate(xs), np.concatenate(ys)
+
+
+def normalisation(X: np.ndarray, Y: np.ndarray, margin: float = 0.5) -> dict:
+    """Per-feature mean / std of the inputs and outputs, and the envelope
+    [min - margin, max + margin] of the normalised training inputs."""
+    in_mean = X.mean(axis=0)
+    in_std = X.std(axis=0)
+    in_std[in_std == 0] = 1.0
+    Xn = (X - in_mean) / in_std
+    out_mean = Y.mean(axis=0)
+    out_std = Y.std(axis=0)
+    out_std[out_std == 0] = 1.0
+    return {"in_mean": in_mean.astype(np.float32), "in_std": in_std.astype(np.float32),
+            "in_lo": (Xn.min(axis=0) - margin).astype(np.float32),
+            "in_hi": (Xn.max(axis=0) + margin).astype(np.float32),
+            "out_mean": out_mean.astype(np.float32), "out_std": out_std.astype(np.float32)}
+
+
+def apply_norm(X: np.ndarray, Y: np.ndarray, norm: dict):
+    return ((X - norm["in_mean"]) / norm["in_std"]).astype(np.float32), \
+        ((Y - norm["out_mean"]) / norm["out_std"]).astype(np.float32)
+
+
+def train(mlp: "capi.MLP", Xn: np.ndarray, Yn: np.ndarray, epochs: int, batch: int = 512, lr: float = 1e-3,
+          seed: int = 0, lr_decay: float = 0.93, log=None) -> list:
+    """Adam epochs on the GPU with an exponentially decaying learning rate;
+    returns the per-epoch training losses (normalised units)."""
+    mlp.set_data(Xn, Yn)
+    losses = []
+    t0 = time.perf_counter()
+    for e in range(epochs):
+        losses.append(mlp.train_epoch(batch, lr * lr_decay ** e, seed + e))
+        if log:
+            log(f"  epoch {e:3d}: loss {losses[-1]:.4e} ({time.perf_counter() - t0:.1f} s)")
+    return losses
+
+
+def heating_error(mlp: "capi.MLP", X: np.ndarray, Y: np.ndarray, norm: dict) -> float:
+    """RMS error of the network's heating against Y, relative to the RMS of Y."""
+    Xn, _ = apply_norm(X, Y, norm)
+    Yp = mlp.forward(Xn) * norm["out_std"] + norm["out_mean"]
+    return float(np.sqrt(((Yp - Y) ** 2).mean()) / np.sqrt((Y ** 2).mean()))
