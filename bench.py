@@ -7,15 +7,26 @@ One "step" = ProfX (Held-Suarez forcing) + Thor (one large RK3 step), the same
 sequence as THOR's main loop (esp.cu:1393-1398). Synthetic input: the rest
 state plus seeded +-1e-3 K temperature noise; all state resident in HBM.
 
+Both forms are timed: the plain FP64 path (THOR's arithmetic) and the mixed-
+precision path (FP32 hyperdiffusion / divergence-damping operators and
+momentum deviation, FP64 elsewhere). `value` is the mixed-precision form's
+throughput; `fp64` and `speedup_vs_fp64` give the FP64 form beside it.
+Held-Suarez has no radiative transfer, so the RT surrogate is not on this
+workload's path.
+
 Prints ONE JSON line. Extra objects:
-  roofline     - dominant kernel: algorithmic bytes per launch (per-cell doubles
-                 of the reference kernels it fuses x P x nv x 8 B) / average
-                 launch time from HIP events on the library's stream, against
-                 8 TB/s; "traffic" = HBM bytes per launch from the committed
-                 rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_*.json);
-                 "step" = the whole large step's algorithmic bytes / wall time;
-  cpu_baseline - the serial FP64 oracle (THOR's kernels as a host loop) timed
-                 on one host core for one step of the same workload.
+  roofline       - dominant kernel of the headline form: algorithmic bytes per
+                   launch (per-cell arrays of the reference kernels it fuses,
+                   each at its storage width, x P x nv) / average launch time
+                   from HIP events on the library's stream, against 8 TB/s;
+                   "traffic" = HBM bytes per launch from the committed
+                   rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_*.json);
+                   "step" = the whole large step's algorithmic bytes / wall time;
+  error_vs_fp64  - both forms run --error-days simulated days from the same
+                   state; RMS / max differences of T, u, v, w and surface
+                   pressure between them;
+  cpu_baseline   - the serial FP64 oracle (THOR's kernels as a host loop) timed
+                   on one host core for one step of the same workload.
 """
 from __future__ import annotations
 
@@ -68,6 +79,105 @@ def measured_traffic(kernel: str, workload: str):
     return k["hbm_bytes_per_launch"]
 
 
+def timed_run(capi, g, p, st, args):
+    """K timed steps after W warm-up steps, then per-kernel HIP-event timing."""
+    d = capi.Dyn(g, p, variant=args.variant)
+    d.set_state(st)
+    d.run(args.warmup)
+    d.sync()
+    t1 = time.perf_counter()
+    d.run(args.steps)
+    d.sync()
+    wall = time.perf_counter() - t1
+    nprof = max(3, min(20, args.steps // 5))
+    d.kernel_timing(True)
+    d.run(nprof)
+    d.sync()
+    stats = d.kernel_stats()
+    d.kernel_timing(False)
+    d.close()
+    tot = sum(s["total_ms"] for s in stats)
+    for s in sorted(stats, key=lambda s: -s["total_ms"]):
+        avg = s["total_ms"] / s["launches"]
+        gbs = s["bytes_per_launch"] / (avg * 1e-3) / 1e9
+        log(f"  {s['name']:20s} {s['launches'] / nprof:6.1f}/step {avg * 1e3:9.1f} us "
+            f"{100 * s['total_ms'] / tot:5.1f}%  {gbs:7.0f} GB/s alg")
+    return {"ms_step": 1000.0 * wall / args.steps, "sdpd": args.dt * args.steps / wall, "stats": stats,
+            "nprof": nprof}
+
+
+def roofline(run, workload):
+    stats, nprof = run["stats"], run["nprof"]
+    dom = max(stats, key=lambda s: s["total_ms"])
+    dom_avg_ms = dom["total_ms"] / dom["launches"]
+    achieved = dom["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
+    # whole step: algorithmic bytes of every launch except the per-stage Thomas
+    # factorisation (no reference counterpart); in FP64 this is the reference
+    # decomposition's N * 8 B * (470 + 70 n_sub + 24)
+    step_bytes = sum(s["bytes_per_launch"] * s["launches"] for s in stats if s["name"] != "thomas_factor") / nprof
+    step_gbs = step_bytes / (run["ms_step"] * 1e-3) / 1e9
+    return {
+        "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(dom["name"], workload),
+        "alg_bytes_per_launch": round(dom["bytes_per_launch"]),
+        "avg_launch_us": round(dom_avg_ms * 1e3, 2),
+        "step": {"alg_bytes": round(step_bytes), "achieved": round(step_gbs, 1),
+                 "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+    }
+
+
+def diagnostics(g, p, st):
+    """T, zonal / meridional wind, w and surface pressure of a state."""
+    P = g.P
+    rho, pr = st["Rho"], st["pressure"]
+    T = pr / (rho * p.Rd)
+    v = st["Mh"] / rho[:, :, None]
+    lon, lat = np.asarray(g.lonlat).reshape(P, 2).T
+    el = np.stack([-np.sin(lon), np.cos(lon), np.zeros(P)], axis=1)
+    ep = np.stack([-np.sin(lat) * np.cos(lon), -np.sin(lat) * np.sin(lon), np.cos(lat)], axis=1)
+    u = np.einsum("plc,pc->pl", v, el)
+    vm = np.einsum("plc,pc->pl", v, ep)
+    w = st["W"] / rho
+    alt = np.asarray(g.Altitude)
+    # surface pressure as the Held-Suarez forcing forms it (profx_held_suarez.h:89-91)
+    psm1 = pr[:, 1] - rho[:, 0] * p.Gravit * (-alt[0] - alt[1])
+    ps = 0.5 * (pr[:, 0] + psm1)
+    return {"T": T, "u": u, "v": vm, "w": w, "ps": ps}
+
+
+def error_run(capi, g, p64, pmx, st, args):
+    nsteps = int(round(args.error_days * 86400.0 / args.dt))
+    ds = [capi.Dyn(g, q, variant=args.variant) for q in (p64, pmx)]
+    for d in ds:
+        d.set_state(st)
+    chunk = 240
+    t0 = time.perf_counter()
+    done = 0
+    while done < nsteps:
+        n = min(chunk, nsteps - done)
+        for d in ds:
+            d.run(n)
+        for d in ds:
+            d.sync()
+        done += n
+        log(f"error run: {done}/{nsteps} steps ({done * args.dt / 86400:.1f} days, "
+            f"{time.perf_counter() - t0:.1f} s)")
+    a = diagnostics(g, p64, ds[0].get_state())
+    b = diagnostics(g, pmx, ds[1].get_state())
+    gd = [d.globdiag() for d in ds]
+    for d in ds:
+        d.close()
+    out = {"days": args.error_days, "steps": nsteps}
+    for k in ("T", "u", "v", "w", "ps"):
+        diff = b[k] - a[k]
+        out[k] = {"rms": float(f"{np.sqrt((diff ** 2).mean()):.4e}"), "max": float(f"{np.abs(diff).max():.4e}"),
+                  "fp64_rms": float(f"{np.sqrt((a[k] ** 2).mean()):.4e}")}
+    out["units"] = {"T": "K", "u": "m/s", "v": "m/s", "w": "m/s", "ps": "Pa"}
+    out["mass_rel_diff"] = float(f"{(gd[1][0] - gd[0][0]) / gd[0][0]:.3e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,7 +188,10 @@ def main():
     ap.add_argument("--dt", type=float, default=300.0)
     ap.add_argument("--substeps", type=int, default=8)
     ap.add_argument("--variant", default="", help='"" production build, "pm" parity build')
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="forms to time; the headline is the mixed form when it is timed")
+    ap.add_argument("--error-days", type=float, default=10.0,
+                    help="simulated days of the FP64-vs-mixed error run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (three times); "
@@ -91,68 +204,37 @@ def main():
 
     import thor_amd.capi as capi
 
-    mixed = args.precision == "mixed"
-
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     os.makedirs(CACHE, exist_ok=True)
     t0 = time.perf_counter()
     g = capi.Grid(args.glevel, args.nv, 6.371e6, 36000.0, spring=True, beta=1.15, cache_dir=CACHE)
     log(f"grid g{args.glevel} L{args.nv}: P={g.P} built in {time.perf_counter() - t0:.1f}s")
-    p = capi.default_params(glevel=args.glevel, timestep=args.dt, n_substeps=args.substeps,
-                            precision=1 if args.precision == "mixed" else 0)
-    st = capi.ic_rest(g, p, seed=1002, temperature_noise=1e-3)
+    base = dict(glevel=args.glevel, timestep=args.dt, n_substeps=args.substeps)
+    p64 = capi.default_params(precision=0, **base)
+    pmx = capi.default_params(precision=1, **base)
+    st = capi.ic_rest(g, p64, seed=1002, temperature_noise=1e-3)
+    wl = f"g{args.glevel}L{args.nv}"
 
-    d = capi.Dyn(g, p, variant=args.variant)
-    d.set_state(st)
-    # warm-up (untimed)
-    d.run(args.warmup)
-    d.sync()
-    # timed region: exactly K steps, synchronised on both sides
-    d.sync()
-    t1 = time.perf_counter()
-    d.run(args.steps)
-    d.sync()
-    wall = time.perf_counter() - t1
-    ms_step = 1000.0 * wall / args.steps
-    sdpd = args.dt * args.steps / wall  # simulated seconds per wall second = sim-days per wall-day
+    runs = {}
+    if args.precision in ("both", "fp64"):
+        log("FP64 form:")
+        runs["fp64"] = timed_run(capi, g, p64, st, args)
+    if args.precision in ("both", "mixed"):
+        log("mixed-precision form:")
+        runs["mixed"] = timed_run(capi, g, pmx, st, args)
+    head = "mixed" if "mixed" in runs else "fp64"
+    hr = runs[head]
+    roof = roofline(hr, wl + ("_mixed" if head == "mixed" else ""))
 
-    # per-kernel device times (HIP events on the library's stream) over a few steps
-    nprof = max(3, min(20, args.steps // 5))
-    d.kernel_timing(True)
-    d.run(nprof)
-    d.sync()
-    stats = d.kernel_stats()
-    d.kernel_timing(False)
-    tot = sum(s["total_ms"] for s in stats)
-    for s in sorted(stats, key=lambda s: -s["total_ms"]):
-        avg = s["total_ms"] / s["launches"]
-        gbs = s["bytes_per_launch"] / (avg * 1e-3) / 1e9
-        log(f"  {s['name']:20s} {s['launches'] / nprof:6.1f}/step {avg * 1e3:9.1f} us "
-            f"{100 * s['total_ms'] / tot:5.1f}%  {gbs:7.0f} GB/s alg")
-    dom = max(stats, key=lambda s: s["total_ms"])
-    dom_avg_ms = dom["total_ms"] / dom["launches"]
-    achieved = dom["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
-    # whole step: the reference decomposition's algorithmic bytes per large step,
-    # N * 8 B * (470 + 70 * n_sub + 24) with n_sub = 1 + ns/2 + ns acoustic
-    # substeps (ProfX included), over the timed wall time per step
-    n_sub = 1 + args.substeps // 2 + args.substeps
-    step_bytes = g.P * args.nv * 8.0 * (470 + 70 * n_sub + 24)
-    step_gbs = step_bytes / (ms_step * 1e-3) / 1e9
-    roofline = {
-        "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(dom["name"], f"g{args.glevel}L{args.nv}" + ("_mixed" if mixed else "")),
-        "alg_bytes_per_launch": round(dom["bytes_per_launch"]),
-        "avg_launch_us": round(dom_avg_ms * 1e3, 2),
-        "step": {"alg_bytes": round(step_bytes), "achieved": round(step_gbs, 1),
-                 "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
-    }
+    err = None
+    if args.precision == "both" and args.error_days > 0:
+        err = error_run(capi, g, p64, pmx, st, args)
 
     cpu = None
     if not args.no_cpu:
         from oracle.oracle import Oracle
-        o = Oracle(g, p, variant=args.variant)
+        o = Oracle(g, p64, variant=args.variant)
         o.set_state(st)
         tc = time.perf_counter()
         o.steps(1)
@@ -160,31 +242,38 @@ def main():
         cpu = {"value": round(args.dt / cpu_s, 3), "unit": "sim-days/wall-day", "cores": 1, "kind": "port",
                "sample": f"1 step (ProfX + Thor) of the same g{args.glevel} L{args.nv} workload, serial FP64 "
                          f"oracle on 1 host core: {cpu_s:.2f} s"}
-        # parity of the same first step on the GPU
-        d2 = capi.Dyn(g, p, variant=args.variant)
+        d2 = capi.Dyn(g, p64, variant=args.variant)
         d2.set_state(st)
         d2.run(1)
         d2.sync()
         gs, os_ = d2.get_state(), o.get_state()
         errs = {k: float(np.abs(gs[k] - os_[k]).max() / max(np.abs(os_[k]).max(), 1e-300))
                 for k in ("Mh", "Wh", "Rho", "pressure")}
-        cpu["first_step_max_rel_err"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
+        cpu["fp64_first_step_max_rel_err"] = {k: float(f"{v:.3e}") for k, v in errs.items()}
         d2.close()
 
+    mixed = head == "mixed"
     out = {
-        "metric": METRIC, "value": round(sdpd, 1), "unit": "sim-days/wall-day", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "metric": METRIC, "value": round(hr["sdpd"], 1), "unit": "sim-days/wall-day", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(hr["ms_step"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic (rest state + seeded 1e-3 K noise)",
         "config": {"workload": f"Held-Suarez Earth g{args.glevel} L{args.nv} ({g.P} columns), dt={args.dt:g} s, "
                                f"{args.substeps} acoustic substeps, HyDiff 4 + DivDamp, "
-                               + ("mixed precision (FP32 damping operators and momentum, FP64 rest)" if mixed
-                                  else "FP64"),
+                               + ("mixed precision (FP32 damping operators and momentum deviation, FP64 rest)"
+                                  if mixed else "FP64"),
                    "glevel": args.glevel, "nv": args.nv, "dt": args.dt, "n_substeps": args.substeps,
-                   "precision": args.precision, "build": args.variant or "production"},
-        "roofline": roofline, "cpu_baseline": cpu,
+                   "precision": head, "build": args.variant or "production"},
+        "roofline": roof,
     }
+    if "fp64" in runs:
+        out["fp64"] = {"value": round(runs["fp64"]["sdpd"], 1), "ms_per_step": round(runs["fp64"]["ms_step"], 4),
+                       "roofline": roofline(runs["fp64"], wl)}
+        if mixed:
+            out["speedup_vs_fp64"] = round(runs["fp64"]["ms_step"] / hr["ms_step"], 3)
+    out["error_vs_fp64"] = err
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
 
 

@@ -176,8 +176,13 @@ static Fields stage_fields(Dyn &d, int rk) {
 // Prepare_Implicit_Vertical 10, Vertical_Eq 12, Density_Pressure_Eqs 22,
 // ProfX (Held-Suarez) 24. The per-stage Thomas factorisation has no reference
 // counterpart and books its own traffic (hh, gtilh in; three planes out).
+// In mixed precision each array counts with its storage width: FP32 for Mhs,
+// Adv, diff, diffmh, diffw/rh/pr, divg, DivM, SlowMh (and THOR's vertical-
+// diffusion arrays, zero here), FP64 for the rest; below, (n64, n32) pairs.
 template <bool MX> static int thor_step_t(Dyn &d) {
     const double N = (double)d.P * d.nv, B = 8.0 * N;
+    // bytes per cell for n64 FP64 arrays and n32 mixed-precision arrays
+    auto W = [&](double n64, double n32) { return MX ? 8.0 * N * n64 + 4.0 * N * n32 : 8.0 * N * (n64 + n32); };
     const int ns_total = d.prm.n_substeps;
     const int hydiff = d.prm.HyDiff, divdamp = d.prm.DivDampP;
     const dim3 blk(kTile), grid_s((d.P + kTile - 1) / kTile, (d.nv + kLC - 1) / kLC);
@@ -192,33 +197,37 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
-        launch(d, "column_prep", (12 + 11) * B, [&] { k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f); });
-        launch(d, "adv_diff1", (11 + (hydiff ? 13 : 0)) * B,
+        launch(d, "column_prep", W(12 + 5, 6), [&] { k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f); });
+        launch(d, "adv_diff1", W(8 + (hydiff ? 7 : 0), 3 + (hydiff ? 6 : 0)),
                [&] { k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
         if (hydiff || divdamp)
-            launch(d, "diff2_divm0", ((hydiff ? 13 + 12 : 0) + (divdamp ? 7 : 0)) * B, [&] {
+            launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
+                   [&] {
                 k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
             });
-        launch(d, "slow_modes", ((divdamp ? 6 : 0) + 40 + (rk ? 19 : 0)) * B,
+        launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
-        if (divdamp) launch(d, "divm0", 7 * B, [&] { k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f); });
+        if (divdamp) launch(d, "divm0", W(1, 6), [&] { k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f); });
         launch(d, "thomas_factor", 5 * B, [&] {
-            if (cb == 32) k_thomas_factor<32><<<grid_t, 256, lds_t, st>>>(g, f, times);
-            else k_thomas_factor<16><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            if (cb == 32) k_thomas_factor<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            else k_thomas_factor<16, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
         });
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
-            launch(d, "momentum", ((divdamp ? 6 : 0) + 13) * B,
+            launch(d, "momentum", W(1, (divdamp ? 6 : 0) + 12),
                    [&] { k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
-            launch(d, "prepare", 10 * B, [&] { k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f); });
-            launch(d, "thomas_solve", 12 * B, [&] {
-                if (cb == 32) k_thomas_solve<32><<<grid_t, 256, lds_t, st>>>(g, f, times);
-                else k_thomas_solve<16><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            // Prepare_Implicit_Vertical, plus the horizontal flux divergences of
+            // Density_Pressure_Eqs (booked with the column kernel below)
+            launch(d, "prepare", W(7, 3), [&] { k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f); });
+            // Vertical_Eq (12) + Density_Pressure_Eqs (22, of which Mhs and diffpr FP32 in mixed)
+            launch(d, "thomas_dp", W(12 + 18, 4), [&] {
+                if (cb == 32) k_thomas_dp<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
+                else k_thomas_dp<16, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
             });
-            launch(d, "density_pressure", (22 + (next ? 7 : 0)) * B,
-                   [&] { k_density_pressure<MX><<<grid_t2, blk, 0, st>>>(g, f, times, next); });
+            // next substep's divergence-damping pass 0
+            if (next) launch(d, "divm0", W(1, 6), [&] { k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f); });
         }
-        launch(d, "update_rk2", 19 * B, [&] { k_update_rk2<kLC, MX><<<grid_s, blk, 0, st>>>(g, f); });
+        launch(d, "update_rk2", W(16, 3), [&] { k_update_rk2<kLC, MX><<<grid_s, blk, 0, st>>>(g, f); });
     }
     std::swap(d.Mh, d.Mhk);
     std::swap(d.Wh, d.WhkA);
@@ -486,7 +495,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     A(&f.Adv, n3); A(&f.diff, 6 * n1); A(&f.diffmh, n3);
     A(&f.diffw, n1); A(&f.diffrh, n1); A(&f.diffpr, n1); A(&f.divg, n3);
     A(&f.SlowMh, n3); A(&f.SlowWh, ni); A(&f.SlowRho, n1); A(&f.Slowp, n1);
-    A(&f.Sp, n1); A(&f.Sd, n1); A(&f.tAA, n1); A(&f.tCC, n1); A(&f.tTT, n1);
+    A(&f.Sp, n1); A(&f.Sd, n1); A(&f.hflx_r, n1); A(&f.hflx_pt, n1); A(&f.tAA, n1); A(&f.tCC, n1); A(&f.tTT, n1);
     A(&f.Qheat, n1); A(&f.dMh, n3); A(&f.dWh, ni);
     const long ncell = (long)P * nv;
     d.npart = (int)((ncell + 2047) / 2048);

@@ -27,6 +27,11 @@ namespace thor {
 #define PRAGMA_STR(x) _Pragma(#x)
 #define SLOT_UNROLL(n) PRAGMA_STR(unroll n)
 
+// pow of the mixed-precision instantiations: always the portable one, which is
+// faster than the device libm's (accuracy 2e-14 relative); the FP64 path keeps
+// MPOW so that each build matches its oracle variant.
+#define XPOW(x, y) (MX ? pm::pow((x), (y)) : MPOW((x), (y)))
+
 #define LV(arr, lev) (arr)[(size_t)(lev) * S + p]
 #define LVP(arr, lev, q) (arr)[(size_t)(lev) * S + (q)]
 #define VEC(arr, c, lev, q) (arr)[((size_t)(c) * nv + (lev)) * S + (q)]
@@ -428,7 +433,7 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
     };
     auto pt_at = [&](int l) {
         const double pr = LV(pk, l), r = LV(Rho, l);
-        return (P_Ref / (Rd * r)) * MPOW(pr / P_Ref, CvoCp);
+        return (P_Ref / (Rd * r)) * XPOW(pr / P_Ref, CvoCp);
     };
     // interpolation weights of the point xi between xim and xip
     auto wts = [](double xi, double xim, double xip, double &intt, double &intl) {
@@ -451,7 +456,7 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
         const double temperature = pressure / (rho * Rd);
         LV(f.T, lev) = temperature;
         LV(f.h, lev) = Cv * temperature + pressure / rho;
-        LV(f.pt, lev) = (P_Ref / (Rd * rho)) * MPOW(pressure / P_Ref, CvoCp);
+        LV(f.pt, lev) = (P_Ref / (Rd * rho)) * XPOW(pressure / P_Ref, CvoCp);
         // gtil at this centre: the sweep forms it at interface lev + 1 from the
         // interface values below and above, weighted at (Alt, Alth, Alth+1) of
         // that sweep step
@@ -471,7 +476,7 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
         const double extr = (-alt - Alt[1]) / (Alt[1] - alt);
         LV(f.hh, 0) = h_at(0);
         double pl;
-        if (g.ghv) pl = LV(pk, 1) - rho * Gravit * MPOW(A / (A + Alt[0]), 2) * (-alt - Alt[1]);
+        if (g.ghv) pl = LV(pk, 1) - rho * Gravit * XPOW(A / (A + Alt[0]), 2) * (-alt - Alt[1]);
         else pl = LV(pk, 1) - rho * Gravit * (-alt - Alt[1]);
         const double rhol = LV(Rho, 1) + (LV(Rho, 1) - rho) * extr;
         const double rhoh = 0.5 * (rho + rhol);
@@ -492,7 +497,7 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
         double pp;
         if (g.ghv)
             pp = LV(pk, nv - 2)
-                 - LV(Rho, nv - 1) * Gravit * MPOW(A / (A + Alt[nv - 1]), 2) * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+                 - LV(Rho, nv - 1) * Gravit * XPOW(A / (A + Alt[nv - 1]), 2) * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
         else
             pp = LV(pk, nv - 2) - LV(Rho, nv - 1) * Gravit * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
         const double dpdz = (pp - LV(pk, nv - 1)) / dz;
@@ -542,9 +547,9 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
     }
     double r2p, r2m, r2l;
     if (g.deep) {
-        r2p = MPOW(altht + A, 2.0);
-        r2m = MPOW(alt + A, 2.0);
-        r2l = MPOW(althl + A, 2.0);
+        r2p = XPOW(altht + A, 2.0);
+        r2m = XPOW(alt + A, 2.0);
+        r2l = XPOW(althl + A, 2.0);
     } else {
         r2p = r2m = r2l = 1.0;
     }
@@ -559,8 +564,8 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
         Cy = 2.0 * Omega * vc[0] * rho;
         Cz = 0.0;
     } else {
-        Cx = 2 * Omega * (vc[2] * fr[2] * fr[1] - vc[1] * MPOW(fr[2], 2)) * rho;
-        Cy = -2 * Omega * (vc[2] * fr[2] * fr[0] - vc[0] * MPOW(fr[2], 2)) * rho;
+        Cx = 2 * Omega * (vc[2] * fr[2] * fr[1] - vc[1] * XPOW(fr[2], 2)) * rho;
+        Cy = -2 * Omega * (vc[2] * fr[2] * fr[0] - vc[0] * XPOW(fr[2], 2)) * rho;
         Cz = 2 * Omega * (vc[1] * fr[2] * fr[0] - vc[0] * fr[2] * fr[1]) * rho;
     }
     auto *Adv = FS<MX>::Adv(f);
@@ -805,21 +810,21 @@ __device__ __forceinline__ void slow_point(const Geo &g, const Fields &f, int p,
     const double altht = Alth[lev + 1], althl = Alth[lev], alt = Alt[lev];
     const double M0x = mv(0, 0), M0y = mv(0, 1), M0z = mv(0, 2);
     if (!g.nonhydro) {
-        advr = (MPOW(M0x, 2) + MPOW(M0y, 2) + MPOW(M0z, 2)) / (rho * (alt + A));
+        advr = (XPOW(M0x, 2) + XPOW(M0y, 2) + XPOW(M0z, 2)) / (rho * (alt + A));
         if (lev != 0) {
             const double m1x = VEC(f.Mhk, 0, lev - 1, p), m1y = VEC(f.Mhk, 1, lev - 1, p),
                          m1z = VEC(f.Mhk, 2, lev - 1, p);
-            advrl = (MPOW(m1x, 2) + MPOW(m1y, 2) + MPOW(m1z, 2)) / (rhol * (altl + A));
+            advrl = (XPOW(m1x, 2) + XPOW(m1y, 2) + XPOW(m1z, 2)) / (rhol * (altl + A));
         }
     }
     const double dz = altht - althl;
     double r2p, r2m, r2l, rscale, dr2dz;
     if (g.deep) {
-        r2p = MPOW(altht + A, 2.0);
-        r2m = MPOW(alt + A, 2.0);
-        r2l = MPOW(althl + A, 2.0);
+        r2p = XPOW(altht + A, 2.0);
+        r2m = XPOW(alt + A, 2.0);
+        r2l = XPOW(althl + A, 2.0);
         rscale = A / (alt + A);
-        dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3;
+        dr2dz = (XPOW(altht + A, 3.0) - XPOW(althl + A, 3.0)) / 3;
     } else {
         r2p = r2m = r2l = 1.0;
         rscale = 1.0;
@@ -873,13 +878,13 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         double swr;
         if (g.deep) {
             if (g.ghv)
-                swr = (advrl + rhol * Gravit * MPOW(A / (A + altl), 2)) * intt
-                      + (advr + rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
+                swr = (advrl + rhol * Gravit * XPOW(A / (A + altl), 2)) * intt
+                      + (advr + rho * Gravit * XPOW(A / (A + alt), 2)) * intl;
             else
                 swr = (advrl + rhol * Gravit) * intt + (advr + rho * Gravit) * intl;
         } else {
             if (g.ghv)
-                swr = (rhol * Gravit * MPOW(A / (A + altl), 2)) * intt + (rho * Gravit * MPOW(A / (A + alt), 2)) * intl;
+                swr = (rhol * Gravit * XPOW(A / (A + altl), 2)) * intt + (rho * Gravit * XPOW(A / (A + alt), 2)) * intl;
             else
                 swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
         }
@@ -1104,27 +1109,39 @@ __host__ __device__ constexpr int kThomasCB(int nv) { return 4 * (nv + 1) * 32 *
 
 // Prepare_Implicit_Vertical + _Poles (thor_vertical_int.h:566-771): source
 // terms Sp = -(Rd/Cv) div(Mhs h) + Slowpressure, Sd = -div(Mhs) + SlowRho.
-template <class GetM, class GetH>
-__device__ __forceinline__ void prepare_point(const Geo &g, const Fields &f, int p, int lev, GetM mv, GetH hv) {
+// ... and the horizontal flux divergences of this substep's
+// Density_Pressure_Eqs (thor_fastmodes.h:255-847), which see the same Mhs:
+// hflx_r = rscale sum div Mhs (the Sd sum, accumulated with the opposite sign)
+// and hflx_pt = rscale sum div (Mhs + Mhk) pt.
+template <class GetM, class GetH, class GetV1, class GetPt>
+__device__ __forceinline__ void prepare_point(const Geo &g, const Fields &f, int p, int lev, GetM mv, GetH hv,
+                                              GetV1 v1, GetPt ptv) {
     const size_t S = g.S;
     const double RoC = g.Rd / (g.Cp - g.Rd);
     const double rs = rscale_at(g, lev);
-    double nflxr = 0.0, nflxp = 0.0;
+    double nflxr = 0.0, nflxp = 0.0, dr = 0.0, dpt = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        double sr = g.div[(size_t)(0 * 3 + k) * S + p] * mv(0, k);
-        double sp = g.div[(size_t)(0 * 3 + k) * S + p] * mv(0, k) * hv(0);
+        const double d0 = g.div[(size_t)(0 * 3 + k) * S + p];
+        double sr = d0 * mv(0, k);
+        double sp = d0 * mv(0, k) * hv(0);
+        double st = d0 * v1(0, k) * ptv(0);
 SLOT_UNROLL(THOR_SLOT_UNROLL)
         for (int s = 1; s < 7; s++) {
             const double dd = g.div[(size_t)(s * 3 + k) * S + p];
             sr = sr + dd * mv(s, k);
             sp = sp + dd * mv(s, k) * hv(s);
+            st = st + dd * v1(s, k) * ptv(s);
         }
         nflxr -= rs * sr;
         nflxp -= (RoC)*rs * sp;
+        dr += rs * sr;
+        dpt += rs * st;
     }
     LV(f.Sp, lev) = nflxp + LV(f.Slowp, lev);
     LV(f.Sd, lev) = nflxr + LV(f.SlowRho, lev);
+    LV(f.hflx_r, lev) = dr;
+    LV(f.hflx_pt, lev) = dpt;
 }
 
 template <bool MX> __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
@@ -1136,28 +1153,37 @@ template <bool MX> __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
         PoleCtx c;
         if (!pole_ctx(g, c)) return;
         const auto *Mhs = FS<MX>::Mhs(f);
-        prepare_point(g, f, c.p, lev, [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)); },
-                      [&](int s) { return LVP(f.h, lev, c.gi(s)); });
+        prepare_point(
+            g, f, c.p, lev, [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.h, lev, c.gi(s)); },
+            [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
+            [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
         return;
     }
     using R = typename FS<MX>::R;
     __shared__ R sM[3][NHL2];
-    __shared__ double sH[NHL2];
+    __shared__ double sH[NHL2], sV1[3][NHL2], sPt[NHL2];
     const TileThread T = tile_thread(g, r);
     const auto *Mhs = FS<MX>::Mhs(f);
     stage_tile(g, r, T, [&](int pos, int q) {
         if (q < 0) {
-            for (int k = 0; k < 3; k++) sM[k][pos] = 0.0;
-            sH[pos] = 0.0;
+            for (int k = 0; k < 3; k++) sM[k][pos] = sV1[k][pos] = 0.0;
+            sH[pos] = sPt[pos] = 0.0;
             return;
         }
 #pragma unroll
-        for (int k = 0; k < 3; k++) sM[k][pos] = VEC(Mhs, k, lev, q);
+        for (int k = 0; k < 3; k++) {
+            const R m = VEC(Mhs, k, lev, q);
+            sM[k][pos] = m;
+            sV1[k][pos] = (double)m + VEC(f.Mhk, k, lev, q);
+        }
         sH[pos] = LVP(f.h, lev, q);
+        sPt[pos] = LVP(f.pt, lev, q);
     });
     __syncthreads();
     prepare_point(g, f, T.p, lev, [&](int s, int k) { return (double)sM[k][T.pos(s)]; },
-                  [&](int s) { return sH[T.pos(s)]; });
+                  [&](int s) { return sH[T.pos(s)]; }, [&](int s, int k) { return sV1[k][T.pos(s)]; },
+                  [&](int s) { return sPt[T.pos(s)]; });
 }
 
 // Vertical_Eq (thor_vertical_int.h:52-563): per column, the tridiagonal solve
@@ -1181,7 +1207,7 @@ struct Plane {
 // once; then one lane per column runs the serial recurrence out of LDS. The
 // arithmetic of each term and of the recurrence is the reference's.
 // LDS: 4 planes of (nv + 1) x CB doubles.
-template <int CB>
+template <int CB, bool MX>
 __global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double deltat) {
     extern __shared__ double sm[];
     const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
@@ -1194,7 +1220,7 @@ __global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double d
     const bool ghv = g.ghv, nonh = g.nonhydro;
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
     auto gcor = [&](double altx) {
-        return ghv ? Gravit * MPOW(A / (A + altx), 2) * (Cp - Rd) / Rd : Gravit * (Cp - Rd) / Rd;
+        return ghv ? Gravit * XPOW(A / (A + altx), 2) * (Cp - Rd) / Rd : Gravit * (Cp - Rd) / Rd;
     };
     if (p < g.P) {
         for (int lev = 1 + row; lev < nv; lev += NROW) {
@@ -1211,11 +1237,11 @@ __global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double d
             const double inttm = -(altl - alth) * dzm * dzh;
             double aa, bb, cc;
             if (g.deep) {
-                hp = hp * MPOW(altht + A, 2);
-                h = h * MPOW(alth + A, 2);
-                hm = hm * MPOW(althl + A, 2);
-                const double or2 = 1 / MPOW(alth + A, 2);
-                const double tor3 = 2 / MPOW(alth + A, 3);
+                hp = hp * XPOW(altht + A, 2);
+                h = h * XPOW(alth + A, 2);
+                hm = hm * XPOW(althl + A, 2);
+                const double or2 = 1 / XPOW(alth + A, 2);
+                const double tor3 = 2 / XPOW(alth + A, 3);
                 cc = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
                 if (nonh) bb = CRdd + (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
                 else bb = (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
@@ -1250,12 +1276,68 @@ __global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double d
     }
 }
 
-// Vertical_Eq (thor_vertical_int.h:52-563) with the stage's factorisation:
-// right-hand side C0 from the substep's sources, forward elimination into Whs
-// (as the reference's dd), back substitution Whs_k = -cc'_k Whs_{k+1} + dd_k,
-// and Whs = 0 at the bottom and top interfaces.
-template <int CB>
-__global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double deltat) {
+// ===================================================================== K_G2
+// Density_Pressure_Eqs / _Poles (thor_fastmodes.h:255-847, entropy form),
+// column part: the horizontal flux divergences nflxr = rscale sum div M' and
+// nflxpt = rscale sum div (Mk + M') pt come from k_prepare (same sums, same
+// order), the vertical fluxes from the freshly solved Whs of the column
+// (whs(k) = Whs at interface k).
+template <bool MX, class GetW>
+__device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
+                                          double nflxr, double nflxpt, GetW whs) {
+    const size_t S = g.S;
+    const double A = g.A, P_Ref = g.P_Ref, Rd = g.Rd, Cp = g.Cp, Cv = Cp - Rd;
+    const double altht = g.Alth[lev + 1], althl = g.Alth[lev], dz = altht - althl;
+    double r2p, r2m, r2l, dr2dz;
+    if (g.deep) {
+        const double alt = g.Alt[lev];
+        r2p = XPOW(altht + A, 2.0);
+        r2m = XPOW(alt + A, 2.0);
+        r2l = XPOW(althl + A, 2.0);
+        dr2dz = (XPOW(altht + A, 3.0) - XPOW(althl + A, 3.0)) / 3.0;
+    } else {
+        r2p = r2m = r2l = 1.0;
+        dr2dz = r2m * dz;
+    }
+    double whl, wht, whl2, wht2, phl, pht;
+    if (lev == 0) {
+        whl = 0.0;
+        wht = whs(lev + 1);
+        whl2 = 0.0;
+        wht2 = whs(lev + 1) + LV(f.Whk, lev + 1);
+        phl = 0.0;
+        pht = LV(f.pth, lev + 1);
+    } else {
+        whl = whs(lev);
+        wht = whs(lev + 1);
+        whl2 = whs(lev) + LV(f.Whk, lev);
+        wht2 = whs(lev + 1) + LV(f.Whk, lev + 1);
+        phl = LV(f.pth, lev);
+        pht = LV(f.pth, lev + 1);
+    }
+    const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
+    const double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
+    double aux = -(nflxpt + dwptdz) * dt;
+    const double rhos = LV(f.Rhos, lev), pss = LV(f.ps, lev), pkv = LV(f.pk, lev);
+    const double r = LV(f.Rhok, lev) + rhos;
+    nflxr += dwdz;
+    LV(f.Rhos, lev) = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
+    const double ptc = (P_Ref / (Rd * r)) * XPOW((pss + pkv) / P_Ref, Cv / Cp);
+    aux += ptc * r;
+    const double pnew = P_Ref * XPOW(Rd * aux / P_Ref, Cp / Cv);
+    const double psn = pnew - pkv + (LV(FS<MX>::diffpr(f), lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
+    LV(f.ps, lev) = psn;
+    if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
+}
+
+// Vertical_Eq (thor_vertical_int.h:52-563) with the stage's factorisation
+// (right-hand side C0 from the substep's sources, forward elimination into
+// Whs as the reference's dd, back substitution Whs_k = -cc'_k Whs_{k+1} + dd_k,
+// Whs = 0 at the bottom and top interfaces), followed by the column part of
+// the substep's Density_Pressure_Eqs on the new Whs, which the block keeps in
+// LDS: all threads take a share of the levels again after the solve.
+template <int CB, bool MX>
+__global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double deltat) {
     extern __shared__ double sm[];
     const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
     constexpr int NROW = 256 / CB;
@@ -1266,7 +1348,7 @@ __global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double de
     const double *Alt = g.Alt, *Alth = g.Alth;
     const bool ghv = g.ghv, nonh = g.nonhydro;
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
-    const double dt2 = MPOW(deltat, 2.0);
+    const double dt2 = XPOW(deltat, 2.0);
     if (p < g.P) {
         const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
         for (int lev = 1 + row; lev < nv; lev += NROW) {
@@ -1284,8 +1366,8 @@ __global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double de
             double C0;
             if (!nonh) {
                 if (ghv)
-                    C0 = (dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
-                          + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
+                    C0 = (dt2 * dSpdz + dt2 * Gravit * XPOW(A / (A + alt), 2) * Sdh
+                          + Gravit * XPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
                          * (CRdd);
                 else
                     C0 = (dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
@@ -1294,8 +1376,8 @@ __global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double de
             } else {
                 const double whs = LV(f.Whs, lev);
                 if (ghv)
-                    C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * MPOW(A / (A + alt), 2) * Sdh
-                          + Gravit * MPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
+                    C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * XPOW(A / (A + alt), 2) * Sdh
+                          + Gravit * XPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
                          * (CRdd);
                 else
                     C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
@@ -1309,157 +1391,33 @@ __global__ __launch_bounds__(256) void k_thomas_solve(Geo g, Fields f, double de
         }
     }
     __syncthreads();
-    if (row != 0 || p >= g.P) return;
-    const Plane W{f.Whs, S, p};
-    double ddp = 0.0;
-    for (int lev = 1; lev < nv; lev++) {
-        const double C0 = sD[lev * CB + c];
-        if (lev == 1) ddp = (-C0) / sT[CB + c];
-        else ddp = (-C0 - ddp * sA[lev * CB + c]) * sT[lev * CB + c];
-        sD[lev * CB + c] = ddp;
-    }
-    W[nv] = 0.0;
-    W[0] = 0.0;
-    double wup = 0.0;
-    for (int lev = nv - 1; lev > 0; lev--) {
-        const double w = (-sC[lev * CB + c] * wup + sD[lev * CB + c]);
-        W[lev] = w;
-        wup = w;
-    }
-}
-
-// ===================================================================== K_G2
-// Density_Pressure_Eqs / _Poles (thor_fastmodes.h:255-847, entropy form) and,
-// unless this is the stage's last substep, divergence-damping pass 0 on the
-// updated deviations for the next substep.
-template <bool MX, class GetM, class GetV1, class GetS>
-__device__ __forceinline__ void dp_point(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
-                                         GetM vv, GetV1 v1, GetS ptv) {
-    const size_t S = g.S;
-    const int nv = g.nv;
-    const double A = g.A, P_Ref = g.P_Ref, Rd = g.Rd, Cp = g.Cp, Cv = Cp - Rd;
-    const double altht = g.Alth[lev + 1], althl = g.Alth[lev], dz = altht - althl;
-    double r2p, r2m, r2l, rscale, dr2dz;
-    if (g.deep) {
-        const double alt = g.Alt[lev];
-        r2p = MPOW(altht + A, 2.0);
-        r2m = MPOW(alt + A, 2.0);
-        r2l = MPOW(althl + A, 2.0);
-        rscale = A / (alt + A);
-        dr2dz = (MPOW(altht + A, 3.0) - MPOW(althl + A, 3.0)) / 3.0;
-    } else {
-        r2p = r2m = r2l = 1.0;
-        rscale = 1.0;
-        dr2dz = r2m * dz;
-    }
-    double nflxr = 0.0, nflxpt = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        double sr = g.div[(size_t)(0 * 3 + k) * S + p] * vv(0, k);
-        double sp = g.div[(size_t)(0 * 3 + k) * S + p] * v1(0, k) * ptv(0);
-SLOT_UNROLL(THOR_SLOT_UNROLL)
-        for (int s = 1; s < 7; s++) {
-            const double dd = g.div[(size_t)(s * 3 + k) * S + p];
-            sr = sr + dd * vv(s, k);
-            sp = sp + dd * v1(s, k) * ptv(s);
+    if (row == 0 && p < g.P) {
+        const Plane W{f.Whs, S, p};
+        double ddp = 0.0;
+        for (int lev = 1; lev < nv; lev++) {
+            const double C0 = sD[lev * CB + c];
+            if (lev == 1) ddp = (-C0) / sT[CB + c];
+            else ddp = (-C0 - ddp * sA[lev * CB + c]) * sT[lev * CB + c];
+            sD[lev * CB + c] = ddp;
         }
-        nflxr += rscale * sr;
-        nflxpt += rscale * sp;
+        W[nv] = 0.0;
+        W[0] = 0.0;
+        double wup = 0.0;
+        for (int lev = nv - 1; lev > 0; lev--) {
+            const double w = (-sC[lev * CB + c] * wup + sD[lev * CB + c]);
+            W[lev] = w;
+            sD[lev * CB + c] = w;
+            wup = w;
+        }
+        sD[c] = 0.0;
+        sD[nv * CB + c] = 0.0;
     }
-    double whl, wht, whl2, wht2, phl, pht;
-    if (lev == 0) {
-        whl = 0.0;
-        wht = LV(f.Whs, lev + 1);
-        whl2 = 0.0;
-        wht2 = LV(f.Whs, lev + 1) + LV(f.Whk, lev + 1);
-        phl = 0.0;
-        pht = LV(f.pth, lev + 1);
-    } else {
-        whl = LV(f.Whs, lev);
-        wht = LV(f.Whs, lev + 1);
-        whl2 = LV(f.Whs, lev) + LV(f.Whk, lev);
-        wht2 = LV(f.Whs, lev + 1) + LV(f.Whk, lev + 1);
-        phl = LV(f.pth, lev);
-        pht = LV(f.pth, lev + 1);
-    }
-    const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
-    const double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
-    double aux = -(nflxpt + dwptdz) * dt;
-    const double rhos = LV(f.Rhos, lev), pss = LV(f.ps, lev), pkv = LV(f.pk, lev);
-    const double r = LV(f.Rhok, lev) + rhos;
-    nflxr += dwdz;
-    LV(f.Rhos, lev) = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
-    const double ptc = (P_Ref / (Rd * r)) * MPOW((pss + pkv) / P_Ref, Cv / Cp);
-    aux += ptc * r;
-    const double pnew = P_Ref * MPOW(Rd * aux / P_Ref, Cp / Cv);
-    const double psn = pnew - pkv + (LV(FS<MX>::diffpr(f), lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
-    LV(f.ps, lev) = psn;
-    if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
-}
-
-template <bool MX>
-__global__ STENCIL_BOUNDS void k_density_pressure(Geo g, Fields f, double dt, int next_divm) {
-    int r, lev;
-    if (!block_tile(g, r, lev)) return;
-    const size_t S = g.S;
-    const int nv = g.nv;
-    using R = typename FS<MX>::R;
-    const R *Mhs = FS<MX>::Mhs(f);
-    R *divg = FS<MX>::divg(f);
-    // next substep's divergence-damping pass 0 of one point, written to divg
-    auto next = [&](int p, int nf, bool pole, auto a, auto wl, auto wt) {
-        if constexpr (MX) {
-            float o3[3];
-            divm_w<false>(g, p, lev, a, [&](int s) { return (float)(wl(s) - wt(s)); }, o3);
-#pragma unroll
-            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
-        } else {
-            double o3[3];
-            divm_nf<false>(g, p, lev, nf, pole, a, wl, wt, o3);
-#pragma unroll
-            for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o3[k];
-        }
-    };
-    if (r == g.nr) {
-        PoleCtx c;
-        if (!pole_ctx(g, c)) return;
-        auto Ms = [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)); };
-        dp_point<MX>(
-            g, f, c.p, lev, true, dt, [&](int s, int k) { return (double)Ms(s, k); },
-            [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
-            [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
-        if (next_divm)
-            next(c.p, 5, true, Ms, [&](int s) { return LVP(f.Whs, lev, c.gi(s)); },
-                 [&](int s) { return LVP(f.Whs, lev + 1, c.gi(s)); });
-        return;
-    }
-    __shared__ R sM[3][NHL2];
-    __shared__ double sV1[3][NHL2], sPt[NHL2], sWl[NHL2], sWt[NHL2];
-    const TileThread T = tile_thread(g, r);
-    stage_tile(g, r, T, [&](int pos, int q) {
-        if (q < 0) {
-            for (int k = 0; k < 3; k++) sM[k][pos] = sV1[k][pos] = 0.0;
-            sPt[pos] = sWl[pos] = sWt[pos] = 0.0;
-            return;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const R m = VEC(Mhs, k, lev, q);
-            sM[k][pos] = m;
-            sV1[k][pos] = (double)m + VEC(f.Mhk, k, lev, q);
-        }
-        sPt[pos] = LVP(f.pt, lev, q);
-        if (next_divm) {
-            sWl[pos] = LVP(f.Whs, lev, q);
-            sWt[pos] = LVP(f.Whs, lev + 1, q);
-        }
-    });
     __syncthreads();
-    auto Ms = [&](int s, int k) { return sM[k][T.pos(s)]; };
-    dp_point<MX>(g, f, T.p, lev, false, dt, [&](int s, int k) { return (double)Ms(s, k); },
-             [&](int s, int k) { return sV1[k][T.pos(s)]; }, [&](int s) { return sPt[T.pos(s)]; });
-    if (next_divm)
-        next(T.p, T.nf, false, Ms, [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; });
+    if (p >= g.P) return;
+    const bool pole = p >= g.P - 2;
+    for (int lev = row; lev < nv; lev += NROW)
+        dp_column<MX>(g, f, p, lev, pole, deltat, LV(f.hflx_r, lev), LV(f.hflx_pt, lev),
+                      [&](int k) { return sD[k * CB + c]; });
 }
 
 // ===================================================================== K_H
