@@ -24,6 +24,11 @@
  *   thor_ic_rest           <- ESP::initial_values(...) rest branch, src/ESP/esp_initial.cu:502-1139
  *   thor_dyn_create        <- ESP::ESP(...) + ESP::alloc_data + the Kd set-up of
  *                             ESP::initial_values (src/ESP/esp_initial.cu:68-483, 1292-1392)
+ *   thor_dyn_set_hooks     <- phy_modules_dyn_core_loop_init / _slow_modes / _fast_modes / _end
+ *                             src/headers/phy_modules.h:76-86, called at thor_driver.cu:116,641,1290,1465
+ *   thor_dyn_register_rk_array
+ *                          <- device_RK_array_manager::register_array
+ *                             src/headers/dyn/phy_modules_device.h:49-77, src/ESP/phy_modules_device.cu:53-73
  */
 #ifndef THOR_DYN_H
 #define THOR_DYN_H
@@ -40,7 +45,8 @@ enum {
     THOR_ERR_NAN = -3,
     THOR_ERR_NO_DEVICE = -4,
     THOR_ERR_IO = -5,
-    THOR_ERR_UNSUPPORTED = -6
+    THOR_ERR_UNSUPPORTED = -6,
+    THOR_ERR_HOOK = -7
 };
 
 /* Message describing the last failure on this thread ("" if none). */
@@ -210,6 +216,60 @@ thor_status thor_dyn_kernel_stat(thor_dyn *h, int i, const char **name, long *la
 /* Milliseconds of device time for n steps of thor_dyn_run, measured with HIP
  * events around the whole sequence on the handle's stream. */
 thor_status thor_dyn_time_run(thor_dyn *h, int nsteps, int forcing, double *ms);
+
+/* -------------------------------------------------------- physics modules */
+
+/* What a physics module sees of the step when a hook runs (the reference
+ * hands its modules `const ESP&`). Device pointers in the library's
+ * level-planar layout: element (component c, level k, point p) of a field
+ * with L levels is at [(c * L + k) * plane + p]; L = nv for layer fields,
+ * nv + 1 for interface fields (Whk, Whs, pth). Stage values ("k") are those
+ * of the current RK stage, deviations ("s") those of the current substep.
+ * Mhs is FP32 when mhs_fp32 = 1 (THOR_PRECISION_MIXED), else FP64. */
+typedef struct thor_step_view {
+    long plane;
+    int point_num, nv, mhs_fp32;
+    const double *Mhk, *Whk, *Wk, *Rhok, *pressurek;
+    const void *Mhs;
+    const double *Whs, *Rhos, *pressures;
+    const double *temperature, *h, *pt, *pth;
+} thor_step_view;
+
+/* Physics-module hook of the step (phy_modules.h:76-86). Each callback runs
+ * on the calling host thread, in stream order, where the reference calls it:
+ *   dyn_core_loop_init        before the first RK stage (thor_driver.cu:116);
+ *   dyn_core_loop_slow_modes  in every RK stage after the hyperdiffusion and
+ *                             before the slow tendencies (:641);
+ *   dyn_core_loop_fast_modes  in every acoustic substep after the vertical
+ *                             solve and before the density/pressure update (:1290);
+ *   dyn_core_loop_end         after the new state is in place (:1465).
+ * stream is the handle's hipStream_t (a module enqueues its kernels on it),
+ * nstep the 1-based number of the Thor step this handle is taking, times the
+ * RK stage's step (dt/3, dt/ns, dt/ns) as in the reference. A non-zero return
+ * stops the step with THOR_ERR_HOOK (the state is then partly updated). Any
+ * callback may be NULL; hooks = NULL removes them. */
+typedef struct thor_hooks {
+    void *user;
+    int (*dyn_core_loop_init)(void *user, void *stream, int nstep, const thor_step_view *v);
+    int (*dyn_core_loop_slow_modes)(void *user, void *stream, int nstep, double times, const thor_step_view *v);
+    int (*dyn_core_loop_fast_modes)(void *user, void *stream, int nstep, double times, const thor_step_view *v);
+    int (*dyn_core_loop_end)(void *user, void *stream, int nstep, const thor_step_view *v);
+} thor_hooks;
+thor_status thor_dyn_set_hooks(thor_dyn *h, const thor_hooks *hooks);
+
+/* Registers a module array that the RK scheme advances like the state
+ * (device_RK_array_manager::register_array; advanced inside UpdateRK and
+ * UpdateRK2, thor_auxiliary.h:376-387 and :482-494): at the start of RK
+ * stages 2 and 3, array = arrayi - arrayk; at the end of every stage,
+ * arrayk += array. Caller-owned device arrays in THOR's layout
+ * [(p * nv + lev) * dimensions + i]; the module itself clears array and sets
+ * arrayk at dyn_core_loop_init and copies arrayk back at dyn_core_loop_end,
+ * as THOR's modules do. At most THOR_MAX_RK_ARRAYS arrays
+ * (NUM_PHY_MODULES_DYN_CORE_ARRAYS). */
+#define THOR_MAX_RK_ARRAYS 10
+thor_status thor_dyn_register_rk_array(thor_dyn *h, double *array_d, double *arrayk_d, double *arrayi_d,
+                                       int dimensions);
+thor_status thor_dyn_clear_rk_arrays(thor_dyn *h);
 
 #ifdef __cplusplus
 }

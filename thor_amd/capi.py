@@ -26,6 +26,7 @@ THOR_ERR_NAN = -3
 THOR_ERR_NO_DEVICE = -4
 THOR_ERR_IO = -5
 THOR_ERR_UNSUPPORTED = -6
+THOR_ERR_HOOK = -7
 
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -57,6 +58,26 @@ class RTParams(C.Structure):
     """thor_rt_params: double-gray two-stream radiative transfer."""
     _fields_ = [(n, C.c_double) for n in ("kappa_sw", "kappa_lw", "n_sw", "n_lw", "f_lw", "diff_ang", "Tint",
                                           "albedo", "incflx", "ps0", "lon_sub")]
+
+
+class StepView(C.Structure):
+    """thor_step_view: device pointers of the step handed to physics hooks."""
+    _fields_ = [("plane", C.c_long), ("point_num", C.c_int), ("nv", C.c_int), ("mhs_fp32", C.c_int)] + \
+        [(n, C.c_void_p) for n in ("Mhk", "Whk", "Wk", "Rhok", "pressurek", "Mhs", "Whs", "Rhos", "pressures",
+                                    "temperature", "h", "pt", "pth")]
+
+
+HOOK_INIT = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(StepView))
+HOOK_MODES = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.POINTER(StepView))
+
+
+class Hooks(C.Structure):
+    """thor_hooks: physics-module callbacks of the step (phy_modules.h:76-86)."""
+    _fields_ = [("user", C.c_void_p), ("dyn_core_loop_init", HOOK_INIT), ("dyn_core_loop_slow_modes", HOOK_MODES),
+                ("dyn_core_loop_fast_modes", HOOK_MODES), ("dyn_core_loop_end", HOOK_INIT)]
+
+
+MAX_RK_ARRAYS = 10
 
 
 class ThorError(RuntimeError):
@@ -141,6 +162,9 @@ def lib(variant: str = "") -> C.CDLL:
             L.thor_dyn_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
                                                C.POINTER(C.c_long), _dp, _dp]
             L.thor_dyn_time_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+            L.thor_dyn_set_hooks.argtypes = [C.c_void_p, C.POINTER(Hooks)]
+            L.thor_dyn_register_rk_array.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+            L.thor_dyn_clear_rk_arrays.argtypes = [C.c_void_p]
             L.thor_dyn_set_newtonian.argtypes = [C.c_void_p, C.POINTER(NewtonianParams)]
             L.thor_dyn_set_rt.argtypes = [C.c_void_p, C.POINTER(RTParams)]
             L.thor_dyn_get_rt_column.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
@@ -362,6 +386,42 @@ class Dyn:
         ms = C.c_double()
         self._chk(self.L.thor_dyn_time_run(self._h, nsteps, forcing, C.byref(ms)))
         return ms.value
+
+    def set_hooks(self, init=None, slow_modes=None, fast_modes=None, end=None):
+        """Installs Python physics hooks: init(stream, nstep, view), slow_modes /
+        fast_modes(stream, nstep, times, view), end(stream, nstep, view), where
+        stream is the handle's hipStream_t as an int and view a StepView. A
+        hook returning a true value (or raising) stops the step. None: no hook."""
+        def wrap(fn, typ, modes):
+            if fn is None:
+                return typ()
+
+            def cb(user, stream, nstep, *rest):
+                try:
+                    if modes:
+                        return 1 if fn(stream or 0, nstep, rest[0], rest[1].contents) else 0
+                    return 1 if fn(stream or 0, nstep, rest[0].contents) else 0
+                except Exception:
+                    import traceback
+                    traceback.print_exc()
+                    return 1
+            return typ(cb)
+        hk = Hooks(None, wrap(init, HOOK_INIT, False), wrap(slow_modes, HOOK_MODES, True),
+                   wrap(fast_modes, HOOK_MODES, True), wrap(end, HOOK_INIT, False))
+        self._hooks = hk  # the callbacks must outlive the handle's use of them
+        self._chk(self.L.thor_dyn_set_hooks(self._h, C.byref(hk)))
+
+    def clear_hooks(self):
+        self._chk(self.L.thor_dyn_set_hooks(self._h, None))
+        self._hooks = None
+
+    def register_rk_array(self, array_d: int, arrayk_d: int, arrayi_d: int, dimensions: int):
+        """Registers device arrays (addresses, THOR layout [(p*nv + lev)*dims + i])
+        that the RK scheme advances like the state."""
+        self._chk(self.L.thor_dyn_register_rk_array(self._h, array_d, arrayk_d, arrayi_d, dimensions))
+
+    def clear_rk_arrays(self):
+        self._chk(self.L.thor_dyn_clear_rk_arrays(self._h))
 
 
 class MLP:

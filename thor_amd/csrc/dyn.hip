@@ -58,6 +58,13 @@ struct Dyn {
     std::vector<std::string> stat_order;
     double diff_sign = 1.0;
     bool mixed = false;  // THOR_PRECISION_MIXED
+    int lc = 12;         // levels per block of the mixed-precision momentum and divergence stencils
+    int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
+    int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
+    // physics-module hooks and RK-advanced module arrays
+    thor_hooks hooks{};
+    RKArrays rka{};
+    long nstep = 0;  // Thor steps completed by this handle
     // forcings
     bool nr_set = false, rt_set = false;
     NRPar nr{};
@@ -187,11 +194,53 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     const int hydiff = d.prm.HyDiff, divdamp = d.prm.DivDampP;
     const dim3 blk(kTile), grid_s((d.P + kTile - 1) / kTile, (d.nv + kLC - 1) / kLC);
     const dim3 grid_t2(8 * d.g.tiles_per_xcd * d.nv);  // LDS-tiled stencils: one tile x one level per block
+    // mixed-precision substep stencils: one tile x lc levels per block
+    const dim3 grid_lc(8 * d.g.tiles_per_xcd * ((d.nv + d.lc - 1) / d.lc));
+    const dim3 grid_lp(8 * d.g.tiles_per_xcd * ((d.nv + d.lc_prep - 1) / d.lc_prep));
+    const int lc = d.lc, lcp = d.lc_prep;
     const dim3 grid_c((d.P + kTile - 1) / kTile, d.nv + 1);  // column_prep: (column, level 0..nv)
     const int cb = kThomasCB(d.nv);
     const dim3 grid_t((d.P + cb - 1) / cb);
     const size_t lds_t = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
     hipStream_t st = d.stream;
+    const int nstep = (int)(d.nstep + 1);
+    auto view = [&](const Fields &f) {
+        thor_step_view v{};
+        v.plane = d.S;
+        v.point_num = d.P;
+        v.nv = d.nv;
+        v.mhs_fp32 = MX ? 1 : 0;
+        v.Mhk = f.Mhk;
+        v.Whk = f.Whk;
+        v.Wk = f.Wk;
+        v.Rhok = f.Rhok;
+        v.pressurek = f.pk;
+        v.Mhs = MX ? (const void *)f.mMhs : (const void *)f.Mhs;
+        v.Whs = f.Whs;
+        v.Rhos = f.Rhos;
+        v.pressures = f.ps;
+        v.temperature = f.T;
+        v.h = f.h;
+        v.pt = f.pt;
+        v.pth = f.pth;
+        return v;
+    };
+    auto hook_fail = [&](const char *which) {
+        return fail(THOR_ERR_HOOK, std::string("thor step: physics hook ") + which + " returned an error");
+    };
+    const long rk_count = d.rka.n ? *std::max_element(d.rka.count, d.rka.count + d.rka.n) : 0;
+    auto rk_arrays = [&](int mode) {
+        if (!d.rka.n) return;
+        double bytes = 0.0;
+        for (int j = 0; j < d.rka.n; j++) bytes += 8.0 * d.rka.count[j] * 3;
+        launch(d, mode == 0 ? "rk_arrays_start" : "rk_arrays_end", bytes, [&] {
+            k_rk_arrays<<<(unsigned)((rk_count + 255) / 256), blk, 0, st>>>(d.rka, mode);
+        });
+    };
+    if (d.hooks.dyn_core_loop_init) {
+        const thor_step_view v = view(stage_fields(d, 0));
+        if (d.hooks.dyn_core_loop_init(d.hooks.user, (void *)st, nstep, &v)) return hook_fail("dyn_core_loop_init");
+    }
     for (int rk = 0; rk < 3; rk++) {
         const int ns_it = rk == 0 ? 1 : (rk == 1 ? ns_total / 2 : ns_total);
         const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
@@ -205,29 +254,64 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                    [&] {
                 k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
             });
+        if (d.hooks.dyn_core_loop_slow_modes) {
+            const thor_step_view v = view(f);
+            if (d.hooks.dyn_core_loop_slow_modes(d.hooks.user, (void *)st, nstep, times, &v))
+                return hook_fail("dyn_core_loop_slow_modes");
+        }
         launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
-        if (divdamp) launch(d, "divm0", W(1, 6), [&] { k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f); });
+        if (rk > 0) rk_arrays(0);
+        auto divm0 = [&] {
+            launch(d, "divm0", W(1, 6), [&] {
+                if (MX) k_divm0_lc<<<grid_lc, blk, 0, st>>>(g, f, lc);
+                else k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f);
+            });
+        };
+        if (divdamp) divm0();
         launch(d, "thomas_factor", 5 * B, [&] {
             if (cb == 32) k_thomas_factor<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
             else k_thomas_factor<16, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
         });
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
-            launch(d, "momentum", W(1, (divdamp ? 6 : 0) + 12),
-                   [&] { k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times); });
+            launch(d, "momentum", W(1, (divdamp ? 6 : 0) + 12), [&] {
+                if (MX) k_momentum_lc<<<grid_lc, blk, 0, st>>>(g, f, divdamp, times, lc);
+                else k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times);
+            });
             // Prepare_Implicit_Vertical, plus the horizontal flux divergences of
             // Density_Pressure_Eqs (booked with the column kernel below)
-            launch(d, "prepare", W(7, 3), [&] { k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f); });
-            // Vertical_Eq (12) + Density_Pressure_Eqs (22, of which Mhs and diffpr FP32 in mixed)
-            launch(d, "thomas_dp", W(12 + 18, 4), [&] {
-                if (cb == 32) k_thomas_dp<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
-                else k_thomas_dp<16, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            launch(d, "prepare", W(7, 3), [&] {
+                if (MX) k_prepare_lc<<<grid_lp, blk, 0, st>>>(g, f, lcp);
+                else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
+            // Vertical_Eq (12) + Density_Pressure_Eqs (22, of which Mhs and diffpr FP32 in mixed)
+            const size_t lds = (size_t)(d.nv + 1) * d.tcb * sizeof(double);
+            const dim3 grid_th((d.P + d.tcb - 1) / d.tcb);
+            if (!d.hooks.dyn_core_loop_fast_modes) {
+                launch(d, "thomas_dp", W(12 + 18, 4), [&] {
+                    if (d.tcb == 64) k_thomas_dp<64, MX><<<grid_th, 256, lds, st>>>(g, f, times);
+                    else if (d.tcb == 32) k_thomas_dp<32, MX><<<grid_th, 256, lds, st>>>(g, f, times);
+                    else k_thomas_dp<16, MX><<<grid_th, 256, lds, st>>>(g, f, times);
+                });
+            } else {
+                // the module runs between the vertical solve and the density/pressure update
+                launch(d, "thomas", W(12, 0), [&] {
+                    if (d.tcb == 64) k_thomas_dp<64, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
+                    else if (d.tcb == 32) k_thomas_dp<32, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
+                    else k_thomas_dp<16, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
+                });
+                const thor_step_view v = view(f);
+                if (d.hooks.dyn_core_loop_fast_modes(d.hooks.user, (void *)st, nstep, times, &v))
+                    return hook_fail("dyn_core_loop_fast_modes");
+                launch(d, "density_pressure", W(18, 4),
+                       [&] { k_dp<MX><<<dim3((d.P + kTile - 1) / kTile, d.nv), blk, 0, st>>>(g, f, times); });
+            }
             // next substep's divergence-damping pass 0
-            if (next) launch(d, "divm0", W(1, 6), [&] { k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f); });
+            if (next) divm0();
         }
         launch(d, "update_rk2", W(16, 3), [&] { k_update_rk2<kLC, MX><<<grid_s, blk, 0, st>>>(g, f); });
+        rk_arrays(1);
     }
     std::swap(d.Mh, d.Mhk);
     std::swap(d.Wh, d.WhkA);
@@ -236,6 +320,11 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     std::swap(d.p, d.pk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("thor step launch: ") + hipGetErrorString(e));
+    d.nstep++;
+    if (d.hooks.dyn_core_loop_end) {
+        const thor_step_view v = view(stage_fields(d, 0));
+        if (d.hooks.dyn_core_loop_end(d.hooks.user, (void *)st, nstep, &v)) return hook_fail("dyn_core_loop_end");
+    }
     return THOR_OK;
 }
 
@@ -396,6 +485,16 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     Dyn &d = *h;
     d.prm = *prm;
     d.mixed = prm->precision == THOR_PRECISION_MIXED;
+    // Levels per block of the mixed-precision substep stencils (tuning knob).
+    if (const char *e = std::getenv("THOR_STENCIL_LEVELS")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= 128) d.lc = d.lc_prep = v;
+    }
+    // Columns per block of the substep Thomas kernel (tuning knob).
+    if (const char *e = std::getenv("THOR_THOMAS_COLUMNS")) {
+        const int v = std::atoi(e);
+        if (v == 16 || v == 32 || v == 64) d.tcb = v;
+    }
     d.P = gd->point_num;
     d.nv = gd->nv;
     d.S = (d.P + kAlign - 1) / kAlign * kAlign;
@@ -774,6 +873,36 @@ extern "C" thor_status thor_dyn_set_field(thor_dyn *h, const char *name, const d
     if (n != (long)h->P * nlev * ncomp) return fail(THOR_ERR_BAD_ARG, "set_field: size mismatch");
     HIP_TRY(hipStreamSynchronize(h->stream));
     return copy_field(*h, dev, nlev, ncomp, const_cast<double *>(src), true, false);
+}
+
+extern "C" thor_status thor_dyn_set_hooks(thor_dyn *h, const thor_hooks *hooks) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "thor_dyn_set_hooks: null handle");
+    h->hooks = hooks ? *hooks : thor_hooks{};
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_register_rk_array(thor_dyn *h, double *array_d, double *arrayk_d, double *arrayi_d,
+                                                  int dimensions) {
+    using namespace thor;
+    if (!h || !array_d || !arrayk_d || !arrayi_d || dimensions < 1)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_register_rk_array: null array or dimensions < 1");
+    thor::Dyn &d = *h;
+    if (d.rka.n >= THOR_MAX_RK_ARRAYS)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_register_rk_array: at most THOR_MAX_RK_ARRAYS arrays");
+    const int j = d.rka.n++;
+    d.rka.a[j] = array_d;
+    d.rka.ak[j] = arrayk_d;
+    d.rka.ai[j] = arrayi_d;
+    d.rka.count[j] = (long)d.P * d.nv * dimensions;
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_clear_rk_arrays(thor_dyn *h) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "thor_dyn_clear_rk_arrays: null handle");
+    h->rka = thor::RKArrays{};
+    return THOR_OK;
 }
 
 extern "C" thor_status thor_dyn_kernel_timing(thor_dyn *h, int enable) {

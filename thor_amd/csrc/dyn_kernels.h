@@ -1113,22 +1113,24 @@ __host__ __device__ constexpr int kThomasCB(int nv) { return 4 * (nv + 1) * 32 *
 // Density_Pressure_Eqs (thor_fastmodes.h:255-847), which see the same Mhs:
 // hflx_r = rscale sum div Mhs (the Sd sum, accumulated with the opposite sign)
 // and hflx_pt = rscale sum div (Mhs + Mhk) pt.
-template <class GetM, class GetH, class GetV1, class GetPt>
-__device__ __forceinline__ void prepare_point(const Geo &g, const Fields &f, int p, int lev, GetM mv, GetH hv,
-                                              GetV1 v1, GetPt ptv) {
+// wv(s, k) is the divergence coefficient of slot s, component k; U is the
+// unroll count of the slot loop (7 when wv reads a register array).
+template <int U, class GetW, class GetM, class GetH, class GetV1, class GetPt>
+__device__ __forceinline__ void prepare_point_w(const Geo &g, const Fields &f, int p, int lev, GetW wv, GetM mv,
+                                                GetH hv, GetV1 v1, GetPt ptv, double slowp, double slowrho) {
     const size_t S = g.S;
     const double RoC = g.Rd / (g.Cp - g.Rd);
     const double rs = rscale_at(g, lev);
     double nflxr = 0.0, nflxp = 0.0, dr = 0.0, dpt = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const double d0 = g.div[(size_t)(0 * 3 + k) * S + p];
+        const double d0 = wv(0, k);
         double sr = d0 * mv(0, k);
         double sp = d0 * mv(0, k) * hv(0);
         double st = d0 * v1(0, k) * ptv(0);
-SLOT_UNROLL(THOR_SLOT_UNROLL)
+#pragma unroll U
         for (int s = 1; s < 7; s++) {
-            const double dd = g.div[(size_t)(s * 3 + k) * S + p];
+            const double dd = wv(s, k);
             sr = sr + dd * mv(s, k);
             sp = sp + dd * mv(s, k) * hv(s);
             st = st + dd * v1(s, k) * ptv(s);
@@ -1138,10 +1140,19 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         dr += rs * sr;
         dpt += rs * st;
     }
-    LV(f.Sp, lev) = nflxp + LV(f.Slowp, lev);
-    LV(f.Sd, lev) = nflxr + LV(f.SlowRho, lev);
+    LV(f.Sp, lev) = nflxp + slowp;
+    LV(f.Sd, lev) = nflxr + slowrho;
     LV(f.hflx_r, lev) = dr;
     LV(f.hflx_pt, lev) = dpt;
+}
+
+template <class GetM, class GetH, class GetV1, class GetPt>
+__device__ __forceinline__ void prepare_point(const Geo &g, const Fields &f, int p, int lev, GetM mv, GetH hv,
+                                              GetV1 v1, GetPt ptv) {
+    const size_t S = g.S;
+    prepare_point_w<THOR_SLOT_UNROLL>(
+        g, f, p, lev, [&](int s, int k) { return g.div[(size_t)(s * 3 + k) * S + p]; }, mv, hv, v1, ptv,
+        LV(f.Slowp, lev), LV(f.SlowRho, lev));
 }
 
 template <bool MX> __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
@@ -1186,6 +1197,397 @@ template <bool MX> __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
                   [&](int s) { return sPt[T.pos(s)]; });
 }
 
+// ============================================== mixed precision, level chunks
+// The substep stencils of the mixed-precision path (momentum, prepare and the
+// divergence-damping pass 0) run one block per (rhombus tile, chunk of lc
+// consecutive levels). Their per-point operator weights do not depend on the
+// level, so each thread loads them once into registers and reuses them at
+// every level of its chunk. The tile of level lev + 1 and the thread's own
+// inputs for lev + 1 are loaded while level lev is computed and then written
+// to the second of two LDS buffers, so each level costs one barrier and the
+// loads of the next level overlap the arithmetic of this one. Block i runs on
+// XCD i % 8 and each XCD takes a contiguous range of tiles. The pole block
+// walks the same levels with direct gathers.
+__device__ __forceinline__ bool block_tile_lc(const Geo &g, int lc, int &r, int &l0, int &l1) {
+    const int i = blockIdx.x, xcd = i & 7, idx = i >> 3;
+    const int nch = (g.nv + lc - 1) / lc;
+    r = xcd * g.tiles_per_xcd + idx / nch;
+    l0 = (idx % nch) * lc;
+    l1 = min(g.nv, l0 + lc);
+    return r <= g.nr;
+}
+
+// Buffer loads: one 128-bit resource descriptor per array (scalar registers),
+// a 32-bit per-lane byte offset (the point) and a wave-uniform byte offset
+// (level and component). A level-chunked kernel then keeps one 32-bit offset
+// per staged point instead of a 64-bit address per array. A load outside
+// [0, bytes) returns 0 instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0,
+                                             (int)(bytes < 0x7fffffffu ? bytes : 0x7fffffffu), 0x00020000);
+}
+__device__ __forceinline__ float ldb_f(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ double ldb_d(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void stb_f(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void stb_d(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, double v) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+}
+
+// Pole block of a level-chunked kernel: thread t takes pole P - 2 + (t & 1) at
+// level l0 + t / 2 (two poles x lc levels <= 256 threads).
+__device__ __forceinline__ bool pole_ctx_lc(const Geo &g, int l0, int l1, PoleCtx &c, int &lev) {
+    const int t = threadIdx.x;
+    lev = l0 + (t >> 1);
+    if (lev >= l1) return false;
+    c.p = g.P - 2 + (t & 1);
+    c.ring = g.ring;
+    c.S = g.S;
+    return true;
+}
+
+#ifndef THOR_LC_MIN_BLOCKS
+#define THOR_LC_MIN_BLOCKS 3
+#endif
+#define LC_BOUNDS __launch_bounds__(256, THOR_LC_MIN_BLOCKS)
+
+// Momentum_Eq (thor_fastmodes.h:54-253) after the divergence-damping pass 1,
+// in FP32: gradient of the FP64 pressure deviation with FP32 weights, DivM
+// from the probed FP32 weights, FP32 momentum deviation.
+__global__ LC_BOUNDS void k_momentum_lc(Geo g, Fields f, int divdamp, double dt, int lc) {
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    float *Mhs = f.mMhs;
+    const float *divg = f.mdivg, *SlowMh = f.mSlowMh;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev))
+            momentum_point<true>(
+                g, f, c.p, lev, 5, true, divdamp, dt, [&](int s, int k) { return VEC(divg, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.ps, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ float sG[2][3][NHL2], sP[2][NHL2];
+    const TileThread T = tile_thread(g, r);
+    const int p = T.p, t = threadIdx.x;
+    const bool hth = t < NHALO;
+    const int qh = hth ? g.halo[r * NHALO + t] : -1;
+    const int hp = hth ? halo_pos(t) : 0;
+    int pos[7];
+#pragma unroll
+    for (int s = 0; s < 7; s++) pos[s] = T.pos(s);
+    float wg[21], wd[63];
+#pragma unroll
+    for (int i = 0; i < 21; i++) wg[i] = (float)g.grad[(size_t)i * S + p];
+    if (divdamp) {
+#pragma unroll
+        for (int i = 0; i < 63; i++) wd[i] = g.wdd[(size_t)i * S + p];
+    }
+    const float fx = (float)g.fr[p], fy = (float)g.fr[S + p], fz = (float)g.fr[2 * S + p];
+    const float dtf = (float)dt;
+    const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
+    const auto rG = rsrc(divg, 3 * nv * pl4), rP = rsrc(f.ps, nv * pl8), rS = rsrc(SlowMh, 3 * nv * pl4),
+               rM = rsrc(Mhs, 3 * nv * pl4);
+    const unsigned o4 = 4u * p, o8 = 8u * p, h4 = 4u * (unsigned)qh, h8 = 8u * (unsigned)qh;
+    struct In {
+        float og[3], hg[3], op, hpv, sm[3], mh[3];
+    };
+    auto load = [&](In &v, int lev) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) v.og[k] = v.hg[k] = 0.f;
+        v.hpv = 0.f;
+        if (divdamp) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) v.og[k] = ldb_f(rG, o4, (unsigned)((k * nv + lev) * pl4));
+        }
+        v.op = (float)ldb_d(rP, o8, (unsigned)(lev * pl8));
+        if (qh >= 0) {
+            if (divdamp) {
+#pragma unroll
+                for (int k = 0; k < 3; k++) v.hg[k] = ldb_f(rG, h4, (unsigned)((k * nv + lev) * pl4));
+            }
+            v.hpv = (float)ldb_d(rP, h8, (unsigned)(lev * pl8));
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.sm[k] = ldb_f(rS, o4, (unsigned)((k * nv + lev) * pl4));
+            v.mh[k] = ldb_f(rM, o4, (unsigned)((k * nv + lev) * pl4));
+        }
+    };
+    auto put = [&](const In &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) sG[b][k][T.ir] = v.og[k];
+        sP[b][T.ir] = v.op;
+        if (hth) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sG[b][k][hp] = v.hg[k];
+            sP[b][hp] = v.hpv;
+        }
+    };
+    In cur;
+    load(cur, l0);
+    put(cur, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev++) {
+        const int b = (lev - l0) & 1;
+        const bool more = lev + 1 < l1;
+        In nxt;
+        if (more) load(nxt, lev + 1);
+        const double rsd = rscale_at(g, lev);
+        const float rs = (float)rsd, rs2 = (float)(rsd * rsd);
+        float DivM[3] = {0.f, 0.f, 0.f};
+        if (divdamp) {
+            float o[3];
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                float sum = 0.f;
+#pragma unroll
+                for (int s = 0; s < 7; s++)
+#pragma unroll
+                    for (int k = 0; k < 3; k++) sum += wd[c * 21 + s * 3 + k] * sG[b][k][pos[s]];
+                o[c] = rs2 * sum;
+            }
+            const float sdiff = (float)g.Kdhz[lev];
+            const float lapr = fx * o[0] + fy * o[1] + fz * o[2];
+            DivM[0] = -(o[0] - fx * lapr) * sdiff;
+            DivM[1] = -(o[1] - fy * lapr) * sdiff;
+            DivM[2] = -(o[2] - fz * lapr) * sdiff;
+        }
+        float M[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            float s0 = 0.f;
+#pragma unroll
+            for (int s = 0; s < 7; s++) s0 += wg[s * 3 + k] * sP[b][pos[s]];
+            M[k] = (-rs * s0 + cur.sm[k] + DivM[k]) * dtf;
+        }
+        const float vr = M[0] * fx + M[1] * fy + M[2] * fz;
+        stb_f(rM, o4, (unsigned)((0 * nv + lev) * pl4), cur.mh[0] + (M[0] - vr * fx));
+        stb_f(rM, o4, (unsigned)((1 * nv + lev) * pl4), cur.mh[1] + (M[1] - vr * fy));
+        stb_f(rM, o4, (unsigned)((2 * nv + lev) * pl4), cur.mh[2] + (M[2] - vr * fz));
+        if (more) {
+            put(nxt, b ^ 1);
+            cur = nxt;
+        }
+        __syncthreads();
+    }
+}
+
+// Prepare_Implicit_Vertical (thor_vertical_int.h:566-771) and the horizontal
+// flux divergences of Density_Pressure_Eqs in FP64 arithmetic, as in
+// prepare_point (the momentum deviation is read from its FP32 storage).
+__global__ LC_BOUNDS void k_prepare_lc(Geo g, Fields f, int lc) {
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const float *Mhs = f.mMhs;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev))
+            prepare_point(
+                g, f, c.p, lev, [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.h, lev, c.gi(s)); },
+                [&](int s, int k) { return (double)VEC(Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ float sM[2][3][NHL2];
+    __shared__ double sH[2][NHL2], sV1[2][3][NHL2], sPt[2][NHL2];
+    const TileThread T = tile_thread(g, r);
+    const int p = T.p, t = threadIdx.x;
+    const bool hth = t < NHALO;
+    const int qh = hth ? g.halo[r * NHALO + t] : -1;
+    const int hp = hth ? halo_pos(t) : 0;
+    int pos[7];
+#pragma unroll
+    for (int s = 0; s < 7; s++) pos[s] = T.pos(s);
+    double wv[21];
+#pragma unroll
+    for (int i = 0; i < 21; i++) wv[i] = g.div[(size_t)i * S + p];
+    const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
+    const auto rM = rsrc(Mhs, 3 * nv * pl4), rMk = rsrc(f.Mhk, 3 * nv * pl8), rH = rsrc(f.h, nv * pl8),
+               rPt = rsrc(f.pt, nv * pl8), rSp = rsrc(f.Slowp, nv * pl8), rSr = rsrc(f.SlowRho, nv * pl8);
+    const unsigned o4 = 4u * p, o8 = 8u * p, h4 = 4u * (unsigned)qh, h8 = 8u * (unsigned)qh;
+    struct In {
+        float om[3], hm[3];
+        double omk[3], hmk[3], oh, hh, opt, hpt, slowp, slowrho;
+    };
+    auto load = [&](In &v, int lev) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.om[k] = ldb_f(rM, o4, (unsigned)((k * nv + lev) * pl4));
+            v.omk[k] = ldb_d(rMk, o8, (unsigned)((k * nv + lev) * pl8));
+            v.hm[k] = 0.f;
+            v.hmk[k] = 0.0;
+        }
+        v.oh = ldb_d(rH, o8, (unsigned)(lev * pl8));
+        v.opt = ldb_d(rPt, o8, (unsigned)(lev * pl8));
+        v.hh = v.hpt = 0.0;
+        if (qh >= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                v.hm[k] = ldb_f(rM, h4, (unsigned)((k * nv + lev) * pl4));
+                v.hmk[k] = ldb_d(rMk, h8, (unsigned)((k * nv + lev) * pl8));
+            }
+            v.hh = ldb_d(rH, h8, (unsigned)(lev * pl8));
+            v.hpt = ldb_d(rPt, h8, (unsigned)(lev * pl8));
+        }
+        v.slowp = ldb_d(rSp, o8, (unsigned)(lev * pl8));
+        v.slowrho = ldb_d(rSr, o8, (unsigned)(lev * pl8));
+    };
+    auto put = [&](const In &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sM[b][k][T.ir] = v.om[k];
+            sV1[b][k][T.ir] = (double)v.om[k] + v.omk[k];
+        }
+        sH[b][T.ir] = v.oh;
+        sPt[b][T.ir] = v.opt;
+        if (hth) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sM[b][k][hp] = v.hm[k];
+                sV1[b][k][hp] = qh >= 0 ? (double)v.hm[k] + v.hmk[k] : 0.0;
+            }
+            sH[b][hp] = v.hh;
+            sPt[b][hp] = v.hpt;
+        }
+    };
+    In cur;
+    load(cur, l0);
+    put(cur, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev++) {
+        const int b = (lev - l0) & 1;
+        const bool more = lev + 1 < l1;
+        In nxt;
+        if (more) load(nxt, lev + 1);
+        prepare_point_w<7>(
+            g, f, p, lev, [&](int s, int k) { return wv[s * 3 + k]; },
+            [&](int s, int k) { return (double)sM[b][k][pos[s]]; }, [&](int s) { return sH[b][pos[s]]; },
+            [&](int s, int k) { return sV1[b][k][pos[s]]; }, [&](int s) { return sPt[b][pos[s]]; }, cur.slowp,
+            cur.slowrho);
+        if (more) {
+            put(nxt, b ^ 1);
+            cur = nxt;
+        }
+        __syncthreads();
+    }
+}
+
+// Divergence-damping pass 0 (DivM_Op) on the deviations Mhs / Whs in FP32,
+// as divm_w<false>. Whs lives on interfaces: the top value of level lev is
+// the bottom value of lev + 1, so each level loads one new interface.
+__global__ LC_BOUNDS void k_divm0_lc(Geo g, Fields f, int lc) {
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const float *Mhs = f.mMhs;
+    float *divg = f.mdivg;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev)) {
+            float o3[3];
+            divm_w<false>(
+                g, c.p, lev, [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)); },
+                [&](int s) { return (float)(LVP(f.Whs, lev, c.gi(s)) - LVP(f.Whs, lev + 1, c.gi(s))); }, o3);
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(divg, k, lev, c.p) = o3[k];
+        }
+        return;
+    }
+    __shared__ float sM[2][3][NHL2], sD[2][NHL2];
+    const TileThread T = tile_thread(g, r);
+    const int p = T.p, t = threadIdx.x;
+    const bool hth = t < NHALO;
+    const int qh = hth ? g.halo[r * NHALO + t] : -1;
+    const int hp = hth ? halo_pos(t) : 0;
+    int pos[7];
+#pragma unroll
+    for (int s = 0; s < 7; s++) pos[s] = T.pos(s);
+    float wd[63], wu[21];
+#pragma unroll
+    for (int i = 0; i < 63; i++) wd[i] = g.wdd[(size_t)i * S + p];
+#pragma unroll
+    for (int i = 0; i < 21; i++) wu[i] = g.udd[(size_t)i * S + p];
+    const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
+    const auto rM = rsrc(Mhs, 3 * nv * pl4), rW = rsrc(f.Whs, (nv + 1) * pl8), rD = rsrc(divg, 3 * nv * pl4);
+    const unsigned o4 = 4u * p, o8 = 8u * p, h4 = 4u * (unsigned)qh, h8 = 8u * (unsigned)qh;
+    // loaded values of one level: Mhs and the top interface's Whs
+    struct In {
+        float om[3], hm[3];
+        double ow, hw;
+    };
+    auto load = [&](In &v, int lev) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.om[k] = ldb_f(rM, o4, (unsigned)((k * nv + lev) * pl4));
+            v.hm[k] = 0.f;
+        }
+        v.ow = ldb_d(rW, o8, (unsigned)((lev + 1) * pl8));
+        v.hw = 0.0;
+        if (qh >= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) v.hm[k] = ldb_f(rM, h4, (unsigned)((k * nv + lev) * pl4));
+            v.hw = ldb_d(rW, h8, (unsigned)((lev + 1) * pl8));
+        }
+    };
+    // wl: Whs at the bottom interface of the level being put
+    double owl = ldb_d(rW, o8, (unsigned)(l0 * pl8)), hwl = qh >= 0 ? ldb_d(rW, h8, (unsigned)(l0 * pl8)) : 0.0;
+    auto put = [&](const In &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) sM[b][k][T.ir] = v.om[k];
+        sD[b][T.ir] = (float)(owl - v.ow);
+        owl = v.ow;
+        if (hth) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sM[b][k][hp] = v.hm[k];
+            sD[b][hp] = (float)(hwl - v.hw);
+            hwl = v.hw;
+        }
+    };
+    In cur;
+    load(cur, l0);
+    put(cur, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev++) {
+        const int b = (lev - l0) & 1;
+        const bool more = lev + 1 < l1;
+        In nxt;
+        if (more) load(nxt, lev + 1);
+        const double rsd = rscale_at(g, lev);
+        const float rs = (float)rsd, rs2 = (float)(rsd * rsd);
+        const float dz = (float)(1.0 / (g.Alth[lev] - g.Alth[lev + 1]));
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float sum = 0.f;
+#pragma unroll
+            for (int s = 0; s < 7; s++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) sum += wd[c * 21 + s * 3 + k] * sM[b][k][pos[s]];
+            float su = 0.f;
+#pragma unroll
+            for (int s = 0; s < 7; s++) su += wu[c * 7 + s] * sD[b][pos[s]];
+            stb_f(rD, o4, (unsigned)((c * nv + lev) * pl4), rs2 * sum + rs * dz * su);
+        }
+        if (more) put(nxt, b ^ 1);
+        __syncthreads();
+    }
+}
+
 // Vertical_Eq (thor_vertical_int.h:52-563): per column, the tridiagonal solve
 // for Whs at interfaces 1..nv-1 with the modify-in-place Thomas algorithm;
 // cc/dd live in LDS.
@@ -1201,12 +1603,13 @@ struct Plane {
     __device__ __forceinline__ double &operator[](int lev) const { return b[(size_t)lev * S + p]; }
 };
 
-// Thomas kernels: a block of 256 threads serves kThomasCB(nv) columns. First
-// every thread computes per-level terms (coefficients or right-hand sides)
-// for a share of the levels of its column into LDS, all loads in flight at
-// once; then one lane per column runs the serial recurrence out of LDS. The
+// Thomas kernels: a block of 256 threads serves CB columns. First every
+// thread computes per-level terms (coefficients or right-hand sides) for a
+// share of the levels of its column into LDS, all loads in flight at once;
+// then one lane per column runs the serial recurrence out of LDS. The
 // arithmetic of each term and of the recurrence is the reference's.
-// LDS: 4 planes of (nv + 1) x CB doubles.
+// LDS: k_thomas_factor 4 planes of (nv + 1) x kThomasCB(nv) doubles;
+// k_thomas_dp one plane of (nv + 1) x CB doubles.
 template <int CB, bool MX>
 __global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double deltat) {
     extern __shared__ double sm[];
@@ -1336,21 +1739,22 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
 // Whs = 0 at the bottom and top interfaces), followed by the column part of
 // the substep's Density_Pressure_Eqs on the new Whs, which the block keeps in
 // LDS: all threads take a share of the levels again after the solve.
-template <int CB, bool MX>
+// DP = false: the vertical solve alone (Whs written, density/pressure left to
+// k_dp), for a physics hook that runs between the two as in the reference.
+template <int CB, bool MX, bool DP = true>
 __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double deltat) {
     extern __shared__ double sm[];
     const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
     constexpr int NROW = 256 / CB;
     const int p = blockIdx.x * CB + c;
     const size_t S = g.S;
-    double *sD = sm, *sA = sm + (nv + 1) * CB, *sT = sm + 2 * (nv + 1) * CB, *sC = sm + 3 * (nv + 1) * CB;
+    double *sD = sm;
     const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
     const double *Alt = g.Alt, *Alth = g.Alth;
     const bool ghv = g.ghv, nonh = g.nonhydro;
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
     const double dt2 = XPOW(deltat, 2.0);
     if (p < g.P) {
-        const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
         for (int lev = 1 + row; lev < nv; lev += NROW) {
             const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
             const double Sp = LV(f.Sp, lev), Sd = LV(f.Sd, lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
@@ -1385,27 +1789,27 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
                          * (CRdd);
             }
             sD[lev * CB + c] = C0;
-            sA[lev * CB + c] = lev > 1 ? AA[lev] : 0.0;  // AA[1] is never formed
-            sT[lev * CB + c] = TT[lev];
-            sC[lev * CB + c] = CC[lev];
         }
     }
     __syncthreads();
+    // One lane per column: forward sweep and back substitution out of LDS,
+    // with the stage's factorisation read straight from HBM (its loads do not
+    // depend on the recurrence, so they are issued ahead of it).
     if (row == 0 && p < g.P) {
-        const Plane W{f.Whs, S, p};
-        double ddp = 0.0;
-        for (int lev = 1; lev < nv; lev++) {
-            const double C0 = sD[lev * CB + c];
-            if (lev == 1) ddp = (-C0) / sT[CB + c];
-            else ddp = (-C0 - ddp * sA[lev * CB + c]) * sT[lev * CB + c];
-            sD[lev * CB + c] = ddp;
+        const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+        if (nv > 1) {
+            double ddp = (-sD[CB + c]) / TT[1];
+            sD[CB + c] = ddp;
+#pragma unroll 8
+            for (int lev = 2; lev < nv; lev++) {
+                ddp = (-sD[lev * CB + c] - ddp * AA[lev]) * TT[lev];
+                sD[lev * CB + c] = ddp;
+            }
         }
-        W[nv] = 0.0;
-        W[0] = 0.0;
         double wup = 0.0;
+#pragma unroll 8
         for (int lev = nv - 1; lev > 0; lev--) {
-            const double w = (-sC[lev * CB + c] * wup + sD[lev * CB + c]);
-            W[lev] = w;
+            const double w = (-CC[lev] * wup + sD[lev * CB + c]);
             sD[lev * CB + c] = w;
             wup = w;
         }
@@ -1414,10 +1818,42 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     }
     __syncthreads();
     if (p >= g.P) return;
+    {
+        const Plane W{f.Whs, S, p};
+        for (int lev = row; lev <= nv; lev += NROW) W[lev] = sD[lev * CB + c];
+    }
+    if (!DP) return;
     const bool pole = p >= g.P - 2;
     for (int lev = row; lev < nv; lev += NROW)
         dp_column<MX>(g, f, p, lev, pole, deltat, LV(f.hflx_r, lev), LV(f.hflx_pt, lev),
                       [&](int k) { return sD[k * CB + c]; });
+}
+
+// Column part of Density_Pressure_Eqs after a separate vertical solve
+// (k_thomas_dp<.., false>): one thread per (point, level), Whs from HBM.
+template <bool MX> __global__ __launch_bounds__(256) void k_dp(Geo g, Fields f, double deltat) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    dp_column<MX>(g, f, p, lev, p >= g.P - 2, deltat, LV(f.hflx_r, lev), LV(f.hflx_pt, lev),
+                  [&](int k) { return LVP(f.Whs, k, p); });
+}
+
+// Physics-module arrays advanced by the RK scheme (UpdateRK / UpdateRK2,
+// thor_auxiliary.h:376-387, 482-494), THOR layout [(p * nv + lev) * dims + i].
+struct RKArrays {
+    int n;
+    double *a[10], *ak[10], *ai[10];
+    long count[10];  // P * nv * dims
+};
+// mode 0 (start of RK stages 2, 3): a = ai - ak; mode 1 (end of a stage): ak += a.
+__global__ __launch_bounds__(256) void k_rk_arrays(RKArrays r, int mode) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int j = 0; j < r.n; j++) {
+        if (i >= r.count[j]) continue;
+        if (mode == 0) r.a[j][i] = r.ai[j][i] - r.ak[j][i];
+        else r.ak[j][i] += r.a[j][i];
+    }
 }
 
 // ===================================================================== K_H
