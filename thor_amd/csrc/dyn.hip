@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -58,7 +59,7 @@ struct Dyn {
     std::vector<std::string> stat_order;
     double diff_sign = 1.0;
     bool mixed = false;  // THOR_PRECISION_MIXED
-    int lc = 12;         // levels per block of the mixed-precision momentum and divergence stencils
+    int lc = 16;         // levels per block of the mixed-precision momentum and divergence stencils
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
     // physics-module hooks and RK-advanced module arrays
@@ -488,7 +489,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     // Levels per block of the mixed-precision substep stencils (tuning knob).
     if (const char *e = std::getenv("THOR_STENCIL_LEVELS")) {
         const int v = std::atoi(e);
-        if (v >= 1 && v <= 128) d.lc = d.lc_prep = v;
+        if (v >= 1 && v <= kLcMax) d.lc = d.lc_prep = v;
     }
     // Columns per block of the substep Thomas kernel (tuning knob).
     if (const char *e = std::getenv("THOR_THOMAS_COLUMNS")) {
@@ -613,8 +614,21 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         };
         F(&f.mMhs, n3); F(&f.mAdv, n3); F(&f.mdiff, 6 * n1); F(&f.mdiffmh, n3);
         F(&f.mdiffw, n1); F(&f.mdiffrh, n1); F(&f.mdiffpr, n1); F(&f.mdivg, n3); F(&f.mSlowMh, n3);
-        float *wlap = nullptr, *wdd = nullptr, *udd = nullptr;
-        F(&wlap, (size_t)7 * S); F(&wdd, (size_t)63 * S); F(&udd, (size_t)21 * S);
+        float *wlap = nullptr, *wdd = nullptr, *udd = nullptr, *lvc = nullptr;
+        F(&wlap, (size_t)7 * S); F(&wdd, (size_t)63 * S); F(&udd, (size_t)21 * S); F(&lvc, (size_t)4 * nv);
+        if (e == hipSuccess) {
+            // per-level constants, with the same double arithmetic as rscale_at and divm_w
+            std::vector<float> lv(4 * (size_t)nv);
+            for (int lev = 0; lev < nv; lev++) {
+                const double rsd = prm->DeepModel ? prm->A / (gd->Altitude[lev] + prm->A) : 1.0;
+                lv[lev] = (float)rsd;
+                lv[nv + lev] = (float)(rsd * rsd);
+                lv[2 * nv + lev] = (float)(1.0 / (gd->Altitudeh[lev] - gd->Altitudeh[lev + 1]));
+                lv[3 * nv + lev] = (float)Kdhz[lev];
+            }
+            e = hipMemcpy(lvc, lv.data(), lv.size() * sizeof(float), hipMemcpyHostToDevice);
+            g.lvc = lvc;
+        }
         if (e == hipSuccess) {
             Geo gu = g;
             gu.deep = 0;  // rscale = 1

@@ -40,6 +40,9 @@ struct Geo {
     const float *wlap;  // [7][S]
     const float *wdd;   // [63][S]
     const float *udd;   // [21][S]
+    // Mixed precision only: per-level FP32 constants [4][nv]: rscale, rscale^2,
+    // 1 / (Alth[lev] - Alth[lev + 1]) and Kdhz.
+    const float *lvc;
     double A, Omega, Gravit, P_Ref, Rd, Cp;
     int deep, nonhydro, ghv;
 };

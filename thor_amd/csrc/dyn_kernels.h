@@ -1117,10 +1117,10 @@ __host__ __device__ constexpr int kThomasCB(int nv) { return 4 * (nv + 1) * 32 *
 // unroll count of the slot loop (7 when wv reads a register array).
 template <int U, class GetW, class GetM, class GetH, class GetV1, class GetPt>
 __device__ __forceinline__ void prepare_point_w(const Geo &g, const Fields &f, int p, int lev, GetW wv, GetM mv,
-                                                GetH hv, GetV1 v1, GetPt ptv, double slowp, double slowrho) {
+                                                GetH hv, GetV1 v1, GetPt ptv, double slowp, double slowrho,
+                                                double rs) {
     const size_t S = g.S;
     const double RoC = g.Rd / (g.Cp - g.Rd);
-    const double rs = rscale_at(g, lev);
     double nflxr = 0.0, nflxp = 0.0, dr = 0.0, dpt = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -1152,7 +1152,7 @@ __device__ __forceinline__ void prepare_point(const Geo &g, const Fields &f, int
     const size_t S = g.S;
     prepare_point_w<THOR_SLOT_UNROLL>(
         g, f, p, lev, [&](int s, int k) { return g.div[(size_t)(s * 3 + k) * S + p]; }, mv, hv, v1, ptv,
-        LV(f.Slowp, lev), LV(f.SlowRho, lev));
+        LV(f.Slowp, lev), LV(f.SlowRho, lev), rscale_at(g, lev));
 }
 
 template <bool MX> __global__ STENCIL_BOUNDS void k_prepare(Geo g, Fields f) {
@@ -1252,15 +1252,49 @@ __device__ __forceinline__ bool pole_ctx_lc(const Geo &g, int l0, int l1, PoleCt
     return true;
 }
 
-#ifndef THOR_LC_MIN_BLOCKS
-#define THOR_LC_MIN_BLOCKS 3
-#endif
-#define LC_BOUNDS __launch_bounds__(256, THOR_LC_MIN_BLOCKS)
+// Thread of a level-chunked tile block: its point p, the LDS positions of its
+// 7 stencil slots and its own and halo staging positions, and 32-bit byte
+// offsets of the point and of its halo point (threads 0..65). A thread with
+// no halo point, or whose halo point is a missing pentagon corner, gets an
+// offset past the end of every buffer, so its halo loads return 0 without a
+// branch; a missing corner is never read (a pentagon's slot 5 reads slot 6).
+struct LcThread {
+    int p, ir, hp;  // hp = -1: no halo point to stage
+    int pos[7];
+    unsigned o4, o8, h4, h8;
+};
+__device__ __forceinline__ LcThread lc_thread(const Geo &g, int r) {
+    const TileThread T = tile_thread(g, r);
+    LcThread L;
+    L.p = T.p;
+    L.ir = T.ir;
+#pragma unroll
+    for (int s = 0; s < 7; s++) L.pos[s] = T.pos(s);
+    const int t = threadIdx.x;
+    const int q = t < NHALO ? g.halo[r * NHALO + t] : -1;
+    L.hp = t < NHALO ? halo_pos(t) : -1;
+    L.o4 = 4u * (unsigned)T.p;
+    L.o8 = 8u * (unsigned)T.p;
+    L.h4 = q >= 0 ? 4u * (unsigned)q : 0x80000000u;
+    L.h8 = q >= 0 ? 8u * (unsigned)q : 0x80000000u;
+    return L;
+}
+
+// Per-level constants of a chunk (Geo::lvc rows for levels l0..l1-1) staged in
+// LDS before the level loop: read as LDS values inside the loop, they do not
+// join the in-order vector-memory counter that the pipelined loads rely on.
+constexpr int kLcMax = 64;  // largest level chunk
+__device__ __forceinline__ void stage_lvc(const Geo &g, int l0, int l1, float (*sL)[kLcMax]) {
+    const int t = threadIdx.x, n = l1 - l0;
+    if (t < 4 * n) sL[t / n][t % n] = g.lvc[(t / n) * g.nv + l0 + t % n];
+}
+
 
 // Momentum_Eq (thor_fastmodes.h:54-253) after the divergence-damping pass 1,
 // in FP32: gradient of the FP64 pressure deviation with FP32 weights, DivM
 // from the probed FP32 weights, FP32 momentum deviation.
-__global__ LC_BOUNDS void k_momentum_lc(Geo g, Fields f, int divdamp, double dt, int lc) {
+__global__ __launch_bounds__(256, 3) void k_momentum_lc(Geo g, Fields f, int divdamp, double dt, int lc) {
+#pragma clang fp contract(fast)
     int r, l0, l1;
     if (!block_tile_lc(g, lc, r, l0, l1)) return;
     const size_t S = g.S;
@@ -1276,116 +1310,111 @@ __global__ LC_BOUNDS void k_momentum_lc(Geo g, Fields f, int divdamp, double dt,
                 [&](int s) { return LVP(f.ps, lev, c.gi(s)); });
         return;
     }
-    __shared__ float sG[2][3][NHL2], sP[2][NHL2];
-    const TileThread T = tile_thread(g, r);
-    const int p = T.p, t = threadIdx.x;
-    const bool hth = t < NHALO;
-    const int qh = hth ? g.halo[r * NHALO + t] : -1;
-    const int hp = hth ? halo_pos(t) : 0;
-    int pos[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) pos[s] = T.pos(s);
+    __shared__ float sG[2][3][NHL2], sP[2][NHL2], sL[4][kLcMax];
+    stage_lvc(g, l0, l1, sL);
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
     float wg[21], wd[63];
 #pragma unroll
     for (int i = 0; i < 21; i++) wg[i] = (float)g.grad[(size_t)i * S + p];
-    if (divdamp) {
 #pragma unroll
-        for (int i = 0; i < 63; i++) wd[i] = g.wdd[(size_t)i * S + p];
-    }
+    for (int i = 0; i < 63; i++) wd[i] = divdamp ? g.wdd[(size_t)i * S + p] : 0.f;
     const float fx = (float)g.fr[p], fy = (float)g.fr[S + p], fz = (float)g.fr[2 * S + p];
     const float dtf = (float)dt;
     const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
     const auto rG = rsrc(divg, 3 * nv * pl4), rP = rsrc(f.ps, nv * pl8), rS = rsrc(SlowMh, 3 * nv * pl4),
                rM = rsrc(Mhs, 3 * nv * pl4);
-    const unsigned o4 = 4u * p, o8 = 8u * p, h4 = 4u * (unsigned)qh, h8 = 8u * (unsigned)qh;
     struct In {
-        float og[3], hg[3], op, hpv, sm[3], mh[3];
+        float sm[3], mh[3], og[3], op, hg[3], hpv;
     };
     auto load = [&](In &v, int lev) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) v.og[k] = v.hg[k] = 0.f;
-        v.hpv = 0.f;
-        if (divdamp) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) v.og[k] = ldb_f(rG, o4, (unsigned)((k * nv + lev) * pl4));
-        }
-        v.op = (float)ldb_d(rP, o8, (unsigned)(lev * pl8));
-        if (qh >= 0) {
-            if (divdamp) {
-#pragma unroll
-                for (int k = 0; k < 3; k++) v.hg[k] = ldb_f(rG, h4, (unsigned)((k * nv + lev) * pl4));
-            }
-            v.hpv = (float)ldb_d(rP, h8, (unsigned)(lev * pl8));
+        for (int k = 0; k < 3; k++) {
+            v.sm[k] = ldb_f(rS, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.mh[k] = ldb_f(rM, T.o4, (unsigned)((k * nv + lev) * pl4));
         }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v.sm[k] = ldb_f(rS, o4, (unsigned)((k * nv + lev) * pl4));
-            v.mh[k] = ldb_f(rM, o4, (unsigned)((k * nv + lev) * pl4));
+            v.og[k] = ldb_f(rG, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.hg[k] = ldb_f(rG, T.h4, (unsigned)((k * nv + lev) * pl4));
         }
+        v.op = (float)ldb_d(rP, T.o8, (unsigned)(lev * pl8));
+        v.hpv = (float)ldb_d(rP, T.h8, (unsigned)(lev * pl8));
     };
     auto put = [&](const In &v, int b) {
 #pragma unroll
         for (int k = 0; k < 3; k++) sG[b][k][T.ir] = v.og[k];
         sP[b][T.ir] = v.op;
-        if (hth) {
+        if (T.hp >= 0) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) sG[b][k][hp] = v.hg[k];
-            sP[b][hp] = v.hpv;
+            for (int k = 0; k < 3; k++) sG[b][k][T.hp] = v.hg[k];
+            sP[b][T.hp] = v.hpv;
         }
     };
-    In cur;
-    load(cur, l0);
-    put(cur, 0);
-    __syncthreads();
-    for (int lev = l0; lev < l1; lev++) {
+    float sm[3], mh[3];
+    auto body = [&](int lev, In &R, In &L) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sm[k] = L.sm[k];
+            mh[k] = L.mh[k];
+        }
+        load(L, min(lev + 2, l1 - 1));
         const int b = (lev - l0) & 1;
-        const bool more = lev + 1 < l1;
-        In nxt;
-        if (more) load(nxt, lev + 1);
-        const double rsd = rscale_at(g, lev);
-        const float rs = (float)rsd, rs2 = (float)(rsd * rsd);
+        const float rs = sL[0][lev - l0], rs2 = sL[1][lev - l0];
         float DivM[3] = {0.f, 0.f, 0.f};
         if (divdamp) {
+            float a[7][3];
+#pragma unroll
+            for (int s = 0; s < 7; s++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) a[s][k] = sG[b][k][T.pos[s]];
             float o[3];
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                float sum = 0.f;
+                float sum = wd[c * 21] * a[0][0];
 #pragma unroll
-                for (int s = 0; s < 7; s++)
-#pragma unroll
-                    for (int k = 0; k < 3; k++) sum += wd[c * 21 + s * 3 + k] * sG[b][k][pos[s]];
+                for (int i = 1; i < 21; i++) sum += wd[c * 21 + i] * a[i / 3][i % 3];
                 o[c] = rs2 * sum;
             }
-            const float sdiff = (float)g.Kdhz[lev];
+            const float sdiff = sL[3][lev - l0];
             const float lapr = fx * o[0] + fy * o[1] + fz * o[2];
             DivM[0] = -(o[0] - fx * lapr) * sdiff;
             DivM[1] = -(o[1] - fy * lapr) * sdiff;
             DivM[2] = -(o[2] - fz * lapr) * sdiff;
         }
-        float M[3];
+        float pv[7], M[3];
+#pragma unroll
+        for (int s = 0; s < 7; s++) pv[s] = sP[b][T.pos[s]];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            float s0 = 0.f;
+            float s0 = wg[k] * pv[0];
 #pragma unroll
-            for (int s = 0; s < 7; s++) s0 += wg[s * 3 + k] * sP[b][pos[s]];
-            M[k] = (-rs * s0 + cur.sm[k] + DivM[k]) * dtf;
+            for (int s = 1; s < 7; s++) s0 += wg[s * 3 + k] * pv[s];
+            M[k] = (-rs * s0 + sm[k] + DivM[k]) * dtf;
         }
         const float vr = M[0] * fx + M[1] * fy + M[2] * fz;
-        stb_f(rM, o4, (unsigned)((0 * nv + lev) * pl4), cur.mh[0] + (M[0] - vr * fx));
-        stb_f(rM, o4, (unsigned)((1 * nv + lev) * pl4), cur.mh[1] + (M[1] - vr * fy));
-        stb_f(rM, o4, (unsigned)((2 * nv + lev) * pl4), cur.mh[2] + (M[2] - vr * fz));
-        if (more) {
-            put(nxt, b ^ 1);
-            cur = nxt;
-        }
+        stb_f(rM, T.o4, (unsigned)((0 * nv + lev) * pl4), mh[0] + (M[0] - vr * fx));
+        stb_f(rM, T.o4, (unsigned)((1 * nv + lev) * pl4), mh[1] + (M[1] - vr * fy));
+        stb_f(rM, T.o4, (unsigned)((2 * nv + lev) * pl4), mh[2] + (M[2] - vr * fz));
+        if (lev + 1 < l1) put(R, b ^ 1);
         __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
     }
 }
 
 // Prepare_Implicit_Vertical (thor_vertical_int.h:566-771) and the horizontal
 // flux divergences of Density_Pressure_Eqs in FP64 arithmetic, as in
 // prepare_point (the momentum deviation is read from its FP32 storage).
-__global__ LC_BOUNDS void k_prepare_lc(Geo g, Fields f, int lc) {
+__global__ __launch_bounds__(256, 2) void k_prepare_lc(Geo g, Fields f, int lc) {
+#pragma clang fp contract(fast)
     int r, l0, l1;
     if (!block_tile_lc(g, lc, r, l0, l1)) return;
     const size_t S = g.S;
@@ -1403,48 +1432,35 @@ __global__ LC_BOUNDS void k_prepare_lc(Geo g, Fields f, int lc) {
         return;
     }
     __shared__ float sM[2][3][NHL2];
-    __shared__ double sH[2][NHL2], sV1[2][3][NHL2], sPt[2][NHL2];
-    const TileThread T = tile_thread(g, r);
-    const int p = T.p, t = threadIdx.x;
-    const bool hth = t < NHALO;
-    const int qh = hth ? g.halo[r * NHALO + t] : -1;
-    const int hp = hth ? halo_pos(t) : 0;
-    int pos[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) pos[s] = T.pos(s);
+    __shared__ double sH[2][NHL2], sV1[2][3][NHL2], sPt[2][NHL2], sRs[kLcMax];
+    if ((int)threadIdx.x < l1 - l0) sRs[threadIdx.x] = rscale_at(g, l0 + threadIdx.x);
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
     double wv[21];
 #pragma unroll
     for (int i = 0; i < 21; i++) wv[i] = g.div[(size_t)i * S + p];
     const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
     const auto rM = rsrc(Mhs, 3 * nv * pl4), rMk = rsrc(f.Mhk, 3 * nv * pl8), rH = rsrc(f.h, nv * pl8),
                rPt = rsrc(f.pt, nv * pl8), rSp = rsrc(f.Slowp, nv * pl8), rSr = rsrc(f.SlowRho, nv * pl8);
-    const unsigned o4 = 4u * p, o8 = 8u * p, h4 = 4u * (unsigned)qh, h8 = 8u * (unsigned)qh;
     struct In {
+        double slowp, slowrho;
         float om[3], hm[3];
-        double omk[3], hmk[3], oh, hh, opt, hpt, slowp, slowrho;
+        double omk[3], hmk[3], oh, hh, opt, hpt;
     };
     auto load = [&](In &v, int lev) {
+        v.slowp = ldb_d(rSp, T.o8, (unsigned)(lev * pl8));
+        v.slowrho = ldb_d(rSr, T.o8, (unsigned)(lev * pl8));
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v.om[k] = ldb_f(rM, o4, (unsigned)((k * nv + lev) * pl4));
-            v.omk[k] = ldb_d(rMk, o8, (unsigned)((k * nv + lev) * pl8));
-            v.hm[k] = 0.f;
-            v.hmk[k] = 0.0;
+            v.om[k] = ldb_f(rM, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.omk[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.hm[k] = ldb_f(rM, T.h4, (unsigned)((k * nv + lev) * pl4));
+            v.hmk[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
         }
-        v.oh = ldb_d(rH, o8, (unsigned)(lev * pl8));
-        v.opt = ldb_d(rPt, o8, (unsigned)(lev * pl8));
-        v.hh = v.hpt = 0.0;
-        if (qh >= 0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                v.hm[k] = ldb_f(rM, h4, (unsigned)((k * nv + lev) * pl4));
-                v.hmk[k] = ldb_d(rMk, h8, (unsigned)((k * nv + lev) * pl8));
-            }
-            v.hh = ldb_d(rH, h8, (unsigned)(lev * pl8));
-            v.hpt = ldb_d(rPt, h8, (unsigned)(lev * pl8));
-        }
-        v.slowp = ldb_d(rSp, o8, (unsigned)(lev * pl8));
-        v.slowrho = ldb_d(rSr, o8, (unsigned)(lev * pl8));
+        v.oh = ldb_d(rH, T.o8, (unsigned)(lev * pl8));
+        v.opt = ldb_d(rPt, T.o8, (unsigned)(lev * pl8));
+        v.hh = ldb_d(rH, T.h8, (unsigned)(lev * pl8));
+        v.hpt = ldb_d(rPt, T.h8, (unsigned)(lev * pl8));
     };
     auto put = [&](const In &v, int b) {
 #pragma unroll
@@ -1454,42 +1470,44 @@ __global__ LC_BOUNDS void k_prepare_lc(Geo g, Fields f, int lc) {
         }
         sH[b][T.ir] = v.oh;
         sPt[b][T.ir] = v.opt;
-        if (hth) {
+        if (T.hp >= 0) {
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                sM[b][k][hp] = v.hm[k];
-                sV1[b][k][hp] = qh >= 0 ? (double)v.hm[k] + v.hmk[k] : 0.0;
+                sM[b][k][T.hp] = v.hm[k];
+                sV1[b][k][T.hp] = (double)v.hm[k] + v.hmk[k];
             }
-            sH[b][hp] = v.hh;
-            sPt[b][hp] = v.hpt;
+            sH[b][T.hp] = v.hh;
+            sPt[b][T.hp] = v.hpt;
         }
     };
-    In cur;
-    load(cur, l0);
-    put(cur, 0);
-    __syncthreads();
-    for (int lev = l0; lev < l1; lev++) {
+    auto body = [&](int lev, In &R, In &L) {
+        const double slowp = L.slowp, slowrho = L.slowrho;
+        load(L, min(lev + 2, l1 - 1));
         const int b = (lev - l0) & 1;
-        const bool more = lev + 1 < l1;
-        In nxt;
-        if (more) load(nxt, lev + 1);
         prepare_point_w<7>(
             g, f, p, lev, [&](int s, int k) { return wv[s * 3 + k]; },
-            [&](int s, int k) { return (double)sM[b][k][pos[s]]; }, [&](int s) { return sH[b][pos[s]]; },
-            [&](int s, int k) { return sV1[b][k][pos[s]]; }, [&](int s) { return sPt[b][pos[s]]; }, cur.slowp,
-            cur.slowrho);
-        if (more) {
-            put(nxt, b ^ 1);
-            cur = nxt;
-        }
+            [&](int s, int k) { return (double)sM[b][k][T.pos[s]]; }, [&](int s) { return sH[b][T.pos[s]]; },
+            [&](int s, int k) { return sV1[b][k][T.pos[s]]; }, [&](int s) { return sPt[b][T.pos[s]]; }, slowp,
+            slowrho, sRs[lev - l0]);
+        if (lev + 1 < l1) put(R, b ^ 1);
         __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
     }
 }
 
 // Divergence-damping pass 0 (DivM_Op) on the deviations Mhs / Whs in FP32,
 // as divm_w<false>. Whs lives on interfaces: the top value of level lev is
 // the bottom value of lev + 1, so each level loads one new interface.
-__global__ LC_BOUNDS void k_divm0_lc(Geo g, Fields f, int lc) {
+__global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
+#pragma clang fp contract(fast)
     int r, l0, l1;
     if (!block_tile_lc(g, lc, r, l0, l1)) return;
     const size_t S = g.S;
@@ -1509,15 +1527,10 @@ __global__ LC_BOUNDS void k_divm0_lc(Geo g, Fields f, int lc) {
         }
         return;
     }
-    __shared__ float sM[2][3][NHL2], sD[2][NHL2];
-    const TileThread T = tile_thread(g, r);
-    const int p = T.p, t = threadIdx.x;
-    const bool hth = t < NHALO;
-    const int qh = hth ? g.halo[r * NHALO + t] : -1;
-    const int hp = hth ? halo_pos(t) : 0;
-    int pos[7];
-#pragma unroll
-    for (int s = 0; s < 7; s++) pos[s] = T.pos(s);
+    __shared__ float sM[2][3][NHL2], sD[2][NHL2], sL[4][kLcMax];
+    stage_lvc(g, l0, l1, sL);
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
     float wd[63], wu[21];
 #pragma unroll
     for (int i = 0; i < 63; i++) wd[i] = g.wdd[(size_t)i * S + p];
@@ -1525,7 +1538,6 @@ __global__ LC_BOUNDS void k_divm0_lc(Geo g, Fields f, int lc) {
     for (int i = 0; i < 21; i++) wu[i] = g.udd[(size_t)i * S + p];
     const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
     const auto rM = rsrc(Mhs, 3 * nv * pl4), rW = rsrc(f.Whs, (nv + 1) * pl8), rD = rsrc(divg, 3 * nv * pl4);
-    const unsigned o4 = 4u * p, o8 = 8u * p, h4 = 4u * (unsigned)qh, h8 = 8u * (unsigned)qh;
     // loaded values of one level: Mhs and the top interface's Whs
     struct In {
         float om[3], hm[3];
@@ -1534,57 +1546,58 @@ __global__ LC_BOUNDS void k_divm0_lc(Geo g, Fields f, int lc) {
     auto load = [&](In &v, int lev) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v.om[k] = ldb_f(rM, o4, (unsigned)((k * nv + lev) * pl4));
-            v.hm[k] = 0.f;
+            v.om[k] = ldb_f(rM, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.hm[k] = ldb_f(rM, T.h4, (unsigned)((k * nv + lev) * pl4));
         }
-        v.ow = ldb_d(rW, o8, (unsigned)((lev + 1) * pl8));
-        v.hw = 0.0;
-        if (qh >= 0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) v.hm[k] = ldb_f(rM, h4, (unsigned)((k * nv + lev) * pl4));
-            v.hw = ldb_d(rW, h8, (unsigned)((lev + 1) * pl8));
-        }
+        v.ow = ldb_d(rW, T.o8, (unsigned)((lev + 1) * pl8));
+        v.hw = ldb_d(rW, T.h8, (unsigned)((lev + 1) * pl8));
     };
-    // wl: Whs at the bottom interface of the level being put
-    double owl = ldb_d(rW, o8, (unsigned)(l0 * pl8)), hwl = qh >= 0 ? ldb_d(rW, h8, (unsigned)(l0 * pl8)) : 0.0;
+    // Whs at the bottom interface of the next level to be put
+    double owl = ldb_d(rW, T.o8, (unsigned)(l0 * pl8)), hwl = ldb_d(rW, T.h8, (unsigned)(l0 * pl8));
     auto put = [&](const In &v, int b) {
 #pragma unroll
         for (int k = 0; k < 3; k++) sM[b][k][T.ir] = v.om[k];
         sD[b][T.ir] = (float)(owl - v.ow);
         owl = v.ow;
-        if (hth) {
+        if (T.hp >= 0) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) sM[b][k][hp] = v.hm[k];
-            sD[b][hp] = (float)(hwl - v.hw);
-            hwl = v.hw;
+            for (int k = 0; k < 3; k++) sM[b][k][T.hp] = v.hm[k];
+            sD[b][T.hp] = (float)(hwl - v.hw);
         }
+        hwl = v.hw;
     };
-    In cur;
-    load(cur, l0);
-    put(cur, 0);
-    __syncthreads();
-    for (int lev = l0; lev < l1; lev++) {
+    auto body = [&](int lev, In &R, In &L) {
+        load(L, min(lev + 2, l1 - 1));
         const int b = (lev - l0) & 1;
-        const bool more = lev + 1 < l1;
-        In nxt;
-        if (more) load(nxt, lev + 1);
-        const double rsd = rscale_at(g, lev);
-        const float rs = (float)rsd, rs2 = (float)(rsd * rsd);
-        const float dz = (float)(1.0 / (g.Alth[lev] - g.Alth[lev + 1]));
+        const float rs = sL[0][lev - l0], rs2 = sL[1][lev - l0], dz = sL[2][lev - l0];
+        float a[7][3], d[7];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) a[s][k] = sM[b][k][T.pos[s]];
+            d[s] = sD[b][T.pos[s]];
+        }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            float sum = 0.f;
+            float sum = wd[c * 21] * a[0][0];
 #pragma unroll
-            for (int s = 0; s < 7; s++)
+            for (int i = 1; i < 21; i++) sum += wd[c * 21 + i] * a[i / 3][i % 3];
+            float su = wu[c * 7] * d[0];
 #pragma unroll
-                for (int k = 0; k < 3; k++) sum += wd[c * 21 + s * 3 + k] * sM[b][k][pos[s]];
-            float su = 0.f;
-#pragma unroll
-            for (int s = 0; s < 7; s++) su += wu[c * 7 + s] * sD[b][pos[s]];
-            stb_f(rD, o4, (unsigned)((c * nv + lev) * pl4), rs2 * sum + rs * dz * su);
+            for (int s = 1; s < 7; s++) su += wu[c * 7 + s] * d[s];
+            stb_f(rD, T.o4, (unsigned)((c * nv + lev) * pl4), rs2 * sum + rs * dz * su);
         }
-        if (more) put(nxt, b ^ 1);
+        if (lev + 1 < l1) put(R, b ^ 1);
         __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
     }
 }
 
@@ -1794,18 +1807,19 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     __syncthreads();
     // One lane per column: forward sweep and back substitution out of LDS,
     // with the stage's factorisation read straight from HBM (its loads do not
-    // depend on the recurrence, so they are issued ahead of it).
-    if (row == 0 && p < g.P) {
-        const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
-        if (nv > 1) {
-            double ddp = (-sD[CB + c]) / TT[1];
-            sD[CB + c] = ddp;
+    // depend on the recurrence, so they are issued ahead of it). Staging the
+    // factorisation in LDS instead measured no faster at g6 L48.
+    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+    if (row == 0 && p < g.P && nv > 1) {
+        double ddp = (-sD[CB + c]) / TT[1];
+        sD[CB + c] = ddp;
 #pragma unroll 8
-            for (int lev = 2; lev < nv; lev++) {
-                ddp = (-sD[lev * CB + c] - ddp * AA[lev]) * TT[lev];
-                sD[lev * CB + c] = ddp;
-            }
+        for (int lev = 2; lev < nv; lev++) {
+            ddp = (-sD[lev * CB + c] - ddp * AA[lev]) * TT[lev];
+            sD[lev * CB + c] = ddp;
         }
+    }
+    if (row == 0 && p < g.P) {
         double wup = 0.0;
 #pragma unroll 8
         for (int lev = nv - 1; lev > 0; lev--) {
