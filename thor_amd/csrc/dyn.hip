@@ -248,8 +248,10 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
         launch(d, "column_prep", W(12 + 5, 6), [&] { k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f); });
-        launch(d, "adv_diff1", W(8 + (hydiff ? 7 : 0), 3 + (hydiff ? 6 : 0)),
-               [&] { k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff); });
+        launch(d, "adv_diff1", W(8 + (hydiff ? 7 : 0), 3 + (hydiff ? 6 : 0)), [&] {
+            if (MX) k_adv_diff1_lc<<<grid_lc, blk, 0, st>>>(g, f, hydiff, lc);
+            else k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff);
+        });
         if (hydiff || divdamp)
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
                    [&] {

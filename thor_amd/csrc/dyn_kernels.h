@@ -1601,6 +1601,146 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
     }
 }
 
+// Compute_Advec_Cori1 (thor_adv_cor.h:51-235) and the first Diffusion_Op pass
+// (thor_diff.h:54-522) of the mixed-precision path, in FP32 over level
+// chunks. Each staged point carries its momentum, the advecting velocity
+// v = M / rho + (W / rho) r and the diffused variables rho, M / rho, W / rho
+// and Rd T (the reference's variables 0..5), formed once per point from the
+// FP64 stage values and rounded to FP32, so no quotient is recomputed per
+// stencil slot. The horizontal advection is added to Adv (which holds the
+// vertical part from k_column_prep).
+__global__ __launch_bounds__(256, 3) void k_adv_diff1_lc(Geo g, Fields f, int hydiff, int lc) {
+#pragma clang fp contract(fast)
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    float *Adv = f.mAdv, *diff = f.mdiff;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev)) {
+            auto vel = [&](int q, int k) {
+                const double rho = LVP(f.Rhok, lev, q), w = LVP(f.Wk, lev, q);
+                return VEC(f.Mhk, k, lev, q) / rho + (w / rho) * g.fr[(size_t)k * S + q];
+            };
+            adv_diff1_point<true>(
+                g, f, c.p, lev, 5, hydiff, [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); },
+                [&](int s, int k) { return vel(c.gi(s), k); }, [&](int s) { return LVP(f.Rhok, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.Wk, lev, c.gi(s)); }, [&](int s) { return LVP(f.T, lev, c.gi(s)); });
+        }
+        return;
+    }
+    __shared__ float sM[2][3][NHL2], sV[2][3][NHL2], sA[2][6][NHL2], sL[4][kLcMax];
+    stage_lvc(g, l0, l1, sL);
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
+    float wd[21], wl[6];
+#pragma unroll
+    for (int i = 0; i < 21; i++) wd[i] = (float)g.div[(size_t)i * S + p];
+#pragma unroll
+    for (int s = 0; s < 6; s++) wl[s] = g.wlap[(size_t)(s + 1) * S + p];
+    // unit radial vector of the point and of its halo point
+    const int qh = T.hp >= 0 ? g.halo[r * NHALO + threadIdx.x] : -1;
+    float ofr[3], hfr[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        ofr[k] = (float)g.fr[(size_t)k * S + p];
+        hfr[k] = qh >= 0 ? (float)g.fr[(size_t)k * S + qh] : 0.f;
+    }
+    const double Rd = g.Rd;
+    const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
+    const auto rAd = rsrc(Adv, 3 * nv * pl4), rDf = rsrc(diff, 6 * nv * pl4), rM = rsrc(f.Mhk, 3 * nv * pl8),
+               rR = rsrc(f.Rhok, nv * pl8), rW = rsrc(f.Wk, nv * pl8), rT = rsrc(f.T, nv * pl8);
+    struct In {
+        float adv[3];
+        double om[3], orho, ow, ot, hm[3], hrho, hw, ht;
+    };
+    auto load = [&](In &v, int lev) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) v.adv[k] = ldb_f(rAd, T.o4, (unsigned)((k * nv + lev) * pl4));
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.om[k] = ldb_d(rM, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.hm[k] = ldb_d(rM, T.h8, (unsigned)((k * nv + lev) * pl8));
+        }
+        v.orho = ldb_d(rR, T.o8, (unsigned)(lev * pl8));
+        v.hrho = ldb_d(rR, T.h8, (unsigned)(lev * pl8));
+        v.ow = ldb_d(rW, T.o8, (unsigned)(lev * pl8));
+        v.hw = ldb_d(rW, T.h8, (unsigned)(lev * pl8));
+        v.ot = ldb_d(rT, T.o8, (unsigned)(lev * pl8));
+        v.ht = ldb_d(rT, T.h8, (unsigned)(lev * pl8));
+    };
+    // one staged point: M, v, and the six diffused variables
+    auto put1 = [&](int b, int pos, const double m[3], double rho, double w, double t, const float fr[3]) {
+        const double wr = w / rho;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double u = m[k] / rho;
+            sM[b][k][pos] = (float)m[k];
+            sV[b][k][pos] = (float)(u + wr * fr[k]);
+            sA[b][1 + k][pos] = (float)u;
+        }
+        sA[b][0][pos] = (float)rho;
+        sA[b][4][pos] = (float)wr;
+        sA[b][5][pos] = (float)(Rd * t);
+    };
+    auto put = [&](const In &v, int b) {
+        put1(b, T.ir, v.om, v.orho, v.ow, v.ot, ofr);
+        if (T.hp >= 0) {
+            if (qh >= 0) {
+                put1(b, T.hp, v.hm, v.hrho, v.hw, v.ht, hfr);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 3; k++) sM[b][k][T.hp] = sV[b][k][T.hp] = 0.f;
+#pragma unroll
+                for (int k = 0; k < 6; k++) sA[b][k][T.hp] = 0.f;
+            }
+        }
+    };
+    float adv[3];
+    auto body = [&](int lev, In &R, In &L) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) adv[k] = L.adv[k];
+        load(L, min(lev + 2, l1 - 1));
+        const int b = (lev - l0) & 1;
+        const float rs = sL[0][lev - l0], rs2 = sL[1][lev - l0];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float sum = 0.f;
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) {
+                float sk = wd[kk] * sV[b][c][T.pos[0]] * sM[b][kk][T.pos[0]];
+#pragma unroll
+                for (int s = 1; s < 7; s++) sk += wd[s * 3 + kk] * sV[b][c][T.pos[s]] * sM[b][kk][T.pos[s]];
+                sum += rs * sk;
+            }
+            stb_f(rAd, T.o4, (unsigned)((c * nv + lev) * pl4), sum + adv[c]);
+        }
+        if (hydiff) {
+#pragma unroll
+            for (int var = 0; var < 6; var++) {
+                const float a0 = sA[b][var][T.pos[0]];
+                float sum = wl[0] * (sA[b][var][T.pos[1]] - a0);
+#pragma unroll
+                for (int s = 2; s < 7; s++) sum += wl[s - 1] * (sA[b][var][T.pos[s]] - a0);
+                stb_f(rDf, T.o4, (unsigned)((var * nv + lev) * pl4), rs2 * sum);
+            }
+        }
+        if (lev + 1 < l1) put(R, b ^ 1);
+        __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
+    }
+}
+
 // Vertical_Eq (thor_vertical_int.h:52-563): per column, the tridiagonal solve
 // for Whs at interfaces 1..nv-1 with the modify-in-place Thomas algorithm;
 // cc/dd live in LDS.
