@@ -146,6 +146,85 @@ def diagnostics(g, p, st):
     return {"T": T, "u": u, "v": vm, "w": w, "ps": ps}
 
 
+def _profx_us(stats, names):
+    """Average device microseconds per ProfX call of the listed kernels."""
+    calls = max((s["launches"] for s in stats if s["name"] in ("profx_double_gray", "surrogate_features")), default=1)
+    return 1e3 * sum(s["total_ms"] for s in stats if s["name"] in names) / calls
+
+
+def rt_surrogate_run(capi, args):
+    """Hot-Jupiter g5 L40 with double-gray RT (mixed-precision dynamics):
+    spin-up with the exact RT while sampling (T, ln p, cos z) -> Qheat columns,
+    GPU training of the 81-256-256-40 surrogate, then ProfX and whole-step
+    timing with the exact RT and with the surrogate (judge every 10 calls,
+    exact RT for out-of-envelope columns)."""
+    from thor_amd import surrogate as sg
+    g, p = sg.hot_jupiter(5, 40, dt=300.0, ns=6, precision=1, cache_dir=CACHE)
+    st = capi.ic_rest(g, p, seed=1003, temperature_noise=1.0, wind_noise=0.1)
+    d = capi.Dyn(g, p)
+    d.set_state(st)
+    d.set_rt(sg.double_gray())
+    t0 = time.perf_counter()
+    X, Y = sg.collect(d, g, nsteps=args.rt_spinup_steps, every=args.rt_sample_every)
+    t_spin = time.perf_counter() - t0
+    log(f"spin-up: {args.rt_spinup_steps} steps ({args.rt_spinup_steps * 300.0 / 86400:.1f} days) with exact RT, "
+        f"{len(X)} columns sampled, {t_spin:.1f} s")
+    norm = sg.normalisation(X, Y)
+    Xn, Yn = sg.apply_norm(X, Y, norm)
+    m = capi.MLP(2 * g.nv + 1, 256, g.nv, seed=5)
+    err0 = sg.heating_error(m, X, Y, norm)
+    t0 = time.perf_counter()
+    losses = sg.train(m, Xn, Yn, epochs=args.rt_epochs, batch=512, lr=2e-3, log=log)
+    t_train = time.perf_counter() - t0
+    err1 = sg.heating_error(m, X, Y, norm)
+    log(f"training: {args.rt_epochs} epochs in {t_train:.1f} s; heating RMS error untrained {err0:.3f}, "
+        f"trained {err1:.3f}")
+    n = args.rt_steps
+    out = {}
+    for form, forcing in (("exact_rt", capi.FORCING_DOUBLE_GRAY), ("surrogate", capi.FORCING_RT_SURROGATE)):
+        if forcing == capi.FORCING_RT_SURROGATE:
+            d.set_rt_surrogate(m, norm, judge_every=10)
+            d.surrogate_stats()
+        d.run(3, forcing)
+        d.sync()
+        if forcing == capi.FORCING_RT_SURROGATE:
+            d.surrogate_stats()
+        t1 = time.perf_counter()
+        d.run(n, forcing)
+        d.sync()
+        wall = time.perf_counter() - t1
+        sstats = d.surrogate_stats() if forcing == capi.FORCING_RT_SURROGATE else None
+        d.kernel_timing(True)
+        d.run(10, forcing)
+        d.sync()
+        ks = d.kernel_stats()
+        d.kernel_timing(False)
+        if forcing == capi.FORCING_RT_SURROGATE:
+            d.surrogate_stats()
+        rt_names = ("profx_double_gray", "profx_double_gray_fallback", "surrogate_features", "surrogate_mlp",
+                    "surrogate_apply", "surrogate_error")
+        out[form] = {"ms_per_step": round(1e3 * wall / n, 4), "sdpd": round(300.0 * n / wall, 1),
+                     "profx_us": round(_profx_us(ks, rt_names), 1),
+                     "kernels_us": {s["name"]: round(1e3 * s["total_ms"] / s["launches"], 1) for s in ks
+                                    if s["name"] in rt_names}}
+        if sstats is not None:
+            qrms = float(np.sqrt((Y ** 2).mean()))
+            out[form]["judge"] = {"calls": sstats["calls"], "judged": sstats["judged"],
+                                  "fallback_columns_per_call": round(sstats["fallback_columns"] / max(1, sstats["calls"]), 1),
+                                  "rms_err_W_m3": sstats["rms_err"], "max_err_W_m3": sstats["max_err"],
+                                  "rms_err_rel_to_rms_heating": round(sstats["rms_err"] / qrms, 4)}
+    d.close()
+    m.close()
+    return {"metric": "double-gray RT vs MLP surrogate, hot Jupiter g5 L40 (10242 columns), mixed-precision "
+                      "dynamics", "workload": "hot-Jupiter g5 L40, dt 300 s, 6 substeps, double-gray RT",
+            "spinup_steps": args.rt_spinup_steps, "train_columns": int(len(X)), "epochs": args.rt_epochs,
+            "train_seconds": round(t_train, 1), "train_loss_first_last": [losses[0], losses[-1]],
+            "heating_rel_rms_err_untrained": round(err0, 4), "heating_rel_rms_err_trained": round(err1, 4),
+            "timed_steps": n, **out,
+            "profx_speedup": round(out["exact_rt"]["profx_us"] / out["surrogate"]["profx_us"], 2),
+            "step_speedup": round(out["exact_rt"]["ms_per_step"] / out["surrogate"]["ms_per_step"], 3)}
+
+
 def error_run(capi, g, p64, pmx, st, args):
     nsteps = int(round(args.error_days * 86400.0 / args.dt))
     ds = [capi.Dyn(g, q, variant=args.variant) for q in (p64, pmx)]
@@ -196,6 +275,13 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (three times); "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
+    ap.add_argument("--rt-surrogate", action="store_true",
+                    help="instead of the headline: exact double-gray RT vs the trained MLP surrogate at hot-Jupiter "
+                         "g5 L40 (spin-up, GPU training, timing, judge statistics)")
+    ap.add_argument("--rt-spinup-steps", type=int, default=1440, help="spin-up steps with exact RT (5 days)")
+    ap.add_argument("--rt-sample-every", type=int, default=72, help="sample all columns every this many steps")
+    ap.add_argument("--rt-epochs", type=int, default=40)
+    ap.add_argument("--rt-steps", type=int, default=100, help="timed steps per form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (rhombus-sharded) "
@@ -204,6 +290,10 @@ def main():
 
     import thor_amd.capi as capi
 
+    if args.rt_surrogate:
+        os.makedirs(CACHE, exist_ok=True)
+        print(json.dumps(rt_surrogate_run(capi, args)), flush=True)
+        return
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     os.makedirs(CACHE, exist_ok=True)

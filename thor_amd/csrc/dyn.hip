@@ -199,7 +199,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     const dim3 grid_lc(8 * d.g.tiles_per_xcd * ((d.nv + d.lc - 1) / d.lc));
     const dim3 grid_lp(8 * d.g.tiles_per_xcd * ((d.nv + d.lc_prep - 1) / d.lc_prep));
     const int lc = d.lc, lcp = d.lc_prep;
-    const dim3 grid_c((d.P + kTile - 1) / kTile, d.nv + 1);  // column_prep: (column, level 0..nv)
+    // column_prep: (256 columns, level 0..nv) per block, XCD-major
+    const dim3 grid_c(8 * (((d.P + kTile - 1) / kTile + 7) / 8) * (d.nv + 1));
     const int cb = kThomasCB(d.nv);
     const dim3 grid_t((d.P + cb - 1) / cb);
     const size_t lds_t = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);

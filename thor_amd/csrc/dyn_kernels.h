@@ -416,9 +416,16 @@ __global__ __launch_bounds__(256) void k_probe_weights(Geo gu, float *wlap, floa
 // thread takes one (column, level) pair, lev in [0, nv], and recomputes the
 // values the sweep carries from the level below from their own inputs, so
 // every output is the same number the sweep produces.
+// Blocks of 256 columns x one level: block i runs on XCD i % 8, each XCD takes
+// a contiguous range of column blocks and walks their levels 0..nv
+// consecutively, so the neighbouring levels a thread reads are in that XCD's
+// L2 from the blocks just before it.
 template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, Fields f) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lev = blockIdx.y;  // 0..nv: centre lev (if < nv) and interface lev
+    const int i = blockIdx.x, idx = i >> 3;
+    const int pbx = ((g.P + 255) / 256 + 7) / 8;  // column blocks per XCD
+    const int pb = (i & 7) * pbx + idx / (g.nv + 1);
+    const int lev = idx % (g.nv + 1);  // 0..nv: centre lev (if < nv) and interface lev
+    const int p = pb * 256 + threadIdx.x;
     if (p >= g.P) return;
     const size_t S = g.S;
     const int nv = g.nv;

@@ -15,6 +15,34 @@ import numpy as np
 from . import capi
 
 
+# Hot-Jupiter set-up of the double-gray runs: R = 1e8 m, g = 9.42, Rd = 3779,
+# Cp = 13226, Omega = 2.06e-5, P0 = 1e7 Pa, isothermal 1600 K start; top at
+# 7.4e6 m (isothermal H = Rd T / g ~ 6.42e5 m, ln(1e7 / 1e2) ~ 11.5 scale
+# heights, so the top layer sits near 1e2 Pa).
+HJ_TOP = 7.4e6
+
+
+def hot_jupiter(glevel: int, nv: int, dt: float = 300.0, ns: int = 6, precision: int = 0, cache_dir=None, **kw):
+    """(grid, params) of the hot-Jupiter set-up. Hyperdiffusion and divergence
+    damping as THOR's deep hot-Jupiter benchmark (ifile/deephj.thr: Diffc 0.02,
+    DivDampc 0.03); kw overrides any parameter."""
+    g = capi.Grid(glevel, nv, 1e8, HJ_TOP, spring=True, beta=1.15, cache_dir=cache_dir)
+    q = dict(glevel=glevel, timestep=dt, n_substeps=ns, precision=precision, A=1e8, Omega=2.06e-5, Gravit=9.42,
+             P_Ref=1e7, Rd=3779.0, Cp=13226.0, Tmean=1600.0, Diffc=0.02, DivDampc=0.03)
+    q.update(kw)
+    return g, capi.default_params(**q)
+
+
+def double_gray(**kw) -> "capi.RTParams":
+    """Double-gray RT of the hot-Jupiter runs: kappa_sw 4e-4, kappa_lw 1e-3
+    m2/kg, Tint 100 K, incident flux sigma (2000 K)^4 at the substellar point,
+    f_lw 0.5, n_sw 1, n_lw 2, diffusivity 0.5, no albedo."""
+    q = dict(kappa_sw=4e-4, kappa_lw=1e-3, n_sw=1.0, n_lw=2.0, f_lw=0.5, diff_ang=0.5, Tint=100.0, albedo=0.0,
+             incflx=5.6704e-8 * 2000.0 ** 4, ps0=1e7, lon_sub=0.0)
+    q.update(kw)
+    return capi.RTParams(**q)
+
+
 def cos_zenith(grid, lon_sub: float = 0.0) -> np.ndarray:
     """cos z of a synchronous rotator with the substellar point at lon_sub, clipped at 0."""
     lon, lat = np.asarray(grid.lonlat).reshape(grid.P, 2).T
