@@ -48,6 +48,28 @@ def test_forward_matches_oracle(built):
     m.close()
 
 
+def test_fused_inference_equals_training_forward(built):
+    """thor_mlp_forward runs the fused three-layer inference kernel; the
+    training path (thor_mlp_gradients) runs three separate GEMMs with the same
+    K tiling and epilogues. With zero targets the training loss is mean(Y^2)
+    of the GEMM outputs, so it must equal mean(Y^2) of the fused outputs."""
+    capi = built
+    print("case: fused inference vs three-GEMM forward, 1000 rows (partial last block)", flush=True)
+    m = capi.MLP(*DIMS, seed=6)
+    rng = np.random.default_rng(7)
+    p = m.params()
+    nb = DIMS[0] * DIMS[1]
+    p[nb:nb + DIMS[1]] = 0.1 * rng.standard_normal(DIMS[1])
+    m.set_params(p)
+    X = rng.standard_normal((1000, DIMS[0])).astype(np.float32)
+    Y = m.forward(X).astype(np.float64)
+    _, loss = m.gradients(X, np.zeros((1000, DIMS[2]), np.float32))
+    want = float((Y ** 2).mean())
+    print(f"mean(Y^2): fused {want:.12e}, GEMM path {loss:.12e}")
+    assert abs(loss / want - 1) < 1e-9
+    m.close()
+
+
 def test_gradients_match_oracle(built):
     from oracle import mlp_oracle as mo
     capi = built

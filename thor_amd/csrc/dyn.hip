@@ -71,7 +71,6 @@ struct Dyn {
     NRPar nr{};
     RTPar rt{};
     double *insol = nullptr, *asr = nullptr, *olr = nullptr;  // [P] RT column diagnostics
-    double *fdn_sw = nullptr, *fdn_lw = nullptr;              // [nv+1][S] RT scratch
     // RT surrogate
     thor_mlp *sur = nullptr;
     SurPar sp{};
@@ -334,6 +333,18 @@ template <bool MX> static int thor_step_t(Dyn &d) {
 
 static int thor_step(Dyn &d) { return d.mixed ? thor_step_t<true>(d) : thor_step_t<false>(d); }
 
+
+// Double-gray ProfX over all columns (only = nullptr) or the flagged ones:
+// 32 columns per block up to 63 levels, 16 above (4 LDS planes of
+// (nv + 1) x columns doubles within 64 KiB).
+static void launch_rt(Dyn &d, const Geo &g, const Fields &f, double dt, const int *only) {
+    const int cb = d.nv <= 63 ? 32 : 16;
+    const size_t lds = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
+    const dim3 grid((d.P + cb - 1) / cb);
+    if (cb == 32) k_profx_rt<32><<<grid, 256, lds, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, only);
+    else k_profx_rt<16><<<grid, 256, lds, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, only);
+}
+
 static int profx(Dyn &d, int forcing) {
     if (forcing == THOR_FORCING_NONE) return THOR_OK;
     Fields f = stage_fields(d, 0);
@@ -354,10 +365,7 @@ static int profx(Dyn &d, int forcing) {
     } else if (forcing == THOR_FORCING_DOUBLE_GRAY) {
         if (!d.rt_set) return fail(THOR_ERR_BAD_ARG, "profx: RT parameters not set");
         // Wh 2, p, rho in; W, T, Qheat, p out; two flux planes written and read twice
-        launch(d, "profx_double_gray", 14 * B, [&] {
-            k_profx_rt<<<(d.P + kTile - 1) / kTile, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr,
-                                                                       d.fdn_sw, d.fdn_lw, nullptr);
-        });
+        launch(d, "profx_double_gray", 14 * B, [&] { launch_rt(d, g, f, dt, nullptr); });
     } else if (forcing == THOR_FORCING_RT_SURROGATE) {
         if (!d.rt_set || !d.sur) return fail(THOR_ERR_BAD_ARG, "profx: RT surrogate not set");
         const bool judge = d.judge_every > 0 && d.sur_calls % d.judge_every == 0;
@@ -368,7 +376,8 @@ static int profx(Dyn &d, int forcing) {
         uint16_t *X = mlp::input(d.sur);
         const float *Y = mlp::output(d.sur);
         launch(d, "surrogate_features", 6 * B, [&] {
-            k_sur_features<<<gp, kTile, 0, d.stream>>>(g, f, d.sp, X, d.sur_mask, d.sur_nflag);
+            const size_t lds = (size_t)64 * (2 * d.nv + 1) * sizeof(uint16_t);
+            k_sur_features<<<(d.P + 63) / 64, kTile, lds, d.stream>>>(g, f, d.sp, X, d.sur_mask, d.sur_nflag);
         });
         // BF16 inputs and weights, 193 kFLOP per column for 81 -> 256 -> 256 -> 40
         launch(d, "surrogate_mlp", 2.0 * d.P * (2 * d.nv + 1 + d.nv) * 2, [&] { mlp::forward_on(d.sur, d.P, d.stream); });
@@ -378,10 +387,7 @@ static int profx(Dyn &d, int forcing) {
         if (judge) {
             const size_t n1 = (size_t)d.nv * d.S;
             hipMemcpyAsync(d.qsur, f.Qheat, n1 * sizeof(double), hipMemcpyDeviceToDevice, d.stream);
-            launch(d, "profx_double_gray", 14 * B, [&] {
-                k_profx_rt<<<gp, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, d.fdn_sw, d.fdn_lw,
-                                                       nullptr);
-            });
+            launch(d, "profx_double_gray", 14 * B, [&] { launch_rt(d, g, f, dt, nullptr); });
             k_sur_error<<<d.err_nblk, 256, 0, d.stream>>>(g, f.Qheat, d.qsur, d.sur_mask, (long)n1, d.err_part,
                                                           d.err_part + d.err_nblk, d.err_part + 2 * d.err_nblk);
             std::vector<double> part((size_t)3 * d.err_nblk);
@@ -395,10 +401,7 @@ static int profx(Dyn &d, int forcing) {
             }
             d.sur_judged++;
         } else {
-            launch(d, "profx_double_gray_fallback", 14 * B, [&] {
-                k_profx_rt<<<gp, kTile, 0, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, d.fdn_sw, d.fdn_lw,
-                                                       d.sur_mask);
-            });
+            launch(d, "profx_double_gray_fallback", 14 * B, [&] { launch_rt(d, g, f, dt, d.sur_mask); });
         }
     } else {
         return fail(THOR_ERR_BAD_ARG, "unknown forcing");
@@ -731,8 +734,6 @@ extern "C" thor_status thor_dyn_set_rt(thor_dyn *h, const thor_rt_params *q) {
         hipError_t e = dalloc(*h, &h->insol, h->S);
         if (e == hipSuccess) e = dalloc(*h, &h->asr, h->S);
         if (e == hipSuccess) e = dalloc(*h, &h->olr, h->S);
-        if (e == hipSuccess) e = dalloc(*h, &h->fdn_sw, ni);
-        if (e == hipSuccess) e = dalloc(*h, &h->fdn_lw, ni);
         if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("set_rt: alloc: ") + hipGetErrorString(e));
     }
     h->rt = RTPar{q->kappa_sw, q->kappa_lw, q->n_sw, q->n_lw, q->f_lw, q->diff_ang,

@@ -2172,25 +2172,32 @@ __device__ __forceinline__ double rt_source(double bb, double bl, double bt, dou
 // column's insolation, ASR and OLR. One thread per column. Shortwave and
 // longwave downward fluxes go to two interface planes (fdn_sw, fdn_lw); the
 // upward longwave flux is carried level to level.
+// Double-gray ProfX, CB columns per 256-thread block. Phase 1 (all threads,
+// a share of the levels each): W and T, and per layer the shortwave and
+// longwave transmissions and the longwave source terms of both directions
+// into LDS; phase 2 (one lane per column): the three serial recurrences of
+// the reference (shortwave down, longwave down, longwave up) over those
+// terms; phase 3 (all threads): the layer heating, the pressure clip and
+// p = rho Rd T. Each term and each recurrence step is the reference's
+// arithmetic, so the results are those of the one-thread-per-column form.
+// LDS: 4 planes of (nv + 1) x CB doubles (A: SW transmission -> SW flux,
+// B: LW transmission, C: LW down source -> LW down flux, D: LW up source ->
+// LW up flux). only != nullptr: just the flagged columns (surrogate fallback).
+template <int CB>
 __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, double time_step, double *insol,
-                                                  double *asr, double *olr, double *fdn_sw, double *fdn_lw,
-                                                  const int *only) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
-    if (only && !only[p]) return;  // surrogate fallback: only the flagged columns
+                                                  double *asr, double *olr, const int *only) {
+    extern __shared__ double rts[];
+    constexpr int NROW = 256 / CB;
+    const int c = threadIdx.x % CB, row = threadIdx.x / CB;
+    const int p = blockIdx.x * CB + c;
+    const bool act = p < g.P && (!only || only[p]);
     const size_t S = g.S;
-    const int nv = g.nv;
+    const int nv = g.nv, L = nv + 1;
+    double *sA = rts, *sB = rts + L * CB, *sC = rts + 2 * L * CB, *sD = rts + 3 * L * CB;
     const double *Alt = g.Alt, *Alth = g.Alth;
     const double Rd = g.Rd, Cp = g.Cp, Gravit = g.Gravit, A = g.A;
     const double bc = 5.677036E-8;  // the reference's Stefan-Boltzmann constant (profx_RT.h:178)
-    // Recompute_W and T = p / (rho Rd), as ProfX does before its physics
-    for (int lev = 0; lev < nv; lev++) {
-        const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
-        const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
-        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-        LV(f.W, lev) = whl * intt + wht * intl;
-        LV(f.T, lev) = LV(f.p, lev) / (Rd * LV(f.Rho, lev));
-    }
+    auto Tl = [&](int lev) { return LV(f.p, lev) / (Rd * LV(f.Rho, lev)); };
     auto pint = [&](int lev) {  // interface pressure (phtemp)
         if (lev == 0) {
             const double psm = LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
@@ -2207,11 +2214,11 @@ __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, doub
         return LV(f.p, lev - 1) * a + LV(f.p, lev) * b;
     };
     auto tint = [&](int lev) {  // interface temperature (thtemp)
-        if (lev == 0) return LV(f.T, 0);
-        if (lev == nv) return LV(f.T, nv - 1);
+        if (lev == 0) return Tl(0);
+        if (lev == nv) return Tl(nv - 1);
         const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
         const double a = (xi - xip) / (xim - xip), b = (xi - xim) / (xip - xim);
-        return LV(f.T, lev - 1) * a + LV(f.T, lev) * b;
+        return Tl(lev - 1) * a + Tl(lev) * b;
     };
     auto tau_sw = [&](int lev) {
         return q.n_sw * q.kappa_sw * MPOW(LV(f.p, lev) / q.ps0, q.n_sw - 1) * LV(f.Rho, lev)
@@ -2223,58 +2230,79 @@ __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, doub
         return t < 0.0 ? 0.0 : t;
     };
     auto planck = [&](double t) { return bc * t * t * t * t; };
-    const double lon = g.lonlat[p], lat = g.lonlat[S + p];
-    double coszrs = MCOS(lat) * MCOS(lon - q.lon_sub);
-    if (coszrs < 0.0) coszrs = 0.0;
-    // shortwave, downward
-    double ins = 0.0;
+    double coszrs = 0.0;
+    if (act) {
+        const double lon = g.lonlat[p], lat = g.lonlat[S + p];
+        coszrs = MCOS(lat) * MCOS(lon - q.lon_sub);
+        if (coszrs < 0.0) coszrs = 0.0;
+    }
     const bool day = coszrs > 0.0;
-    if (day) {
-        ins = q.incflx * coszrs;
-        const double flux_top = ins * (1.0 - q.albedo);
-        const double tau = (q.kappa_sw / Gravit) * pint(nv);
-        double fd = flux_top * MEXP(-(1.0 / coszrs) * tau);
-        LV(fdn_sw, nv) = fd;
-        for (int lev = nv; lev >= 1; lev--) {
-            fd = fd * MEXP(-(1.0 / coszrs) * tau_sw(lev - 1));
-            LV(fdn_sw, lev - 1) = fd;
+    // phase 1
+    if (act) {
+        for (int lev = row; lev < nv; lev += NROW) {
+            const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+            const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            LV(f.W, lev) = whl * intt + wht * intl;
+            const double T = Tl(lev);
+            LV(f.T, lev) = T;
+            sA[lev * CB + c] = day ? MEXP(-(1.0 / coszrs) * tau_sw(lev)) : 0.0;
+            const double tl_ = tau_lw(lev);
+            sB[lev * CB + c] = MEXP(-(1. / q.diff_ang) * tl_);
+            const double bt = planck(tint(lev + 1)), bm = planck(T), bb = planck(tint(lev));
+            sC[lev * CB + c] = rt_source(bb, bm, bt, tl_, q.diff_ang);           // downward, layer lev
+            sD[(lev + 1) * CB + c] = rt_source(bt, bm, bb, tl_, q.diff_ang);     // upward, to interface lev + 1
         }
-    } else {
-        for (int lev = 0; lev <= nv; lev++) LV(fdn_sw, lev) = 0.0;
     }
-    // longwave, downward
-    double fl = 0.0;
-    LV(fdn_lw, nv) = 0.0;
-    for (int lev = nv - 1; lev >= 0; lev--) {
-        const double tl_ = tau_lw(lev);
-        const double ed = rt_source(planck(tint(lev)), planck(LV(f.T, lev)), planck(tint(lev + 1)), tl_, q.diff_ang);
-        fl = ed + fl * MEXP(-(1. / q.diff_ang) * tl_);
-        LV(fdn_lw, lev) = fl;
+    __syncthreads();
+    // phase 2
+    if (act && row == 0) {
+        if (day) {
+            const double flux_top = q.incflx * coszrs * (1.0 - q.albedo);
+            const double tau = (q.kappa_sw / Gravit) * pint(nv);
+            double fd = flux_top * MEXP(-(1.0 / coszrs) * tau);
+            sA[nv * CB + c] = fd;
+            for (int lev = nv; lev >= 1; lev--) {
+                fd = fd * sA[(lev - 1) * CB + c];
+                sA[(lev - 1) * CB + c] = fd;
+            }
+        } else {
+            sA[nv * CB + c] = 0.0;
+        }
+        double fl = 0.0;
+        sC[nv * CB + c] = 0.0;
+        for (int lev = nv - 1; lev >= 0; lev--) {
+            fl = sC[lev * CB + c] + fl * sB[lev * CB + c];
+            sC[lev * CB + c] = fl;
+        }
+        double fu_lo = planck(q.Tint);
+        fu_lo += sC[c];
+        sD[c] = fu_lo;
+        for (int lev = 1; lev <= nv; lev++) {
+            const double fu = sD[lev * CB + c] + fu_lo * sB[(lev - 1) * CB + c];
+            sD[lev * CB + c] = fu;
+            fu_lo = fu;
+        }
+        insol[p] = day ? q.incflx * coszrs : 0.0;
+        const double rscale = g.deep ? (A + Alth[nv]) / A : 1.0;
+        asr[p] = sA[nv * CB + c] * g.areasT[p] * MPOW(rscale, 2);
+        olr[p] = sD[nv * CB + c] * g.areasT[p] * MPOW(rscale, 2);
     }
-    // longwave, upward, and the heating of each layer below the new interface
-    double fu_lo = planck(q.Tint);
-    fu_lo += LV(fdn_lw, 0);
-    double fu = fu_lo;
-    for (int lev = 1; lev <= nv; lev++) {
-        const double tl_ = tau_lw(lev - 1);
-        const double eu = rt_source(planck(tint(lev)), planck(LV(f.T, lev - 1)), planck(tint(lev - 1)), tl_, q.diff_ang);
-        fu = eu + fu_lo * MEXP(-(1. / q.diff_ang) * tl_);
-        const int l = lev - 1;
+    __syncthreads();
+    // phase 3
+    if (!act) return;
+    for (int l = row; l < nv; l += NROW) {
         double dtemp = 0.0;
-        if (day)
-            dtemp = -((0.0 - LV(fdn_sw, l)) - (0.0 - LV(fdn_sw, l + 1))) / ((Alth[l] - Alth[l + 1]));
-        dtemp = dtemp - ((fu_lo - LV(fdn_lw, l)) - (fu - LV(fdn_lw, l + 1))) / ((Alth[l] - Alth[l + 1]));
+        if (day) dtemp = -((0.0 - sA[l * CB + c]) - (0.0 - sA[(l + 1) * CB + c])) / ((Alth[l] - Alth[l + 1]));
+        dtemp = dtemp
+                - ((sD[l * CB + c] - sC[l * CB + c]) - (sD[(l + 1) * CB + c] - sC[(l + 1) * CB + c]))
+                      / ((Alth[l] - Alth[l + 1]));
         const double pr = LV(f.p, l);
         if (pr + Rd / (Cp - Rd) * dtemp * time_step < 0) dtemp = -pr / time_step;
         LV(f.Qheat, l) = dtemp;  // Qheat is cleared at the start of ProfX, then the module adds
-        fu_lo = fu;
+        // Compute_pressure (profx_auxiliary.h:95-108) closes ProfX
+        LV(f.p, l) = Tl(l) * Rd * LV(f.Rho, l);
     }
-    // Compute_pressure (profx_auxiliary.h:95-108) closes ProfX
-    for (int lev = 0; lev < nv; lev++) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
-    insol[p] = ins;
-    const double rscale = g.deep ? (A + Alth[nv]) / A : 1.0;
-    asr[p] = LV(fdn_sw, nv) * g.areasT[p] * MPOW(rscale, 2);
-    olr[p] = fu * g.areasT[p] * MPOW(rscale, 2);
 }
 
 // ------------------------------------------------------ RT surrogate glue
@@ -2288,35 +2316,55 @@ struct SurPar {
 // Per column: Recompute_W and T = p / (rho Rd) as ProfX does, the network's
 // normalised inputs [T_k, ln p_k, cos z] as BF16 row p of X, and the flag of
 // a column whose inputs leave the training envelope.
+// 64 columns x 4 level rows per block: W from Wh and T at every level (as
+// ProfX does), the normalised inputs [T_k, ln p_k, cos z] of each column staged
+// in LDS and written out as contiguous BF16 rows; a column is outside the
+// training envelope if any of its normalised inputs is.
 __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q, uint16_t *X, int *mask,
                                                       int *nflag) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= g.P) return;
+    constexpr int CB = 64, NROW = 4;
+    extern __shared__ uint16_t sx[];  // [CB][nin]
+    __shared__ int sout[CB];
+    const int c = threadIdx.x % CB, row = threadIdx.x / CB;
+    const int p0 = blockIdx.x * CB, p = p0 + c;
     const size_t S = g.S;
     const int nv = g.nv, nin = 2 * nv + 1;
     const double *Alt = g.Alt, *Alth = g.Alth;
+    if (threadIdx.x < CB) sout[threadIdx.x] = 0;
+    __syncthreads();
     bool out = false;
     auto put = [&](int j, double x) {
         const float xn = (float)((x - q.in_mean[j]) / q.in_std[j]);
         out |= !(xn >= q.in_lo[j] && xn <= q.in_hi[j]);
-        X[(size_t)p * nin + j] = mlp::f2bf(xn);
+        sx[c * nin + j] = mlp::f2bf(xn);
     };
-    for (int lev = 0; lev < nv; lev++) {
-        const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
-        const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
-        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-        LV(f.W, lev) = whl * intt + wht * intl;
-        const double pr = LV(f.p, lev);
-        const double T = pr / (g.Rd * LV(f.Rho, lev));
-        LV(f.T, lev) = T;
-        put(lev, T);
-        put(nv + lev, MLOG(pr));
+    if (p < g.P) {
+        for (int lev = row; lev < nv; lev += NROW) {
+            const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+            const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            LV(f.W, lev) = whl * intt + wht * intl;
+            const double pr = LV(f.p, lev);
+            const double T = pr / (g.Rd * LV(f.Rho, lev));
+            LV(f.T, lev) = T;
+            put(lev, T);
+            put(nv + lev, MLOG(pr));
+        }
+        if (row == 0) {
+            double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
+            if (coszrs < 0.0) coszrs = 0.0;
+            put(2 * nv, coszrs);
+        }
+        if (out) atomicOr(&sout[c], 1);
     }
-    double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
-    if (coszrs < 0.0) coszrs = 0.0;
-    put(2 * nv, coszrs);
-    mask[p] = out ? 1 : 0;
-    if (out) atomicAdd(nflag, 1);
+    __syncthreads();
+    const int ncols = min(CB, g.P - p0);
+    for (int i = threadIdx.x; i < ncols * nin; i += 256) X[(size_t)p0 * nin + i] = sx[i];
+    if ((int)threadIdx.x < ncols) {
+        const int o = sout[threadIdx.x];
+        mask[p0 + threadIdx.x] = o;
+        if (o) atomicAdd(nflag, 1);
+    }
 }
 
 // Qheat from the network's outputs Y[p][lev] (with the RT's positivity clip),

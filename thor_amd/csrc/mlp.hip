@@ -27,6 +27,12 @@ struct Net {
     float *gm = nullptr, *gv = nullptr;  // Adam moments
     float *grad = nullptr;             // gradients, same layout
     uint16_t *wb = nullptr;            // BF16 copies of W1, W2, W3 (biases stay FP32)
+    // Transposed BF16 copies for the fused inference kernel (n_hidden = 256,
+    // n_out <= 64, n_in <= 256 only): W1^T [256][kpad], W2^T [256][256],
+    // W3^T [64][256], zero-padded.
+    bool fused = false;
+    int kpad = 0;
+    uint16_t *wt1 = nullptr, *wt2 = nullptr, *wt3 = nullptr;
     long step = 0;
     // activations for up to cap rows
     long cap = 0;
@@ -141,6 +147,28 @@ static void refresh_bf16(Net &n) {
     const long src[3] = {n.oW1(), n.oW2(), n.oW3()}, dst[3] = {n.bW1(), n.bW2(), n.bW3()};
     for (int i = 0; i < 3; i++)
         k_to_bf16<<<(unsigned)((nw[i] + 255) / 256), 256, 0, n.stream>>>(n.w + src[i], n.wb + dst[i], nw[i]);
+    if (n.fused) {
+        const int K[3] = {n.nin, n.nh, n.nh}, N[3] = {n.nh, n.nh, n.nout};
+        const int Kp[3] = {n.kpad, n.nh, n.nh}, Np[3] = {n.nh, n.nh, 64};
+        uint16_t *wt[3] = {n.wt1, n.wt2, n.wt3};
+        for (int i = 0; i < 3; i++) {
+            const long cnt = (long)Np[i] * Kp[i];
+            k_transpose_bf16<<<(unsigned)((cnt + 255) / 256), 256, 0, n.stream>>>(n.w + src[i], K[i], N[i], wt[i],
+                                                                                  Kp[i], Np[i]);
+        }
+    }
+}
+
+// Inference over rows of X0 into Y: the fused kernel when the shape allows it,
+// else the three GEMMs of the training forward pass (same results).
+static void infer(Net &n, long rows) {
+    if (!n.fused) {
+        forward(n, rows);
+        return;
+    }
+    InferArgs a{n.X0, rows, n.nin, n.nout, n.kpad, n.wt1, n.wt2, n.wt3, n.w + n.ob1(), n.w + n.ob2(), n.w + n.ob3(),
+                n.Y};
+    k_mlp_infer<256><<<(unsigned)((rows + 31) / 32), 256, 0, n.stream>>>(a);
 }
 
 static double sum_loss(Net &n, int slots) {
@@ -183,6 +211,13 @@ extern "C" thor_status thor_mlp_create(int n_in, int n_hidden, int n_out, unsign
     if (e == hipSuccess) e = alloc(n, &n.gv, n.np);
     if (e == hipSuccess) e = alloc(n, &n.grad, n.np);
     if (e == hipSuccess) e = alloc(n, &n.wb, n.nwb());
+    n.fused = n_hidden == 256 && n_out <= 64 && n_in <= 256;
+    if (n.fused) {
+        n.kpad = (n_in + 15) / 16 * 16;
+        if (e == hipSuccess) e = alloc(n, &n.wt1, (size_t)256 * n.kpad);
+        if (e == hipSuccess) e = alloc(n, &n.wt2, (size_t)256 * 256);
+        if (e == hipSuccess) e = alloc(n, &n.wt3, (size_t)64 * 256);
+    }
     if (e != hipSuccess) {
         thor_mlp_destroy(m);
         return fail(THOR_ERR_HIP, std::string("thor_mlp_create: ") + hipGetErrorString(e));
@@ -292,7 +327,7 @@ extern "C" thor_status thor_mlp_forward(thor_mlp *m, const float *X, float *Y, l
     hipError_t e = hipMemcpyAsync(dX, X, (size_t)n * net.nin * sizeof(float), hipMemcpyHostToDevice, net.stream);
     if (e == hipSuccess) {
         k_to_bf16<<<(unsigned)((n * net.nin + 255) / 256), 256, 0, net.stream>>>(dX, net.X0, n * net.nin);
-        forward(net, n);
+        infer(net, n);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(Y, net.Y, (size_t)n * net.nout * sizeof(float), hipMemcpyDeviceToHost, net.stream);
@@ -333,9 +368,9 @@ extern "C" thor_status thor_mlp_time_forward(thor_mlp *m, long n, int repeats, d
     hipEvent_t a, b;
     MLP_TRY(hipEventCreate(&a));
     MLP_TRY(hipEventCreate(&b));
-    forward(net, n);  // warm-up
+    infer(net, n);  // warm-up
     (void)hipEventRecord(a, net.stream);
-    for (int i = 0; i < repeats; i++) forward(net, n);
+    for (int i = 0; i < repeats; i++) infer(net, n);
     (void)hipEventRecord(b, net.stream);
     hipError_t e = hipEventSynchronize(b);
     float t = 0.f;
@@ -357,7 +392,7 @@ void forward_on(thor_mlp *m, long rows, hipStream_t s) {
     Net &n = *m;
     const hipStream_t keep = n.stream;
     n.stream = s;
-    forward(n, rows);
+    infer(n, rows);
     n.stream = keep;
 }
 void dims(const thor_mlp *m, int *nin, int *nout) {

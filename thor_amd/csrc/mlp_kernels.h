@@ -106,6 +106,106 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
             }
 }
 
+// Fused inference for the surrogate: 32 rows per 256-thread block, all three
+// layers in one kernel with the hidden activations in LDS (two row-major
+// buffers of 32 x (NH + 8) BF16: X then H2 share one, H1 the other). The K
+// tiling (16 per MFMA, ascending) and the epilogues are those of k_gemm, so
+// the outputs equal the three-GEMM forward pass. Weights come from
+// transposed BF16 copies W^T [n][k] (zero-padded), so a lane's 4 consecutive
+// k of one output column are one 8-byte load.
+struct InferArgs {
+    const uint16_t *X;  // [rows][nin]
+    long rows;
+    int nin, nout, kpad;  // kpad = nin rounded up to 16
+    const uint16_t *W1t, *W2t, *W3t;  // [NH][kpad], [NH][NH], [64][NH]
+    const float *b1, *b2, *b3;
+    float *Y;  // [rows][nout]
+};
+
+// acc[ti][tj] += A[16 ti.., k] * W[k, n0 + 16 tj..] over k in [0, K), A in LDS
+// (row stride lda), W^T in global memory (row stride ldw).
+template <int NT>
+__device__ __forceinline__ void infer_layer(const uint16_t *sA, int lda, int K, const uint16_t *Wt, int ldw, int n0,
+                                            v4f acc[2][NT]) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = (lane >> 4) * 4;
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < NT; tj++) acc[ti][tj] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        v4s a[2], b[NT];
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++) a[ti] = *(const v4s *)&sA[(16 * ti + li) * lda + k0 + lk];
+#pragma unroll
+        for (int tj = 0; tj < NT; tj++) b[tj] = *(const v4s *)&Wt[(size_t)(n0 + 16 * tj + li) * ldw + k0 + lk];
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int tj = 0; tj < NT; tj++)
+                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+    }
+}
+
+template <int NH> __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
+    constexpr int R = 32, LDH = NH + 8;
+    __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = (lane >> 4) * 4;
+    const long r0 = (long)blockIdx.x * R;
+    const int ldx = g.kpad + 8;
+    for (int i = tid; i < R * g.kpad; i += 256) {
+        const int r = i / g.kpad, k = i % g.kpad;
+        bufA[r * ldx + k] = (r0 + r < g.rows && k < g.nin) ? g.X[(r0 + r) * g.nin + k] : (uint16_t)0;
+    }
+    __syncthreads();
+    v4f acc[2][4];
+    // layer 1: X (bufA) -> H1 (bufB); wave w owns output columns [64 w, 64 w + 64)
+    infer_layer<4>(bufA, ldx, g.kpad, g.W1t, g.kpad, 64 * wave, acc);
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = 16 * ti + lk + e, col = 64 * wave + 16 * tj + li;
+                bufB[row * LDH + col] = f2bf(gelu(acc[ti][tj][e] + g.b1[col]));
+            }
+    __syncthreads();
+    // layer 2: H1 (bufB) -> H2 (bufA)
+    infer_layer<4>(bufB, LDH, NH, g.W2t, NH, 64 * wave, acc);
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int tj = 0; tj < 4; tj++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int row = 16 * ti + lk + e, col = 64 * wave + 16 * tj + li;
+                bufA[row * LDH + col] = f2bf(gelu(acc[ti][tj][e] + g.b2[col]));
+            }
+    __syncthreads();
+    // layer 3: H2 (bufA) -> Y; wave w owns output columns [16 w, 16 w + 16) of 64 padded
+    v4f acc3[2][1];
+    infer_layer<1>(bufA, LDH, NH, g.W3t, NH, 16 * wave, acc3);
+    const int col = 16 * wave + li;
+    if (col >= g.nout) return;
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const long row = r0 + 16 * ti + lk + e;
+            if (row < g.rows) g.Y[row * g.nout + col] = acc3[ti][0][e] + g.b3[col];
+        }
+}
+
+// dst[n][k] = bf16(src[k][n]) for k < K, n < N; zero in the padding up to
+// [Npad][Kpad].
+__global__ void k_transpose_bf16(const float *src, int K, int N, uint16_t *dst, int Kpad, int Npad) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)Npad * Kpad) return;
+    const int n = (int)(i / Kpad), k = (int)(i % Kpad);
+    dst[i] = (n < N && k < K) ? f2bf(src[(size_t)k * N + n]) : (uint16_t)0;
+}
+
 __global__ void k_to_bf16(const float *src, uint16_t *dst, long n) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = f2bf(src[i]);
