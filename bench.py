@@ -148,7 +148,8 @@ def diagnostics(g, p, st):
 
 def _profx_us(stats, names):
     """Average device microseconds per ProfX call of the listed kernels."""
-    calls = max((s["launches"] for s in stats if s["name"] in ("profx_double_gray", "surrogate_features")), default=1)
+    calls = max((s["launches"] for s in stats
+                 if s["name"] in ("profx_double_gray", "surrogate_features", "surrogate_profx")), default=1)
     return 1e3 * sum(s["total_ms"] for s in stats if s["name"] in names) / calls
 
 
@@ -202,7 +203,7 @@ def rt_surrogate_run(capi, args):
         if forcing == capi.FORCING_RT_SURROGATE:
             d.surrogate_stats()
         rt_names = ("profx_double_gray", "profx_double_gray_fallback", "surrogate_features", "surrogate_mlp",
-                    "surrogate_apply", "surrogate_error")
+                    "surrogate_apply", "surrogate_profx")
         out[form] = {"ms_per_step": round(1e3 * wall / n, 4), "sdpd": round(300.0 * n / wall, 1),
                      "profx_us": round(_profx_us(ks, rt_names), 1),
                      "kernels_us": {s["name"]: round(1e3 * s["total_ms"] / s["launches"], 1) for s in ks

@@ -143,3 +143,48 @@ def test_surrogate_in_the_hot_jupiter_step(built):
     assert all(np.isfinite(v).all() for v in d.get_state().values())
     d.close()
     m.close()
+
+
+def test_surrogate_profx_is_the_network_on_column_features(built):
+    """One surrogate ProfX (no judge) against the network evaluated by
+    thor_mlp_forward on features formed on the host from the same state: the
+    fused kernel's features, envelope test, inference and heating must agree
+    (up to one ulp of ln p in the features)."""
+    from thor_amd import surrogate as sg
+    capi = built
+    print("case: hot Jupiter g4 L20, surrogate ProfX vs host features + thor_mlp_forward", flush=True)
+    g = hj_grid(capi, 4, 20)
+    p = hj_params(capi, 4)
+    st = capi.ic_rest(g, p, seed=1005, temperature_noise=1.0, wind_noise=0.1)
+    d = capi.Dyn(g, p)
+    d.set_state(st)
+    d.set_rt(rt_params(capi))
+    X, Y = sg.collect(d, g, nsteps=6, every=2)
+    norm = sg.normalisation(X, Y)
+    nv = g.nv
+    m = capi.MLP(2 * nv + 1, 256, nv, seed=8)
+    sg.train(m, *sg.apply_norm(X, Y, norm), epochs=3, batch=256, lr=2e-3)
+    d.set_rt_surrogate(m, norm, judge_every=0)
+    pres = d.get_field("pressure")
+    rho = d.get_field("Rho")
+    T = pres / (p.Rd * rho)
+    Xh = sg.features(T, pres, sg.cos_zenith(g))
+    Xn = ((Xh - norm["in_mean"].astype(np.float64)) / norm["in_std"].astype(np.float64)).astype(np.float32)
+    inside = ((Xn >= norm["in_lo"]) & (Xn <= norm["in_hi"])).all(axis=1)
+    d.profx(capi.FORCING_RT_SURROGATE)
+    d.sync()
+    Q = d.get_field("profx_Qheat")
+    Yn = m.forward(Xn).astype(np.float64)
+    Qh = Yn * norm["out_std"].astype(np.float64) + norm["out_mean"].astype(np.float64)
+    dt = p.timestep
+    clip = pres + p.Rd / (p.Cp - p.Rd) * Qh * dt < 0
+    Qh = np.where(clip, -pres / dt, Qh)
+    err = np.abs(Q[inside] - Qh[inside]).max() / np.abs(Qh[inside]).max()
+    stats = d.surrogate_stats()
+    print(f"{inside.sum()} of {g.P} columns inside the envelope (library: {g.P - stats['fallback_columns']}); "
+          f"max rel diff {err:.2e}")
+    assert stats["fallback_columns"] == int((~inside).sum())
+    assert err < 1e-5
+    d.close()
+    m.close()
+

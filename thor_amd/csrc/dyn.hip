@@ -375,6 +375,14 @@ static int profx(Dyn &d, int forcing) {
         const dim3 gp((d.P + kTile - 1) / kTile), gpl((d.P + kTile - 1) / kTile, d.nv);
         uint16_t *X = mlp::input(d.sur);
         const float *Y = mlp::output(d.sur);
+        mlp::InferWeights iw;
+        if (d.nv <= 64 && mlp::fused_weights(d.sur, &iw)) {
+            // features, network and heating in one kernel
+            launch(d, "surrogate_profx", 10 * B, [&] {
+                k_sur_fused<256><<<(d.P + 31) / 32, 256, 0, d.stream>>>(g, f, d.sp, iw, d.sur_mask, d.sur_nflag, dt,
+                                                                       judge ? 0 : 1);
+            });
+        } else {
         launch(d, "surrogate_features", 6 * B, [&] {
             const size_t lds = (size_t)64 * (2 * d.nv + 1) * sizeof(uint16_t);
             k_sur_features<<<(d.P + 63) / 64, kTile, lds, d.stream>>>(g, f, d.sp, X, d.sur_mask, d.sur_nflag);
@@ -384,6 +392,7 @@ static int profx(Dyn &d, int forcing) {
         launch(d, "surrogate_apply", 4 * B, [&] {
             k_sur_apply<<<gpl, kTile, 0, d.stream>>>(g, f, d.sp, Y, d.sur_mask, dt, judge ? 0 : 1);
         });
+        }
         if (judge) {
             const size_t n1 = (size_t)d.nv * d.S;
             hipMemcpyAsync(d.qsur, f.Qheat, n1 * sizeof(double), hipMemcpyDeviceToDevice, d.stream);

@@ -10,6 +10,7 @@
 #pragma once
 #include "dyn_common.h"
 #include "bf16.h"
+#include "mlp_layer.h"
 #include "portable_math.h"
 
 namespace thor {
@@ -2370,6 +2371,91 @@ __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q,
 // Qheat from the network's outputs Y[p][lev] (with the RT's positivity clip),
 // for the columns inside the envelope; with set_p, also Compute_pressure
 // p = T Rd rho for them (the exact RT does it for the others).
+// The whole surrogate ProfX for 32 columns per 256-thread block (32 columns x
+// 8 level groups): ProfX's W from Wh and T, the normalised inputs [T_k, ln p_k,
+// cos z] staged in LDS as BF16 with the envelope flag of each column, the
+// three MFMA layers (mlp3_rows), and Qheat = y out_std + out_mean with the
+// reference's pressure clip for the columns inside the envelope; set_p: also
+// p = rho Rd T, which closes ProfX (not on judge calls, where the exact RT
+// follows). The same values as k_sur_features + the inference kernel +
+// k_sur_apply. Needs nv <= 64 (the network's 64 padded outputs).
+template <int NH>
+__global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, mlp::InferWeights w, int *mask,
+                                                   int *nflag, double time_step, int set_p) {
+    constexpr int R = 32, LDH = NH + 8;
+    __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
+    __shared__ int sout[R];
+    const int tid = threadIdx.x, r = tid & (R - 1), grp = tid / R;
+    const int p = blockIdx.x * R + r;
+    const size_t S = g.S;
+    const int nv = g.nv, nin = 2 * nv + 1, ldx = w.kpad + 8;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    if (tid < R) sout[tid] = 0;
+    __syncthreads();
+    const bool valid = p < g.P;
+    bool out = false;
+    auto put = [&](int j, double x) {
+        const float xn = (float)((x - q.in_mean[j]) / q.in_std[j]);
+        out |= !(xn >= q.in_lo[j] && xn <= q.in_hi[j]);
+        bufA[r * ldx + j] = mlp::f2bf(xn);
+    };
+    for (int lev = grp; lev < nv; lev += 256 / R) {
+        if (valid) {
+            const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+            const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            LV(f.W, lev) = whl * intt + wht * intl;
+            const double pr = LV(f.p, lev);
+            const double T = pr / (g.Rd * LV(f.Rho, lev));
+            LV(f.T, lev) = T;
+            put(lev, T);
+            put(nv + lev, MLOG(pr));
+        } else {
+            bufA[r * ldx + lev] = bufA[r * ldx + nv + lev] = 0;
+        }
+    }
+    if (grp == 0) {
+        if (valid) {
+            double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
+            if (coszrs < 0.0) coszrs = 0.0;
+            put(2 * nv, coszrs);
+        } else {
+            bufA[r * ldx + 2 * nv] = 0;
+        }
+        for (int k = nin; k < w.kpad; k++) bufA[r * ldx + k] = 0;
+    }
+    if (out) atomicOr(&sout[r], 1);
+    __syncthreads();
+    float y[2][4];
+    mlp::mlp3_rows<NH>(bufA, bufB, w, y);
+    // bufB is free once layer 2 is done: stage the outputs there as [32][64] FP32
+    float *sY = reinterpret_cast<float *>(bufB);
+    {
+        const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = (lane >> 4) * 4;
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) sY[(16 * ti + lk + e) * 64 + 16 * wave + li] = y[ti][e];
+    }
+    __syncthreads();
+    if (!valid) return;
+    const int o = sout[r];
+    if (!o) {
+        const double Rd = g.Rd, Cp = g.Cp;
+        for (int lev = grp; lev < nv; lev += 256 / R) {
+            double Q = (double)sY[r * 64 + lev] * q.out_std[lev] + q.out_mean[lev];
+            const double pr = LV(f.p, lev);
+            if (pr + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr / time_step;
+            LV(f.Qheat, lev) = Q;
+            if (set_p) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
+        }
+    }
+    if (grp == 0) {
+        mask[p] = o;
+        if (o) atomicAdd(nflag, 1);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sur_apply(Geo g, Fields f, SurPar q, const float *Y, const int *mask,
                                                    double time_step, int set_p) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;

@@ -159,6 +159,10 @@ static void refresh_bf16(Net &n) {
     }
 }
 
+static InferWeights infer_weights(const Net &n) {
+    return InferWeights{n.wt1, n.wt2, n.wt3, n.w + n.ob1(), n.w + n.ob2(), n.w + n.ob3(), n.nin, n.nout, n.kpad};
+}
+
 // Inference over rows of X0 into Y: the fused kernel when the shape allows it,
 // else the three GEMMs of the training forward pass (same results).
 static void infer(Net &n, long rows) {
@@ -166,8 +170,7 @@ static void infer(Net &n, long rows) {
         forward(n, rows);
         return;
     }
-    InferArgs a{n.X0, rows, n.nin, n.nout, n.kpad, n.wt1, n.wt2, n.wt3, n.w + n.ob1(), n.w + n.ob2(), n.w + n.ob3(),
-                n.Y};
+    InferArgs a{n.X0, rows, infer_weights(n), n.Y};
     k_mlp_infer<256><<<(unsigned)((rows + 31) / 32), 256, 0, n.stream>>>(a);
 }
 
@@ -394,6 +397,11 @@ void forward_on(thor_mlp *m, long rows, hipStream_t s) {
     n.stream = s;
     infer(n, rows);
     n.stream = keep;
+}
+bool fused_weights(const thor_mlp *m, InferWeights *w) {
+    if (!m->fused) return false;
+    *w = infer_weights(*m);
+    return true;
 }
 void dims(const thor_mlp *m, int *nin, int *nout) {
     *nin = m->nin;

@@ -7,6 +7,8 @@
 
 #include <cstdint>
 
+#include "mlp_layer.h"
+
 namespace thor {
 namespace mlp {
 // Makes room for rows input rows; input() is then the BF16 [rows][n_in]
@@ -17,5 +19,7 @@ uint16_t *input(thor_mlp *m);
 const float *output(thor_mlp *m);
 void forward_on(thor_mlp *m, long rows, hipStream_t s);
 void dims(const thor_mlp *m, int *nin, int *nout);
+// Weights of the fused inference kernel (false if the network's shape has none).
+bool fused_weights(const thor_mlp *m, InferWeights *w);
 } // namespace mlp
 } // namespace thor

@@ -5,12 +5,11 @@
 #include <cstdint>
 
 #include "bf16.h"
+#include "mlp_layer.h"
 
 namespace thor {
 namespace mlp {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
-typedef float v4f __attribute__((ext_vector_type(4)));
 
 enum { EPI_F32 = 0, EPI_BIAS_GELU = 1, EPI_BIAS_F32 = 2 };
 
@@ -31,7 +30,6 @@ struct GemmArgs {
     const float *bias;
 };
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ inline float gelu_grad(float z) {
     return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
@@ -106,94 +104,47 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
             }
 }
 
-// Fused inference for the surrogate: 32 rows per 256-thread block, all three
-// layers in one kernel with the hidden activations in LDS (two row-major
-// buffers of 32 x (NH + 8) BF16: X then H2 share one, H1 the other). The K
-// tiling (16 per MFMA, ascending) and the epilogues are those of k_gemm, so
-// the outputs equal the three-GEMM forward pass. Weights come from
-// transposed BF16 copies W^T [n][k] (zero-padded), so a lane's 4 consecutive
-// k of one output column are one 8-byte load.
+// Fused inference: 32 rows per 256-thread block, all three layers in one
+// kernel with the hidden activations in LDS (mlp3_rows). The block's input
+// rows are one contiguous run of X; each thread loads its share of one row
+// (all loads issued before the first store).
 struct InferArgs {
     const uint16_t *X;  // [rows][nin]
     long rows;
-    int nin, nout, kpad;  // kpad = nin rounded up to 16
-    const uint16_t *W1t, *W2t, *W3t;  // [NH][kpad], [NH][NH], [64][NH]
-    const float *b1, *b2, *b3;
+    InferWeights w;
     float *Y;  // [rows][nout]
 };
-
-// acc[ti][tj] += A[16 ti.., k] * W[k, n0 + 16 tj..] over k in [0, K), A in LDS
-// (row stride lda), W^T in global memory (row stride ldw).
-template <int NT>
-__device__ __forceinline__ void infer_layer(const uint16_t *sA, int lda, int K, const uint16_t *Wt, int ldw, int n0,
-                                            v4f acc[2][NT]) {
-    const int lane = threadIdx.x & 63, li = lane & 15, lk = (lane >> 4) * 4;
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < NT; tj++) acc[ti][tj] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        v4s a[2], b[NT];
-#pragma unroll
-        for (int ti = 0; ti < 2; ti++) a[ti] = *(const v4s *)&sA[(16 * ti + li) * lda + k0 + lk];
-#pragma unroll
-        for (int tj = 0; tj < NT; tj++) b[tj] = *(const v4s *)&Wt[(size_t)(n0 + 16 * tj + li) * ldw + k0 + lk];
-#pragma unroll
-        for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-            for (int tj = 0; tj < NT; tj++)
-                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
-    }
-}
 
 template <int NH> __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
     constexpr int R = 32, LDH = NH + 8;
     __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = (lane >> 4) * 4;
     const long r0 = (long)blockIdx.x * R;
-    const int ldx = g.kpad + 8;
-    for (int i = tid; i < R * g.kpad; i += 256) {
-        const int r = i / g.kpad, k = i % g.kpad;
-        bufA[r * ldx + k] = (r0 + r < g.rows && k < g.nin) ? g.X[(r0 + r) * g.nin + k] : (uint16_t)0;
+    const int nin = g.w.nin, kpad = g.w.kpad, ldx = kpad + 8;
+    {
+        const int r = tid >> 3, part = tid & 7, kper = kpad / 8;  // 32 rows x 8 parts, kper <= 32
+        const bool rv = r0 + r < g.rows;
+        uint16_t v[32];
+#pragma unroll
+        for (int j = 0; j < 32; j++) {
+            const int k = part * kper + j;
+            v[j] = (j < kper && rv && k < nin) ? g.X[(r0 + r) * nin + k] : (uint16_t)0;
+        }
+#pragma unroll
+        for (int j = 0; j < 32; j++)
+            if (j < kper) bufA[r * ldx + part * kper + j] = v[j];
     }
     __syncthreads();
-    v4f acc[2][4];
-    // layer 1: X (bufA) -> H1 (bufB); wave w owns output columns [64 w, 64 w + 64)
-    infer_layer<4>(bufA, ldx, g.kpad, g.W1t, g.kpad, 64 * wave, acc);
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 4; tj++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int row = 16 * ti + lk + e, col = 64 * wave + 16 * tj + li;
-                bufB[row * LDH + col] = f2bf(gelu(acc[ti][tj][e] + g.b1[col]));
-            }
-    __syncthreads();
-    // layer 2: H1 (bufB) -> H2 (bufA)
-    infer_layer<4>(bufB, LDH, NH, g.W2t, NH, 64 * wave, acc);
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int tj = 0; tj < 4; tj++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int row = 16 * ti + lk + e, col = 64 * wave + 16 * tj + li;
-                bufA[row * LDH + col] = f2bf(gelu(acc[ti][tj][e] + g.b2[col]));
-            }
-    __syncthreads();
-    // layer 3: H2 (bufA) -> Y; wave w owns output columns [16 w, 16 w + 16) of 64 padded
-    v4f acc3[2][1];
-    infer_layer<1>(bufA, LDH, NH, g.W3t, NH, 16 * wave, acc3);
+    float y[2][4];
+    mlp3_rows<NH>(bufA, bufB, g.w, y);
     const int col = 16 * wave + li;
-    if (col >= g.nout) return;
+    if (col >= g.w.nout) return;
 #pragma unroll
     for (int ti = 0; ti < 2; ti++)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
             const long row = r0 + 16 * ti + lk + e;
-            if (row < g.rows) g.Y[row * g.nout + col] = acc3[ti][0][e] + g.b3[col];
+            if (row < g.rows) g.Y[row * g.w.nout + col] = y[ti][e];
         }
 }
 
