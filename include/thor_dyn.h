@@ -24,6 +24,8 @@
  *   thor_ic_rest           <- ESP::initial_values(...) rest branch, src/ESP/esp_initial.cu:502-1139
  *   thor_dyn_create        <- ESP::ESP(...) + ESP::alloc_data + the Kd set-up of
  *                             ESP::initial_values (src/ESP/esp_initial.cu:68-483, 1292-1392)
+ *   thor_dyn_set_sponge    <- RayleighSponge (raysp_calc_mode = imp) of ESP::ProfX,
+ *                             src/ESP/profx_driver.cu:92-195, src/ESP/profx_sponge.cu:50-437
  *   thor_dyn_set_hooks     <- phy_modules_dyn_core_loop_init / _slow_modes / _fast_modes / _end
  *                             src/headers/phy_modules.h:76-86, called at thor_driver.cu:116,641,1290,1465
  *   thor_dyn_register_rk_array
@@ -184,6 +186,22 @@ thor_status thor_dyn_set_rt(thor_dyn *h, const thor_rt_params *p);
 /* Per-column RT diagnostics of the last double-gray ProfX: insolation
  * [W/m2], ASR and OLR [W] (reference ASR_d, OLR_d: flux x areasT). n = P. */
 thor_status thor_dyn_get_rt_column(thor_dyn *h, const char *name, double *dst, long n);
+
+/* Rayleigh sponge at the top of the model: THOR's RayleighSponge with
+ * raysp_calc_mode = imp (profx_driver.cu:92-195; zonal_uv, zonal_w and
+ * sponge_layer, profx_sponge.cu:50-437; latitude bins, grid.cu:3215-3262).
+ * At the start of every ProfX with a forcing, in the layers with
+ * Altitude / Top_altitude >= ns, the horizontal winds u, v and the vertical
+ * wind w relax implicitly towards their zonal means over nlat_bins latitude
+ * bands (towards zero when damp_*_to_mean = 0) at the rates Ruv, Rw [1/s]
+ * times sin^2(pi/2 (n - ns) / (1 - ns)); Mh, W and the interior Wh are
+ * rebuilt from the new winds. p = NULL switches the sponge off. THOR's
+ * temperature sponge and shrinking sponge are not supported. */
+typedef struct thor_sponge_params {
+    double ns, Ruv, Rw;
+    int nlat_bins, damp_uv_to_mean, damp_w_to_mean;
+} thor_sponge_params;
+thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params *p);
 
 /* nsteps x (ProfX + Thor) without host synchronisation. */
 thor_status thor_dyn_run(thor_dyn *h, int nsteps, int forcing);

@@ -72,6 +72,10 @@ def lib(variant: str = ""):
         for f in (L.oracle_set_newtonian, L.oracle_set_rt):
             f.argtypes = [C.c_void_p, _dp]
             f.restype = None
+        L.oracle_set_sponge.argtypes = [C.c_void_p, _dp]
+        L.oracle_set_sponge.restype = None
+        L.oracle_zonal_table.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_long, C.POINTER(C.c_int)]
+        L.oracle_zonal_table.restype = C.c_int
         L.oracle_profx.argtypes = [C.c_void_p, C.c_int]
         L.oracle_profx.restype = C.c_int
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
@@ -162,6 +166,23 @@ class Oracle:
         """q: a ctypes RTParams (or 11 numbers in its field order)."""
         a = np.array([getattr(q, n) for n, _ in q._fields_] if hasattr(q, "_fields_") else q, dtype=np.float64)
         self.L.oracle_set_rt(self._h, a.ctypes.data_as(_dp))
+
+    def set_sponge(self, q):
+        """q: a ctypes SpongeParams (or its 6 numbers in field order); None: off."""
+        if q is None:
+            self.L.oracle_set_sponge(self._h, None)
+            return
+        a = np.array([float(getattr(q, n)) for n, _ in q._fields_] if hasattr(q, "_fields_") else q,
+                     dtype=np.float64)
+        self.L.oracle_set_sponge(self._h, a.ctypes.data_as(_dp))
+
+    def zonal_table(self):
+        """(table [P][3] = bin, points in bin, index in bin; max_count)."""
+        t = np.zeros((self.grid.P, 3), dtype=np.int32)
+        mc = C.c_int()
+        if self.L.oracle_zonal_table(self._h, t.ctypes.data_as(C.POINTER(C.c_int)), t.size, C.byref(mc)) != 0:
+            raise ValueError("sponge not set")
+        return t, mc.value
 
     def profx_forcing(self, forcing: int):
         if self.L.oracle_profx(self._h, forcing) != 0:

@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+double oracle_tree_sum(const double *d, long n);
+
 namespace {
 
 const int NL = 16, NHL = 18, NHL2 = NHL * NHL;
@@ -44,6 +46,11 @@ struct Oracle {
     std::map<std::string, std::vector<double>> arr;
     std::vector<double> Kdh4, Kdhz;
     double nrq[8] = {0}, rtq[11] = {0};  // forcing parameters
+    // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean
+    bool sponge_on = false;
+    double spq[6] = {0};
+    std::vector<int> ztab;  // [P][3]: latitude bin, points in the bin, index within the bin
+    int max_count = 0;
 
     double *A_(const char *n) { return arr[n].data(); }
 };
@@ -1323,8 +1330,8 @@ void profx_hs(Oracle &o) {
            *Rho = o.A_("Rho"), *Rd = o.A_("Rd"), *Cp = o.A_("Cp");
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     zero(o, "profx_Qheat");
-    // Recompute_W (profx_auxiliary.h:110-130)
-    for (int id = 0; id < o.P; id++)
+    // Recompute_W (profx_auxiliary.h:110-130); with the sponge it ran before it
+    for (int id = 0; id < o.P && !o.sponge_on; id++)
         for (int lev = 0; lev < nv; lev++) {
             double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
             double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
@@ -1391,7 +1398,7 @@ void profx_nr(Oracle &o) {
             double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
             double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            W[e] = whl * intt + wht * intl;
+            if (!o.sponge_on) W[e] = whl * intt + wht * intl;
             const double pre = p[e], rho = Rho[e];
             double t = pre / (Rd * rho);
             const double lnp = MLOG(pre);
@@ -1450,7 +1457,7 @@ void profx_rt(Oracle &o) {
             double whl = Wh[id * nvi + lev], wht = Wh[id * nvi + lev + 1];
             double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            W[e] = whl * intt + wht * intl;
+            if (!o.sponge_on) W[e] = whl * intt + wht * intl;
             T[e] = p[e] / (Rd * Rho[e]);
         }
     std::vector<double> phtemp(nvi), thtemp(nvi), ttemp(nv), tau(2 * nv), dtemp(nv), fsw_up(nvi), fsw_dn(nvi),
@@ -1533,6 +1540,152 @@ void profx_rt(Oracle &o) {
         }
     }
     for (int e = 0; e < o.P * nv; e++) p[e] = T[e] * Rd * Rho[e];
+}
+
+// ------------------------------------------------------ Rayleigh sponge
+// Latitude bins of the zonal means (grid.cu:3215-3262): nlat bins of width
+// pi / nlat from -pi/2 (edges accumulated one by one); each point gets its bin,
+// the number of points in that bin and its index within the bin in point
+// order; points at or above the last edge go to the top bin. max_count is the
+// largest bin rounded up to a power of two.
+void zonal_table(Oracle &o, int nlat) {
+    const int P = o.P;
+    const double *ll = o.g.lonlat;
+    const double des_lat = M_PI / nlat;
+    std::vector<int> count(nlat, 0);
+    std::vector<double> edge(nlat + 1, 0.0);
+    o.ztab.assign((size_t)P * 3, 0);
+    edge[0] = -M_PI / 2.0;
+    for (int j = 1; j < nlat + 1; j++) edge[j] = edge[j - 1] + des_lat;
+    for (int i = 0; i < P; i++) {
+        const double lat = ll[i * 2 + 1];
+        for (int j = 0; j < nlat; j++)
+            if (lat >= edge[j] && lat < edge[j + 1]) {
+                o.ztab[i * 3] = j;
+                o.ztab[i * 3 + 2] = count[j]++;
+                break;
+            }
+        if (lat >= edge[nlat]) {
+            o.ztab[i * 3] = nlat - 1;
+            o.ztab[i * 3 + 2] = count[nlat - 1]++;
+        }
+    }
+    int mc = 0;
+    for (int i = 0; i < P; i++) {
+        o.ztab[i * 3 + 1] = count[o.ztab[i * 3]];
+        mc = std::max(mc, o.ztab[i * 3 + 1]);
+    }
+    o.max_count = (int)std::pow(2, std::ceil(std::log(mc) / std::log(2)));
+}
+
+// RayleighSponge with raysp_calc_mode = imp at the start of ProfX
+// (profx_driver.cu:92-195): Recompute_W, the zonal means of u, v, w per
+// (bin, level) (zonal_uv / zonal_w, profx_sponge.cu:50-102: each point stores
+// value / count in its bin's slot; gpu_sum_on_device_sponge, reduction_add.h:
+// 193-282: 1024-thread tree per 2048 slots, block results added on the host in
+// order), then sponge_layer (profx_sponge.cu:138-437, implicit branch): winds
+// interpolated in latitude between bin centres, relaxed with
+// k = R sin^2(pi/2 (n - ns) / (1 - ns)) for n = Altitude / Top >= ns, M and W
+// rebuilt from the new winds, Wh interpolated from W at interior interfaces.
+void sponge(Oracle &o) {
+    const int nv = o.nv, P = o.P, nlat = (int)o.spq[3];
+    const double ns = o.spq[0], Ruv = o.spq[1], Rw = o.spq[2];
+    const bool dmu = o.spq[4] != 0.0, dmw = o.spq[5] != 0.0;
+    double *W = o.A_("W"), *Wh = o.A_("Wh"), *Mh = o.A_("Mh"), *Rho = o.A_("Rho");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat;
+    const double dt = o.p.timestep;
+    for (int id = 0; id < P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
+            double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+            double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            W[id * nv + lev] = whl * intt + wht * intl;
+        }
+    auto lon_of = [&](int id) { return ll[id * 2] < 0 ? ll[id * 2] + 2.0 * M_PI : ll[id * 2]; };
+    auto uv = [&](int id, int lev, double &u, double &v) {
+        const double lon = lon_of(id), lat = ll[id * 2 + 1], rho = Rho[id * nv + lev];
+        const double *M = &Mh[(id * nv + lev) * 3];
+        u = (M[0] * (-MSIN(lon)) + M[1] * (MCOS(lon)) + M[2] * 0) / rho;
+        v = (M[0] * (-MSIN(lat) * MCOS(lon)) + M[1] * (-MSIN(lat) * MSIN(lon)) + M[2] * (MCOS(lat))) / rho;
+    };
+    const long L = o.max_count;
+    std::vector<double> vbar((size_t)nlat * nv * 3, 0.0);
+    if (dmu || dmw) {
+        std::vector<double> ut((size_t)nlat * nv * L, 0.0), vt(ut.size(), 0.0), wt(ut.size(), 0.0);
+        for (int id = 0; id < P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                const int b = o.ztab[id * 3], cnt = o.ztab[id * 3 + 1], k = o.ztab[id * 3 + 2];
+                const size_t at = ((size_t)b * nv + lev) * L + k;
+                double u, v;
+                uv(id, lev, u, v);
+                ut[at] = u / cnt;
+                vt[at] = v / cnt;
+                wt[at] = (W[id * nv + lev] / Rho[id * nv + lev]) / cnt;
+            }
+        for (int b = 0; b < nlat; b++)
+            for (int lev = 0; lev < nv; lev++) {
+                const size_t row = ((size_t)b * nv + lev) * L, v3 = ((size_t)b * nv + lev) * 3;
+                if (dmu) {
+                    vbar[v3 + 0] = oracle_tree_sum(&ut[row], L);
+                    vbar[v3 + 1] = oracle_tree_sum(&vt[row], L);
+                }
+                if (dmw) vbar[v3 + 2] = oracle_tree_sum(&wt[row], L);
+            }
+    }
+    const double ztop = Alth[nv], des_lat = M_PI / nlat;
+    for (int id = 0; id < P; id++) {
+        const int intlat = o.ztab[id * 3];
+        const double lat1 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat - 1);
+        const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
+        const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
+        const double lat = ll[id * 2 + 1], lon = lon_of(id);
+        // zonal mean of component c at this latitude (interpolated between bin centres)
+        auto mean_at = [&](int lev, int c) {
+            const double *vb = vbar.data();
+            if (lat <= -M_PI / 2 + 0.5 * des_lat) return vb[((size_t)intlat * nv + lev) * 3 + c];
+            if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vb[((size_t)intlat * nv + lev) * 3 + c];
+            if (lat >= lat1 && lat < lat2) {
+                const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
+                return vb[((size_t)(intlat - 1) * nv + lev) * 3 + c] * intt + vb[((size_t)intlat * nv + lev) * 3 + c] * intl;
+            }
+            const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
+            return vb[((size_t)intlat * nv + lev) * 3 + c] * intt + vb[((size_t)(intlat + 1) * nv + lev) * 3 + c] * intl;
+        };
+        for (int lev = 0; lev < nv; lev++) {
+            const double rho = Rho[id * nv + lev];
+            double u, v;
+            uv(id, lev, u, v);
+            const double w = W[id * nv + lev] / rho;
+            const double n = Alt[lev] / ztop;
+            double kuv = 0.0, kw = 0.0;
+            if (n >= ns) {
+                kuv = 1.0 * Ruv * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0);
+                kw = 1.0 * Rw * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0);
+            }
+            double unew, vnew, wnew;
+            if (dmu) {
+                unew = (u / dt + kuv * mean_at(lev, 0)) / (1.0 / dt + kuv);
+                vnew = (v / dt + kuv * mean_at(lev, 1)) / (1.0 / dt + kuv);
+            } else {
+                unew = (u / dt) / (1.0 / dt + kuv);
+                vnew = (v / dt) / (1.0 / dt + kuv);
+            }
+            if (dmw) wnew = (w / dt + kw * mean_at(lev, 2)) / (1.0 / dt + kw);
+            else wnew = (w / dt) / (1.0 / dt + kw);
+            const double vx = unew * (-MSIN(lon)) + vnew * (-MSIN(lat) * MCOS(lon));
+            const double vy = unew * (MCOS(lon)) + vnew * (-MSIN(lat) * MSIN(lon));
+            const double vz = vnew * (MCOS(lat));
+            Mh[(id * nv + lev) * 3 + 0] = vx * rho;
+            Mh[(id * nv + lev) * 3 + 1] = vy * rho;
+            Mh[(id * nv + lev) * 3 + 2] = vz * rho;
+            W[id * nv + lev] = wnew * rho;
+        }
+        for (int lev = 1; lev < nv; lev++) {
+            const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            Wh[id * (nv + 1) + lev] = W[id * nv + lev - 1] * intt + W[id * nv + lev] * intl;
+        }
+    }
 }
 
 void make_kd(Oracle &o) {
@@ -1663,8 +1816,21 @@ void oracle_steps(oracle *o, int n) {
 }
 void oracle_set_newtonian(oracle *o, const double q[8]) { std::memcpy(o->nrq, q, sizeof(o->nrq)); }
 void oracle_set_rt(oracle *o, const double q[11]) { std::memcpy(o->rtq, q, sizeof(o->rtq)); }
+void oracle_set_sponge(oracle *o, const double q[6]) {
+    o->sponge_on = q != nullptr;
+    if (!q) return;
+    std::memcpy(o->spq, q, sizeof(o->spq));
+    zonal_table(*o, (int)q[3]);
+}
+int oracle_zonal_table(oracle *o, int *dst, long n, int *max_count) {
+    if (n != (long)o->ztab.size()) return -1;
+    std::memcpy(dst, o->ztab.data(), n * sizeof(int));
+    *max_count = o->max_count;
+    return 0;
+}
 int oracle_profx(oracle *o, int forcing) {
     if (forcing == 0) return 0;
+    if (o->sponge_on) sponge(*o);
     if (forcing == 1) profx_hs(*o);
     else if (forcing == 2) profx_nr(*o);
     else if (forcing == 3) profx_rt(*o);

@@ -65,6 +65,11 @@ void oracle_steps(oracle *o, int n);
 void oracle_set_newtonian(oracle *o, const double q[8]);
 void oracle_set_rt(oracle *o, const double q[11]);
 // ProfX with forcing 0..3; returns -1 for an unknown forcing.
+/* Rayleigh sponge in ProfX (implicit mode): q = {ns, Ruv, Rw, nlat_bins,
+ * damp_uv_to_mean, damp_w_to_mean}; NULL switches it off. */
+void oracle_set_sponge(oracle *o, const double q[6]);
+/* The sponge's latitude-bin table [P][3] (bin, points in bin, index in bin). */
+int oracle_zonal_table(oracle *o, int *dst, long n, int *max_count);
 int oracle_profx(oracle *o, int forcing);
 // n x (ProfX with forcing + Thor).
 int oracle_run(oracle *o, int n, int forcing);

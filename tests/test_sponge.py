@@ -1,0 +1,154 @@
+"""The oracle's Rayleigh sponge (THOR's RayleighSponge, implicit mode,
+profx_driver.cu:92-195, profx_sponge.cu:50-437) pinned by closed forms (CPU).
+
+* Latitude bins (grid.cu:3215-3262): every point is in exactly one bin, the
+  indices within a bin run 0..count-1 in point order, bins follow latitude,
+  and the slot count is the largest bin rounded up to a power of two.
+* Damping towards zero: in the local east/north frame the winds become
+  u / (1 + k dt) with k = R sin^2(pi/2 (n - ns) / (1 - ns)) above n = ns and
+  are unchanged below; W likewise, and Wh at interior interfaces is the
+  linear interpolation of the new W.
+* Damping towards the zonal mean with a rate so large that the relaxation is
+  complete: the winds of a point in a polar half-band become the arithmetic
+  mean of its band (numpy's mean over the band's points); a wind field that
+  is the same everywhere is left as it is.
+The Newtonian forcing, which leaves Mh and W alone, drives ProfX.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from common import hj_grid, hj_params, nr_params
+
+
+@pytest.fixture(scope="module")
+def hj(built):
+    capi = built
+    g = hj_grid(capi, 4, 20)
+    p = hj_params(capi, 4)
+    st = dict(capi.ic_rest(g, p, seed=1006, temperature_noise=1.0, wind_noise=5.0))
+    # a vertical wind too: random Wh at the interior interfaces
+    Wh = st["Wh"].reshape(g.P, g.nv + 1).copy()
+    Wh[:, 1:g.nv] = 1e-3 * np.random.default_rng(1).standard_normal((g.P, g.nv - 1))
+    st["Wh"] = Wh.reshape(st["Wh"].shape)
+    return capi, g, p, st
+
+
+def _frame(g):
+    lon, lat = g.lonlat.reshape(g.P, 2).T
+    lon = np.where(lon < 0, lon + 2 * np.pi, lon)
+    e_lon = np.stack([-np.sin(lon), np.cos(lon), np.zeros_like(lon)], axis=1)
+    e_lat = np.stack([-np.sin(lat) * np.cos(lon), -np.sin(lat) * np.sin(lon), np.cos(lat)], axis=1)
+    return lat, e_lon, e_lat
+
+
+def _run(capi, g, p, st, q):
+    from oracle.oracle import Oracle
+    o = Oracle(g, p)
+    o.set_state(st)
+    o.set_newtonian(nr_params(capi))
+    o.set_sponge(q)
+    o.profx_forcing(capi.FORCING_NEWTONIAN)
+    return o
+
+
+def _w_of(g, Wh):
+    """Recompute_W: W at layer centres from Wh."""
+    alt, alth = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    intt = (alt - alth[1:]) / (alth[:-1] - alth[1:])
+    intl = (alt - alth[:-1]) / (alth[1:] - alth[:-1])
+    return Wh[:, :-1] * intt + Wh[:, 1:] * intl
+
+
+def test_zonal_table(hj):
+    capi, g, p, st = hj
+    q = capi.SpongeParams(0.75, 1e-4, 1e-4, 20, 1, 1)
+    o = _run(capi, g, p, st, q)
+    t, mc = o.zonal_table()
+    b, cnt, idx = t.T
+    lat = g.lonlat.reshape(g.P, 2)[:, 1]
+    counts = np.bincount(b, minlength=20)
+    assert counts.sum() == g.P and (cnt == counts[b]).all()
+    for j in range(20):
+        ids = np.nonzero(b == j)[0]
+        assert (idx[ids] == np.arange(len(ids))).all()  # point order within the bin
+    assert (np.diff(b[np.argsort(lat, kind="stable")]) >= 0).all()
+    assert mc == 1 << int(np.ceil(np.log2(counts.max())))
+    assert counts[0] > 0 and counts[-1] > 0  # poles at the ends
+
+
+def test_damping_to_zero_is_implicit_relaxation(hj):
+    capi, g, p, st = hj
+    ns, R, Rw, dt = 0.5, 3e-4, 2e-4, p.timestep
+    o = _run(capi, g, p, st, capi.SpongeParams(ns, R, Rw, 12, 0, 0))
+    P, nv = g.P, g.nv
+    alt, alth = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    rho = st["Rho"].reshape(P, nv)
+    M0 = st["Mh"].reshape(P, nv, 3)
+    W0 = _w_of(g, st["Wh"].reshape(P, nv + 1))
+    lat, e_lon, e_lat = _frame(g)
+    u = np.einsum("pk,plk->pl", e_lon, M0) / rho
+    v = np.einsum("pk,plk->pl", e_lat, M0) / rho
+    n = alt / alth[nv]
+    s2 = np.where(n >= ns, np.sin(0.5 * np.pi * (n - ns) / (1 - ns)) ** 2, 0.0)
+    fu, fw = 1 / (1 + R * s2 * dt), 1 / (1 + Rw * s2 * dt)
+    M = (u * fu)[:, :, None] * e_lon[:, None, :] + (v * fu)[:, :, None] * e_lat[:, None, :]
+    M *= rho[:, :, None]
+    W = W0 * fw
+    got_M = o.get("Mh").reshape(P, nv, 3)
+    got_W = o.get("W").reshape(P, nv)
+    errM = np.abs(got_M - M).max() / np.abs(M0).max()
+    errW = np.abs(got_W - W).max() / np.abs(W0).max()
+    print(f"sponge to zero: Mh {errM:.2e}, W {errW:.2e}; damped fraction at the top {1 - fu[-1]:.3f}")
+    assert errM < 1e-13 and errW < 1e-13
+    assert fu[-1] < 0.99 and (fu[n < ns] == 1).all()
+    # interior Wh interpolates the new W; the boundary interfaces are untouched
+    Wh = o.get("Wh").reshape(P, nv + 1)
+    xi, xim, xip = alth[1:nv], alt[:-1], alt[1:]
+    want = W[:, :-1] * ((xi - xip) / (xim - xip)) + W[:, 1:] * ((xi - xim) / (xip - xim))
+    assert np.abs(Wh[:, 1:nv] - want).max() <= 1e-13 * np.abs(W0).max()
+    np.testing.assert_array_equal(Wh[:, [0, nv]], st["Wh"].reshape(P, nv + 1)[:, [0, nv]])
+
+
+def test_full_damping_to_the_zonal_mean(hj):
+    capi, g, p, st = hj
+    nlat, P, nv = 10, g.P, g.nv
+    o = _run(capi, g, p, st, capi.SpongeParams(0.0, 1e9, 1e9, nlat, 1, 1))
+    t, _ = o.zonal_table()
+    b = t[:, 0]
+    rho = st["Rho"].reshape(P, nv)
+    M0 = st["Mh"].reshape(P, nv, 3)
+    W0 = _w_of(g, st["Wh"].reshape(P, nv + 1))
+    lat, e_lon, e_lat = _frame(g)
+    u = np.einsum("pk,plk->pl", e_lon, M0) / rho
+    w = W0 / rho
+    got_u = np.einsum("pk,plk->pl", e_lon, o.get("Mh").reshape(P, nv, 3)) / rho
+    got_w = o.get("W").reshape(P, nv) / rho
+    des = np.pi / nlat
+    polar = (lat <= -np.pi / 2 + 0.5 * des) | (lat >= -np.pi / 2 + 0.5 * des + des * (nlat - 1))
+    lev = nv - 1  # n = Altitude / Top close to 1: k = 1e9 s^-1, relaxation complete to ~1e-12
+    for j in (0, nlat - 1):
+        ids = np.nonzero(polar & (b == j))[0]
+        assert len(ids) > 0
+        mu, mw = u[b == j, lev].mean(), w[b == j, lev].mean()
+        eu = np.abs(got_u[ids, lev] - mu).max() / np.abs(u[:, lev]).max()
+        ew = np.abs(got_w[ids, lev] - mw).max() / np.abs(w[:, lev]).max()
+        print(f"band {j}: {len(ids)} polar points, u vs band mean {eu:.1e}, w {ew:.1e}")
+        assert eu < 1e-9 and ew < 1e-9
+
+
+def test_uniform_wind_is_left_alone(hj):
+    capi, g, p, st = hj
+    P, nv = g.P, g.nv
+    lat, e_lon, e_lat = _frame(g)
+    rho = st["Rho"].reshape(P, nv)
+    u0, v0 = 30.0, -4.0
+    M = rho[:, :, None] * (u0 * e_lon[:, None, :] + v0 * e_lat[:, None, :])
+    s2 = dict(st)
+    s2["Mh"] = M.reshape(st["Mh"].shape)
+    o = _run(capi, g, p, s2, capi.SpongeParams(0.5, 1e-3, 1e-3, 20, 1, 1))
+    got = o.get("Mh").reshape(P, nv, 3)
+    err = np.abs(got - M).max() / np.abs(M).max()
+    print(f"uniform (u, v) = ({u0}, {v0}) after the sponge: max rel change {err:.2e}")
+    assert err < 1e-12

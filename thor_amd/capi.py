@@ -60,6 +60,12 @@ class RTParams(C.Structure):
                                           "albedo", "incflx", "ps0", "lon_sub")]
 
 
+class SpongeParams(C.Structure):
+    """thor_sponge_params: Rayleigh sponge (THOR's RayleighSponge, implicit mode)."""
+    _fields_ = [("ns", C.c_double), ("Ruv", C.c_double), ("Rw", C.c_double), ("nlat_bins", C.c_int),
+                ("damp_uv_to_mean", C.c_int), ("damp_w_to_mean", C.c_int)]
+
+
 class StepView(C.Structure):
     """thor_step_view: device pointers of the step handed to physics hooks."""
     _fields_ = [("plane", C.c_long), ("point_num", C.c_int), ("nv", C.c_int), ("mhs_fp32", C.c_int)] + \
@@ -162,6 +168,7 @@ def lib(variant: str = "") -> C.CDLL:
             L.thor_dyn_kernel_stat.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_char_p),
                                                C.POINTER(C.c_long), _dp, _dp]
             L.thor_dyn_time_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+            L.thor_dyn_set_sponge.argtypes = [C.c_void_p, C.POINTER(SpongeParams)]
             L.thor_dyn_set_hooks.argtypes = [C.c_void_p, C.POINTER(Hooks)]
             L.thor_dyn_register_rk_array.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
             L.thor_dyn_clear_rk_arrays.argtypes = [C.c_void_p]
@@ -386,6 +393,9 @@ class Dyn:
         ms = C.c_double()
         self._chk(self.L.thor_dyn_time_run(self._h, nsteps, forcing, C.byref(ms)))
         return ms.value
+
+    def set_sponge(self, q: "SpongeParams | None"):
+        self._chk(self.L.thor_dyn_set_sponge(self._h, C.byref(q) if q is not None else None))
 
     def set_hooks(self, init=None, slow_modes=None, fast_modes=None, end=None):
         """Installs Python physics hooks: init(stream, nstep, view), slow_modes /

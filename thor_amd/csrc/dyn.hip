@@ -62,6 +62,11 @@ struct Dyn {
     int lc = 16;         // levels per block of the mixed-precision momentum and divergence stencils
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
+    // Rayleigh sponge
+    bool sponge_on = false;
+    SpongePar spg{};
+    int *ztab = nullptr;                                // [3][S]
+    double *ztmp = nullptr, *zout = nullptr, *vbar = nullptr;
     // physics-module hooks and RK-advanced module arrays
     thor_hooks hooks{};
     RKArrays rka{};
@@ -352,6 +357,26 @@ static int profx(Dyn &d, int forcing) {
     const dim3 blk(kTile), grid_c((d.P + 63) / 64);  // 64 columns x 4 level rows per block
     const double B = 8.0 * d.P * d.nv;
     const double dt = d.prm.timestep;
+    if (d.sponge_on) {
+        const SpongePar sp = d.spg;
+        const dim3 gpl((d.P + kTile - 1) / kTile, d.nv);
+        // Wh 2, Mh 3, rho in; W and the three zonal slots out
+        launch(d, "sponge_zonal", 10 * B, [&] {
+            k_sponge_zonal<<<gpl, blk, 0, d.stream>>>(g, f, sp, d.ztab, d.ztmp);
+        });
+        if (sp.dmu || sp.dmw) {
+            launch(d, "sponge_mean", 8.0 * 3 * sp.nlat * d.nv * sp.max_count, [&] {
+                k_sponge_reduce<<<dim3(sp.nb, d.nv, 3 * sp.nlat), 1024, 0, d.stream>>>(sp, d.nv, d.ztmp, d.zout);
+                k_sponge_vbar<<<(3 * sp.nlat * d.nv + 255) / 256, 256, 0, d.stream>>>(sp, d.nv, d.zout, d.vbar);
+            });
+        }
+        // Mh 3, W, rho in; Mh 3, W out; then W 2 in, Wh out
+        launch(d, "sponge_apply", 12 * B, [&] {
+            k_sponge_apply<<<gpl, blk, 0, d.stream>>>(g, f, sp, d.ztab, d.vbar, dt);
+            k_sponge_wh<<<dim3((d.P + kTile - 1) / kTile, d.nv - 1), blk, 0, d.stream>>>(g, f);
+        });
+        f.skip_w = 1;
+    }
     if (forcing == THOR_FORCING_HELD_SUAREZ) {
         launch(d, "profx_held_suarez", 24 * B, [&] {
             k_profx_relax<THOR_FORCING_HELD_SUAREZ><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr);
@@ -900,6 +925,69 @@ extern "C" thor_status thor_dyn_set_field(thor_dyn *h, const char *name, const d
     if (n != (long)h->P * nlev * ncomp) return fail(THOR_ERR_BAD_ARG, "set_field: size mismatch");
     HIP_TRY(hipStreamSynchronize(h->stream));
     return copy_field(*h, dev, nlev, ncomp, const_cast<double *>(src), true, false);
+}
+
+extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params *q) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "thor_dyn_set_sponge: null handle");
+    if (!q) {
+        h->sponge_on = false;
+        return THOR_OK;
+    }
+    if (q->nlat_bins < 2 || q->nlat_bins > 1000 || !(q->ns >= 0.0 && q->ns < 1.0) || q->Ruv < 0 || q->Rw < 0)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_set_sponge: need 2 <= nlat_bins <= 1000, 0 <= ns < 1, rates >= 0");
+    const int P = h->P, S = h->S, nv = h->nv, nlat = q->nlat_bins;
+    // latitude bins (grid.cu:3215-3262): edges accumulated from -pi/2, index
+    // within a bin in point order, points at or above the last edge in the top bin
+    std::vector<double> lat((size_t)P);
+    HIP_TRY(hipMemcpy(lat.data(), h->g.lonlat + S, P * sizeof(double), hipMemcpyDeviceToHost));
+    const double des_lat = M_PI / nlat;
+    std::vector<double> edge((size_t)nlat + 1);
+    edge[0] = -M_PI / 2.0;
+    for (int j = 1; j < nlat + 1; j++) edge[j] = edge[j - 1] + des_lat;
+    std::vector<int> count((size_t)nlat, 0), tab((size_t)3 * S, 0);
+    for (int i = 0; i < P; i++) {
+        for (int j = 0; j < nlat; j++)
+            if (lat[i] >= edge[j] && lat[i] < edge[j + 1]) {
+                tab[i] = j;
+                tab[2 * (size_t)S + i] = count[j]++;
+                break;
+            }
+        if (lat[i] >= edge[nlat]) {
+            tab[i] = nlat - 1;
+            tab[2 * (size_t)S + i] = count[nlat - 1]++;
+        }
+    }
+    int mc = 0;
+    for (int i = 0; i < P; i++) {
+        tab[(size_t)S + i] = count[tab[i]];
+        mc = std::max(mc, tab[(size_t)S + i]);
+    }
+    const int max_count = (int)std::pow(2, std::ceil(std::log(mc) / std::log(2)));
+    SpongePar sp{q->ns, q->Ruv, q->Rw, nlat, q->damp_uv_to_mean != 0, q->damp_w_to_mean != 0, max_count,
+                 (max_count + 2047) / 2048};
+    // (re)allocate the tables; the zonal slots beyond each bin's count stay zero
+    for (void *ptr : {(void *)h->ztab, (void *)h->ztmp, (void *)h->zout, (void *)h->vbar}) {
+        if (!ptr) continue;
+        auto it = std::find(h->allocs.begin(), h->allocs.end(), ptr);
+        if (it != h->allocs.end()) h->allocs.erase(it);
+        (void)hipFree(ptr);
+    }
+    h->ztab = nullptr;
+    h->ztmp = h->zout = h->vbar = nullptr;
+    h->sponge_on = false;
+    hipError_t e = hipMalloc((void **)&h->ztab, tab.size() * sizeof(int));
+    if (e == hipSuccess) {
+        h->allocs.push_back(h->ztab);
+        e = hipMemcpy(h->ztab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = dalloc(*h, &h->ztmp, (size_t)3 * nlat * nv * max_count);
+    if (e == hipSuccess) e = dalloc(*h, &h->zout, (size_t)3 * nlat * nv * sp.nb);
+    if (e == hipSuccess) e = dalloc(*h, &h->vbar, (size_t)3 * nlat * nv);
+    if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("thor_dyn_set_sponge: ") + hipGetErrorString(e));
+    h->spg = sp;
+    h->sponge_on = true;
+    return THOR_OK;
 }
 
 extern "C" thor_status thor_dyn_set_hooks(thor_dyn *h, const thor_hooks *hooks) {

@@ -68,6 +68,7 @@ struct Fields {
     double *tAA, *tCC, *tTT;              // Thomas factorisation of the stage [nv]
     double *Qheat, *dMh, *dWh;            // ProfX tendencies
     unsigned int *flags;                  // bit 0: NaN pressure, bit 1: NaN temperature
+    int skip_w;                           // ProfX: W already set (by the sponge), do not recompute it from Wh
     // Mixed precision: FP32 storage of the momentum deviation, the advection,
     // hyperdiffusion and divergence-damping tendencies and the slow momentum
     // tendency (same layouts as their FP64 counterparts above).

@@ -2113,7 +2113,7 @@ __global__ __launch_bounds__(256) void k_profx_relax(Geo g, Fields f, double tim
             const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
             const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            LV(f.W, lev) = whl * intt + wht * intl;
+            if (!f.skip_w) LV(f.W, lev) = whl * intt + wht * intl;
         }
         const double pre = LV(f.p, lev), rho = LV(f.Rho, lev);
         double T = pre / (Rd * rho);
@@ -2244,7 +2244,7 @@ __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, doub
             const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
             const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            LV(f.W, lev) = whl * intt + wht * intl;
+            if (!f.skip_w) LV(f.W, lev) = whl * intt + wht * intl;
             const double T = Tl(lev);
             LV(f.T, lev) = T;
             sA[lev * CB + c] = day ? MEXP(-(1.0 / coszrs) * tau_sw(lev)) : 0.0;
@@ -2344,7 +2344,7 @@ __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q,
             const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
             const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            LV(f.W, lev) = whl * intt + wht * intl;
+            if (!f.skip_w) LV(f.W, lev) = whl * intt + wht * intl;
             const double pr = LV(f.p, lev);
             const double T = pr / (g.Rd * LV(f.Rho, lev));
             LV(f.T, lev) = T;
@@ -2404,7 +2404,7 @@ __global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, ml
             const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
             const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            LV(f.W, lev) = whl * intt + wht * intl;
+            if (!f.skip_w) LV(f.W, lev) = whl * intt + wht * intl;
             const double pr = LV(f.p, lev);
             const double T = pr / (g.Rd * LV(f.Rho, lev));
             LV(f.T, lev) = T;
@@ -2540,6 +2540,146 @@ __device__ __forceinline__ void cell_diag(const Geo &g, const Fields &f, long e,
     out[3] = AMy * Vol;
     out[4] = AMz * Vol;
 }
+
+// ===================================================================== sponge
+// THOR's RayleighSponge with raysp_calc_mode = imp (profx_driver.cu:92-195),
+// at the start of ProfX: (1) Recompute_W and each (point, level)'s u, v, w
+// divided by its latitude bin's point count into the bin's slot (zonal_uv /
+// zonal_w, profx_sponge.cu:50-102); (2) per (component, bin, level) the
+// 1024-thread tree sum of gpu_sum_on_device_sponge (reduction_add.h:193-282)
+// over 2048 slots per block, and the block sums added in order; (3) the
+// implicit relaxation of sponge_layer (profx_sponge.cu:138-437) per (point,
+// level); (4) Wh at the interior interfaces from the new W.
+struct SpongePar {
+    double ns, Ruv, Rw;
+    int nlat, dmu, dmw, max_count, nb;  // nb = blocks of 2048 slots per row
+};
+
+__device__ __forceinline__ void sponge_uv(const Geo &g, const Fields &f, int p, int lev, double &u, double &v,
+                                          double &lon, double &lat) {
+    const size_t S = g.S;
+    const int nv = g.nv;
+    lon = g.lonlat[p] < 0 ? g.lonlat[p] + 2.0 * M_PI : g.lonlat[p];
+    lat = g.lonlat[S + p];
+    const double rho = LV(f.Rho, lev);
+    const double m0 = VEC(f.Mh, 0, lev, p), m1 = VEC(f.Mh, 1, lev, p), m2 = VEC(f.Mh, 2, lev, p);
+    u = (m0 * (-MSIN(lon)) + m1 * (MCOS(lon)) + m2 * 0) / rho;
+    v = (m0 * (-MSIN(lat) * MCOS(lon)) + m1 * (-MSIN(lat) * MSIN(lon)) + m2 * (MCOS(lat))) / rho;
+}
+
+// ztab: [3][S] latitude bin, points in the bin, index within the bin;
+// tmp: [3][nlat][nv][max_count] (u, v, w), zero beyond each bin's count.
+__global__ __launch_bounds__(256) void k_sponge_zonal(Geo g, Fields f, SpongePar sp, const int *ztab, double *tmp) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+    const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+    const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+    const double w = whl * intt + wht * intl;
+    LV(f.W, lev) = w;
+    if (!sp.dmu && !sp.dmw) return;
+    double u, v, lon, lat;
+    sponge_uv(g, f, p, lev, u, v, lon, lat);
+    const int cnt = ztab[S + p];
+    const size_t T = (size_t)sp.nlat * nv * sp.max_count;
+    const size_t at = ((size_t)ztab[p] * nv + lev) * sp.max_count + ztab[2 * S + p];
+    tmp[at] = u / cnt;
+    tmp[T + at] = v / cnt;
+    tmp[2 * T + at] = (w / LV(f.Rho, lev)) / cnt;
+}
+
+// grid (nb, nv, 3 nlat): block b of row (component, bin, level); out[row][b].
+__global__ __launch_bounds__(1024) void k_sponge_reduce(SpongePar sp, int nv, const double *tmp, double *out) {
+    __shared__ double ds[2048];
+    const int t = threadIdx.x, lev = blockIdx.y, cb = blockIdx.z;  // cb = component * nlat + bin
+    const long L = sp.max_count;
+    const double *row = tmp + ((size_t)cb * nv + lev) * L;
+    const long o = 2 * ((long)blockDim.x * blockIdx.x + t);
+    ds[2 * t] = o < L ? row[o] : 0.0;
+    ds[2 * t + 1] = o + 1 < L ? row[o + 1] : 0.0;
+    for (int stride = blockDim.x; stride > 0; stride /= 2) {
+        __syncthreads();
+        if (t < stride) ds[t] += ds[t + stride];
+    }
+    __syncthreads();
+    if (t == 0) out[((size_t)cb * nv + lev) * gridDim.x + blockIdx.x] = ds[0];
+}
+
+// vbar[bin][lev][component] = 0 + the block sums in order (the host loop of
+// gpu_sum_on_device_sponge).
+__global__ void k_sponge_vbar(SpongePar sp, int nv, const double *out, double *vbar) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (component, bin, lev)
+    if (i >= 3 * sp.nlat * nv) return;
+    const int comp = i / (sp.nlat * nv), rem = i % (sp.nlat * nv);
+    double s = 0.0;
+    for (int b = 0; b < sp.nb; b++) s += out[(size_t)i * sp.nb + b];
+    vbar[(size_t)rem * 3 + comp] = s;
+}
+
+__global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar sp, const int *ztab,
+                                                     const double *vbar, double dt) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv, nlat = sp.nlat;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const int intlat = ztab[p];
+    const double des_lat = M_PI / nlat;
+    const double lat1 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat - 1);
+    const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
+    const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
+    double u, v, lon, lat;
+    sponge_uv(g, f, p, lev, u, v, lon, lat);
+    auto mean_at = [&](int c) {
+        if (lat <= -M_PI / 2 + 0.5 * des_lat) return vbar[((size_t)intlat * nv + lev) * 3 + c];
+        if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vbar[((size_t)intlat * nv + lev) * 3 + c];
+        if (lat >= lat1 && lat < lat2) {
+            const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
+            return vbar[((size_t)(intlat - 1) * nv + lev) * 3 + c] * intt + vbar[((size_t)intlat * nv + lev) * 3 + c] * intl;
+        }
+        const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
+        return vbar[((size_t)intlat * nv + lev) * 3 + c] * intt + vbar[((size_t)(intlat + 1) * nv + lev) * 3 + c] * intl;
+    };
+    const double rho = LV(f.Rho, lev);
+    const double w = LV(f.W, lev) / rho;
+    const double n = Alt[lev] / Alth[nv];
+    double kuv = 0.0, kw = 0.0;
+    if (n >= sp.ns) {
+        kuv = 1.0 * sp.Ruv * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
+        kw = 1.0 * sp.Rw * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
+    }
+    double unew, vnew, wnew;
+    if (sp.dmu) {
+        unew = (u / dt + kuv * mean_at(0)) / (1.0 / dt + kuv);
+        vnew = (v / dt + kuv * mean_at(1)) / (1.0 / dt + kuv);
+    } else {
+        unew = (u / dt) / (1.0 / dt + kuv);
+        vnew = (v / dt) / (1.0 / dt + kuv);
+    }
+    if (sp.dmw) wnew = (w / dt + kw * mean_at(2)) / (1.0 / dt + kw);
+    else wnew = (w / dt) / (1.0 / dt + kw);
+    const double vx = unew * (-MSIN(lon)) + vnew * (-MSIN(lat) * MCOS(lon));
+    const double vy = unew * (MCOS(lon)) + vnew * (-MSIN(lat) * MSIN(lon));
+    const double vz = vnew * (MCOS(lat));
+    VEC(f.Mh, 0, lev, p) = vx * rho;
+    VEC(f.Mh, 1, lev, p) = vy * rho;
+    VEC(f.Mh, 2, lev, p) = vz * rho;
+    LV(f.W, lev) = wnew * rho;
+}
+
+// Wh at interior interface lev (1..nv-1) from the relaxed W (blockIdx.y = lev - 1).
+__global__ __launch_bounds__(256) void k_sponge_wh(Geo g, Fields f) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y + 1;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const double xi = g.Alth[lev], xim = g.Alt[lev - 1], xip = g.Alt[lev];
+    const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+    LV(f.Wh, lev) = LV(f.W, lev - 1) * intt + LV(f.W, lev) * intl;
+}
+
 
 __global__ __launch_bounds__(1024) void k_globdiag(Geo g, Fields f, double *partials, long n) {
     __shared__ double buf[5][2048];

@@ -43,6 +43,13 @@ def double_gray(**kw) -> "capi.RTParams":
     return capi.RTParams(**q)
 
 
+def hot_jupiter_sponge() -> "capi.SpongeParams":
+    """Rayleigh sponge of THOR's double-gray hot-Jupiter example
+    (ifile/wasp43b_ex.thr): top quarter of the model, 1e-4 1/s for the winds,
+    damping towards the zonal means over 20 latitude bands."""
+    return capi.SpongeParams(0.75, 1e-4, 1e-4, 20, 1, 1)
+
+
 def cos_zenith(grid, lon_sub: float = 0.0) -> np.ndarray:
     """cos z of a synchronous rotator with the substellar point at lon_sub, clipped at 0."""
     lon, lat = np.asarray(grid.lonlat).reshape(grid.P, 2).T
