@@ -2399,7 +2399,10 @@ __global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, ml
         out |= !(xn >= q.in_lo[j] && xn <= q.in_hi[j]);
         bufA[r * ldx + j] = mlp::f2bf(xn);
     };
-    for (int lev = grp; lev < nv; lev += 256 / R) {
+#pragma unroll
+    for (int j = 0; j < 64 / (256 / R); j++) {  // nv <= 64: at most 8 levels per thread
+        const int lev = grp + j * (256 / R);
+        if (lev >= nv) break;
         if (valid) {
             const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
             const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
@@ -2442,7 +2445,10 @@ __global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, ml
     const int o = sout[r];
     if (!o) {
         const double Rd = g.Rd, Cp = g.Cp;
-        for (int lev = grp; lev < nv; lev += 256 / R) {
+#pragma unroll
+        for (int j = 0; j < 64 / (256 / R); j++) {
+            const int lev = grp + j * (256 / R);
+            if (lev >= nv) break;
             double Q = (double)sY[r * 64 + lev] * q.out_std[lev] + q.out_mean[lev];
             const double pr = LV(f.p, lev);
             if (pr + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr / time_step;

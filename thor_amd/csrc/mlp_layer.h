@@ -25,7 +25,7 @@ __device__ __forceinline__ void infer_layer(const uint16_t *sA, int lda, int K, 
     for (int ti = 0; ti < 2; ti++)
 #pragma unroll
         for (int tj = 0; tj < NT; tj++) acc[ti][tj] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
+#pragma unroll 8
     for (int k0 = 0; k0 < K; k0 += 16) {
         v4s a[2], b[NT];
 #pragma unroll
