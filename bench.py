@@ -43,6 +43,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Sim-days/wall-day at g6 L48, mixed+MLP vs FP64 at stated err; HBM GB/s % peak"
 HBM_PEAK_GBS = 8000.0
+# launches whose bytes stay out of the step's algorithmic total (no counterpart in the reference's kernels)
+NOT_REFERENCE = ("thomas_factor", "divm1")
 CACHE = os.path.join(ROOT, ".thor_cache")
 
 
@@ -111,10 +113,11 @@ def roofline(run, workload):
     dom = max(stats, key=lambda s: s["total_ms"])
     dom_avg_ms = dom["total_ms"] / dom["launches"]
     achieved = dom["bytes_per_launch"] / (dom_avg_ms * 1e-3) / 1e9
-    # whole step: algorithmic bytes of every launch except the per-stage Thomas
-    # factorisation (no reference counterpart); in FP64 this is the reference
-    # decomposition's N * 8 B * (470 + 70 n_sub + 24)
-    step_bytes = sum(s["bytes_per_launch"] * s["launches"] for s in stats if s["name"] != "thomas_factor") / nprof
+    # whole step: algorithmic bytes of every launch except those with no
+    # reference counterpart (the per-stage Thomas factorisation, the separate
+    # divergence-damping pass of the chunked slow modes); in FP64 this is the
+    # reference decomposition's N * 8 B * (470 + 70 n_sub + 24)
+    step_bytes = sum(s["bytes_per_launch"] * s["launches"] for s in stats if s["name"] not in NOT_REFERENCE) / nprof
     step_gbs = step_bytes / (run["ms_step"] * 1e-3) / 1e9
     return {
         "bound": "hbm", "kernel": dom["name"], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -243,6 +246,109 @@ def rt_surrogate_run(capi, args):
             "mixed_mlp_vs_fp64_plain": round(out["fp64_exact_rt"]["ms_per_step"] / out["surrogate"]["ms_per_step"], 3)}
 
 
+def _spinup_and_train(capi, args, nv):
+    """Trains the surrogate for nv levels on columns of a hot-Jupiter g5 spin-up
+    with the exact double-gray RT (mixed dynamics, dt 300 s, Rayleigh sponge)."""
+    from thor_amd import surrogate as sg
+    g, p = sg.hot_jupiter(5, nv, dt=300.0, ns=6, precision=1, cache_dir=CACHE)
+    d = capi.Dyn(g, p)
+    d.set_state(capi.ic_rest(g, p, seed=1003, temperature_noise=1.0, wind_noise=0.1))
+    d.set_rt(sg.double_gray())
+    d.set_sponge(sg.hot_jupiter_sponge())
+    t0 = time.perf_counter()
+    X, Y = sg.collect(d, g, nsteps=args.rt_spinup_steps, every=args.rt_sample_every)
+    d.close()
+    log(f"g5 L{nv} spin-up: {args.rt_spinup_steps} steps ({args.rt_spinup_steps * 300.0 / 86400:.1f} days) with "
+        f"exact RT, {len(X)} columns sampled, {time.perf_counter() - t0:.1f} s")
+    norm = sg.normalisation(X, Y)
+    m = capi.MLP(2 * nv + 1, 256, nv, seed=5)
+    t0 = time.perf_counter()
+    losses = sg.train(m, *sg.apply_norm(X, Y, norm), epochs=args.rt_epochs, batch=512, lr=2e-3, log=log)
+    err = sg.heating_error(m, X, Y, norm)
+    log(f"training: {args.rt_epochs} epochs in {time.perf_counter() - t0:.1f} s, heating RMS error {err:.3f}")
+    return m, norm, {"train_columns": int(len(X)), "epochs": args.rt_epochs,
+                     "train_loss_first_last": [losses[0], losses[-1]], "heating_rel_rms_err_trained": round(err, 4),
+                     "rms_heating_W_m3": float(np.sqrt((Y ** 2).mean()))}
+
+
+def at_scale_run(capi, args):
+    """Hot Jupiter at glevel 7 (163 842 columns), 48 levels, dt 120 s, 6 acoustic
+    substeps, Rayleigh sponge, seeded +-0.1 m/s wind noise: mixed-precision
+    dynamics with the MLP surrogate (exact RT for out-of-envelope columns, judge
+    every 10 calls) against the FP64 form with the exact RT, from the same state
+    over the same simulated days. The network is trained on a g5 L48 exact-RT
+    spin-up: its inputs (T, ln p, cos z per column) and outputs do not depend on
+    the horizontal grid. Reports SDPD, the step's algorithmic HBM rate and the
+    differences between the two runs at the end."""
+    from thor_amd import surrogate as sg
+    nv, gl, dt = 48, args.scale_glevel, 120.0
+    m, norm, train = _spinup_and_train(capi, args, nv)
+    t0 = time.perf_counter()
+    g, p64 = sg.hot_jupiter(gl, nv, dt=dt, ns=6, precision=0, cache_dir=CACHE)
+    pmx = sg.hot_jupiter_params(gl, dt=dt, ns=6, precision=1)
+    log(f"grid g{gl} L{nv}: P={g.P} in {time.perf_counter() - t0:.1f} s")
+    st = capi.ic_rest(g, p64, seed=1010, temperature_noise=0.0, wind_noise=0.1)
+    nsteps = int(round(args.scale_days * 86400.0 / dt))
+    forms = {"fp64_exact_rt": (p64, capi.FORCING_DOUBLE_GRAY), "mixed_surrogate": (pmx, capi.FORCING_RT_SURROGATE)}
+    out, final = {}, {}
+    for name, (prm, forcing) in forms.items():
+        d = capi.Dyn(g, prm)
+        d.set_state(st)
+        d.set_rt(sg.double_gray())
+        d.set_sponge(sg.hot_jupiter_sponge())
+        if forcing == capi.FORCING_RT_SURROGATE:
+            d.set_rt_surrogate(m, norm, judge_every=10)
+        chunk, done, wall = 720, 0, 0.0
+        while done < nsteps:
+            n = min(chunk, nsteps - done)
+            t1 = time.perf_counter()
+            d.run(n, forcing)
+            d.sync()
+            wall += time.perf_counter() - t1
+            done += n
+            log(f"{name}: {done}/{nsteps} steps ({done * dt / 86400:.1f} days), {wall:.1f} s")
+        final[name] = d.get_state()
+        sstats = d.surrogate_stats() if forcing == capi.FORCING_RT_SURROGATE else None
+        d.kernel_timing(True)
+        d.run(10, forcing)
+        d.sync()
+        ks = d.kernel_stats()
+        d.kernel_timing(False)
+        d.close()
+        step_bytes = sum(k["bytes_per_launch"] * k["launches"] for k in ks if k["name"] not in NOT_REFERENCE) / 10
+        ms = 1e3 * wall / nsteps
+        gbs = step_bytes / (ms * 1e-3) / 1e9
+        tot = sum(k["total_ms"] for k in ks)
+        top = sorted(ks, key=lambda k: -k["total_ms"])[:6]
+        out[name] = {"ms_per_step": round(ms, 3), "sdpd": round(dt * nsteps / wall, 1),
+                     "step_alg_bytes": round(step_bytes), "hbm_gbs_alg": round(gbs, 1),
+                     "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "top_kernels_us": {k["name"]: round(1e3 * k["total_ms"] / 10, 1) for k in top},
+                     "kernel_sum_ms_per_step": round(tot / 10, 3)}
+        if sstats is not None:
+            out[name]["judge"] = {"calls": sstats["calls"], "judged": sstats["judged"],
+                                  "fallback_columns_per_call": round(sstats["fallback_columns"] / max(1, sstats["calls"]), 1),
+                                  "rms_err_rel_to_rms_heating": round(sstats["rms_err"] / train["rms_heating_W_m3"], 4)}
+        log(f"{name}: {out[name]}")
+    a = diagnostics(g, p64, final["fp64_exact_rt"])
+    b = diagnostics(g, p64, final["mixed_surrogate"])
+    err = {}
+    for k in ("T", "u", "v", "w"):
+        diff = b[k] - a[k]
+        err[k] = {"rms": float(f"{np.sqrt((diff ** 2).mean()):.4e}"), "max": float(f"{np.abs(diff).max():.4e}"),
+                  "fp64_rms": float(f"{np.sqrt((a[k] ** 2).mean()):.4e}")}
+    err["units"] = {"T": "K", "u": "m/s", "v": "m/s", "w": "m/s"}
+    m.close()
+    return {"metric": "SDPD and algorithmic HBM rate, hot Jupiter g7 L48: mixed-precision dynamics + MLP RT "
+                      "surrogate vs FP64 dynamics + exact double-gray RT",
+            "workload": f"hot-Jupiter g{gl} L{nv} ({g.P} columns), dt {dt:g} s, 6 substeps, double-gray RT, Rayleigh "
+                        f"sponge (top quarter, 1e-4 1/s, 20 bands), +-0.1 m/s wind noise, {args.scale_days:g} days",
+            "steps": nsteps, "surrogate_training": {"spinup": f"g5 L{nv}, {args.rt_spinup_steps} steps of 300 s",
+                                                    **train},
+            **out, "speedup": round(out["fp64_exact_rt"]["ms_per_step"] / out["mixed_surrogate"]["ms_per_step"], 3),
+            "diff_mixed_surrogate_vs_fp64_exact": err}
+
+
 def error_run(capi, g, p64, pmx, st, args):
     nsteps = int(round(args.error_days * 86400.0 / args.dt))
     ds = [capi.Dyn(g, q, variant=args.variant) for q in (p64, pmx)]
@@ -300,6 +406,11 @@ def main():
     ap.add_argument("--rt-sample-every", type=int, default=720, help="sample all columns every this many steps")
     ap.add_argument("--rt-epochs", type=int, default=40)
     ap.add_argument("--rt-steps", type=int, default=100, help="timed steps per form")
+    ap.add_argument("--at-scale", action="store_true",
+                    help="instead of the headline: hot Jupiter g7 L48, dt 120 s, mixed dynamics + MLP surrogate vs "
+                         "FP64 + exact RT over --scale-days (surrogate trained on a g5 L48 spin-up)")
+    ap.add_argument("--scale-days", type=float, default=5.0)
+    ap.add_argument("--scale-glevel", type=int, default=7)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (rhombus-sharded) "
@@ -311,6 +422,10 @@ def main():
     if args.rt_surrogate:
         os.makedirs(CACHE, exist_ok=True)
         print(json.dumps(rt_surrogate_run(capi, args)), flush=True)
+        return
+    if args.at_scale:
+        os.makedirs(CACHE, exist_ok=True)
+        print(json.dumps(at_scale_run(capi, args)), flush=True)
         return
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)

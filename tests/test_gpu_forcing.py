@@ -151,3 +151,32 @@ def test_sponge_steps_bitwise(hj, forcing):
     assert _max_rel(d2.get_state()["Mh"], gs["Mh"]) > 1e-6
     d.close()
     d2.close()
+
+
+def test_sponge_many_slot_blocks_bitwise(built):
+    """Two latitude bands on g5 (about 5 100 points each, 8 192 slots): the
+    zonal sums run over four 2 048-slot blocks per row, added in order. One
+    Newtonian ProfX + Thor step of the parity build equals the oracle."""
+    from oracle.oracle import Oracle
+    capi = built
+    g = hj_grid(capi, 5, 20)
+    p = hj_params(capi, 5)
+    print("case: parity build, hot Jupiter g5 L20 with winds, sponge over 2 bands, 1 step", flush=True)
+    st = capi.ic_rest(g, p, seed=1009, temperature_noise=1.0, wind_noise=20.0)
+    q = capi.SpongeParams(0.5, 3e-4, 2e-4, 2, 1, 1)
+    d = capi.Dyn(g, p, variant="pm")
+    o = Oracle(g, p, variant="pm")
+    for x in (d, o):
+        x.set_state(st)
+        x.set_newtonian(nr_params(capi))
+        x.set_sponge(q)
+    _, mc = o.zonal_table()
+    assert mc == 8192
+    d.run(1, capi.FORCING_NEWTONIAN)
+    d.sync()
+    o.run(1, capi.FORCING_NEWTONIAN)
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("sponge, 2 bands", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()

@@ -145,15 +145,17 @@ def test_surrogate_in_the_hot_jupiter_step(built):
     m.close()
 
 
-def test_surrogate_profx_is_the_network_on_column_features(built):
+@pytest.mark.parametrize("nv", [20, 48])
+def test_surrogate_profx_is_the_network_on_column_features(built, nv):
     """One surrogate ProfX (no judge) against the network evaluated by
     thor_mlp_forward on features formed on the host from the same state: the
     fused kernel's features, envelope test, inference and heating must agree
-    (up to one ulp of ln p in the features)."""
+    (up to one ulp of ln p in the features). L48 has 97 inputs (padded K)
+    and 48 outputs (three 16-wide output tiles)."""
     from thor_amd import surrogate as sg
     capi = built
-    print("case: hot Jupiter g4 L20, surrogate ProfX vs host features + thor_mlp_forward", flush=True)
-    g = hj_grid(capi, 4, 20)
+    print(f"case: hot Jupiter g4 L{nv}, surrogate ProfX vs host features + thor_mlp_forward", flush=True)
+    g = hj_grid(capi, 4, nv)
     p = hj_params(capi, 4)
     st = capi.ic_rest(g, p, seed=1005, temperature_noise=1.0, wind_noise=0.1)
     d = capi.Dyn(g, p)
@@ -161,7 +163,6 @@ def test_surrogate_profx_is_the_network_on_column_features(built):
     d.set_rt(rt_params(capi))
     X, Y = sg.collect(d, g, nsteps=6, every=2)
     norm = sg.normalisation(X, Y)
-    nv = g.nv
     m = capi.MLP(2 * nv + 1, 256, nv, seed=8)
     sg.train(m, *sg.apply_norm(X, Y, norm), epochs=3, batch=256, lr=2e-3)
     d.set_rt_surrogate(m, norm, judge_every=0)

@@ -62,6 +62,7 @@ struct Dyn {
     int lc = 16;         // levels per block of the mixed-precision momentum and divergence stencils
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
+    bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
     // Rayleigh sponge
     bool sponge_on = false;
     SpongePar spg{};
@@ -267,8 +268,16 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             if (d.hooks.dyn_core_loop_slow_modes(d.hooks.user, (void *)st, nstep, times, &v))
                 return hook_fail("dyn_core_loop_slow_modes");
         }
-        launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
-               [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
+        if (MX && d.slow_lc) {
+            // DivM into the idle hyperdiffusion scratch (its first three planes)
+            if (divdamp)
+                launch(d, "divm1", W(0, 6), [&] { k_divm1_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, lc); });
+            launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
+                   [&] { k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc); });
+        } else {
+            launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
+                   [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
+        }
         if (rk > 0) rk_arrays(0);
         auto divm0 = [&] {
             launch(d, "divm0", W(1, 6), [&] {
@@ -610,6 +619,14 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     for (int r = 0; r < gd->nr; r++)
         for (int t = 0; t < NHALO; t++) halo[(size_t)r * NHALO + t] = gd->maps[(size_t)r * NHL2 + halo_pos(t)];
     if (gd->nr * kTile + 2 != P) return cleanup_fail(fail(THOR_ERR_BAD_ARG, "thor_dyn_create: P != 256 nr + 2"));
+    // The level-chunked slow-mode kernel uses the gradient weights for the
+    // neighbours of the divergence too; THOR's operators share them, checked
+    // here bit for bit.
+    d.slow_lc = d.mixed;
+    for (int p = 0; p < P && d.slow_lc; p++)
+        if (std::memcmp(gd->div + (size_t)p * 21 + 3, gd->grad + (size_t)p * 21 + 3, 18 * sizeof(double)) != 0)
+            d.slow_lc = false;
+    if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) d.slow_lc = d.slow_lc && std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
               && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)

@@ -1609,6 +1609,386 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
     }
 }
 
+// Divergence-damping pass 1 of the slow part (the DivM term of
+// Compute_Slow_Modes, thor_slowmodes.h:82-105) in FP32 over level chunks, with
+// the operation order of divm_w<true>, written to dm [3][nv] for k_slow_lc.
+// Tiles only: the pole block of k_slow_lc forms its own DivM.
+__global__ __launch_bounds__(256, 3) void k_divm1_lc(Geo g, Fields f, float *dm, int lc) {
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1) || r == g.nr) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    __shared__ float sG[2][3][NHL2], sL[4][kLcMax];
+    stage_lvc(g, l0, l1, sL);
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
+    float wd[63];
+#pragma unroll
+    for (int i = 0; i < 63; i++) wd[i] = g.wdd[(size_t)i * S + p];
+    const float fx = (float)g.fr[p], fy = (float)g.fr[S + p], fz = (float)g.fr[2 * S + p];
+    const size_t pl4 = S * 4;
+    const auto rG = rsrc(f.mdivg, 3 * nv * pl4), rD = rsrc(dm, 3 * nv * pl4);
+    struct In {
+        float og[3], hg[3];
+    };
+    auto load = [&](In &v, int lev) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.og[k] = ldb_f(rG, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.hg[k] = ldb_f(rG, T.h4, (unsigned)((k * nv + lev) * pl4));
+        }
+    };
+    auto put = [&](const In &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) sG[b][k][T.ir] = v.og[k];
+        if (T.hp >= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sG[b][k][T.hp] = v.hg[k];
+        }
+    };
+    auto body = [&](int lev, In &R, In &L) {
+        load(L, min(lev + 2, l1 - 1));
+        const int b = (lev - l0) & 1;
+        const float rs2 = sL[1][lev - l0], sdiff = sL[3][lev - l0];
+        float av[7][3];
+#pragma unroll
+        for (int s = 0; s < 7; s++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) av[s][k] = sG[b][k][T.pos[s]];
+        float o[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            float sum = 0.f;
+#pragma unroll
+            for (int s = 0; s < 7; s++)
+#pragma unroll
+                for (int k = 0; k < 3; k++) sum += wd[c * 21 + s * 3 + k] * av[s][k];
+            o[c] = rs2 * sum;
+        }
+        const float lapr = fx * o[0] + fy * o[1] + fz * o[2];
+        stb_f(rD, T.o4, (unsigned)((0 * nv + lev) * pl4), -(o[0] - fx * lapr) * sdiff);
+        stb_f(rD, T.o4, (unsigned)((1 * nv + lev) * pl4), -(o[1] - fy * lapr) * sdiff);
+        stb_f(rD, T.o4, (unsigned)((2 * nv + lev) * pl4), -(o[2] - fz * lapr) * sdiff);
+        if (lev + 1 < l1) put(R, b ^ 1);
+        __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
+    }
+}
+
+// Compute_Slow_Modes (thor_slowmodes.h:51-313) of the mixed-precision path
+// over level chunks, with slow_point's arithmetic and operation order, so its
+// results equal k_slow<true>'s bit for bit. The FP64 gradient weights of the
+// point stay in registers for the chunk and also serve the divergence: THOR's
+// div and grad operators share their neighbour weights (grid.cu:2656-2831,
+// 3015-3160 differ only in the centre weight; the host checks this before
+// choosing this kernel). DivM comes from k_divm1_lc. Each level carries its
+// lower neighbour's values (advection, density, pressure, momentum, diffw) to
+// the next, and stages the stage momentum, pressure and h of the tile.
+__global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float *dm, int divdamp, int rk, int lc) {
+    constexpr bool MX = true;
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev)) {
+            const float *divg = f.mdivg;
+            slow_point<true>(
+                g, f, c.p, lev, 5, true, divdamp, rk, [&](int s, int k) { return VEC(divg, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.pk, lev, c.gi(s)); }, [&](int s) { return LVP(f.h, lev, c.gi(s)); },
+                [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); });
+        }
+        return;
+    }
+    __shared__ double sM[2][3][NHL2], sP[2][NHL2], sH[2][NHL2];
+    __shared__ double sW[24][kTile];  // the thread's 21 gradient weights and 3 divergence centre weights
+    // Per-level constants of the chunk, formed once with slow_point's
+    // arithmetic: Alt[l0 - 1 .. l1 - 1], Alth[l0 .. l1], (A / (A + Alt))^2 at
+    // the same levels, and per level r2p, r2m, r2l, rscale and dr2dz.
+    __shared__ double sAlt[kLcMax + 1], sAlth[kLcMax + 1], sG2[kLcMax + 1], sR[5][kLcMax];
+    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cv = g.Cp - g.Rd;
+    {
+        const int t = threadIdx.x, n = l1 - l0;
+        if (t <= n) {
+            const double alt = l0 - 1 + t >= 0 ? g.Alt[l0 - 1 + t] : 0.0;
+            sAlt[t] = alt;
+            sAlth[t] = g.Alth[l0 + t];
+            sG2[t] = g.ghv ? XPOW(A / (A + alt), 2) : 0.0;
+        }
+        if (t < n) {
+            const int lev = l0 + t;
+            const double altht = g.Alth[lev + 1], althl = g.Alth[lev], alt = g.Alt[lev];
+            if (g.deep) {
+                sR[0][t] = XPOW(altht + A, 2.0);
+                sR[1][t] = XPOW(alt + A, 2.0);
+                sR[2][t] = XPOW(althl + A, 2.0);
+                sR[3][t] = A / (alt + A);
+                sR[4][t] = (XPOW(altht + A, 3.0) - XPOW(althl + A, 3.0)) / 3;
+            } else {
+                sR[0][t] = sR[1][t] = sR[2][t] = 1.0;
+                sR[3][t] = 1.0;
+                sR[4][t] = 1.0 * (altht - althl);
+            }
+        }
+    }
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < 21; i++) sW[i][t] = g.grad[(size_t)i * S + p];
+#pragma unroll
+        for (int k = 0; k < 3; k++) sW[21 + k][t] = g.div[(size_t)k * S + p];
+    }
+    auto wg = [&](int i) { return sW[i][threadIdx.x]; };
+    auto wd0 = [&](int k) { return sW[21 + k][threadIdx.x]; };
+    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+    const size_t pl4 = S * 4, pl8 = S * 8;
+    const auto rMk = rsrc(f.Mhk, 3 * nv * pl8), rPk = rsrc(f.pk, nv * pl8), rH = rsrc(f.h, nv * pl8),
+               rAdv = rsrc(f.mAdv, 3 * nv * pl4), rRk = rsrc(f.Rhok, nv * pl8),
+               rWk = rsrc(f.Whk, (nv + 1) * pl8), rHh = rsrc(f.hh, (nv + 1) * pl8),
+               rDmh = rsrc(f.mdiffmh, 3 * nv * pl4), rdM = rsrc(f.dMh, 3 * nv * pl8),
+               rDw = rsrc(f.mdiffw, nv * pl4), rdW = rsrc(f.dWh, (nv + 1) * pl8),
+               rDr = rsrc(f.mdiffrh, nv * pl4), rDp = rsrc(f.mdiffpr, nv * pl4), rGt = rsrc(f.gtil, nv * pl8),
+               rQ = rsrc(f.Qheat, nv * pl8), rDM = rsrc(dm, 3 * nv * pl4), rM = rsrc(f.Mh, 3 * nv * pl8),
+               rR = rsrc(f.Rho, nv * pl8), rPr = rsrc(f.p, nv * pl8), rW = rsrc(f.Wh, (nv + 1) * pl8);
+    struct Stg {  // staged tile values: own point and halo point, two levels ahead
+        double om[3], op, oh, hm[3], hp, hh;
+    };
+    struct Pt {  // the point's own values, one level ahead
+        float adv[3], dmh[3], dM[3], dw, drh, dpr;
+        double rhok, whk1, hh1, dmv[3], dwh, gtil, q;
+        double mh[3], rho, pr, wh;  // state, for the deviations of rk > 0
+    };
+    auto load_stg = [&](Stg &v, int lev) {
+        const unsigned l8 = (unsigned)(lev * pl8);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.om[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.hm[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
+        }
+        v.op = ldb_d(rPk, T.o8, l8);
+        v.hp = ldb_d(rPk, T.h8, l8);
+        v.oh = ldb_d(rH, T.o8, l8);
+        v.hh = ldb_d(rH, T.h8, l8);
+    };
+    auto load_pt = [&](Pt &v, int lev) {
+        const unsigned l8 = (unsigned)(lev * pl8), l4 = (unsigned)(lev * pl4), u8 = (unsigned)((lev + 1) * pl8);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.adv[k] = ldb_f(rAdv, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.dmh[k] = ldb_f(rDmh, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.dM[k] = divdamp ? ldb_f(rDM, T.o4, (unsigned)((k * nv + lev) * pl4)) : 0.f;
+            v.dmv[k] = ldb_d(rdM, T.o8, (unsigned)((k * nv + lev) * pl8));
+        }
+        v.dw = ldb_f(rDw, T.o4, l4);
+        v.drh = ldb_f(rDr, T.o4, l4);
+        v.dpr = ldb_f(rDp, T.o4, l4);
+        v.rhok = ldb_d(rRk, T.o8, l8);
+        v.whk1 = ldb_d(rWk, T.o8, u8);
+        v.hh1 = ldb_d(rHh, T.o8, u8);
+        v.dwh = ldb_d(rdW, T.o8, l8);
+        v.gtil = ldb_d(rGt, T.o8, l8);
+        v.q = ldb_d(rQ, T.o8, l8);
+        if (rk) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) v.mh[k] = ldb_d(rM, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.rho = ldb_d(rR, T.o8, l8);
+            v.pr = ldb_d(rPr, T.o8, l8);
+            v.wh = ldb_d(rW, T.o8, l8);
+        }
+    };
+    auto put = [&](const Stg &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) sM[b][k][T.ir] = v.om[k];
+        sP[b][T.ir] = v.op;
+        sH[b][T.ir] = v.oh;
+        if (T.hp >= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sM[b][k][T.hp] = v.hm[k];
+            sP[b][T.hp] = v.hp;
+            sH[b][T.hp] = v.hh;
+        }
+    };
+    // values of level lev - 1 carried to level lev
+    float cadv[3] = {0.f, 0.f, 0.f}, cdw = 0.f;
+    double crho = 0.0, cpk = 0.0, cm[3] = {0.0, 0.0, 0.0}, cwh, chh;
+    if (l0 > 0) {
+        const int lm = l0 - 1;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            cadv[k] = ldb_f(rAdv, T.o4, (unsigned)((k * nv + lm) * pl4));
+            cm[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lm) * pl8));
+        }
+        cdw = ldb_f(rDw, T.o4, (unsigned)(lm * pl4));
+        crho = ldb_d(rRk, T.o8, (unsigned)(lm * pl8));
+        cpk = ldb_d(rPk, T.o8, (unsigned)(lm * pl8));
+    }
+    cwh = ldb_d(rWk, T.o8, (unsigned)(l0 * pl8));  // Whk and hh at the chunk's lowest interface
+    chh = ldb_d(rHh, T.o8, (unsigned)(l0 * pl8));
+    auto *SlowMh = f.mSlowMh;
+    float *Mhs = f.mMhs;
+    const auto rSm = rsrc(SlowMh, 3 * nv * pl4);
+    auto body = [&](int lev, Stg &R, Stg &L, const Pt &v, Pt &nxt) {
+        load_pt(nxt, min(lev + 1, l1 - 1));
+        load_stg(L, min(lev + 2, l1 - 1));
+        const int b = (lev - l0) & 1, i = lev - l0;
+        double advx = v.adv[0], advy = v.adv[1], advz = v.adv[2];
+        const double rho = v.rhok;
+        double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
+        if (lev != 0) {
+            const double ax = cadv[0], ay = cadv[1], az = cadv[2];
+            advrl = ax * fx + ay * fy + az * fz;
+            rhol = crho;
+            altl = sAlt[i];
+            pressurel = cpk;
+        }
+        double advr = advx * fx + advy * fy + advz * fz;
+        advx += -advr * fx;
+        advy += -advr * fy;
+        advz += -advr * fz;
+        const double altht = sAlth[i + 1], althl = sAlth[i], alt = sAlt[i + 1];
+        const double M0x = sM[b][0][T.pos[0]], M0y = sM[b][1][T.pos[0]], M0z = sM[b][2][T.pos[0]];
+        if (!g.nonhydro) {
+            advr = (XPOW(M0x, 2) + XPOW(M0y, 2) + XPOW(M0z, 2)) / (rho * (alt + A));
+            if (lev != 0) advrl = (XPOW(cm[0], 2) + XPOW(cm[1], 2) + XPOW(cm[2], 2)) / (rhol * (altl + A));
+        }
+        const double dz = altht - althl;
+        const double r2p = sR[0][i], r2m = sR[1][i], r2l = sR[2][i], rscale = sR[3][i], dr2dz = sR[4][i];
+        double pv[7], hv[7];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            pv[s] = sP[b][T.pos[s]];
+            hv[s] = sH[b][T.pos[s]];
+        }
+        double nflxv[3], nflxr = 0.0, nflxp = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double m0 = sM[b][k][T.pos[0]];
+            double sg = wg(k) * pv[0];
+            double sr = wd0(k) * m0;
+            double sp = wd0(k) * m0 * hv[0];
+#pragma unroll
+            for (int s = 1; s < 7; s++) {
+                const double w = wg(s * 3 + k), m = sM[b][k][T.pos[s]];
+                sg = sg + w * pv[s];
+                sr = sr + w * m;
+                sp = sp + w * m * hv[s];
+            }
+            nflxv[k] = rscale * sg;
+            nflxr += rscale * sr;
+            nflxp += rscale * sp;
+        }
+        double whl, wht, hhl, hht;
+        if (lev == 0) {
+            whl = 0.0;
+            wht = v.whk1;
+            hhl = 0.0;
+            hht = v.hh1;
+        } else {
+            whl = cwh;
+            wht = v.whk1;
+            hhl = chh;
+            hht = v.hh1;
+        }
+        const double dwdz = (wht * r2p - whl * r2l) / (dr2dz);
+        const double dwhdz = (wht * r2p * hht - whl * r2l * hhl) / (dz * r2m);
+        const double dpr = nflxv[0] * fx + nflxv[1] * fy + nflxv[2] * fz;
+        const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double DivM = v.dM[k];
+            const float o = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM + v.dmh[k] + 0.0 + v.dmv[k];
+            stb_f(rSm, T.o4, (unsigned)((k * nv + lev) * pl4), o);
+        }
+        if (lev == 0) {
+            LV(f.SlowWh, 0) = 0.0;
+            LV(f.SlowWh, nv) = 0.0;
+        } else {
+            const double xi = althl, xim = altl, xip = alt;
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            double swr;
+            if (g.deep) {
+                if (g.ghv)
+                    swr = (advrl + rhol * Gravit * sG2[i]) * intt + (advr + rho * Gravit * sG2[i + 1]) * intl;
+                else
+                    swr = (advrl + rhol * Gravit) * intt + (advr + rho * Gravit) * intl;
+            } else {
+                if (g.ghv)
+                    swr = (rhol * Gravit * sG2[i]) * intt + (rho * Gravit * sG2[i + 1]) * intl;
+                else
+                    swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
+            }
+            if (g.nonhydro) swr += (-cdw - 0.0) * intt + (-v.dw - 0.0) * intl;
+            const double dzl = alt - altl;
+            const double dpdz = (pv[0] - pressurel) / dzl;
+            swr += dpdz;
+            swr -= v.dwh;
+            LV(f.SlowWh, lev) = -swr;
+        }
+        nflxr += dwdz;
+        LV(f.SlowRho, lev) = -nflxr + v.drh + 0.0;
+        const double rr = 1.0 / rho;
+        const double vgp = (nflxv[0] - dpr * fx) * M0x * rr + (nflxv[1] - dpr * fy) * M0y * rr
+                           + (nflxv[2] - dpr * fz) * M0z * rr;
+        double sp = (Rd / Cv) * (-nflxp - v.gtil - dwhdz + vgp + v.q);
+        sp += v.dpr + 0.0;
+        LV(f.Slowp, lev) = sp;
+        if (rk == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(Mhs, k, lev, p) = 0.0;
+            LV(f.Rhos, lev) = 0.0;
+            LV(f.ps, lev) = 0.0;
+            LV(f.Whs, lev) = 0.0;
+            if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+        } else {
+            const double Mx = v.mh[0] - M0x, My = v.mh[1] - M0y, Mz = v.mh[2] - M0z;
+            const double aux = Mx * fx + My * fy + Mz * fz;
+            VEC(Mhs, 0, lev, p) = Mx - aux * fx;
+            VEC(Mhs, 1, lev, p) = My - aux * fy;
+            VEC(Mhs, 2, lev, p) = Mz - aux * fz;
+            LV(f.Rhos, lev) = v.rho - rho;
+            LV(f.ps, lev) = v.pr - pv[0];
+            LV(f.Whs, lev) = lev == 0 ? 0.0 : v.wh - cwh;
+            if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+        }
+        // carry this level's values to the next
+#pragma unroll
+        for (int k = 0; k < 3; k++) cadv[k] = v.adv[k];
+        cm[0] = M0x;
+        cm[1] = M0y;
+        cm[2] = M0z;
+        cdw = v.dw;
+        crho = rho;
+        cpk = pv[0];
+        cwh = v.whk1;
+        chh = v.hh1;
+        if (lev + 1 < l1) put(R, b ^ 1);
+        __syncthreads();
+    };
+    Stg S0, S1;
+    Pt P0, P1;
+    load_pt(P0, l0);
+    load_stg(S0, l0);
+    load_stg(S1, min(l0 + 1, l1 - 1));
+    put(S0, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, S1, S0, P0, P1);
+        if (lev + 1 < l1) body(lev + 1, S0, S1, P1, P0);
+    }
+}
+
 // Compute_Advec_Cori1 (thor_adv_cor.h:51-235) and the first Diffusion_Op pass
 // (thor_diff.h:54-522) of the mixed-precision path, in FP32 over level
 // chunks. Each staged point carries its momentum, the advecting velocity

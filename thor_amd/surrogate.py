@@ -27,10 +27,15 @@ def hot_jupiter(glevel: int, nv: int, dt: float = 300.0, ns: int = 6, precision:
     damping as THOR's deep hot-Jupiter benchmark (ifile/deephj.thr: Diffc 0.02,
     DivDampc 0.03); kw overrides any parameter."""
     g = capi.Grid(glevel, nv, 1e8, HJ_TOP, spring=True, beta=1.15, cache_dir=cache_dir)
+    return g, hot_jupiter_params(glevel, dt, ns, precision, **kw)
+
+
+def hot_jupiter_params(glevel: int, dt: float = 300.0, ns: int = 6, precision: int = 0, **kw):
+    """The parameters of hot_jupiter() alone."""
     q = dict(glevel=glevel, timestep=dt, n_substeps=ns, precision=precision, A=1e8, Omega=2.06e-5, Gravit=9.42,
              P_Ref=1e7, Rd=3779.0, Cp=13226.0, Tmean=1600.0, Diffc=0.02, DivDampc=0.03)
     q.update(kw)
-    return g, capi.default_params(**q)
+    return capi.default_params(**q)
 
 
 def double_gray(**kw) -> "capi.RTParams":
