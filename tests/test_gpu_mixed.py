@@ -134,7 +134,8 @@ def test_chunked_slow_modes_equal_per_level_kernel(built, deep, nonhydro, ghv):
     """The level-chunked slow-mode kernels (DivM pass, then the slow
     tendencies with the gradient weights serving the divergence) reproduce
     the one-level-per-block kernel bit for bit over two steps (all three RK
-    stages, so both the zero start and the state deviations of the stage)."""
+    stages, so both the zero start and the state deviations of the stage),
+    when the substep kernel forms the whole pressure flux divergence itself."""
     capi = built
     g = grid(capi, 4, 20)
     p = earth_params(capi, 4, dt=600.0, precision=1, HyDiff=1, DivDampP=1, DeepModel=deep, NonHydro=nonhydro,
@@ -143,6 +144,30 @@ def test_chunked_slow_modes_equal_per_level_kernel(built, deep, nonhydro, ghv):
           f"slow modes", flush=True)
     st = capi.ic_rest(g, p, seed=1011, temperature_noise=1.0, wind_noise=2.0)
     a = _two_steps(capi, g, p, st, {"THOR_SLOW_CHUNKED": "0"})
-    b = _two_steps(capi, g, p, st, {})
+    b = _two_steps(capi, g, p, st, {"THOR_PREPARE_SPLIT": "0"})
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+    # Default: the substep pressure flux divergence adds the stage part
+    # (formed by the slow-mode kernel) to the deviation part; the same sum
+    # grouped differently, so equal to FP64 rounding.
+    c = _two_steps(capi, g, p, st, {})
+    for k in a:
+        err = float(np.abs(c[k] - a[k]).max() / np.abs(a[k]).max())
+        print(f"  split vs unsplit pressure flux, {k}: max rel diff {err:.2e}")
+        assert err < 1e-11, k
+
+
+def test_thomas_loop_unroll_is_bitwise_unchanged(built):
+    """The fused vertical-solve kernel with its level loops unrolled by two
+    (the default) gives the same states as without unrolling, in both
+    precision forms."""
+    capi = built
+    g = grid(capi, 4, 20)
+    for prec in (0, 1):
+        p = earth_params(capi, 4, dt=600.0, precision=prec, HyDiff=1, DivDampP=1)
+        print(f"case: precision {prec} g4 L20, Thomas level loops unrolled by 2 vs 1", flush=True)
+        st = capi.ic_rest(g, p, seed=1012, temperature_noise=1.0, wind_noise=2.0)
+        a = _two_steps(capi, g, p, st, {"THOR_THOMAS_UNROLL": "1"})
+        b = _two_steps(capi, g, p, st, {"THOR_THOMAS_UNROLL": "2"})
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (prec, k)

@@ -63,6 +63,8 @@ struct Dyn {
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
     bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
+    int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
+    bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
     // Rayleigh sponge
     bool sponge_on = false;
     SpongePar spg{};
@@ -299,7 +301,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             // Prepare_Implicit_Vertical, plus the horizontal flux divergences of
             // Density_Pressure_Eqs (booked with the column kernel below)
             launch(d, "prepare", W(7, 3), [&] {
-                if (MX) k_prepare_lc<<<grid_lp, blk, 0, st>>>(g, f, lcp);
+                if (MX && d.prep_split) k_prepare_lc<true><<<grid_lp, blk, 0, st>>>(g, f, lcp);
+                else if (MX) k_prepare_lc<false><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
             // Vertical_Eq (12) + Density_Pressure_Eqs (22, of which Mhs and diffpr FP32 in mixed)
@@ -307,7 +310,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             const dim3 grid_th((d.P + d.tcb - 1) / d.tcb);
             if (!d.hooks.dyn_core_loop_fast_modes) {
                 launch(d, "thomas_dp", W(12 + 18, 4), [&] {
-                    if (d.tcb == 64) k_thomas_dp<64, MX><<<grid_th, 256, lds, st>>>(g, f, times);
+                    if (d.tcb == 64 && d.tur == 2) k_thomas_dp<64, MX, true, 2><<<grid_th, 256, lds, st>>>(g, f, times);
+                    else if (d.tcb == 64) k_thomas_dp<64, MX><<<grid_th, 256, lds, st>>>(g, f, times);
                     else if (d.tcb == 32) k_thomas_dp<32, MX><<<grid_th, 256, lds, st>>>(g, f, times);
                     else k_thomas_dp<16, MX><<<grid_th, 256, lds, st>>>(g, f, times);
                 });
@@ -626,7 +630,13 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     for (int p = 0; p < P && d.slow_lc; p++)
         if (std::memcmp(gd->div + (size_t)p * 21 + 3, gd->grad + (size_t)p * 21 + 3, 18 * sizeof(double)) != 0)
             d.slow_lc = false;
+    if (const char *e = std::getenv("THOR_THOMAS_UNROLL")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2) d.tur = v;
+    }
     if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) d.slow_lc = d.slow_lc && std::atoi(e) != 0;
+    d.prep_split = d.slow_lc;
+    if (const char *e = std::getenv("THOR_PREPARE_SPLIT")) d.prep_split = d.prep_split && std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
               && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)
@@ -654,6 +664,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     A(&f.SlowMh, n3); A(&f.SlowWh, ni); A(&f.SlowRho, n1); A(&f.Slowp, n1);
     A(&f.Sp, n1); A(&f.Sd, n1); A(&f.hflx_r, n1); A(&f.hflx_pt, n1); A(&f.tAA, n1); A(&f.tCC, n1); A(&f.tTT, n1);
     A(&f.Qheat, n1); A(&f.dMh, n3); A(&f.dWh, ni);
+    if (d.mixed) A(&f.hflx_ptk, n1);
     const long ncell = (long)P * nv;
     d.npart = (int)((ncell + 2047) / 2048);
     A(&d.partials, (size_t)5 * d.npart);

@@ -65,6 +65,7 @@ struct Fields {
     double *SlowMh, *SlowWh, *SlowRho, *Slowp;
     double *Sp, *Sd;                      // implicit-solve sources [nv]
     double *hflx_r, *hflx_pt;             // horizontal flux divergences of the density / pressure update [nv]
+    double *hflx_ptk;                     // mixed path: the stage part rs sum_k div_k(Mhk_k pt) of hflx_pt [nv]
     double *tAA, *tCC, *tTT;              // Thomas factorisation of the stage [nv]
     double *Qheat, *dMh, *dWh;            // ProfX tendencies
     unsigned int *flags;                  // bit 0: NaN pressure, bit 1: NaN temperature

@@ -1123,10 +1123,13 @@ __host__ __device__ constexpr int kThomasCB(int nv) { return 4 * (nv + 1) * 32 *
 // and hflx_pt = rscale sum div (Mhs + Mhk) pt.
 // wv(s, k) is the divergence coefficient of slot s, component k; U is the
 // unroll count of the slot loop (7 when wv reads a register array).
-template <int U, class GetW, class GetM, class GetH, class GetV1, class GetPt>
+// SPLIT (mixed path): v1 is the momentum deviation alone and the stage part
+// dptk = rs sum_k div_k(Mhk_k pt) of the pressure flux divergence, formed once
+// per stage, is added at the end.
+template <int U, bool SPLIT = false, class GetW, class GetM, class GetH, class GetV1, class GetPt>
 __device__ __forceinline__ void prepare_point_w(const Geo &g, const Fields &f, int p, int lev, GetW wv, GetM mv,
                                                 GetH hv, GetV1 v1, GetPt ptv, double slowp, double slowrho,
-                                                double rs) {
+                                                double rs, double dptk = 0.0) {
     const size_t S = g.S;
     const double RoC = g.Rd / (g.Cp - g.Rd);
     double nflxr = 0.0, nflxp = 0.0, dr = 0.0, dpt = 0.0;
@@ -1151,7 +1154,7 @@ __device__ __forceinline__ void prepare_point_w(const Geo &g, const Fields &f, i
     LV(f.Sp, lev) = nflxp + slowp;
     LV(f.Sd, lev) = nflxr + slowrho;
     LV(f.hflx_r, lev) = dr;
-    LV(f.hflx_pt, lev) = dpt;
+    LV(f.hflx_pt, lev) = SPLIT ? dpt + dptk : dpt;
 }
 
 template <class GetM, class GetH, class GetV1, class GetPt>
@@ -1421,7 +1424,10 @@ __global__ __launch_bounds__(256, 3) void k_momentum_lc(Geo g, Fields f, int div
 // Prepare_Implicit_Vertical (thor_vertical_int.h:566-771) and the horizontal
 // flux divergences of Density_Pressure_Eqs in FP64 arithmetic, as in
 // prepare_point (the momentum deviation is read from its FP32 storage).
-__global__ __launch_bounds__(256, 2) void k_prepare_lc(Geo g, Fields f, int lc) {
+// SPLIT: the stage part of the pressure flux divergence comes from
+// f.hflx_ptk (formed by k_slow_lc), so the stage momentum Mhk is not staged.
+template <bool SPLIT>
+__global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields f, int lc) {
 #pragma clang fp contract(fast)
     int r, l0, l1;
     if (!block_tile_lc(g, lc, r, l0, l1)) return;
@@ -1440,7 +1446,7 @@ __global__ __launch_bounds__(256, 2) void k_prepare_lc(Geo g, Fields f, int lc) 
         return;
     }
     __shared__ float sM[2][3][NHL2];
-    __shared__ double sH[2][NHL2], sV1[2][3][NHL2], sPt[2][NHL2], sRs[kLcMax];
+    __shared__ double sH[2][NHL2], sV1[2][SPLIT ? 1 : 3][NHL2], sPt[2][NHL2], sRs[kLcMax];
     if ((int)threadIdx.x < l1 - l0) sRs[threadIdx.x] = rscale_at(g, l0 + threadIdx.x);
     const LcThread T = lc_thread(g, r);
     const int p = T.p;
@@ -1449,21 +1455,25 @@ __global__ __launch_bounds__(256, 2) void k_prepare_lc(Geo g, Fields f, int lc) 
     for (int i = 0; i < 21; i++) wv[i] = g.div[(size_t)i * S + p];
     const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
     const auto rM = rsrc(Mhs, 3 * nv * pl4), rMk = rsrc(f.Mhk, 3 * nv * pl8), rH = rsrc(f.h, nv * pl8),
-               rPt = rsrc(f.pt, nv * pl8), rSp = rsrc(f.Slowp, nv * pl8), rSr = rsrc(f.SlowRho, nv * pl8);
+               rPt = rsrc(f.pt, nv * pl8), rSp = rsrc(f.Slowp, nv * pl8), rSr = rsrc(f.SlowRho, nv * pl8),
+               rPk = rsrc(f.hflx_ptk, nv * pl8);
     struct In {
         double slowp, slowrho;
         float om[3], hm[3];
-        double omk[3], hmk[3], oh, hh, opt, hpt;
+        double omk[SPLIT ? 1 : 3], hmk[SPLIT ? 1 : 3], oh, hh, opt, hpt;
     };
     auto load = [&](In &v, int lev) {
         v.slowp = ldb_d(rSp, T.o8, (unsigned)(lev * pl8));
         v.slowrho = ldb_d(rSr, T.o8, (unsigned)(lev * pl8));
+        if constexpr (SPLIT) v.omk[0] = ldb_d(rPk, T.o8, (unsigned)(lev * pl8));  // the stage part of hflx_pt
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             v.om[k] = ldb_f(rM, T.o4, (unsigned)((k * nv + lev) * pl4));
-            v.omk[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lev) * pl8));
             v.hm[k] = ldb_f(rM, T.h4, (unsigned)((k * nv + lev) * pl4));
-            v.hmk[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
+            if constexpr (!SPLIT) {
+                v.omk[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lev) * pl8));
+                v.hmk[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
+            }
         }
         v.oh = ldb_d(rH, T.o8, (unsigned)(lev * pl8));
         v.opt = ldb_d(rPt, T.o8, (unsigned)(lev * pl8));
@@ -1474,7 +1484,7 @@ __global__ __launch_bounds__(256, 2) void k_prepare_lc(Geo g, Fields f, int lc) 
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             sM[b][k][T.ir] = v.om[k];
-            sV1[b][k][T.ir] = (double)v.om[k] + v.omk[k];
+            if constexpr (!SPLIT) sV1[b][k][T.ir] = (double)v.om[k] + v.omk[k];
         }
         sH[b][T.ir] = v.oh;
         sPt[b][T.ir] = v.opt;
@@ -1482,21 +1492,24 @@ __global__ __launch_bounds__(256, 2) void k_prepare_lc(Geo g, Fields f, int lc) 
 #pragma unroll
             for (int k = 0; k < 3; k++) {
                 sM[b][k][T.hp] = v.hm[k];
-                sV1[b][k][T.hp] = (double)v.hm[k] + v.hmk[k];
+                if constexpr (!SPLIT) sV1[b][k][T.hp] = (double)v.hm[k] + v.hmk[k];
             }
             sH[b][T.hp] = v.hh;
             sPt[b][T.hp] = v.hpt;
         }
     };
     auto body = [&](int lev, In &R, In &L) {
-        const double slowp = L.slowp, slowrho = L.slowrho;
+        const double slowp = L.slowp, slowrho = L.slowrho, dptk = SPLIT ? L.omk[0] : 0.0;
         load(L, min(lev + 2, l1 - 1));
         const int b = (lev - l0) & 1;
-        prepare_point_w<7>(
+        prepare_point_w<7, SPLIT>(
             g, f, p, lev, [&](int s, int k) { return wv[s * 3 + k]; },
             [&](int s, int k) { return (double)sM[b][k][T.pos[s]]; }, [&](int s) { return sH[b][T.pos[s]]; },
-            [&](int s, int k) { return sV1[b][k][T.pos[s]]; }, [&](int s) { return sPt[b][T.pos[s]]; }, slowp,
-            slowrho, sRs[lev - l0]);
+            [&](int s, int k) {
+                if constexpr (SPLIT) return (double)sM[b][k][T.pos[s]];
+                else return sV1[b][k][T.pos[s]];
+            },
+            [&](int s) { return sPt[b][T.pos[s]]; }, slowp, slowrho, sRs[lev - l0], dptk);
         if (lev + 1 < l1) put(R, b ^ 1);
         __syncthreads();
     };
@@ -1710,7 +1723,7 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
         }
         return;
     }
-    __shared__ double sM[2][3][NHL2], sP[2][NHL2], sH[2][NHL2];
+    __shared__ double sM[2][3][NHL2], sP[2][NHL2], sH[2][NHL2], sT[2][NHL2];
     __shared__ double sW[24][kTile];  // the thread's 21 gradient weights and 3 divergence centre weights
     // Per-level constants of the chunk, formed once with slow_point's
     // arithmetic: Alt[l0 - 1 .. l1 - 1], Alth[l0 .. l1], (A / (A + Alt))^2 at
@@ -1755,6 +1768,7 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
     const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
     const size_t pl4 = S * 4, pl8 = S * 8;
     const auto rMk = rsrc(f.Mhk, 3 * nv * pl8), rPk = rsrc(f.pk, nv * pl8), rH = rsrc(f.h, nv * pl8),
+               rPt = rsrc(f.pt, nv * pl8),
                rAdv = rsrc(f.mAdv, 3 * nv * pl4), rRk = rsrc(f.Rhok, nv * pl8),
                rWk = rsrc(f.Whk, (nv + 1) * pl8), rHh = rsrc(f.hh, (nv + 1) * pl8),
                rDmh = rsrc(f.mdiffmh, 3 * nv * pl4), rdM = rsrc(f.dMh, 3 * nv * pl8),
@@ -1763,7 +1777,7 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
                rQ = rsrc(f.Qheat, nv * pl8), rDM = rsrc(dm, 3 * nv * pl4), rM = rsrc(f.Mh, 3 * nv * pl8),
                rR = rsrc(f.Rho, nv * pl8), rPr = rsrc(f.p, nv * pl8), rW = rsrc(f.Wh, (nv + 1) * pl8);
     struct Stg {  // staged tile values: own point and halo point, two levels ahead
-        double om[3], op, oh, hm[3], hp, hh;
+        double om[3], op, oh, ot, hm[3], hp, hh, ht;
     };
     struct Pt {  // the point's own values, one level ahead
         float adv[3], dmh[3], dM[3], dw, drh, dpr;
@@ -1781,6 +1795,8 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
         v.hp = ldb_d(rPk, T.h8, l8);
         v.oh = ldb_d(rH, T.o8, l8);
         v.hh = ldb_d(rH, T.h8, l8);
+        v.ot = ldb_d(rPt, T.o8, l8);
+        v.ht = ldb_d(rPt, T.h8, l8);
     };
     auto load_pt = [&](Pt &v, int lev) {
         const unsigned l8 = (unsigned)(lev * pl8), l4 = (unsigned)(lev * pl4), u8 = (unsigned)((lev + 1) * pl8);
@@ -1813,11 +1829,13 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
         for (int k = 0; k < 3; k++) sM[b][k][T.ir] = v.om[k];
         sP[b][T.ir] = v.op;
         sH[b][T.ir] = v.oh;
+        sT[b][T.ir] = v.ot;
         if (T.hp >= 0) {
 #pragma unroll
             for (int k = 0; k < 3; k++) sM[b][k][T.hp] = v.hm[k];
             sP[b][T.hp] = v.hp;
             sH[b][T.hp] = v.hh;
+            sT[b][T.hp] = v.ht;
         }
     };
     // values of level lev - 1 carried to level lev
@@ -1865,30 +1883,38 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
         }
         const double dz = altht - althl;
         const double r2p = sR[0][i], r2m = sR[1][i], r2l = sR[2][i], rscale = sR[3][i], dr2dz = sR[4][i];
-        double pv[7], hv[7];
+        double pv[7], hv[7], tv[7];
 #pragma unroll
         for (int s = 0; s < 7; s++) {
             pv[s] = sP[b][T.pos[s]];
             hv[s] = sH[b][T.pos[s]];
+            tv[s] = sT[b][T.pos[s]];
         }
-        double nflxv[3], nflxr = 0.0, nflxp = 0.0;
+        // nflxv, nflxr, nflxp as slow_point; dptk: the stage part of the
+        // pressure flux divergence of the substeps (prepare_point_w's st with
+        // the stage momentum), formed here once per stage
+        double nflxv[3], nflxr = 0.0, nflxp = 0.0, dptk = 0.0;
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const double m0 = sM[b][k][T.pos[0]];
             double sg = wg(k) * pv[0];
             double sr = wd0(k) * m0;
             double sp = wd0(k) * m0 * hv[0];
+            double st = wd0(k) * m0 * tv[0];
 #pragma unroll
             for (int s = 1; s < 7; s++) {
                 const double w = wg(s * 3 + k), m = sM[b][k][T.pos[s]];
                 sg = sg + w * pv[s];
                 sr = sr + w * m;
                 sp = sp + w * m * hv[s];
+                st = st + w * m * tv[s];
             }
             nflxv[k] = rscale * sg;
             nflxr += rscale * sr;
             nflxp += rscale * sp;
+            dptk += rscale * st;
         }
+        LV(f.hflx_ptk, lev) = dptk;
         double whl, wht, hhl, hht;
         if (lev == 0) {
             whl = 0.0;
@@ -2282,7 +2308,7 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
 // LDS: all threads take a share of the levels again after the solve.
 // DP = false: the vertical solve alone (Whs written, density/pressure left to
 // k_dp), for a physics hook that runs between the two as in the reference.
-template <int CB, bool MX, bool DP = true>
+template <int CB, bool MX, bool DP = true, int UR = 1>
 __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double deltat) {
     extern __shared__ double sm[];
     const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
@@ -2296,6 +2322,7 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
     const double dt2 = XPOW(deltat, 2.0);
     if (p < g.P) {
+#pragma unroll UR
         for (int lev = 1 + row; lev < nv; lev += NROW) {
             const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
             const double Sp = LV(f.Sp, lev), Sd = LV(f.Sd, lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
@@ -2366,6 +2393,7 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     }
     if (!DP) return;
     const bool pole = p >= g.P - 2;
+#pragma unroll UR
     for (int lev = row; lev < nv; lev += NROW)
         dp_column<MX>(g, f, p, lev, pole, deltat, LV(f.hflx_r, lev), LV(f.hflx_pt, lev),
                       [&](int k) { return sD[k * CB + c]; });
