@@ -2292,10 +2292,36 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
     const double r = LV(f.Rhok, lev) + rhos;
     nflxr += dwdz;
     LV(f.Rhos, lev) = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
-    const double ptc = (P_Ref / (Rd * r)) * XPOW((pss + pkv) / P_Ref, Cv / Cp);
-    aux += ptc * r;
-    const double pnew = P_Ref * XPOW(Rd * aux / P_Ref, Cp / Cv);
-    const double psn = pnew - pkv + (LV(FS<MX>::diffpr(f), lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
+    double psn;
+    if constexpr (MX) {
+        // The same update with one power. With pf = pss + pkv, q = (pf / P_Ref)^(Cv/Cp)
+        // and x = -Rd (nflxpt + dwptdz) dt / (P_Ref q): Rd aux / P_Ref = q (1 + x),
+        // so pnew = pf (1 + x)^(Cp/Cv) and pnew - pkv = pss + pf ((1 + x)^(Cp/Cv) - 1),
+        // formed without the cancellation of pnew - pkv. For |x| < 2^-7 the
+        // binomial series (terms to x^9) gives (1 + x)^gamma - 1.
+        const double pf = pss + pkv, gam = Cp / Cv;
+        const double q = pm::pow(pf / P_Ref, Cv / Cp);
+        const double x = aux * Rd / (P_Ref * q);
+        double g1;
+        if (fabs(x) < 0.0078125) {
+            double c[10];
+            c[1] = gam;
+#pragma unroll
+            for (int k = 2; k <= 9; k++) c[k] = c[k - 1] * (gam - (k - 1)) / k;
+            double h = c[9];
+#pragma unroll
+            for (int k = 8; k >= 1; k--) h = h * x + c[k];
+            g1 = h * x;
+        } else {
+            g1 = pm::pow(1.0 + x, gam) - 1.0;
+        }
+        psn = pss + pf * g1 + (LV(FS<MX>::diffpr(f), lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
+    } else {
+        const double ptc = (P_Ref / (Rd * r)) * XPOW((pss + pkv) / P_Ref, Cv / Cp);
+        aux += ptc * r;
+        const double pnew = P_Ref * XPOW(Rd * aux / P_Ref, Cp / Cv);
+        psn = pnew - pkv + (LV(FS<MX>::diffpr(f), lev) + 0.0) * dt + Rd / Cv * LV(f.Qheat, lev) * dt;
+    }
     LV(f.ps, lev) = psn;
     if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
 }
