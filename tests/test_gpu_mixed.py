@@ -171,3 +171,22 @@ def test_thomas_loop_unroll_is_bitwise_unchanged(built):
         b = _two_steps(capi, g, p, st, {"THOR_THOMAS_UNROLL": "2"})
         for k in a:
             assert np.array_equal(a[k], b[k]), (prec, k)
+
+
+def test_chunked_last_diffusion_pass_stays_close(built):
+    """The last hyperdiffusion pass and the stage's divergence-damping pass 0
+    over level chunks (FP32 with fused multiply-adds) against the
+    one-level-per-block kernel: two steps agree to FP32 rounding of those
+    tendencies, far inside the mixed-vs-FP64 distances."""
+    capi = built
+    g = grid(capi, 4, 20)
+    p = earth_params(capi, 4, dt=600.0, precision=1, HyDiff=1, DivDampP=1)
+    print("case: mixed g4 L20, chunked vs per-level last diffusion pass + divergence-damping pass 0", flush=True)
+    st = capi.ic_rest(g, p, seed=1013, temperature_noise=1.0, wind_noise=2.0)
+    a = _two_steps(capi, g, p, st, {"THOR_DIFF2_CHUNKED": "0"})
+    b = _two_steps(capi, g, p, st, {})
+    tol = {"Mh": 1e-5, "Wh": 1e-5, "W": 1e-5, "Rho": 1e-7, "pressure": 1e-7}
+    for k in a:
+        err = float(np.abs(b[k] - a[k]).max() / np.abs(a[k]).max())
+        print(f"  {k}: max rel diff {err:.2e}")
+        assert err < tol[k], k

@@ -65,6 +65,7 @@ struct Dyn {
     bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
     bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
+    bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
     // Rayleigh sponge
     bool sponge_on = false;
     SpongePar spg{};
@@ -260,11 +261,19 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             if (MX) k_adv_diff1_lc<<<grid_lc, blk, 0, st>>>(g, f, hydiff, lc);
             else k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff);
         });
-        if (hydiff || divdamp)
+        if (MX && d.diff2_lc) {
+            // the last hyperdiffusion pass and the stage's divergence-damping
+            // pass 0, each over level chunks
+            if (hydiff)
+                launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, f, d.diff_sign, lc); });
+            if (divdamp)
+                launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_lc<true><<<grid_lc, blk, 0, st>>>(g, f, lc); });
+        } else if (hydiff || divdamp) {
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
                    [&] {
                 k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
             });
+        }
         if (d.hooks.dyn_core_loop_slow_modes) {
             const thor_step_view v = view(f);
             if (d.hooks.dyn_core_loop_slow_modes(d.hooks.user, (void *)st, nstep, times, &v))
@@ -283,7 +292,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         if (rk > 0) rk_arrays(0);
         auto divm0 = [&] {
             launch(d, "divm0", W(1, 6), [&] {
-                if (MX) k_divm0_lc<<<grid_lc, blk, 0, st>>>(g, f, lc);
+                if (MX) k_divm0_lc<false><<<grid_lc, blk, 0, st>>>(g, f, lc);
                 else k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
         };
@@ -636,6 +645,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     }
     if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) d.slow_lc = d.slow_lc && std::atoi(e) != 0;
     d.prep_split = d.slow_lc;
+    if (const char *e = std::getenv("THOR_DIFF2_CHUNKED")) d.diff2_lc = std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_PREPARE_SPLIT")) d.prep_split = d.prep_split && std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)

@@ -1524,9 +1524,106 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
     }
 }
 
+// Last Diffusion_Op pass with Correct_Horizontal (thor_diff.h:524-915) of
+// the mixed-precision path over level chunks: the FP32 Laplacian of the first
+// pass's output (variables 1..5 weighted by the stage density, as
+// diff2_divm0_point forms them), scaled by -Kdh4 sign, with the radial part
+// of the momentum components removed.
+__global__ __launch_bounds__(256, 3) void k_diff2_lc(Geo g, Fields f, double sign, int lc) {
+#pragma clang fp contract(fast)
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const float *diff = f.mdiff;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev))
+            diff2_divm0_point<true>(
+                g, f, c.p, lev, 5, true, 1, 0, sign, [&](int s, int var) { return VEC(diff, var, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.Rhok, lev, c.gi(s)); }, [&](int, int) { return 0.0; },
+                [&](int) { return 0.0; }, [&](int) { return 0.0; });
+        return;
+    }
+    __shared__ float sA[2][6][NHL2], sL[4][kLcMax];
+    __shared__ double sK[kLcMax];
+    stage_lvc(g, l0, l1, sL);
+    if ((int)threadIdx.x < l1 - l0) sK[threadIdx.x] = -g.Kdh4[l0 + threadIdx.x] * sign;
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
+    float wl[6];
+#pragma unroll
+    for (int s = 0; s < 6; s++) wl[s] = g.wlap[(size_t)(s + 1) * S + p];
+    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+    const size_t pl4 = S * 4, pl8 = S * 8;
+    const auto rDf = rsrc(diff, 6 * nv * pl4), rR = rsrc(f.Rhok, nv * pl8), rRh = rsrc(f.mdiffrh, nv * pl4),
+               rWd = rsrc(f.mdiffw, nv * pl4), rPd = rsrc(f.mdiffpr, nv * pl4), rMd = rsrc(f.mdiffmh, 3 * nv * pl4);
+    struct In {
+        float od[6], hd[6];
+        double orho, hrho;
+    };
+    auto load = [&](In &v, int lev) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            v.od[k] = ldb_f(rDf, T.o4, (unsigned)((k * nv + lev) * pl4));
+            v.hd[k] = ldb_f(rDf, T.h4, (unsigned)((k * nv + lev) * pl4));
+        }
+        v.orho = ldb_d(rR, T.o8, (unsigned)(lev * pl8));
+        v.hrho = ldb_d(rR, T.h8, (unsigned)(lev * pl8));
+    };
+    auto put = [&](const In &v, int b) {
+        sA[b][0][T.ir] = v.od[0];
+#pragma unroll
+        for (int k = 1; k < 6; k++) sA[b][k][T.ir] = (float)(v.od[k] * v.orho);
+        if (T.hp >= 0) {
+            sA[b][0][T.hp] = v.hd[0];
+#pragma unroll
+            for (int k = 1; k < 6; k++) sA[b][k][T.hp] = (float)(v.hd[k] * v.hrho);
+        }
+    };
+    auto body = [&](int lev, In &R, In &L) {
+        load(L, min(lev + 2, l1 - 1));
+        const int b = (lev - l0) & 1, i = lev - l0;
+        const float rs2 = sL[1][i];
+        const double sdiff = sK[i];
+        double out[6];
+#pragma unroll
+        for (int var = 0; var < 6; var++) {
+            const float a0 = sA[b][var][T.pos[0]];
+            float sum = wl[0] * (sA[b][var][T.pos[1]] - a0);
+#pragma unroll
+            for (int s = 2; s < 7; s++) sum += wl[s - 1] * (sA[b][var][T.pos[s]] - a0);
+            out[var] = (double)(rs2 * sum) * sdiff;
+        }
+        const unsigned l4 = (unsigned)(lev * pl4);
+        stb_f(rRh, T.o4, l4, (float)out[0]);
+        stb_f(rWd, T.o4, l4, (float)out[4]);
+        stb_f(rPd, T.o4, l4, (float)out[5]);
+        const double dmhr = fx * out[1] + fy * out[2] + fz * out[3];
+        stb_f(rMd, T.o4, (unsigned)((0 * nv + lev) * pl4), (float)(out[1] + -fx * dmhr));
+        stb_f(rMd, T.o4, (unsigned)((1 * nv + lev) * pl4), (float)(out[2] + -fy * dmhr));
+        stb_f(rMd, T.o4, (unsigned)((2 * nv + lev) * pl4), (float)(out[3] + -fz * dmhr));
+        if (lev + 1 < l1) put(R, b ^ 1);
+        __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
+    }
+}
+
 // Divergence-damping pass 0 (DivM_Op) on the deviations Mhs / Whs in FP32,
 // as divm_w<false>. Whs lives on interfaces: the top value of level lev is
 // the bottom value of lev + 1, so each level loads one new interface.
+// STAGE: pass 0 of the slow part, on the stage values Mhk / Whk (FP64 storage,
+// rounded to FP32 as diff2_divm0_point does).
+template <bool STAGE>
 __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
 #pragma clang fp contract(fast)
     int r, l0, l1;
@@ -1540,9 +1637,14 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
         int lev;
         if (pole_ctx_lc(g, l0, l1, c, lev)) {
             float o3[3];
+            const double *Wv = STAGE ? f.Whk : f.Whs;
             divm_w<false>(
-                g, c.p, lev, [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)); },
-                [&](int s) { return (float)(LVP(f.Whs, lev, c.gi(s)) - LVP(f.Whs, lev + 1, c.gi(s))); }, o3);
+                g, c.p, lev,
+                [&](int s, int k) {
+                    if constexpr (STAGE) return (float)VEC(f.Mhk, k, lev, c.gi(s));
+                    else return VEC(Mhs, k, lev, c.gi(s));
+                },
+                [&](int s) { return (float)(LVP(Wv, lev, c.gi(s)) - LVP(Wv, lev + 1, c.gi(s))); }, o3);
 #pragma unroll
             for (int k = 0; k < 3; k++) VEC(divg, k, lev, c.p) = o3[k];
         }
@@ -1558,7 +1660,8 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
 #pragma unroll
     for (int i = 0; i < 21; i++) wu[i] = g.udd[(size_t)i * S + p];
     const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
-    const auto rM = rsrc(Mhs, 3 * nv * pl4), rW = rsrc(f.Whs, (nv + 1) * pl8), rD = rsrc(divg, 3 * nv * pl4);
+    const auto rM = STAGE ? rsrc(f.Mhk, 3 * nv * pl8) : rsrc(Mhs, 3 * nv * pl4),
+               rW = rsrc(STAGE ? f.Whk : f.Whs, (nv + 1) * pl8), rD = rsrc(divg, 3 * nv * pl4);
     // loaded values of one level: Mhs and the top interface's Whs
     struct In {
         float om[3], hm[3];
@@ -1567,8 +1670,13 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
     auto load = [&](In &v, int lev) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
-            v.om[k] = ldb_f(rM, T.o4, (unsigned)((k * nv + lev) * pl4));
-            v.hm[k] = ldb_f(rM, T.h4, (unsigned)((k * nv + lev) * pl4));
+            if constexpr (STAGE) {
+                v.om[k] = (float)ldb_d(rM, T.o8, (unsigned)((k * nv + lev) * pl8));
+                v.hm[k] = (float)ldb_d(rM, T.h8, (unsigned)((k * nv + lev) * pl8));
+            } else {
+                v.om[k] = ldb_f(rM, T.o4, (unsigned)((k * nv + lev) * pl4));
+                v.hm[k] = ldb_f(rM, T.h4, (unsigned)((k * nv + lev) * pl4));
+            }
         }
         v.ow = ldb_d(rW, T.o8, (unsigned)((lev + 1) * pl8));
         v.hw = ldb_d(rW, T.h8, (unsigned)((lev + 1) * pl8));
