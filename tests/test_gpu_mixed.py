@@ -190,3 +190,23 @@ def test_chunked_last_diffusion_pass_stays_close(built):
         err = float(np.abs(b[k] - a[k]).max() / np.abs(a[k]).max())
         print(f"  {k}: max rel diff {err:.2e}")
         assert err < tol[k], k
+
+
+@pytest.mark.parametrize("deep,nonhydro,ghv", [(1, 1, 0), (0, 0, 1)])
+def test_column_prep_level_window_is_bitwise_unchanged(built, deep, nonhydro, ghv):
+    """The column-preparation kernel with 2 or 4 levels per thread (per-level
+    values formed once into a window) gives the same states as with one
+    level per thread, in both precision forms."""
+    capi = built
+    g = grid(capi, 4, 20)
+    for prec in (0, 1):
+        p = earth_params(capi, 4, dt=600.0, precision=prec, HyDiff=1, DivDampP=1, DeepModel=deep,
+                         NonHydro=nonhydro, GravHeightVar=ghv)
+        print(f"case: precision {prec} g4 L20, DeepModel {deep} NonHydro {nonhydro} GravHeightVar {ghv}: column "
+              f"preparation with 1, 2, 4 levels per thread", flush=True)
+        st = capi.ic_rest(g, p, seed=1014, temperature_noise=1.0, wind_noise=2.0)
+        a = _two_steps(capi, g, p, st, {"THOR_COLPREP_LEVELS": "1"})
+        for lv in ("2", "4"):
+            b = _two_steps(capi, g, p, st, {"THOR_COLPREP_LEVELS": lv})
+            for k in a:
+                assert np.array_equal(a[k], b[k]), (prec, lv, k)

@@ -66,6 +66,7 @@ struct Dyn {
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
     bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
     bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
+    int cp_lv = 2;            // levels per thread of the column-preparation kernel (1, 2 or 4)
     // Rayleigh sponge
     bool sponge_on = false;
     SpongePar spg{};
@@ -209,6 +210,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     const int lc = d.lc, lcp = d.lc_prep;
     // column_prep: (256 columns, level 0..nv) per block, XCD-major
     const dim3 grid_c(8 * (((d.P + kTile - 1) / kTile + 7) / 8) * (d.nv + 1));
+    const dim3 grid_c2(8 * (((d.P + kTile - 1) / kTile + 7) / 8) * ((d.nv + 2) / 2));
+    const dim3 grid_c4(8 * (((d.P + kTile - 1) / kTile + 7) / 8) * ((d.nv + 4) / 4));
     const int cb = kThomasCB(d.nv);
     const dim3 grid_t((d.P + cb - 1) / cb);
     const size_t lds_t = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
@@ -256,7 +259,11 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
-        launch(d, "column_prep", W(12 + 5, 6), [&] { k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f); });
+        launch(d, "column_prep", W(12 + 5, 6), [&] {
+            if (d.cp_lv == 4) k_column_prep_lv<MX, 4><<<grid_c4, blk, 0, st>>>(g, f);
+            else if (d.cp_lv == 2) k_column_prep_lv<MX, 2><<<grid_c2, blk, 0, st>>>(g, f);
+            else k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f);
+        });
         launch(d, "adv_diff1", W(8 + (hydiff ? 7 : 0), 3 + (hydiff ? 6 : 0)), [&] {
             if (MX) k_adv_diff1_lc<<<grid_lc, blk, 0, st>>>(g, f, hydiff, lc);
             else k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff);
@@ -646,6 +653,10 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) d.slow_lc = d.slow_lc && std::atoi(e) != 0;
     d.prep_split = d.slow_lc;
     if (const char *e = std::getenv("THOR_DIFF2_CHUNKED")) d.diff2_lc = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_COLPREP_LEVELS")) {
+        const int v = std::atoi(e);
+        if (v == 1 || v == 2 || v == 4) d.cp_lv = v;
+    }
     if (const char *e = std::getenv("THOR_PREPARE_SPLIT")) d.prep_split = d.prep_split && std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)

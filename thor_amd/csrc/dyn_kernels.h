@@ -582,6 +582,194 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
     VEC(Adv, 2, lev, p) = dav[2] + Cz;
 }
 
+// The same outputs with LV consecutive levels per thread: the per-level
+// values each output needs from its own and the neighbouring levels
+// (temperature, h, pt, the advecting velocity, the interface gtil term) are
+// formed once per level into a window in registers instead of once per use,
+// with the same expressions, so the outputs equal k_column_prep's bit for bit
+// (one power per level instead of three). Block i: XCD i % 8, a contiguous
+// range of column blocks per XCD, level groups of one column block
+// consecutive.
+template <bool MX, int LV> __global__ __launch_bounds__(256) void k_column_prep_lv(Geo g, Fields f) {
+    const int nv = g.nv, ngrp = (nv + 1 + LV - 1) / LV;
+    const int i = blockIdx.x, idx = i >> 3;
+    const int pbx = ((g.P + 255) / 256 + 7) / 8;  // column blocks per XCD
+    const int pb = (i & 7) * pbx + idx / ngrp;
+    const int l0 = (idx % ngrp) * LV;  // levels l0 .. l0 + LV - 1 (centre if < nv, interface if <= nv)
+    const int p = pb * 256 + threadIdx.x;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const double Rd = g.Rd, Cp = g.Cp, P_Ref = g.P_Ref, Gravit = g.Gravit, A = g.A;
+    const double Cv = Cp - Rd, CvoCp = Cv / Cp;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const double *pk = f.pk, *Rho = f.Rhok, *Wh = f.Whk;
+    const double fr[3] = {g.fr[p], g.fr[S + p], g.fr[2 * S + p]};
+    auto wts = [](double xi, double xim, double xip, double &intt, double &intl) {
+        intt = (xi - xip) / (xim - xip);
+        intl = (xi - xim) / (xip - xim);
+    };
+    // window of levels m = l0 - 1 + j, j = 0 .. LV + 1 (centre values valid for 0 <= m < nv)
+    constexpr int NW = LV + 2;
+    double wp[NW], wr[NW], wT[NW], wh[NW], wpt[NW], wv[NW][3], wgt[NW], wwh[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+        const int m = l0 - 1 + j;
+        wwh[j] = (m >= 0 && m <= nv) ? LV(Wh, m) : 0.0;
+        if (m >= 0 && m < nv) {
+            const double pr = LV(pk, m), r = LV(Rho, m), w = LV(f.Wk, m);
+            wp[j] = pr;
+            wr[j] = r;
+            const double temperature = pr / (r * Rd);
+            wT[j] = temperature;
+            wh[j] = Cv * temperature + pr / r;
+            wpt[j] = (P_Ref / (Rd * r)) * XPOW(pr / P_Ref, CvoCp);
+#pragma unroll
+            for (int c = 0; c < 3; c++) wv[j][c] = VEC(f.Mhk, c, m, p) / r + (w / r) * fr[c];
+        } else {
+            wp[j] = wr[j] = wT[j] = wh[j] = wpt[j] = 0.0;
+            wv[j][0] = wv[j][1] = wv[j][2] = 0.0;
+        }
+    }
+    // -(Wh/rho_h) dp/dz at interior interface m (the sweep's gtilht), 0 at m = 0
+#pragma unroll
+    for (int j = 0; j < NW; j++) {
+        const int m = l0 - 1 + j;
+        if (m >= 1 && m < nv) {
+            const double pm1 = j > 0 ? wp[j - 1] : LV(pk, m - 1), rm1 = j > 0 ? wr[j - 1] : LV(Rho, m - 1);
+            const double dp_dz = (wp[j] - pm1) / (Alt[m] - Alt[m - 1]);
+            double intt, intl;
+            wts(Alth[m], Alt[m - 1], Alt[m], intt, intl);
+            const double rhoh = rm1 * intt + wr[j] * intl;
+            wgt[j] = -(wwh[j] / rhoh) * dp_dz;
+        } else {
+            wgt[j] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < LV; q++) {
+        const int lev = l0 + q, j = q + 1;  // window index of lev
+        if (lev > nv) break;
+        // ---- centre values
+        if (lev < nv) {
+            const double pressure = wp[j], rho = wr[j];
+            LV(f.T, lev) = wT[j];
+            LV(f.h, lev) = wh[j];
+            LV(f.pt, lev) = wpt[j];
+            (void)pressure;
+            (void)rho;
+            double intt, intl;
+            if (lev < nv - 1) {
+                wts(Alt[lev + 1], Alth[lev + 1], Alth[lev + 2], intt, intl);
+                LV(f.gtil, lev) = wgt[j] * intt + wgt[j + 1] * intl;
+            } else {
+                wts(Alt[nv - 1], Alth[nv - 1], Alth[nv], intt, intl);
+                const double gtilht = 0.0;
+                LV(f.gtil, nv - 1) = wgt[j] * intt + gtilht * intl;
+            }
+        }
+        // ---- interface values
+        if (lev == 0) {
+            const double rho = wr[j], alt = Alt[0];
+            const double extr = (-alt - Alt[1]) / (Alt[1] - alt);
+            LV(f.hh, 0) = wh[j];
+            double pl;
+            if (g.ghv) pl = wp[j + 1] - rho * Gravit * XPOW(A / (A + Alt[0]), 2) * (-alt - Alt[1]);
+            else pl = wp[j + 1] - rho * Gravit * (-alt - Alt[1]);
+            const double rhol = wr[j + 1] + (wr[j + 1] - rho) * extr;
+            const double rhoh = 0.5 * (rho + rhol);
+            const double dz = 2.0 * Alt[0];
+            const double dpdz = (wp[j] - pl) / dz;
+            LV(f.gtilh, 0) = -(1.0 / rhoh) * dpdz;
+            const double pt0 = wpt[j], pt1 = wpt[j + 1];
+            LV(f.pth, 0) = pt1 + (pt1 - pt0) * (Alth[0] - Alt[1]) / (Alt[1] - Alt[0]);
+        } else if (lev == nv) {
+            // window index j - 1 is nv - 1; nv - 2 is j - 2 when q >= 1, else read directly
+            const double alt = Alt[nv - 1], rho = wr[j - 1];
+            const double rho2 = q >= 1 ? wr[j - 2] : LV(Rho, nv - 2), p2 = q >= 1 ? wp[j - 2] : LV(pk, nv - 2);
+            double ptm;
+            if (q >= 1) {
+                ptm = wpt[j - 2];
+            } else {
+                const double pr = LV(pk, nv - 2), r = LV(Rho, nv - 2);
+                ptm = (P_Ref / (Rd * r)) * XPOW(pr / P_Ref, CvoCp);
+            }
+            const double extr = (2 * Alth[nv] - alt - Alt[nv - 2]) / (Alt[nv - 1] - Alt[nv - 2]);
+            LV(f.hh, nv) = wh[j - 1];
+            const double rhop = rho2 + (wr[j - 1] - rho2) * extr;
+            const double rhoh = 0.5 * (rho + rhop);
+            const double ptc = wpt[j - 1];
+            LV(f.pth, nv) = ptm + (ptc - ptm) * extr;
+            const double dz = 2.0 * (Alth[nv] - Alt[nv - 1]);
+            double pp;
+            if (g.ghv)
+                pp = p2 - wr[j - 1] * Gravit * XPOW(A / (A + Alt[nv - 1]), 2) * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+            else
+                pp = p2 - wr[j - 1] * Gravit * (2 * Alth[nv] - Alt[nv - 1] - Alt[nv - 2]);
+            const double dpdz = (pp - wp[j - 1]) / dz;
+            LV(f.gtilh, nv) = -(1.0 / rhoh) * dpdz;
+        } else {
+            const double dz = Alt[lev] - Alt[lev - 1];
+            const double dp_dz = (wp[j] - wp[j - 1]) / dz;
+            double intt, intl;
+            wts(Alth[lev], Alt[lev - 1], Alt[lev], intt, intl);
+            LV(f.hh, lev) = wh[j - 1] * intt + wh[j] * intl;
+            LV(f.pth, lev) = wpt[j - 1] * intt + wpt[j] * intl;
+            const double rhoh = wr[j - 1] * intt + wr[j] * intl;
+            LV(f.gtilh, lev) = -(1.0 / rhoh) * dp_dz;
+        }
+        if (lev == nv) break;
+
+        // ---- vertical advection of momentum and Coriolis (Compute_Advec_Cori2)
+        const double Omega = g.Omega;
+        const double althl = Alth[lev], altht = Alth[lev + 1], alt = Alt[lev];
+        double dvl[3], dvt[3], intt, intl;
+        const double *vc = wv[j];
+        if (lev == 0) {
+            dvl[0] = dvl[1] = dvl[2] = 0.0;
+        } else {
+            const double whl = wwh[j];
+            wts(althl, Alt[lev - 1], alt, intt, intl);
+#pragma unroll
+            for (int c = 0; c < 3; c++) dvl[c] = (wv[j - 1][c] * intt + vc[c] * intl) * whl;
+        }
+        if (lev == nv - 1) {
+            dvt[0] = dvt[1] = dvt[2] = 0.0;
+        } else {
+            const double wht = wwh[j + 1];
+            wts(altht, alt, Alt[lev + 1], intt, intl);
+#pragma unroll
+            for (int c = 0; c < 3; c++) dvt[c] = (vc[c] * intt + wv[j + 1][c] * intl) * wht;
+        }
+        double r2p, r2m, r2l;
+        if (g.deep) {
+            r2p = XPOW(altht + A, 2.0);
+            r2m = XPOW(alt + A, 2.0);
+            r2l = XPOW(althl + A, 2.0);
+        } else {
+            r2p = r2m = r2l = 1.0;
+        }
+        const double dzi = 1.0 / ((althl - altht) * r2m);
+        double dav[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) dav[c] = (dvl[c] * r2l - dvt[c] * r2p) * dzi;
+        const double rho = wr[j];
+        double Cx, Cy, Cz;
+        if (g.deep) {
+            Cx = -2.0 * Omega * vc[1] * rho;
+            Cy = 2.0 * Omega * vc[0] * rho;
+            Cz = 0.0;
+        } else {
+            Cx = 2 * Omega * (vc[2] * fr[2] * fr[1] - vc[1] * XPOW(fr[2], 2)) * rho;
+            Cy = -2 * Omega * (vc[2] * fr[2] * fr[0] - vc[0] * XPOW(fr[2], 2)) * rho;
+            Cz = 2 * Omega * (vc[1] * fr[2] * fr[0] - vc[0] * fr[2] * fr[1]) * rho;
+        }
+        auto *Adv = FS<MX>::Adv(f);
+        VEC(Adv, 0, lev, p) = dav[0] + Cx;
+        VEC(Adv, 1, lev, p) = dav[1] + Cy;
+        VEC(Adv, 2, lev, p) = dav[2] + Cz;
+    }
+}
+
 // ===================================================================== K_B
 // Horizontal advection (Compute_Advec_Cori1 / _Poles, thor_adv_cor.h:51-362)
 // added to the vertical part, and the first Laplacian pass of the 4th-order
