@@ -51,6 +51,9 @@ struct Oracle {
     double spq[6] = {0};
     std::vector<int> ztab;  // [P][3]: latitude bin, points in the bin, index within the bin
     int max_count = 0;
+    // vertical solve: keep each column's tridiagonal system (a, b, c, d per
+    // interface, as "tri_a" .. "tri_d" [P][nv+1]) for the residual checks
+    bool capture_tri = false;
 
     double *A_(const char *n) { return arr[n].data(); }
 };
@@ -976,6 +979,11 @@ void vertical_eq(Oracle &o, double deltat) {
     const double A = o.p.A, Gravit = o.p.Gravit;
     const bool deep = o.p.DeepModel, ghv = o.p.GravHeightVar, nonh = o.p.NonHydro;
     std::vector<double> cc(nvi), dd(nvi);
+    double *ta = nullptr, *tb = nullptr, *tc = nullptr, *td = nullptr;
+    if (o.capture_tri) {
+        for (const char *n : {"tri_a", "tri_b", "tri_c", "tri_d"}) o.arr[n].assign((size_t)o.P * nvi, 0.0);
+        ta = o.A_("tri_a"), tb = o.A_("tri_b"), tc = o.A_("tri_c"), td = o.A_("tri_d");
+    }
     for (int id = 0; id < o.P; id++) {
         double altht = 0, althl = 0, alth = 0, alt = 0, altl = 0, hp = 0, h = 0, hm = 0, gp = 0, g = 0, gm = 0;
         double Sp = 0, Spl = 0, p = 0, pl = 0, r = 0, rl = 0, Sdl = 0, Sd = 0;
@@ -1124,6 +1132,12 @@ void vertical_eq(Oracle &o, double deltat) {
                 }
             }
             dd[lev] = -C0;
+            if (ta) {
+                ta[id * nvi + lev] = aa;
+                tb[id * nvi + lev] = bb;
+                tc[id * nvi + lev] = cc[lev];
+                td[id * nvi + lev] = -C0;
+            }
             if (lev == 1) {
                 cc[1] = cc[1] / bb;
                 dd[1] = dd[1] / bb;
@@ -1803,6 +1817,10 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         slow_modes(*o);
     } else if (n == "held_suarez") {
         held_suarez(*o);
+    } else if (n == "vertical_eq") {  // a0: the substep length in milliseconds
+        o->capture_tri = a1 != 0;
+        vertical_eq(*o, a0 * 1e-3);
+        o->capture_tri = false;
     } else {
         return -1;
     }

@@ -53,6 +53,10 @@ void oracle_profx_held_suarez(oracle *o);
 //   "divm"         DivM_Op + _Poles on Mhk/Whk, a0 = laststep (thor_div.h:49-535)
 //   "slow_modes"   Compute_Slow_Modes + _Poles    (thor_slowmodes.h:48-683)
 //   "held_suarez"  held_suarez                    (profx_held_suarez.h:51-117)
+//   "vertical_eq"  Vertical_Eq, a0 = substep length in ms, a1 != 0 keeps each
+//                  column's system as "tri_a" .. "tri_d" [P][nv+1]
+//                  (a x_{l-1} + b x_l + c x_{l+1} = d at interfaces 1..nv-1)
+//                                                 (thor_vertical_int.h:52-563)
 // Returns 0, or -1 for an unknown name. "Kdh4"/"Kdhz" are readable arrays.
 int oracle_kernel(oracle *o, const char *name, int a0, int a1);
 // ProfX + Thor, n times.

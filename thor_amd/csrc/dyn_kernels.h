@@ -2630,6 +2630,11 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
 // LDS: all threads take a share of the levels again after the solve.
 // DP = false: the vertical solve alone (Whs written, density/pressure left to
 // k_dp), for a physics hook that runs between the two as in the reference.
+// Iterations of the serial sweeps unrolled together: their factorisation loads
+// do not depend on the recurrence and are issued ahead of it.
+#ifndef THOR_SWEEP_UNROLL
+#define THOR_SWEEP_UNROLL 8
+#endif
 template <int CB, bool MX, bool DP = true, int UR = 1>
 __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double deltat) {
     extern __shared__ double sm[];
@@ -2690,7 +2695,7 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     if (row == 0 && p < g.P && nv > 1) {
         double ddp = (-sD[CB + c]) / TT[1];
         sD[CB + c] = ddp;
-#pragma unroll 8
+#pragma unroll THOR_SWEEP_UNROLL
         for (int lev = 2; lev < nv; lev++) {
             ddp = (-sD[lev * CB + c] - ddp * AA[lev]) * TT[lev];
             sD[lev * CB + c] = ddp;
@@ -2698,7 +2703,7 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     }
     if (row == 0 && p < g.P) {
         double wup = 0.0;
-#pragma unroll 8
+#pragma unroll THOR_SWEEP_UNROLL
         for (int lev = nv - 1; lev > 0; lev--) {
             const double w = (-CC[lev] * wup + sD[lev * CB + c]);
             sD[lev * CB + c] = w;
