@@ -1633,8 +1633,10 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
                 [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
         return;
     }
-    __shared__ float sM[2][3][NHL2];
-    __shared__ double sH[2][NHL2], sV1[2][SPLIT ? 1 : 3][NHL2], sPt[2][NHL2], sRs[kLcMax];
+    // staged point: (Mhs, 0) as one 16-byte LDS word and (h, pt) as another
+    __shared__ float4 sM4[2][NHL2];
+    __shared__ double2 sHT[2][NHL2];
+    __shared__ double sV1[2][SPLIT ? 1 : 3][NHL2], sRs[kLcMax];
     if ((int)threadIdx.x < l1 - l0) sRs[threadIdx.x] = rscale_at(g, l0 + threadIdx.x);
     const LcThread T = lc_thread(g, r);
     const int p = T.p;
@@ -1669,35 +1671,63 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
         v.hpt = ldb_d(rPt, T.h8, (unsigned)(lev * pl8));
     };
     auto put = [&](const In &v, int b) {
+        sM4[b][T.ir] = make_float4(v.om[0], v.om[1], v.om[2], 0.f);
+        sHT[b][T.ir] = make_double2(v.oh, v.opt);
+        if constexpr (!SPLIT) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            sM[b][k][T.ir] = v.om[k];
-            if constexpr (!SPLIT) sV1[b][k][T.ir] = (double)v.om[k] + v.omk[k];
+            for (int k = 0; k < 3; k++) sV1[b][k][T.ir] = (double)v.om[k] + v.omk[k];
         }
-        sH[b][T.ir] = v.oh;
-        sPt[b][T.ir] = v.opt;
         if (T.hp >= 0) {
+            sM4[b][T.hp] = make_float4(v.hm[0], v.hm[1], v.hm[2], 0.f);
+            sHT[b][T.hp] = make_double2(v.hh, v.hpt);
+            if constexpr (!SPLIT) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                sM[b][k][T.hp] = v.hm[k];
-                if constexpr (!SPLIT) sV1[b][k][T.hp] = (double)v.hm[k] + v.hmk[k];
+                for (int k = 0; k < 3; k++) sV1[b][k][T.hp] = (double)v.hm[k] + v.hmk[k];
             }
-            sH[b][T.hp] = v.hh;
-            sPt[b][T.hp] = v.hpt;
         }
     };
     auto body = [&](int lev, In &R, In &L) {
         const double slowp = L.slowp, slowrho = L.slowrho, dptk = SPLIT ? L.omk[0] : 0.0;
         load(L, min(lev + 2, l1 - 1));
         const int b = (lev - l0) & 1;
-        prepare_point_w<7, SPLIT>(
-            g, f, p, lev, [&](int s, int k) { return wv[s * 3 + k]; },
-            [&](int s, int k) { return (double)sM[b][k][T.pos[s]]; }, [&](int s) { return sH[b][T.pos[s]]; },
-            [&](int s, int k) {
-                if constexpr (SPLIT) return (double)sM[b][k][T.pos[s]];
-                else return sV1[b][k][T.pos[s]];
-            },
-            [&](int s) { return sPt[b][T.pos[s]]; }, slowp, slowrho, sRs[lev - l0], dptk);
+        // prepare_point_w's sums, slot by slot (each component's sum keeps its
+        // slot order): one 16-byte LDS read of (Mhs, 0) and one of (h, pt) per slot
+        double sr[3], sp[3], st[3];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            const float4 mq = sM4[b][T.pos[s]];
+            const double2 hq = sHT[b][T.pos[s]];
+            const double m[3] = {mq.x, mq.y, mq.z};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double dd = wv[s * 3 + k];
+                const double v1 = SPLIT ? m[k] : sV1[b][SPLIT ? 0 : k][T.pos[s]];
+                if (s == 0) {
+                    sr[k] = dd * m[k];
+                    sp[k] = dd * m[k] * hq.x;
+                    st[k] = dd * v1 * hq.y;
+                } else {
+                    sr[k] = sr[k] + dd * m[k];
+                    sp[k] = sp[k] + dd * m[k] * hq.x;
+                    st[k] = st[k] + dd * v1 * hq.y;
+                }
+            }
+        }
+        {
+            const double rs = sRs[lev - l0], RoC = g.Rd / (g.Cp - g.Rd);
+            double nflxr = 0.0, nflxp = 0.0, dr = 0.0, dpt = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                nflxr -= rs * sr[k];
+                nflxp -= (RoC)*rs * sp[k];
+                dr += rs * sr[k];
+                dpt += rs * st[k];
+            }
+            LV(f.Sp, lev) = nflxp + slowp;
+            LV(f.Sd, lev) = nflxr + slowrho;
+            LV(f.hflx_r, lev) = dr;
+            LV(f.hflx_pt, lev) = SPLIT ? dpt + dptk : dpt;
+        }
         if (lev + 1 < l1) put(R, b ^ 1);
         __syncthreads();
     };
@@ -1838,7 +1868,9 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
         }
         return;
     }
-    __shared__ float sM[2][3][NHL2], sD[2][NHL2], sL[4][kLcMax];
+    // staged point: (M_x, M_y, M_z, Wl - Wt) as one 16-byte LDS word per slot
+    __shared__ float4 sMD[2][NHL2];
+    __shared__ float sL[4][kLcMax];
     stage_lvc(g, l0, l1, sL);
     const LcThread T = lc_thread(g, r);
     const int p = T.p;
@@ -1872,15 +1904,9 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
     // Whs at the bottom interface of the next level to be put
     double owl = ldb_d(rW, T.o8, (unsigned)(l0 * pl8)), hwl = ldb_d(rW, T.h8, (unsigned)(l0 * pl8));
     auto put = [&](const In &v, int b) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) sM[b][k][T.ir] = v.om[k];
-        sD[b][T.ir] = (float)(owl - v.ow);
+        sMD[b][T.ir] = make_float4(v.om[0], v.om[1], v.om[2], (float)(owl - v.ow));
         owl = v.ow;
-        if (T.hp >= 0) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) sM[b][k][T.hp] = v.hm[k];
-            sD[b][T.hp] = (float)(hwl - v.hw);
-        }
+        if (T.hp >= 0) sMD[b][T.hp] = make_float4(v.hm[0], v.hm[1], v.hm[2], (float)(hwl - v.hw));
         hwl = v.hw;
     };
     auto body = [&](int lev, In &R, In &L) {
@@ -1890,9 +1916,11 @@ __global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
         float a[7][3], d[7];
 #pragma unroll
         for (int s = 0; s < 7; s++) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) a[s][k] = sM[b][k][T.pos[s]];
-            d[s] = sD[b][T.pos[s]];
+            const float4 q = sMD[b][T.pos[s]];
+            a[s][0] = q.x;
+            a[s][1] = q.y;
+            a[s][2] = q.z;
+            d[s] = q.w;
         }
 #pragma unroll
         for (int c = 0; c < 3; c++) {
