@@ -211,3 +211,21 @@ def test_column_prep_level_window_is_bitwise_unchanged(built, deep, nonhydro, gh
             b = _two_steps(capi, g, p, st, {"THOR_COLPREP_LEVELS": lv})
             for k in a:
                 assert np.array_equal(a[k], b[k]), (prec, lv, k)
+
+
+def test_row_parallel_vertical_sweeps_stay_close(built):
+    """The mixed path's vertical solve with both sweeps on all rows of the
+    block (segment maps chained through LDS) against the one-lane sweeps:
+    two steps agree to FP64 rounding."""
+    capi = built
+    for nv in (20, 48):
+        g = grid(capi, 4, nv)
+        p = earth_params(capi, 4, dt=600.0, precision=1, HyDiff=1, DivDampP=1)
+        print(f"case: mixed g4 L{nv}, row-parallel vs one-lane vertical sweeps", flush=True)
+        st = capi.ic_rest(g, p, seed=1017, temperature_noise=1.0, wind_noise=2.0)
+        a = _two_steps(capi, g, p, st, {"THOR_THOMAS_ROWS": "0"})
+        b = _two_steps(capi, g, p, st, {})
+        for k in a:
+            err = float(np.abs(b[k] - a[k]).max() / np.abs(a[k]).max())
+            print(f"  L{nv} {k}: max rel diff {err:.2e}")
+            assert err < 1e-9, (nv, k)

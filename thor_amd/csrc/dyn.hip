@@ -322,7 +322,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
             // Vertical_Eq (12) + Density_Pressure_Eqs (22, of which Mhs and diffpr FP32 in mixed)
-            const size_t lds = (size_t)(d.nv + 1) * d.tcb * sizeof(double);
+            // the solve's plane plus the row maps of the mixed path's sweeps
+            const size_t lds = ((size_t)(d.nv + 1) * d.tcb + 2 * 256) * sizeof(double);
             const dim3 grid_th((d.P + d.tcb - 1) / d.tcb);
             if (!d.hooks.dyn_core_loop_fast_modes) {
                 launch(d, "thomas_dp", W(12 + 18, 4), [&] {
@@ -653,6 +654,8 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) d.slow_lc = d.slow_lc && std::atoi(e) != 0;
     d.prep_split = d.slow_lc;
     if (const char *e = std::getenv("THOR_DIFF2_CHUNKED")) d.diff2_lc = std::atoi(e) != 0;
+    d.f.thomas_rows = 1;
+    if (const char *e = std::getenv("THOR_THOMAS_ROWS")) d.f.thomas_rows = std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_COLPREP_LEVELS")) {
         const int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4) d.cp_lv = v;

@@ -70,6 +70,7 @@ struct Fields {
     double *Qheat, *dMh, *dWh;            // ProfX tendencies
     unsigned int *flags;                  // bit 0: NaN pressure, bit 1: NaN temperature
     int skip_w;                           // ProfX: W already set (by the sponge), do not recompute it from Wh
+    int thomas_rows;                      // mixed path: both sweeps of the vertical solve on all rows of the block
     // Mixed precision: FP32 storage of the momentum deviation, the advection,
     // hyperdiffusion and divergence-damping tendencies and the slow momentum
     // tendency (same layouts as their FP64 counterparts above).

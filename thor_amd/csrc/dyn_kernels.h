@@ -2658,6 +2658,105 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
 // LDS: all threads take a share of the levels again after the solve.
 // DP = false: the vertical solve alone (Whs written, density/pressure left to
 // k_dp), for a physics hook that runs between the two as in the reference.
+// Mixed path: both sweeps of the vertical solve on all rows of the block, each
+// row on a contiguous segment of at most kSeg interfaces of its column. The
+// forward elimination x_l = (-C_l - x_{l-1} AA_l) TT_l and the back
+// substitution w_l = -CC_l w_{l+1} + x_l are first-order linear recurrences:
+// each row composes its segment into one affine map, the rows' maps are
+// chained in order through LDS (sX: 2 x NROW x CB doubles), and each row then
+// runs its segment from its true starting value with the recurrence's own
+// arithmetic. The loads of the factorisation are spread over all rows and the
+// serial chain is a quarter of the one-lane sweep's. All threads call it.
+constexpr int kSeg = 12;
+template <int CB>
+__device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const Plane &AA, const Plane &CC,
+                                                   const Plane &TT, int nv, int c, int row, bool valid) {
+    constexpr int NROW = 256 / CB;
+    const int seg = (nv - 1 + NROW - 1) / NROW;
+    const int lo = 1 + row * seg, hi = min(nv, lo + seg);  // this row's interfaces [lo, hi)
+    double *sa = sX, *sb = sX + NROW * CB;
+    // forward elimination: the segment's map x_{hi-1} = a x_{lo-1} + b; the
+    // back substitution's coefficients are loaded here too, so both sweeps
+    // wait for memory once
+    double a = 1.0, b = 0.0;
+    double ccl[kSeg];
+    {
+        double al[kSeg], tl[kSeg], cl[kSeg];
+#pragma unroll
+        for (int i = 0; i < kSeg; i++) {
+            const int l = lo + i;
+            const bool in = valid && l < hi;
+            al[i] = in ? AA[l] : 0.0;
+            tl[i] = in ? TT[l] : 1.0;
+            ccl[i] = in ? CC[l] : 0.0;
+            cl[i] = in ? sD[l * CB + c] : 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kSeg; i++) {
+            const int l = lo + i;
+            if (l < hi) {
+                if (l == 1) {
+                    a = 0.0;
+                    b = (-cl[i]) / tl[i];
+                } else {
+                    const double m = -al[i] * tl[i];
+                    a = m * a;
+                    b = m * b + (-cl[i]) * tl[i];
+                }
+            }
+        }
+    }
+    sa[row * CB + c] = a;
+    sb[row * CB + c] = b;
+    __syncthreads();
+    double x = 0.0;  // x at interface lo - 1
+    for (int r = 0; r < row; r++) x = sa[r * CB + c] * x + sb[r * CB + c];
+    {
+        double al[kSeg], tl[kSeg];
+#pragma unroll
+        for (int i = 0; i < kSeg; i++) {
+            const int l = lo + i;
+            const bool in = valid && l < hi;
+            al[i] = in ? AA[l] : 0.0;
+            tl[i] = in ? TT[l] : 1.0;
+        }
+#pragma unroll
+        for (int i = 0; i < kSeg; i++) {
+            const int l = lo + i;
+            if (l < hi) {
+                x = l == 1 ? (-sD[CB + c]) / tl[i] : (-sD[l * CB + c] - x * al[i]) * tl[i];
+                sD[l * CB + c] = x;
+            }
+        }
+    }
+    __syncthreads();
+    // back substitution: the segment's map w_lo = e w_hi + f
+    double e = 1.0, fo = 0.0;
+    {
+#pragma unroll
+        for (int i = kSeg - 1; i >= 0; i--) {
+            const int l = lo + i;
+            if (l < hi) {
+                e = -ccl[i] * e;
+                fo = -ccl[i] * fo + sD[l * CB + c];
+            }
+        }
+        sa[row * CB + c] = e;
+        sb[row * CB + c] = fo;
+        __syncthreads();
+        double w = 0.0;  // w at interface hi (0 at the top, interface nv)
+        for (int r = NROW - 1; r > row; r--) w = sa[r * CB + c] * w + sb[r * CB + c];
+#pragma unroll
+        for (int i = kSeg - 1; i >= 0; i--) {
+            const int l = lo + i;
+            if (l < hi) {
+                w = (-ccl[i] * w + sD[l * CB + c]);
+                sD[l * CB + c] = w;
+            }
+        }
+    }
+}
+
 // Iterations of the serial sweeps unrolled together: their factorisation loads
 // do not depend on the recurrence and are issued ahead of it.
 #ifndef THOR_SWEEP_UNROLL
@@ -2720,7 +2819,18 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     // depend on the recurrence, so they are issued ahead of it). Staging the
     // factorisation in LDS instead measured no faster at g6 L48.
     const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
-    if (row == 0 && p < g.P && nv > 1) {
+    bool rows_done = false;
+    if constexpr (MX && NROW * kSeg >= 8) {
+        if (f.thomas_rows && nv > 1 && nv - 1 <= NROW * kSeg) {
+            thomas_sweeps_rows<CB>(sD, sm + (size_t)(nv + 1) * CB, AA, CC, TT, nv, c, row, p < g.P);
+            if (row == 0) {
+                sD[c] = 0.0;
+                sD[nv * CB + c] = 0.0;
+            }
+            rows_done = true;
+        }
+    }
+    if (!rows_done && row == 0 && p < g.P && nv > 1) {
         double ddp = (-sD[CB + c]) / TT[1];
         sD[CB + c] = ddp;
 #pragma unroll THOR_SWEEP_UNROLL
@@ -2729,7 +2839,7 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
             sD[lev * CB + c] = ddp;
         }
     }
-    if (row == 0 && p < g.P) {
+    if (!rows_done && row == 0 && p < g.P) {
         double wup = 0.0;
 #pragma unroll THOR_SWEEP_UNROLL
         for (int lev = nv - 1; lev > 0; lev--) {
