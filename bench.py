@@ -323,10 +323,14 @@ def at_scale_run(capi, args):
         gbs = step_bytes / (ms * 1e-3) / 1e9
         tot = sum(k["total_ms"] for k in ks)
         top = sorted(ks, key=lambda k: -k["total_ms"])[:6]
+        rt_names = ("profx_double_gray", "profx_double_gray_fallback", "surrogate_profx", "sponge_zonal",
+                    "sponge_mean", "sponge_apply")
         out[name] = {"ms_per_step": round(ms, 3), "sdpd": round(dt * nsteps / wall, 1),
                      "step_alg_bytes": round(step_bytes), "hbm_gbs_alg": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                      "top_kernels_us": {k["name"]: round(1e3 * k["total_ms"] / 10, 1) for k in top},
+                     "profx_kernels_us_per_step": {k["name"]: round(1e3 * k["total_ms"] / 10, 1) for k in ks
+                                                   if k["name"] in rt_names},
                      "kernel_sum_ms_per_step": round(tot / 10, 3)}
         if sstats is not None:
             out[name]["judge"] = {"calls": sstats["calls"], "judged": sstats["judged"],

@@ -71,6 +71,8 @@ struct Dyn {
     bool sponge_on = false;
     SpongePar spg{};
     int *ztab = nullptr;                                // [3][S]
+    int *zcnt = nullptr;                                // [nlat] points per latitude bin
+    double *ztrig = nullptr, *zkf = nullptr;            // [4][S] sin/cos of lon and lat, [2][nv] rates
     double *ztmp = nullptr, *zout = nullptr, *vbar = nullptr;
     // physics-module hooks and RK-advanced module arrays
     thor_hooks hooks{};
@@ -396,7 +398,7 @@ static int profx(Dyn &d, int forcing) {
         });
         if (sp.dmu || sp.dmw) {
             launch(d, "sponge_mean", 8.0 * 3 * sp.nlat * d.nv * sp.max_count, [&] {
-                k_sponge_reduce<<<dim3(sp.nb, d.nv, 3 * sp.nlat), 1024, 0, d.stream>>>(sp, d.nv, d.ztmp, d.zout);
+                k_sponge_reduce<<<d.nv * 3 * sp.nlat, 1024, 0, d.stream>>>(sp, d.nv, d.ztmp, d.zout);
                 k_sponge_vbar<<<(3 * sp.nlat * d.nv + 255) / 256, 256, 0, d.stream>>>(sp, d.nv, d.zout, d.vbar);
             });
         }
@@ -1019,23 +1021,39 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     SpongePar sp{q->ns, q->Ruv, q->Rw, nlat, q->damp_uv_to_mean != 0, q->damp_w_to_mean != 0, max_count,
                  (max_count + 2047) / 2048};
     // (re)allocate the tables; the zonal slots beyond each bin's count stay zero
-    for (void *ptr : {(void *)h->ztab, (void *)h->ztmp, (void *)h->zout, (void *)h->vbar}) {
+    for (void *ptr : {(void *)h->ztab, (void *)h->ztmp, (void *)h->zout, (void *)h->vbar, (void *)h->ztrig,
+                      (void *)h->zkf, (void *)h->zcnt}) {
         if (!ptr) continue;
         auto it = std::find(h->allocs.begin(), h->allocs.end(), ptr);
         if (it != h->allocs.end()) h->allocs.erase(it);
         (void)hipFree(ptr);
     }
-    h->ztab = nullptr;
-    h->ztmp = h->zout = h->vbar = nullptr;
+    h->ztab = h->zcnt = nullptr;
+    h->ztmp = h->zout = h->vbar = h->ztrig = h->zkf = nullptr;
     h->sponge_on = false;
     hipError_t e = hipMalloc((void **)&h->ztab, tab.size() * sizeof(int));
     if (e == hipSuccess) {
         h->allocs.push_back(h->ztab);
         e = hipMemcpy(h->ztab, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice);
     }
+    if (e == hipSuccess) e = hipMalloc((void **)&h->zcnt, count.size() * sizeof(int));
+    if (e == hipSuccess) {
+        h->allocs.push_back(h->zcnt);
+        e = hipMemcpy(h->zcnt, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = dalloc(*h, &h->ztmp, (size_t)3 * nlat * nv * max_count);
     if (e == hipSuccess) e = dalloc(*h, &h->zout, (size_t)3 * nlat * nv * sp.nb);
     if (e == hipSuccess) e = dalloc(*h, &h->vbar, (size_t)3 * nlat * nv);
+    if (e == hipSuccess) e = dalloc(*h, &h->ztrig, (size_t)4 * S);
+    if (e == hipSuccess) e = dalloc(*h, &h->zkf, (size_t)2 * nv);
+    if (e == hipSuccess) {
+        sp.trig = h->ztrig;
+        sp.kf = h->zkf;
+        sp.cnt = h->zcnt;
+        k_sponge_tables<<<(std::max(P, nv) + 255) / 256, 256, 0, h->stream>>>(h->g, sp, h->ztrig, h->zkf);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    }
     if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("thor_dyn_set_sponge: ") + hipGetErrorString(e));
     h->spg = sp;
     h->sponge_on = true;

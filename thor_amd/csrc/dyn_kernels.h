@@ -3252,10 +3252,10 @@ __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q,
 // p = rho Rd T, which closes ProfX (not on judge calls, where the exact RT
 // follows). The same values as k_sur_features + the inference kernel +
 // k_sur_apply. Needs nv <= 64 (the network's 64 padded outputs).
-template <int NH>
+template <int NH, int RT = 2>
 __global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, mlp::InferWeights w, int *mask,
                                                    int *nflag, double time_step, int set_p) {
-    constexpr int R = 32, LDH = NH + 8;
+    constexpr int R = 16 * RT, LDH = NH + 8;  // R columns per block
     __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
     __shared__ int sout[R];
     const int tid = threadIdx.x, r = tid & (R - 1), grp = tid / R;
@@ -3302,14 +3302,14 @@ __global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, ml
     }
     if (out) atomicOr(&sout[r], 1);
     __syncthreads();
-    float y[2][4];
-    mlp::mlp3_rows<NH>(bufA, bufB, w, y);
-    // bufB is free once layer 2 is done: stage the outputs there as [32][64] FP32
+    float y[RT][4];
+    mlp::mlp3_rows<NH, RT>(bufA, bufB, w, y);
+    // bufB is free once layer 2 is done: stage the outputs there as [R][64] FP32
     float *sY = reinterpret_cast<float *>(bufB);
     {
         const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = (lane >> 4) * 4;
 #pragma unroll
-        for (int ti = 0; ti < 2; ti++)
+        for (int ti = 0; ti < RT; ti++)
 #pragma unroll
             for (int e = 0; e < 4; e++) sY[(16 * ti + lk + e) * 64 + 16 * wave + li] = y[ti][e];
     }
@@ -3432,18 +3432,45 @@ __device__ __forceinline__ void cell_diag(const Geo &g, const Fields &f, long e,
 struct SpongePar {
     double ns, Ruv, Rw;
     int nlat, dmu, dmw, max_count, nb;  // nb = blocks of 2048 slots per row
+    const double *trig;                 // [4][S]: sin, cos of the longitude (in [0, 2 pi)) and of the latitude
+    const double *kf;                   // [2][nv]: the implicit rates of u, v and of w per level
+    const int *cnt;                     // [nlat]: points per latitude bin
 };
 
-__device__ __forceinline__ void sponge_uv(const Geo &g, const Fields &f, int p, int lev, double &u, double &v,
-                                          double &lon, double &lat) {
+// The per-point and per-level factors of the sponge, formed once with the
+// expressions of the per-cell formulas (so the results are unchanged).
+__global__ __launch_bounds__(256) void k_sponge_tables(Geo g, SpongePar sp, double *trig, double *kf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const size_t S = g.S;
     const int nv = g.nv;
-    lon = g.lonlat[p] < 0 ? g.lonlat[p] + 2.0 * M_PI : g.lonlat[p];
-    lat = g.lonlat[S + p];
+    if (i < g.P) {
+        const double lon = g.lonlat[i] < 0 ? g.lonlat[i] + 2.0 * M_PI : g.lonlat[i], lat = g.lonlat[S + i];
+        trig[i] = MSIN(lon);
+        trig[S + i] = MCOS(lon);
+        trig[2 * S + i] = MSIN(lat);
+        trig[3 * S + i] = MCOS(lat);
+    }
+    if (i < nv) {
+        const double n = g.Alt[i] / g.Alth[nv];
+        double kuv = 0.0, kw = 0.0;
+        if (n >= sp.ns) {
+            kuv = 1.0 * sp.Ruv * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
+            kw = 1.0 * sp.Rw * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
+        }
+        kf[i] = kuv;
+        kf[nv + i] = kw;
+    }
+}
+
+__device__ __forceinline__ void sponge_uv(const Geo &g, const Fields &f, const SpongePar &sp, int p, int lev,
+                                          double &u, double &v) {
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double sl = sp.trig[p], cl = sp.trig[S + p], sa = sp.trig[2 * S + p], ca = sp.trig[3 * S + p];
     const double rho = LV(f.Rho, lev);
     const double m0 = VEC(f.Mh, 0, lev, p), m1 = VEC(f.Mh, 1, lev, p), m2 = VEC(f.Mh, 2, lev, p);
-    u = (m0 * (-MSIN(lon)) + m1 * (MCOS(lon)) + m2 * 0) / rho;
-    v = (m0 * (-MSIN(lat) * MCOS(lon)) + m1 * (-MSIN(lat) * MSIN(lon)) + m2 * (MCOS(lat))) / rho;
+    u = (m0 * (-sl) + m1 * (cl) + m2 * 0) / rho;
+    v = (m0 * (-sa * cl) + m1 * (-sa * sl) + m2 * (ca)) / rho;
 }
 
 // ztab: [3][S] latitude bin, points in the bin, index within the bin;
@@ -3460,8 +3487,8 @@ __global__ __launch_bounds__(256) void k_sponge_zonal(Geo g, Fields f, SpongePar
     const double w = whl * intt + wht * intl;
     LV(f.W, lev) = w;
     if (!sp.dmu && !sp.dmw) return;
-    double u, v, lon, lat;
-    sponge_uv(g, f, p, lev, u, v, lon, lat);
+    double u, v;
+    sponge_uv(g, f, sp, p, lev, u, v);
     const int cnt = ztab[S + p];
     const size_t T = (size_t)sp.nlat * nv * sp.max_count;
     const size_t at = ((size_t)ztab[p] * nv + lev) * sp.max_count + ztab[2 * S + p];
@@ -3470,21 +3497,33 @@ __global__ __launch_bounds__(256) void k_sponge_zonal(Geo g, Fields f, SpongePar
     tmp[2 * T + at] = (w / LV(f.Rho, lev)) / cnt;
 }
 
-// grid (nb, nv, 3 nlat): block b of row (component, bin, level); out[row][b].
+// One block per row (component, bin, level): the row's 2048-slot blocks in
+// turn, each summed by the 1024-thread tree into out[row][b]. Slots at or
+// beyond the bin's count hold zero and are not read; a slot block with none
+// below the count has the sum +0.
 __global__ __launch_bounds__(1024) void k_sponge_reduce(SpongePar sp, int nv, const double *tmp, double *out) {
     __shared__ double ds[2048];
-    const int t = threadIdx.x, lev = blockIdx.y, cb = blockIdx.z;  // cb = component * nlat + bin
-    const long L = sp.max_count;
+    const int t = threadIdx.x, lev = blockIdx.x % nv, cb = blockIdx.x / nv;  // cb = component * nlat + bin
+    const long L = sp.max_count, n = min(L, (long)sp.cnt[cb % sp.nlat]);
     const double *row = tmp + ((size_t)cb * nv + lev) * L;
-    const long o = 2 * ((long)blockDim.x * blockIdx.x + t);
-    ds[2 * t] = o < L ? row[o] : 0.0;
-    ds[2 * t + 1] = o + 1 < L ? row[o + 1] : 0.0;
-    for (int stride = blockDim.x; stride > 0; stride /= 2) {
+    double *o = out + ((size_t)cb * nv + lev) * sp.nb;
+    for (int b = 0; b < sp.nb; b++) {
+        const long base = 2048L * b;
+        if (base >= n) {
+            if (t == 0) o[b] = 0.0;
+            continue;
+        }
+        const long i = base + 2 * t;
+        ds[2 * t] = i < n ? row[i] : 0.0;
+        ds[2 * t + 1] = i + 1 < n ? row[i + 1] : 0.0;
+        for (int stride = 1024; stride > 0; stride /= 2) {
+            __syncthreads();
+            if (t < stride) ds[t] += ds[t + stride];
+        }
         __syncthreads();
-        if (t < stride) ds[t] += ds[t + stride];
+        if (t == 0) o[b] = ds[0];
+        __syncthreads();
     }
-    __syncthreads();
-    if (t == 0) out[((size_t)cb * nv + lev) * gridDim.x + blockIdx.x] = ds[0];
 }
 
 // vbar[bin][lev][component] = 0 + the block sums in order (the host loop of
@@ -3504,14 +3543,14 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     if (p >= g.P) return;
     const size_t S = g.S;
     const int nv = g.nv, nlat = sp.nlat;
-    const double *Alt = g.Alt, *Alth = g.Alth;
     const int intlat = ztab[p];
     const double des_lat = M_PI / nlat;
     const double lat1 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat - 1);
     const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
     const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
-    double u, v, lon, lat;
-    sponge_uv(g, f, p, lev, u, v, lon, lat);
+    double u, v;
+    sponge_uv(g, f, sp, p, lev, u, v);
+    const double lat = g.lonlat[S + p];
     auto mean_at = [&](int c) {
         if (lat <= -M_PI / 2 + 0.5 * des_lat) return vbar[((size_t)intlat * nv + lev) * 3 + c];
         if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vbar[((size_t)intlat * nv + lev) * 3 + c];
@@ -3524,12 +3563,7 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     };
     const double rho = LV(f.Rho, lev);
     const double w = LV(f.W, lev) / rho;
-    const double n = Alt[lev] / Alth[nv];
-    double kuv = 0.0, kw = 0.0;
-    if (n >= sp.ns) {
-        kuv = 1.0 * sp.Ruv * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
-        kw = 1.0 * sp.Rw * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
-    }
+    const double kuv = sp.kf[lev], kw = sp.kf[nv + lev];
     double unew, vnew, wnew;
     if (sp.dmu) {
         unew = (u / dt + kuv * mean_at(0)) / (1.0 / dt + kuv);
@@ -3540,9 +3574,10 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     }
     if (sp.dmw) wnew = (w / dt + kw * mean_at(2)) / (1.0 / dt + kw);
     else wnew = (w / dt) / (1.0 / dt + kw);
-    const double vx = unew * (-MSIN(lon)) + vnew * (-MSIN(lat) * MCOS(lon));
-    const double vy = unew * (MCOS(lon)) + vnew * (-MSIN(lat) * MSIN(lon));
-    const double vz = vnew * (MCOS(lat));
+    const double sl = sp.trig[p], cl = sp.trig[S + p], sa = sp.trig[2 * S + p], ca = sp.trig[3 * S + p];
+    const double vx = unew * (-sl) + vnew * (-sa * cl);
+    const double vy = unew * (cl) + vnew * (-sa * sl);
+    const double vz = vnew * (ca);
     VEC(f.Mh, 0, lev, p) = vx * rho;
     VEC(f.Mh, 1, lev, p) = vy * rho;
     VEC(f.Mh, 2, lev, p) = vz * rho;

@@ -153,8 +153,8 @@ static void refresh_bf16(Net &n) {
         uint16_t *wt[3] = {n.wt1, n.wt2, n.wt3};
         for (int i = 0; i < 3; i++) {
             const long cnt = (long)Np[i] * Kp[i];
-            k_transpose_bf16<<<(unsigned)((cnt + 255) / 256), 256, 0, n.stream>>>(n.w + src[i], K[i], N[i], wt[i],
-                                                                                  Kp[i], Np[i]);
+            k_frag_bf16<<<(unsigned)((cnt + 255) / 256), 256, 0, n.stream>>>(n.w + src[i], K[i], N[i], wt[i], Kp[i],
+                                                                             Np[i]);
         }
     }
 }

@@ -150,10 +150,19 @@ template <int NH> __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g
 
 // dst[n][k] = bf16(src[k][n]) for k < K, n < N; zero in the padding up to
 // [Npad][Kpad].
-__global__ void k_transpose_bf16(const float *src, int K, int N, uint16_t *dst, int Kpad, int Npad) {
+// BF16 copy of W [K][N] (FP32, row-major) in the order the inference kernels
+// read their MFMA B operands: for each 16 x 16 tile (nt, kt) of W^T, lane l's
+// four values W[16 kt + 4 (l / 16) + e][16 nt + l % 16], e = 0..3, are at
+// ((nt * Kpad / 16 + kt) * 64 + l) * 4 + e, so a wave's operand load is 512
+// contiguous bytes. Zero beyond K and N.
+__global__ void k_frag_bf16(const float *src, int K, int N, uint16_t *dst, int Kpad, int Npad) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)Npad * Kpad) return;
-    const int n = (int)(i / Kpad), k = (int)(i % Kpad);
+    const int KT = Kpad / 16;
+    const int e = (int)(i & 3), l = (int)((i >> 2) & 63);
+    const long tile = i >> 8;
+    const int kt = (int)(tile % KT), nt = (int)(tile / KT);
+    const int n = 16 * nt + (l & 15), k = 16 * kt + 4 * (l >> 4) + e;
     dst[i] = (n < N && k < K) ? f2bf(src[(size_t)k * N + n]) : (uint16_t)0;
 }
 
