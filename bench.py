@@ -251,7 +251,7 @@ def _spinup_and_train(capi, args, nv):
     with the exact double-gray RT (mixed dynamics, dt 300 s, Rayleigh sponge)."""
     from thor_amd import surrogate as sg
     g, p = sg.hot_jupiter(5, nv, dt=300.0, ns=6, precision=1, cache_dir=CACHE)
-    d = capi.Dyn(g, p)
+    d = capi.Dyn(g, p, variant=args.variant)
     d.set_state(capi.ic_rest(g, p, seed=1003, temperature_noise=1.0, wind_noise=0.1))
     d.set_rt(sg.double_gray())
     d.set_sponge(sg.hot_jupiter_sponge())
@@ -261,7 +261,7 @@ def _spinup_and_train(capi, args, nv):
     log(f"g5 L{nv} spin-up: {args.rt_spinup_steps} steps ({args.rt_spinup_steps * 300.0 / 86400:.1f} days) with "
         f"exact RT, {len(X)} columns sampled, {time.perf_counter() - t0:.1f} s")
     norm = sg.normalisation(X, Y)
-    m = capi.MLP(2 * nv + 1, 256, nv, seed=5)
+    m = capi.MLP(2 * nv + 1, 256, nv, seed=5, variant=args.variant)
     t0 = time.perf_counter()
     losses = sg.train(m, *sg.apply_norm(X, Y, norm), epochs=args.rt_epochs, batch=512, lr=2e-3, log=log)
     err = sg.heating_error(m, X, Y, norm)
@@ -295,7 +295,7 @@ def at_scale_run(capi, args):
              "mixed_surrogate": (pmx, capi.FORCING_RT_SURROGATE)}
     out, final = {}, {}
     for name, (prm, forcing) in forms.items():
-        d = capi.Dyn(g, prm)
+        d = capi.Dyn(g, prm, variant=args.variant)
         d.set_state(st)
         d.set_rt(sg.double_gray())
         d.set_sponge(sg.hot_jupiter_sponge())

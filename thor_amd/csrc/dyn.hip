@@ -436,8 +436,9 @@ static int profx(Dyn &d, int forcing) {
         if (d.nv <= 64 && mlp::fused_weights(d.sur, &iw)) {
             // features, network and heating in one kernel
             launch(d, "surrogate_profx", 10 * B, [&] {
-                k_sur_fused<256><<<(d.P + 31) / 32, 256, 0, d.stream>>>(g, f, d.sp, iw, d.sur_mask, d.sur_nflag, dt,
-                                                                       judge ? 0 : 1);
+                // 32 columns per block (16 or 64 measured slower at g7)
+                k_sur_fused<256, 2><<<(d.P + 31) / 32, 256, 0, d.stream>>>(g, f, d.sp, iw, d.sur_mask, d.sur_nflag, dt,
+                                                                          judge ? 0 : 1);
             });
         } else {
         launch(d, "surrogate_features", 6 * B, [&] {

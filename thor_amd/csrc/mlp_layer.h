@@ -10,6 +10,12 @@
 namespace thor {
 namespace mlp {
 
+// K steps of an inference layer unrolled together (their operand loads are
+// issued ahead of the MFMAs)
+#ifndef THOR_MLP_KUNROLL
+#define THOR_MLP_KUNROLL 8
+#endif
+
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -27,7 +33,7 @@ __device__ __forceinline__ void infer_layer(const uint16_t *sA, int lda, int K, 
     for (int ti = 0; ti < RT; ti++)
 #pragma unroll
         for (int tj = 0; tj < NT; tj++) acc[ti][tj] = v4f{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 8
+#pragma unroll THOR_MLP_KUNROLL
     for (int k0 = 0; k0 < K; k0 += 16) {
         v4s a[RT], b[NT];
 #pragma unroll
