@@ -148,14 +148,16 @@ def test_chunked_slow_modes_equal_per_level_kernel(built, deep, nonhydro, ghv):
     for k in a:
         assert np.array_equal(a[k], b[k]), k
     # Default: the substep pressure flux divergence adds the stage part
-    # (formed by the slow-mode kernel) to the deviation part; the same sums
-    # grouped differently, so equal to FP64 rounding carried through two steps
-    # (the mixed form's distance to FP64 is 1e-4 in Mh).
+    # (formed by the slow-mode kernel) to the deviation part, whose h and pt
+    # factors are read as FP32 copies (they multiply the FP32 momentum
+    # deviation); the same sums grouped differently with one more FP32
+    # rounding, so equal to ~1e-8 carried through two steps, far below the
+    # mixed form's distance to FP64 (1e-4 in Mh).
     c = _two_steps(capi, g, p, st, {})
     for k in a:
         err = float(np.abs(c[k] - a[k]).max() / np.abs(a[k]).max())
         print(f"  split vs unsplit pressure flux, {k}: max rel diff {err:.2e}")
-        assert err < 1e-9, k
+        assert err < 1e-6, k
 
 
 def test_thomas_loop_unroll_is_bitwise_unchanged(built):

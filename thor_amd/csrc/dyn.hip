@@ -317,8 +317,9 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 else k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times);
             });
             // Prepare_Implicit_Vertical, plus the horizontal flux divergences of
-            // Density_Pressure_Eqs (booked with the column kernel below)
-            launch(d, "prepare", W(7, 3), [&] {
+            // Density_Pressure_Eqs (booked with the column kernel below); the
+            // split form reads h and pt as FP32 copies
+            launch(d, "prepare", MX && d.prep_split ? W(5, 5) : W(7, 3), [&] {
                 if (MX && d.prep_split) k_prepare_lc<true><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else if (MX) k_prepare_lc<false><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);
@@ -709,6 +710,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         };
         F(&f.mMhs, n3); F(&f.mAdv, n3); F(&f.mdiff, 6 * n1); F(&f.mdiffmh, n3);
         F(&f.mdiffw, n1); F(&f.mdiffrh, n1); F(&f.mdiffpr, n1); F(&f.mdivg, n3); F(&f.mSlowMh, n3);
+        F(&f.mhpt, 2 * n1);
         float *wlap = nullptr, *wdd = nullptr, *udd = nullptr, *lvc = nullptr;
         F(&wlap, (size_t)7 * S); F(&wdd, (size_t)63 * S); F(&udd, (size_t)21 * S); F(&lvc, (size_t)4 * nv);
         if (e == hipSuccess) {

@@ -75,6 +75,7 @@ struct Fields {
     // hyperdiffusion and divergence-damping tendencies and the slow momentum
     // tendency (same layouts as their FP64 counterparts above).
     float *mMhs, *mAdv, *mdiff, *mdiffmh, *mdiffw, *mdiffrh, *mdiffpr, *mdivg, *mSlowMh;
+    float *mhpt;  // [2][nv]: h and pt of the stage rounded to FP32, for the substep flux divergences
 };
 
 // Storage of the mode-dependent fields: FP64 in the FP64 path, FP32 in the

@@ -8,6 +8,8 @@
 // poles are one extra block that gathers its neighbours directly. Column
 // kernels (vertical sweeps, the Thomas solve) map one thread to one column.
 #pragma once
+#include <type_traits>
+
 #include "dyn_common.h"
 #include "bf16.h"
 #include "mlp_layer.h"
@@ -463,8 +465,14 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
         const double pressure = LV(pk, lev), rho = LV(Rho, lev);
         const double temperature = pressure / (rho * Rd);
         LV(f.T, lev) = temperature;
-        LV(f.h, lev) = Cv * temperature + pressure / rho;
-        LV(f.pt, lev) = (P_Ref / (Rd * rho)) * XPOW(pressure / P_Ref, CvoCp);
+        const double hv = Cv * temperature + pressure / rho;
+        const double ptv = (P_Ref / (Rd * rho)) * XPOW(pressure / P_Ref, CvoCp);
+        LV(f.h, lev) = hv;
+        LV(f.pt, lev) = ptv;
+        if constexpr (MX) {  // FP32 copies for the substep flux divergences of the deviation
+            f.mhpt[(size_t)lev * S + p] = (float)hv;
+            f.mhpt[((size_t)nv + lev) * S + p] = (float)ptv;
+        }
         // gtil at this centre: the sweep forms it at interface lev + 1 from the
         // interface values below and above, weighted at (Alt, Alth, Alth+1) of
         // that sweep step
@@ -655,6 +663,10 @@ template <bool MX, int LV> __global__ __launch_bounds__(256) void k_column_prep_
             LV(f.T, lev) = wT[j];
             LV(f.h, lev) = wh[j];
             LV(f.pt, lev) = wpt[j];
+            if constexpr (MX) {
+                f.mhpt[(size_t)lev * S + p] = (float)wh[j];
+                f.mhpt[((size_t)nv + lev) * S + p] = (float)wpt[j];
+            }
             (void)pressure;
             (void)rho;
             double intt, intl;
@@ -1635,7 +1647,8 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
     }
     // staged point: (Mhs, 0) as one 16-byte LDS word and (h, pt) as another
     __shared__ float4 sM4[2][NHL2];
-    __shared__ double2 sHT[2][NHL2];
+    using HT = std::conditional_t<SPLIT, float2, double2>;
+    __shared__ HT sHT[2][NHL2];
     __shared__ double sV1[2][SPLIT ? 1 : 3][NHL2], sRs[kLcMax];
     if ((int)threadIdx.x < l1 - l0) sRs[threadIdx.x] = rscale_at(g, l0 + threadIdx.x);
     const LcThread T = lc_thread(g, r);
@@ -1646,11 +1659,12 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
     const size_t pl4 = S * 4, pl8 = S * 8;  // bytes of one FP32 / FP64 level plane
     const auto rM = rsrc(Mhs, 3 * nv * pl4), rMk = rsrc(f.Mhk, 3 * nv * pl8), rH = rsrc(f.h, nv * pl8),
                rPt = rsrc(f.pt, nv * pl8), rSp = rsrc(f.Slowp, nv * pl8), rSr = rsrc(f.SlowRho, nv * pl8),
-               rPk = rsrc(f.hflx_ptk, nv * pl8);
+               rPk = rsrc(f.hflx_ptk, nv * pl8), rHP = rsrc(f.mhpt, 2 * nv * pl4);
     struct In {
         double slowp, slowrho;
         float om[3], hm[3];
-        double omk[SPLIT ? 1 : 3], hmk[SPLIT ? 1 : 3], oh, hh, opt, hpt;
+        double omk[SPLIT ? 1 : 3], hmk[SPLIT ? 1 : 3];
+        std::conditional_t<SPLIT, float, double> oh, hh, opt, hpt;
     };
     auto load = [&](In &v, int lev) {
         v.slowp = ldb_d(rSp, T.o8, (unsigned)(lev * pl8));
@@ -1665,21 +1679,28 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
                 v.hmk[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
             }
         }
-        v.oh = ldb_d(rH, T.o8, (unsigned)(lev * pl8));
-        v.opt = ldb_d(rPt, T.o8, (unsigned)(lev * pl8));
-        v.hh = ldb_d(rH, T.h8, (unsigned)(lev * pl8));
-        v.hpt = ldb_d(rPt, T.h8, (unsigned)(lev * pl8));
+        if constexpr (SPLIT) {  // h and pt of the stage in FP32 (they multiply the FP32 deviation)
+            v.oh = ldb_f(rHP, T.o4, (unsigned)(lev * pl4));
+            v.opt = ldb_f(rHP, T.o4, (unsigned)((nv + lev) * pl4));
+            v.hh = ldb_f(rHP, T.h4, (unsigned)(lev * pl4));
+            v.hpt = ldb_f(rHP, T.h4, (unsigned)((nv + lev) * pl4));
+        } else {
+            v.oh = ldb_d(rH, T.o8, (unsigned)(lev * pl8));
+            v.opt = ldb_d(rPt, T.o8, (unsigned)(lev * pl8));
+            v.hh = ldb_d(rH, T.h8, (unsigned)(lev * pl8));
+            v.hpt = ldb_d(rPt, T.h8, (unsigned)(lev * pl8));
+        }
     };
     auto put = [&](const In &v, int b) {
         sM4[b][T.ir] = make_float4(v.om[0], v.om[1], v.om[2], 0.f);
-        sHT[b][T.ir] = make_double2(v.oh, v.opt);
+        sHT[b][T.ir] = HT{v.oh, v.opt};
         if constexpr (!SPLIT) {
 #pragma unroll
             for (int k = 0; k < 3; k++) sV1[b][k][T.ir] = (double)v.om[k] + v.omk[k];
         }
         if (T.hp >= 0) {
             sM4[b][T.hp] = make_float4(v.hm[0], v.hm[1], v.hm[2], 0.f);
-            sHT[b][T.hp] = make_double2(v.hh, v.hpt);
+            sHT[b][T.hp] = HT{v.hh, v.hpt};
             if constexpr (!SPLIT) {
 #pragma unroll
                 for (int k = 0; k < 3; k++) sV1[b][k][T.hp] = (double)v.hm[k] + v.hmk[k];
@@ -1696,7 +1717,7 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
 #pragma unroll
         for (int s = 0; s < 7; s++) {
             const float4 mq = sM4[b][T.pos[s]];
-            const double2 hq = sHT[b][T.pos[s]];
+            const HT hq = sHT[b][T.pos[s]];
             const double m[3] = {mq.x, mq.y, mq.z};
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -1704,12 +1725,12 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
                 const double v1 = SPLIT ? m[k] : sV1[b][SPLIT ? 0 : k][T.pos[s]];
                 if (s == 0) {
                     sr[k] = dd * m[k];
-                    sp[k] = dd * m[k] * hq.x;
-                    st[k] = dd * v1 * hq.y;
+                    sp[k] = dd * m[k] * (double)hq.x;
+                    st[k] = dd * v1 * (double)hq.y;
                 } else {
                     sr[k] = sr[k] + dd * m[k];
-                    sp[k] = sp[k] + dd * m[k] * hq.x;
-                    st[k] = st[k] + dd * v1 * hq.y;
+                    sp[k] = sp[k] + dd * m[k] * (double)hq.x;
+                    st[k] = st[k] + dd * v1 * (double)hq.y;
                 }
             }
         }
