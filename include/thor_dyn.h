@@ -102,7 +102,7 @@ typedef struct thor_params {
     double DivDampc; /* divergence-damping coefficient */
     double timestep; /* large time step [s] */
     int glevel;
-    int HyDiff, HyDiffOrder, DivDampP;
+    int HyDiff, HyDiffOrder, DivDampP; /* HyDiffOrder: even, >= 4 */
     int DeepModel, NonHydro, GravHeightVar;
     int n_substeps; /* acoustic substeps per large step (reference: 6) */
     int precision;  /* THOR_PRECISION_FP64 or THOR_PRECISION_MIXED */

@@ -5,6 +5,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from tests.common import ROOT
 
@@ -46,3 +47,15 @@ def test_bad_arguments_are_rejected(built):
     else:
         raise AssertionError("glevel 3 accepted")
     assert "glevel" in built.lib().thor_last_error().decode() or True
+
+
+@pytest.mark.parametrize("order", [2, 5])
+def test_unsupported_hyperdiffusion_order_is_rejected(built, order):
+    """Orders below 4 or odd orders are refused before any device call."""
+    from tests.common import earth_params, grid
+    g = grid(built, 4, 8)
+    p = earth_params(built, 4, HyDiffOrder=order)
+    with pytest.raises(built.ThorError) as e:
+        built.Dyn(g, p)
+    assert e.value.code == built.THOR_ERR_UNSUPPORTED
+    assert "HyDiffOrder" in str(e.value)

@@ -108,6 +108,31 @@ def test_mixed_step_stays_close_to_fp64(setup):
         assert err < t, k
 
 
+def test_mixed_order_six_stays_close_to_fp64(built):
+    """HyDiffOrder 6: the intermediate pass runs on the FP32 weights in the
+    mixed form; three steps stay within the order-4 mixed-vs-FP64 bounds."""
+    capi = built
+    g = grid(capi, 4, 16)
+    print("case: g4 L16, HyDiffOrder 6, 3 steps mixed vs FP64", flush=True)
+    p64 = earth_params(capi, 4, dt=600.0, HyDiffOrder=6)
+    st = capi.ic_rest(g, p64, seed=1001, temperature_noise=1e-3, wind_noise=1.0)
+    out = {}
+    for prec in (0, 1):
+        d = capi.Dyn(g, earth_params(capi, 4, dt=600.0, precision=prec, HyDiffOrder=6))
+        d.set_state(st)
+        d.run(3)
+        d.sync()
+        out[prec] = d.get_state()
+        d.close()
+    tol = {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}
+    for k, t in tol.items():
+        a, b = out[1][k], out[0][k]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(f"3 steps mixed vs FP64, HyDiffOrder 6, {k}: max rel diff {err:.2e}")
+        assert np.isfinite(a).all()
+        assert err < t, k
+
+
 def _two_steps(capi, g, p, st, env):
     """Two HS steps of a Dyn created with the given tuning variables set."""
     import os

@@ -73,3 +73,24 @@ def test_native_step_with_winds_g4(built):
     d.sync()
     o.steps(1)
     _compare(d, o, TOL_STEP, "native wind g4", TOL_VERTICAL)
+
+
+@pytest.mark.parametrize("order", [6, 8])
+def test_pm_higher_order_hyperdiffusion_g4(built, order):
+    """HyDiffOrder / 2 - 2 intermediate Laplacian passes between the first and
+    the last (each reading the previous pass's whole output)."""
+    print(f"case: parity build, g4 L20, HyDiffOrder {order}, one Thor step with 1 m/s winds", flush=True)
+    g, p, st, d, o = _pair(built, 4, 20, variant="pm", wnoise=1.0, HyDiffOrder=order)
+    d.thor()
+    d.sync()
+    o.thor()
+    _compare(d, o, TOL_PM, f"pm HyDiffOrder {order}")
+
+
+def test_native_order_six_hyperdiffusion_g4(built):
+    print("case: production build, g4 L20, HyDiffOrder 6, 2 x (ProfX HS + Thor) with 1 m/s winds", flush=True)
+    g, p, st, d, o = _pair(built, 4, 20, wnoise=1.0, HyDiffOrder=6)
+    d.run(2)
+    d.sync()
+    o.steps(2)
+    _compare(d, o, TOL_STEP, "native HyDiffOrder 6", TOL_VERTICAL)

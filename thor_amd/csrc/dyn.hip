@@ -270,17 +270,29 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             if (MX) k_adv_diff1_lc<<<grid_lc, blk, 0, st>>>(g, f, hydiff, lc);
             else k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff);
         });
+        // HyDiffOrder / 2 - 2 intermediate passes (HyDiffOrder >= 6) between
+        // the first pass (above) and the last (below); fd points the last pass
+        // at the output of the final one
+        Fields fd = f;
+        if (hydiff) {
+            for (int ih = 0; ih < d.prm.HyDiffOrder / 2 - 2; ih++) {
+                auto *src = FS<MX>::diff(fd);
+                auto *dst = ih % 2 ? FS<MX>::diff(f) : FS<MX>::diffx(f);
+                launch(d, "diff_mid", W(0, 12), [&] { k_diff_mid<MX><<<grid_t2, blk, 0, st>>>(g, src, dst); });
+                FS<MX>::set_diff(fd, dst);
+            }
+        }
         if (MX && d.diff2_lc) {
             // the last hyperdiffusion pass and the stage's divergence-damping
             // pass 0, each over level chunks
             if (hydiff)
-                launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, f, d.diff_sign, lc); });
+                launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, fd, d.diff_sign, lc); });
             if (divdamp)
                 launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_lc<true><<<grid_lc, blk, 0, st>>>(g, f, lc); });
         } else if (hydiff || divdamp) {
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
                    [&] {
-                k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff, divdamp, d.diff_sign);
+                k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, fd, hydiff, divdamp, d.diff_sign);
             });
         }
         if (d.hooks.dyn_core_loop_slow_modes) {
@@ -542,9 +554,8 @@ struct thor_dyn : Dyn {};
 extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_params *prm, thor_dyn **out) {
     using namespace thor;
     if (!gd || !prm || !out) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: null argument");
-    if (prm->HyDiff && prm->HyDiffOrder != 4)
-        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: only HyDiffOrder = 4 is supported (the reference's "
-                                          "intermediate hyperdiffusion passes race, thor_diff.h:214-217)");
+    if (prm->HyDiff && (prm->HyDiffOrder < 4 || prm->HyDiffOrder % 2))
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: HyDiffOrder must be even and >= 4");
     if (prm->n_substeps < 2 || prm->n_substeps % 2)
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: n_substeps must be even and >= 2");
     if (prm->precision != THOR_PRECISION_FP64 && prm->precision != THOR_PRECISION_MIXED)
@@ -693,6 +704,8 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     A(&f.Sp, n1); A(&f.Sd, n1); A(&f.hflx_r, n1); A(&f.hflx_pt, n1); A(&f.tAA, n1); A(&f.tCC, n1); A(&f.tTT, n1);
     A(&f.Qheat, n1); A(&f.dMh, n3); A(&f.dWh, ni);
     if (d.mixed) A(&f.hflx_ptk, n1);
+    const bool diff_mid = prm->HyDiff && prm->HyDiffOrder >= 6;  // intermediate passes alternate two arrays
+    if (diff_mid && !d.mixed) A(&f.diffx, 6 * n1);
     const long ncell = (long)P * nv;
     d.npart = (int)((ncell + 2047) / 2048);
     A(&d.partials, (size_t)5 * d.npart);
@@ -711,6 +724,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         F(&f.mMhs, n3); F(&f.mAdv, n3); F(&f.mdiff, 6 * n1); F(&f.mdiffmh, n3);
         F(&f.mdiffw, n1); F(&f.mdiffrh, n1); F(&f.mdiffpr, n1); F(&f.mdivg, n3); F(&f.mSlowMh, n3);
         F(&f.mhpt, 2 * n1);
+        if (diff_mid) F(&f.mdiffx, 6 * n1);
         float *wlap = nullptr, *wdd = nullptr, *udd = nullptr, *lvc = nullptr;
         F(&wlap, (size_t)7 * S); F(&wdd, (size_t)63 * S); F(&udd, (size_t)21 * S); F(&lvc, (size_t)4 * nv);
         if (e == hipSuccess) {

@@ -75,6 +75,8 @@ struct Fields {
     // hyperdiffusion and divergence-damping tendencies and the slow momentum
     // tendency (same layouts as their FP64 counterparts above).
     float *mMhs, *mAdv, *mdiff, *mdiffmh, *mdiffw, *mdiffrh, *mdiffpr, *mdivg, *mSlowMh;
+    double *diffx;  // [6][nv]: second hyperdiffusion array for HyDiffOrder >= 6 (passes alternate)
+    float *mdiffx;
     float *mhpt;  // [2][nv]: h and pt of the stage rounded to FP32, for the substep flux divergences
 };
 
@@ -86,6 +88,8 @@ template <> struct FS<false> {
     static __host__ __device__ double *Mhs(const Fields &f) { return f.Mhs; }
     static __host__ __device__ double *Adv(const Fields &f) { return f.Adv; }
     static __host__ __device__ double *diff(const Fields &f) { return f.diff; }
+    static __host__ __device__ double *diffx(const Fields &f) { return f.diffx; }
+    static __host__ __device__ void set_diff(Fields &f, double *a) { f.diff = a; }
     static __host__ __device__ double *diffmh(const Fields &f) { return f.diffmh; }
     static __host__ __device__ double *diffw(const Fields &f) { return f.diffw; }
     static __host__ __device__ double *diffrh(const Fields &f) { return f.diffrh; }
@@ -98,6 +102,8 @@ template <> struct FS<true> {
     static __host__ __device__ float *Mhs(const Fields &f) { return f.mMhs; }
     static __host__ __device__ float *Adv(const Fields &f) { return f.mAdv; }
     static __host__ __device__ float *diff(const Fields &f) { return f.mdiff; }
+    static __host__ __device__ float *diffx(const Fields &f) { return f.mdiffx; }
+    static __host__ __device__ void set_diff(Fields &f, float *a) { f.mdiff = a; }
     static __host__ __device__ float *diffmh(const Fields &f) { return f.mdiffmh; }
     static __host__ __device__ float *diffw(const Fields &f) { return f.mdiffw; }
     static __host__ __device__ float *diffrh(const Fields &f) { return f.mdiffrh; }

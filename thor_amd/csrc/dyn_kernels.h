@@ -976,6 +976,50 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int di
         [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; });
 }
 
+// Intermediate hyperdiffusion pass of HyDiffOrder >= 6 (Diffusion_Op with
+// stepnum > 0 and laststep false, thor_driver.cu:435-470, argument
+// thor_diff.h:218-221): the Laplacian of each of the six fields of the
+// previous pass, unweighted and unscaled. THOR reads and writes diff_d in
+// place in this pass, so a block's halo may already hold this pass's values;
+// here each pass reads src and writes dst (the caller alternates two arrays),
+// so every value is the Laplacian of the previous pass's output.
+template <bool MX>
+__global__ STENCIL_BOUNDS void k_diff_mid(Geo g, const typename FS<MX>::R *src, typename FS<MX>::R *dst) {
+    int r, lev;
+    if (!block_tile(g, r, lev)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double rs = rscale_at(g, lev);
+    auto lap = [&](int p, int nf, auto a) {
+        if constexpr (MX) {
+            float o[6];
+            laplacians_w<6>(g, p, (float)(rs * rs), a, o);
+#pragma unroll
+            for (int var = 0; var < 6; var++) VEC(dst, var, lev, p) = o[var];
+        } else {
+            double o[6];
+            laplacians<6>(g, p, nf, rs, a, o);
+#pragma unroll
+            for (int var = 0; var < 6; var++) VEC(dst, var, lev, p) = o[var];
+        }
+    };
+    if (r == g.nr) {
+        PoleCtx c;
+        if (!pole_ctx(g, c)) return;
+        lap(c.p, 5, [&](int var, int s) { return VEC(src, var, lev, c.gi(s)); });
+        return;
+    }
+    using R = typename FS<MX>::R;
+    __shared__ R sD[6][NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile(g, r, T, [&](int pos, int q) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) sD[k][pos] = q < 0 ? (R)0 : VEC(src, k, lev, q);
+    });
+    __syncthreads();
+    lap(T.p, T.nf, [&](int var, int s) { return sD[var][T.pos(s)]; });
+}
+
 // ===================================================================== K_D
 // Divergence-damping pass 1 on the stage values + Compute_Slow_Modes / _Poles
 // (thor_slowmodes.h:48-683, entropy form) + UpdateRK (thor_auxiliary.h:311-418)
