@@ -106,6 +106,12 @@ typedef struct thor_params {
     int DeepModel, NonHydro, GravHeightVar;
     int n_substeps; /* acoustic substeps per large step (reference: 6) */
     int precision;  /* THOR_PRECISION_FP64 or THOR_PRECISION_MIXED */
+    /* Vertical hyperdiffusion (SimulationSetup::VertHyDiff, VertHyDiffOrder,
+     * Diffc_v, simulation_setup.h:81-90): runs with HyDiff, as in THOR
+     * (thor_driver.cu:590-612); Kdv6 = Diffc_v dz^order / timestep
+     * (esp_initial.cu:1327-1334). VertHyDiffOrder: even, >= 4. */
+    int VertHyDiff, VertHyDiffOrder;
+    double Diffc_v;
 } thor_params;
 
 /* Fills p with THOR's defaults for the Earth Held-Suarez case (ifile/earth_hstest.thr). */

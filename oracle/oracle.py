@@ -38,7 +38,8 @@ class _Params(C.Structure):
                 ("Rd", C.c_double), ("Cp", C.c_double), ("Diffc", C.c_double), ("DivDampc", C.c_double),
                 ("timestep", C.c_double), ("glevel", C.c_int), ("HyDiffOrder", C.c_int),
                 ("n_substeps", C.c_int), ("DeepModel", C.c_int), ("NonHydro", C.c_int), ("HyDiff", C.c_int),
-                ("DivDampP", C.c_int), ("GravHeightVar", C.c_int)]
+                ("DivDampP", C.c_int), ("GravHeightVar", C.c_int), ("VertHyDiff", C.c_int),
+                ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double)]
 
 
 _libs: dict = {}
@@ -102,7 +103,7 @@ def _structs(grid, params):
     p = _Params(params.A, params.Omega, params.Gravit, params.P_Ref, params.Rd, params.Cp,
                 params.Diffc, params.DivDampc, params.timestep, params.glevel, params.HyDiffOrder,
                 params.n_substeps, params.DeepModel, params.NonHydro, params.HyDiff, params.DivDampP,
-                params.GravHeightVar)
+                params.GravHeightVar, params.VertHyDiff, params.VertHyDiffOrder, params.Diffc_v)
     return g, p
 
 

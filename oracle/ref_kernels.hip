@@ -4,7 +4,7 @@
 // tree, so tests can check the serial restatement in thor_oracle.cpp kernel
 // by kernel. The kernels come from the reference headers
 //   src/headers/dyn/thor_adv_cor.h    Compute_Advec_Cori1, _Poles, Compute_Advec_Cori2
-//   src/headers/dyn/thor_diff.h       Diffusion_Op, _Poles, Correct_Horizontal
+//   src/headers/dyn/thor_diff.h       Diffusion_Op, _Poles, Correct_Horizontal, vertical_diff
 //   src/headers/dyn/thor_div.h        DivM_Op, _Poles
 //   src/headers/dyn/thor_slowmodes.h  Compute_Slow_Modes, _Poles
 //   src/headers/phy/profx_held_suarez.h  held_suarez
@@ -102,7 +102,8 @@ void *ref_create(const oracle_grid *g, const oracle_params *p) {
     for (const char *n : {"Mh", "profx_dMh", "Mhk", "Adv", "v", "SlowMh", "diffmh", "diffmv", "DivM", "divg_Mh"})
         r->add(n, 3 * N);
     for (const char *n : {"diff", "diff_sponge", "boundary_flux"}) r->add(n, 6 * N);
-    for (const char *n : {"Kdh4", "Kdhz", "Kdh2"}) r->add(n, r->nv);
+    for (const char *n : {"Kdh4", "Kdhz", "Kdh2", "Kdv6"}) r->add(n, r->nv);
+    r->add("diff2", 6 * P * (r->nv + 2));  // vertical_diff's ghost-extended columns
     if (!r->ok) {
         delete r;
         return nullptr;
@@ -165,6 +166,11 @@ int ref_kernel(void *h, const char *name, int a0, int a1) {
                                            r["nvecte"], r["Kdh4"], r["Altitude"], r["Altitudeh"], A, r["Rd"],
                                            r["Cp"], r.point_local, P, stepnum, last, deep, false, 4, r["Kdh2"],
                                            r["boundary_flux"], false, r.p.HyDiffOrder);
+    } else if (n == "vertical_diff") {
+        const dim3 NBALL0(P / NTH + 1, 1, 6);  // thor_driver.cu:86
+        vertical_diff<<<NBALL0, NTH>>>(r["diffmv"], r["diffwv"], r["diffrv"], r["diffprv"], r["diff"], r["diff2"],
+                                       r["Mhk"], r["Rhok"], r["temperature"], r["Wk"], r["Kdv6"], r["Altitude"],
+                                       r["Altitudeh"], A, r.p.Rd, r.p.VertHyDiffOrder, deep, P, nv);
     } else if (n == "correct_horizontal") {
         Correct_Horizontal<<<NBALL1, NTH>>>(r["diffmh"], r["diffmv"], r["func_r"], P);
     } else if (n == "divm") {

@@ -44,7 +44,7 @@ struct Oracle {
     oracle_params p;
     int P, nv, nvi, nr;
     std::map<std::string, std::vector<double>> arr;
-    std::vector<double> Kdh4, Kdhz;
+    std::vector<double> Kdh4, Kdhz, Kdv6;
     double nrq[8] = {0}, rtq[11] = {0};  // forcing parameters
     // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean
     bool sponge_on = false;
@@ -502,18 +502,100 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
             }
 }
 
-// Correct_Horizontal (thor_diff.h:882-915); diffmv is identically zero here.
+// Correct_Horizontal (thor_diff.h:882-915): the radial component is removed
+// from the horizontal and the vertical momentum diffusion (the latter is zero
+// unless VertHyDiff).
 void correct_horizontal(Oracle &o) {
     const int nv = o.nv;
-    double *dm = o.A_("diffmh");
     const double *fr = o.g.func_r;
-    for (int id = 0; id < o.P; id++)
-        for (int lev = 0; lev < nv; lev++) {
-            double *d = dm + (id * nv + lev) * 3;
-            double r = fr[id * 3 + 0] * d[0] + fr[id * 3 + 1] * d[1] + fr[id * 3 + 2] * d[2];
-            d[0] += -fr[id * 3 + 0] * r;
-            d[1] += -fr[id * 3 + 1] * r;
-            d[2] += -fr[id * 3 + 2] * r;
+    for (const char *name : {"diffmh", "diffmv"}) {
+        double *dm = o.A_(name);
+        for (int id = 0; id < o.P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                double *d = dm + (id * nv + lev) * 3;
+                double r = fr[id * 3 + 0] * d[0] + fr[id * 3 + 1] * d[1] + fr[id * 3 + 2] * d[2];
+                d[0] += -fr[id * 3 + 0] * r;
+                d[1] += -fr[id * 3 + 1] * r;
+                d[2] += -fr[id * 3 + 2] * r;
+            }
+    }
+}
+
+// vertical_diff (thor_diff.h:1198-1426, called at thor_driver.cu:590-612):
+// per column and variable (0 rho, 1-3 the momentum components over rho,
+// 4 W over rho, 5 T), a vertical Laplacian in flux form with r^2 weights,
+// ghost levels below the first and above the last layer holding copies of
+// the layer next to them, applied VertHyDiffOrder / 2 times. Before the last
+// application the field is multiplied by sign Kdv6 (and by rho for
+// variables 1-5), sign = (-1)^(order/2 + 1); the T result is multiplied by
+// Rd. The density result is not used (diffrv = 0, include_rho = false).
+void vertical_diff(Oracle &o) {
+    const int nv = o.nv, P = o.P, order = o.p.VertHyDiffOrder;
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *Kv = o.Kdv6.data();
+    const double *Rho = o.A_("Rhok"), *Mh = o.A_("Mhk"), *W = o.A_("Wk"), *T = o.A_("temperature");
+    double *dmv = o.A_("diffmv"), *dwv = o.A_("diffwv"), *drv = o.A_("diffrv"), *dpv = o.A_("diffprv");
+    const double sign = MPOW(-1.0, order / 2 + 1), Rd = o.p.Rd;
+    // one application at layer lev of the ghost-extended column a[0..nv+1]
+    // (a[lev + 1] is layer lev): flux difference over the layer, / r^2
+    auto flux_div = [&](const std::vector<double> &a, int lev) {
+        const double dzi = 1.0 / (Alth[lev + 1] - Alth[lev]);
+        double ri = 1.0, rup = 1.0, rlow = 1.0;
+        if (o.p.DeepModel) {
+            ri = o.p.A + Alt[lev];
+            rup = o.p.A + Alth[lev + 1];
+            rlow = o.p.A + Alth[lev];
+        }
+        double dzup, dzlow;
+        if (lev == 0) {
+            dzup = 1.0 / (Alt[1] - Alt[0]);
+            dzlow = 1.0 / (2.0 * Alt[0]);
+        } else if (lev == nv - 1) {
+            dzup = 1.0 / (2 * (Alth[lev + 1] - Alt[lev]));
+            dzlow = 1.0 / (Alt[lev] - Alt[lev - 1]);
+        } else {
+            dzup = 1.0 / (Alt[lev + 1] - Alt[lev]);
+            dzlow = 1.0 / (Alt[lev] - Alt[lev - 1]);
+        }
+        const double flux = MPOW(rup, 2) * (a[lev + 2] - a[lev + 1]) * dzup - MPOW(rlow, 2) * (a[lev + 1] - a[lev]) * dzlow;
+        return std::make_pair(flux * dzi, MPOW(ri, 2));
+    };
+    std::vector<double> a(nv + 2), d(nv);
+    for (int id = 0; id < P; id++)
+        for (int var = 0; var < 6; var++) {
+            auto value = [&](int lev) {
+                const long e = (long)id * nv + lev;
+                const double r = Rho[e];
+                if (var == 0) return r;
+                if (var <= 3) return Mh[e * 3 + var - 1] / r;
+                if (var == 4) return W[e] / r;
+                return T[e];
+            };
+            for (int lev = 0; lev < nv; lev++) a[lev + 1] = value(lev);
+            a[0] = a[1];
+            a[nv + 1] = a[nv];
+            const int passes = order / 2 - 1;
+            for (int l = 0; l < passes; l++) {
+                for (int lev = 0; lev < nv; lev++) {
+                    auto q = flux_div(a, lev);
+                    d[lev] = q.first / q.second;
+                }
+                for (int lev = 0; lev < nv; lev++) {
+                    if (l < passes - 1) a[lev + 1] = d[lev];
+                    else if (var == 0) a[lev + 1] = sign * Kv[lev] * d[lev];
+                    else a[lev + 1] = sign * Rho[(long)id * nv + lev] * Kv[lev] * d[lev];
+                }
+                a[0] = a[1];
+                a[nv + 1] = a[nv];
+            }
+            for (int lev = 0; lev < nv; lev++) {
+                auto q = flux_div(a, lev);
+                const double out = var != 5 ? q.first / q.second : q.first * Rd / q.second;
+                const long e = (long)id * nv + lev;
+                if (var == 0) drv[e] = 0.0;
+                else if (var <= 3) dmv[e * 3 + var - 1] = out;
+                else if (var == 4) dwv[e] = out;
+                else dpv[e] = out;
+            }
         }
 }
 
@@ -647,8 +729,9 @@ void divm_pass(Oracle &o, const char *Mname, const char *Whname, bool laststep) 
 }
 
 // -------------------------------------------------------------- slow modes
-// Compute_Slow_Modes + _Poles (thor_slowmodes.h:48-683), entropy form; the
-// vertical-diffusion terms diffmv/diffwv/diffrv/diffprv are identically zero.
+// Compute_Slow_Modes + _Poles (thor_slowmodes.h:48-683), entropy form, with
+// the vertical-diffusion terms diffmv/diffwv/diffrv/diffprv (zero unless
+// VertHyDiff).
 void slow_modes(Oracle &o) {
     const int nv = o.nv, P = o.P;
     double *SlowMh = o.A_("SlowMh"), *SlowWh = o.A_("SlowWh"), *SlowRho = o.A_("SlowRho"),
@@ -657,6 +740,8 @@ void slow_modes(Oracle &o) {
            *diffrh = o.A_("diffrh"), *diffpr = o.A_("diffpr"), *pk = o.A_("pressurek"), *h_d = o.A_("h"),
            *hh = o.A_("hh"), *gtil = o.A_("gtil"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd"),
            *dMh = o.A_("profx_dMh"), *dWh = o.A_("profx_dWh"), *Qh = o.A_("profx_Qheat");
+    const double *diffmv = o.A_("diffmv"), *diffwv = o.A_("diffwv"), *diffrv = o.A_("diffrv"),
+                 *diffprv = o.A_("diffprv");
     const double *fr = o.g.func_r, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *div = o.g.div, *grad = o.g.grad;
     const double A = o.p.A, Gravit = o.p.Gravit;
 
@@ -730,7 +815,7 @@ void slow_modes(Oracle &o) {
         const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
         for (int k = 0; k < 3; k++) {
             int e = (id * nv + lev) * 3 + k;
-            SlowMh[e] = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[e] + diffmh[e] + 0.0 + dMh[e];
+            SlowMh[e] = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[e] + diffmh[e] + diffmv[e] + dMh[e];
         }
         if (lev == 0) {
             SlowWh[id * (nv + 1) + 0] = 0.0;
@@ -753,7 +838,8 @@ void slow_modes(Oracle &o) {
                     swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
             }
             if (o.p.NonHydro)
-                swr += (-diffw[id * nv + lev - 1] - 0.0) * intt + (-diffw[id * nv + lev] - 0.0) * intl;
+                swr += (-diffw[id * nv + lev - 1] - diffwv[id * nv + lev - 1]) * intt
+                       + (-diffw[id * nv + lev] - diffwv[id * nv + lev]) * intl;
             double dzl = alt - altl;
             double dpdz = (pk[id * nv + lev] - pressurel) / dzl;
             swr += dpdz;
@@ -761,14 +847,14 @@ void slow_modes(Oracle &o) {
             SlowWh[id * (nv + 1) + lev] = -swr;
         }
         nflxr += dwdz;
-        SlowRho[id * nv + lev] = -nflxr + diffrh[id * nv + lev] + 0.0;
+        SlowRho[id * nv + lev] = -nflxr + diffrh[id * nv + lev] + diffrv[id * nv + lev];
         double r = 1.0 / rho;
         const double *M0 = Mh + (id * nv + lev) * 3;
         double vgp = (nflxv[0] - dpr * fx) * M0[0] * r + (nflxv[1] - dpr * fy) * M0[1] * r
                      + (nflxv[2] - dpr * fz) * M0[2] * r;
         Slowp[id * nv + lev] = (Rd[id * nv + lev] / Cv)
                                * (-nflxp - gtil[id * nv + lev] - dwhdz + vgp + Qh[id * nv + lev]);
-        Slowp[id * nv + lev] += diffpr[id * nv + lev] + 0.0;
+        Slowp[id * nv + lev] += diffpr[id * nv + lev] + diffprv[id * nv + lev];
     };
     for (int ib = 0; ib < o.nr; ib++) {
         const int *m = o.g.maps + ib * NHL2;
@@ -1173,7 +1259,7 @@ void density_pressure(Oracle &o, double dt) {
     double *ps = o.A_("pressures"), *pk = o.A_("pressurek"), *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"),
            *Ms = o.A_("Mhs"), *Mk = o.A_("Mhk"), *Whs = o.A_("Whs"), *Whk = o.A_("Whk"), *pt = o.A_("pt"),
            *pth = o.A_("pth"), *SlowRho = o.A_("SlowRho"), *Qh = o.A_("profx_Qheat"), *diffpr = o.A_("diffpr"),
-           *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+           *diffprv = o.A_("diffprv"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
     const double *div = o.g.div, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     const double A = o.p.A, P_Ref = o.p.P_Ref;
     auto body = [&](int id, int lev, int nslots, bool pole, auto vv, auto v1v, auto ptv) {
@@ -1229,7 +1315,7 @@ void density_pressure(Oracle &o, double dt) {
                      * MPOW((ps[id * nv + lev] + pk[id * nv + lev]) / P_Ref, Cv / Cp[id * nv + lev]);
         aux += ptc * r;
         double p = P_Ref * MPOW(Rd[id * nv + lev] * aux / P_Ref, Cp[id * nv + lev] / Cv);
-        ps[id * nv + lev] = p - pk[id * nv + lev] + (diffpr[id * nv + lev] + 0.0) * dt
+        ps[id * nv + lev] = p - pk[id * nv + lev] + (diffpr[id * nv + lev] + diffprv[id * nv + lev]) * dt
                             + Rd[id * nv + lev] / Cv * Qh[id * nv + lev] * dt;
     };
     double vs[3][NHL2], v1s[3][NHL2], pts[NHL2];
@@ -1304,6 +1390,7 @@ void thor(Oracle &o) {
             }
             zero(o, "diffrh");
             diffusion_pass(o, stepnum, true);
+            if (o.p.VertHyDiff) vertical_diff(o);
             correct_horizontal(o);
         }
         zero(o, "DivM");
@@ -1711,6 +1798,12 @@ void make_kd(Oracle &o) {
         o.Kdh4[lev] = (o.p.Diffc) * std::pow(dbar, 1.0 * o.p.HyDiffOrder) / o.p.timestep;
         o.Kdhz[lev] = (o.p.DivDampc) * std::pow(dbar, 4.) / o.p.timestep;
     }
+    // Vertical hyperdiffusion strength (esp_initial.cu:1327-1334).
+    o.Kdv6.assign(o.nv, 0.0);
+    for (int lev = 0; lev < o.nv; lev++) {
+        const double dz = o.g.Altitudeh[lev + 1] - o.g.Altitudeh[lev];
+        o.Kdv6[lev] = o.p.Diffc_v * std::pow(dz, 1.0 * o.p.VertHyDiffOrder) / o.p.timestep;
+    }
 }
 
 } // namespace
@@ -1760,9 +1853,9 @@ oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
     const long N = (long)o->P * o->nv, Ni = (long)o->P * o->nvi;
     const char *sc[] = {"W", "Rho", "pressure", "temperature", "Rd", "Cp", "profx_Qheat", "Wk", "Rhok",
                         "pressurek", "Ws", "Rhos", "pressures", "h", "pt", "gtil", "SlowRho", "Slowpressure",
-                        "diffw", "diffrh", "diffpr", "Sp", "Sd"};
+                        "diffw", "diffrh", "diffpr", "Sp", "Sd", "diffwv", "diffrv", "diffprv"};
     const char *it[] = {"Wh", "profx_dWh", "Whk", "Whs", "hh", "pth", "gtilh", "SlowWh"};
-    const char *vc[] = {"Mh", "profx_dMh", "Mhk", "Mhs", "Adv", "v", "SlowMh", "diffmh", "DivM", "divg_Mh"};
+    const char *vc[] = {"Mh", "profx_dMh", "Mhk", "Mhs", "Adv", "v", "SlowMh", "diffmh", "diffmv", "DivM", "divg_Mh"};
     for (auto n : sc) o->arr[n].assign(N, 0.0);
     for (auto n : it) o->arr[n].assign(Ni, 0.0);
     for (auto n : vc) o->arr[n].assign(3 * N, 0.0);
@@ -1777,6 +1870,7 @@ oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
     o->arr["OLR"].assign(o->P, 0.0);
     o->arr["Kdh4"] = o->Kdh4;  // read-only copies for callers
     o->arr["Kdhz"] = o->Kdhz;
+    o->arr["Kdv6"] = o->Kdv6;
     return o;
 }
 
@@ -1809,6 +1903,8 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         advec_cori2(*o);
     } else if (n == "diffusion") {
         diffusion_pass(*o, a0, a1 != 0);
+    } else if (n == "vertical_diff") {
+        vertical_diff(*o);
     } else if (n == "correct_horizontal") {
         correct_horizontal(*o);
     } else if (n == "divm") {

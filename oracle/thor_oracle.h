@@ -28,6 +28,8 @@ typedef struct oracle_params {
     double Diffc, DivDampc, timestep;
     int glevel, HyDiffOrder, n_substeps;
     int DeepModel, NonHydro, HyDiff, DivDampP, GravHeightVar;
+    int VertHyDiff, VertHyDiffOrder;
+    double Diffc_v;
 } oracle_params;
 
 typedef struct oracle oracle;
@@ -49,6 +51,7 @@ void oracle_profx_held_suarez(oracle *o);
 //   "advec_cori1"  Compute_Advec_Cori1 + _Poles   (thor_adv_cor.h:51-362)
 //   "advec_cori2"  Compute_Advec_Cori2            (thor_adv_cor.h:366-536)
 //   "diffusion"    Diffusion_Op + _Poles, a0 = stepnum, a1 = laststep (thor_diff.h:54-880)
+//   "vertical_diff" vertical_diff                 (thor_diff.h:1198-1426)
 //   "correct_horizontal"                          (thor_diff.h:882-915)
 //   "divm"         DivM_Op + _Poles on Mhk/Whk, a0 = laststep (thor_div.h:49-535)
 //   "slow_modes"   Compute_Slow_Modes + _Poles    (thor_slowmodes.h:48-683)

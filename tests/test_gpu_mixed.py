@@ -133,6 +133,35 @@ def test_mixed_order_six_stays_close_to_fp64(built):
         assert err < t, k
 
 
+def test_mixed_vertical_hyperdiffusion_stays_close_to_fp64(built):
+    """VertHyDiff: the vertical terms are formed in FP64 and added to the FP32
+    horizontal tendencies in the mixed form; three steps stay within the
+    mixed-vs-FP64 bounds, and the vertical terms do change the state."""
+    capi = built
+    g = grid(capi, 4, 16)
+    print("case: g4 L16, VertHyDiff order 6, 3 steps mixed vs FP64", flush=True)
+    kw = dict(VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=2e-2)
+    p64 = earth_params(capi, 4, dt=600.0, **kw)
+    st = capi.ic_rest(g, p64, seed=1001, temperature_noise=1.0, wind_noise=1.0)
+    out = {}
+    for prec, vert in ((0, 1), (1, 1), (0, 0)):
+        d = capi.Dyn(g, earth_params(capi, 4, dt=600.0, precision=prec, **dict(kw, VertHyDiff=vert)))
+        d.set_state(st)
+        d.run(3)
+        d.sync()
+        out[prec, vert] = d.get_state()
+        d.close()
+    tol = {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}
+    for k, t in tol.items():
+        a, b = out[1, 1][k], out[0, 1][k]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        eff = np.abs(out[0, 0][k] - b).max() / np.abs(b).max()
+        print(f"3 steps mixed vs FP64, VertHyDiff, {k}: max rel diff {err:.2e} (vertical terms move it by {eff:.2e})")
+        assert np.isfinite(a).all()
+        assert err < t, k
+    assert np.abs(out[0, 0]["Mh"] - out[0, 1]["Mh"]).max() > 0
+
+
 def _two_steps(capi, g, p, st, env):
     """Two HS steps of a Dyn created with the given tuning variables set."""
     import os

@@ -94,3 +94,26 @@ def test_native_order_six_hyperdiffusion_g4(built):
     d.sync()
     o.steps(2)
     _compare(d, o, TOL_STEP, "native HyDiffOrder 6", TOL_VERTICAL)
+
+
+@pytest.mark.parametrize("deep", [1, 0])
+def test_pm_vertical_hyperdiffusion_g4(built, deep):
+    """VertHyDiff (order 6) on top of the horizontal hyperdiffusion: the
+    vertical terms enter the slow modes and the pressure update."""
+    print(f"case: parity build, g4 L20, VertHyDiff order 6, DeepModel {deep}, 2 x (ProfX HS + Thor)", flush=True)
+    g, p, st, d, o = _pair(built, 4, 20, variant="pm", wnoise=1.0, DeepModel=deep, VertHyDiff=1,
+                           VertHyDiffOrder=6, Diffc_v=2e-3)
+    for i in range(2):
+        d.run(1)
+        d.sync()
+        o.steps(1)
+        _compare(d, o, TOL_PM, f"pm VertHyDiff step {i + 1}")
+
+
+def test_native_vertical_hyperdiffusion_g4(built):
+    print("case: production build, g4 L20, VertHyDiff order 6, 2 x (ProfX HS + Thor)", flush=True)
+    g, p, st, d, o = _pair(built, 4, 20, wnoise=1.0, VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=2e-3)
+    d.run(2)
+    d.sync()
+    o.steps(2)
+    _compare(d, o, TOL_STEP, "native VertHyDiff", TOL_VERTICAL)

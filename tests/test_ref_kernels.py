@@ -28,12 +28,14 @@ pytestmark = pytest.mark.gpu
 # dr2dz = (r_top^3 - r_bot^3) / 3 (thor_slowmodes.h:237), a cancellation that
 # magnifies a 1-ulp difference in pow(r, 3) by about r / (3 dz) ~ 1e3 at this
 # grid's 2.25 km layers, hence 1e-12 there; a dropped term or a wrong sign
-# would still show up as an O(1) difference.
+# would still show up as an O(1) difference. vertical_diff calls pow(r, 2)
+# (1e-14; measured 8e-16).
 CASES = [
     ("advec_cori1", 0, 0, ["Adv", "v"], 0.0),
     ("advec_cori2", 0, 0, ["Adv"], 1e-14),
     ("diffusion", 0, 0, ["diff"], 0.0),
     ("diffusion", 1, 1, ["diffmh", "diffw", "diffrh", "diffpr"], 0.0),
+    ("vertical_diff", 0, 0, ["diffmv", "diffwv", "diffprv"], 1e-14),
     ("correct_horizontal", 0, 0, ["diffmh"], 0.0),
     ("divm", 0, 0, ["divg_Mh"], 0.0),
     ("divm", 1, 0, ["DivM"], 0.0),
@@ -44,7 +46,7 @@ CASES = [
 NAMES = ["Mh", "Wh", "W", "Rho", "pressure", "temperature", "Rd", "Cp", "profx_Qheat", "profx_dMh", "profx_dWh",
          "Mhk", "Whk", "Wk", "Rhok", "pressurek", "h", "pt", "gtil", "hh", "pth", "gtilh", "Adv", "v", "SlowMh",
          "SlowWh", "SlowRho", "Slowpressure", "diff", "diffmh", "diffw", "diffrh", "diffpr", "DivM", "divg_Mh",
-         "Kdh4", "Kdhz"]
+         "Kdh4", "Kdhz", "Kdv6", "diffmv", "diffwv", "diffrv", "diffprv"]
 
 
 @pytest.fixture(scope="module")
@@ -54,7 +56,8 @@ def setup(built):
         pytest.skip("THOR's kernels are not built (oracle/_ref/libthor_ref.so missing)")
     capi = built
     g = grid(capi, 4, 16)
-    p = earth_params(capi, 4, dt=600.0)
+    # vertical hyperdiffusion on, so the slow modes also see its terms
+    p = earth_params(capi, 4, dt=600.0, VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=2e-3)
     st = capi.ic_rest(g, p, seed=1000, temperature_noise=1e-3, wind_noise=1.0)
     o = om.Oracle(g, p)
     o.set_state(st)

@@ -115,6 +115,7 @@ class Params(C.Structure):
         ("glevel", C.c_int), ("HyDiff", C.c_int), ("HyDiffOrder", C.c_int), ("DivDampP", C.c_int),
         ("DeepModel", C.c_int), ("NonHydro", C.c_int), ("GravHeightVar", C.c_int),
         ("n_substeps", C.c_int), ("precision", C.c_int),
+        ("VertHyDiff", C.c_int), ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double),
     ]
 
     def as_dict(self):

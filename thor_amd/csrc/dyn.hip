@@ -295,6 +295,14 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, fd, hydiff, divdamp, d.diff_sign);
             });
         }
+        if (hydiff && d.prm.VertHyDiff) {
+            // vertical hyperdiffusion, after the horizontal passes as in THOR
+            const int cbv = vert_diff_columns(d.nv);
+            const size_t lds = vert_diff_lds(d.nv, cbv);
+            launch(d, "vert_diff", MX ? W(6, 10) : W(6, 7), [&] {
+                k_vert_diff<MX><<<dim3((d.P + cbv - 1) / cbv), dim3(cbv, 6), lds, st>>>(g, f, d.prm.VertHyDiffOrder);
+            });
+        }
         if (d.hooks.dyn_core_loop_slow_modes) {
             const thor_step_view v = view(f);
             if (d.hooks.dyn_core_loop_slow_modes(d.hooks.user, (void *)st, nstep, times, &v))
@@ -556,6 +564,8 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     if (!gd || !prm || !out) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: null argument");
     if (prm->HyDiff && (prm->HyDiffOrder < 4 || prm->HyDiffOrder % 2))
         return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: HyDiffOrder must be even and >= 4");
+    if (prm->HyDiff && prm->VertHyDiff && (prm->VertHyDiffOrder < 4 || prm->VertHyDiffOrder % 2))
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: VertHyDiffOrder must be even and >= 4");
     if (prm->n_substeps < 2 || prm->n_substeps % 2)
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: n_substeps must be even and >= 2");
     if (prm->precision != THOR_PRECISION_FP64 && prm->precision != THOR_PRECISION_MIXED)
@@ -619,6 +629,12 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         Kdhz[lev] = (prm->DivDampc) * std::pow(dbar, 4.) / prm->timestep;
     }
     d.diff_sign = std::pow(-1.0, prm->HyDiffOrder / 2);
+    // vertical hyperdiffusion strength (esp_initial.cu:1327-1334)
+    std::vector<double> Kdv6(nv);
+    for (int lev = 0; lev < nv; lev++) {
+        const double dz = gd->Altitudeh[lev + 1] - gd->Altitudeh[lev];
+        Kdv6[lev] = prm->Diffc_v * std::pow(dz, 1.0 * prm->VertHyDiffOrder) / prm->timestep;
+    }
 
     auto up_d = [&](const std::vector<double> &v, const double **dst) -> bool {
         double *ptr;
@@ -684,7 +700,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
               && up_d(planar(gd->areasT, 1), &g.areasT)
               && up_d(std::vector<double>(gd->Altitude, gd->Altitude + nv), &g.Alt)
               && up_d(std::vector<double>(gd->Altitudeh, gd->Altitudeh + nv + 1), &g.Alth)
-              && up_d(Kdh4, &g.Kdh4) && up_d(Kdhz, &g.Kdhz);
+              && up_d(Kdh4, &g.Kdh4) && up_d(Kdhz, &g.Kdhz) && up_d(Kdv6, &g.Kdv6);
     if (!ok) return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: geometry upload"));
 
     const size_t n1 = (size_t)nv * S, ni = (size_t)(nv + 1) * S, n3 = 3 * n1;
@@ -706,6 +722,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     if (d.mixed) A(&f.hflx_ptk, n1);
     const bool diff_mid = prm->HyDiff && prm->HyDiffOrder >= 6;  // intermediate passes alternate two arrays
     if (diff_mid && !d.mixed) A(&f.diffx, 6 * n1);
+    if (prm->HyDiff && prm->VertHyDiff && !d.mixed) A(&f.diffmv, n3);
     const long ncell = (long)P * nv;
     d.npart = (int)((ncell + 2047) / 2048);
     A(&d.partials, (size_t)5 * d.npart);

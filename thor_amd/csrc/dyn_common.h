@@ -31,6 +31,7 @@ struct Geo {
     const double *lonlat;                      // [2][S]
     const double *areasT;                      // [S]
     const double *Alt, *Alth, *Kdh4, *Kdhz;    // [nv], [nv+1], [nv], [nv]
+    const double *Kdv6;                         // [nv] vertical hyperdiffusion strength (VertHyDiff)
     // Mixed precision only: the hyperdiffusion Laplacian and the divergence-
     // damping operator as per-point weights on the 7 stencil slots (0 = self,
     // 1..6 = ring), for rscale = 1:
@@ -76,6 +77,7 @@ struct Fields {
     // tendency (same layouts as their FP64 counterparts above).
     float *mMhs, *mAdv, *mdiff, *mdiffmh, *mdiffw, *mdiffrh, *mdiffpr, *mdivg, *mSlowMh;
     double *diffx;  // [6][nv]: second hyperdiffusion array for HyDiffOrder >= 6 (passes alternate)
+    double *diffmv; // [3][nv]: vertical momentum hyperdiffusion (FP64 form with VertHyDiff; else null)
     float *mdiffx;
     float *mhpt;  // [2][nv]: h and pt of the stage rounded to FP32, for the substep flux divergences
 };

@@ -976,6 +976,119 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0(Geo g, Fields f, int hydiff, int di
         [&](int s) { return sWl[T.pos(s)]; }, [&](int s) { return sWt[T.pos(s)]; });
 }
 
+// Vertical hyperdiffusion (vertical_diff, thor_diff.h:1198-1426, run at
+// thor_driver.cu:590-612 when VertHyDiff) and its part of Correct_Horizontal
+// (thor_diff.h:882-915), in FP64 arithmetic in both precision forms. Per
+// column and variable (0 rho, 1-3 momentum / rho, 4 W / rho, 5 T): a vertical
+// Laplacian in flux form with r^2 weights on the column extended by one ghost
+// level at each end (a copy of the layer next to it), applied order / 2
+// times; before the last application the field is multiplied by sign Kdv6
+// (and by rho for variables 1-5), sign = (-1)^(order/2 + 1), and the T result
+// is multiplied by Rd. The density result is unused (THOR's diffrv is 0).
+// A block is blockDim.x columns x 6 variables; each thread holds its column
+// in LDS and updates it in place, carrying the old value below in a register.
+// Outputs: the momentum part with its radial component removed goes to
+// diffmv (FP64 form) or is added to the FP32 horizontal momentum diffusion
+// (mixed form); the w and R T parts are added to diffw and diffpr, which
+// THOR only reads as diffw + diffwv and diffpr + diffprv
+// (thor_slowmodes.h:348-380, thor_fastmodes.h:504-571).
+__host__ __device__ inline int vert_diff_columns(int nv) {
+    return (6 * nv + (nv + 2) * 6 * 16) * 8 <= 64 * 1024 ? 16 : 8;
+}
+__host__ __device__ inline size_t vert_diff_lds(int nv, int cb) { return (size_t)(6 * nv + (nv + 2) * 6 * cb) * 8; }
+
+template <bool MX> __global__ __launch_bounds__(96) void k_vert_diff(Geo g, Fields f, int order) {
+    extern __shared__ double vsh[];
+    const int nv = g.nv, CB = blockDim.x, c = threadIdx.x, var = threadIdx.y;
+    const size_t S = g.S;
+    double *fac = vsh;           // [6][nv]: 1/dz, r_up^2, r_low^2, r^2, 1/dz_up, 1/dz_low
+    double *col = vsh + 6 * nv;  // [nv + 2][6][CB]: ghost-extended columns
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    for (int lev = var * CB + c; lev < nv; lev += 6 * CB) {
+        double ri = 1.0, rup = 1.0, rlow = 1.0;
+        if (g.deep) {
+            ri = g.A + Alt[lev];
+            rup = g.A + Alth[lev + 1];
+            rlow = g.A + Alth[lev];
+        }
+        double dzup, dzlow;
+        if (lev == 0) {
+            dzup = 1.0 / (Alt[1] - Alt[0]);
+            dzlow = 1.0 / (2.0 * Alt[0]);
+        } else if (lev == nv - 1) {
+            dzup = 1.0 / (2 * (Alth[lev + 1] - Alt[lev]));
+            dzlow = 1.0 / (Alt[lev] - Alt[lev - 1]);
+        } else {
+            dzup = 1.0 / (Alt[lev + 1] - Alt[lev]);
+            dzlow = 1.0 / (Alt[lev] - Alt[lev - 1]);
+        }
+        fac[lev] = 1.0 / (Alth[lev + 1] - Alth[lev]);
+        fac[nv + lev] = XPOW(rup, 2);
+        fac[2 * nv + lev] = XPOW(rlow, 2);
+        fac[3 * nv + lev] = XPOW(ri, 2);
+        fac[4 * nv + lev] = dzup;
+        fac[5 * nv + lev] = dzlow;
+    }
+    const int p = blockIdx.x * CB + c;
+    const bool on = p < g.P;
+    auto A = [&](int i) -> double & { return col[((size_t)i * 6 + var) * CB + c]; };
+    if (on) {
+        for (int lev = 0; lev < nv; lev++) {
+            const double r = LV(f.Rhok, lev);
+            double a;
+            if (var == 0) a = r;
+            else if (var <= 3) a = VEC(f.Mhk, var - 1, lev, p) / r;
+            else if (var == 4) a = LV(f.Wk, lev) / r;
+            else a = LV(f.T, lev);
+            A(lev + 1) = a;
+        }
+        A(0) = A(1);
+        A(nv + 1) = A(nv);
+    }
+    __syncthreads();
+    if (on) {
+        const double sign = XPOW(-1.0, order / 2 + 1);
+        const int passes = order / 2 - 1;
+        for (int l = 0; l <= passes; l++) {
+            double am = A(0), a0 = A(1);
+            for (int lev = 0; lev < nv; lev++) {
+                const double ap = A(lev + 2);
+                const double flux =
+                    fac[nv + lev] * (ap - a0) * fac[4 * nv + lev] - fac[2 * nv + lev] * (a0 - am) * fac[5 * nv + lev];
+                double d;
+                if (l < passes) {
+                    d = flux * fac[lev] / fac[3 * nv + lev];
+                    if (l == passes - 1) d = var == 0 ? sign * g.Kdv6[lev] * d : sign * LV(f.Rhok, lev) * g.Kdv6[lev] * d;
+                } else {
+                    d = var != 5 ? flux * fac[lev] / fac[3 * nv + lev] : flux * fac[lev] * g.Rd / fac[3 * nv + lev];
+                }
+                A(lev + 1) = d;
+                am = a0;
+                a0 = ap;
+            }
+            A(0) = A(1);
+            A(nv + 1) = A(nv);
+        }
+    }
+    __syncthreads();
+    if (!on || var == 0) return;
+    if (var <= 3) {
+        const int k = var - 1;
+        const double fk = g.fr[(size_t)k * S + p], fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+        for (int lev = 0; lev < nv; lev++) {
+            const double *q = &col[((size_t)(lev + 1) * 6) * CB + c];
+            const double dk = q[var * CB];
+            const double dmvr = fx * q[1 * CB] + fy * q[2 * CB] + fz * q[3 * CB];
+            const double out = dk + -fk * dmvr;
+            if constexpr (MX) VEC(f.mdiffmh, k, lev, p) = (float)((double)VEC(f.mdiffmh, k, lev, p) + out);
+            else VEC(f.diffmv, k, lev, p) = out;
+        }
+    } else {
+        auto *dst = var == 4 ? FS<MX>::diffw(f) : FS<MX>::diffpr(f);
+        for (int lev = 0; lev < nv; lev++) LV(dst, lev) = (double)LV(dst, lev) + A(lev + 1);
+    }
+}
+
 // Intermediate hyperdiffusion pass of HyDiffOrder >= 6 (Diffusion_Op with
 // stepnum > 0 and laststep false, thor_driver.cu:435-470, argument
 // thor_diff.h:218-221): the Laplacian of each of the six fields of the
@@ -1119,8 +1232,8 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     const auto *diffmh = FS<MX>::diffmh(f);
 #pragma unroll
     for (int k = 0; k < 3; k++)
-        VEC(SlowMh, k, lev, p) = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[k] + VEC(diffmh, k, lev, p) + 0.0
-                                 + VEC(f.dMh, k, lev, p);
+        VEC(SlowMh, k, lev, p) = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM[k] + VEC(diffmh, k, lev, p)
+                                 + (f.diffmv ? VEC(f.diffmv, k, lev, p) : 0.0) + VEC(f.dMh, k, lev, p);
     if (lev == 0) {
         LV(f.SlowWh, 0) = 0.0;
         LV(f.SlowWh, nv) = 0.0;

@@ -104,6 +104,9 @@ extern "C" void thor_params_default_earth(thor_params *p) {
     p->GravHeightVar = 0;
     p->n_substeps = 6;
     p->precision = THOR_PRECISION_FP64;
+    p->VertHyDiff = 0;  // define.h:84-85
+    p->VertHyDiffOrder = 6;
+    p->Diffc_v = 0.0;
 }
 
 extern "C" thor_status thor_ic_rest(const thor_grid_desc *g, const thor_params *p, unsigned long long seed,
