@@ -112,6 +112,13 @@ typedef struct thor_params {
      * (esp_initial.cu:1327-1334). VertHyDiffOrder: even, >= 4. */
     int VertHyDiff, VertHyDiffOrder;
     double Diffc_v;
+    /* Diffusive sponge (SimulationSetup::DiffSponge, order_diff_sponge,
+     * Dv_sponge, ns_diff_sponge; define.h:70-73), with HyDiff: strength
+     * Dv_sponge sin^2(pi/2 (z/Top - ns) / (1 - ns)) above ns
+     * (esp_initial.cu:1297-1321). order_diff_sponge: even, from
+     * HyDiffOrder - 4 to HyDiffOrder and >= 2. */
+    int DiffSponge, order_diff_sponge;
+    double Dv_sponge, ns_diff_sponge;
 } thor_params;
 
 /* Fills p with THOR's defaults for the Earth Held-Suarez case (ifile/earth_hstest.thr). */

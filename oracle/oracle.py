@@ -39,7 +39,8 @@ class _Params(C.Structure):
                 ("timestep", C.c_double), ("glevel", C.c_int), ("HyDiffOrder", C.c_int),
                 ("n_substeps", C.c_int), ("DeepModel", C.c_int), ("NonHydro", C.c_int), ("HyDiff", C.c_int),
                 ("DivDampP", C.c_int), ("GravHeightVar", C.c_int), ("VertHyDiff", C.c_int),
-                ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double)]
+                ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double), ("DiffSponge", C.c_int),
+                ("order_diff_sponge", C.c_int), ("Dv_sponge", C.c_double), ("ns_diff_sponge", C.c_double)]
 
 
 _libs: dict = {}
@@ -103,7 +104,8 @@ def _structs(grid, params):
     p = _Params(params.A, params.Omega, params.Gravit, params.P_Ref, params.Rd, params.Cp,
                 params.Diffc, params.DivDampc, params.timestep, params.glevel, params.HyDiffOrder,
                 params.n_substeps, params.DeepModel, params.NonHydro, params.HyDiff, params.DivDampP,
-                params.GravHeightVar, params.VertHyDiff, params.VertHyDiffOrder, params.Diffc_v)
+                params.GravHeightVar, params.VertHyDiff, params.VertHyDiffOrder, params.Diffc_v,
+                params.DiffSponge, params.order_diff_sponge, params.Dv_sponge, params.ns_diff_sponge)
     return g, p
 
 

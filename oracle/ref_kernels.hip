@@ -158,14 +158,15 @@ int ref_kernel(void *h, const char *name, int a0, int a1) {
                                           r["diff_sponge"], r["Mhk"], r["Rhok"], r["temperature"], r["Wk"],
                                           r["areasTr"], r["areas"], r["nvecoa"], r["nvecti"], r["nvecte"],
                                           r["func_r"], r["Kdh4"], r["Altitude"], A, r["Rd"], r["Cp"], r.maps, 16,
-                                          stepnum, last, deep, false, 4, r["Kdh2"], r["boundary_flux"], false,
-                                          r.p.HyDiffOrder);
+                                          stepnum, last, deep, r.p.DiffSponge != 0, r.p.order_diff_sponge,
+                                          r["Kdh2"], r["boundary_flux"], false, r.p.HyDiffOrder);
         Diffusion_Op_Poles<5><<<NBDP, 1>>>(r["diffmh"], r["diffw"], r["diffrh"], r["diffpr"], r["diff"],
                                            r["diff_sponge"], r["Mhk"], r["Rhok"], r["temperature"], r["Wk"],
                                            r["func_r"], r["areasTr"], r["areas"], r["nvecoa"], r["nvecti"],
                                            r["nvecte"], r["Kdh4"], r["Altitude"], r["Altitudeh"], A, r["Rd"],
-                                           r["Cp"], r.point_local, P, stepnum, last, deep, false, 4, r["Kdh2"],
-                                           r["boundary_flux"], false, r.p.HyDiffOrder);
+                                           r["Cp"], r.point_local, P, stepnum, last, deep, r.p.DiffSponge != 0,
+                                           r.p.order_diff_sponge, r["Kdh2"], r["boundary_flux"], false,
+                                           r.p.HyDiffOrder);
     } else if (n == "vertical_diff") {
         const dim3 NBALL0(P / NTH + 1, 1, 6);  // thor_driver.cu:86
         vertical_diff<<<NBALL0, NTH>>>(r["diffmv"], r["diffwv"], r["diffrv"], r["diffprv"], r["diff"], r["diff2"],

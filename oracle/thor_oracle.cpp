@@ -44,7 +44,7 @@ struct Oracle {
     oracle_params p;
     int P, nv, nvi, nr;
     std::map<std::string, std::vector<double>> arr;
-    std::vector<double> Kdh4, Kdhz, Kdv6;
+    std::vector<double> Kdh4, Kdhz, Kdv6, Kdh2;
     double nrq[8] = {0}, rtq[11] = {0};  // forcing parameters
     // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean
     bool sponge_on = false;
@@ -450,6 +450,18 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
         return diff_in[(gi * nv + lev) * 6 + var];
     };
     const double sign = std::pow(-1.0, o.p.HyDiffOrder / 2);
+    // diffusive sponge (thor_diff.h:453-509): at pass order_diff_sponge / 2 the
+    // pass input at the point, times (-1)^(order/2 + 1) rho Kdh2, is kept for
+    // the momentum and w fields and added to the last pass's result
+    const bool sponge_on = o.p.DiffSponge && o.p.order_diff_sponge != o.p.HyDiffOrder;
+    const double ssp = MPOW(-1.0, o.p.order_diff_sponge / 2 + 1);
+    double *dsp = o.A_("diff_sponge");
+    auto sponge = [&](int id, int lev, int var, double &lap) {
+        if (!sponge_on || var < 1 || var > 4) return;
+        const long e = ((long)id * nv + lev) * 6 + var;
+        if (stepnum == o.p.order_diff_sponge / 2) dsp[e] = ssp * Rho[id * nv + lev] * o.Kdh2[lev] * diff_in[e];
+        if (laststep) lap += dsp[e];
+    };
     auto store = [&](int id, int lev, int var, double lap) {
         if (laststep) {
             if (var == 0) diffrh[id * nv + lev] = lap;
@@ -481,6 +493,7 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
                         double lap = laplacian(o, s.id, rs, nf,
                                                [&](int hp) { return hp < 0 ? as[s.ir] : as[hp]; }, ring, s.ir);
                         if (laststep) lap *= sdiff;
+                        sponge(s.id, lev, var, lap);
                         store(s.id, lev, var, lap);
                     }
             }
@@ -498,6 +511,7 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
                                        [&](int gi) { return gi < 0 ? arg(id, lev, var) : arg(gi, lev, var); },
                                        ring, id);
                 if (laststep) lap = sdiff * lap;
+                sponge(id, lev, var, lap);
                 store(id, lev, var, lap);
             }
 }
@@ -1791,12 +1805,24 @@ void sponge(Oracle &o) {
 
 void make_kd(Oracle &o) {
     // Horizontal diffusion / divergence-damping strengths (esp_initial.cu:1292-1321).
+    // With the diffusive sponge (esp_initial.cu:1297-1321), its strength adds
+    // to Kdh4 when its order is HyDiffOrder and fills Kdh2 otherwise.
     o.Kdh4.assign(o.nv, 0.0);
     o.Kdhz.assign(o.nv, 0.0);
+    o.Kdh2.assign(o.nv, 0.0);
+    const double top = o.g.Altitudeh[o.nv];
     for (int lev = 0; lev < o.nv; lev++) {
         double dbar = std::sqrt(2 * M_PI / 5) * o.p.A / (std::pow(2, o.p.glevel));
         o.Kdh4[lev] = (o.p.Diffc) * std::pow(dbar, 1.0 * o.p.HyDiffOrder) / o.p.timestep;
         o.Kdhz[lev] = (o.p.DivDampc) * std::pow(dbar, 4.) / o.p.timestep;
+        if (o.p.DiffSponge) {
+            const double n = o.g.Altitude[lev] / top, ns = o.p.ns_diff_sponge;
+            const double ks = n > ns ? o.p.Dv_sponge * std::pow(std::sin(0.5 * M_PI * (n - ns) / (1.0 - ns)), 2) : 0.0;
+            if (o.p.order_diff_sponge == o.p.HyDiffOrder)
+                o.Kdh4[lev] += ks * std::pow(dbar, 1.0 * o.p.order_diff_sponge) / o.p.timestep;
+            else
+                o.Kdh2[lev] = ks * std::pow(dbar, 1.0 * o.p.order_diff_sponge) / o.p.timestep;
+        }
     }
     // Vertical hyperdiffusion strength (esp_initial.cu:1327-1334).
     o.Kdv6.assign(o.nv, 0.0);
@@ -1871,6 +1897,8 @@ oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
     o->arr["Kdh4"] = o->Kdh4;  // read-only copies for callers
     o->arr["Kdhz"] = o->Kdhz;
     o->arr["Kdv6"] = o->Kdv6;
+    o->arr["Kdh2"] = o->Kdh2;
+    o->arr["diff_sponge"].assign(6 * N, 0.0);
     return o;
 }
 

@@ -30,6 +30,8 @@ typedef struct oracle_params {
     int DeepModel, NonHydro, HyDiff, DivDampP, GravHeightVar;
     int VertHyDiff, VertHyDiffOrder;
     double Diffc_v;
+    int DiffSponge, order_diff_sponge;
+    double Dv_sponge, ns_diff_sponge;
 } oracle_params;
 
 typedef struct oracle oracle;

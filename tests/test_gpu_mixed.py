@@ -162,6 +162,44 @@ def test_mixed_vertical_hyperdiffusion_stays_close_to_fp64(built):
     assert np.abs(out[0, 0]["Mh"] - out[0, 1]["Mh"]).max() > 0
 
 
+@pytest.mark.parametrize("chunked", ["1", "0"])
+def test_mixed_diffusive_sponge_stays_close_to_fp64(built, chunked):
+    """The diffusive sponge in the mixed form's last hyperdiffusion pass (the
+    level-chunked kernel and the one-level-per-block one): three steps within
+    the mixed-vs-FP64 bounds, and the sponge does change the state."""
+    import os
+    capi = built
+    g = grid(capi, 4, 16)
+    print(f"case: g4 L16, DiffSponge order 2, THOR_DIFF2_CHUNKED={chunked}, 3 steps mixed vs FP64", flush=True)
+    kw = dict(DiffSponge=1, order_diff_sponge=2, Dv_sponge=0.05, ns_diff_sponge=0.33)
+    st = capi.ic_rest(g, earth_params(capi, 4, dt=600.0), seed=1001, temperature_noise=1.0, wind_noise=1.0)
+    out = {}
+    old = os.environ.get("THOR_DIFF2_CHUNKED")
+    os.environ["THOR_DIFF2_CHUNKED"] = chunked
+    try:
+        for prec, on in ((0, 1), (1, 1), (0, 0)):
+            d = capi.Dyn(g, earth_params(capi, 4, dt=600.0, precision=prec, **dict(kw, DiffSponge=on)))
+            d.set_state(st)
+            d.run(3)
+            d.sync()
+            out[prec, on] = d.get_state()
+            d.close()
+    finally:
+        if old is None:
+            os.environ.pop("THOR_DIFF2_CHUNKED", None)
+        else:
+            os.environ["THOR_DIFF2_CHUNKED"] = old
+    tol = {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}
+    for k, t in tol.items():
+        a, b = out[1, 1][k], out[0, 1][k]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        eff = np.abs(out[0, 0][k] - b).max() / np.abs(b).max()
+        print(f"3 steps mixed vs FP64, DiffSponge, {k}: max rel diff {err:.2e} (the sponge moves it by {eff:.2e})")
+        assert np.isfinite(a).all()
+        assert err < t, k
+    assert np.abs(out[0, 0]["Mh"] - out[0, 1]["Mh"]).max() > 0
+
+
 def _two_steps(capi, g, p, st, env):
     """Two HS steps of a Dyn created with the given tuning variables set."""
     import os

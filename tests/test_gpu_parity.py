@@ -117,3 +117,19 @@ def test_native_vertical_hyperdiffusion_g4(built):
     d.sync()
     o.steps(2)
     _compare(d, o, TOL_STEP, "native VertHyDiff", TOL_VERTICAL)
+
+
+@pytest.mark.parametrize("order,sponge_order", [(4, 2), (4, 4), (6, 2), (6, 4)])
+def test_pm_diffusive_sponge_g4(built, order, sponge_order):
+    """The diffusive sponge from a third of the model top: added to the last
+    hyperdiffusion pass (order_diff_sponge below HyDiffOrder, reading the
+    right pass's output) or folded into Kdh4 (equal orders)."""
+    print(f"case: parity build, g4 L20, HyDiffOrder {order}, order_diff_sponge {sponge_order}, "
+          f"2 x (ProfX HS + Thor)", flush=True)
+    g, p, st, d, o = _pair(built, 4, 20, variant="pm", wnoise=1.0, HyDiffOrder=order, DiffSponge=1,
+                           order_diff_sponge=sponge_order, Dv_sponge=0.05, ns_diff_sponge=0.33)
+    for i in range(2):
+        d.run(1)
+        d.sync()
+        o.steps(1)
+        _compare(d, o, TOL_PM, f"pm sponge step {i + 1}")

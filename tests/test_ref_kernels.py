@@ -34,7 +34,7 @@ CASES = [
     ("advec_cori1", 0, 0, ["Adv", "v"], 0.0),
     ("advec_cori2", 0, 0, ["Adv"], 1e-14),
     ("diffusion", 0, 0, ["diff"], 0.0),
-    ("diffusion", 1, 1, ["diffmh", "diffw", "diffrh", "diffpr"], 0.0),
+    ("diffusion", 1, 1, ["diffmh", "diffw", "diffrh", "diffpr", "diff_sponge"], 0.0),
     ("vertical_diff", 0, 0, ["diffmv", "diffwv", "diffprv"], 1e-14),
     ("correct_horizontal", 0, 0, ["diffmh"], 0.0),
     ("divm", 0, 0, ["divg_Mh"], 0.0),
@@ -46,7 +46,7 @@ CASES = [
 NAMES = ["Mh", "Wh", "W", "Rho", "pressure", "temperature", "Rd", "Cp", "profx_Qheat", "profx_dMh", "profx_dWh",
          "Mhk", "Whk", "Wk", "Rhok", "pressurek", "h", "pt", "gtil", "hh", "pth", "gtilh", "Adv", "v", "SlowMh",
          "SlowWh", "SlowRho", "Slowpressure", "diff", "diffmh", "diffw", "diffrh", "diffpr", "DivM", "divg_Mh",
-         "Kdh4", "Kdhz", "Kdv6", "diffmv", "diffwv", "diffrv", "diffprv"]
+         "Kdh4", "Kdhz", "Kdv6", "diffmv", "diffwv", "diffrv", "diffprv", "Kdh2", "diff_sponge"]
 
 
 @pytest.fixture(scope="module")
@@ -56,8 +56,11 @@ def setup(built):
         pytest.skip("THOR's kernels are not built (oracle/_ref/libthor_ref.so missing)")
     capi = built
     g = grid(capi, 4, 16)
-    # vertical hyperdiffusion on, so the slow modes also see its terms
-    p = earth_params(capi, 4, dt=600.0, VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=2e-3)
+    # vertical hyperdiffusion on, so the slow modes also see its terms; the
+    # second-order diffusive sponge on from half the model top, so the last
+    # hyperdiffusion pass also forms and adds THOR's diff_sponge
+    p = earth_params(capi, 4, dt=600.0, VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=2e-3, DiffSponge=1,
+                     order_diff_sponge=2, Dv_sponge=0.01, ns_diff_sponge=0.5)
     st = capi.ic_rest(g, p, seed=1000, temperature_noise=1e-3, wind_noise=1.0)
     o = om.Oracle(g, p)
     o.set_state(st)

@@ -116,6 +116,8 @@ class Params(C.Structure):
         ("DeepModel", C.c_int), ("NonHydro", C.c_int), ("GravHeightVar", C.c_int),
         ("n_substeps", C.c_int), ("precision", C.c_int),
         ("VertHyDiff", C.c_int), ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double),
+        ("DiffSponge", C.c_int), ("order_diff_sponge", C.c_int), ("Dv_sponge", C.c_double),
+        ("ns_diff_sponge", C.c_double),
     ]
 
     def as_dict(self):

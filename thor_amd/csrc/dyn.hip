@@ -67,6 +67,7 @@ struct Dyn {
     bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
     bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
     int cp_lv = 2;            // levels per thread of the column-preparation kernel (1, 2 or 4)
+    int sponge_pass = -1;     // hyperdiffusion pass whose output the diffusive sponge reads (-1: none)
     // Rayleigh sponge
     bool sponge_on = false;
     SpongePar spg{};
@@ -281,6 +282,10 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 launch(d, "diff_mid", W(0, 12), [&] { k_diff_mid<MX><<<grid_t2, blk, 0, st>>>(g, src, dst); });
                 FS<MX>::set_diff(fd, dst);
             }
+            // pass s writes diff (s even) or the second array (s odd); the
+            // order limits keep the sponge pass's output there until the last pass
+            if (d.sponge_pass >= 0)
+                FS<MX>::set_dsp(fd, d.sponge_pass % 2 ? FS<MX>::diffx(f) : FS<MX>::diff(f));
         }
         if (MX && d.diff2_lc) {
             // the last hyperdiffusion pass and the stage's divergence-damping
@@ -566,6 +571,11 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: HyDiffOrder must be even and >= 4");
     if (prm->HyDiff && prm->VertHyDiff && (prm->VertHyDiffOrder < 4 || prm->VertHyDiffOrder % 2))
         return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: VertHyDiffOrder must be even and >= 4");
+    if (prm->HyDiff && prm->DiffSponge
+        && (prm->order_diff_sponge < 2 || prm->order_diff_sponge % 2 || prm->order_diff_sponge > prm->HyDiffOrder
+            || prm->order_diff_sponge < prm->HyDiffOrder - 4))
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: order_diff_sponge must be even, >= 2 and within "
+                                          "HyDiffOrder - 4 .. HyDiffOrder");
     if (prm->n_substeps < 2 || prm->n_substeps % 2)
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: n_substeps must be even and >= 2");
     if (prm->precision != THOR_PRECISION_FP64 && prm->precision != THOR_PRECISION_MIXED)
@@ -622,12 +632,26 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
             for (int c = 0; c < ncoef; c++) v[(size_t)c * S + p] = src[(size_t)p * ncoef + c];
         return v;
     };
-    std::vector<double> Kdh4(nv), Kdhz(nv);
+    // horizontal diffusion and divergence damping, and the diffusive sponge,
+    // whose strength adds to Kdh4 when its order is HyDiffOrder and fills Kdh2
+    // otherwise (esp_initial.cu:1292-1321)
+    std::vector<double> Kdh4(nv), Kdhz(nv), Kdh2(nv, 0.0);
     for (int lev = 0; lev < nv; lev++) {
         double dbar = std::sqrt(2 * M_PI / 5) * prm->A / (std::pow(2, prm->glevel));
         Kdh4[lev] = (prm->Diffc) * std::pow(dbar, 1.0 * prm->HyDiffOrder) / prm->timestep;
         Kdhz[lev] = (prm->DivDampc) * std::pow(dbar, 4.) / prm->timestep;
+        if (prm->DiffSponge) {
+            const double n = gd->Altitude[lev] / gd->Altitudeh[nv], ns = prm->ns_diff_sponge;
+            const double ks = n > ns ? prm->Dv_sponge * std::pow(std::sin(0.5 * M_PI * (n - ns) / (1.0 - ns)), 2) : 0.0;
+            if (prm->order_diff_sponge == prm->HyDiffOrder)
+                Kdh4[lev] += ks * std::pow(dbar, 1.0 * prm->order_diff_sponge) / prm->timestep;
+            else
+                Kdh2[lev] = ks * std::pow(dbar, 1.0 * prm->order_diff_sponge) / prm->timestep;
+        }
     }
+    d.sponge_pass = prm->HyDiff && prm->DiffSponge && prm->order_diff_sponge != prm->HyDiffOrder
+                        ? prm->order_diff_sponge / 2 - 1 : -1;
+    d.g.ssp = (prm->order_diff_sponge / 2 + 1) % 2 ? -1.0 : 1.0;
     d.diff_sign = std::pow(-1.0, prm->HyDiffOrder / 2);
     // vertical hyperdiffusion strength (esp_initial.cu:1327-1334)
     std::vector<double> Kdv6(nv);
@@ -700,7 +724,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
               && up_d(planar(gd->areasT, 1), &g.areasT)
               && up_d(std::vector<double>(gd->Altitude, gd->Altitude + nv), &g.Alt)
               && up_d(std::vector<double>(gd->Altitudeh, gd->Altitudeh + nv + 1), &g.Alth)
-              && up_d(Kdh4, &g.Kdh4) && up_d(Kdhz, &g.Kdhz) && up_d(Kdv6, &g.Kdv6);
+              && up_d(Kdh4, &g.Kdh4) && up_d(Kdhz, &g.Kdhz) && up_d(Kdv6, &g.Kdv6) && up_d(Kdh2, &g.Kdh2);
     if (!ok) return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: geometry upload"));
 
     const size_t n1 = (size_t)nv * S, ni = (size_t)(nv + 1) * S, n3 = 3 * n1;

@@ -32,6 +32,8 @@ struct Geo {
     const double *areasT;                      // [S]
     const double *Alt, *Alth, *Kdh4, *Kdhz;    // [nv], [nv+1], [nv], [nv]
     const double *Kdv6;                         // [nv] vertical hyperdiffusion strength (VertHyDiff)
+    const double *Kdh2;                         // [nv] diffusive-sponge strength (DiffSponge)
+    double ssp;                                 // its sign, (-1)^(order_diff_sponge / 2 + 1)
     // Mixed precision only: the hyperdiffusion Laplacian and the divergence-
     // damping operator as per-point weights on the 7 stencil slots (0 = self,
     // 1..6 = ring), for rscale = 1:
@@ -78,6 +80,8 @@ struct Fields {
     float *mMhs, *mAdv, *mdiff, *mdiffmh, *mdiffw, *mdiffrh, *mdiffpr, *mdivg, *mSlowMh;
     double *diffx;  // [6][nv]: second hyperdiffusion array for HyDiffOrder >= 6 (passes alternate)
     double *diffmv; // [3][nv]: vertical momentum hyperdiffusion (FP64 form with VertHyDiff; else null)
+    double *dsp;    // [6][nv]: the hyperdiffusion pass output the diffusive sponge reads (else null)
+    float *mdsp;
     float *mdiffx;
     float *mhpt;  // [2][nv]: h and pt of the stage rounded to FP32, for the substep flux divergences
 };
@@ -92,6 +96,8 @@ template <> struct FS<false> {
     static __host__ __device__ double *diff(const Fields &f) { return f.diff; }
     static __host__ __device__ double *diffx(const Fields &f) { return f.diffx; }
     static __host__ __device__ void set_diff(Fields &f, double *a) { f.diff = a; }
+    static __host__ __device__ const double *dsp(const Fields &f) { return f.dsp; }
+    static __host__ __device__ void set_dsp(Fields &f, double *a) { f.dsp = a; }
     static __host__ __device__ double *diffmh(const Fields &f) { return f.diffmh; }
     static __host__ __device__ double *diffw(const Fields &f) { return f.diffw; }
     static __host__ __device__ double *diffrh(const Fields &f) { return f.diffrh; }
@@ -106,6 +112,8 @@ template <> struct FS<true> {
     static __host__ __device__ float *diff(const Fields &f) { return f.mdiff; }
     static __host__ __device__ float *diffx(const Fields &f) { return f.mdiffx; }
     static __host__ __device__ void set_diff(Fields &f, float *a) { f.mdiff = a; }
+    static __host__ __device__ const float *dsp(const Fields &f) { return f.mdsp; }
+    static __host__ __device__ void set_dsp(Fields &f, float *a) { f.mdsp = a; }
     static __host__ __device__ float *diffmh(const Fields &f) { return f.mdiffmh; }
     static __host__ __device__ float *diffw(const Fields &f) { return f.mdiffw; }
     static __host__ __device__ float *diffrh(const Fields &f) { return f.mdiffrh; }

@@ -900,6 +900,12 @@ __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f,
         }
 #pragma unroll
         for (int var = 0; var < 6; var++) out[var] = pole ? sdiff * out[var] : out[var] * sdiff;
+        if (const auto *dsp = FS<MX>::dsp(f)) {
+            // diffusive sponge (thor_diff.h:453-509): (-1)^(order/2 + 1) rho Kdh2
+            // times the sponge pass's input at the point, for u, v, w
+            const double rr = rho(0), k2 = g.Kdh2[lev];
+            for (int var = 1; var <= 4; var++) out[var] += g.ssp * rr * k2 * (double)VEC(dsp, var, lev, p);
+        }
         LV(FS<MX>::diffrh(f), lev) = out[0];
         LV(FS<MX>::diffw(f), lev) = out[4];
         LV(FS<MX>::diffpr(f), lev) = out[5];
@@ -1991,6 +1997,10 @@ __global__ __launch_bounds__(256, 3) void k_diff2_lc(Geo g, Fields f, double sig
 #pragma unroll
             for (int s = 2; s < 7; s++) sum += wl[s - 1] * (sA[b][var][T.pos[s]] - a0);
             out[var] = (double)(rs2 * sum) * sdiff;
+        }
+        if (f.mdsp) {  // diffusive sponge, as diff2_divm0_point
+            const double rr = LV(f.Rhok, lev), k2 = g.Kdh2[lev];
+            for (int var = 1; var <= 4; var++) out[var] += g.ssp * rr * k2 * (double)VEC(f.mdsp, var, lev, p);
         }
         const unsigned l4 = (unsigned)(lev * pl4);
         stb_f(rRh, T.o4, l4, (float)out[0]);

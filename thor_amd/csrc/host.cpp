@@ -107,6 +107,10 @@ extern "C" void thor_params_default_earth(thor_params *p) {
     p->VertHyDiff = 0;  // define.h:84-85
     p->VertHyDiffOrder = 6;
     p->Diffc_v = 0.0;
+    p->DiffSponge = 0;  // define.h:70-73
+    p->order_diff_sponge = 2;
+    p->Dv_sponge = 0.01;
+    p->ns_diff_sponge = 0.75;
 }
 
 extern "C" thor_status thor_ic_rest(const thor_grid_desc *g, const thor_params *p, unsigned long long seed,
