@@ -208,11 +208,17 @@ thor_status thor_dyn_get_rt_column(thor_dyn *h, const char *name, double *dst, l
  * wind w relax implicitly towards their zonal means over nlat_bins latitude
  * bands (towards zero when damp_*_to_mean = 0) at the rates Ruv, Rw [1/s]
  * times sin^2(pi/2 (n - ns) / (1 - ns)); Mh, W and the interior Wh are
- * rebuilt from the new winds. p = NULL switches the sponge off. THOR's
- * temperature sponge and shrinking sponge are not supported. */
+ * rebuilt from the new winds. With temp_sponge (RayleighSpongeT) the
+ * temperature relaxes the same way towards its zonal mean at the rate
+ * RT_sponge, through the pressure (profx_sponge.cu:104-127, 388-404; THOR sums
+ * slots of the mean that no point writes without clearing them, here they
+ * are zero). p = NULL switches the sponge off. THOR's shrinking sponge is not
+ * supported. */
 typedef struct thor_sponge_params {
     double ns, Ruv, Rw;
     int nlat_bins, damp_uv_to_mean, damp_w_to_mean;
+    int temp_sponge;  /* appended: zero-filled callers keep the wind-only sponge */
+    double RT_sponge; /* [1/s] */
 } thor_sponge_params;
 thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params *p);
 

@@ -46,9 +46,10 @@ struct Oracle {
     std::map<std::string, std::vector<double>> arr;
     std::vector<double> Kdh4, Kdhz, Kdv6, Kdh2;
     double nrq[8] = {0}, rtq[11] = {0};  // forcing parameters
-    // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean
+    // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean,
+    // temp_sponge, RT_sponge
     bool sponge_on = false;
-    double spq[6] = {0};
+    double spq[8] = {0};
     std::vector<int> ztab;  // [P][3]: latitude bin, points in the bin, index within the bin
     int max_count = 0;
     // vertical solve: keep each column's tridiagonal system (a, b, c, d per
@@ -1702,11 +1703,18 @@ void zonal_table(Oracle &o, int nlat) {
 // interpolated in latitude between bin centres, relaxed with
 // k = R sin^2(pi/2 (n - ns) / (1 - ns)) for n = Altitude / Top >= ns, M and W
 // rebuilt from the new winds, Wh interpolated from W at interior interfaces.
+// With RayleighSpongeT the temperature p / Rd / rho relaxes the same way
+// towards its zonal mean (zonal_temp, profx_sponge.cu:104-127, summed by
+// gpu_sum_on_device at profx_driver.cu:131-149; the new pressure
+// rho Rd (T / dt + kT Tbar) / (1 / dt + kT), profx_sponge.cu:388-404). THOR
+// does not clear the slots of the temperature table that no point writes;
+// here every unwritten slot is zero.
 void sponge(Oracle &o) {
     const int nv = o.nv, P = o.P, nlat = (int)o.spq[3];
-    const double ns = o.spq[0], Ruv = o.spq[1], Rw = o.spq[2];
-    const bool dmu = o.spq[4] != 0.0, dmw = o.spq[5] != 0.0;
-    double *W = o.A_("W"), *Wh = o.A_("Wh"), *Mh = o.A_("Mh"), *Rho = o.A_("Rho");
+    const double ns = o.spq[0], Ruv = o.spq[1], Rw = o.spq[2], RT = o.spq[7];
+    const bool dmu = o.spq[4] != 0.0, dmw = o.spq[5] != 0.0, dmt = o.spq[6] != 0.0;
+    double *W = o.A_("W"), *Wh = o.A_("Wh"), *Mh = o.A_("Mh"), *Rho = o.A_("Rho"), *pr = o.A_("pressure"),
+           *Rd = o.A_("Rd");
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat;
     const double dt = o.p.timestep;
     for (int id = 0; id < P; id++)
@@ -1747,6 +1755,18 @@ void sponge(Oracle &o) {
                 if (dmw) vbar[v3 + 2] = oracle_tree_sum(&wt[row], L);
             }
     }
+    std::vector<double> tbar((size_t)nlat * nv, 0.0);
+    if (dmt) {
+        std::vector<double> tt((size_t)nlat * nv * L, 0.0);
+        for (int id = 0; id < P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                const int b = o.ztab[id * 3], cnt = o.ztab[id * 3 + 1], k = o.ztab[id * 3 + 2];
+                const double temperature = pr[id * nv + lev] / Rd[id * nv + lev] / Rho[id * nv + lev];
+                tt[((size_t)b * nv + lev) * L + k] = temperature / cnt;
+            }
+        for (int b = 0; b < nlat; b++)
+            for (int lev = 0; lev < nv; lev++) tbar[(size_t)b * nv + lev] = oracle_tree_sum(&tt[((size_t)b * nv + lev) * L], L);
+    }
     const double ztop = Alth[nv], des_lat = M_PI / nlat;
     for (int id = 0; id < P; id++) {
         const int intlat = o.ztab[id * 3];
@@ -1754,8 +1774,20 @@ void sponge(Oracle &o) {
         const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
         const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
         const double lat = ll[id * 2 + 1], lon = lon_of(id);
-        // zonal mean of component c at this latitude (interpolated between bin centres)
+        // zonal mean of component c at this latitude (interpolated between bin
+        // centres); c = 3: the temperature
         auto mean_at = [&](int lev, int c) {
+            if (c == 3) {
+                const double *tb = tbar.data();
+                if (lat <= -M_PI / 2 + 0.5 * des_lat) return tb[(size_t)intlat * nv + lev];
+                if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return tb[(size_t)intlat * nv + lev];
+                if (lat >= lat1 && lat < lat2) {
+                    const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
+                    return tb[(size_t)(intlat - 1) * nv + lev] * intt + tb[(size_t)intlat * nv + lev] * intl;
+                }
+                const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
+                return tb[(size_t)intlat * nv + lev] * intt + tb[(size_t)(intlat + 1) * nv + lev] * intl;
+            }
             const double *vb = vbar.data();
             if (lat <= -M_PI / 2 + 0.5 * des_lat) return vb[((size_t)intlat * nv + lev) * 3 + c];
             if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vb[((size_t)intlat * nv + lev) * 3 + c];
@@ -1794,6 +1826,12 @@ void sponge(Oracle &o) {
             Mh[(id * nv + lev) * 3 + 1] = vy * rho;
             Mh[(id * nv + lev) * 3 + 2] = vz * rho;
             W[id * nv + lev] = wnew * rho;
+            if (dmt) {
+                const double kT = n >= ns ? 1.0 * RT * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0) : 0.0;
+                const double temperature = pr[id * nv + lev] / Rd[id * nv + lev] / Rho[id * nv + lev];
+                pr[id * nv + lev] = Rho[id * nv + lev] * Rd[id * nv + lev] * (temperature / dt + kT * mean_at(lev, 3))
+                                    / (1.0 / dt + kT);
+            }
         }
         for (int lev = 1; lev < nv; lev++) {
             const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
@@ -1958,7 +1996,7 @@ void oracle_steps(oracle *o, int n) {
 }
 void oracle_set_newtonian(oracle *o, const double q[8]) { std::memcpy(o->nrq, q, sizeof(o->nrq)); }
 void oracle_set_rt(oracle *o, const double q[11]) { std::memcpy(o->rtq, q, sizeof(o->rtq)); }
-void oracle_set_sponge(oracle *o, const double q[6]) {
+void oracle_set_sponge(oracle *o, const double q[8]) {
     o->sponge_on = q != nullptr;
     if (!q) return;
     std::memcpy(o->spq, q, sizeof(o->spq));

@@ -113,16 +113,18 @@ def test_hot_jupiter_mixed_stays_close_to_fp64(hj, forcing):
         assert e < t, k
 
 
-@pytest.mark.parametrize("forcing", ["newtonian", "double_gray"])
-def test_sponge_steps_bitwise(hj, forcing):
-    """Parity build with the Rayleigh sponge (damping to the zonal means) in
-    ProfX: two steps reproduce the oracle bit for bit."""
+@pytest.mark.parametrize("forcing,temp", [("newtonian", 0), ("double_gray", 0), ("newtonian", 1), ("double_gray", 1)])
+def test_sponge_steps_bitwise(hj, forcing, temp):
+    """Parity build with the Rayleigh sponge (damping to the zonal means, with
+    and without the temperature sponge) in ProfX: two steps reproduce the
+    oracle bit for bit."""
     from oracle.oracle import Oracle
     capi, g, p, st0 = hj
     code = capi.FORCING_NEWTONIAN if forcing == "newtonian" else capi.FORCING_DOUBLE_GRAY
-    print(f"case: parity build, hot Jupiter g4 L20 with winds, sponge + {forcing}, 2 steps", flush=True)
+    print(f"case: parity build, hot Jupiter g4 L20 with winds, sponge (temperature {temp}) + {forcing}, 2 steps",
+          flush=True)
     st = capi.ic_rest(g, p, seed=1007, temperature_noise=1.0, wind_noise=20.0)
-    q = capi.SpongeParams(0.6, 2e-4, 1e-4, 16, 1, 1)
+    q = capi.SpongeParams(0.6, 2e-4, 1e-4, 16, 1, 1, temp, 3e-4)
     d = capi.Dyn(g, p, variant="pm")
     o = Oracle(g, p, variant="pm")
     for x in (d, o):
@@ -155,15 +157,16 @@ def test_sponge_steps_bitwise(hj, forcing):
 
 def test_sponge_many_slot_blocks_bitwise(built):
     """Two latitude bands on g5 (about 5 100 points each, 8 192 slots): the
-    zonal sums run over four 2 048-slot blocks per row, added in order. One
-    Newtonian ProfX + Thor step of the parity build equals the oracle."""
+    zonal sums (winds and temperature) run over four 2 048-slot blocks per
+    row, added in order. One Newtonian ProfX + Thor step of the parity build
+    equals the oracle."""
     from oracle.oracle import Oracle
     capi = built
     g = hj_grid(capi, 5, 20)
     p = hj_params(capi, 5)
     print("case: parity build, hot Jupiter g5 L20 with winds, sponge over 2 bands, 1 step", flush=True)
     st = capi.ic_rest(g, p, seed=1009, temperature_noise=1.0, wind_noise=20.0)
-    q = capi.SpongeParams(0.5, 3e-4, 2e-4, 2, 1, 1)
+    q = capi.SpongeParams(0.5, 3e-4, 2e-4, 2, 1, 1, 1, 2e-4)
     d = capi.Dyn(g, p, variant="pm")
     o = Oracle(g, p, variant="pm")
     for x in (d, o):

@@ -12,7 +12,12 @@ profx_driver.cu:92-195, profx_sponge.cu:50-437) pinned by closed forms (CPU).
   complete: the winds of a point in a polar half-band become the arithmetic
   mean of its band (numpy's mean over the band's points); a wind field that
   is the same everywhere is left as it is.
-The Newtonian forcing, which leaves Mh and W alone, drives ProfX.
+* Temperature sponge (RayleighSpongeT): with a rate so large that the
+  relaxation is complete, the temperature p / (Rd rho) of a point in a polar
+  half-band becomes the arithmetic mean of its band; a temperature that is
+  the same in every column of a level is left as it is.
+The Newtonian forcing drives ProfX; the sponge tests read the state right
+after the sponge by running the forcing with a zero rate where it matters.
 """
 from __future__ import annotations
 
@@ -151,4 +156,55 @@ def test_uniform_wind_is_left_alone(hj):
     got = o.get("Mh").reshape(P, nv, 3)
     err = np.abs(got - M).max() / np.abs(M).max()
     print(f"uniform (u, v) = ({u0}, {v0}) after the sponge: max rel change {err:.2e}")
+    assert err < 1e-12
+
+
+def _sponge_only(capi, g, p, st, q):
+    """The oracle state right after the sponge: the sponge runs at the start
+    of ProfX; the Newtonian forcing after it gets zero rates."""
+    from oracle.oracle import Oracle
+    o = Oracle(g, p)
+    o.set_state(st)
+    nr = nr_params(capi)
+    nr.tau_top = nr.tau_bottom = 1e300  # relaxation times so long that T is left as it is
+    o.set_newtonian(nr)
+    o.set_sponge(q)
+    o.profx_forcing(capi.FORCING_NEWTONIAN)
+    return o
+
+
+def test_full_temperature_damping_to_the_zonal_mean(hj):
+    capi, g, p, st = hj
+    nlat, P, nv = 10, g.P, g.nv
+    o = _sponge_only(capi, g, p, st, capi.SpongeParams(0.0, 0.0, 0.0, nlat, 0, 0, 1, 1e9))
+    t, _ = o.zonal_table()
+    b = t[:, 0]
+    lat = g.lonlat.reshape(P, 2)[:, 1]
+    T0 = (st["pressure"] / (p.Rd * st["Rho"])).reshape(P, nv)
+    T1 = (o.get("pressure") / (p.Rd * o.get("Rho"))).reshape(P, nv)
+    des = np.pi / nlat
+    polar = (lat <= -np.pi / 2 + 0.5 * des) | (lat >= -np.pi / 2 + 0.5 * des + des * (nlat - 1))
+    lev = nv - 1
+    for j in (0, nlat - 1):
+        ids = np.nonzero(polar & (b == j))[0]
+        assert len(ids) > 0
+        mt = T0[b == j, lev].mean()
+        e = np.abs(T1[ids, lev] - mt).max() / mt
+        print(f"band {j}: {len(ids)} polar points, T vs band mean {e:.1e} (spread before {np.ptp(T0[b == j, lev]):.2f} K)")
+        assert e < 1e-12
+    # the wind rates are zero: the winds are rebuilt unchanged (to rounding)
+    M0 = st["Mh"].reshape(-1)
+    assert np.abs(o.get("Mh") - M0).max() < 1e-13 * np.abs(M0).max()
+
+
+def test_uniform_temperature_is_left_alone(hj):
+    capi, g, p, st = hj
+    P, nv = g.P, g.nv
+    s2 = dict(st)
+    Tl = 1500.0 + 10.0 * np.arange(nv)
+    s2["pressure"] = (p.Rd * st["Rho"].reshape(P, nv) * Tl[None, :]).reshape(st["pressure"].shape)
+    o = _sponge_only(capi, g, p, s2, capi.SpongeParams(0.3, 0.0, 0.0, 20, 0, 0, 1, 1e-3))
+    p0 = np.asarray(s2["pressure"]).reshape(-1)
+    err = np.abs(o.get("pressure") - p0).max() / np.abs(p0).max()
+    print(f"level-uniform T after the temperature sponge: max rel change of p {err:.2e}")
     assert err < 1e-12

@@ -422,10 +422,11 @@ static int profx(Dyn &d, int forcing) {
         launch(d, "sponge_zonal", 10 * B, [&] {
             k_sponge_zonal<<<gpl, blk, 0, d.stream>>>(g, f, sp, d.ztab, d.ztmp);
         });
-        if (sp.dmu || sp.dmw) {
-            launch(d, "sponge_mean", 8.0 * 3 * sp.nlat * d.nv * sp.max_count, [&] {
-                k_sponge_reduce<<<d.nv * 3 * sp.nlat, 1024, 0, d.stream>>>(sp, d.nv, d.ztmp, d.zout);
-                k_sponge_vbar<<<(3 * sp.nlat * d.nv + 255) / 256, 256, 0, d.stream>>>(sp, d.nv, d.zout, d.vbar);
+        if (sp.dmu || sp.dmw || sp.dmt) {
+            const int nc = sp.ncomp();
+            launch(d, "sponge_mean", 8.0 * nc * sp.nlat * d.nv * sp.max_count, [&] {
+                k_sponge_reduce<<<d.nv * nc * sp.nlat, 1024, 0, d.stream>>>(sp, d.nv, d.ztmp, d.zout);
+                k_sponge_vbar<<<(nc * sp.nlat * d.nv + 255) / 256, 256, 0, d.stream>>>(sp, d.nv, d.zout, d.vbar);
             });
         }
         // Mh 3, W, rho in; Mh 3, W out; then W 2 in, Wh out
@@ -1046,7 +1047,8 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
         h->sponge_on = false;
         return THOR_OK;
     }
-    if (q->nlat_bins < 2 || q->nlat_bins > 1000 || !(q->ns >= 0.0 && q->ns < 1.0) || q->Ruv < 0 || q->Rw < 0)
+    if (q->nlat_bins < 2 || q->nlat_bins > 1000 || !(q->ns >= 0.0 && q->ns < 1.0) || q->Ruv < 0 || q->Rw < 0
+        || (q->temp_sponge && q->RT_sponge < 0))
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_set_sponge: need 2 <= nlat_bins <= 1000, 0 <= ns < 1, rates >= 0");
     const int P = h->P, S = h->S, nv = h->nv, nlat = q->nlat_bins;
     // latitude bins (grid.cu:3215-3262): edges accumulated from -pi/2, index
@@ -1077,7 +1079,7 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     }
     const int max_count = (int)std::pow(2, std::ceil(std::log(mc) / std::log(2)));
     SpongePar sp{q->ns, q->Ruv, q->Rw, nlat, q->damp_uv_to_mean != 0, q->damp_w_to_mean != 0, max_count,
-                 (max_count + 2047) / 2048};
+                 (max_count + 2047) / 2048, q->temp_sponge != 0, q->temp_sponge ? q->RT_sponge : 0.0};
     // (re)allocate the tables; the zonal slots beyond each bin's count stay zero
     for (void *ptr : {(void *)h->ztab, (void *)h->ztmp, (void *)h->zout, (void *)h->vbar, (void *)h->ztrig,
                       (void *)h->zkf, (void *)h->zcnt}) {
@@ -1099,11 +1101,12 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
         h->allocs.push_back(h->zcnt);
         e = hipMemcpy(h->zcnt, count.data(), count.size() * sizeof(int), hipMemcpyHostToDevice);
     }
-    if (e == hipSuccess) e = dalloc(*h, &h->ztmp, (size_t)3 * nlat * nv * max_count);
-    if (e == hipSuccess) e = dalloc(*h, &h->zout, (size_t)3 * nlat * nv * sp.nb);
-    if (e == hipSuccess) e = dalloc(*h, &h->vbar, (size_t)3 * nlat * nv);
+    const int nc = sp.ncomp();
+    if (e == hipSuccess) e = dalloc(*h, &h->ztmp, (size_t)nc * nlat * nv * max_count);
+    if (e == hipSuccess) e = dalloc(*h, &h->zout, (size_t)nc * nlat * nv * sp.nb);
+    if (e == hipSuccess) e = dalloc(*h, &h->vbar, (size_t)4 * nlat * nv);
     if (e == hipSuccess) e = dalloc(*h, &h->ztrig, (size_t)4 * S);
-    if (e == hipSuccess) e = dalloc(*h, &h->zkf, (size_t)2 * nv);
+    if (e == hipSuccess) e = dalloc(*h, &h->zkf, (size_t)3 * nv);
     if (e == hipSuccess) {
         sp.trig = h->ztrig;
         sp.kf = h->zkf;

@@ -3620,9 +3620,12 @@ __device__ __forceinline__ void cell_diag(const Geo &g, const Fields &f, long e,
 struct SpongePar {
     double ns, Ruv, Rw;
     int nlat, dmu, dmw, max_count, nb;  // nb = blocks of 2048 slots per row
+    int dmt;                            // temperature sponge (RayleighSpongeT)
+    double RT;
     const double *trig;                 // [4][S]: sin, cos of the longitude (in [0, 2 pi)) and of the latitude
-    const double *kf;                   // [2][nv]: the implicit rates of u, v and of w per level
+    const double *kf;                   // [3][nv]: the implicit rates of u, v, of w and of T per level
     const int *cnt;                     // [nlat]: points per latitude bin
+    __host__ __device__ int ncomp() const { return dmt ? 4 : 3; }  // zonal-mean components (u, v, w, T)
 };
 
 // The per-point and per-level factors of the sponge, formed once with the
@@ -3640,13 +3643,15 @@ __global__ __launch_bounds__(256) void k_sponge_tables(Geo g, SpongePar sp, doub
     }
     if (i < nv) {
         const double n = g.Alt[i] / g.Alth[nv];
-        double kuv = 0.0, kw = 0.0;
+        double kuv = 0.0, kw = 0.0, kT = 0.0;
         if (n >= sp.ns) {
             kuv = 1.0 * sp.Ruv * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
             kw = 1.0 * sp.Rw * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
+            kT = 1.0 * sp.RT * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
         }
         kf[i] = kuv;
         kf[nv + i] = kw;
+        kf[2 * nv + i] = kT;
     }
 }
 
@@ -3662,7 +3667,7 @@ __device__ __forceinline__ void sponge_uv(const Geo &g, const Fields &f, const S
 }
 
 // ztab: [3][S] latitude bin, points in the bin, index within the bin;
-// tmp: [3][nlat][nv][max_count] (u, v, w), zero beyond each bin's count.
+// tmp: [ncomp][nlat][nv][max_count] (u, v, w, T), zero beyond each bin's count.
 __global__ __launch_bounds__(256) void k_sponge_zonal(Geo g, Fields f, SpongePar sp, const int *ztab, double *tmp) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
     if (p >= g.P) return;
@@ -3674,15 +3679,18 @@ __global__ __launch_bounds__(256) void k_sponge_zonal(Geo g, Fields f, SpongePar
     const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
     const double w = whl * intt + wht * intl;
     LV(f.W, lev) = w;
-    if (!sp.dmu && !sp.dmw) return;
-    double u, v;
-    sponge_uv(g, f, sp, p, lev, u, v);
+    if (!sp.dmu && !sp.dmw && !sp.dmt) return;
     const int cnt = ztab[S + p];
     const size_t T = (size_t)sp.nlat * nv * sp.max_count;
     const size_t at = ((size_t)ztab[p] * nv + lev) * sp.max_count + ztab[2 * S + p];
-    tmp[at] = u / cnt;
-    tmp[T + at] = v / cnt;
-    tmp[2 * T + at] = (w / LV(f.Rho, lev)) / cnt;
+    if (sp.dmu || sp.dmw) {
+        double u, v;
+        sponge_uv(g, f, sp, p, lev, u, v);
+        tmp[at] = u / cnt;
+        tmp[T + at] = v / cnt;
+        tmp[2 * T + at] = (w / LV(f.Rho, lev)) / cnt;
+    }
+    if (sp.dmt) tmp[3 * T + at] = (LV(f.p, lev) / g.Rd / LV(f.Rho, lev)) / cnt;  // zonal_temp
 }
 
 // One block per row (component, bin, level): the row's 2048-slot blocks in
@@ -3714,15 +3722,15 @@ __global__ __launch_bounds__(1024) void k_sponge_reduce(SpongePar sp, int nv, co
     }
 }
 
-// vbar[bin][lev][component] = 0 + the block sums in order (the host loop of
-// gpu_sum_on_device_sponge).
+// vbar[bin][lev][component, stride 4] = 0 + the block sums in order (the
+// host loops of gpu_sum_on_device_sponge and gpu_sum_on_device).
 __global__ void k_sponge_vbar(SpongePar sp, int nv, const double *out, double *vbar) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // (component, bin, lev)
-    if (i >= 3 * sp.nlat * nv) return;
+    if (i >= sp.ncomp() * sp.nlat * nv) return;
     const int comp = i / (sp.nlat * nv), rem = i % (sp.nlat * nv);
     double s = 0.0;
     for (int b = 0; b < sp.nb; b++) s += out[(size_t)i * sp.nb + b];
-    vbar[(size_t)rem * 3 + comp] = s;
+    vbar[(size_t)rem * 4 + comp] = s;
 }
 
 __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar sp, const int *ztab,
@@ -3740,14 +3748,14 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     sponge_uv(g, f, sp, p, lev, u, v);
     const double lat = g.lonlat[S + p];
     auto mean_at = [&](int c) {
-        if (lat <= -M_PI / 2 + 0.5 * des_lat) return vbar[((size_t)intlat * nv + lev) * 3 + c];
-        if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vbar[((size_t)intlat * nv + lev) * 3 + c];
+        if (lat <= -M_PI / 2 + 0.5 * des_lat) return vbar[((size_t)intlat * nv + lev) * 4 + c];
+        if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vbar[((size_t)intlat * nv + lev) * 4 + c];
         if (lat >= lat1 && lat < lat2) {
             const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
-            return vbar[((size_t)(intlat - 1) * nv + lev) * 3 + c] * intt + vbar[((size_t)intlat * nv + lev) * 3 + c] * intl;
+            return vbar[((size_t)(intlat - 1) * nv + lev) * 4 + c] * intt + vbar[((size_t)intlat * nv + lev) * 4 + c] * intl;
         }
         const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
-        return vbar[((size_t)intlat * nv + lev) * 3 + c] * intt + vbar[((size_t)(intlat + 1) * nv + lev) * 3 + c] * intl;
+        return vbar[((size_t)intlat * nv + lev) * 4 + c] * intt + vbar[((size_t)(intlat + 1) * nv + lev) * 4 + c] * intl;
     };
     const double rho = LV(f.Rho, lev);
     const double w = LV(f.W, lev) / rho;
@@ -3770,6 +3778,10 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     VEC(f.Mh, 1, lev, p) = vy * rho;
     VEC(f.Mh, 2, lev, p) = vz * rho;
     LV(f.W, lev) = wnew * rho;
+    if (sp.dmt) {  // temperature towards its zonal mean, through the pressure
+        const double kT = sp.kf[2 * nv + lev], temperature = LV(f.p, lev) / g.Rd / rho;
+        LV(f.p, lev) = rho * g.Rd * (temperature / dt + kT * mean_at(3)) / (1.0 / dt + kT);
+    }
 }
 
 // Wh at interior interface lev (1..nv-1) from the relaxed W (blockIdx.y = lev - 1).
