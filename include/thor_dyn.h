@@ -219,6 +219,12 @@ typedef struct thor_sponge_params {
     int nlat_bins, damp_uv_to_mean, damp_w_to_mean;
     int temp_sponge;  /* appended: zero-filled callers keep the wind-only sponge */
     double RT_sponge; /* [1/s] */
+    /* raysp_calc_mode: 0 imp (above, at the start of ProfX); 1 exp1 / 3 exp3:
+     * in every RK stage before the slow modes, the drag -k (u - ubar) rho etc.
+     * goes into profx_dMh / profx_dWh and the heating into profx_Qheat, with
+     * the zonal means of the stage values formed in the first stage (exp1) or
+     * in every stage (exp3) (thor_driver.cu:749-860). */
+    int raysp_calc_mode;
 } thor_sponge_params;
 thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params *p);
 

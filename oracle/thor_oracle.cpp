@@ -49,7 +49,9 @@ struct Oracle {
     // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean,
     // temp_sponge, RT_sponge
     bool sponge_on = false;
-    double spq[8] = {0};
+    double spq[9] = {0};  // ..., [8] raysp_calc_mode: 0 imp, 1 exp1, 3 exp3
+    std::vector<double> vbar, tbar;  // the sponge's zonal means (kept over the stages in exp1)
+    bool sponge_imp() const { return sponge_on && (int)spq[8] == 0; }
     std::vector<int> ztab;  // [P][3]: latitude bin, points in the bin, index within the bin
     int max_count = 0;
     // vertical solve: keep each column's tridiagonal system (a, b, c, d per
@@ -58,6 +60,8 @@ struct Oracle {
 
     double *A_(const char *n) { return arr[n].data(); }
 };
+
+void sponge_explicit(Oracle &o, int rk);
 
 // Fills a shared tile: f(global index) for loaded positions whose map entry is
 // valid, 0.0 where the map holds -1 (missing pentagon corner).
@@ -1414,6 +1418,7 @@ void thor(Oracle &o) {
             divm_pass(o, "Mhk", "Whk", false);
             divm_pass(o, "Mhk", "Whk", true);
         }
+        if (o.sponge_on && !o.sponge_imp()) sponge_explicit(o, rk);
         slow_modes(o);
         if (rk > 0) update_rk(o);
         for (int ns = 0; ns < ns_it; ns++) {
@@ -1447,7 +1452,7 @@ void profx_hs(Oracle &o) {
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     zero(o, "profx_Qheat");
     // Recompute_W (profx_auxiliary.h:110-130); with the sponge it ran before it
-    for (int id = 0; id < o.P && !o.sponge_on; id++)
+    for (int id = 0; id < o.P && !o.sponge_imp(); id++)
         for (int lev = 0; lev < nv; lev++) {
             double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
             double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
@@ -1514,7 +1519,7 @@ void profx_nr(Oracle &o) {
             double whl = Wh[id * (nv + 1) + lev], wht = Wh[id * (nv + 1) + lev + 1];
             double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            if (!o.sponge_on) W[e] = whl * intt + wht * intl;
+            if (!o.sponge_imp()) W[e] = whl * intt + wht * intl;
             const double pre = p[e], rho = Rho[e];
             double t = pre / (Rd * rho);
             const double lnp = MLOG(pre);
@@ -1573,7 +1578,7 @@ void profx_rt(Oracle &o) {
             double whl = Wh[id * nvi + lev], wht = Wh[id * nvi + lev + 1];
             double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
             double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            if (!o.sponge_on) W[e] = whl * intt + wht * intl;
+            if (!o.sponge_imp()) W[e] = whl * intt + wht * intl;
             T[e] = p[e] / (Rd * Rho[e]);
         }
     std::vector<double> phtemp(nvi), thtemp(nvi), ttemp(nv), tau(2 * nv), dtemp(nv), fsw_up(nvi), fsw_dn(nvi),
@@ -1709,9 +1714,82 @@ void zonal_table(Oracle &o, int nlat) {
 // rho Rd (T / dt + kT Tbar) / (1 / dt + kT), profx_sponge.cu:388-404). THOR
 // does not clear the slots of the temperature table that no point writes;
 // here every unwritten slot is zero.
-void sponge(Oracle &o) {
+// Zonal means of u, v, w (and T) per (bin, level) from the given state
+// arrays into o.vbar [bin][lev][3] and o.tbar [bin][lev].
+void sponge_means(Oracle &o, const double *Mh, const double *Rho, const double *W, const double *pr) {
     const int nv = o.nv, P = o.P, nlat = (int)o.spq[3];
-    const double ns = o.spq[0], Ruv = o.spq[1], Rw = o.spq[2], RT = o.spq[7];
+    const bool dmu = o.spq[4] != 0.0, dmw = o.spq[5] != 0.0, dmt = o.spq[6] != 0.0;
+    const double *Rd = o.A_("Rd"), *ll = o.g.lonlat;
+    const long L = o.max_count;
+    o.vbar.assign((size_t)nlat * nv * 3, 0.0);
+    o.tbar.assign((size_t)nlat * nv, 0.0);
+    if (dmu || dmw) {
+        std::vector<double> ut((size_t)nlat * nv * L, 0.0), vt(ut.size(), 0.0), wt(ut.size(), 0.0);
+        for (int id = 0; id < P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                const int b = o.ztab[id * 3], cnt = o.ztab[id * 3 + 1], k = o.ztab[id * 3 + 2];
+                const size_t at = ((size_t)b * nv + lev) * L + k;
+                const double lon = ll[id * 2] < 0 ? ll[id * 2] + 2.0 * M_PI : ll[id * 2], lat = ll[id * 2 + 1];
+                const double rho = Rho[id * nv + lev], *M = &Mh[(id * nv + lev) * 3];
+                const double u = (M[0] * (-MSIN(lon)) + M[1] * (MCOS(lon)) + M[2] * 0) / rho;
+                const double v = (M[0] * (-MSIN(lat) * MCOS(lon)) + M[1] * (-MSIN(lat) * MSIN(lon)) + M[2] * (MCOS(lat))) / rho;
+                ut[at] = u / cnt;
+                vt[at] = v / cnt;
+                wt[at] = (W[id * nv + lev] / rho) / cnt;
+            }
+        for (int b = 0; b < nlat; b++)
+            for (int lev = 0; lev < nv; lev++) {
+                const size_t row = ((size_t)b * nv + lev) * L, v3 = ((size_t)b * nv + lev) * 3;
+                if (dmu) {
+                    o.vbar[v3 + 0] = oracle_tree_sum(&ut[row], L);
+                    o.vbar[v3 + 1] = oracle_tree_sum(&vt[row], L);
+                }
+                if (dmw) o.vbar[v3 + 2] = oracle_tree_sum(&wt[row], L);
+            }
+    }
+    if (dmt) {
+        std::vector<double> tt((size_t)nlat * nv * L, 0.0);
+        for (int id = 0; id < P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                const int b = o.ztab[id * 3], cnt = o.ztab[id * 3 + 1], k = o.ztab[id * 3 + 2];
+                const double temperature = pr[id * nv + lev] / Rd[id * nv + lev] / Rho[id * nv + lev];
+                tt[((size_t)b * nv + lev) * L + k] = temperature / cnt;
+            }
+        for (int b = 0; b < nlat; b++)
+            for (int lev = 0; lev < nv; lev++)
+                o.tbar[(size_t)b * nv + lev] = oracle_tree_sum(&tt[((size_t)b * nv + lev) * L], L);
+    }
+}
+
+// The zonal mean of component c (0 u, 1 v, 2 w, 3 T) at point id, level lev:
+// the polar bands' own value, else interpolated between bin centres.
+double sponge_mean_at(const Oracle &o, int id, int lev, int c) {
+    const int nv = o.nv, nlat = (int)o.spq[3], intlat = o.ztab[id * 3];
+    const double des_lat = M_PI / nlat, lat = o.g.lonlat[id * 2 + 1];
+    const double lat1 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat - 1);
+    const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
+    const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
+    auto at = [&](int b) { return c == 3 ? o.tbar[(size_t)b * nv + lev] : o.vbar[((size_t)b * nv + lev) * 3 + c]; };
+    if (lat <= -M_PI / 2 + 0.5 * des_lat) return at(intlat);
+    if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return at(intlat);
+    if (lat >= lat1 && lat < lat2) {
+        const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
+        return at(intlat - 1) * intt + at(intlat) * intl;
+    }
+    const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
+    return at(intlat) * intt + at(intlat + 1) * intl;
+}
+
+// The sponge rate of level lev for the rate R: R sin^2(pi/2 (n - ns) / (1 - ns))
+// above n = Altitude / Top = ns.
+double sponge_rate(const Oracle &o, int lev, double R) {
+    const double n = o.g.Altitude[lev] / o.g.Altitudeh[o.nv], ns = o.spq[0];
+    return n >= ns ? 1.0 * R * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0) : 0.0;
+}
+
+void sponge(Oracle &o) {
+    const int nv = o.nv, P = o.P;
+    const double Ruv = o.spq[1], Rw = o.spq[2], RT = o.spq[7];
     const bool dmu = o.spq[4] != 0.0, dmw = o.spq[5] != 0.0, dmt = o.spq[6] != 0.0;
     double *W = o.A_("W"), *Wh = o.A_("Wh"), *Mh = o.A_("Mh"), *Rho = o.A_("Rho"), *pr = o.A_("pressure"),
            *Rd = o.A_("Rd");
@@ -1724,100 +1802,25 @@ void sponge(Oracle &o) {
             double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
             W[id * nv + lev] = whl * intt + wht * intl;
         }
-    auto lon_of = [&](int id) { return ll[id * 2] < 0 ? ll[id * 2] + 2.0 * M_PI : ll[id * 2]; };
-    auto uv = [&](int id, int lev, double &u, double &v) {
-        const double lon = lon_of(id), lat = ll[id * 2 + 1], rho = Rho[id * nv + lev];
-        const double *M = &Mh[(id * nv + lev) * 3];
-        u = (M[0] * (-MSIN(lon)) + M[1] * (MCOS(lon)) + M[2] * 0) / rho;
-        v = (M[0] * (-MSIN(lat) * MCOS(lon)) + M[1] * (-MSIN(lat) * MSIN(lon)) + M[2] * (MCOS(lat))) / rho;
-    };
-    const long L = o.max_count;
-    std::vector<double> vbar((size_t)nlat * nv * 3, 0.0);
-    if (dmu || dmw) {
-        std::vector<double> ut((size_t)nlat * nv * L, 0.0), vt(ut.size(), 0.0), wt(ut.size(), 0.0);
-        for (int id = 0; id < P; id++)
-            for (int lev = 0; lev < nv; lev++) {
-                const int b = o.ztab[id * 3], cnt = o.ztab[id * 3 + 1], k = o.ztab[id * 3 + 2];
-                const size_t at = ((size_t)b * nv + lev) * L + k;
-                double u, v;
-                uv(id, lev, u, v);
-                ut[at] = u / cnt;
-                vt[at] = v / cnt;
-                wt[at] = (W[id * nv + lev] / Rho[id * nv + lev]) / cnt;
-            }
-        for (int b = 0; b < nlat; b++)
-            for (int lev = 0; lev < nv; lev++) {
-                const size_t row = ((size_t)b * nv + lev) * L, v3 = ((size_t)b * nv + lev) * 3;
-                if (dmu) {
-                    vbar[v3 + 0] = oracle_tree_sum(&ut[row], L);
-                    vbar[v3 + 1] = oracle_tree_sum(&vt[row], L);
-                }
-                if (dmw) vbar[v3 + 2] = oracle_tree_sum(&wt[row], L);
-            }
-    }
-    std::vector<double> tbar((size_t)nlat * nv, 0.0);
-    if (dmt) {
-        std::vector<double> tt((size_t)nlat * nv * L, 0.0);
-        for (int id = 0; id < P; id++)
-            for (int lev = 0; lev < nv; lev++) {
-                const int b = o.ztab[id * 3], cnt = o.ztab[id * 3 + 1], k = o.ztab[id * 3 + 2];
-                const double temperature = pr[id * nv + lev] / Rd[id * nv + lev] / Rho[id * nv + lev];
-                tt[((size_t)b * nv + lev) * L + k] = temperature / cnt;
-            }
-        for (int b = 0; b < nlat; b++)
-            for (int lev = 0; lev < nv; lev++) tbar[(size_t)b * nv + lev] = oracle_tree_sum(&tt[((size_t)b * nv + lev) * L], L);
-    }
-    const double ztop = Alth[nv], des_lat = M_PI / nlat;
+    sponge_means(o, Mh, Rho, W, pr);
     for (int id = 0; id < P; id++) {
-        const int intlat = o.ztab[id * 3];
-        const double lat1 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat - 1);
-        const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
-        const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
-        const double lat = ll[id * 2 + 1], lon = lon_of(id);
-        // zonal mean of component c at this latitude (interpolated between bin
-        // centres); c = 3: the temperature
-        auto mean_at = [&](int lev, int c) {
-            if (c == 3) {
-                const double *tb = tbar.data();
-                if (lat <= -M_PI / 2 + 0.5 * des_lat) return tb[(size_t)intlat * nv + lev];
-                if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return tb[(size_t)intlat * nv + lev];
-                if (lat >= lat1 && lat < lat2) {
-                    const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
-                    return tb[(size_t)(intlat - 1) * nv + lev] * intt + tb[(size_t)intlat * nv + lev] * intl;
-                }
-                const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
-                return tb[(size_t)intlat * nv + lev] * intt + tb[(size_t)(intlat + 1) * nv + lev] * intl;
-            }
-            const double *vb = vbar.data();
-            if (lat <= -M_PI / 2 + 0.5 * des_lat) return vb[((size_t)intlat * nv + lev) * 3 + c];
-            if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vb[((size_t)intlat * nv + lev) * 3 + c];
-            if (lat >= lat1 && lat < lat2) {
-                const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
-                return vb[((size_t)(intlat - 1) * nv + lev) * 3 + c] * intt + vb[((size_t)intlat * nv + lev) * 3 + c] * intl;
-            }
-            const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
-            return vb[((size_t)intlat * nv + lev) * 3 + c] * intt + vb[((size_t)(intlat + 1) * nv + lev) * 3 + c] * intl;
-        };
+        const double lat = ll[id * 2 + 1], lon = ll[id * 2] < 0 ? ll[id * 2] + 2.0 * M_PI : ll[id * 2];
         for (int lev = 0; lev < nv; lev++) {
             const double rho = Rho[id * nv + lev];
-            double u, v;
-            uv(id, lev, u, v);
+            const double *M = &Mh[(id * nv + lev) * 3];
+            const double u = (M[0] * (-MSIN(lon)) + M[1] * (MCOS(lon)) + M[2] * 0) / rho;
+            const double v = (M[0] * (-MSIN(lat) * MCOS(lon)) + M[1] * (-MSIN(lat) * MSIN(lon)) + M[2] * (MCOS(lat))) / rho;
             const double w = W[id * nv + lev] / rho;
-            const double n = Alt[lev] / ztop;
-            double kuv = 0.0, kw = 0.0;
-            if (n >= ns) {
-                kuv = 1.0 * Ruv * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0);
-                kw = 1.0 * Rw * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0);
-            }
+            const double kuv = sponge_rate(o, lev, Ruv), kw = sponge_rate(o, lev, Rw);
             double unew, vnew, wnew;
             if (dmu) {
-                unew = (u / dt + kuv * mean_at(lev, 0)) / (1.0 / dt + kuv);
-                vnew = (v / dt + kuv * mean_at(lev, 1)) / (1.0 / dt + kuv);
+                unew = (u / dt + kuv * sponge_mean_at(o, id, lev, 0)) / (1.0 / dt + kuv);
+                vnew = (v / dt + kuv * sponge_mean_at(o, id, lev, 1)) / (1.0 / dt + kuv);
             } else {
                 unew = (u / dt) / (1.0 / dt + kuv);
                 vnew = (v / dt) / (1.0 / dt + kuv);
             }
-            if (dmw) wnew = (w / dt + kw * mean_at(lev, 2)) / (1.0 / dt + kw);
+            if (dmw) wnew = (w / dt + kw * sponge_mean_at(o, id, lev, 2)) / (1.0 / dt + kw);
             else wnew = (w / dt) / (1.0 / dt + kw);
             const double vx = unew * (-MSIN(lon)) + vnew * (-MSIN(lat) * MCOS(lon));
             const double vy = unew * (MCOS(lon)) + vnew * (-MSIN(lat) * MSIN(lon));
@@ -1827,16 +1830,76 @@ void sponge(Oracle &o) {
             Mh[(id * nv + lev) * 3 + 2] = vz * rho;
             W[id * nv + lev] = wnew * rho;
             if (dmt) {
-                const double kT = n >= ns ? 1.0 * RT * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0) : 0.0;
+                const double kT = sponge_rate(o, lev, RT);
                 const double temperature = pr[id * nv + lev] / Rd[id * nv + lev] / Rho[id * nv + lev];
-                pr[id * nv + lev] = Rho[id * nv + lev] * Rd[id * nv + lev] * (temperature / dt + kT * mean_at(lev, 3))
-                                    / (1.0 / dt + kT);
+                pr[id * nv + lev] = Rho[id * nv + lev] * Rd[id * nv + lev]
+                                    * (temperature / dt + kT * sponge_mean_at(o, id, lev, 3)) / (1.0 / dt + kT);
             }
         }
         for (int lev = 1; lev < nv; lev++) {
             const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
             const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
             Wh[id * (nv + 1) + lev] = W[id * nv + lev - 1] * intt + W[id * nv + lev] * intl;
+        }
+    }
+}
+
+// RayleighSponge with raysp_calc_mode = exp1 / exp3, in each RK stage before
+// the slow modes (thor_driver.cu:749-860): profx_dMh, profx_dWh and profx_dW
+// cleared; the zonal means from the stage values Mhk, Rhok, Wk, pressurek in
+// the first stage (exp1) or every stage (exp3); then sponge_layer's explicit
+// branch (profx_sponge.cu:255-292, 388-404, 417-434): the drag -k (u - ubar)
+// etc. times rho into profx_dMh and profx_dW, interpolated to profx_dWh at the
+// interior interfaces, and the heating Rd rho (-k_T (T - Tbar)) added to
+// profx_Qheat (which ProfX cleared: it adds up over the stages of a step).
+void sponge_explicit(Oracle &o, int rk) {
+    const int nv = o.nv, P = o.P;
+    const double Ruv = o.spq[1], Rw = o.spq[2], RT = o.spq[7];
+    const bool dmu = o.spq[4] != 0.0, dmw = o.spq[5] != 0.0, dmt = o.spq[6] != 0.0;
+    double *Mh = o.A_("Mhk"), *Rho = o.A_("Rhok"), *W = o.A_("Wk"), *pr = o.A_("pressurek"), *Rd = o.A_("Rd");
+    double *dMh = o.A_("profx_dMh"), *dWh = o.A_("profx_dWh"), *Qh = o.A_("profx_Qheat");
+    const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *ll = o.g.lonlat;
+    zero(o, "profx_dMh");
+    zero(o, "profx_dWh");
+    std::vector<double> dW((size_t)P * nv, 0.0);
+    if (rk == 0 || (int)o.spq[8] == 3) sponge_means(o, Mh, Rho, W, pr);
+    for (int id = 0; id < P; id++) {
+        const double lat = ll[id * 2 + 1], lon = ll[id * 2] < 0 ? ll[id * 2] + 2.0 * M_PI : ll[id * 2];
+        for (int lev = 0; lev < nv; lev++) {
+            const long e = (long)id * nv + lev;
+            const double rho = Rho[e];
+            const double *M = &Mh[e * 3];
+            const double u = (M[0] * (-MSIN(lon)) + M[1] * (MCOS(lon)) + M[2] * 0) / rho;
+            const double v = (M[0] * (-MSIN(lat) * MCOS(lon)) + M[1] * (-MSIN(lat) * MSIN(lon)) + M[2] * (MCOS(lat))) / rho;
+            const double w = W[e] / rho;
+            const double kuv = sponge_rate(o, lev, Ruv), kw = sponge_rate(o, lev, Rw);
+            double du, dv, dw;
+            if (dmu) {
+                du = -kuv * (u - sponge_mean_at(o, id, lev, 0));
+                dv = -kuv * (v - sponge_mean_at(o, id, lev, 1));
+            } else {
+                du = -kuv * u;
+                dv = -kuv * v;
+            }
+            const double vx = du * (-MSIN(lon)) + dv * (-MSIN(lat) * MCOS(lon));
+            const double vy = du * (MCOS(lon)) + dv * (-MSIN(lat) * MSIN(lon));
+            const double vz = dv * (MCOS(lat));
+            dMh[e * 3 + 0] += vx * rho;
+            dMh[e * 3 + 1] += vy * rho;
+            dMh[e * 3 + 2] += vz * rho;
+            dw = dmw ? -kw * (w - sponge_mean_at(o, id, lev, 2)) : -kw * (w);
+            dW[e] += dw * rho;
+            if (dmt) {
+                const double kT = sponge_rate(o, lev, RT);
+                const double temperature = pr[e] / Rd[e] / Rho[e];
+                const double dT = -kT * (temperature - sponge_mean_at(o, id, lev, 3));
+                Qh[e] += Rd[e] * Rho[e] * dT;
+            }
+        }
+        for (int lev = 1; lev < nv; lev++) {
+            const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            dWh[id * (nv + 1) + lev] = dW[(long)id * nv + lev - 1] * intt + dW[(long)id * nv + lev] * intl;
         }
     }
 }
@@ -1969,6 +2032,9 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         advec_cori2(*o);
     } else if (n == "diffusion") {
         diffusion_pass(*o, a0, a1 != 0);
+    } else if (n == "sponge_explicit") {  // a0: the RK stage
+        if (!o->sponge_on) return -1;
+        sponge_explicit(*o, a0);
     } else if (n == "vertical_diff") {
         vertical_diff(*o);
     } else if (n == "correct_horizontal") {
@@ -1996,7 +2062,7 @@ void oracle_steps(oracle *o, int n) {
 }
 void oracle_set_newtonian(oracle *o, const double q[8]) { std::memcpy(o->nrq, q, sizeof(o->nrq)); }
 void oracle_set_rt(oracle *o, const double q[11]) { std::memcpy(o->rtq, q, sizeof(o->rtq)); }
-void oracle_set_sponge(oracle *o, const double q[8]) {
+void oracle_set_sponge(oracle *o, const double q[9]) {
     o->sponge_on = q != nullptr;
     if (!q) return;
     std::memcpy(o->spq, q, sizeof(o->spq));
@@ -2010,7 +2076,7 @@ int oracle_zonal_table(oracle *o, int *dst, long n, int *max_count) {
 }
 int oracle_profx(oracle *o, int forcing) {
     if (forcing == 0) return 0;
-    if (o->sponge_on) sponge(*o);
+    if (o->sponge_imp()) sponge(*o);
     if (forcing == 1) profx_hs(*o);
     else if (forcing == 2) profx_nr(*o);
     else if (forcing == 3) profx_rt(*o);

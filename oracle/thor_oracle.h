@@ -54,6 +54,7 @@ void oracle_profx_held_suarez(oracle *o);
 //   "advec_cori2"  Compute_Advec_Cori2            (thor_adv_cor.h:366-536)
 //   "diffusion"    Diffusion_Op + _Poles, a0 = stepnum, a1 = laststep (thor_diff.h:54-880)
 //   "vertical_diff" vertical_diff                 (thor_diff.h:1198-1426)
+//   "sponge_explicit" explicit Rayleigh sponge of RK stage a0 (thor_driver.cu:749-860)
 //   "correct_horizontal"                          (thor_diff.h:882-915)
 //   "divm"         DivM_Op + _Poles on Mhk/Whk, a0 = laststep (thor_div.h:49-535)
 //   "slow_modes"   Compute_Slow_Modes + _Poles    (thor_slowmodes.h:48-683)

@@ -155,6 +155,52 @@ def test_sponge_steps_bitwise(hj, forcing, temp):
     d2.close()
 
 
+@pytest.mark.parametrize("forcing,mode", [("newtonian", 1), ("double_gray", 3), ("newtonian", 3)])
+def test_explicit_sponge_steps_bitwise(hj, forcing, mode):
+    """Parity build with the explicit Rayleigh sponge (raysp_calc_mode exp1 /
+    exp3: drag and heating into profx_dMh / profx_dWh / profx_Qheat in every
+    RK stage, the zonal means of the first or of every stage), winds and
+    temperature towards their zonal means: two steps equal the oracle bit for
+    bit, and the mixed form stays within its usual distance of FP64."""
+    from oracle.oracle import Oracle
+    capi, g, p, st0 = hj
+    code = capi.FORCING_NEWTONIAN if forcing == "newtonian" else capi.FORCING_DOUBLE_GRAY
+    print(f"case: parity build, hot Jupiter g4 L20 with winds, explicit sponge exp{mode} + {forcing}, 2 steps",
+          flush=True)
+    st = capi.ic_rest(g, p, seed=1017, temperature_noise=1.0, wind_noise=20.0)
+    q = capi.SpongeParams(0.6, 2e-4, 1e-4, 16, 1, 1, 1, 3e-4, mode)
+
+    def run(x):
+        x.set_state(st)
+        if code == capi.FORCING_NEWTONIAN:
+            x.set_newtonian(nr_params(capi))
+        else:
+            x.set_rt(rt_params(capi))
+        x.set_sponge(q)
+        x.run(2, code)
+        return x
+
+    d = run(capi.Dyn(g, p, variant="pm"))
+    d.sync()
+    o = run(Oracle(g, p, variant="pm"))
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("explicit sponge", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()
+    pm = capi.Params.from_buffer_copy(p)
+    pm.precision = 1
+    dm = run(capi.Dyn(g, pm))
+    dm.sync()
+    gm = dm.get_state()
+    tol = {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}
+    for k, t in tol.items():
+        err = _max_rel(gm[k], os_[k])
+        print(f"  mixed vs FP64 {k}: {err:.2e}")
+        assert err < t, k
+    dm.close()
+
+
 def test_sponge_many_slot_blocks_bitwise(built):
     """Two latitude bands on g5 (about 5 100 points each, 8 192 slots): the
     zonal sums (winds and temperature) run over four 2 048-slot blocks per

@@ -208,3 +208,46 @@ def test_uniform_temperature_is_left_alone(hj):
     err = np.abs(o.get("pressure") - p0).max() / np.abs(p0).max()
     print(f"level-uniform T after the temperature sponge: max rel change of p {err:.2e}")
     assert err < 1e-12
+
+
+def test_explicit_sponge_drag_towards_zero(hj):
+    """raysp_calc_mode exp1 damping to zero: the drag of stage 0 is
+    profx_dMh = -k_uv (M - (M.r) r) (the horizontal momentum), profx_dWh the
+    interpolation of -k_w W at the interior interfaces (0 at the ends), and
+    the heating of the temperature sponge Rd rho (-k_T (T - Tbar)) vanishes for
+    a temperature that is the same in every column of a level."""
+    from oracle.oracle import Oracle
+    capi, g, p, st = hj
+    P, nv = g.P, g.nv
+    ns, R, Rw, RT = 0.4, 3e-4, 2e-4, 5e-4
+    s2 = dict(st)
+    Tl = 1500.0 + 10.0 * np.arange(nv)
+    s2["pressure"] = (p.Rd * st["Rho"].reshape(P, nv) * Tl[None, :]).reshape(np.shape(st["pressure"]))
+    o = Oracle(g, p)
+    o.set_state(s2)
+    o.set_sponge(capi.SpongeParams(ns, R, Rw, 12, 0, 0, 1, RT, 1))
+    for a, b in (("Mhk", "Mh"), ("Rhok", "Rho"), ("pressurek", "pressure")):
+        o.set(a, o.get(b))
+    o.set("Wk", _w_of(g, st["Wh"].reshape(P, nv + 1)).reshape(-1))  # the fixture's random vertical wind
+    q0 = o.get("profx_Qheat").copy()
+    o.kernel("sponge_explicit", 0)
+    alt, alth = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    n = alt / alth[nv]
+    s2n = np.where(n >= ns, np.sin(0.5 * np.pi * (n - ns) / (1 - ns)) ** 2, 0.0)
+    M = o.get("Mhk").reshape(P, nv, 3)
+    r = g.func_r.reshape(P, 3)
+    Mt = M - np.einsum("plk,pk->pl", M, r)[:, :, None] * r[:, None, :]
+    want = -(R * s2n)[None, :, None] * Mt
+    got = o.get("profx_dMh").reshape(P, nv, 3)
+    err = np.abs(got - want).max() / np.abs(want).max()
+    W = o.get("Wk").reshape(P, nv)
+    dW = -(Rw * s2n)[None, :] * W
+    xi, xim, xip = alth[1:nv], alt[:-1], alt[1:]
+    dWh = o.get("profx_dWh").reshape(P, nv + 1)
+    wantWh = dW[:, :-1] * ((xi - xip) / (xim - xip)) + dW[:, 1:] * ((xi - xim) / (xip - xim))
+    errW = np.abs(dWh[:, 1:nv] - wantWh).max() / np.abs(wantWh).max()
+    dq = np.abs(o.get("profx_Qheat") - q0).max()
+    print(f"explicit sponge drag vs closed form: dMh {err:.1e}, dWh {errW:.1e}; level-uniform T heating {dq:.1e}")
+    assert err < 1e-12 and errW < 1e-12
+    np.testing.assert_array_equal(dWh[:, [0, nv]], 0.0)
+    assert dq < 1e-12 * np.abs(p.Rd * o.get("Rhok") * 1600 * RT).max()

@@ -73,7 +73,8 @@ struct Dyn {
     SpongePar spg{};
     int *ztab = nullptr;                                // [3][S]
     int *zcnt = nullptr;                                // [nlat] points per latitude bin
-    double *ztrig = nullptr, *zkf = nullptr;            // [4][S] sin/cos of lon and lat, [2][nv] rates
+    double *ztrig = nullptr, *zkf = nullptr;            // [4][S] sin/cos of lon and lat, [3][nv] rates
+    double *zdw = nullptr;                              // [nv][S] explicit sponge: drag on W
     double *ztmp = nullptr, *zout = nullptr, *vbar = nullptr;
     // physics-module hooks and RK-advanced module arrays
     thor_hooks hooks{};
@@ -313,6 +314,30 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             if (d.hooks.dyn_core_loop_slow_modes(d.hooks.user, (void *)st, nstep, times, &v))
                 return hook_fail("dyn_core_loop_slow_modes");
         }
+        if (d.sponge_on && d.spg.mode != 0) {
+            // explicit Rayleigh sponge on the stage values, before the slow modes
+            Fields fz = f;
+            fz.Mh = f.Mhk;
+            fz.Rho = f.Rhok;
+            fz.W = f.Wk;
+            fz.p = f.pk;
+            const SpongePar sp = d.spg;
+            const dim3 gpl((d.P + kTile - 1) / kTile, d.nv);
+            if (rk == 0 || sp.mode == 3) {
+                const int nc = sp.ncomp();
+                launch(d, "sponge_zonal", 6 * B, [&] {
+                    k_sponge_zonal<<<gpl, blk, 0, st>>>(g, fz, sp, d.ztab, d.ztmp);
+                });
+                launch(d, "sponge_mean", 8.0 * nc * sp.nlat * d.nv * sp.max_count, [&] {
+                    k_sponge_reduce<<<d.nv * nc * sp.nlat, 1024, 0, st>>>(sp, d.nv, d.ztmp, d.zout);
+                    k_sponge_vbar<<<(nc * sp.nlat * d.nv + 255) / 256, 256, 0, st>>>(sp, d.nv, d.zout, d.vbar);
+                });
+            }
+            launch(d, "sponge_drag", 12 * B, [&] {
+                k_sponge_drag<<<gpl, blk, 0, st>>>(g, fz, sp, d.ztab, d.vbar, f.dMh, d.zdw, f.Qheat);
+                k_sponge_dwh<<<dim3((d.P + kTile - 1) / kTile, d.nv + 1), blk, 0, st>>>(g, d.zdw, f.dWh);
+            });
+        }
         if (MX && d.slow_lc) {
             // DivM into the idle hyperdiffusion scratch (its first three planes)
             if (divdamp)
@@ -415,7 +440,7 @@ static int profx(Dyn &d, int forcing) {
     const dim3 blk(kTile), grid_c((d.P + 63) / 64);  // 64 columns x 4 level rows per block
     const double B = 8.0 * d.P * d.nv;
     const double dt = d.prm.timestep;
-    if (d.sponge_on) {
+    if (d.sponge_on && d.spg.mode == 0) {
         const SpongePar sp = d.spg;
         const dim3 gpl((d.P + kTile - 1) / kTile, d.nv);
         // Wh 2, Mh 3, rho in; W and the three zonal slots out
@@ -1050,6 +1075,8 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     if (q->nlat_bins < 2 || q->nlat_bins > 1000 || !(q->ns >= 0.0 && q->ns < 1.0) || q->Ruv < 0 || q->Rw < 0
         || (q->temp_sponge && q->RT_sponge < 0))
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_set_sponge: need 2 <= nlat_bins <= 1000, 0 <= ns < 1, rates >= 0");
+    if (q->raysp_calc_mode != 0 && q->raysp_calc_mode != 1 && q->raysp_calc_mode != 3)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_set_sponge: raysp_calc_mode must be 0 (imp), 1 (exp1) or 3 (exp3)");
     const int P = h->P, S = h->S, nv = h->nv, nlat = q->nlat_bins;
     // latitude bins (grid.cu:3215-3262): edges accumulated from -pi/2, index
     // within a bin in point order, points at or above the last edge in the top bin
@@ -1079,17 +1106,18 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     }
     const int max_count = (int)std::pow(2, std::ceil(std::log(mc) / std::log(2)));
     SpongePar sp{q->ns, q->Ruv, q->Rw, nlat, q->damp_uv_to_mean != 0, q->damp_w_to_mean != 0, max_count,
-                 (max_count + 2047) / 2048, q->temp_sponge != 0, q->temp_sponge ? q->RT_sponge : 0.0};
+                 (max_count + 2047) / 2048, q->temp_sponge != 0, q->temp_sponge ? q->RT_sponge : 0.0,
+                 q->raysp_calc_mode};
     // (re)allocate the tables; the zonal slots beyond each bin's count stay zero
     for (void *ptr : {(void *)h->ztab, (void *)h->ztmp, (void *)h->zout, (void *)h->vbar, (void *)h->ztrig,
-                      (void *)h->zkf, (void *)h->zcnt}) {
+                      (void *)h->zkf, (void *)h->zcnt, (void *)h->zdw}) {
         if (!ptr) continue;
         auto it = std::find(h->allocs.begin(), h->allocs.end(), ptr);
         if (it != h->allocs.end()) h->allocs.erase(it);
         (void)hipFree(ptr);
     }
     h->ztab = h->zcnt = nullptr;
-    h->ztmp = h->zout = h->vbar = h->ztrig = h->zkf = nullptr;
+    h->ztmp = h->zout = h->vbar = h->ztrig = h->zkf = h->zdw = nullptr;
     h->sponge_on = false;
     hipError_t e = hipMalloc((void **)&h->ztab, tab.size() * sizeof(int));
     if (e == hipSuccess) {
@@ -1107,6 +1135,7 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     if (e == hipSuccess) e = dalloc(*h, &h->vbar, (size_t)4 * nlat * nv);
     if (e == hipSuccess) e = dalloc(*h, &h->ztrig, (size_t)4 * S);
     if (e == hipSuccess) e = dalloc(*h, &h->zkf, (size_t)3 * nv);
+    if (e == hipSuccess && sp.mode != 0) e = dalloc(*h, &h->zdw, (size_t)nv * S);  // the explicit drag on W
     if (e == hipSuccess) {
         sp.trig = h->ztrig;
         sp.kf = h->zkf;

@@ -3622,6 +3622,7 @@ struct SpongePar {
     int nlat, dmu, dmw, max_count, nb;  // nb = blocks of 2048 slots per row
     int dmt;                            // temperature sponge (RayleighSpongeT)
     double RT;
+    int mode;                           // raysp_calc_mode: 0 imp, 1 exp1, 3 exp3
     const double *trig;                 // [4][S]: sin, cos of the longitude (in [0, 2 pi)) and of the latitude
     const double *kf;                   // [3][nv]: the implicit rates of u, v, of w and of T per level
     const int *cnt;                     // [nlat]: points per latitude bin
@@ -3674,11 +3675,16 @@ __global__ __launch_bounds__(256) void k_sponge_zonal(Geo g, Fields f, SpongePar
     const size_t S = g.S;
     const int nv = g.nv;
     const double *Alt = g.Alt, *Alth = g.Alth;
-    const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
-    const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
-    const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-    const double w = whl * intt + wht * intl;
-    LV(f.W, lev) = w;
+    double w;
+    if (sp.mode == 0) {  // Recompute_W first (profx_driver.cu:92)
+        const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
+        const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
+        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+        w = whl * intt + wht * intl;
+        LV(f.W, lev) = w;
+    } else {  // explicit modes: the stage's W (the caller passes Wk as f.W)
+        w = LV(f.W, lev);
+    }
     if (!sp.dmu && !sp.dmw && !sp.dmt) return;
     const int cnt = ztab[S + p];
     const size_t T = (size_t)sp.nlat * nv * sp.max_count;
@@ -3782,6 +3788,79 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
         const double kT = sp.kf[2 * nv + lev], temperature = LV(f.p, lev) / g.Rd / rho;
         LV(f.p, lev) = rho * g.Rd * (temperature / dt + kT * mean_at(3)) / (1.0 / dt + kT);
     }
+}
+
+// The zonal mean of component c (0 u, 1 v, 2 w, 3 T) at point p, level lev,
+// as k_sponge_apply forms it.
+__device__ __forceinline__ double sponge_mean(const Geo &g, const SpongePar &sp, const int *ztab, const double *vbar,
+                                              int p, int lev, int c) {
+    const size_t S = g.S;
+    const int nv = g.nv, nlat = sp.nlat, intlat = ztab[p];
+    const double des_lat = M_PI / nlat, lat = g.lonlat[S + p];
+    const double lat1 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat - 1);
+    const double lat2 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat);
+    const double lat3 = -M_PI / 2 + 0.5 * des_lat + des_lat * (intlat + 1);
+    if (lat <= -M_PI / 2 + 0.5 * des_lat) return vbar[((size_t)intlat * nv + lev) * 4 + c];
+    if (lat >= -M_PI / 2 + 0.5 * des_lat + des_lat * (nlat - 1)) return vbar[((size_t)intlat * nv + lev) * 4 + c];
+    if (lat >= lat1 && lat < lat2) {
+        const double intt = (lat - lat2) / (lat1 - lat2), intl = (lat - lat1) / (lat2 - lat1);
+        return vbar[((size_t)(intlat - 1) * nv + lev) * 4 + c] * intt + vbar[((size_t)intlat * nv + lev) * 4 + c] * intl;
+    }
+    const double intt = (lat - lat3) / (lat2 - lat3), intl = (lat - lat2) / (lat3 - lat2);
+    return vbar[((size_t)intlat * nv + lev) * 4 + c] * intt + vbar[((size_t)(intlat + 1) * nv + lev) * 4 + c] * intl;
+}
+
+// Explicit sponge (sponge_layer with implicit = false, profx_sponge.cu:255-292,
+// 388-404): f.Mh, f.Rho, f.W, f.p are the stage values Mhk, Rhok, Wk,
+// pressurek; the drag times rho goes into dMh and dW (cleared first, as THOR's
+// memsets do: 0 + x), the heating Rd rho dT is added to Qheat.
+__global__ __launch_bounds__(256) void k_sponge_drag(Geo g, Fields f, SpongePar sp, const int *ztab,
+                                                    const double *vbar, double *dMh, double *dW, double *Qheat) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    double u, v;
+    sponge_uv(g, f, sp, p, lev, u, v);
+    const double rho = LV(f.Rho, lev), w = LV(f.W, lev) / rho;
+    const double kuv = sp.kf[lev], kw = sp.kf[nv + lev];
+    double du, dv;
+    if (sp.dmu) {
+        du = -kuv * (u - sponge_mean(g, sp, ztab, vbar, p, lev, 0));
+        dv = -kuv * (v - sponge_mean(g, sp, ztab, vbar, p, lev, 1));
+    } else {
+        du = -kuv * u;
+        dv = -kuv * v;
+    }
+    const double sl = sp.trig[p], cl = sp.trig[S + p], sa = sp.trig[2 * S + p], ca = sp.trig[3 * S + p];
+    const double vx = du * (-sl) + dv * (-sa * cl);
+    const double vy = du * (cl) + dv * (-sa * sl);
+    const double vz = dv * (ca);
+    VEC(dMh, 0, lev, p) = 0.0 + vx * rho;
+    VEC(dMh, 1, lev, p) = 0.0 + vy * rho;
+    VEC(dMh, 2, lev, p) = 0.0 + vz * rho;
+    const double dw = sp.dmw ? -kw * (w - sponge_mean(g, sp, ztab, vbar, p, lev, 2)) : -kw * (w);
+    LV(dW, lev) = 0.0 + dw * rho;
+    if (sp.dmt) {
+        const double kT = sp.kf[2 * nv + lev], temperature = LV(f.p, lev) / g.Rd / rho;
+        const double dT = -kT * (temperature - sponge_mean(g, sp, ztab, vbar, p, lev, 3));
+        LV(Qheat, lev) += g.Rd * rho * dT;
+    }
+}
+
+// dWh from the explicit drag dW: interpolated at the interior interfaces,
+// zero at the bottom and the top (blockIdx.y = interface 0..nv).
+__global__ __launch_bounds__(256) void k_sponge_dwh(Geo g, const double *dW, double *dWh) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    if (lev == 0 || lev == g.nv) {
+        LV(dWh, lev) = 0.0;
+        return;
+    }
+    const double xi = g.Alth[lev], xim = g.Alt[lev - 1], xip = g.Alt[lev];
+    const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+    LV(dWh, lev) = LV(dW, lev - 1) * intt + LV(dW, lev) * intl;
 }
 
 // Wh at interior interface lev (1..nv-1) from the relaxed W (blockIdx.y = lev - 1).
