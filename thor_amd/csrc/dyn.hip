@@ -368,8 +368,9 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             });
             // Prepare_Implicit_Vertical, plus the horizontal flux divergences of
             // Density_Pressure_Eqs (booked with the column kernel below); the
-            // split form reads h and pt as FP32 copies
-            launch(d, "prepare", MX && d.prep_split ? W(5, 5) : W(7, 3), [&] {
+            // split form reads h and pt as FP32 copies and writes no Sd (the
+            // mixed vertical solve forms it from hflx_r and SlowRho)
+            launch(d, "prepare", MX && d.prep_split ? W(4, 5) : W(7, 3), [&] {
                 if (MX && d.prep_split) k_prepare_lc<true><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else if (MX) k_prepare_lc<false><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);

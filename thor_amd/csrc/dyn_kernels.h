@@ -1908,8 +1908,7 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
                 dpt += rs * st[k];
             }
             LV(f.Sp, lev) = nflxp + slowp;
-            LV(f.Sd, lev) = nflxr + slowrho;
-            LV(f.hflx_r, lev) = dr;
+            LV(f.hflx_r, lev) = dr;  // Sd = -dr + SlowRho is formed by the vertical solve
             LV(f.hflx_pt, lev) = SPLIT ? dpt + dptk : dpt;
         }
         if (lev + 1 < l1) put(R, b ^ 1);
@@ -2967,8 +2966,15 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
 #pragma unroll UR
         for (int lev = 1 + row; lev < nv; lev += NROW) {
             const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
-            const double Sp = LV(f.Sp, lev), Sd = LV(f.Sd, lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
-            const double Spl = LV(f.Sp, lev - 1), Sdl = LV(f.Sd, lev - 1), pl = LV(f.ps, lev - 1),
+            // mixed form: Sd = -div(Mhs) + SlowRho from the flux divergence the
+            // density update reads anyway (the same sum, opposite sign: equal bit
+            // for bit), so the prepare kernel writes no Sd plane
+            auto sd_at = [&](int l) {
+                if constexpr (MX) return -LV(f.hflx_r, l) + LV(f.SlowRho, l);
+                else return LV(f.Sd, l);
+            };
+            const double Sp = LV(f.Sp, lev), Sd = sd_at(lev), pr = LV(f.ps, lev), r = LV(f.Rhos, lev);
+            const double Spl = LV(f.Sp, lev - 1), Sdl = sd_at(lev - 1), pl = LV(f.ps, lev - 1),
                          rl = LV(f.Rhos, lev - 1);
             const double dzh = 1.0 / (alt - altl);
             const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
