@@ -212,8 +212,7 @@ thor_status thor_dyn_get_rt_column(thor_dyn *h, const char *name, double *dst, l
  * temperature relaxes the same way towards its zonal mean at the rate
  * RT_sponge, through the pressure (profx_sponge.cu:104-127, 388-404; THOR sums
  * slots of the mean that no point writes without clearing them, here they
- * are zero). p = NULL switches the sponge off. THOR's shrinking sponge is not
- * supported. */
+ * are zero). p = NULL switches the sponge off. */
 typedef struct thor_sponge_params {
     double ns, Ruv, Rw;
     int nlat_bins, damp_uv_to_mean, damp_w_to_mean;
@@ -225,6 +224,10 @@ typedef struct thor_sponge_params {
      * the zonal means of the stage values formed in the first stage (exp1) or
      * in every stage (exp3) (thor_driver.cu:749-860). */
     int raysp_calc_mode;
+    /* shrink_sponge: from step t_shrink on (steps counted from 1 per handle),
+     * every sponge rate is scaled by exp(-(step - t_shrink) dt / (1000 dt))
+     * (profx_driver.cu:153-160, thor_driver.cu:822-830). */
+    int shrink_sponge, t_shrink;
 } thor_sponge_params;
 thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params *p);
 

@@ -171,13 +171,13 @@ class Oracle:
         self.L.oracle_set_rt(self._h, a.ctypes.data_as(_dp))
 
     def set_sponge(self, q):
-        """q: a ctypes SpongeParams (or its first 6 to 9 numbers in field order); None: off."""
+        """q: a ctypes SpongeParams (or its first 6 to 11 numbers in field order); None: off."""
         if q is None:
             self.L.oracle_set_sponge(self._h, None)
             return
         a = np.array([float(getattr(q, n)) for n, _ in q._fields_] if hasattr(q, "_fields_") else q,
                      dtype=np.float64)
-        a = np.concatenate([a, np.zeros(9 - a.size)])
+        a = np.concatenate([a, np.zeros(11 - a.size)])
         self.L.oracle_set_sponge(self._h, a.ctypes.data_as(_dp))
 
     def zonal_table(self):

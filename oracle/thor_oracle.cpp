@@ -49,7 +49,8 @@ struct Oracle {
     // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean,
     // temp_sponge, RT_sponge
     bool sponge_on = false;
-    double spq[9] = {0};  // ..., [8] raysp_calc_mode: 0 imp, 1 exp1, 3 exp3
+    double spq[11] = {0};  // ..., [8] raysp_calc_mode: 0 imp, 1 exp1, 3 exp3; [9] shrink_sponge, [10] t_shrink
+    long nstep = 0;        // Thor steps completed (the current step is nstep + 1)
     std::vector<double> vbar, tbar;  // the sponge's zonal means (kept over the stages in exp1)
     bool sponge_imp() const { return sponge_on && (int)spq[8] == 0; }
     std::vector<int> ztab;  // [P][3]: latitude bin, points in the bin, index within the bin
@@ -1440,6 +1441,7 @@ void thor(Oracle &o) {
     copy(o, "W", "Wk");
     copy(o, "Rho", "Rhok");
     copy(o, "pressure", "pressurek");
+    o.nstep++;
 }
 
 void held_suarez(Oracle &o);
@@ -1780,11 +1782,22 @@ double sponge_mean_at(const Oracle &o, int id, int lev, int c) {
     return at(intlat) * intt + at(intlat + 1) * intl;
 }
 
-// The sponge rate of level lev for the rate R: R sin^2(pi/2 (n - ns) / (1 - ns))
-// above n = Altitude / Top = ns.
+// The sponge rate of level lev for the rate R: Rv_fac R sin^2(pi/2 (n - ns) /
+// (1 - ns)) above n = Altitude / Top = ns; Rv_fac is 1, or with shrink_sponge
+// exp(-(step - t_shrink) dt / (1000 dt)) from step t_shrink on
+// (profx_driver.cu:153-160, thor_driver.cu:822-830).
 double sponge_rate(const Oracle &o, int lev, double R) {
     const double n = o.g.Altitude[lev] / o.g.Altitudeh[o.nv], ns = o.spq[0];
-    return n >= ns ? 1.0 * R * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0) : 0.0;
+    double Rv_fac = 1;
+    if (o.spq[9] != 0.0) {
+        const long current_step = o.nstep + 1, t_shrink = (long)o.spq[10];
+        const double timestep = o.p.timestep;
+        if (current_step * timestep >= t_shrink * timestep) {
+            const double shrink_scale = timestep * 1000;
+            Rv_fac = std::exp(-(current_step * timestep - t_shrink * timestep) / shrink_scale);
+        }
+    }
+    return n >= ns ? Rv_fac * R * MPOW(MSIN(0.5 * M_PI * (n - ns) * (1.0) / (1.0 - ns)), 2.0) : 0.0;
 }
 
 void sponge(Oracle &o) {
@@ -2062,7 +2075,7 @@ void oracle_steps(oracle *o, int n) {
 }
 void oracle_set_newtonian(oracle *o, const double q[8]) { std::memcpy(o->nrq, q, sizeof(o->nrq)); }
 void oracle_set_rt(oracle *o, const double q[11]) { std::memcpy(o->rtq, q, sizeof(o->rtq)); }
-void oracle_set_sponge(oracle *o, const double q[9]) {
+void oracle_set_sponge(oracle *o, const double q[11]) {
     o->sponge_on = q != nullptr;
     if (!q) return;
     std::memcpy(o->spq, q, sizeof(o->spq));

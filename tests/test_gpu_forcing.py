@@ -201,6 +201,30 @@ def test_explicit_sponge_steps_bitwise(hj, forcing, mode):
     dm.close()
 
 
+@pytest.mark.parametrize("mode", [0, 3])
+def test_shrinking_sponge_steps_bitwise(hj, mode):
+    """shrink_sponge from step 2 (implicit and exp3 modes, winds and
+    temperature): three steps of the parity build equal the oracle bit for bit."""
+    from oracle.oracle import Oracle
+    capi, g, p, st0 = hj
+    print(f"case: parity build, hot Jupiter g4 L20, shrinking sponge (mode {mode}) + newtonian, 3 steps", flush=True)
+    st = capi.ic_rest(g, p, seed=1019, temperature_noise=1.0, wind_noise=20.0)
+    q = capi.SpongeParams(0.6, 2e-4, 1e-4, 16, 1, 1, 1, 3e-4, mode, 1, 2)
+    d = capi.Dyn(g, p, variant="pm")
+    o = Oracle(g, p, variant="pm")
+    for x in (d, o):
+        x.set_state(st)
+        x.set_newtonian(nr_params(capi))
+        x.set_sponge(q)
+        x.run(3, capi.FORCING_NEWTONIAN)
+    d.sync()
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("shrinking sponge", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()
+
+
 def test_sponge_many_slot_blocks_bitwise(built):
     """Two latitude bands on g5 (about 5 100 points each, 8 192 slots): the
     zonal sums (winds and temperature) run over four 2 048-slot blocks per

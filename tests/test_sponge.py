@@ -251,3 +251,29 @@ def test_explicit_sponge_drag_towards_zero(hj):
     assert err < 1e-12 and errW < 1e-12
     np.testing.assert_array_equal(dWh[:, [0, nv]], 0.0)
     assert dq < 1e-12 * np.abs(p.Rd * o.get("Rhok") * 1600 * RT).max()
+
+
+def test_shrinking_sponge_scales_the_rates(hj):
+    """shrink_sponge with t_shrink = 0: at step 1 every rate carries
+    Rv_fac = exp(-(1 - 0) dt / (1000 dt)) = exp(-1e-3), so damping to zero
+    gives u / (1 + Rv_fac k dt)."""
+    capi, g, p, st = hj
+    ns, R, dt = 0.5, 3e-4, p.timestep
+    q = capi.SpongeParams(ns, R, R, 12, 0, 0, 0, 0.0, 0, 1, 0)
+    o = _run(capi, g, p, st, q)
+    P, nv = g.P, g.nv
+    alt, alth = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    n = alt / alth[nv]
+    s2 = np.where(n >= ns, np.sin(0.5 * np.pi * (n - ns) / (1 - ns)) ** 2, 0.0)
+    rv = np.exp(-1e-3)
+    fu = 1 / (1 + rv * R * s2 * dt)
+    rho = st["Rho"].reshape(P, nv)
+    lat, e_lon, e_lat = _frame(g)
+    M0 = st["Mh"].reshape(P, nv, 3)
+    u0 = np.einsum("pk,plk->pl", e_lon, M0) / rho
+    u1 = np.einsum("pk,plk->pl", e_lon, o.get("Mh").reshape(P, nv, 3)) / rho
+    err = np.abs(u1 - u0 * fu).max() / np.abs(u0).max()
+    plain = 1 / (1 + R * s2 * dt)
+    sep = np.abs(u0 * (fu - plain)).max() / np.abs(u0).max()
+    print(f"shrinking sponge: u vs closed form {err:.1e}; the factor changes u by {sep:.1e}")
+    assert err < 1e-13 and sep > 1e3 * max(err, 1e-16)

@@ -64,7 +64,8 @@ class SpongeParams(C.Structure):
     """thor_sponge_params: Rayleigh sponge (THOR's RayleighSponge, implicit mode)."""
     _fields_ = [("ns", C.c_double), ("Ruv", C.c_double), ("Rw", C.c_double), ("nlat_bins", C.c_int),
                 ("damp_uv_to_mean", C.c_int), ("damp_w_to_mean", C.c_int), ("temp_sponge", C.c_int),
-                ("RT_sponge", C.c_double), ("raysp_calc_mode", C.c_int)]
+                ("RT_sponge", C.c_double), ("raysp_calc_mode", C.c_int), ("shrink_sponge", C.c_int),
+                ("t_shrink", C.c_int)]
 
 
 class StepView(C.Structure):

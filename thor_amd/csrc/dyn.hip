@@ -80,6 +80,7 @@ struct Dyn {
     thor_hooks hooks{};
     RKArrays rka{};
     long nstep = 0;  // Thor steps completed by this handle
+    int shrink_sponge = 0, t_shrink = 0;  // the sponge's shrinking (thor_sponge_params)
     // forcings
     bool nr_set = false, rt_set = false;
     NRPar nr{};
@@ -156,6 +157,20 @@ template <class Fn> static void launch(Dyn &d, const char *name, double bytes, F
     } else {
         fn();
     }
+}
+
+// Rv_fac of the shrinking sponge at step `step` (profx_driver.cu:153-160,
+// thor_driver.cu:822-830): 1 before t_shrink, then exp(-(step - t_shrink) dt / (1000 dt)).
+static double sponge_shrink(const Dyn &d, long step) {
+    double rv = 1;
+    if (d.shrink_sponge) {
+        const double timestep = d.prm.timestep;
+        if (step * timestep >= d.t_shrink * timestep) {
+            const double shrink_scale = timestep * 1000;
+            rv = std::exp(-(step * timestep - d.t_shrink * timestep) / shrink_scale);
+        }
+    }
+    return rv;
 }
 
 static Fields stage_fields(Dyn &d, int rk) {
@@ -334,7 +349,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 });
             }
             launch(d, "sponge_drag", 12 * B, [&] {
-                k_sponge_drag<<<gpl, blk, 0, st>>>(g, fz, sp, d.ztab, d.vbar, f.dMh, d.zdw, f.Qheat);
+                k_sponge_drag<<<gpl, blk, 0, st>>>(g, fz, sp, d.ztab, d.vbar, f.dMh, d.zdw, f.Qheat,
+                                                   sponge_shrink(d, nstep));
                 k_sponge_dwh<<<dim3((d.P + kTile - 1) / kTile, d.nv + 1), blk, 0, st>>>(g, d.zdw, f.dWh);
             });
         }
@@ -457,7 +473,7 @@ static int profx(Dyn &d, int forcing) {
         }
         // Mh 3, W, rho in; Mh 3, W out; then W 2 in, Wh out
         launch(d, "sponge_apply", 12 * B, [&] {
-            k_sponge_apply<<<gpl, blk, 0, d.stream>>>(g, f, sp, d.ztab, d.vbar, dt);
+            k_sponge_apply<<<gpl, blk, 0, d.stream>>>(g, f, sp, d.ztab, d.vbar, dt, sponge_shrink(d, d.nstep + 1));
             k_sponge_wh<<<dim3((d.P + kTile - 1) / kTile, d.nv - 1), blk, 0, d.stream>>>(g, f);
         });
         f.skip_w = 1;
@@ -1135,7 +1151,7 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     if (e == hipSuccess) e = dalloc(*h, &h->zout, (size_t)nc * nlat * nv * sp.nb);
     if (e == hipSuccess) e = dalloc(*h, &h->vbar, (size_t)4 * nlat * nv);
     if (e == hipSuccess) e = dalloc(*h, &h->ztrig, (size_t)4 * S);
-    if (e == hipSuccess) e = dalloc(*h, &h->zkf, (size_t)3 * nv);
+    if (e == hipSuccess) e = dalloc(*h, &h->zkf, (size_t)nv);
     if (e == hipSuccess && sp.mode != 0) e = dalloc(*h, &h->zdw, (size_t)nv * S);  // the explicit drag on W
     if (e == hipSuccess) {
         sp.trig = h->ztrig;
@@ -1147,6 +1163,8 @@ extern "C" thor_status thor_dyn_set_sponge(thor_dyn *h, const thor_sponge_params
     }
     if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("thor_dyn_set_sponge: ") + hipGetErrorString(e));
     h->spg = sp;
+    h->shrink_sponge = q->shrink_sponge;
+    h->t_shrink = q->t_shrink;
     h->sponge_on = true;
     return THOR_OK;
 }

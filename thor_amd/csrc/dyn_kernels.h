@@ -3630,7 +3630,7 @@ struct SpongePar {
     double RT;
     int mode;                           // raysp_calc_mode: 0 imp, 1 exp1, 3 exp3
     const double *trig;                 // [4][S]: sin, cos of the longitude (in [0, 2 pi)) and of the latitude
-    const double *kf;                   // [3][nv]: the implicit rates of u, v, of w and of T per level
+    const double *kf;                   // [nv]: sin^2(pi/2 (n - ns) / (1 - ns)) above ns, else 0
     const int *cnt;                     // [nlat]: points per latitude bin
     __host__ __device__ int ncomp() const { return dmt ? 4 : 3; }  // zonal-mean components (u, v, w, T)
 };
@@ -3650,15 +3650,7 @@ __global__ __launch_bounds__(256) void k_sponge_tables(Geo g, SpongePar sp, doub
     }
     if (i < nv) {
         const double n = g.Alt[i] / g.Alth[nv];
-        double kuv = 0.0, kw = 0.0, kT = 0.0;
-        if (n >= sp.ns) {
-            kuv = 1.0 * sp.Ruv * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
-            kw = 1.0 * sp.Rw * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
-            kT = 1.0 * sp.RT * MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0);
-        }
-        kf[i] = kuv;
-        kf[nv + i] = kw;
-        kf[2 * nv + i] = kT;
+        kf[i] = n >= sp.ns ? MPOW(MSIN(0.5 * M_PI * (n - sp.ns) * (1.0) / (1.0 - sp.ns)), 2.0) : 0.0;
     }
 }
 
@@ -3745,8 +3737,10 @@ __global__ void k_sponge_vbar(SpongePar sp, int nv, const double *out, double *v
     vbar[(size_t)rem * 4 + comp] = s;
 }
 
+// rv: the shrinking factor Rv_fac (1 without shrink_sponge); each rate is
+// Rv_fac R sin^2(..), as sponge_layer forms it.
 __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar sp, const int *ztab,
-                                                     const double *vbar, double dt) {
+                                                     const double *vbar, double dt, double rv) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
     if (p >= g.P) return;
     const size_t S = g.S;
@@ -3771,7 +3765,7 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     };
     const double rho = LV(f.Rho, lev);
     const double w = LV(f.W, lev) / rho;
-    const double kuv = sp.kf[lev], kw = sp.kf[nv + lev];
+    const double kuv = rv * sp.Ruv * sp.kf[lev], kw = rv * sp.Rw * sp.kf[lev];
     double unew, vnew, wnew;
     if (sp.dmu) {
         unew = (u / dt + kuv * mean_at(0)) / (1.0 / dt + kuv);
@@ -3791,7 +3785,7 @@ __global__ __launch_bounds__(256) void k_sponge_apply(Geo g, Fields f, SpongePar
     VEC(f.Mh, 2, lev, p) = vz * rho;
     LV(f.W, lev) = wnew * rho;
     if (sp.dmt) {  // temperature towards its zonal mean, through the pressure
-        const double kT = sp.kf[2 * nv + lev], temperature = LV(f.p, lev) / g.Rd / rho;
+        const double kT = rv * sp.RT * sp.kf[lev], temperature = LV(f.p, lev) / g.Rd / rho;
         LV(f.p, lev) = rho * g.Rd * (temperature / dt + kT * mean_at(3)) / (1.0 / dt + kT);
     }
 }
@@ -3821,7 +3815,8 @@ __device__ __forceinline__ double sponge_mean(const Geo &g, const SpongePar &sp,
 // pressurek; the drag times rho goes into dMh and dW (cleared first, as THOR's
 // memsets do: 0 + x), the heating Rd rho dT is added to Qheat.
 __global__ __launch_bounds__(256) void k_sponge_drag(Geo g, Fields f, SpongePar sp, const int *ztab,
-                                                    const double *vbar, double *dMh, double *dW, double *Qheat) {
+                                                    const double *vbar, double *dMh, double *dW, double *Qheat,
+                                                    double rv) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x, lev = blockIdx.y;
     if (p >= g.P) return;
     const size_t S = g.S;
@@ -3829,7 +3824,7 @@ __global__ __launch_bounds__(256) void k_sponge_drag(Geo g, Fields f, SpongePar 
     double u, v;
     sponge_uv(g, f, sp, p, lev, u, v);
     const double rho = LV(f.Rho, lev), w = LV(f.W, lev) / rho;
-    const double kuv = sp.kf[lev], kw = sp.kf[nv + lev];
+    const double kuv = rv * sp.Ruv * sp.kf[lev], kw = rv * sp.Rw * sp.kf[lev];
     double du, dv;
     if (sp.dmu) {
         du = -kuv * (u - sponge_mean(g, sp, ztab, vbar, p, lev, 0));
@@ -3848,7 +3843,7 @@ __global__ __launch_bounds__(256) void k_sponge_drag(Geo g, Fields f, SpongePar 
     const double dw = sp.dmw ? -kw * (w - sponge_mean(g, sp, ztab, vbar, p, lev, 2)) : -kw * (w);
     LV(dW, lev) = 0.0 + dw * rho;
     if (sp.dmt) {
-        const double kT = sp.kf[2 * nv + lev], temperature = LV(f.p, lev) / g.Rd / rho;
+        const double kT = rv * sp.RT * sp.kf[lev], temperature = LV(f.p, lev) / g.Rd / rho;
         const double dT = -kT * (temperature - sponge_mean(g, sp, ztab, vbar, p, lev, 3));
         LV(Qheat, lev) += g.Rd * rho * dT;
     }
