@@ -225,6 +225,49 @@ def test_shrinking_sponge_steps_bitwise(hj, mode):
     d.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_all_switches_together_bitwise(built, mode, hj):
+    """Every optional part of the step at once: sixth-order hyperdiffusion,
+    vertical hyperdiffusion, the fourth-order diffusive sponge, and the
+    Rayleigh sponge with its temperature part and shrinking (implicit or
+    exp1), under the Newtonian forcing. Two steps of the parity build equal
+    the oracle bit for bit; the mixed form stays within its usual distance
+    of FP64."""
+    from oracle.oracle import Oracle
+    capi, g, _, _ = hj
+    p = hj_params(capi, 4, HyDiffOrder=6, VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=1e-3, DiffSponge=1,
+                  order_diff_sponge=4, Dv_sponge=0.02, ns_diff_sponge=0.6)
+    print(f"case: parity build, hot Jupiter g4 L20, all switches (sponge mode {mode}), 2 steps", flush=True)
+    st = capi.ic_rest(g, p, seed=1023, temperature_noise=1.0, wind_noise=20.0)
+    q = capi.SpongeParams(0.6, 2e-4, 1e-4, 16, 1, 1, 1, 3e-4, mode, 1, 2)
+
+    def run(x):
+        x.set_state(st)
+        x.set_newtonian(nr_params(capi))
+        x.set_sponge(q)
+        x.run(2, capi.FORCING_NEWTONIAN)
+        return x
+
+    d = run(capi.Dyn(g, p, variant="pm"))
+    d.sync()
+    o = run(Oracle(g, p, variant="pm"))
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("all switches", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()
+    pm = capi.Params.from_buffer_copy(p)
+    pm.precision = 1
+    dm = run(capi.Dyn(g, pm))
+    dm.sync()
+    gm = dm.get_state()
+    for k, t in {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}.items():
+        err = _max_rel(gm[k], os_[k])
+        print(f"  mixed vs FP64 {k}: {err:.2e}")
+        assert err < t, k
+    dm.close()
+
+
 def test_sponge_many_slot_blocks_bitwise(built):
     """Two latitude bands on g5 (about 5 100 points each, 8 192 slots): the
     zonal sums (winds and temperature) run over four 2 048-slot blocks per

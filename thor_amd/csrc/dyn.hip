@@ -68,6 +68,7 @@ struct Dyn {
     bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
     int cp_lv = 2;            // levels per thread of the column-preparation kernel (1, 2 or 4)
     int sponge_pass = -1;     // hyperdiffusion pass whose output the diffusive sponge reads (-1: none)
+    double mid_scale = 1.0;   // mixed form: the factor each intermediate hyperdiffusion pass applies
     // Rayleigh sponge
     bool sponge_on = false;
     SpongePar spg{};
@@ -291,11 +292,15 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         // the first pass (above) and the last (below); fd points the last pass
         // at the output of the final one
         Fields fd = f;
+        // the last pass's sign, divided by the intermediate passes' scaling (mixed form)
+        const double last_sign =
+            MX && hydiff ? d.diff_sign / std::pow(d.mid_scale, d.prm.HyDiffOrder / 2 - 2) : d.diff_sign;
         if (hydiff) {
             for (int ih = 0; ih < d.prm.HyDiffOrder / 2 - 2; ih++) {
                 auto *src = FS<MX>::diff(fd);
                 auto *dst = ih % 2 ? FS<MX>::diff(f) : FS<MX>::diffx(f);
-                launch(d, "diff_mid", W(0, 12), [&] { k_diff_mid<MX><<<grid_t2, blk, 0, st>>>(g, src, dst); });
+                launch(d, "diff_mid", W(0, 12),
+                       [&] { k_diff_mid<MX><<<grid_t2, blk, 0, st>>>(g, src, dst, (float)d.mid_scale); });
                 FS<MX>::set_diff(fd, dst);
             }
             // pass s writes diff (s even) or the second array (s odd); the
@@ -307,13 +312,13 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             // the last hyperdiffusion pass and the stage's divergence-damping
             // pass 0, each over level chunks
             if (hydiff)
-                launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, fd, d.diff_sign, lc); });
+                launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, fd, last_sign, lc); });
             if (divdamp)
                 launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_lc<true><<<grid_lc, blk, 0, st>>>(g, f, lc); });
         } else if (hydiff || divdamp) {
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
                    [&] {
-                k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, fd, hydiff, divdamp, d.diff_sign);
+                k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, fd, hydiff, divdamp, last_sign);
             });
         }
         if (hydiff && d.prm.VertHyDiff) {
@@ -695,6 +700,14 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     d.sponge_pass = prm->HyDiff && prm->DiffSponge && prm->order_diff_sponge != prm->HyDiffOrder
                         ? prm->order_diff_sponge / 2 - 1 : -1;
     d.g.ssp = (prm->order_diff_sponge / 2 + 1) % 2 ? -1.0 : 1.0;
+    // mixed form, HyDiffOrder >= 6: each intermediate pass scales its FP32
+    // output by dbar^2 (k_diff_mid); the last pass and the sponge undo it
+    d.g.dsp_scale = 1.0;
+    if (prm->precision == THOR_PRECISION_MIXED && prm->HyDiff && prm->HyDiffOrder >= 6) {
+        const double dbar = std::sqrt(2 * M_PI / 5) * prm->A / (std::pow(2, prm->glevel));
+        d.mid_scale = dbar * dbar;
+        if (d.sponge_pass >= 1) d.g.dsp_scale = 1.0 / std::pow(d.mid_scale, d.sponge_pass);
+    }
     d.diff_sign = std::pow(-1.0, prm->HyDiffOrder / 2);
     // vertical hyperdiffusion strength (esp_initial.cu:1327-1334)
     std::vector<double> Kdv6(nv);

@@ -34,6 +34,7 @@ struct Geo {
     const double *Kdv6;                         // [nv] vertical hyperdiffusion strength (VertHyDiff)
     const double *Kdh2;                         // [nv] diffusive-sponge strength (DiffSponge)
     double ssp;                                 // its sign, (-1)^(order_diff_sponge / 2 + 1)
+    double dsp_scale;                           // 1 / the scale of the pass output it reads (mixed form)
     // Mixed precision only: the hyperdiffusion Laplacian and the divergence-
     // damping operator as per-point weights on the 7 stencil slots (0 = self,
     // 1..6 = ring), for rscale = 1:

@@ -904,7 +904,8 @@ __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f,
             // diffusive sponge (thor_diff.h:453-509): (-1)^(order/2 + 1) rho Kdh2
             // times the sponge pass's input at the point, for u, v, w
             const double rr = rho(0), k2 = g.Kdh2[lev];
-            for (int var = 1; var <= 4; var++) out[var] += g.ssp * rr * k2 * (double)VEC(dsp, var, lev, p);
+            for (int var = 1; var <= 4; var++)
+                out[var] += g.ssp * rr * k2 * (double)VEC(dsp, var, lev, p) * g.dsp_scale;
         }
         LV(FS<MX>::diffrh(f), lev) = out[0];
         LV(FS<MX>::diffw(f), lev) = out[4];
@@ -1102,8 +1103,12 @@ template <bool MX> __global__ __launch_bounds__(96) void k_vert_diff(Geo g, Fiel
 // place in this pass, so a block's halo may already hold this pass's values;
 // here each pass reads src and writes dst (the caller alternates two arrays),
 // so every value is the Laplacian of the previous pass's output.
+// Mixed form: the FP32 output is multiplied by scale (dbar^2), so repeated
+// Laplacians (each a factor of about 1/dbar^2) stay far from FP32 underflow
+// on large planets; the last pass divides the coefficient by the same factors.
 template <bool MX>
-__global__ STENCIL_BOUNDS void k_diff_mid(Geo g, const typename FS<MX>::R *src, typename FS<MX>::R *dst) {
+__global__ STENCIL_BOUNDS void k_diff_mid(Geo g, const typename FS<MX>::R *src, typename FS<MX>::R *dst,
+                                         float scale) {
     int r, lev;
     if (!block_tile(g, r, lev)) return;
     const size_t S = g.S;
@@ -1114,7 +1119,7 @@ __global__ STENCIL_BOUNDS void k_diff_mid(Geo g, const typename FS<MX>::R *src, 
             float o[6];
             laplacians_w<6>(g, p, (float)(rs * rs), a, o);
 #pragma unroll
-            for (int var = 0; var < 6; var++) VEC(dst, var, lev, p) = o[var];
+            for (int var = 0; var < 6; var++) VEC(dst, var, lev, p) = o[var] * scale;
         } else {
             double o[6];
             laplacians<6>(g, p, nf, rs, a, o);
@@ -1999,7 +2004,8 @@ __global__ __launch_bounds__(256, 3) void k_diff2_lc(Geo g, Fields f, double sig
         }
         if (f.mdsp) {  // diffusive sponge, as diff2_divm0_point
             const double rr = LV(f.Rhok, lev), k2 = g.Kdh2[lev];
-            for (int var = 1; var <= 4; var++) out[var] += g.ssp * rr * k2 * (double)VEC(f.mdsp, var, lev, p);
+            for (int var = 1; var <= 4; var++)
+                out[var] += g.ssp * rr * k2 * (double)VEC(f.mdsp, var, lev, p) * g.dsp_scale;
         }
         const unsigned l4 = (unsigned)(lev * pl4);
         stb_f(rRh, T.o4, l4, (float)out[0]);
