@@ -67,7 +67,11 @@ enum { THOR_FORCING_RT_SURROGATE = 4 };
 
 /* Attaches a trained network (n_in = 2 nv + 1, n_out = nv) to a dynamical
  * core that already has thor_dyn_set_rt parameters (the fallback and judge).
- * The network must outlive the handle's use of it. */
+ * The network must outlive the handle's use of it. Stream order between the
+ * two is kept by the library: the network's entry points that rewrite its
+ * weights or buffers (set_params, set_data, train_epoch, forward, gradients,
+ * time_forward) first wait for the work every attached handle has queued,
+ * and a surrogate ProfX waits for the network's queued work. */
 thor_status thor_dyn_set_rt_surrogate(thor_dyn *h, thor_mlp *m, const thor_rt_surrogate_params *p);
 /* Surrogate statistics since the last call: ProfX calls, judge calls,
  * columns that fell back to the exact RT, and the RMS and max absolute

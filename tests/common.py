@@ -6,7 +6,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-CACHE = os.path.join(ROOT, "tests", "_cache")
+# grid constructions shared with bench.py (g5-g7 take seconds to minutes to build)
+CACHE = os.path.join(ROOT, ".thor_cache")
 
 STATE = ("Mh", "Wh", "W", "Rho", "pressure")
 
@@ -60,3 +61,25 @@ def rt_params(capi, **kw):
              incflx=5.6704e-8 * 2000.0 ** 4, ps0=1e7, lon_sub=0.0)
     q.update(kw)
     return capi.RTParams(**q)
+
+
+def hip_runtime():
+    """The HIP runtime the library itself links, for device buffers a test
+    owns (the caller's side of the boundary)."""
+    import ctypes as C
+    hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemcpyAsync.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]
+    hip.hipMemsetAsync.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_void_p]
+    hip.hipStreamSynchronize.argtypes = [C.c_void_p]
+    hip.hipDeviceSynchronize.argtypes = []
+    return hip
+
+
+HIP_H2D, HIP_D2H, HIP_D2D = 1, 2, 3
+
+
+def max_rel(a: np.ndarray, b: np.ndarray) -> float:
+    return rel_err(a, b)

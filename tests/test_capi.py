@@ -46,7 +46,8 @@ def test_bad_arguments_are_rejected(built):
         assert e.code == built.THOR_ERR_BAD_ARG
     else:
         raise AssertionError("glevel 3 accepted")
-    assert "glevel" in built.lib().thor_last_error().decode() or True
+    msg = built.lib().thor_last_error().decode()
+    assert "glevel" in msg and "4..8" in msg, msg
 
 
 @pytest.mark.parametrize("order", [2, 5])

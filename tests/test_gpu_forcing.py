@@ -268,6 +268,48 @@ def test_all_switches_together_bitwise(built, mode, hj):
     dm.close()
 
 
+@pytest.mark.parametrize("sponge_order", [4, 6])
+def test_mixed_order_eight_hot_jupiter(hj, sponge_order):
+    """HyDiffOrder 8 on the large planet (A = 1e8 m) in the mixed form: two
+    intermediate passes scale their FP32 output by dbar^2 (alternating the two
+    arrays), the last pass divides its sign by dbar^4, and a sixth-order
+    diffusive sponge reads the second pass's scaled output (dsp_scale =
+    dbar^-2). The FP64 parity build equals the oracle bit for bit; the mixed
+    form stays within the usual mixed-vs-FP64 distance of it."""
+    from oracle.oracle import Oracle
+    capi, g, _, _ = hj
+    p = hj_params(capi, 4, HyDiffOrder=8, DiffSponge=1, order_diff_sponge=sponge_order, Dv_sponge=0.02,
+                  ns_diff_sponge=0.6)
+    print(f"case: hot Jupiter g4 L20, HyDiffOrder 8, order_diff_sponge {sponge_order}, 2 steps, "
+          f"parity build vs oracle and mixed vs oracle", flush=True)
+    st = capi.ic_rest(g, p, seed=1029, temperature_noise=1.0, wind_noise=20.0)
+
+    def run(x):
+        x.set_state(st)
+        x.set_newtonian(nr_params(capi))
+        x.run(2, capi.FORCING_NEWTONIAN)
+        return x
+
+    d = run(capi.Dyn(g, p, variant="pm"))
+    d.sync()
+    o = run(Oracle(g, p, variant="pm"))
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("order 8 FP64", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    d.close()
+    pm = capi.Params.from_buffer_copy(p)
+    pm.precision = 1
+    dm = run(capi.Dyn(g, pm))
+    dm.sync()
+    gm = dm.get_state()
+    for k, t in {"Mh": 1e-4, "Wh": 1e-3, "W": 1e-3, "Rho": 1e-6, "pressure": 1e-6}.items():
+        err = _max_rel(gm[k], os_[k])
+        print(f"  mixed vs FP64 {k}: {err:.2e}")
+        assert err < t, k
+    dm.close()
+
+
 def test_sponge_many_slot_blocks_bitwise(built):
     """Two latitude bands on g5 (about 5 100 points each, 8 192 slots): the
     zonal sums (winds and temperature) run over four 2 048-slot blocks per

@@ -145,18 +145,19 @@ def test_surrogate_in_the_hot_jupiter_step(built):
     m.close()
 
 
-@pytest.mark.parametrize("nv", [20, 48])
-def test_surrogate_profx_is_the_network_on_column_features(built, nv):
+@pytest.mark.parametrize("glevel,nv", [(4, 20), (4, 48), (5, 40)])
+def test_surrogate_profx_is_the_network_on_column_features(built, glevel, nv):
     """One surrogate ProfX (no judge) against the network evaluated by
     thor_mlp_forward on features formed on the host from the same state: the
     fused kernel's features, envelope test, inference and heating must agree
     (up to one ulp of ln p in the features). L48 has 97 inputs (padded K)
-    and 48 outputs (three 16-wide output tiles)."""
+    and 48 outputs (three 16-wide output tiles); g5 L40 is the learned-RT
+    workload's own size (81 inputs, 40 outputs, 10 242 columns)."""
     from thor_amd import surrogate as sg
     capi = built
-    print(f"case: hot Jupiter g4 L{nv}, surrogate ProfX vs host features + thor_mlp_forward", flush=True)
-    g = hj_grid(capi, 4, nv)
-    p = hj_params(capi, 4)
+    print(f"case: hot Jupiter g{glevel} L{nv}, surrogate ProfX vs host features + thor_mlp_forward", flush=True)
+    g = hj_grid(capi, glevel, nv)
+    p = hj_params(capi, glevel)
     st = capi.ic_rest(g, p, seed=1005, temperature_noise=1.0, wind_noise=0.1)
     d = capi.Dyn(g, p)
     d.set_state(st)

@@ -137,10 +137,10 @@ _libs: dict = {}
 def lib(variant: str = "") -> C.CDLL:
     """Loads libthor_amd.so (variant "") or the parity build libthor_amd_pm.so
     (variant "pm"), both built by thor_amd.build. Raises if it is missing."""
+    if variant not in ("", "pm"):
+        raise ValueError(f"unknown library variant {variant!r} (\"\" or \"pm\")")
     if variant not in _libs:
         path = LIB_PATH_PM if variant == "pm" else LIB_PATH
-        if variant.startswith("/"):  # explicit library path (experiments with other build flags)
-            path = variant
         if not os.path.exists(path):
             raise ThorError(THOR_ERR_IO, f"{path} not built; run python -m thor_amd.build")
         L = C.CDLL(path)

@@ -502,6 +502,8 @@ static int profx(Dyn &d, int forcing) {
         const bool judge = d.judge_every > 0 && d.sur_calls % d.judge_every == 0;
         d.sur_calls++;
         hipError_t e = mlp::prepare(d.sur, d.P);
+        // the network's queued work (training, new parameters) comes first
+        if (e == hipSuccess) e = mlp::order_on(d.sur, d.stream);
         if (e != hipSuccess) return fail(THOR_ERR_HIP, std::string("profx surrogate: ") + hipGetErrorString(e));
         const dim3 gp((d.P + kTile - 1) / kTile), gpl((d.P + kTile - 1) / kTile, d.nv);
         uint16_t *X = mlp::input(d.sur);
@@ -705,7 +707,8 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     d.g.dsp_scale = 1.0;
     if (prm->precision == THOR_PRECISION_MIXED && prm->HyDiff && prm->HyDiffOrder >= 6) {
         const double dbar = std::sqrt(2 * M_PI / 5) * prm->A / (std::pow(2, prm->glevel));
-        d.mid_scale = dbar * dbar;
+        // the FP32 value the passes multiply by, so the undo divides by the same number
+        d.mid_scale = (double)(float)(dbar * dbar);
         if (d.sponge_pass >= 1) d.g.dsp_scale = 1.0 / std::pow(d.mid_scale, d.sponge_pass);
     }
     d.diff_sign = std::pow(-1.0, prm->HyDiffOrder / 2);
@@ -857,6 +860,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
 extern "C" void thor_dyn_destroy(thor_dyn *h) {
     if (!h) return;
     if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->sur) thor::mlp::detach(h->sur, h->stream);
     for (auto &pd : h->pending) {
         hipEventDestroy(pd.a);
         hipEventDestroy(pd.b);
@@ -995,6 +999,8 @@ extern "C" thor_status thor_dyn_set_rt_surrogate(thor_dyn *h, thor_mlp *m, const
     for (int i = 0; i < 6; i++)
         HIP_TRY(hipMemcpy(dst[i], src[i], cnt[i] * sizeof(float), hipMemcpyHostToDevice));
     h->sp.lon_sub = h->rt.lon_sub;
+    if (h->sur && h->sur != m) mlp::detach(h->sur, h->stream);
+    mlp::attach(m, h->stream);
     h->sur = m;
     h->judge_every = q->judge_every;
     h->sur_calls = h->sur_judged = 0;

@@ -25,8 +25,11 @@ struct thor_grid {
 extern "C" thor_status thor_grid_build(int glevel, int nv, double radius, double top_altitude,
                                        int spring_dynamics, double spring_beta, const char *cache_dir,
                                        thor_grid **out) {
-    if (!out || glevel < 4 || glevel > 8 || nv < 2 || radius <= 0 || top_altitude <= 0)
-        return thor::fail(THOR_ERR_BAD_ARG, "thor_grid_build: bad argument");
+    if (!out) return thor::fail(THOR_ERR_BAD_ARG, "thor_grid_build: null output pointer");
+    if (glevel < 4 || glevel > 8)
+        return thor::fail(THOR_ERR_BAD_ARG, "thor_grid_build: glevel must be in 4..8 (got " + std::to_string(glevel) + ")");
+    if (nv < 2 || radius <= 0 || top_altitude <= 0)
+        return thor::fail(THOR_ERR_BAD_ARG, "thor_grid_build: need nv >= 2, radius > 0 and top_altitude > 0");
     try {
         thor_grid *tg = new thor_grid();
         bool loaded = false;

@@ -46,6 +46,11 @@ struct Net {
     int *idx = nullptr;
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
+    // streams of dynamical cores that run this network's inference kernels
+    // (thor_dyn_set_rt_surrogate): work that rewrites the weights or the
+    // activation buffers first waits for what those streams have queued
+    std::vector<hipStream_t> users;
+    hipEvent_t ev_user = nullptr, ev_self = nullptr;
 
     long oW1() const { return 0; }
     long ob1() const { return (long)nin * nh; }
@@ -197,6 +202,20 @@ using namespace thor::mlp;
         if (_e != hipSuccess) return fail(THOR_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
     } while (0)
 
+namespace thor {
+namespace mlp {
+// Orders the network's own stream after everything its users have queued.
+static hipError_t wait_users(Net &n) {
+    for (hipStream_t s : n.users) {
+        hipError_t e = hipEventRecord(n.ev_user, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(n.stream, n.ev_user, 0);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+} // namespace mlp
+} // namespace thor
+
 extern "C" thor_status thor_mlp_create(int n_in, int n_hidden, int n_out, unsigned long long seed, thor_mlp **out) {
     if (!out || n_in < 1 || n_hidden < 1 || n_out < 1 || n_in > 4096 || n_hidden > 4096 || n_out > 4096)
         return fail(THOR_ERR_BAD_ARG, "thor_mlp_create: bad sizes");
@@ -209,6 +228,8 @@ extern "C" thor_status thor_mlp_create(int n_in, int n_hidden, int n_out, unsign
     n.nout = n_out;
     n.np = n.ob3() + n_out;
     hipError_t e = hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&n.ev_user, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&n.ev_self, hipEventDisableTiming);
     if (e == hipSuccess) e = alloc(n, &n.w, n.np);
     if (e == hipSuccess) e = alloc(n, &n.gm, n.np);
     if (e == hipSuccess) e = alloc(n, &n.gv, n.np);
@@ -241,8 +262,11 @@ extern "C" thor_status thor_mlp_create(int n_in, int n_hidden, int n_out, unsign
 
 extern "C" void thor_mlp_destroy(thor_mlp *m) {
     if (!m) return;
+    for (hipStream_t s : m->users) (void)hipStreamSynchronize(s);
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     for (void *p : m->allocs) (void)hipFree(p);
+    if (m->ev_user) (void)hipEventDestroy(m->ev_user);
+    if (m->ev_self) (void)hipEventDestroy(m->ev_self);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -258,6 +282,7 @@ extern "C" thor_status thor_mlp_get_params(thor_mlp *m, float *dst, long n) {
 
 extern "C" thor_status thor_mlp_set_params(thor_mlp *m, const float *src, long n) {
     if (!m || !src || n != m->np) return fail(THOR_ERR_BAD_ARG, "thor_mlp_set_params: bad argument");
+    MLP_TRY(wait_users(*m));
     MLP_TRY(hipMemcpyAsync(m->w, src, n * sizeof(float), hipMemcpyHostToDevice, m->stream));
     MLP_TRY(hipMemsetAsync(m->gm, 0, n * sizeof(float), m->stream));
     MLP_TRY(hipMemsetAsync(m->gv, 0, n * sizeof(float), m->stream));
@@ -270,6 +295,8 @@ extern "C" thor_status thor_mlp_set_params(thor_mlp *m, const float *src, long n
 
 extern "C" thor_status thor_mlp_set_data(thor_mlp *m, const float *X, const float *Y, long n) {
     if (!m || !X || !Y || n < 1) return fail(THOR_ERR_BAD_ARG, "thor_mlp_set_data: bad argument");
+    MLP_TRY(wait_users(*m));
+    MLP_TRY(hipStreamSynchronize(m->stream));  // the old training set may still be read
     for (void *p : {(void *)m->Xall, (void *)m->Yall, (void *)m->idx}) {
         if (!p) continue;
         auto it = std::find(m->allocs.begin(), m->allocs.end(), p);
@@ -293,6 +320,7 @@ extern "C" thor_status thor_mlp_train_epoch(thor_mlp *m, int batch, float lr, un
     if (m->ndata < 1) return fail(THOR_ERR_BAD_ARG, "thor_mlp_train_epoch: no training data");
     Net &n = *m;
     const long nb = (n.ndata + batch - 1) / batch;
+    MLP_TRY(wait_users(n));
     MLP_TRY(ensure_cap(n, batch));
     // seeded shuffle on the host
     std::vector<int> perm((size_t)n.ndata);
@@ -324,6 +352,7 @@ extern "C" thor_status thor_mlp_train_epoch(thor_mlp *m, int batch, float lr, un
 extern "C" thor_status thor_mlp_forward(thor_mlp *m, const float *X, float *Y, long n) {
     if (!m || !X || !Y || n < 1) return fail(THOR_ERR_BAD_ARG, "thor_mlp_forward: bad argument");
     Net &net = *m;
+    MLP_TRY(wait_users(net));
     MLP_TRY(ensure_cap(net, n));
     float *dX = nullptr;
     MLP_TRY(hipMalloc((void **)&dX, (size_t)n * net.nin * sizeof(float)));
@@ -344,6 +373,7 @@ extern "C" thor_status thor_mlp_gradients(thor_mlp *m, const float *X, const flo
                                           double *loss) {
     if (!m || !X || !Y || !grad || n < 1 || n > 65536) return fail(THOR_ERR_BAD_ARG, "thor_mlp_gradients: bad argument");
     Net &net = *m;
+    MLP_TRY(wait_users(net));
     MLP_TRY(ensure_cap(net, n));
     float *dX = nullptr;
     MLP_TRY(hipMalloc((void **)&dX, (size_t)n * net.nin * sizeof(float)));
@@ -367,6 +397,7 @@ extern "C" thor_status thor_mlp_gradients(thor_mlp *m, const float *X, const flo
 extern "C" thor_status thor_mlp_time_forward(thor_mlp *m, long n, int repeats, double *ms) {
     if (!m || n < 1 || repeats < 1 || !ms) return fail(THOR_ERR_BAD_ARG, "thor_mlp_time_forward: bad argument");
     Net &net = *m;
+    MLP_TRY(wait_users(net));
     MLP_TRY(ensure_cap(net, n));
     hipEvent_t a, b;
     MLP_TRY(hipEventCreate(&a));
@@ -388,7 +419,30 @@ extern "C" thor_status thor_mlp_time_forward(thor_mlp *m, long n, int repeats, d
 // ------------------------------------------------------ surrogate helpers
 namespace thor {
 namespace mlp {
-hipError_t prepare(thor_mlp *m, long rows) { return ensure_cap(*m, rows); }
+hipError_t prepare(thor_mlp *m, long rows) {
+    // growing the buffers frees the old ones: nothing queued may still use them
+    if (rows > m->cap) {
+        for (hipStream_t s : m->users) {
+            hipError_t e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipStreamSynchronize(m->stream);
+        if (e != hipSuccess) return e;
+    }
+    return ensure_cap(*m, rows);
+}
+void attach(thor_mlp *m, hipStream_t s) {
+    if (std::find(m->users.begin(), m->users.end(), s) == m->users.end()) m->users.push_back(s);
+}
+void detach(thor_mlp *m, hipStream_t s) {
+    auto it = std::find(m->users.begin(), m->users.end(), s);
+    if (it != m->users.end()) m->users.erase(it);
+}
+hipError_t order_on(thor_mlp *m, hipStream_t s) {
+    hipError_t e = hipEventRecord(m->ev_self, m->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, m->ev_self, 0);
+    return e;
+}
 uint16_t *input(thor_mlp *m) { return m->X0; }
 const float *output(thor_mlp *m) { return m->Y; }
 void forward_on(thor_mlp *m, long rows, hipStream_t s) {

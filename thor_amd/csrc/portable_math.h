@@ -5,8 +5,10 @@
 // removes libm differences (glibc vs ocml) from the comparison; production
 // builds call the platform's pow/log/sin/cos.
 //
-// Accuracy (checked against glibc in tests/test_portable_math.py): log <= 2
-// ulp, exp <= 1 ulp, pow <= 1 + 0.5 |y log x| ulp (x > 0), sin/cos <= 1e-15
+// Accuracy (checked against glibc in tests/test_oracle.py, test_elementary_functions
+// and test_portable_pow_over_the_step_ranges): log <= 2
+// ulp, exp <= 1 ulp, pow <= 2.5 (1 + |y log x|) ulp (x > 0; the error of log x is
+// multiplied by |y log x| in the exponent), sin/cos <= 1e-15
 // absolute for |x| <= 4; not a replacement for libm elsewhere.
 #pragma once
 
