@@ -2062,6 +2062,18 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         o->capture_tri = a1 != 0;
         vertical_eq(*o, a0 * 1e-3);
         o->capture_tri = false;
+    } else if (n == "temperature_h_pt_geff") {
+        temperature_h_pt_geff(*o);
+    } else if (n == "update_rk") {
+        update_rk(*o);
+    } else if (n == "update_rk2") {
+        update_rk2(*o);
+    } else if (n == "momentum_eq") {  // a0: the substep length in milliseconds
+        momentum_eq(*o, a0 * 1e-3);
+    } else if (n == "prepare_implicit_vertical") {
+        prepare_implicit_vertical(*o);
+    } else if (n == "density_pressure") {  // a0: the substep length in milliseconds
+        density_pressure(*o, a0 * 1e-3);
     } else {
         return -1;
     }

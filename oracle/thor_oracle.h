@@ -63,6 +63,12 @@ void oracle_profx_held_suarez(oracle *o);
 //                  column's system as "tri_a" .. "tri_d" [P][nv+1]
 //                  (a x_{l-1} + b x_l + c x_{l+1} = d at interfaces 1..nv-1)
 //                                                 (thor_vertical_int.h:52-563)
+//   "temperature_h_pt_geff"                       (thor_auxiliary.h:55-309)
+//   "update_rk" / "update_rk2"                    (thor_auxiliary.h:311-529)
+//   "momentum_eq"  Momentum_Eq + _Poles, a0 = substep length in ms (thor_fastmodes.h:54-253)
+//   "prepare_implicit_vertical" + _Poles          (thor_vertical_int.h:566-771)
+//   "density_pressure" Density_Pressure_Eqs + _Poles, a0 = substep length in ms
+//                                                 (thor_fastmodes.h:255-847)
 // Returns 0, or -1 for an unknown name. "Kdh4"/"Kdhz" are readable arrays.
 int oracle_kernel(oracle *o, const char *name, int a0, int a1);
 // ProfX + Thor, n times.
