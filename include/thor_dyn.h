@@ -10,7 +10,10 @@
  * calls exit().
  *
  * Reference interfaces replaced (paths relative to exoclime/THOR):
- *   thor_dyn_thor          <- ESP::Thor(const SimulationSetup&, kernel_diagnostics&)
+ *   thor_dyn_step          <- ESP::Thor(const SimulationSetup&, kernel_diagnostics&) on the
+ *                             caller's device arrays, src/headers/esp.h:466,
+ *                             src/ESP/thor_driver.cu:70-1466, called at src/esp.cu:1395-1399
+ *   thor_dyn_thor          <- ESP::Thor on the handle's own copy of the state
  *                             src/headers/esp.h:466, src/ESP/thor_driver.cu:70-1466
  *   thor_dyn_profx         <- ESP::ProfX(...) with core_benchmark = HeldSuarez
  *                             src/headers/esp.h:468, src/ESP/profx_driver.cu:70-474
@@ -157,6 +160,29 @@ thor_status thor_dyn_get_state(thor_dyn *h, const thor_state *s);
 
 /* One large step of the dynamical core (ESP::Thor). Asynchronous. */
 thor_status thor_dyn_thor(thor_dyn *h);
+
+/* THOR's forcing arrays, written by its ProfX and physics modules before the
+ * dynamics step (ESP members profx_Qheat_d [P][nv] in W/m3, profx_dMh_d
+ * [P][nv][3] and profx_dWh_d [P][nv+1], esp.h:224-371); device pointers. */
+typedef struct thor_forcing {
+    const double *Qheat, *dMh, *dWh;
+} thor_forcing;
+
+/* One large step on the caller's own arrays, as ESP::Thor works on the ESP
+ * object's (thor_driver.cu:70-1466; the new state overwrites Mh_d, Wh_d,
+ * W_d, Rho_d, pressure_d at :1458-1462): state must be device pointers in
+ * THOR's layout (on_device = 1); they are read at the start and overwritten
+ * with the new state. forcing gives THOR's profx_* arrays as device pointers
+ * (read once, at the start); NULL keeps the handle's own forcing arrays (what
+ * thor_dyn_profx or the previous call left there). dt > 0 and n_substeps > 0
+ * replace the time step and substep count given at create (the Kd profiles
+ * are recomputed, which waits for the handle's queued work); 0 keeps them.
+ * Asynchronous: the step waits for the work queued on `stream` (a
+ * hipStream_t, NULL for the default stream) before it reads the caller's
+ * arrays, and work queued on `stream` afterwards waits for the new state.
+ * Replaces ESP::Thor's signature esp.h:466 called from esp.cu:1395-1399. */
+thor_status thor_dyn_step(thor_dyn *h, const thor_state *state, const thor_forcing *forcing, double dt,
+                          int n_substeps, void *stream);
 
 enum {
     THOR_FORCING_NONE = 0,

@@ -10,24 +10,30 @@ import pytest
 from tests.common import ROOT
 
 
-def _declared():
+def _declared(headers):
     names = set()
     inc = os.path.join(ROOT, "include")
-    for fn in os.listdir(inc):
-        if fn.endswith(".h"):
-            src = open(os.path.join(inc, fn)).read()
-            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-            names |= set(re.findall(r"\b(thor_\w+)\s*\(", src))
+    for fn in headers:
+        src = open(os.path.join(inc, fn)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(thor_\w+)\s*\(", src))
     return names
 
 
 def test_exports_every_declared_symbol(built):
-    names = _declared()
+    """thor_dyn.h and thor_mlp.h: both builds of libthor_amd; thor_io.h: libthor_io."""
+    headers = sorted(fn for fn in os.listdir(os.path.join(ROOT, "include")) if fn.endswith(".h"))
+    assert headers == ["thor_dyn.h", "thor_io.h", "thor_mlp.h"], headers
+    names = _declared(["thor_dyn.h", "thor_mlp.h"])
     assert len(names) > 15
     for variant in ("", "pm"):
         L = built.lib(variant)
         missing = [n for n in names if not hasattr(L, n)]
         assert not missing, f"{variant or 'default'} build lacks {missing}"
+    from thor_amd import io
+    names = _declared(["thor_io.h"])
+    missing = [n for n in names if not hasattr(io.lib(), n)]
+    assert len(names) > 10 and not missing, f"libthor_io lacks {missing}"
 
 
 def test_no_device_is_reported_not_crashed(built):

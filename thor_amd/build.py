@@ -20,6 +20,11 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 HOST_SRC = ["host.cpp", "grid.cpp"]
+# THOR's file formats (.thr, HDF5 output and restart) and the CLI driver:
+# built against the HDF5 C library of the image, beside the core library.
+IO_LIB = os.path.join(HERE, "libthor_io.so")
+ESP_EXE = os.path.join(HERE, "thor_esp")
+HDF5 = os.environ.get("HDF5_ROOT", "/opt/conda")
 HIP_SRC = ["dyn.hip", "mlp.hip"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
@@ -58,7 +63,29 @@ def build(verbose: bool = True) -> str:
             _run([hipcc, *HIPFLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", o])
             objs.append(o)
         _link(objs, lib)
+    build_io()
     return LIB
+
+
+def build_io() -> None:
+    """libthor_io.so (HDF5 and .thr) and the thor_esp driver, linked against
+    libthor_amd.so through an rpath relative to themselves."""
+    if not os.path.exists(os.path.join(HDF5, "include", "hdf5.h")):
+        print(f"HDF5 not found under {HDF5}: libthor_io.so and thor_esp are not built", flush=True)
+        return
+    rp = "-Wl,-rpath,$ORIGIN"
+    # the system C++ runtime ahead of the HDF5 installation's own (older) one,
+    # which ROCm's libraries cannot use
+    sysdir = "/usr/lib/x86_64-linux-gnu"
+    _run(["g++", *CXXFLAGS, "-shared", "-I", os.path.join(HDF5, "include"), os.path.join(CSRC, "esp_io.cpp"),
+          "-o", IO_LIB + ".tmp", f"-L{HERE}", "-l:libthor_amd.so", f"-L{HDF5}/lib", "-lhdf5", rp,
+          f"-Wl,-rpath,{sysdir}", f"-Wl,-rpath,{HDF5}/lib"])
+    os.replace(IO_LIB + ".tmp", IO_LIB)
+    _run(["g++", *CXXFLAGS, "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROCM, "include"),
+          os.path.join(CSRC, "esp_main.cpp"), "-o", ESP_EXE + ".tmp", f"-L{HERE}", "-l:libthor_io.so",
+          "-l:libthor_amd.so", f"-L{ROCM}/lib", "-lamdhip64", rp, f"-Wl,-rpath,{ROCM}/lib",
+          f"-Wl,-rpath,{sysdir}", f"-Wl,-rpath,{HDF5}/lib"])
+    os.replace(ESP_EXE + ".tmp", ESP_EXE)
 
 
 if __name__ == "__main__":

@@ -131,6 +131,11 @@ class State(C.Structure):
                 ("pressure", _dp)]
 
 
+class Forcing(C.Structure):
+    """thor_forcing: THOR's profx_Qheat, profx_dMh, profx_dWh as device pointers."""
+    _fields_ = [("Qheat", _dp), ("dMh", _dp), ("dWh", _dp)]
+
+
 _libs: dict = {}
 
 
@@ -162,6 +167,8 @@ def lib(variant: str = "") -> C.CDLL:
             for f in ("thor_dyn_set_state", "thor_dyn_get_state"):
                 getattr(L, f).argtypes = [C.c_void_p, C.POINTER(State)]
             L.thor_dyn_thor.argtypes = [C.c_void_p]
+            L.thor_dyn_step.argtypes = [C.c_void_p, C.POINTER(State), C.POINTER(Forcing), C.c_double, C.c_int,
+                                        C.c_void_p]
             L.thor_dyn_profx.argtypes = [C.c_void_p, C.c_int]
             L.thor_dyn_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
             L.thor_dyn_sync.argtypes = [C.c_void_p]
@@ -335,6 +342,17 @@ class Dyn:
 
     def thor(self):
         self._chk(self.L.thor_dyn_thor(self._h))
+
+    def step(self, state: dict, forcing: "dict | None" = None, dt: float = 0.0, n_substeps: int = 0,
+             stream: int = 0):
+        """thor_dyn_step on caller-owned device arrays: state and forcing map
+        the field names to device addresses (THOR layout); the new state
+        overwrites the state arrays in stream order."""
+        cast = lambda a: C.cast(C.c_void_p(a), _dp)
+        s = State(1, *(cast(state[k]) for k in ("Mh", "Wh", "W", "Rho", "pressure")))
+        f = Forcing(*(cast(forcing[k]) for k in ("Qheat", "dMh", "dWh"))) if forcing else None
+        self._chk(self.L.thor_dyn_step(self._h, C.byref(s), C.byref(f) if f else None, dt, n_substeps,
+                                       C.c_void_p(stream) if stream else None))
 
     def set_newtonian(self, q: NewtonianParams):
         self._chk(self.L.thor_dyn_set_newtonian(self._h, C.byref(q)))

@@ -44,6 +44,8 @@ struct Dyn {
     Geo g{};
     thor_params prm{};
     int P = 0, S = 0, nv = 0;
+    std::vector<double> alt_host, alth_host;  // Altitude [nv], Altitudeh [nv + 1]
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;  // ordering against a caller's stream (thor_dyn_step)
     hipStream_t stream = nullptr;
     std::vector<void *> allocs;
     // state y(n) and step buffers (see Fields)
@@ -609,6 +611,65 @@ static int copy_field(Dyn &d, double *dev, int nlev, int ncomp, double *user, bo
     return THOR_OK;
 }
 
+// Kd profiles of ESP::initial_values (esp_initial.cu:1292-1335): horizontal
+// hyperdiffusion and divergence damping, and the diffusive sponge, whose
+// strength adds to Kdh4 when its order is HyDiffOrder and fills Kdh2
+// otherwise; the vertical hyperdiffusion strength.
+struct KdProfiles {
+    std::vector<double> Kdh4, Kdhz, Kdh2, Kdv6;
+};
+static KdProfiles kd_profiles(const thor_params &prm, const double *Alt, const double *Alth, int nv) {
+    KdProfiles k;
+    k.Kdh4.assign(nv, 0.0);
+    k.Kdhz.assign(nv, 0.0);
+    k.Kdh2.assign(nv, 0.0);
+    k.Kdv6.assign(nv, 0.0);
+    for (int lev = 0; lev < nv; lev++) {
+        double dbar = std::sqrt(2 * M_PI / 5) * prm.A / (std::pow(2, prm.glevel));
+        k.Kdh4[lev] = (prm.Diffc) * std::pow(dbar, 1.0 * prm.HyDiffOrder) / prm.timestep;
+        k.Kdhz[lev] = (prm.DivDampc) * std::pow(dbar, 4.) / prm.timestep;
+        if (prm.DiffSponge) {
+            const double n = Alt[lev] / Alth[nv], ns = prm.ns_diff_sponge;
+            const double ks = n > ns ? prm.Dv_sponge * std::pow(std::sin(0.5 * M_PI * (n - ns) / (1.0 - ns)), 2) : 0.0;
+            if (prm.order_diff_sponge == prm.HyDiffOrder)
+                k.Kdh4[lev] += ks * std::pow(dbar, 1.0 * prm.order_diff_sponge) / prm.timestep;
+            else
+                k.Kdh2[lev] = ks * std::pow(dbar, 1.0 * prm.order_diff_sponge) / prm.timestep;
+        }
+        const double dz = Alth[lev + 1] - Alth[lev];
+        k.Kdv6[lev] = prm.Diffc_v * std::pow(dz, 1.0 * prm.VertHyDiffOrder) / prm.timestep;
+    }
+    return k;
+}
+
+// A new large time step: the Kd profiles scale with 1 / timestep. Waits for
+// the queued work that reads the old profiles.
+static int set_timestep(Dyn &d, double dt) {
+    d.prm.timestep = dt;
+    const KdProfiles k = kd_profiles(d.prm, d.alt_host.data(), d.alth_host.data(), d.nv);
+    const size_t nb = (size_t)d.nv * sizeof(double);
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    HIP_TRY(hipMemcpy((void *)d.g.Kdh4, k.Kdh4.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)d.g.Kdhz, k.Kdhz.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)d.g.Kdh2, k.Kdh2.data(), nb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy((void *)d.g.Kdv6, k.Kdv6.data(), nb, hipMemcpyHostToDevice));
+    if (d.g.lvc) {
+        std::vector<float> kz(d.nv);
+        for (int lev = 0; lev < d.nv; lev++) kz[lev] = (float)k.Kdhz[lev];
+        HIP_TRY(hipMemcpy((void *)(d.g.lvc + 3 * d.nv), kz.data(), d.nv * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return THOR_OK;
+}
+
+// Caller's array in THOR's layout <-> the library's level-planar copy, in
+// stream order on the handle's stream (no host synchronisation).
+static void transpose_async(Dyn &d, const double *src, double *dst, int nlev, int ncomp, bool to_planar) {
+    const long n = (long)d.P * nlev * ncomp;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (to_planar) k_aos_to_soa<<<nb, 256, 0, d.stream>>>(src, dst, d.P, d.S, nlev, ncomp);
+    else k_soa_to_aos<<<nb, 256, 0, d.stream>>>(src, dst, d.P, d.S, nlev, ncomp);
+}
+
 } // namespace thor
 
 using thor::Dyn;
@@ -659,7 +720,9 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         thor_dyn_destroy(h);
         return code;
     };
-    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking) != hipSuccess
+        || hipEventCreateWithFlags(&d.ev_in, hipEventDisableTiming) != hipSuccess
+        || hipEventCreateWithFlags(&d.ev_out, hipEventDisableTiming) != hipSuccess)
         return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: stream"));
 
     // Geometry in level-planar layout.
@@ -682,23 +745,11 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
             for (int c = 0; c < ncoef; c++) v[(size_t)c * S + p] = src[(size_t)p * ncoef + c];
         return v;
     };
-    // horizontal diffusion and divergence damping, and the diffusive sponge,
-    // whose strength adds to Kdh4 when its order is HyDiffOrder and fills Kdh2
-    // otherwise (esp_initial.cu:1292-1321)
-    std::vector<double> Kdh4(nv), Kdhz(nv), Kdh2(nv, 0.0);
-    for (int lev = 0; lev < nv; lev++) {
-        double dbar = std::sqrt(2 * M_PI / 5) * prm->A / (std::pow(2, prm->glevel));
-        Kdh4[lev] = (prm->Diffc) * std::pow(dbar, 1.0 * prm->HyDiffOrder) / prm->timestep;
-        Kdhz[lev] = (prm->DivDampc) * std::pow(dbar, 4.) / prm->timestep;
-        if (prm->DiffSponge) {
-            const double n = gd->Altitude[lev] / gd->Altitudeh[nv], ns = prm->ns_diff_sponge;
-            const double ks = n > ns ? prm->Dv_sponge * std::pow(std::sin(0.5 * M_PI * (n - ns) / (1.0 - ns)), 2) : 0.0;
-            if (prm->order_diff_sponge == prm->HyDiffOrder)
-                Kdh4[lev] += ks * std::pow(dbar, 1.0 * prm->order_diff_sponge) / prm->timestep;
-            else
-                Kdh2[lev] = ks * std::pow(dbar, 1.0 * prm->order_diff_sponge) / prm->timestep;
-        }
-    }
+    // Kd profiles (esp_initial.cu:1292-1335)
+    d.alt_host.assign(gd->Altitude, gd->Altitude + nv);
+    d.alth_host.assign(gd->Altitudeh, gd->Altitudeh + nv + 1);
+    const KdProfiles kd = kd_profiles(*prm, gd->Altitude, gd->Altitudeh, nv);
+    const std::vector<double> &Kdh4 = kd.Kdh4, &Kdhz = kd.Kdhz, &Kdh2 = kd.Kdh2, &Kdv6 = kd.Kdv6;
     d.sponge_pass = prm->HyDiff && prm->DiffSponge && prm->order_diff_sponge != prm->HyDiffOrder
                         ? prm->order_diff_sponge / 2 - 1 : -1;
     d.g.ssp = (prm->order_diff_sponge / 2 + 1) % 2 ? -1.0 : 1.0;
@@ -712,13 +763,6 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         if (d.sponge_pass >= 1) d.g.dsp_scale = 1.0 / std::pow(d.mid_scale, d.sponge_pass);
     }
     d.diff_sign = std::pow(-1.0, prm->HyDiffOrder / 2);
-    // vertical hyperdiffusion strength (esp_initial.cu:1327-1334)
-    std::vector<double> Kdv6(nv);
-    for (int lev = 0; lev < nv; lev++) {
-        const double dz = gd->Altitudeh[lev + 1] - gd->Altitudeh[lev];
-        Kdv6[lev] = prm->Diffc_v * std::pow(dz, 1.0 * prm->VertHyDiffOrder) / prm->timestep;
-    }
-
     auto up_d = [&](const std::vector<double> &v, const double **dst) -> bool {
         double *ptr;
         if (hipMalloc((void **)&ptr, v.size() * sizeof(double)) != hipSuccess) return false;
@@ -867,6 +911,8 @@ extern "C" void thor_dyn_destroy(thor_dyn *h) {
     }
     for (auto ev : h->free_events) hipEventDestroy(ev);
     for (void *ptr : h->allocs) hipFree(ptr);
+    if (h->ev_in) hipEventDestroy(h->ev_in);
+    if (h->ev_out) hipEventDestroy(h->ev_out);
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
@@ -893,6 +939,54 @@ extern "C" thor_status thor_dyn_get_state(thor_dyn *h, const thor_state *s) { re
 extern "C" thor_status thor_dyn_thor(thor_dyn *h) {
     if (!h) return thor::fail(THOR_ERR_BAD_ARG, "thor_dyn_thor: null handle");
     return thor::thor_step(*h);
+}
+
+extern "C" thor_status thor_dyn_step(thor_dyn *h, const thor_state *state, const thor_forcing *forcing, double dt,
+                                     int n_substeps, void *stream) {
+    using namespace thor;
+    if (!h || !state) return fail(THOR_ERR_BAD_ARG, "thor_dyn_step: null argument");
+    if (!state->on_device) return fail(THOR_ERR_BAD_ARG, "thor_dyn_step: the state must be in device memory");
+    double *user[5] = {state->Mh, state->Wh, state->W, state->Rho, state->pressure};
+    for (double *u : user)
+        if (!u) return fail(THOR_ERR_BAD_ARG, "thor_dyn_step: null state field");
+    if (forcing && (!forcing->Qheat || !forcing->dMh || !forcing->dWh))
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_step: null forcing field");
+    if (n_substeps < 0 || (n_substeps > 0 && (n_substeps < 2 || n_substeps % 2)) || dt < 0 || std::isnan(dt))
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_step: need dt >= 0 and n_substeps 0 or even >= 2");
+    Dyn &d = *h;
+    if (dt > 0 && dt != d.prm.timestep) {
+        const int rc = set_timestep(d, dt);
+        if (rc != THOR_OK) return rc;
+    }
+    if (n_substeps > 0) d.prm.n_substeps = n_substeps;
+    // the caller's queued work (its ProfX, its modules) first
+    const hipStream_t cs = (hipStream_t)stream;
+    HIP_TRY(hipEventRecord(d.ev_in, cs));
+    HIP_TRY(hipStreamWaitEvent(d.stream, d.ev_in, 0));
+    const char *names[5] = {"Mh", "Wh", "W", "Rho", "pressure"};
+    double *dev[5];
+    int nlev[5], ncomp[5];
+    for (int i = 0; i < 5; i++) {
+        field_ptr(d, names[i], dev[i], nlev[i], ncomp[i]);
+        transpose_async(d, user[i], dev[i], nlev[i], ncomp[i], true);
+    }
+    if (forcing) {
+        transpose_async(d, forcing->Qheat, d.f.Qheat, d.nv, 1, true);
+        transpose_async(d, forcing->dMh, d.f.dMh, d.nv, 3, true);
+        transpose_async(d, forcing->dWh, d.f.dWh, d.nv + 1, 1, true);
+    }
+    HIP_TRY(hipGetLastError());
+    const int rc = thor_step(d);
+    if (rc != THOR_OK) return rc;
+    for (int i = 0; i < 5; i++) {
+        field_ptr(d, names[i], dev[i], nlev[i], ncomp[i]);  // the step swapped y(n) with the stage arrays
+        transpose_async(d, dev[i], user[i], nlev[i], ncomp[i], false);
+    }
+    HIP_TRY(hipGetLastError());
+    // the caller's later work on its stream sees the new state
+    HIP_TRY(hipEventRecord(d.ev_out, d.stream));
+    HIP_TRY(hipStreamWaitEvent(cs, d.ev_out, 0));
+    return THOR_OK;
 }
 
 extern "C" thor_status thor_dyn_profx(thor_dyn *h, int forcing) {
