@@ -343,6 +343,30 @@ thor_status thor_dyn_register_rk_array(thor_dyn *h, double *array_d, double *arr
                                        int dimensions);
 thor_status thor_dyn_clear_rk_arrays(thor_dyn *h);
 
+/* ------------------------------------------------- tracer module (chemistry) */
+
+/* Passive tracers advected with the flow: the dynamical part of THOR's
+ * chemistry module (src/physics/modules/src/chemistry.cu:333-502, kernels
+ * src/physics/modules/inc/thor_chemistry.h). A consumer of the hooks above:
+ * thor_tracers_create registers its tracer arrays with the RK scheme of h
+ * (device_RK_array_manager::register_array, as chemistry.cu:115 does), and
+ * thor_tracers_hooks fills a thor_hooks with its four callbacks, which the
+ * caller installs with thor_dyn_set_hooks:
+ *   init: tracerk = tracer, tracers = 0;
+ *   slow modes (with HyDiff): the stage's fourth-order diffusion of the mixing
+ *         ratio tracerk / Rhok (Tracer_Eq_Diffusion, two passes);
+ *   fast modes: Tracer_Eq, the flux-form advection of the tracer mass by the
+ *         stage momentum plus its deviation, into tracers;
+ *   end: tracer = tracerk.
+ * The tracer is THOR's tracer_d, the mass density rho q [P][nv][ntr] in its
+ * layout; 1 <= ntr <= 6. The module must be destroyed before h. */
+typedef struct thor_tracers thor_tracers;
+thor_status thor_tracers_create(thor_dyn *h, int ntr, thor_tracers **out);
+void thor_tracers_destroy(thor_tracers *t);
+thor_status thor_tracers_set(thor_tracers *t, const double *tracer, long n);
+thor_status thor_tracers_get(thor_tracers *t, double *tracer, long n);
+thor_status thor_tracers_hooks(thor_tracers *t, thor_hooks *hooks);
+
 #ifdef __cplusplus
 }
 #endif

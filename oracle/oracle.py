@@ -82,6 +82,8 @@ def lib(variant: str = ""):
         L.oracle_profx.restype = C.c_int
         L.oracle_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
         L.oracle_run.restype = C.c_int
+        L.oracle_set_tracers.argtypes = [C.c_void_p, C.c_int]
+        L.oracle_set_tracers.restype = None
         _libs[variant] = L
     return _libs[variant]
 
@@ -159,6 +161,10 @@ class Oracle:
         out = np.zeros(5)
         self.L.oracle_globdiag(self._h, out.ctypes.data_as(_dp))
         return out
+
+    def set_tracers(self, ntr: int):
+        """Chemistry tracers driven through the physics hooks (0: none)."""
+        self.L.oracle_set_tracers(self._h, ntr)
 
     def set_newtonian(self, q):
         """q: a ctypes NewtonianParams (or 8 numbers in its field order)."""

@@ -8,6 +8,9 @@
 //   src/headers/dyn/thor_div.h        DivM_Op, _Poles
 //   src/headers/dyn/thor_slowmodes.h  Compute_Slow_Modes, _Poles
 //   src/headers/phy/profx_held_suarez.h  held_suarez
+//   src/physics/modules/inc/thor_chemistry.h  Tracer_Eq, Tracer_Eq_Diffusion, _Poles
+//                                     (kRefTracers tracers; the first diffusion
+//                                     pass goes to "diff", as in chemistry.cu)
 // and are launched with the geometry of src/ESP/thor_driver.cu:70-95 and
 // src/ESP/profx_driver.cu:79,303-314, on device copies of named arrays in
 // THOR's own layout (the oracle's names). Built by `make -C oracle ref`
@@ -20,6 +23,7 @@
 #include "dyn/thor_div.h"
 #include "dyn/thor_slowmodes.h"
 #include "phy/profx_held_suarez.h"
+#include "thor_chemistry.h"
 
 #include "thor_oracle.h"
 
@@ -28,6 +32,8 @@
 #include <vector>
 
 namespace {
+
+constexpr int kRefTracers = 2;
 
 struct RefKernels {
     oracle_params p{};
@@ -99,8 +105,11 @@ void *ref_create(const oracle_grid *g, const oracle_params *p) {
                           "diffrv", "diffprv"})
         r->add(n, N);
     for (const char *n : {"Wh", "profx_dWh", "Whk", "hh", "pth", "gtilh", "SlowWh"}) r->add(n, Ni);
-    for (const char *n : {"Mh", "profx_dMh", "Mhk", "Adv", "v", "SlowMh", "diffmh", "diffmv", "DivM", "divg_Mh"})
+    for (const char *n : {"Mh", "profx_dMh", "Mhk", "Adv", "v", "SlowMh", "diffmh", "diffmv", "DivM", "divg_Mh", "Mhs"})
         r->add(n, 3 * N);
+    r->add("Rhos", N);
+    r->add("Whs", Ni);
+    for (const char *n : {"tracerk", "tracers", "difftr"}) r->add(n, kRefTracers * N);
     for (const char *n : {"diff", "diff_sponge", "boundary_flux"}) r->add(n, 6 * N);
     for (const char *n : {"Kdh4", "Kdhz", "Kdh2", "Kdv6"}) r->add(n, r->nv);
     r->add("diff2", 6 * P * (r->nv + 2));  // vertical_diff's ghost-extended columns
@@ -195,6 +204,23 @@ int ref_kernel(void *h, const char *name, int a0, int a1) {
             r["diffprv"], r["pressurek"], r["h"], r["hh"], r["gtil"], r["grad"], r["div"], r["Altitude"],
             r["Altitudeh"], A, r.p.Gravit, r["Cp"], r["Rd"], r["func_r"], r.point_local, nv, P, deep, nonhydro,
             r["profx_dMh"], r["profx_dWh"], r["profx_Qheat"], false, ghv);
+    } else if (n == "tracer_diffusion") {  // a0: laststep (chemistry.cu:345-427)
+        const dim3 NBTR(r.nr, nv, kRefTracers), NBTRP(2, nv, kRefTracers);
+        Tracer_Eq_Diffusion<16, 16><<<NBTR, NT>>>(r["difftr"], r["diff"], r["tracerk"], r["Rhok"], r["areasTr"],
+                                                  r["nvecoa"], r["nvecti"], r["nvecte"], r["Kdh4"], r["Altitude"], A,
+                                                  r.maps, kRefTracers, 16, a0, deep);
+        Tracer_Eq_Diffusion_Poles<5><<<NBTRP, 1>>>(r["difftr"], r["diff"], r["tracerk"], r["Rhok"], r["areasTr"],
+                                                   r["nvecoa"], r["nvecti"], r["nvecte"], r["Kdh4"], r["Altitude"],
+                                                   r["Altitudeh"], A, r.point_local, kRefTracers, P, a0, deep);
+    } else if (n == "tracer_eq") {  // a0: the substep length in milliseconds (chemistry.cu:429-492)
+        const dim3 NBTR(r.nr, nv, kRefTracers), NBPT(2, 1, kRefTracers);
+        const double dt = a0 * 1e-3;
+        Tracer_Eq<16, 16><<<NBTR, NT>>>(r["tracers"], r["tracerk"], r["Rhos"], r["Rhok"], r["Mhs"], r["Mhk"], r["Whs"],
+                                        r["Whk"], r["difftr"], r["div"], r["Altitude"], r["Altitudeh"], A, dt, r.maps,
+                                        kRefTracers, 16, deep);
+        Tracer_Eq_Poles<6><<<NBPT, 1>>>(r["tracers"], r["tracerk"], r["Rhos"], r["Rhok"], r["Mhs"], r["Mhk"],
+                                        r["Whs"], r["Whk"], r["difftr"], r["div"], r["Altitude"], r["Altitudeh"], A,
+                                        dt, r.point_local, kRefTracers, P, nv, deep);
     } else if (n == "held_suarez") {
         held_suarez<<<NBALL1, NTH>>>(r["Mh"], r["pressure"], r["Rho"], r["temperature"], r.p.Gravit, r.p.Cp, r.p.Rd,
                                      r["Altitude"], r["Altitudeh"], r["lonlat"], r.p.timestep, P);

@@ -58,6 +58,8 @@ struct Oracle {
     // vertical solve: keep each column's tridiagonal system (a, b, c, d per
     // interface, as "tri_a" .. "tri_d" [P][nv+1]) for the residual checks
     bool capture_tri = false;
+    // chemistry tracers (thor_chemistry.h) driven through the physics hooks
+    int ntr = 0;
 
     double *A_(const char *n) { return arr[n].data(); }
 };
@@ -1373,6 +1375,253 @@ void density_pressure(Oracle &o, double dt) {
     }
 }
 
+// ---------------------------------------------------------------- tracers
+// Tracer_Eq_Diffusion + _Poles (thor_chemistry.h:364-760) on tracerk / Rhok:
+// laststep = false writes the pass into "trdiff" (THOR's diff_d), laststep =
+// true the tendency into "difftr". The pole kernel computes only the first
+// pass (its body sits in the !laststep branch), reads tracer 0 for the pole
+// itself, indexes the ring as (k + 1 + 1) % 5 and weights x with nvecoa's z.
+void tracer_diffusion(Oracle &o, bool laststep) {
+    const int nv = o.nv, P = o.P, ntr = o.ntr;
+    double *tr = o.A_("tracerk"), *Rho = o.A_("Rhok"), *dif = o.A_("trdiff"), *difftr = o.A_("difftr");
+    const double *aTr = o.g.areasTr, *noa = o.g.nvecoa, *nti = o.g.nvecti, *nte = o.g.nvecte;
+    const double o3 = 1.0 / 3.0, o6 = 1.0 / 6.0;
+    double as[NHL2], rs_[NHL2];
+    for (int var = 0; var < ntr; var++)
+        for (int ib = 0; ib < o.nr; ib++) {
+            const int *m = o.g.maps + ib * NHL2;
+            for (int lev = 0; lev < nv; lev++) {
+                fill(rs_, m, [&](int gi) { return Rho[gi * nv + lev]; });
+                if (laststep) fill(as, m, [&](int gi) { return dif[(gi * nv + lev) * ntr + var]; });
+                else fill(as, m, [&](int gi) { return tr[(gi * nv + lev) * ntr + var] / Rho[gi * nv + lev]; });
+                const double rscale = rscale_of(o, lev);
+                const double sdiff = laststep ? -o.Kdh4[lev] : 0.0;
+                for (int y = 0; y < NL; y++)
+                    for (int x = 0; x < NL; x++) {
+                        Stencil st = stencil(m, x, y);
+                        const int id = st.id, ir = st.ir, n_faces = st.pent ? 5 : 6;
+                        double lap = 0.0, lap1 = 0, lap2 = 0, lap3 = 0, lap4, lap5, lap6, AT1 = 0, AT2;
+                        double lapx1 = 0, lapx2 = 0, lapy1 = 0, lapy2 = 0, lapz1 = 0, lapz2 = 0, vdiff;
+                        for (int j = 0; j < 6; j++) {
+                            const int jp1 = (j + 1) % n_faces, jp2 = (j + 2) % n_faces;
+                            int pt1, pt2, pt3;
+                            tri_pts(x, y, j, st.pent, pt1, pt2, pt3);
+                            if (laststep)
+                                vdiff = 0.5 * rscale * (2.0 * rs_[ir] + rs_[pt1] + 2.0 * rs_[pt2] + rs_[pt3]) * o6 * sdiff;
+                            else
+                                vdiff = 0.5 * rscale;
+                            if (j == 0) {
+                                AT1 = rscale / aTr[id * 6 + j];
+                                lap1 = o6 * (as[ir] + as[pt1]) - o3 * as[pt2];
+                                lap2 = o6 * (as[pt1] + as[pt2]) - o3 * as[ir];
+                                lap3 = o6 * (as[ir] + as[pt2]) - o3 * as[pt1];
+                            }
+                            AT2 = rscale / aTr[id * 6 + jp1];
+                            lap4 = o6 * (as[ir] + as[pt2]) - o3 * as[pt3];
+                            lap5 = o6 * (as[pt2] + as[pt3]) - o3 * as[ir];
+                            lap6 = o6 * (as[ir] + as[pt3]) - o3 * as[pt2];
+                            auto L = [&](int k, int e1, double l1, double l2, double l3, double A) {
+                                return (-l1 * nti[id * 18 + e1 * 3 + k] + l2 * nte[id * 18 + e1 * 3 + k]
+                                        + l3 * nti[id * 18 + ((e1 + 1) % n_faces) * 3 + k])
+                                       * A;
+                            };
+                            if (j == 0) {
+                                lapx1 = L(0, j, lap1, lap2, lap3, AT1);
+                                lapx2 = L(0, jp1, lap4, lap5, lap6, AT2);
+                                lapy1 = L(1, j, lap1, lap2, lap3, AT1);
+                                lapy2 = L(1, jp1, lap4, lap5, lap6, AT2);
+                                lapz1 = L(2, j, lap1, lap2, lap3, AT1);
+                                lapz2 = L(2, jp1, lap4, lap5, lap6, AT2);
+                            } else {
+                                lapx1 = lapx2;
+                                lapx2 = L(0, jp1, lap4, lap5, lap6, AT2);
+                                lapy1 = lapy2;
+                                lapy2 = L(1, jp1, lap4, lap5, lap6, AT2);
+                                lapz1 = lapz2;
+                                lapz2 = L(2, jp1, lap4, lap5, lap6, AT2);
+                            }
+                            (void)jp2;
+                            lap += ((lapx1 + lapx2) * noa[id * 18 + j * 3 + 0] + (lapy1 + lapy2) * noa[id * 18 + j * 3 + 1]
+                                    + (lapz1 + lapz2) * noa[id * 18 + j * 3 + 2])
+                                   * vdiff;
+                            if (st.pent && j == 4) break;
+                        }
+                        if (laststep) difftr[(id * nv + lev) * ntr + var] = lap;
+                        else dif[(id * nv + lev) * ntr + var] = lap;
+                    }
+            }
+        }
+    if (laststep) return;  // the pole kernel writes nothing in its last pass
+    for (int var = 0; var < ntr; var++)
+        for (int id = P - 2; id < P; id++)
+            for (int lev = 0; lev < nv; lev++) {
+                int loc[5];
+                for (int i = 0; i < 5; i++) loc[i] = o.g.point_local[id * 6 + i];
+                double a_p[6];
+                a_p[0] = tr[(id * nv + lev) * ntr + 0] / Rho[id * nv + lev];
+                for (int i = 1; i < 6; i++) a_p[i] = tr[(loc[i - 1] * nv + lev) * ntr + var] / Rho[loc[i - 1] * nv + lev];
+                const double rscale = rscale_of(o, lev);
+                double lap = 0.0, lap1 = 0, lap2 = 0, lap3 = 0, lap4, lap5, lap6, AT1 = 0, AT2;
+                double lapx1 = 0, lapx2 = 0, lapy1 = 0, lapy2 = 0, lapz1 = 0, lapz2 = 0;
+                for (int k = 0; k < 5; k++) {
+                    const int j = k + 1, jp1 = (j + 1) % 5, jp2 = (j + 2) % 5, kp1 = (k + 1) % 5, kp2 = (k + 2) % 5;
+                    const double vdiff = 0.5 * rscale;
+                    if (k == 0) {
+                        lap1 = (o6 * (a_p[0] + a_p[j]) - o3 * a_p[jp1]);
+                        lap2 = (o6 * (a_p[j] + a_p[jp1]) - o3 * a_p[0]);
+                        lap3 = (o6 * (a_p[0] + a_p[jp1]) - o3 * a_p[j]);
+                        AT1 = rscale / aTr[id * 6 + k];
+                    }
+                    AT2 = rscale / aTr[id * 6 + kp1];
+                    lap4 = (o6 * (a_p[0] + a_p[jp1]) - o3 * a_p[jp2]);
+                    lap5 = (o6 * (a_p[jp1] + a_p[jp2]) - o3 * a_p[0]);
+                    lap6 = (o6 * (a_p[0] + a_p[jp2]) - o3 * a_p[jp1]);
+                    auto L = [&](int c, int e1, int e2, double l1, double l2, double l3, double A) {
+                        return (-l1 * nti[id * 18 + e1 * 3 + c] + l2 * nte[id * 18 + e1 * 3 + c]
+                                + l3 * nti[id * 18 + e2 * 3 + c])
+                               * A;
+                    };
+                    if (k == 0) {
+                        lapx1 = L(0, k, kp1, lap1, lap2, lap3, AT1);
+                        lapx2 = L(0, kp1, kp2, lap4, lap5, lap6, AT2);
+                        lapy1 = L(1, k, kp1, lap1, lap2, lap3, AT1);
+                        lapy2 = L(1, kp1, kp2, lap4, lap5, lap6, AT2);
+                        lapz1 = L(2, k, kp1, lap1, lap2, lap3, AT1);
+                        lapz2 = L(2, kp1, kp2, lap4, lap5, lap6, AT2);
+                    } else {
+                        lapx1 = lapx2;
+                        lapx2 = L(0, kp1, kp2, lap4, lap5, lap6, AT2);
+                        lapy1 = lapy2;
+                        lapy2 = L(1, kp1, kp2, lap4, lap5, lap6, AT2);
+                        lapz1 = lapz2;
+                        lapz2 = L(2, kp1, kp2, lap4, lap5, lap6, AT2);
+                    }
+                    lap += ((lapx1 + lapx2) * noa[id * 18 + k * 3 + 2] + (lapy1 + lapy2) * noa[id * 18 + k * 3 + 1]
+                            + (lapz1 + lapz2) * noa[id * 18 + k * 3 + 2])
+                           * vdiff;
+                }
+                dif[(id * nv + lev) * ntr + var] = lap;
+            }
+}
+
+// Tracer_Eq + _Poles (thor_chemistry.h:31-361): flux-form advection of the
+// tracer mass by Mhs + Mhk and Whs + Whk into tracers, plus difftr * dt.
+void tracer_eq(Oracle &o, double dt) {
+    const int nv = o.nv, P = o.P, ntr = o.ntr;
+    double *trs = o.A_("tracers"), *trk = o.A_("tracerk"), *Rho = o.A_("Rhos"), *Rhok = o.A_("Rhok"),
+           *Mh = o.A_("Mhs"), *Mhk = o.A_("Mhk"), *Wh = o.A_("Whs"), *Whk = o.A_("Whk"), *difftr = o.A_("difftr");
+    const double *div = o.g.div, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    const double A = o.p.A;
+    auto mixr = [&](int gi, int l, int itr) { return trk[(gi * nv + l) * ntr + itr] / Rhok[gi * nv + l]; };
+    // the column part (identical in both kernels but for the poles' lev-0 quirk)
+    auto column = [&](int id, int lev, int itr, double nflx, double trc, bool pole) {
+        const double altht = Alth[lev + 1], althl = Alth[lev];
+        double r2p, r2m, r2l;
+        if (o.p.DeepModel) {
+            const double alt = Alt[lev];
+            r2p = MPOW(altht + A, 2.0);
+            r2m = MPOW(alt + A, 2.0);
+            r2l = MPOW(althl + A, 2.0);
+        } else {
+            r2p = r2m = r2l = 1.0;
+        }
+        double trhl, trht, xi, xim1, xip1, a, b;
+        if (lev == 0) {
+            trhl = 0.0;
+            xi = Alth[lev + 1];
+            xim1 = Alt[lev];
+            xip1 = Alt[lev + 1];
+            a = (xi - xip1) / (xim1 - xip1);
+            b = (xi - xim1) / (xip1 - xim1);
+            if (pole)
+                trht = trk[(id * nv + lev) * ntr + itr] / Rhok[id * nv + lev] * a
+                       + trk[(id * nv + lev + 1) * ntr + itr] / Rhok[id * nv + lev] * b;
+            else
+                trht = mixr(id, lev, itr) * a + mixr(id, lev + 1, itr) * b;
+        } else if (lev == nv - 1) {
+            xi = Alth[lev];
+            xim1 = Alt[lev - 1];
+            xip1 = Alt[lev];
+            a = (xi - xip1) / (xim1 - xip1);
+            b = (xi - xim1) / (xip1 - xim1);
+            trhl = mixr(id, lev - 1, itr) * a + mixr(id, lev, itr) * b;
+            trht = 0.0;
+        } else {
+            xi = Alth[lev];
+            xim1 = Alt[lev - 1];
+            xip1 = Alt[lev];
+            a = (xi - xip1) / (xim1 - xip1);
+            b = (xi - xim1) / (xip1 - xim1);
+            trhl = mixr(id, lev - 1, itr) * a + mixr(id, lev, itr) * b;
+            xi = Alth[lev + 1];
+            xim1 = Alt[lev];
+            xip1 = Alt[lev + 1];
+            a = (xi - xip1) / (xim1 - xip1);
+            b = (xi - xim1) / (xip1 - xim1);
+            trht = mixr(id, lev, itr) * a + mixr(id, lev + 1, itr) * b;
+        }
+        double whl, wht;
+        if (lev == 0) {
+            whl = 0.0;
+            wht = Wh[id * (nv + 1) + lev + 1] + Whk[id * (nv + 1) + lev + 1];
+        } else {
+            whl = Wh[id * (nv + 1) + lev] + Whk[id * (nv + 1) + lev];
+            wht = Wh[id * (nv + 1) + lev + 1] + Whk[id * (nv + 1) + lev + 1];
+        }
+        const double dz = altht - althl;
+        const double dtr_dalt = (wht * trht * r2p - whl * trhl * r2l) / (dz * r2m);
+        const double dtr = -(nflx + dtr_dalt) * dt;
+        const double r = Rhok[id * nv + lev] + Rho[id * nv + lev];
+        const double trn = trc * r + dtr;
+        const long e = (id * nv + lev) * ntr + itr;
+        trs[e] = trn - trc * Rhok[id * nv + lev];
+        trs[e] = trs[e] + difftr[e] * dt;
+    };
+    double vs[3][NHL2], ts[NHL2];
+    for (int itr = 0; itr < ntr; itr++)
+        for (int ib = 0; ib < o.nr; ib++) {
+            const int *m = o.g.maps + ib * NHL2;
+            for (int lev = 0; lev < nv; lev++) {
+                for (int k = 0; k < 3; k++)
+                    fill(vs[k], m, [&](int gi) { return Mh[(gi * nv + lev) * 3 + k] + Mhk[(gi * nv + lev) * 3 + k]; });
+                fill(ts, m, [&](int gi) { return mixr(gi, lev, itr); });
+                const double rscale = rscale_of(o, lev);
+                for (int y = 0; y < NL; y++)
+                    for (int x = 0; x < NL; x++) {
+                        Stencil st = stencil(m, x, y);
+                        double nflx = 0.0;
+                        for (int k = 0; k < 3; k++) {
+                            const double *d = div + st.id * 21;
+                            nflx += rscale * (d[0 * 3 + k] * vs[k][st.pt[0]] * ts[st.pt[0]] + d[1 * 3 + k] * vs[k][st.pt[1]] * ts[st.pt[1]]
+                                              + d[2 * 3 + k] * vs[k][st.pt[2]] * ts[st.pt[2]] + d[3 * 3 + k] * vs[k][st.pt[3]] * ts[st.pt[3]]
+                                              + d[4 * 3 + k] * vs[k][st.pt[4]] * ts[st.pt[4]] + d[5 * 3 + k] * vs[k][st.pt[5]] * ts[st.pt[5]]
+                                              + d[6 * 3 + k] * vs[k][st.pt[6]] * ts[st.pt[6]]);
+                        }
+                        column(st.id, lev, itr, nflx, ts[st.ir], false);
+                    }
+            }
+        }
+    for (int itr = 0; itr < ntr; itr++)
+        for (int id = P - 2; id < P; id++) {
+            int loc[6];
+            loc[0] = id;
+            for (int i = 0; i < 5; i++) loc[i + 1] = o.g.point_local[id * 6 + i];
+            for (int lev = 0; lev < nv; lev++) {
+                const double rscale = rscale_of(o, lev);
+                double nflx = 0.0;
+                for (int k = 0; k < 3; k++) {
+                    const double *d = div + id * 21;
+                    auto term = [&](int i) {
+                        const int q = loc[i];
+                        return d[i * 3 + k] * (Mh[(q * nv + lev) * 3 + k] + Mhk[(q * nv + lev) * 3 + k]) * mixr(q, lev, itr);
+                    };
+                    nflx += rscale * (term(0) + term(1) + term(2) + term(3) + term(4) + term(5));
+                }
+                column(id, lev, itr, nflx, mixr(id, lev, itr), true);
+            }
+        }
+}
+
 void zero(Oracle &o, const char *n) { std::fill(o.arr[n].begin(), o.arr[n].end(), 0.0); }
 void copy(Oracle &o, const char *dst, const char *src) { o.arr[dst] = o.arr[src]; }
 
@@ -1389,6 +1638,10 @@ void thor(Oracle &o) {
     zero(o, "Whs");
     zero(o, "Ws");
     zero(o, "pressures");
+    if (o.ntr) {  // dyn_core_loop_init of the tracer module (chemistry.cu:333-343)
+        copy(o, "tracerk", "tracer");
+        zero(o, "tracers");
+    }
     for (int rk = 0; rk < 3; rk++) {
         int ns_it = rk == 0 ? 1 : (rk == 1 ? ns_total / 2 : ns_total);
         double times = rk == 0 ? o.p.timestep / 3.0 : o.p.timestep / (double)ns_total;
@@ -1419,9 +1672,17 @@ void thor(Oracle &o) {
             divm_pass(o, "Mhk", "Whk", false);
             divm_pass(o, "Mhk", "Whk", true);
         }
+        if (o.ntr && o.p.HyDiff) {  // dyn_core_loop_slow_modes (chemistry.cu:345-427)
+            tracer_diffusion(o, false);
+            tracer_diffusion(o, true);
+        }
         if (o.sponge_on && !o.sponge_imp()) sponge_explicit(o, rk);
         slow_modes(o);
         if (rk > 0) update_rk(o);
+        if (rk > 0 && o.ntr) {  // the RK arrays at the start of stages 2 and 3
+            double *a = o.A_("tracers"), *ak = o.A_("tracerk"), *ai = o.A_("tracer");
+            for (size_t i = 0; i < o.arr["tracers"].size(); i++) a[i] = ai[i] - ak[i];
+        }
         for (int ns = 0; ns < ns_it; ns++) {
             zero(o, "DivM");
             zero(o, "divg_Mh");
@@ -1432,10 +1693,16 @@ void thor(Oracle &o) {
             momentum_eq(o, times);
             prepare_implicit_vertical(o);
             vertical_eq(o, times);
+            if (o.ntr) tracer_eq(o, times);  // dyn_core_loop_fast_modes (chemistry.cu:429-492)
             density_pressure(o, times);
         }
         update_rk2(o);
+        if (o.ntr) {  // the RK arrays at the end of every stage
+            double *a = o.A_("tracers"), *ak = o.A_("tracerk");
+            for (size_t i = 0; i < o.arr["tracers"].size(); i++) ak[i] += a[i];
+        }
     }
+    if (o.ntr) copy(o, "tracer", "tracerk");  // dyn_core_loop_end (chemistry.cu:495-502)
     copy(o, "Mh", "Mhk");
     copy(o, "Wh", "Whk");
     copy(o, "W", "Wk");
@@ -2062,6 +2329,12 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         o->capture_tri = a1 != 0;
         vertical_eq(*o, a0 * 1e-3);
         o->capture_tri = false;
+    } else if (n == "tracer_diffusion") {  // a0: laststep
+        if (!o->ntr) return -1;
+        tracer_diffusion(*o, a0 != 0);
+    } else if (n == "tracer_eq") {  // a0: the substep length in milliseconds
+        if (!o->ntr) return -1;
+        tracer_eq(*o, a0 * 1e-3);
     } else if (n == "temperature_h_pt_geff") {
         temperature_h_pt_geff(*o);
     } else if (n == "update_rk") {
@@ -2078,6 +2351,11 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         return -1;
     }
     return 0;
+}
+void oracle_set_tracers(oracle *o, int ntr) {
+    o->ntr = ntr;
+    const size_t n = (size_t)o->P * o->nv * (ntr > 0 ? ntr : 0);
+    for (const char *name : {"tracer", "tracerk", "tracers", "difftr", "trdiff"}) o->arr[name].assign(n, 0.0);
 }
 void oracle_steps(oracle *o, int n) {
     for (int i = 0; i < n; i++) {

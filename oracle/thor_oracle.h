@@ -69,10 +69,17 @@ void oracle_profx_held_suarez(oracle *o);
 //   "prepare_implicit_vertical" + _Poles          (thor_vertical_int.h:566-771)
 //   "density_pressure" Density_Pressure_Eqs + _Poles, a0 = substep length in ms
 //                                                 (thor_fastmodes.h:255-847)
+//   "tracer_diffusion" Tracer_Eq_Diffusion + _Poles, a0 = laststep; first pass
+//                  into "trdiff" [P][nv][ntr] (thor_chemistry.h:364-760)
+//   "tracer_eq"    Tracer_Eq + _Poles, a0 = substep length in ms (thor_chemistry.h:31-361)
 // Returns 0, or -1 for an unknown name. "Kdh4"/"Kdhz" are readable arrays.
 int oracle_kernel(oracle *o, const char *name, int a0, int a1);
 // ProfX + Thor, n times.
 void oracle_steps(oracle *o, int n);
+// Chemistry tracers (thor_chemistry.h, driven through the physics hooks as in
+// chemistry.cu:333-502): ntr tracer masses [P][nv][ntr] in the arrays
+// "tracer", "tracerk", "tracers", "difftr" (0 switches them off).
+void oracle_set_tracers(oracle *o, int ntr);
 // Forcings other than Held-Suarez (forcing codes as THOR_FORCING_* of the
 // library): 2 Newtonian relaxation with q = {T_mean, dT, lon_sub, p_top,
 // p_bottom, p_contrast, tau_top, tau_bottom}; 3 double-gray RT with q =

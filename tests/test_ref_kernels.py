@@ -92,3 +92,57 @@ def test_oracle_matches_thor_kernel(setup, name, a0, a1, outputs, tol):
         print(f"  {n}: max rel diff {err:.3e} (output changed by {changed:.3e})", flush=True)
         assert changed > 0, f"{name}: {n} not written by THOR's kernel"
         assert err <= tol, f"{name}: {n} differs from THOR's kernel by {err:.3e}"
+
+
+@pytest.mark.parametrize("deep", [1, 0])
+def test_tracer_kernels_match_thor(built, deep):
+    """The chemistry module's tracer kernels (thor_chemistry.h), two tracers:
+    the first diffusion pass (THOR writes it into diff_d), the last pass
+    (difftr; the pole kernel writes nothing there, which the oracle keeps) and
+    Tracer_Eq (tracers), from the same random tracers on an oracle state with
+    winds, bit for bit (deep: r^2 by pow on both sides, 1e-14)."""
+    from oracle import oracle as om
+    if not om.ref_available():
+        pytest.skip("THOR's kernels are not built (oracle/_ref/libthor_ref.so missing)")
+    capi = built
+    ntr = 2
+    g = grid(capi, 4, 16)
+    p = earth_params(capi, 4, dt=600.0, DeepModel=deep)
+    st = capi.ic_rest(g, p, seed=1008, temperature_noise=1e-3, wind_noise=1.0)
+    o = om.Oracle(g, p)
+    o.set_state(st)
+    o.steps(1)
+    P, nv = g.P, g.nv
+    rng = np.random.default_rng(9)
+    rho = o.get("Rhok").reshape(P, nv)
+    trk = (rho[:, :, None] * (1.0 + 0.2 * rng.uniform(-1, 1, (P, nv, ntr)))).ravel()
+    o.set_tracers(ntr)
+    ref = om.RefKernels(g, p)
+    shared = ["Rhok", "Rhos", "Mhs", "Mhk", "Whs", "Whk", "Kdh4"]
+    vals = {n: o.get(n) for n in shared}
+    vals["Rhos"] = rng.uniform(-1e-4, 1e-4, P * nv)
+    vals["Mhs"] = rng.uniform(-0.1, 0.1, 3 * P * nv)
+    vals["Whs"] = rng.uniform(-1e-3, 1e-3, P * (nv + 1))
+    for n, a in vals.items():
+        o.set(n, a)
+        ref.set(n, a)
+    o.set("tracerk", trk)
+    ref.set("tracerk", trk)
+    tol = 1e-14 if deep else 0.0
+
+    def cmp(label, got, want):
+        err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-300)
+        print(f"  {label}: max rel diff {err:.3e}", flush=True)
+        assert np.abs(want).max() > 0 and err <= tol, label
+
+    print(f"case tracers, DeepModel {deep}", flush=True)
+    o.kernel("tracer_diffusion", 0, 0)
+    ref.kernel("tracer_diffusion", 0, 0)
+    first = ref.get("diff").reshape(P, nv, 6)[:, :, :ntr].ravel()
+    cmp("first pass", o.get("trdiff"), first)
+    o.kernel("tracer_diffusion", 1, 0)
+    ref.kernel("tracer_diffusion", 1, 0)
+    cmp("difftr", o.get("difftr"), ref.get("difftr"))
+    o.kernel("tracer_eq", 100000, 0)
+    ref.kernel("tracer_eq", 100000, 0)
+    cmp("tracers", o.get("tracers"), ref.get("tracers"))

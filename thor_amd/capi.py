@@ -182,6 +182,12 @@ def lib(variant: str = "") -> C.CDLL:
             L.thor_dyn_time_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
             L.thor_dyn_set_sponge.argtypes = [C.c_void_p, C.POINTER(SpongeParams)]
             L.thor_dyn_set_hooks.argtypes = [C.c_void_p, C.POINTER(Hooks)]
+            L.thor_tracers_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
+            L.thor_tracers_destroy.argtypes = [C.c_void_p]
+            L.thor_tracers_destroy.restype = None
+            for f in (L.thor_tracers_set, L.thor_tracers_get):
+                f.argtypes = [C.c_void_p, _dp, C.c_long]
+            L.thor_tracers_hooks.argtypes = [C.c_void_p, C.POINTER(Hooks)]
             L.thor_dyn_register_rk_array.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
             L.thor_dyn_clear_rk_arrays.argtypes = [C.c_void_p]
             L.thor_dyn_set_newtonian.argtypes = [C.c_void_p, C.POINTER(NewtonianParams)]
@@ -455,6 +461,37 @@ class Dyn:
 
     def clear_rk_arrays(self):
         self._chk(self.L.thor_dyn_clear_rk_arrays(self._h))
+
+
+class Tracers:
+    """Passive tracers of THOR's chemistry module (thor_tracers_*): created on a
+    dynamical core, whose hooks it then occupies."""
+
+    def __init__(self, dyn: "Dyn", ntr: int):
+        self.dyn, self.ntr, self.L = dyn, ntr, dyn.L
+        h = C.c_void_p()
+        dyn._chk(self.L.thor_tracers_create(dyn._h, ntr, C.byref(h)))
+        self._h = h
+        self._hooks = Hooks()
+        dyn._chk(self.L.thor_tracers_hooks(h, C.byref(self._hooks)))
+        dyn._chk(self.L.thor_dyn_set_hooks(dyn._h, C.byref(self._hooks)))
+
+    def set(self, tracer: np.ndarray):
+        """tracer mass [P][nv][ntr] (THOR's tracer_d layout)"""
+        a = np.ascontiguousarray(tracer, dtype=np.float64)
+        self.dyn._chk(self.L.thor_tracers_set(self._h, _ptr(a), a.size))
+
+    def get(self) -> np.ndarray:
+        g = self.dyn.grid
+        a = np.zeros((g.P, g.nv, self.ntr))
+        self.dyn._chk(self.L.thor_tracers_get(self._h, _ptr(a), a.size))
+        return a
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.dyn._chk(self.L.thor_dyn_set_hooks(self.dyn._h, None))
+            self.L.thor_tracers_destroy(self._h)
+            self._h = None
 
 
 class MLP:

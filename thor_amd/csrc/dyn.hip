@@ -6,6 +6,7 @@
 #include "common.h"
 #include "dyn_kernels.h"
 #include "mlp_internal.h"
+#include "tracer_kernels.h"
 
 #include <hip/hip_runtime.h>
 
@@ -1361,4 +1362,165 @@ extern "C" thor_status thor_dyn_time_run(thor_dyn *h, int nsteps, int forcing, d
     *ms = f;
     drain_timing(*h);
     return rc;
+}
+
+// ------------------------------------------------------------- tracer module
+// THOR's chemistry module, its dynamical part (chemistry.cu:333-502): the
+// tracer mass in three level-planar arrays advanced by the RK scheme like the
+// state (tracer = y(n), tracerk = the stage's, tracers = the deviation), the
+// diffusion tendency and the first diffusion pass; four hook callbacks.
+struct thor_tracers {
+    Dyn *d = nullptr;
+    int ntr = 0;
+    double *tr = nullptr, *trk = nullptr, *trs = nullptr, *dif = nullptr, *scr = nullptr;
+    std::vector<void *> allocs;
+};
+
+namespace thor {
+namespace {
+TracerArgs tracer_args(thor_tracers *t, const thor_step_view *v) {
+    TracerArgs a{};
+    a.ntr = t->ntr;
+    a.trk = t->trk;
+    a.trs = t->trs;
+    a.dif = t->dif;
+    a.scr = t->scr;
+    a.dif_out = t->dif;
+    a.Rhok = v->Rhok;
+    a.Rhos = v->Rhos;
+    a.Mhk = v->Mhk;
+    a.Whs = v->Whs;
+    a.Whk = v->Whk;
+    a.Mhs = v->Mhs;
+    return a;
+}
+size_t tracer_bytes(const thor_tracers *t) { return (size_t)t->ntr * t->d->nv * t->d->S * sizeof(double); }
+
+// dyn_core_loop_init: tracerk = tracer, tracers = 0 (chemistry.cu:333-343)
+int tracer_init(void *user, void *stream, int, const thor_step_view *) {
+    auto *t = (thor_tracers *)user;
+    const hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(t->trk, t->tr, tracer_bytes(t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemsetAsync(t->trs, 0, tracer_bytes(t), s);
+    return e == hipSuccess ? 0 : 1;
+}
+// dyn_core_loop_slow_modes: the diffusion tendency of the stage (:345-427)
+int tracer_slow(void *user, void *stream, int, double, const thor_step_view *v) {
+    auto *t = (thor_tracers *)user;
+    Dyn &d = *t->d;
+    if (!d.prm.HyDiff) return 0;
+    const TracerArgs a = tracer_args(t, v);
+    const dim3 grid(8 * d.g.tiles_per_xcd * d.nv, t->ntr);
+    k_tracer_diff<false><<<grid, kTile, 0, (hipStream_t)stream>>>(d.g, a);
+    k_tracer_diff<true><<<grid, kTile, 0, (hipStream_t)stream>>>(d.g, a);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+// dyn_core_loop_fast_modes: Tracer_Eq of the substep (:429-492)
+int tracer_fast(void *user, void *stream, int, double times, const thor_step_view *v) {
+    auto *t = (thor_tracers *)user;
+    Dyn &d = *t->d;
+    const TracerArgs a = tracer_args(t, v);
+    const dim3 grid(8 * d.g.tiles_per_xcd * d.nv, t->ntr);
+    if (d.mixed) k_tracer_eq<true><<<grid, kTile, 0, (hipStream_t)stream>>>(d.g, a, times);
+    else k_tracer_eq<false><<<grid, kTile, 0, (hipStream_t)stream>>>(d.g, a, times);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+// dyn_core_loop_end: tracer = tracerk (:495-502)
+int tracer_end(void *user, void *stream, int, const thor_step_view *) {
+    auto *t = (thor_tracers *)user;
+    return hipMemcpyAsync(t->tr, t->trk, tracer_bytes(t), hipMemcpyDeviceToDevice, (hipStream_t)stream) == hipSuccess
+               ? 0 : 1;
+}
+} // namespace
+} // namespace thor
+
+extern "C" thor_status thor_tracers_create(thor_dyn *h, int ntr, thor_tracers **out) {
+    using namespace thor;
+    if (!h || !out) return fail(THOR_ERR_BAD_ARG, "thor_tracers_create: null argument");
+    if (ntr < 1 || ntr > 6)
+        return fail(THOR_ERR_BAD_ARG, "thor_tracers_create: 1 to 6 tracers (THOR's first diffusion pass keeps them in "
+                                      "its six-variable diffusion array)");
+    Dyn &d = *h;
+    if (d.rka.n + 1 > THOR_MAX_RK_ARRAYS) return fail(THOR_ERR_BAD_ARG, "thor_tracers_create: RK-array table full");
+    auto *t = new thor_tracers();
+    t->d = &d;
+    t->ntr = ntr;
+    const size_t n = (size_t)ntr * d.nv * d.S;
+    hipError_t e = hipSuccess;
+    for (double **p : {&t->tr, &t->trk, &t->trs, &t->dif, &t->scr}) {
+        if (e == hipSuccess) e = hipMalloc((void **)p, n * sizeof(double));
+        if (e == hipSuccess) {
+            t->allocs.push_back(*p);
+            e = hipMemset(*p, 0, n * sizeof(double));
+        }
+    }
+    if (e != hipSuccess) {
+        for (void *p : t->allocs) (void)hipFree(p);
+        delete t;
+        return fail(THOR_ERR_HIP, std::string("thor_tracers_create: ") + hipGetErrorString(e));
+    }
+    // advanced by the RK scheme (UpdateRK / UpdateRK2), the padding included
+    const int j = d.rka.n++;
+    d.rka.a[j] = t->trs;
+    d.rka.ak[j] = t->trk;
+    d.rka.ai[j] = t->tr;
+    d.rka.count[j] = (long)n;
+    *out = t;
+    return THOR_OK;
+}
+
+extern "C" void thor_tracers_destroy(thor_tracers *t) {
+    if (!t) return;
+    Dyn &d = *t->d;
+    (void)hipStreamSynchronize(d.stream);
+    for (int j = 0; j < d.rka.n; j++)
+        if (d.rka.a[j] == t->trs) {
+            for (int k = j; k + 1 < d.rka.n; k++) {
+                d.rka.a[k] = d.rka.a[k + 1];
+                d.rka.ak[k] = d.rka.ak[k + 1];
+                d.rka.ai[k] = d.rka.ai[k + 1];
+                d.rka.count[k] = d.rka.count[k + 1];
+            }
+            d.rka.n--;
+            break;
+        }
+    for (void *p : t->allocs) (void)hipFree(p);
+    delete t;
+}
+
+static thor_status tracer_io(thor_tracers *t, double *host, long n, bool to_device) {
+    using namespace thor;
+    if (!t || !host) return fail(THOR_ERR_BAD_ARG, "thor_tracers: null argument");
+    Dyn &d = *t->d;
+    if (n != (long)d.P * d.nv * t->ntr) return fail(THOR_ERR_BAD_ARG, "thor_tracers: n must be P * nv * ntr");
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    // THOR's layout [P][nv][ntr] <-> [ntr][nv][S]
+    std::vector<double> tmp((size_t)t->ntr * d.nv * d.S, 0.0);
+    if (!to_device) HIP_TRY(hipMemcpy(tmp.data(), t->tr, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int p = 0; p < d.P; p++)
+        for (int lev = 0; lev < d.nv; lev++)
+            for (int i = 0; i < t->ntr; i++) {
+                double &dev = tmp[((size_t)i * d.nv + lev) * d.S + p];
+                double &usr = host[((size_t)p * d.nv + lev) * t->ntr + i];
+                if (to_device) dev = usr;
+                else usr = dev;
+            }
+    if (to_device) HIP_TRY(hipMemcpy(t->tr, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_tracers_set(thor_tracers *t, const double *tracer, long n) {
+    return tracer_io(t, const_cast<double *>(tracer), n, true);
+}
+extern "C" thor_status thor_tracers_get(thor_tracers *t, double *tracer, long n) { return tracer_io(t, tracer, n, false); }
+
+extern "C" thor_status thor_tracers_hooks(thor_tracers *t, thor_hooks *hooks) {
+    using namespace thor;
+    if (!t || !hooks) return fail(THOR_ERR_BAD_ARG, "thor_tracers_hooks: null argument");
+    hooks->user = t;
+    hooks->dyn_core_loop_init = tracer_init;
+    hooks->dyn_core_loop_slow_modes = tracer_slow;
+    hooks->dyn_core_loop_fast_modes = tracer_fast;
+    hooks->dyn_core_loop_end = tracer_end;
+    return THOR_OK;
 }
