@@ -89,6 +89,8 @@ void thor_grid_destroy(thor_grid *g);
 /* ------------------------------------------------------------ parameters */
 
 enum { THOR_PRECISION_FP64 = 0, THOR_PRECISION_MIXED = 1 };
+/* thor_params.thermo_equation (thermo_equation_types, define.h:171) */
+enum { THOR_THERMO_ENTROPY = 0, THOR_THERMO_ENERGY = 1 };
 
 /* Scalars of SimulationSetup (src/headers/simulation_setup.h:54-119) and of
  * the ESP object that the dynamics step reads, plus the acoustic substep count
@@ -122,6 +124,13 @@ typedef struct thor_params {
      * HyDiffOrder - 4 to HyDiffOrder and >= 2. */
     int DiffSponge, order_diff_sponge;
     double Dv_sponge, ns_diff_sponge;
+    /* Thermodynamic equation of the fast modes (thermo_equation, esp.cu:441-443,
+     * 696-703): THOR_THERMO_ENTROPY (potential temperature, THOR's default) or
+     * THOR_THERMO_ENERGY (total energy; FP64 precision only, else
+     * THOR_ERR_UNSUPPORTED). With the energy equation the pt and pth arrays of
+     * the physics-hook view hold the flux variable h + ep + ek of the
+     * density/pressure update. */
+    int thermo_equation;
 } thor_params;
 
 /* Fills p with THOR's defaults for the Earth Held-Suarez case (ifile/earth_hstest.thr). */
@@ -271,8 +280,9 @@ thor_status thor_dyn_globdiag(thor_dyn *h, double out[5]);
 
 /* Copies a named field to / from the host in THOR's layout. Names: the state
  * ("Mh","Wh","W","Rho","pressure"), diagnostics of the last step
- * ("temperature","pt","h","hh","pth","gtil","gtilh"), forcing inputs
- * ("profx_Qheat","profx_dMh","profx_dWh"). n is the element count. */
+ * ("temperature","pt","h","hh","pth","gtil","gtilh"; "Etotal", the total
+ * energy of the last acoustic substep, with THOR_THERMO_ENERGY only), forcing
+ * inputs ("profx_Qheat","profx_dMh","profx_dWh"). n is the element count. */
 thor_status thor_dyn_get_field(thor_dyn *h, const char *name, double *dst, long n);
 thor_status thor_dyn_set_field(thor_dyn *h, const char *name, const double *src, long n);
 

@@ -40,7 +40,8 @@ class _Params(C.Structure):
                 ("n_substeps", C.c_int), ("DeepModel", C.c_int), ("NonHydro", C.c_int), ("HyDiff", C.c_int),
                 ("DivDampP", C.c_int), ("GravHeightVar", C.c_int), ("VertHyDiff", C.c_int),
                 ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double), ("DiffSponge", C.c_int),
-                ("order_diff_sponge", C.c_int), ("Dv_sponge", C.c_double), ("ns_diff_sponge", C.c_double)]
+                ("order_diff_sponge", C.c_int), ("Dv_sponge", C.c_double), ("ns_diff_sponge", C.c_double),
+                ("energy_equation", C.c_int)]
 
 
 _libs: dict = {}
@@ -107,7 +108,8 @@ def _structs(grid, params):
                 params.Diffc, params.DivDampc, params.timestep, params.glevel, params.HyDiffOrder,
                 params.n_substeps, params.DeepModel, params.NonHydro, params.HyDiff, params.DivDampP,
                 params.GravHeightVar, params.VertHyDiff, params.VertHyDiffOrder, params.Diffc_v,
-                params.DiffSponge, params.order_diff_sponge, params.Dv_sponge, params.ns_diff_sponge)
+                params.DiffSponge, params.order_diff_sponge, params.Dv_sponge, params.ns_diff_sponge,
+                params.thermo_equation)
     return g, p
 
 

@@ -168,13 +168,13 @@ int ref_kernel(void *h, const char *name, int a0, int a1) {
                                           r["areasTr"], r["areas"], r["nvecoa"], r["nvecti"], r["nvecte"],
                                           r["func_r"], r["Kdh4"], r["Altitude"], A, r["Rd"], r["Cp"], r.maps, 16,
                                           stepnum, last, deep, r.p.DiffSponge != 0, r.p.order_diff_sponge,
-                                          r["Kdh2"], r["boundary_flux"], false, r.p.HyDiffOrder);
+                                          r["Kdh2"], r["boundary_flux"], r.p.energy_equation != 0, r.p.HyDiffOrder);
         Diffusion_Op_Poles<5><<<NBDP, 1>>>(r["diffmh"], r["diffw"], r["diffrh"], r["diffpr"], r["diff"],
                                            r["diff_sponge"], r["Mhk"], r["Rhok"], r["temperature"], r["Wk"],
                                            r["func_r"], r["areasTr"], r["areas"], r["nvecoa"], r["nvecti"],
                                            r["nvecte"], r["Kdh4"], r["Altitude"], r["Altitudeh"], A, r["Rd"],
                                            r["Cp"], r.point_local, P, stepnum, last, deep, r.p.DiffSponge != 0,
-                                           r.p.order_diff_sponge, r["Kdh2"], r["boundary_flux"], false,
+                                           r.p.order_diff_sponge, r["Kdh2"], r["boundary_flux"], r.p.energy_equation != 0,
                                            r.p.HyDiffOrder);
     } else if (n == "vertical_diff") {
         const dim3 NBALL0(P / NTH + 1, 1, 6);  // thor_driver.cu:86
@@ -197,13 +197,13 @@ int ref_kernel(void *h, const char *name, int a0, int a1) {
             r["DivM"], r["diffmh"], r["diffw"], r["diffrh"], r["diffpr"], r["diffmv"], r["diffwv"], r["diffrv"],
             r["diffprv"], r["pressurek"], r["h"], r["hh"], r["gtil"], r["grad"], r["div"], r["Altitude"],
             r["Altitudeh"], A, r.p.Gravit, r["Cp"], r["Rd"], r["func_r"], r.maps, 16, deep, nonhydro,
-            r["profx_dMh"], r["profx_dWh"], r["profx_Qheat"], false, ghv);
+            r["profx_dMh"], r["profx_dWh"], r["profx_Qheat"], r.p.energy_equation != 0, ghv);
         Compute_Slow_Modes_Poles<6><<<2, 1>>>(
             r["SlowMh"], r["SlowWh"], r["SlowRho"], r["Slowpressure"], r["Mhk"], r["Whk"], r["Rhok"], r["Adv"],
             r["DivM"], r["diffmh"], r["diffw"], r["diffrh"], r["diffpr"], r["diffmv"], r["diffwv"], r["diffrv"],
             r["diffprv"], r["pressurek"], r["h"], r["hh"], r["gtil"], r["grad"], r["div"], r["Altitude"],
             r["Altitudeh"], A, r.p.Gravit, r["Cp"], r["Rd"], r["func_r"], r.point_local, nv, P, deep, nonhydro,
-            r["profx_dMh"], r["profx_dWh"], r["profx_Qheat"], false, ghv);
+            r["profx_dMh"], r["profx_dWh"], r["profx_Qheat"], r.p.energy_equation != 0, ghv);
     } else if (n == "tracer_diffusion") {  // a0: laststep (chemistry.cu:345-427)
         const dim3 NBTR(r.nr, nv, kRefTracers), NBTRP(2, nv, kRefTracers);
         Tracer_Eq_Diffusion<16, 16><<<NBTR, NT>>>(r["difftr"], r["diff"], r["tracerk"], r["Rhok"], r["areasTr"],

@@ -287,6 +287,11 @@ void temperature_h_pt_geff(Oracle &o) {
     double *T = o.A_("temperature"), *pk = o.A_("pressurek"), *Rho = o.A_("Rhok"), *h_d = o.A_("h"),
            *hh = o.A_("hh"), *pt_d = o.A_("pt"), *pth = o.A_("pth"), *gtil = o.A_("gtil"),
            *gtilh_d = o.A_("gtilh"), *Wh = o.A_("Whk"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+    // energy equation (thor_auxiliary.h:143-161, 212-230, 281-287): potential
+    // and kinetic energy at centres and interfaces, the stage's total energy
+    const bool energy = o.p.energy_equation;
+    double *ep_d = o.A_("epotential"), *ek_d = o.A_("ekinetic"), *Et_d = o.A_("Etotal"),
+           *eph_d = o.A_("epotentialh"), *ekh_d = o.A_("ekinetich"), *Mk = o.A_("Mhk"), *Wk = o.A_("Wk");
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     const double P_Ref = o.p.P_Ref, Gravit = o.p.Gravit, A = o.p.A;
     const bool ghv = o.p.GravHeightVar;
@@ -294,13 +299,15 @@ void temperature_h_pt_geff(Oracle &o) {
         double temperature, rho = 0, rhoh, rhol = 0, rhop, pressure = 0, h = 0, hl = 0, pt = 0, ptl = 0,
                                      pl = 0, pp, dpdz;
         double gtilh = 0, gtilht, dz, dp_dz, Cv, CvoCp, xi, xim, xip, intt, intl, extr;
-        double alt = 0, altl = 0, alth = 0, altht = 0;
+        double alt = 0, altl = 0, alth = 0, altht = 0, ep = 0, epl = 0, ek = 0, ekl = 0;
         for (int lev = 0; lev < nv + 1; lev++) {
             if (lev < nv) {
                 Cv = Cp[id * nv + lev] - Rd[id * nv + lev];
                 CvoCp = Cv / Cp[id * nv + lev];
                 if (lev > 0) {
-                    pl = pressure; rhol = rho; hl = h; ptl = pt; altl = alt;
+                    pl = pressure; rhol = rho; hl = h; altl = alt;
+                    if (energy) { epl = ep; ekl = ek; }
+                    else ptl = pt;
                 }
                 pressure = pk[id * nv + lev];
                 rho = Rho[id * nv + lev];
@@ -311,8 +318,20 @@ void temperature_h_pt_geff(Oracle &o) {
                 altht = Alth[lev + 1];
                 h = Cv * temperature + pressure / rho;
                 h_d[id * nv + lev] = h;
-                pt = (P_Ref / (Rd[id * nv + lev] * rho)) * MPOW(pressure / P_Ref, CvoCp);
-                pt_d[id * nv + lev] = pt;
+                if (energy) {
+                    if (ghv) ep = Gravit * MPOW(A / (A + Alt[lev]), 2) * Alt[lev];
+                    else ep = Gravit * Alt[lev];
+                    ep_d[id * nv + lev] = ep;
+                    const long e = (long)id * nv + lev;
+                    ek = 0.5 * (Mk[e * 3 + 0] * Mk[e * 3 + 0] + Mk[e * 3 + 1] * Mk[e * 3 + 1] + Mk[e * 3 + 2] * Mk[e * 3 + 2]
+                                + Wk[e] * Wk[e])
+                         / (Rho[e] * Rho[e]);
+                    ek_d[e] = ek;
+                    Et_d[e] = Rho[e] * (ep + ek) + Cv / Rd[e] * pressure;
+                } else {
+                    pt = (P_Ref / (Rd[id * nv + lev] * rho)) * MPOW(pressure / P_Ref, CvoCp);
+                    pt_d[id * nv + lev] = pt;
+                }
             }
             if (lev == 0) {
                 extr = (-alt - Alt[lev + 1]) / (Alt[lev + 1] - alt);
@@ -340,11 +359,26 @@ void temperature_h_pt_geff(Oracle &o) {
                 if (pp < 0) pp = 0;
                 rhop = Rho[id * nv + nv - 2] + (Rho[id * nv + nv - 1] - Rho[id * nv + nv - 2]) * extr;
                 rhoh = 0.5 * (rho + rhop);
-                pth[id * (nv + 1) + nv] =
-                    pt_d[id * nv + nv - 2] + (pt_d[id * nv + nv - 1] - pt_d[id * nv + nv - 2]) * extr;
-                pth[id * (nv + 1) + 0] = pt_d[id * nv + 1]
-                                         + (pt_d[id * nv + 1] - pt_d[id * nv + 0]) * (Alth[0] - Alt[1])
-                                               / (Alt[1] - Alt[0]);
+                if (energy) {
+                    if (ghv) {
+                        eph_d[id * (nv + 1) + nv] = Gravit * MPOW(A / (A + Alth[nv]), 2) * Alth[nv];
+                        eph_d[id * (nv + 1) + 0] = Gravit * MPOW(A / (A + Alth[0]), 2) * Alth[0];
+                    } else {
+                        eph_d[id * (nv + 1) + nv] = Gravit * Alth[nv];
+                        eph_d[id * (nv + 1) + 0] = Gravit * Alth[0];
+                    }
+                    ekh_d[id * (nv + 1) + nv] =
+                        ek_d[id * nv + nv - 2] + (ek_d[id * nv + nv - 1] - ek_d[id * nv + nv - 2]) * extr;
+                    ekh_d[id * (nv + 1) + 0] = ek_d[id * nv + 1]
+                                               + (ek_d[id * nv + 1] - ek_d[id * nv + 0]) * (Alth[0] - Alt[1])
+                                                     / (Alt[1] - Alt[0]);
+                } else {
+                    pth[id * (nv + 1) + nv] =
+                        pt_d[id * nv + nv - 2] + (pt_d[id * nv + nv - 1] - pt_d[id * nv + nv - 2]) * extr;
+                    pth[id * (nv + 1) + 0] = pt_d[id * nv + 1]
+                                             + (pt_d[id * nv + 1] - pt_d[id * nv + 0]) * (Alth[0] - Alt[1])
+                                                   / (Alt[1] - Alt[0]);
+                }
                 dz = 2.0 * (Alth[nv] - Alt[nv - 1]);
                 if (ghv)
                     pp = pk[id * nv + nv - 2]
@@ -367,7 +401,12 @@ void temperature_h_pt_geff(Oracle &o) {
                 intt = (xi - xip) / (xim - xip);
                 intl = (xi - xim) / (xip - xim);
                 hh[id * (nv + 1) + lev] = hl * intt + h * intl;
-                pth[id * (nv + 1) + lev] = ptl * intt + pt * intl;
+                if (energy) {
+                    eph_d[id * (nv + 1) + lev] = epl * intt + ep * intl;
+                    ekh_d[id * (nv + 1) + lev] = ekl * intt + ek * intl;
+                } else {
+                    pth[id * (nv + 1) + lev] = ptl * intt + pt * intl;
+                }
                 rhoh = rhol * intt + rho * intl;
                 gtilht = -(Wh[id * (nv + 1) + lev] / rhoh) * dp_dz;
                 xi = alt; xim = alth; xip = altht;
@@ -448,6 +487,7 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
             if (var == 0) a = rho;
             else if (var <= 3) a = Mh[(gi * nv + lev) * 3 + var - 1];
             else if (var == 4) a = W[gi * nv + lev];
+            else if (o.p.energy_equation) a = (o.A_("Cp")[gi * nv + lev] - Rd[gi * nv + lev]) * T[gi * nv + lev];
             else a = Rd[gi * nv + lev] * T[gi * nv + lev];
             if (var >= 1 && var <= 4) a = a / rho;
             return a;
@@ -876,7 +916,10 @@ void slow_modes(Oracle &o) {
                      + (nflxv[2] - dpr * fz) * M0[2] * r;
         Slowp[id * nv + lev] = (Rd[id * nv + lev] / Cv)
                                * (-nflxp - gtil[id * nv + lev] - dwhdz + vgp + Qh[id * nv + lev]);
-        Slowp[id * nv + lev] += diffpr[id * nv + lev] + diffprv[id * nv + lev];
+        if (o.p.energy_equation)  // thor_slowmodes.h:375-381
+            Slowp[id * nv + lev] += Rd[id * nv + lev] / Cv * (diffpr[id * nv + lev] + diffprv[id * nv + lev]);
+        else
+            Slowp[id * nv + lev] += diffpr[id * nv + lev] + diffprv[id * nv + lev];
     };
     for (int ib = 0; ib < o.nr; ib++) {
         const int *m = o.g.maps + ib * NHL2;
@@ -1275,15 +1318,23 @@ void vertical_eq(Oracle &o, double deltat) {
     }
 }
 
-// Density_Pressure_Eqs + _Poles, entropy form (thor_fastmodes.h:255-847).
+// Density_Pressure_Eqs + _Poles (thor_fastmodes.h:255-847), entropy and
+// total-energy forms. The energy form carries h + ep + ek as the flux
+// variable (centres: thor_fastmodes.h:352-372, interfaces: 459-476) and
+// recovers the pressure from the total energy (501-533).
 void density_pressure(Oracle &o, double dt) {
     const int nv = o.nv, P = o.P;
     double *ps = o.A_("pressures"), *pk = o.A_("pressurek"), *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"),
            *Ms = o.A_("Mhs"), *Mk = o.A_("Mhk"), *Whs = o.A_("Whs"), *Whk = o.A_("Whk"), *pt = o.A_("pt"),
            *pth = o.A_("pth"), *SlowRho = o.A_("SlowRho"), *Qh = o.A_("profx_Qheat"), *diffpr = o.A_("diffpr"),
            *diffprv = o.A_("diffprv"), *Cp = o.A_("Cp"), *Rd = o.A_("Rd");
+    const bool energy = o.p.energy_equation, ghv = o.p.GravHeightVar;
+    double *h = o.A_("h"), *hh = o.A_("hh"), *ep = o.A_("epotential"), *ek = o.A_("ekinetic"),
+           *eph = o.A_("epotentialh"), *ekh = o.A_("ekinetich"), *Et = o.A_("Etotal");
     const double *div = o.g.div, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
-    const double A = o.p.A, P_Ref = o.p.P_Ref;
+    const double A = o.p.A, P_Ref = o.p.P_Ref, Gravit = o.p.Gravit;
+    auto flux_c = [&](long e) { return energy ? (h[e] + ep[e] + ek[e]) : pt[e]; };
+    auto flux_h = [&](long e) { return energy ? (hh[e] + eph[e] + ekh[e]) : pth[e]; };
     auto body = [&](int id, int lev, int nslots, bool pole, auto vv, auto v1v, auto ptv) {
         double Cv = Cp[id * nv + lev] - Rd[id * nv + lev];
         double altht = Alth[lev + 1], althl = Alth[lev], dz = altht - althl;
@@ -1318,14 +1369,14 @@ void density_pressure(Oracle &o, double dt) {
             whl2 = 0.0;
             wht2 = Whs[id * (nv + 1) + lev + 1] + Whk[id * (nv + 1) + lev + 1];
             phl = 0.0;
-            pht = pth[id * (nv + 1) + lev + 1];
+            pht = flux_h(id * (nv + 1) + lev + 1);
         } else {
             whl = Whs[id * (nv + 1) + lev];
             wht = Whs[id * (nv + 1) + lev + 1];
             whl2 = Whs[id * (nv + 1) + lev] + Whk[id * (nv + 1) + lev];
             wht2 = Whs[id * (nv + 1) + lev + 1] + Whk[id * (nv + 1) + lev + 1];
-            phl = pth[id * (nv + 1) + lev];
-            pht = pth[id * (nv + 1) + lev + 1];
+            phl = flux_h(id * (nv + 1) + lev);
+            pht = flux_h(id * (nv + 1) + lev + 1);
         }
         double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
         double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
@@ -1333,6 +1384,18 @@ void density_pressure(Oracle &o, double dt) {
         double r = Rk[id * nv + lev] + Rs[id * nv + lev];
         nflxr += dwdz;
         Rs[id * nv + lev] += (SlowRho[id * nv + lev] - nflxr) * dt;
+        if (energy) {
+            const long e = (long)id * nv + lev;
+            aux += Et[e] + (diffpr[e] + diffprv[e]) * dt + Qh[e] * dt;
+            Et[e] = aux;
+            double w = (whl2 * (Alth[lev + 1] - Alt[lev]) + wht2 * (Alt[lev] - Alth[lev])) / (Alth[lev + 1] - Alth[lev]);
+            double rn = Rs[e] + Rk[e];
+            double epot = ghv ? rn * Gravit * MPOW(A / (A + Alt[lev]), 2) * Alt[lev] : rn * Gravit * Alt[lev];
+            double v0 = v1v(0, 0), v1 = v1v(0, 1), v2 = v1v(0, 2);
+            double epk = epot + 0.5 * (MPOW(v0, 2) + MPOW(v1, 2) + MPOW(v2, 2) + MPOW(w, 2)) / rn;
+            ps[e] = Rd[e] / Cv * (aux - epk) - pk[e];
+            return;
+        }
         double ptc = (P_Ref / (Rd[id * nv + lev] * r))
                      * MPOW((ps[id * nv + lev] + pk[id * nv + lev]) / P_Ref, Cv / Cp[id * nv + lev]);
         aux += ptc * r;
@@ -1354,7 +1417,7 @@ void density_pressure(Oracle &o, double dt) {
                         vs[k][pos] = v1s[k][pos] = 0.0;
                     }
                 }
-                pts[pos] = gi >= 0 ? pt[gi * nv + lev] : 0.0;
+                pts[pos] = gi >= 0 ? flux_c((long)gi * nv + lev) : 0.0;
             }
             for (int y = 0; y < NL; y++)
                 for (int x = 0; x < NL; x++) {
@@ -1371,7 +1434,7 @@ void density_pressure(Oracle &o, double dt) {
         for (int lev = 0; lev < nv; lev++)
             body(id, lev, 6, true, [&](int j, int k) { return Ms[(gs[j] * nv + lev) * 3 + k]; },
                  [&](int j, int k) { return Ms[(gs[j] * nv + lev) * 3 + k] + Mk[(gs[j] * nv + lev) * 3 + k]; },
-                 [&](int j) { return pt[gs[j] * nv + lev]; });
+                 [&](int j) { return flux_c((long)gs[j] * nv + lev); });
     }
 }
 
@@ -2260,8 +2323,9 @@ oracle *oracle_create(const oracle_grid *g, const oracle_params *p) {
     const long N = (long)o->P * o->nv, Ni = (long)o->P * o->nvi;
     const char *sc[] = {"W", "Rho", "pressure", "temperature", "Rd", "Cp", "profx_Qheat", "Wk", "Rhok",
                         "pressurek", "Ws", "Rhos", "pressures", "h", "pt", "gtil", "SlowRho", "Slowpressure",
-                        "diffw", "diffrh", "diffpr", "Sp", "Sd", "diffwv", "diffrv", "diffprv"};
-    const char *it[] = {"Wh", "profx_dWh", "Whk", "Whs", "hh", "pth", "gtilh", "SlowWh"};
+                        "diffw", "diffrh", "diffpr", "Sp", "Sd", "diffwv", "diffrv", "diffprv",
+                        "epotential", "ekinetic", "Etotal"};
+    const char *it[] = {"Wh", "profx_dWh", "Whk", "Whs", "hh", "pth", "gtilh", "SlowWh", "epotentialh", "ekinetich"};
     const char *vc[] = {"Mh", "profx_dMh", "Mhk", "Mhs", "Adv", "v", "SlowMh", "diffmh", "diffmv", "DivM", "divg_Mh"};
     for (auto n : sc) o->arr[n].assign(N, 0.0);
     for (auto n : it) o->arr[n].assign(Ni, 0.0);

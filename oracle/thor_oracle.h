@@ -32,6 +32,7 @@ typedef struct oracle_params {
     double Diffc_v;
     int DiffSponge, order_diff_sponge;
     double Dv_sponge, ns_diff_sponge;
+    int energy_equation;  // thermo_equation = energy
 } oracle_params;
 
 typedef struct oracle oracle;

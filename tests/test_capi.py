@@ -66,3 +66,16 @@ def test_unsupported_hyperdiffusion_order_is_rejected(built, order):
         built.Dyn(g, p)
     assert e.value.code == built.THOR_ERR_UNSUPPORTED
     assert "HyDiffOrder" in str(e.value)
+
+
+@pytest.mark.parametrize("thermo,precision,code", [(1, 1, "THOR_ERR_UNSUPPORTED"), (2, 0, "THOR_ERR_BAD_ARG")])
+def test_thermo_equation_checks(built, thermo, precision, code):
+    """The energy equation exists in the FP64 form only; thermo_equation takes
+    0 (entropy) or 1 (energy). Both are refused before any device call."""
+    from tests.common import earth_params, grid
+    g = grid(built, 4, 8)
+    p = earth_params(built, 4, thermo_equation=thermo, precision=precision)
+    with pytest.raises(built.ThorError) as e:
+        built.Dyn(g, p)
+    assert e.value.code == getattr(built, code)
+    assert ("energy" if thermo == 1 else "thermo_equation") in str(e.value)

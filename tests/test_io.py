@@ -62,6 +62,7 @@ def test_config_values_defaults_and_unknown_keys(io, built, tmp_path):
     assert (p.A, p.Omega, p.Gravit, p.Rd, p.Cp, p.P_Ref) == (6371000.0, 7.292e-5, 9.8, 287.0, 1005.0, 1e5)
     assert (p.glevel, rc.vlevel, rc.top_altitude, p.timestep, rc.num_steps, rc.n_out) == (4, 32, 36000.0, 1000, 4, 2)
     assert p.HyDiffOrder == 4 and p.n_substeps == 6 and p.precision == 0  # defaults (define.h, thor_driver.cu)
+    assert p.thermo_equation == built.THERMO_ENTROPY
     assert rc.forcing == built.FORCING_HELD_SUAREZ and rc.globdiag == 1 and rc.rest == 1
     assert rc.simulation_ID == b"Earth"
     # unknown keys are reported, not fatal; malformed lines and unsupported values are
@@ -71,8 +72,13 @@ def test_config_values_defaults_and_unknown_keys(io, built, tmp_path):
     bad.write_text("glevel = 5\nthis line has no equals sign\n")
     with pytest.raises(built.ThorError, match="line 2"):
         io.Config(str(bad))
-    for text, word in (("thermo_equation = energy\n", "thermo_equation"), ("core_benchmark = JetSteady\n",
-                                                                            "JetSteady"),
+    # thermo_equation (esp.cu:696-710): entropy, energy or an error
+    bad.write_text("thermo_equation = energy\n")
+    ce = io.Config(str(bad))
+    io._check(io.lib().thor_config_run(ce._h, C.byref(rc)))
+    assert rc.params.thermo_equation == built.THERMO_ENERGY
+    for text, word in (("thermo_equation = enthalpy\n", "thermo_equation"), ("core_benchmark = JetSteady\n",
+                                                                             "JetSteady"),
                        ("vert_refined = true\n", "vert_refined"), ("glevel = four\n", "glevel")):
         bad.write_text(text)
         cb = io.Config(str(bad))

@@ -469,3 +469,172 @@ def test_update_rk_stage_deviations(g4):
     assert np.array_equal(whs[:, 1:nv], wh[:, 1:nv] - whk[:, 1:nv])
     ws = get("Ws", nv)
     assert np.abs(ws[:, 1:nv - 1] - (a + b * z)[:, 1:nv - 1]).max() < 1e-12
+
+
+# ------------------------------------------------------- energy equation
+# thermo_equation = energy (esp.cu:696-710): Compute_Temperature_H_Pt_Geff
+# forms the potential, kinetic and total energy (thor_auxiliary.h:143-161,
+# 212-230, 281-287), the hyperdiffusion's fifth field becomes the internal
+# energy (Cp - Rd) T (thor_diff.h:196-202), the slow pressure tendency takes
+# its diffusion times Rd / Cv (thor_slowmodes.h:375-381), and the acoustic
+# substeps advance the total energy with the flux of h + ep + ek and recover
+# the pressure from it (thor_fastmodes.h:352-533).
+
+def _energy_oracle(capi, g, **kw):
+    return _oracle(capi, g, thermo_equation=capi.THERMO_ENERGY, **kw)
+
+
+@pytest.mark.parametrize("ghv", [0, 1])
+def test_energy_column_values(g4, ghv):
+    """Total energy density rho g z + rho |u|^2 / 2 + rho Cv T; the flux
+    variable h + ep + ek = Cp T + g z + |u|^2 / 2 at the centres; at the
+    interfaces, for T and |u|^2 linear in height (g z linear too without
+    GravHeightVar), the exact interface values: hh takes the end layers'
+    values, ep is g z at the ground and the model top, and ek is extrapolated
+    to the ground and to the ghost point 2 Altitudeh[nv] - Altitude[nv-1]."""
+    capi, g = g4
+    p, o = _energy_oracle(capi, g, GravHeightVar=ghv)
+    P, nv = g.P, g.nv
+    rng = np.random.default_rng(31)
+    z, zh = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    rho0 = 1.0 + 0.2 * rng.uniform(0, 1, (P, 1))
+    a_t, b_t = 290 + 10 * rng.uniform(0, 1, (P, 1)), -6e-3 * (1 + 0.2 * rng.uniform(-1, 1, (P, 1)))
+    a_u, b_u = 100 + 50 * rng.uniform(0, 1, (P, 1)), 1e-2 * rng.uniform(0, 1, (P, 1))  # u^2 = a + b z
+    T = a_t + b_t * z
+    u = np.sqrt(a_u + b_u * z)
+    rho = np.broadcast_to(rho0, (P, nv)).copy()
+    # u split over the three components, and a vertical part w with u^2 = |uh|^2 + w^2
+    c = rng.uniform(0.2, 1.0, (P, 1, 4))
+    c /= np.sqrt((c ** 2).sum(axis=2, keepdims=True))
+    mh = rho[:, :, None] * u[:, :, None] * c[:, :, :3]
+    wk = rho * u * c[:, :, 3]
+    o.set("pressurek", rho * p.Rd * T)
+    o.set("Rhok", rho)
+    o.set("Mhk", mh)
+    o.set("Wk", wk)
+    o.set("Whk", np.zeros((P, nv + 1)))
+    o.kernel("temperature_h_pt_geff", 0, 0)
+    get = lambda n, s: o.get(n).reshape(P, s)
+    Cv = p.Cp - p.Rd
+    gz = p.Gravit * (p.A / (p.A + z)) ** 2 * z if ghv else p.Gravit * z
+    gzh = p.Gravit * (p.A / (p.A + zh)) ** 2 * zh if ghv else p.Gravit * zh
+    E = rho * gz + 0.5 * rho * u ** 2 + rho * Cv * T
+    assert np.abs(get("Etotal", nv) / E - 1).max() < 1e-14
+    flux = get("h", nv) + get("epotential", nv) + get("ekinetic", nv)
+    assert np.abs(flux / (p.Cp * T + gz + 0.5 * u ** 2) - 1).max() < 1e-14
+    eph, ekh = get("epotentialh", nv + 1), get("ekinetich", nv + 1)
+    assert np.abs(eph[:, [0, nv]] - gzh[[0, nv]]).max() <= 1e-15 * gzh.max()
+    ek_want = 0.5 * (a_u + b_u * zh)
+    ek_want[:, nv] = (0.5 * (a_u + b_u * (2 * zh[nv] - z[nv - 1])))[:, 0]
+    assert np.abs(ekh / ek_want - 1).max() < 1e-12
+    if not ghv:
+        assert np.abs(eph[:, 1:nv] / gzh[1:nv] - 1).max() < 1e-13
+        hh = get("hh", nv + 1)
+        hh_want = p.Cp * (a_t + b_t * zh)
+        hh_want[:, 0], hh_want[:, nv] = p.Cp * T[:, 0], p.Cp * T[:, nv - 1]
+        assert np.abs(hh / hh_want - 1).max() < 1e-13
+
+
+@pytest.mark.parametrize("deep,ghv", [(1, 0), (0, 0), (1, 1)])
+def test_energy_density_pressure(g4, deep, ghv):
+    """One acoustic substep of the energy form through the point_local
+    neighbour table: E' = E + dt (diffpr + diffprv + Qheat)
+    - dt (rs div((M' + Mk)(h + ep + ek)) + d(W (hh + eph + ekh))/dz), and
+    p' = (Rd/Cv)(E' - rho' g z - |(M' + Mk, w)|^2 / (2 rho')) - pk with the
+    updated density and w the substep's vertical momentum at the centre."""
+    capi, g = g4
+    p, o = _energy_oracle(capi, g, DeepModel=deep, GravHeightVar=ghv)
+    rng = np.random.default_rng(32)
+    st = _dp_setup(o, g, p, rng)
+    P, nv = g.P, g.nv
+    z, zh = np.asarray(g.Altitude), np.asarray(g.Altitudeh)
+    fr = np.asarray(g.func_r)
+    ms = _tangential(rng.uniform(-1, 1, (P, nv, 3)), fr)
+    mk = _tangential(rng.uniform(-20, 20, (P, nv, 3)), fr)
+    whk = rng.uniform(-0.5, 0.5, (P, nv + 1))
+    whk[:, 0] = whk[:, nv] = 0.0
+    arr = {"Mhs": ms, "Mhk": mk, "Whk": whk,
+           "h": 3e5 + 1e4 * rng.uniform(0, 1, (P, nv)), "epotential": rng.uniform(0, 3e5, (P, nv)),
+           "ekinetic": rng.uniform(0, 300, (P, nv)), "hh": 3e5 + 1e4 * rng.uniform(0, 1, (P, nv + 1)),
+           "epotentialh": rng.uniform(0, 3e5, (P, nv + 1)), "ekinetich": rng.uniform(0, 300, (P, nv + 1))}
+    Cv = p.Cp - p.Rd
+    E0 = (st["Rhok"] + st["Rhos"]) * p.Gravit * z + Cv / p.Rd * (st["pressurek"] + st["pressures"])
+    arr["Etotal"] = E0
+    for k, v in arr.items():
+        o.set(k, v)
+    rs, ps = _dp_run(o, g)
+    E1 = o.get("Etotal").reshape(P, nv)
+    nb, div = _nbr(g), _weights(g, "div")
+    rsc = _rscale(g, p)[None, :]
+    flux_c = arr["h"] + arr["epotential"] + arr["ekinetic"]
+    flux_h = arr["hh"] + arr["epotentialh"] + arr["ekinetich"]
+    v1 = ms + mk
+    flx = rsc * _stencil_vec(div, nb, v1 * flux_c[:, :, None])
+    r2 = (lambda x: (x + p.A) ** 2) if deep else (lambda x: np.ones_like(x))
+    wl, wt = whk[:, :-1].copy(), whk[:, 1:]
+    phl, pht = flux_h[:, :-1].copy(), flux_h[:, 1:]
+    wl[:, 0] = phl[:, 0] = 0.0
+    dwpt = (wt * pht * r2(zh[1:]) - wl * phl * r2(zh[:-1])) / ((zh[1:] - zh[:-1]) * r2(z))
+    E_want = E0 + DT * (st["diffpr"] + st["diffprv"]) + DT * st["profx_Qheat"] - DT * (flx + dwpt)
+    e_E = float((np.abs(E1 - E_want) / E0).max())
+    want_r = st["Rhos"] + DT * (st["SlowRho"] - rsc * _stencil_vec(div, nb, ms))
+    rn = st["Rhok"] + want_r
+    w = 0.5 * (wl + wt)  # the centre is the middle of a uniform layer
+    gz = p.Gravit * (p.A / (p.A + z)) ** 2 * z if ghv else p.Gravit * z
+    ekin = 0.5 * ((v1 ** 2).sum(axis=2) + w ** 2) / rn
+    want_p = p.Rd / Cv * (E_want - rn * gz - ekin) - st["pressurek"]
+    ptot = st["pressurek"] + st["pressures"]
+    e_p = float((np.abs(ps - want_p) / ptot).max())
+    dE = float((np.abs(E_want - E0) / E0).max())
+    print(f"deep {deep} ghv {ghv}: E error {e_E:.2e} (substep changes E by up to {dE:.1e}), p' error {e_p:.2e} of p")
+    assert e_E < 1e-15 and e_p < 1e-13 and dE > 1e-6
+    assert float((np.abs(rs - want_r)).max()) < 1e-12 * float(np.abs(want_r).max())
+
+
+_COPY = ["Mh", "Wh", "W", "Rho", "pressure", "temperature", "profx_Qheat", "profx_dMh", "profx_dWh", "Mhk", "Whk",
+         "Wk", "Rhok", "pressurek", "h", "pt", "gtil", "hh", "pth", "gtilh", "Adv", "v", "diff", "diffmh", "diffw",
+         "diffrh", "diffpr", "DivM", "divg_Mh", "diffmv", "diffwv", "diffrv", "diffprv"]
+
+
+def _pair(capi, g):
+    """An entropy-form and an energy-form oracle holding the same arrays."""
+    _, oe = _oracle(capi, g)
+    p, on = _energy_oracle(capi, g)
+    for n in _COPY:
+        on.set(n, oe.get(n))
+    return p, oe, on
+
+
+def test_energy_diffusion_field_is_internal_energy(g4):
+    """The first hyperdiffusion pass is linear in its fields: with the energy
+    equation the fifth plane is the Laplacian of Cv T instead of Rd T, so it
+    equals the entropy form's times Cv / Rd; the other planes are unchanged."""
+    capi, g = g4
+    p, oe, on = _pair(capi, g)
+    oe.kernel("diffusion", 0, 0)
+    on.kernel("diffusion", 0, 0)
+    P, nv = g.P, g.nv
+    de = oe.get("diff").reshape(P, nv, 6)
+    dn = on.get("diff").reshape(P, nv, 6)
+    assert np.array_equal(de[:, :, :5], dn[:, :, :5])
+    ratio = (p.Cp - p.Rd) / p.Rd
+    scale = np.abs(dn[:, :, 5]).max()
+    assert scale > 0 and np.abs(dn[:, :, 5] - ratio * de[:, :, 5]).max() < 1e-13 * scale
+
+
+def test_energy_slow_pressure_scales_the_diffusion(g4):
+    """Slow pressure tendency: the energy form adds (Rd/Cv)(diffpr + diffprv)
+    where the entropy form adds diffpr + diffprv; every other slow tendency
+    is the same."""
+    capi, g = g4
+    p, oe, on = _pair(capi, g)
+    oe.kernel("slow_modes", 0, 0)
+    on.kernel("slow_modes", 0, 0)
+    for n in ("SlowMh", "SlowWh", "SlowRho"):
+        assert np.array_equal(oe.get(n), on.get(n)), n
+    d = oe.get("diffpr") + oe.get("diffprv")
+    want = (p.Rd / (p.Cp - p.Rd) - 1.0) * d
+    got = on.get("Slowpressure") - oe.get("Slowpressure")
+    scale = np.abs(oe.get("Slowpressure")).max()
+    assert np.abs(d).max() > 1e-6 * scale
+    assert np.abs(got - want).max() < 1e-13 * scale

@@ -94,6 +94,36 @@ def test_oracle_matches_thor_kernel(setup, name, a0, a1, outputs, tol):
         assert err <= tol, f"{name}: {n} differs from THOR's kernel by {err:.3e}"
 
 
+@pytest.fixture(scope="module")
+def setup_energy(built):
+    """The same state advanced by the total-energy form (thermo_equation =
+    energy), which switches the fifth hyperdiffused field to (Cp - Rd) T
+    (thor_diff.h:196-202) and scales its slow-mode pressure tendency by
+    Rd / Cv (thor_slowmodes.h:375-381)."""
+    from oracle import oracle as om
+    if not om.ref_available():
+        pytest.skip("THOR's kernels are not built (oracle/_ref/libthor_ref.so missing)")
+    capi = built
+    g = grid(capi, 4, 16)
+    p = earth_params(capi, 4, dt=600.0, VertHyDiff=1, VertHyDiffOrder=6, Diffc_v=2e-3, DiffSponge=1,
+                     order_diff_sponge=2, Dv_sponge=0.01, ns_diff_sponge=0.5, thermo_equation=capi.THERMO_ENERGY)
+    st = capi.ic_rest(g, p, seed=1001, temperature_noise=1e-3, wind_noise=1.0)
+    o = om.Oracle(g, p)
+    o.set_state(st)
+    o.steps(2)
+    ref = om.RefKernels(g, p)
+    return o, ref, {n: o.get(n) for n in NAMES}
+
+
+ENERGY_CASES = [c for c in CASES if c[0] in ("diffusion", "slow_modes")]
+
+
+@pytest.mark.parametrize("name,a0,a1,outputs,tol", ENERGY_CASES,
+                         ids=[f"{c[0]}-{c[1]}{c[2]}" for c in ENERGY_CASES])
+def test_oracle_matches_thor_kernel_energy(setup_energy, name, a0, a1, outputs, tol):
+    test_oracle_matches_thor_kernel(setup_energy, name, a0, a1, outputs, tol)
+
+
 @pytest.mark.parametrize("deep", [1, 0])
 def test_tracer_kernels_match_thor(built, deep):
     """The chemistry module's tracer kernels (thor_chemistry.h), two tracers:

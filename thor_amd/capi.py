@@ -34,6 +34,8 @@ FORCING_NONE = 0
 FORCING_HELD_SUAREZ = 1
 FORCING_NEWTONIAN = 2
 FORCING_DOUBLE_GRAY = 3
+THERMO_ENTROPY = 0
+THERMO_ENERGY = 1
 
 
 class NewtonianParams(C.Structure):
@@ -119,7 +121,7 @@ class Params(C.Structure):
         ("n_substeps", C.c_int), ("precision", C.c_int),
         ("VertHyDiff", C.c_int), ("VertHyDiffOrder", C.c_int), ("Diffc_v", C.c_double),
         ("DiffSponge", C.c_int), ("order_diff_sponge", C.c_int), ("Dv_sponge", C.c_double),
-        ("ns_diff_sponge", C.c_double),
+        ("ns_diff_sponge", C.c_double), ("thermo_equation", C.c_int),
     ]
 
     def as_dict(self):
@@ -287,7 +289,7 @@ def ic_rest(grid: Grid, params: Params, seed: int = 1000, temperature_noise: flo
 FIELD_SHAPES = {
     "Mh": "v", "W": "s", "Rho": "s", "pressure": "s", "Wh": "i", "temperature": "s", "pt": "s",
     "h": "s", "hh": "i", "pth": "i", "gtil": "s", "gtilh": "i", "profx_Qheat": "s",
-    "profx_dMh": "v", "profx_dWh": "i",
+    "profx_dMh": "v", "profx_dWh": "i", "Etotal": "s",
     # mixed precision only: operator weights per point (see Geo in dyn_common.h)
     "wlap": 7, "wdd": 63, "udd": 21,
 }

@@ -287,6 +287,10 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             else if (d.cp_lv == 2) k_column_prep_lv<MX, 2><<<grid_c2, blk, 0, st>>>(g, f);
             else k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f);
         });
+        if (g.energy)
+            launch(d, "energy_prep", W(10, 3), [&] {
+                k_energy_prep<<<dim3((d.P + 255) / 256, d.nv + 1), blk, 0, st>>>(g, f);
+            });
         launch(d, "adv_diff1", W(8 + (hydiff ? 7 : 0), 3 + (hydiff ? 6 : 0)), [&] {
             if (MX) k_adv_diff1_lc<<<grid_lc, blk, 0, st>>>(g, f, hydiff, lc);
             else k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff);
@@ -574,6 +578,7 @@ static bool field_ptr(Dyn &d, const std::string &n, double *&ptr, int &nlev, int
     else if (n == "hh") { ptr = d.f.hh; nlev = nv + 1; }
     else if (n == "pth") { ptr = d.f.pth; nlev = nv + 1; }
     else if (n == "gtilh") { ptr = d.f.gtilh; nlev = nv + 1; }
+    else if (n == "Etotal" && d.f.Etot) ptr = d.f.Etot;
     else if (n == "profx_Qheat") ptr = d.f.Qheat;
     else if (n == "profx_dMh") { ptr = d.f.dMh; ncomp = 3; }
     else if (n == "profx_dWh") { ptr = d.f.dWh; nlev = nv + 1; }
@@ -692,6 +697,10 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: n_substeps must be even and >= 2");
     if (prm->precision != THOR_PRECISION_FP64 && prm->precision != THOR_PRECISION_MIXED)
         return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: unknown precision mode");
+    if (prm->thermo_equation != THOR_THERMO_ENTROPY && prm->thermo_equation != THOR_THERMO_ENERGY)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: thermo_equation must be 0 (entropy) or 1 (energy)");
+    if (prm->thermo_equation == THOR_THERMO_ENERGY && prm->precision != THOR_PRECISION_FP64)
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: the energy equation has the FP64 form only");
     if (gd->nv < 3 || gd->point_num < 12) return fail(THOR_ERR_BAD_ARG, "thor_dyn_create: grid too small");
     if (4 * (gd->nv + 1) * kThomasCB(gd->nv) * 8 > 65536)
         return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_create: at most 127 levels (the vertical solve stages 16 "
@@ -795,6 +804,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     g.deep = prm->DeepModel;
     g.nonhydro = prm->NonHydro;
     g.ghv = prm->GravHeightVar;
+    g.energy = prm->thermo_equation == THOR_THERMO_ENERGY;
     std::vector<int> halo((size_t)gd->nr * NHALO);
     for (int r = 0; r < gd->nr; r++)
         for (int t = 0; t < NHALO; t++) halo[(size_t)r * NHALO + t] = gd->maps[(size_t)r * NHL2 + halo_pos(t)];
@@ -848,6 +858,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     A(&f.Sp, n1); A(&f.Sd, n1); A(&f.hflx_r, n1); A(&f.hflx_pt, n1); A(&f.tAA, n1); A(&f.tCC, n1); A(&f.tTT, n1);
     A(&f.Qheat, n1); A(&f.dMh, n3); A(&f.dWh, ni);
     if (d.mixed) A(&f.hflx_ptk, n1);
+    if (g.energy) A(&f.Etot, n1);
     const bool diff_mid = prm->HyDiff && prm->HyDiffOrder >= 6;  // intermediate passes alternate two arrays
     if (diff_mid && !d.mixed) A(&f.diffx, 6 * n1);
     if (prm->HyDiff && prm->VertHyDiff && !d.mixed) A(&f.diffmv, n3);

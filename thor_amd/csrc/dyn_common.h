@@ -49,6 +49,7 @@ struct Geo {
     const float *lvc;
     double A, Omega, Gravit, P_Ref, Rd, Cp;
     int deep, nonhydro, ghv;
+    int energy;  // total-energy thermodynamic equation (FP64 form only)
 };
 
 // Buffers of one large step. "k" = value at the current RK stage start,
@@ -61,6 +62,10 @@ struct Fields {
     double *Mhs, *Whs, *Rhos, *ps;        // deviations
     double *T, *h, *pt, *gtil;            // [nv]
     double *hh, *pth, *gtilh;             // [nv+1]
+    // Energy equation: pt and pth hold the flux variable h + ep + ek of the
+    // density/pressure update instead of the potential temperature, and Etot
+    // the total energy, advanced by every acoustic substep.
+    double *Etot;                         // [nv] (energy equation only)
     double *Adv;                          // [3][nv]
     double *diff;                         // [6][nv]
     double *diffmh;                       // [3][nv]

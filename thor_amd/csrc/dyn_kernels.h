@@ -782,6 +782,54 @@ template <bool MX, int LV> __global__ __launch_bounds__(256) void k_column_prep_
     }
 }
 
+// Total-energy form of Compute_Temperature_H_Pt_Geff (thor_auxiliary.h:143-161,
+// 212-230, 281-287), run after the column preparation, whose h and hh it
+// reads: potential energy ep = g z (g (A / (A + z))^2 z with GravHeightVar)
+// and kinetic energy ek = |(Mh, W)|^2 / (2 rho^2) of the stage at the centres,
+// at the interfaces the same interpolation and end extrapolations as the
+// other interface values; the stage's total energy rho (ep + ek) + Cv p / Rd,
+// and the flux variable of the density/pressure update, h + ep + ek at the
+// centres and hh + eph + ekh at the interfaces (thor_fastmodes.h:352-372,
+// 459-476), in place of pt and pth. Block (x, lev): 256 columns at centre
+// lev (if < nv) and interface lev.
+__global__ __launch_bounds__(256) void k_energy_prep(Geo g, Fields f) {
+    const int p = blockIdx.x * 256 + threadIdx.x, lev = blockIdx.y;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double Rd = g.Rd, Cp = g.Cp, Gravit = g.Gravit, A = g.A, Cv = Cp - Rd;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    auto ep_at = [&](double z) { return g.ghv ? Gravit * MPOW(A / (A + z), 2) * z : Gravit * z; };
+    auto ek_at = [&](int l) {
+        const double m0 = VEC(f.Mhk, 0, l, p), m1 = VEC(f.Mhk, 1, l, p), m2 = VEC(f.Mhk, 2, l, p);
+        const double w = LV(f.Wk, l), r = LV(f.Rhok, l);
+        return 0.5 * (m0 * m0 + m1 * m1 + m2 * m2 + w * w) / (r * r);
+    };
+    if (lev < nv) {
+        const double ep = ep_at(Alt[lev]), ek = ek_at(lev);
+        LV(f.Etot, lev) = LV(f.Rhok, lev) * (ep + ek) + Cv / Rd * LV(f.pk, lev);
+        LV(f.pt, lev) = LV(f.h, lev) + ep + ek;
+    }
+    double eph, ekh;
+    if (lev == 0) {
+        eph = ep_at(Alth[0]);
+        const double ek0 = ek_at(0), ek1 = ek_at(1);
+        ekh = ek1 + (ek1 - ek0) * (Alth[0] - Alt[1]) / (Alt[1] - Alt[0]);
+    } else if (lev == nv) {
+        eph = ep_at(Alth[nv]);
+        const double alt = Alt[nv - 1];
+        const double extr = (2 * Alth[nv] - alt - Alt[nv - 2]) / (Alt[nv - 1] - Alt[nv - 2]);
+        const double ekm = ek_at(nv - 2), ekc = ek_at(nv - 1);
+        ekh = ekm + (ekc - ekm) * extr;
+    } else {
+        const double xi = Alth[lev], xim = Alt[lev - 1], xip = Alt[lev];
+        const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+        eph = ep_at(Alt[lev - 1]) * intt + ep_at(Alt[lev]) * intl;
+        ekh = ek_at(lev - 1) * intt + ek_at(lev) * intl;
+    }
+    LV(f.pth, lev) = LV(f.hh, lev) + eph + ekh;
+}
+
 // ===================================================================== K_B
 // Horizontal advection (Compute_Advec_Cori1 / _Poles, thor_adv_cor.h:51-362)
 // added to the vertical part, and the first Laplacian pass of the 4th-order
@@ -813,6 +861,7 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         if (var == 0) return rho(s);
         if (var <= 3) return M(s, var - 1) / rho(s);
         if (var == 4) return W(s) / rho(s);
+        if (g.energy) return (g.Cp - Rd) * T(s);  // internal energy (thor_diff.h:196-202)
         return Rd * T(s);
     };
     auto *diff = FS<MX>::diff(f);
@@ -1278,7 +1327,8 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
     const double vgp =
         (nflxv[0] - dpr * fx) * M0x * r + (nflxv[1] - dpr * fy) * M0y * r + (nflxv[2] - dpr * fz) * M0z * r;
     double sp = (Rd / Cv) * (-nflxp - LV(f.gtil, lev) - dwhdz + vgp + LV(f.Qheat, lev));
-    sp += LV(FS<MX>::diffpr(f), lev) + 0.0;
+    if (g.energy) sp += Rd / Cv * (LV(FS<MX>::diffpr(f), lev) + 0.0);  // thor_slowmodes.h:375-377
+    else sp += LV(FS<MX>::diffpr(f), lev) + 0.0;
     LV(f.Slowp, lev) = sp;
 
     // Deviations at the start of the stage.
@@ -2808,8 +2858,33 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
     const double rhos = LV(f.Rhos, lev), pss = LV(f.ps, lev), pkv = LV(f.pk, lev);
     const double r = LV(f.Rhok, lev) + rhos;
     nflxr += dwdz;
-    LV(f.Rhos, lev) = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
+    const double rhosn = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
+    LV(f.Rhos, lev) = rhosn;
     double psn;
+    if constexpr (!MX) {
+        if (g.energy) {
+            // total energy (thor_fastmodes.h:501-533): advance it, then the
+            // pressure from it less the potential and kinetic energy of the
+            // new density, momentum (Mhs + Mhk at the point) and vertical
+            // momentum (the new Whs + Whk interpolated to the centre)
+            const double *Alt = g.Alt, *Alth = g.Alth, Gravit = g.Gravit;
+            const int nv = g.nv;
+            aux += LV(f.Etot, lev) + (LV(f.diffpr, lev) + 0.0) * dt + LV(f.Qheat, lev) * dt;
+            LV(f.Etot, lev) = aux;
+            const double w =
+                (whl2 * (Alth[lev + 1] - Alt[lev]) + wht2 * (Alt[lev] - Alth[lev])) / (Alth[lev + 1] - Alth[lev]);
+            const double v0 = VEC(f.Mhs, 0, lev, p) + VEC(f.Mhk, 0, lev, p),
+                         v1 = VEC(f.Mhs, 1, lev, p) + VEC(f.Mhk, 1, lev, p),
+                         v2 = VEC(f.Mhs, 2, lev, p) + VEC(f.Mhk, 2, lev, p);
+            const double rn = rhosn + LV(f.Rhok, lev);
+            double epk = g.ghv ? rn * Gravit * XPOW(A / (A + Alt[lev]), 2) * Alt[lev] : rn * Gravit * Alt[lev];
+            epk = epk + 0.5 * (XPOW(v0, 2) + XPOW(v1, 2) + XPOW(v2, 2) + XPOW(w, 2)) / rn;
+            psn = Rd / Cv * (aux - epk) - pkv;
+            LV(f.ps, lev) = psn;
+            if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
+            return;
+        }
+    }
     if constexpr (MX) {
         // The same update with one power. With pf = pss + pkv, q = (pf / P_Ref)^(Cv/Cp)
         // and x = -Rd (nflxpt + dwptdz) dt / (P_Ref q): Rd aux / P_Ref = q (1 + x),

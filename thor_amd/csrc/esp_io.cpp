@@ -235,6 +235,7 @@ extern "C" thor_status thor_config_run(const thor_config *c, thor_run_config *r)
     p.ns_diff_sponge = 0.75;
     p.n_substeps = 6;  // thor_driver.cu:91-92
     p.precision = THOR_PRECISION_FP64;
+    p.thermo_equation = THOR_THERMO_ENTROPY;
     r->vlevel = 32;
     r->spring_dynamics = 1;
     r->spring_beta = 1.15;
@@ -396,8 +397,12 @@ extern "C" thor_status thor_config_run(const thor_config *c, thor_run_config *r)
         const char *v = get(k);
         return !v || std::string(v) == want;
     };
-    if (!must("thermo_equation", "entropy"))
-        return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: thermo_equation must be entropy");
+    // thermo_equation (esp.cu:441-443, 696-710): entropy (default) or energy
+    if (const char *v = get("thermo_equation")) {
+        if (std::string(v) == "entropy") p.thermo_equation = THOR_THERMO_ENTROPY;
+        else if (std::string(v) == "energy") p.thermo_equation = THOR_THERMO_ENERGY;
+        else return fail(THOR_ERR_BAD_ARG, std::string("thor_config_run: thermo_equation not recognised: ") + v);
+    }
     if (!must("vert_refined", "false"))
         return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: vert_refined must be false (uniform vertical grid)");
     if (!must("conv_adj", "false")) return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: conv_adj must be false");
