@@ -64,6 +64,12 @@ def lib(variant: str = ""):
             getattr(L, f).restype = None
         L.oracle_steps.argtypes = [C.c_void_p, C.c_int]
         L.oracle_steps.restype = None
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_set_threads.restype = None
+        L.oracle_get_threads.argtypes = []
+        L.oracle_get_threads.restype = C.c_int
+        if os.environ.get("THOR_ORACLE_THREADS"):
+            L.oracle_set_threads(int(os.environ["THOR_ORACLE_THREADS"]))
         L.oracle_globdiag.argtypes = [C.c_void_p, _dp]
         L.oracle_globdiag.restype = None
         L.oracle_tree_sum.argtypes = [_dp, C.c_long]
@@ -87,6 +93,14 @@ def lib(variant: str = ""):
         L.oracle_set_tracers.restype = None
         _libs[variant] = L
     return _libs[variant]
+
+
+def set_threads(n: int, variant: str = "") -> int:
+    """Host threads of the oracle's block and column loops (n < 1: the OpenMP
+    default). The results do not depend on it. Returns the count in force."""
+    L = lib(variant)
+    L.oracle_set_threads(int(n))
+    return L.oracle_get_threads()
 
 
 def tree_sum(a: np.ndarray) -> float:

@@ -5,6 +5,11 @@
 // rhombus maps exactly as the kernel's shared-memory halo load does
 // (src/headers/kernel_halo_helpers.h:47-128). Build with -ffp-contract=off so
 // every product and sum is rounded as written.
+//
+// Loops over independent blocks (rhombus tiles, columns) may run on several
+// host threads (oracle_set_threads; one thread by default, the faithful
+// serial loop). Every output element is written by exactly one iteration with
+// the same arithmetic, so the results do not depend on the thread count.
 #include "thor_oracle.h"
 #include "../thor_amd/csrc/portable_math.h"
 
@@ -14,6 +19,11 @@
 #include <map>
 #include <string>
 #include <vector>
+
+#include <omp.h>
+
+// independent iterations of an outer block or column loop
+#define ORACLE_PAR _Pragma("omp parallel for schedule(dynamic)")
 
 double oracle_tree_sum(const double *d, long n);
 
@@ -117,8 +127,9 @@ void advec_cori1(Oracle &o) {
     const int nv = o.nv;
     double *Adv = o.A_("Adv"), *v = o.A_("v"), *Mh = o.A_("Mhk"), *W = o.A_("Wk"), *Rho = o.A_("Rhok");
     const double *div = o.g.div, *fr = o.g.func_r;
-    double Ms[3][NHL2], vs[3][NHL2];
+    ORACLE_PAR
     for (int ib = 0; ib < o.nr; ib++) {
+        double Ms[3][NHL2], vs[3][NHL2];
         const int *m = o.g.maps + ib * NHL2;
         for (int lev = 0; lev < nv; lev++) {
             for (int pos : kLoad) {
@@ -194,6 +205,7 @@ void advec_cori2(Oracle &o) {
     double *Adv = o.A_("Adv"), *v = o.A_("v"), *Wh = o.A_("Whk"), *Rho = o.A_("Rhok");
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh, *fr = o.g.func_r;
     const double A = o.p.A, Omega = o.p.Omega;
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++) {
         double altht = 0, althl = 0, altt = 0, alt = 0, altl = 0;
         double wht = 0, whl = 0, vxl = 0, vx = 0, vxt = 0, vyl = 0, vy = 0, vyt = 0, vzl = 0, vz = 0, vzt = 0;
@@ -295,6 +307,7 @@ void temperature_h_pt_geff(Oracle &o) {
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     const double P_Ref = o.p.P_Ref, Gravit = o.p.Gravit, A = o.p.A;
     const bool ghv = o.p.GravHeightVar;
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++) {
         double temperature, rho = 0, rhoh, rhol = 0, rhop, pressure = 0, h = 0, hl = 0, pt = 0, ptl = 0,
                                      pl = 0, pp, dpdz;
@@ -477,7 +490,7 @@ inline double laplacian(const Oracle &o, int id, double rscale, int n_faces, Get
 void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
     const int nv = o.nv, P = o.P;
     double *diff = o.A_("diff"), *Mh = o.A_("Mhk"), *Rho = o.A_("Rhok"), *T = o.A_("temperature"),
-           *W = o.A_("Wk"), *Rd = o.A_("Rd");
+           *W = o.A_("Wk"), *Rd = o.A_("Rd"), *Cp = o.A_("Cp");
     double *diffmh = o.A_("diffmh"), *diffw = o.A_("diffw"), *diffrh = o.A_("diffrh"), *diffpr = o.A_("diffpr");
     std::vector<double> diff_in(diff, diff + (size_t)6 * P * nv);
     auto arg = [&](int gi, int lev, int var) -> double {
@@ -487,7 +500,7 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
             if (var == 0) a = rho;
             else if (var <= 3) a = Mh[(gi * nv + lev) * 3 + var - 1];
             else if (var == 4) a = W[gi * nv + lev];
-            else if (o.p.energy_equation) a = (o.A_("Cp")[gi * nv + lev] - Rd[gi * nv + lev]) * T[gi * nv + lev];
+            else if (o.p.energy_equation) a = (Cp[gi * nv + lev] - Rd[gi * nv + lev]) * T[gi * nv + lev];
             else a = Rd[gi * nv + lev] * T[gi * nv + lev];
             if (var >= 1 && var <= 4) a = a / rho;
             return a;
@@ -520,9 +533,11 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
             diff[(id * nv + lev) * 6 + var] = lap;
         }
     };
-    double as[NHL2];
-    for (int var = 0; var < 6; var++)
-        for (int ib = 0; ib < o.nr; ib++) {
+    ORACLE_PAR
+    for (int vb = 0; vb < 6 * o.nr; vb++) {
+        const int var = vb / o.nr, ib = vb % o.nr;
+        {
+            double as[NHL2];
             const int *m = o.g.maps + ib * NHL2;
             for (int lev = 0; lev < nv; lev++) {
                 fill(as, m, [&](int gi) { return arg(gi, lev, var); });
@@ -546,6 +561,7 @@ void diffusion_pass(Oracle &o, int stepnum, bool laststep) {
                     }
             }
         }
+    }
     // Poles.
     for (int var = 0; var < 6; var++)
         for (int id = P - 2; id < P; id++)
@@ -572,6 +588,7 @@ void correct_horizontal(Oracle &o) {
     const double *fr = o.g.func_r;
     for (const char *name : {"diffmh", "diffmv"}) {
         double *dm = o.A_(name);
+        ORACLE_PAR
         for (int id = 0; id < o.P; id++)
             for (int lev = 0; lev < nv; lev++) {
                 double *d = dm + (id * nv + lev) * 3;
@@ -621,8 +638,9 @@ void vertical_diff(Oracle &o) {
         const double flux = MPOW(rup, 2) * (a[lev + 2] - a[lev + 1]) * dzup - MPOW(rlow, 2) * (a[lev + 1] - a[lev]) * dzlow;
         return std::make_pair(flux * dzi, MPOW(ri, 2));
     };
-    std::vector<double> a(nv + 2), d(nv);
-    for (int id = 0; id < P; id++)
+    ORACLE_PAR
+    for (int id = 0; id < P; id++) {
+        std::vector<double> a(nv + 2), d(nv);
         for (int var = 0; var < 6; var++) {
             auto value = [&](int lev) {
                 const long e = (long)id * nv + lev;
@@ -659,6 +677,7 @@ void vertical_diff(Oracle &o) {
                 else dpv[e] = out;
             }
         }
+    }
 }
 
 // ------------------------------------------------------- divergence damping
@@ -669,7 +688,6 @@ void divm_pass(Oracle &o, const char *Mname, const char *Whname, bool laststep) 
     double *DivM = o.A_("DivM"), *divg = o.A_("divg_Mh"), *M = o.A_(Mname), *Wh = o.A_(Whname);
     const double *areas = o.g.areas, *areasTr = o.g.areasTr, *nti = o.g.nvecti, *nte = o.g.nvecte,
                  *noa = o.g.nvecoa, *fr = o.g.func_r, *Alth = o.g.Altitudeh;
-    double as[3][NHL2], wl[NHL2], wt[NHL2];
     std::vector<double> divg_in;
     if (laststep) divg_in.assign(divg, divg + (size_t)3 * P * nv);
 
@@ -745,7 +763,9 @@ void divm_pass(Oracle &o, const char *Mname, const char *Whname, bool laststep) 
             for (int k = 0; k < 3; k++) divg[(id * nv + lev) * 3 + k] = l[k];
         }
     };
+    ORACLE_PAR
     for (int ib = 0; ib < o.nr; ib++) {
+        double as[3][NHL2], wl[NHL2], wt[NHL2];
         const int *m = o.g.maps + ib * NHL2;
         for (int lev = 0; lev < nv; lev++) {
             for (int pos : kLoad) {
@@ -921,6 +941,7 @@ void slow_modes(Oracle &o) {
         else
             Slowp[id * nv + lev] += diffpr[id * nv + lev] + diffprv[id * nv + lev];
     };
+    ORACLE_PAR
     for (int ib = 0; ib < o.nr; ib++) {
         const int *m = o.g.maps + ib * NHL2;
         for (int lev = 0; lev < nv; lev++)
@@ -949,6 +970,7 @@ void update_rk(Oracle &o) {
            *Whi = o.A_("Wh"), *Ws = o.A_("Ws"), *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"), *Ri = o.A_("Rho"),
            *ps = o.A_("pressures"), *pk = o.A_("pressurek"), *pi = o.A_("pressure");
     const double *fr = o.g.func_r, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++) {
         double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
         for (int lev = 0; lev < nv; lev++) {
@@ -988,6 +1010,7 @@ void update_rk2(Oracle &o) {
     double *Ms = o.A_("Mhs"), *Mk = o.A_("Mhk"), *Whs = o.A_("Whs"), *Whk = o.A_("Whk"), *Wk = o.A_("Wk"),
            *Rs = o.A_("Rhos"), *Rk = o.A_("Rhok"), *ps = o.A_("pressures"), *pk = o.A_("pressurek");
     const double *fr = o.g.func_r, *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++) {
         double fx = fr[id * 3 + 0], fy = fr[id * 3 + 1], fz = fr[id * 3 + 2];
         for (int lev = 0; lev < nv; lev++) {
@@ -1052,8 +1075,9 @@ void momentum_eq(Oracle &o, double dt) {
         M[e + 1] += My;
         M[e + 2] += Mz;
     };
-    double sh[NHL2];
+    ORACLE_PAR
     for (int ib = 0; ib < o.nr; ib++) {
+        double sh[NHL2];
         const int *m = o.g.maps + ib * NHL2;
         for (int lev = 0; lev < nv; lev++) {
             fill(sh, m, [&](int gi) { return ps[gi * nv + lev]; });
@@ -1095,8 +1119,9 @@ void prepare_implicit_vertical(Oracle &o) {
         Sp[id * nv + lev] = nflxp + Slowp[id * nv + lev];
         Sd[id * nv + lev] = nflxr + SlowRho[id * nv + lev];
     };
-    double vs[3][NHL2], hs[NHL2];
+    ORACLE_PAR
     for (int ib = 0; ib < o.nr; ib++) {
+        double vs[3][NHL2], hs[NHL2];
         const int *m = o.g.maps + ib * NHL2;
         for (int lev = 0; lev < nv; lev++) {
             for (int k = 0; k < 3; k++) fill(vs[k], m, [&](int gi) { return M[(gi * nv + lev) * 3 + k]; });
@@ -1129,13 +1154,14 @@ void vertical_eq(Oracle &o, double deltat) {
     const double *Alt = o.g.Altitude, *Alth = o.g.Altitudeh;
     const double A = o.p.A, Gravit = o.p.Gravit;
     const bool deep = o.p.DeepModel, ghv = o.p.GravHeightVar, nonh = o.p.NonHydro;
-    std::vector<double> cc(nvi), dd(nvi);
     double *ta = nullptr, *tb = nullptr, *tc = nullptr, *td = nullptr;
     if (o.capture_tri) {
         for (const char *n : {"tri_a", "tri_b", "tri_c", "tri_d"}) o.arr[n].assign((size_t)o.P * nvi, 0.0);
         ta = o.A_("tri_a"), tb = o.A_("tri_b"), tc = o.A_("tri_c"), td = o.A_("tri_d");
     }
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++) {
+        std::vector<double> cc(nvi), dd(nvi);
         double altht = 0, althl = 0, alth = 0, alt = 0, altl = 0, hp = 0, h = 0, hm = 0, gp = 0, g = 0, gm = 0;
         double Sp = 0, Spl = 0, p = 0, pl = 0, r = 0, rl = 0, Sdl = 0, Sd = 0;
         double CRddl = 0, CRddu = 0, GCoRl = 0, GCoRu = 0;
@@ -1403,8 +1429,9 @@ void density_pressure(Oracle &o, double dt) {
         ps[id * nv + lev] = p - pk[id * nv + lev] + (diffpr[id * nv + lev] + diffprv[id * nv + lev]) * dt
                             + Rd[id * nv + lev] / Cv * Qh[id * nv + lev] * dt;
     };
-    double vs[3][NHL2], v1s[3][NHL2], pts[NHL2];
+    ORACLE_PAR
     for (int ib = 0; ib < o.nr; ib++) {
+        double vs[3][NHL2], v1s[3][NHL2], pts[NHL2];
         const int *m = o.g.maps + ib * NHL2;
         for (int lev = 0; lev < nv; lev++) {
             for (int pos : kLoad) {
@@ -1792,6 +1819,7 @@ void profx_hs(Oracle &o) {
             W[id * nv + lev] = whl * intt + wht * intl;
         }
     // Compute_temperature (profx_auxiliary.h:64-93)
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++)
         for (int lev = 0; lev < nv; lev++) {
             int e = id * nv + lev;
@@ -1811,6 +1839,7 @@ void held_suarez(Oracle &o) {
     double *T = o.A_("temperature"), *p = o.A_("pressure"), *Rho = o.A_("Rho"), *Mh = o.A_("Mh");
     const double *Alt = o.g.Altitude, *ll = o.g.lonlat;
     const double Gravit = o.p.Gravit, kappa = o.p.Rd / o.p.Cp, dt = o.p.timestep;
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++)
         for (int lev = 0; lev < nv; lev++) {
             double sigmab = 0.7, ka = (1.0 / 40.0) * (1.0 / 86400.0), kf = 1.0 / 86400.0,
@@ -1913,9 +1942,10 @@ void profx_rt(Oracle &o) {
             if (!o.sponge_imp()) W[e] = whl * intt + wht * intl;
             T[e] = p[e] / (Rd * Rho[e]);
         }
-    std::vector<double> phtemp(nvi), thtemp(nvi), ttemp(nv), tau(2 * nv), dtemp(nv), fsw_up(nvi), fsw_dn(nvi),
-        flw_up(nvi), flw_dn(nvi);
+    ORACLE_PAR
     for (int id = 0; id < o.P; id++) {
+        std::vector<double> phtemp(nvi), thtemp(nvi), ttemp(nv), tau(2 * nv), dtemp(nv), fsw_up(nvi), fsw_dn(nvi),
+            flw_up(nvi), flw_dn(nvi);
         const double *pr = p + (size_t)id * nv, *rh = Rho + (size_t)id * nv, *te = T + (size_t)id * nv;
         for (int lev = 0; lev < nv; lev++) dtemp[lev] = 0.0;
         for (int lev = 0; lev <= nv; lev++) {
@@ -2421,6 +2451,10 @@ void oracle_set_tracers(oracle *o, int ntr) {
     const size_t n = (size_t)o->P * o->nv * (ntr > 0 ? ntr : 0);
     for (const char *name : {"tracer", "tracerk", "tracers", "difftr", "trdiff"}) o->arr[name].assign(n, 0.0);
 }
+void oracle_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+int oracle_get_threads(void) { return omp_get_max_threads(); }
 void oracle_steps(oracle *o, int n) {
     for (int i = 0; i < n; i++) {
         profx_hs(*o);

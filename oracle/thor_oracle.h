@@ -77,6 +77,11 @@ void oracle_profx_held_suarez(oracle *o);
 int oracle_kernel(oracle *o, const char *name, int a0, int a1);
 // ProfX + Thor, n times.
 void oracle_steps(oracle *o, int n);
+// Host threads of the block and column loops (process-wide; n < 1 leaves the
+// OpenMP default, which honours OMP_NUM_THREADS). Results are the same for
+// every thread count; 1 is the serial loop.
+void oracle_set_threads(int n);
+int oracle_get_threads(void);
 // Chemistry tracers (thor_chemistry.h, driven through the physics hooks as in
 // chemistry.cu:333-502): ntr tracer masses [P][nv][ntr] in the arrays
 // "tracer", "tracerk", "tracers", "difftr" (0 switches them off).
