@@ -22,6 +22,17 @@ def grid(capi, glevel, nv, top=36000.0, radius=6.371e6):
     return capi.Grid(glevel, nv, radius, top, spring=True, beta=1.15, cache_dir=CACHE)
 
 
+def momentum_scale_errs(gs: dict, os_: dict, p) -> dict:
+    """Errors of the momenta (Mh, Wh, W) relative to the flow's own momentum
+    scale rho c_s (max rho times the isothermal sound speed at Tmean). Near
+    rest the maximum of a momentum field is itself a round-off residual of
+    hydrostatic balance, so an error relative to it says nothing about the
+    step's accuracy."""
+    cs = np.sqrt(p.Cp / (p.Cp - p.Rd) * p.Rd * p.Tmean)
+    scale = np.abs(os_["Rho"]).max() * cs
+    return {k: float(np.abs(gs[k] - os_[k]).max() / scale) for k in ("Mh", "Wh", "W")}
+
+
 def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     """max |a - b| / max |b| (0 if both are zero)."""
     den = np.abs(b).max()

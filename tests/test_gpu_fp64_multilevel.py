@@ -1,0 +1,71 @@
+"""The FP64 form's multi-level horizontal stencils (one block per tile and
+NL levels, each triangle's geometry loaded once for all NL levels) against
+the per-level kernels (THOR_FP64_MULTILEVEL=0) and against the oracle.
+
+Level counts that NL does not divide (23 = 5 x 4 + 3 = 11 x 2 + 1) exercise
+the last chunk's clamped levels, which are read but never written.
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from common import STATE, earth_params, grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(capi, g, p, st, steps, variant="", multilevel=True):
+    old = os.environ.get("THOR_FP64_MULTILEVEL")
+    os.environ["THOR_FP64_MULTILEVEL"] = "1" if multilevel else "0"
+    try:
+        d = capi.Dyn(g, p, variant=variant)  # the switch is read when the handle is created
+    finally:
+        if old is None:
+            del os.environ["THOR_FP64_MULTILEVEL"]
+        else:
+            os.environ["THOR_FP64_MULTILEVEL"] = old
+    d.set_state(st)
+    d.run(steps)
+    d.sync()
+    out = d.get_state()
+    d.close()
+    return out
+
+
+@pytest.mark.parametrize("nv,deep,nonhydro,order", [(20, 1, 1, 4), (23, 1, 1, 4), (23, 0, 0, 4), (21, 1, 1, 6)])
+def test_multilevel_equals_per_level_kernels(built, nv, deep, nonhydro, order):
+    """Production build, Held-Suarez g4, hyperdiffusion and divergence
+    damping on: two steps with the multi-level kernels equal two steps with
+    the per-level kernels bit for bit."""
+    capi = built
+    print(f"case: FP64 g4 L{nv}, DeepModel {deep}, NonHydro {nonhydro}, HyDiffOrder {order}, multi-level vs "
+          f"per-level, 2 steps", flush=True)
+    g = grid(capi, 4, nv)
+    p = earth_params(capi, 4, dt=600.0, ns=6, DeepModel=deep, NonHydro=nonhydro, HyDiffOrder=order)
+    st = capi.ic_rest(g, p, seed=1040 + nv, temperature_noise=1e-3, wind_noise=1.0)
+    a = _run(capi, g, p, st, 2, multilevel=True)
+    b = _run(capi, g, p, st, 2, multilevel=False)
+    for k in STATE:
+        assert np.isfinite(a[k]).all()
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_multilevel_parity_build_odd_levels_bitwise(built):
+    """Parity build at g4 L23 (both chunk sizes leave a partial last chunk):
+    two ProfX + Thor steps equal the oracle bit for bit."""
+    from oracle.oracle import Oracle
+    capi = built
+    print("case: parity build g4 L23, HS, 2 steps, multi-level kernels vs oracle", flush=True)
+    g = grid(capi, 4, 23)
+    p = earth_params(capi, 4, dt=600.0, ns=6)
+    st = capi.ic_rest(g, p, seed=1063, temperature_noise=1e-3, wind_noise=1.0)
+    gs = _run(capi, g, p, st, 2, variant="pm", multilevel=True)
+    o = Oracle(g, p, variant="pm")
+    o.set_state(st)
+    o.steps(2)
+    os_ = o.get_state()
+    for k in STATE:
+        assert np.array_equal(gs[k], os_[k]), k

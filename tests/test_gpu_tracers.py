@@ -12,7 +12,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from common import STATE, earth_params, grid, max_rel
+from common import STATE, earth_params, grid, max_rel, momentum_scale_errs
 
 pytestmark = pytest.mark.gpu
 
@@ -48,10 +48,18 @@ def test_tracers_through_hooks_match_oracle(built, variant, deep, hydiff, tol):
         d.run(1)
         d.sync()
         o.steps(1)
-        e = {k: max_rel(v, o.get_state()[k]) for k, v in d.get_state().items()}
+        gs, os_ = d.get_state(), o.get_state()
+        e = {k: max_rel(v, os_[k]) for k, v in gs.items()}
         e["tracer"] = max_rel(t.get().ravel(), o.get("tracer"))
-        print(f"step {step}", {k: f"{v:.2e}" for k, v in e.items()}, flush=True)
-        assert max(e.values()) <= tol
+        em = momentum_scale_errs(gs, os_, p)
+        print(f"step {step}", {k: f"{v:.2e}" for k, v in e.items()},
+              "momenta vs rho c_s", {k: f"{v:.2e}" for k, v in em.items()}, flush=True)
+        if tol == 0.0:
+            assert max(e.values()) == 0.0
+        else:
+            # production build: libm ulps; momenta against the momentum scale
+            assert max(e[k] for k in ("Rho", "pressure", "tracer")) <= tol
+            assert max(em.values()) <= tol
     # the tracers moved (advection and diffusion did something)
     assert max_rel(t.get(), tr0) > 1e-9
     t.close()

@@ -5,6 +5,7 @@
 #include "../../include/thor_mlp.h"
 #include "common.h"
 #include "dyn_kernels.h"
+#include "dyn_kernels_ml.h"
 #include "mlp_internal.h"
 #include "tracer_kernels.h"
 
@@ -66,6 +67,7 @@ struct Dyn {
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
     bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
+    bool fp64_ml = true;   // FP64 path: horizontal stencils over several levels per block (*_ml kernels)
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
     bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
     bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
@@ -231,6 +233,9 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     const dim3 grid_lc(8 * d.g.tiles_per_xcd * ((d.nv + d.lc - 1) / d.lc));
     const dim3 grid_lp(8 * d.g.tiles_per_xcd * ((d.nv + d.lc_prep - 1) / d.lc_prep));
     const int lc = d.lc, lcp = d.lc_prep;
+    // FP64 multi-level stencils: one tile x NL levels per block
+    const bool ml = !MX && d.fp64_ml;
+    auto grid_ml = [&](int nl) { return dim3(8 * d.g.tiles_per_xcd * ((d.nv + nl - 1) / nl)); };
     // column_prep: (256 columns, level 0..nv) per block, XCD-major
     const dim3 grid_c(8 * (((d.P + kTile - 1) / kTile + 7) / 8) * (d.nv + 1));
     const dim3 grid_c2(8 * (((d.P + kTile - 1) / kTile + 7) / 8) * ((d.nv + 2) / 2));
@@ -293,6 +298,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             });
         launch(d, "adv_diff1", W(8 + (hydiff ? 7 : 0), 3 + (hydiff ? 6 : 0)), [&] {
             if (MX) k_adv_diff1_lc<<<grid_lc, blk, 0, st>>>(g, f, hydiff, lc);
+            else if (ml) k_adv_diff1_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f, hydiff);
             else k_adv_diff1<MX><<<grid_t2, blk, 0, st>>>(g, f, hydiff);
         });
         // HyDiffOrder / 2 - 2 intermediate passes (HyDiffOrder >= 6) between
@@ -325,7 +331,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         } else if (hydiff || divdamp) {
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
                    [&] {
-                k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, fd, hydiff, divdamp, last_sign);
+                if (ml) k_diff2_divm0_ml<2><<<grid_ml(2), blk, 0, st>>>(g, fd, hydiff, divdamp, last_sign);
+                else k_diff2_divm0<MX><<<grid_t2, blk, 0, st>>>(g, fd, hydiff, divdamp, last_sign);
             });
         }
         if (hydiff && d.prm.VertHyDiff) {
@@ -374,12 +381,16 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                    [&] { k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc); });
         } else {
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
-                   [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
+                   [&] {
+                       if (ml) k_slow_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f, divdamp, rk);
+                       else k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk);
+                   });
         }
         if (rk > 0) rk_arrays(0);
         auto divm0 = [&] {
             launch(d, "divm0", W(1, 6), [&] {
                 if (MX) k_divm0_lc<false><<<grid_lc, blk, 0, st>>>(g, f, lc);
+                else if (ml) k_divm0_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f);
                 else k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
         };
@@ -392,6 +403,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             const int next = divdamp && ns < ns_it - 1;
             launch(d, "momentum", W(1, (divdamp ? 6 : 0) + 12), [&] {
                 if (MX) k_momentum_lc<<<grid_lc, blk, 0, st>>>(g, f, divdamp, times, lc);
+                else if (ml) k_momentum_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f, divdamp, times);
                 else k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times);
             });
             // Prepare_Implicit_Vertical, plus the horizontal flux divergences of
@@ -401,6 +413,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             launch(d, "prepare", MX && d.prep_split ? W(4, 5) : W(7, 3), [&] {
                 if (MX && d.prep_split) k_prepare_lc<true><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else if (MX) k_prepare_lc<false><<<grid_lp, blk, 0, st>>>(g, f, lcp);
+                else if (ml) k_prepare_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f);
                 else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
             // Vertical_Eq (12) + Density_Pressure_Eqs (22, of which Mhs and diffpr FP32 in mixed)
@@ -830,6 +843,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         if (v == 1 || v == 2 || v == 4) d.cp_lv = v;
     }
     if (const char *e = std::getenv("THOR_PREPARE_SPLIT")) d.prep_split = d.prep_split && std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_FP64_MULTILEVEL")) d.fp64_ml = std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
               && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)

@@ -926,6 +926,35 @@ template <bool MX> __global__ STENCIL_BOUNDS void k_adv_diff1(Geo g, Fields f, i
 // Last hyperdiffusion pass (Diffusion_Op laststep) + Correct_Horizontal
 // (thor_diff.h:882-915), and divergence-damping pass 0 on Mhk/Whk
 // (DivM_Op laststep = false).
+// diff2_tail: the last pass's output at one cell from its six Laplacians
+// (the strength and sign, the diffusive sponge, Correct_Horizontal); rr is the
+// cell's own density.
+template <bool MX>
+__device__ __forceinline__ void diff2_tail(const Geo &g, const Fields &f, int p, int lev, bool pole, double sign,
+                                           double out[6], double rr) {
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double sdiff = -g.Kdh4[lev] * sign;
+#pragma unroll
+    for (int var = 0; var < 6; var++) out[var] = pole ? sdiff * out[var] : out[var] * sdiff;
+    if (const auto *dsp = FS<MX>::dsp(f)) {
+        // diffusive sponge (thor_diff.h:453-509): (-1)^(order/2 + 1) rho Kdh2
+        // times the sponge pass's input at the point, for u, v, w
+        const double k2 = g.Kdh2[lev];
+        for (int var = 1; var <= 4; var++)
+            out[var] += g.ssp * rr * k2 * (double)VEC(dsp, var, lev, p) * g.dsp_scale;
+    }
+    LV(FS<MX>::diffrh(f), lev) = out[0];
+    LV(FS<MX>::diffw(f), lev) = out[4];
+    LV(FS<MX>::diffpr(f), lev) = out[5];
+    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+    const double dmhr = fx * out[1] + fy * out[2] + fz * out[3];
+    auto *diffmh = FS<MX>::diffmh(f);
+    VEC(diffmh, 0, lev, p) = out[1] + -fx * dmhr;
+    VEC(diffmh, 1, lev, p) = out[2] + -fy * dmhr;
+    VEC(diffmh, 2, lev, p) = out[3] + -fz * dmhr;
+}
+
 template <bool MX, class GetD, class GetS, class GetM, class GetWl, class GetWt>
 __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
                                                   int hydiff, int divdamp, double sign, GetD dif, GetS rho, GetM M,
@@ -934,7 +963,6 @@ __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f,
     const int nv = g.nv;
     if (hydiff) {
         const double rs = rscale_at(g, lev);
-        const double sdiff = -g.Kdh4[lev] * sign;
         double out[6];
         if constexpr (MX) {
             float o[6];
@@ -947,24 +975,7 @@ __device__ __forceinline__ void diff2_divm0_point(const Geo &g, const Fields &f,
             auto a = [&](int var, int s) -> double { return var == 0 ? dif(s, 0) : dif(s, var) * rho(s); };
             laplacians<6>(g, p, nf, rs, a, out);
         }
-#pragma unroll
-        for (int var = 0; var < 6; var++) out[var] = pole ? sdiff * out[var] : out[var] * sdiff;
-        if (const auto *dsp = FS<MX>::dsp(f)) {
-            // diffusive sponge (thor_diff.h:453-509): (-1)^(order/2 + 1) rho Kdh2
-            // times the sponge pass's input at the point, for u, v, w
-            const double rr = rho(0), k2 = g.Kdh2[lev];
-            for (int var = 1; var <= 4; var++)
-                out[var] += g.ssp * rr * k2 * (double)VEC(dsp, var, lev, p) * g.dsp_scale;
-        }
-        LV(FS<MX>::diffrh(f), lev) = out[0];
-        LV(FS<MX>::diffw(f), lev) = out[4];
-        LV(FS<MX>::diffpr(f), lev) = out[5];
-        const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
-        const double dmhr = fx * out[1] + fy * out[2] + fz * out[3];
-        auto *diffmh = FS<MX>::diffmh(f);
-        VEC(diffmh, 0, lev, p) = out[1] + -fx * dmhr;
-        VEC(diffmh, 1, lev, p) = out[2] + -fy * dmhr;
-        VEC(diffmh, 2, lev, p) = out[3] + -fz * dmhr;
+        diff2_tail<MX>(g, f, p, lev, pole, sign, out, rho(0));
     }
     if (divdamp) {
         auto *divg = FS<MX>::divg(f);
@@ -1198,14 +1209,17 @@ __global__ STENCIL_BOUNDS void k_diff_mid(Geo g, const typename FS<MX>::R *src, 
 // (thor_slowmodes.h:48-683, entropy form) + UpdateRK (thor_auxiliary.h:311-418)
 // for rk > 0, or the zero start of the deviations for rk = 0
 // (thor_driver.cu:104-108).
+// slow_tail: everything of one cell after the horizontal operator sums
+// (DivM, the pressure gradient nflxv and the flux divergences nflxr, nflxp),
+// given the cell's own pressure pv0 and momentum M0.
+template <bool MX>
+__device__ __forceinline__ void slow_tail(const Geo &g, const Fields &f, int p, int lev, bool pole, int rk,
+                                          const double DivM[3], const double nflxv_in[3], double nflxr, double nflxp,
+                                          double pv0, double M0x, double M0y, double M0z);
 template <bool MX, class GetG, class GetP, class GetH, class GetM>
 __device__ __forceinline__ void slow_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
                                            int divdamp, int rk, GetG dg, GetP pv, GetH hv, GetM mv) {
     const size_t S = g.S;
-    const int nv = g.nv;
-    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cv = g.Cp - g.Rd;
-    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
-    const double *Alt = g.Alt, *Alth = g.Alth;
     double DivM[3] = {0.0, 0.0, 0.0};
     if (divdamp) {
         if constexpr (MX) {
@@ -1217,44 +1231,7 @@ __device__ __forceinline__ void slow_point(const Geo &g, const Fields &f, int p,
             divm_nf<true>(g, p, lev, nf, pole, dg, none, none, DivM);
         }
     }
-    const auto *Adv = FS<MX>::Adv(f);
-    double advx = VEC(Adv, 0, lev, p), advy = VEC(Adv, 1, lev, p), advz = VEC(Adv, 2, lev, p);
-    const double rho = LV(f.Rhok, lev);
-    double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
-    if (lev != 0) {
-        const double ax = VEC(Adv, 0, lev - 1, p), ay = VEC(Adv, 1, lev - 1, p), az = VEC(Adv, 2, lev - 1, p);
-        advrl = ax * fx + ay * fy + az * fz;
-        rhol = LV(f.Rhok, lev - 1);
-        altl = Alt[lev - 1];
-        pressurel = LV(f.pk, lev - 1);
-    }
-    double advr = advx * fx + advy * fy + advz * fz;
-    advx += -advr * fx;
-    advy += -advr * fy;
-    advz += -advr * fz;
-    const double altht = Alth[lev + 1], althl = Alth[lev], alt = Alt[lev];
-    const double M0x = mv(0, 0), M0y = mv(0, 1), M0z = mv(0, 2);
-    if (!g.nonhydro) {
-        advr = (XPOW(M0x, 2) + XPOW(M0y, 2) + XPOW(M0z, 2)) / (rho * (alt + A));
-        if (lev != 0) {
-            const double m1x = VEC(f.Mhk, 0, lev - 1, p), m1y = VEC(f.Mhk, 1, lev - 1, p),
-                         m1z = VEC(f.Mhk, 2, lev - 1, p);
-            advrl = (XPOW(m1x, 2) + XPOW(m1y, 2) + XPOW(m1z, 2)) / (rhol * (altl + A));
-        }
-    }
-    const double dz = altht - althl;
-    double r2p, r2m, r2l, rscale, dr2dz;
-    if (g.deep) {
-        r2p = XPOW(altht + A, 2.0);
-        r2m = XPOW(alt + A, 2.0);
-        r2l = XPOW(althl + A, 2.0);
-        rscale = A / (alt + A);
-        dr2dz = (XPOW(altht + A, 3.0) - XPOW(althl + A, 3.0)) / 3;
-    } else {
-        r2p = r2m = r2l = 1.0;
-        rscale = 1.0;
-        dr2dz = r2m * dz;
-    }
+    const double rscale = rscale_at(g, lev);
     double nflxv[3], nflxr = 0.0, nflxp = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -1271,6 +1248,54 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         nflxv[k] = rscale * sg;
         nflxr += rscale * sr;
         nflxp += rscale * sp;
+    }
+    slow_tail<MX>(g, f, p, lev, pole, rk, DivM, nflxv, nflxr, nflxp, pv(0), mv(0, 0), mv(0, 1), mv(0, 2));
+}
+
+template <bool MX>
+__device__ __forceinline__ void slow_tail(const Geo &g, const Fields &f, int p, int lev, bool pole, int rk,
+                                          const double DivM[3], const double nflxv_in[3], double nflxr, double nflxp,
+                                          double pv0, double M0x, double M0y, double M0z) {
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cv = g.Cp - g.Rd;
+    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const double nflxv[3] = {nflxv_in[0], nflxv_in[1], nflxv_in[2]};
+    const auto *Adv = FS<MX>::Adv(f);
+    double advx = VEC(Adv, 0, lev, p), advy = VEC(Adv, 1, lev, p), advz = VEC(Adv, 2, lev, p);
+    const double rho = LV(f.Rhok, lev);
+    double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
+    if (lev != 0) {
+        const double ax = VEC(Adv, 0, lev - 1, p), ay = VEC(Adv, 1, lev - 1, p), az = VEC(Adv, 2, lev - 1, p);
+        advrl = ax * fx + ay * fy + az * fz;
+        rhol = LV(f.Rhok, lev - 1);
+        altl = Alt[lev - 1];
+        pressurel = LV(f.pk, lev - 1);
+    }
+    double advr = advx * fx + advy * fy + advz * fz;
+    advx += -advr * fx;
+    advy += -advr * fy;
+    advz += -advr * fz;
+    const double altht = Alth[lev + 1], althl = Alth[lev], alt = Alt[lev];
+    if (!g.nonhydro) {
+        advr = (XPOW(M0x, 2) + XPOW(M0y, 2) + XPOW(M0z, 2)) / (rho * (alt + A));
+        if (lev != 0) {
+            const double m1x = VEC(f.Mhk, 0, lev - 1, p), m1y = VEC(f.Mhk, 1, lev - 1, p),
+                         m1z = VEC(f.Mhk, 2, lev - 1, p);
+            advrl = (XPOW(m1x, 2) + XPOW(m1y, 2) + XPOW(m1z, 2)) / (rhol * (altl + A));
+        }
+    }
+    const double dz = altht - althl;
+    double r2p, r2m, r2l, dr2dz;
+    if (g.deep) {
+        r2p = XPOW(altht + A, 2.0);
+        r2m = XPOW(alt + A, 2.0);
+        r2l = XPOW(althl + A, 2.0);
+        dr2dz = (XPOW(altht + A, 3.0) - XPOW(althl + A, 3.0)) / 3;
+    } else {
+        r2p = r2m = r2l = 1.0;
+        dr2dz = r2m * dz;
     }
     double whl, wht, hhl, hht;
     if (lev == 0) {
@@ -1316,7 +1341,7 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         const auto *diffw = FS<MX>::diffw(f);
         if (g.nonhydro) swr += (-LV(diffw, lev - 1) - 0.0) * intt + (-LV(diffw, lev) - 0.0) * intl;
         const double dzl = alt - altl;
-        const double dpdz = (pv(0) - pressurel) / dzl;
+        const double dpdz = (pv0 - pressurel) / dzl;
         swr += dpdz;
         swr -= LV(f.dWh, lev);
         LV(f.SlowWh, lev) = -swr;
@@ -1347,7 +1372,7 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         VEC(Mhs, 1, lev, p) = My - aux * fy;
         VEC(Mhs, 2, lev, p) = Mz - aux * fz;
         LV(f.Rhos, lev) = LV(f.Rho, lev) - rho;
-        LV(f.ps, lev) = LV(f.p, lev) - pv(0);
+        LV(f.ps, lev) = LV(f.p, lev) - pv0;
         LV(f.Whs, lev) = lev == 0 ? 0.0 : LV(f.Wh, lev) - LV(f.Whk, lev);
         if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
     }
@@ -1447,13 +1472,15 @@ template <bool MX> __global__ STENCIL_BOUNDS void k_divm0(Geo g, Fields f) {
 
 // ===================================================================== K_F
 // Divergence-damping pass 1 on the deviations + Momentum_Eq / _Poles
-// (thor_fastmodes.h:54-253).
+// (thor_fastmodes.h:54-253). momentum_tail: the update of one cell from its
+// pressure-gradient sums nf3 and DivM.
+template <bool MX>
+__device__ __forceinline__ void momentum_tail(const Geo &g, const Fields &f, int p, int lev, const double nf3[3],
+                                              const double DivM[3], double dt);
 template <bool MX, class GetG, class GetS>
 __device__ __forceinline__ void momentum_point(const Geo &g, const Fields &f, int p, int lev, int nf, bool pole,
                                                int divdamp, double dt, GetG dg, GetS pv) {
     const size_t S = g.S;
-    const int nv = g.nv;
-    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
     double DivM[3] = {0.0, 0.0, 0.0};
     if (divdamp) {
         if constexpr (MX) {
@@ -1474,6 +1501,15 @@ SLOT_UNROLL(THOR_SLOT_UNROLL)
         for (int s = 1; s < 7; s++) s0 = s0 + g.grad[(size_t)(s * 3 + k) * S + p] * pv(s);
         nf3[k] = rs * s0;
     }
+    momentum_tail<MX>(g, f, p, lev, nf3, DivM, dt);
+}
+
+template <bool MX>
+__device__ __forceinline__ void momentum_tail(const Geo &g, const Fields &f, int p, int lev, const double nf3[3],
+                                              const double DivM[3], double dt) {
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
     const auto *SlowMh = FS<MX>::SlowMh(f);
     double Mx = (-nf3[0] + VEC(SlowMh, 0, lev, p) + DivM[0]) * dt;
     double My = (-nf3[1] + VEC(SlowMh, 1, lev, p) + DivM[1]) * dt;
