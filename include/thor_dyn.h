@@ -278,6 +278,19 @@ thor_status thor_dyn_sync(thor_dyn *h);
  * per-cell values summed in the reference's tree order). Synchronous. */
 thor_status thor_dyn_globdiag(thor_dyn *h, double out[5]);
 
+/* thor_dyn_globdiag without allocation (no counterpart in the reference): for
+ * callers that cannot allocate memory in the call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The block
+ * partial sums and the five totals go to a fixed device scratch of the
+ * library (164 KiB per GPU, held until the process ends), the totals are
+ * summed on the device in the same order as thor_dyn_globdiag, so `out` holds
+ * the same bits; nothing is allocated on the device or the host. Limit:
+ * point_num * nv <= 8 388 608 cells (THOR_ERR_UNSUPPORTED beyond). Call with
+ * the handle's device current. All handles on one GPU share the scratch:
+ * calls that overlap on one GPU (from several threads or streams) race.
+ * Synchronous. */
+thor_status thor_dyn_globdiag_noalloc(thor_dyn *h, double out[5]);
+
 /* Copies a named field to / from the host in THOR's layout. Names: the state
  * ("Mh","Wh","W","Rho","pressure"), diagnostics of the last step
  * ("temperature","pt","h","hh","pth","gtil","gtilh"; "Etotal", the total

@@ -175,6 +175,7 @@ def lib(variant: str = "") -> C.CDLL:
             L.thor_dyn_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
             L.thor_dyn_sync.argtypes = [C.c_void_p]
             L.thor_dyn_globdiag.argtypes = [C.c_void_p, _dp]
+            L.thor_dyn_globdiag_noalloc.argtypes = [C.c_void_p, _dp]
             L.thor_dyn_get_field.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
             L.thor_dyn_set_field.argtypes = [C.c_void_p, C.c_char_p, _dp, C.c_long]
             L.thor_dyn_kernel_timing.argtypes = [C.c_void_p, C.c_int]
@@ -402,6 +403,12 @@ class Dyn:
     def globdiag(self) -> np.ndarray:
         out = np.zeros(5)
         self._chk(self.L.thor_dyn_globdiag(self._h, _ptr(out)))
+        return out
+
+    def globdiag_noalloc(self) -> np.ndarray:
+        """globdiag through the allocation-free entry point (device scratch)."""
+        out = np.zeros(5)
+        self._chk(self.L.thor_dyn_globdiag_noalloc(self._h, _ptr(out)))
         return out
 
     def kernel_timing(self, enable: bool):

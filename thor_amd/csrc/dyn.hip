@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -37,6 +38,35 @@ struct KernelStat {
     double bytes = 0.0;  // sum over launches of algorithmic bytes
 };
 
+// Device scratch of thor_dyn_globdiag_noalloc, one copy per GPU, held until
+// the process ends: the per-block partial sums of the five diagnostics
+// (kGdMaxBlocks blocks of 2048 cells, so at most 8 388 608 cells per handle)
+// followed by the five totals. Host code uses the address that
+// hipGetSymbolAddress returns for the current device (gd_scratch_address),
+// never the array's name.
+constexpr int kGdMaxBlocks = 4096;
+constexpr size_t kGdTotals = (size_t)5 * kGdMaxBlocks;
+static __device__ __align__(256) double gd_scratch[kGdTotals + 8];
+
+// The address of gd_scratch on the current device, resolved once per device
+// (called by thor_dyn_create with the handle's device current).
+static hipError_t gd_scratch_address(int dev, double **out) {
+    static std::mutex mu;
+    static std::map<int, double *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(dev);
+    if (it != addr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    double *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(gd_scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = ptr;
+    *out = ptr;
+    return hipSuccess;
+}
+
 struct Pending {
     std::string name;
     hipEvent_t a, b;
@@ -56,6 +86,8 @@ struct Dyn {
     Fields f{};
     double *partials = nullptr;
     int npart = 0;
+    int device = 0;                  // the HIP device the handle was created on
+    double *gd_scratch = nullptr;    // that device's address of gd_scratch (thor_dyn_globdiag_noalloc)
     bool timing = false;
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
@@ -747,6 +779,11 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         || hipEventCreateWithFlags(&d.ev_in, hipEventDisableTiming) != hipSuccess
         || hipEventCreateWithFlags(&d.ev_out, hipEventDisableTiming) != hipSuccess)
         return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: stream"));
+    if (hipGetDevice(&d.device) != hipSuccess)
+        return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: hipGetDevice"));
+    if (const hipError_t e = gd_scratch_address(d.device, &d.gd_scratch); e != hipSuccess)
+        return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: hipGetSymbolAddress(gd_scratch): ")
+                                                   + hipGetErrorString(e)));
 
     // Geometry in level-planar layout.
     std::vector<int> ring((size_t)6 * S, 0), nfc(S, 0);
@@ -1177,6 +1214,41 @@ extern "C" thor_status thor_dyn_globdiag(thor_dyn *h, double out[5]) {
         for (int b = 0; b < h->npart; b++) s += part[(size_t)q * h->npart + b];
         out[q] = s;
     }
+    return THOR_OK;
+}
+
+// Totals of thor_dyn_globdiag_noalloc: thread q sums quantity q's block
+// partials one after the other in block order, the host loop of
+// thor_dyn_globdiag, so the totals carry the same bits.
+__global__ void k_globdiag_totals(const double *partials, int nblk, double *totals) {
+    const int q = threadIdx.x;
+    if (q >= 5) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += partials[(size_t)q * nblk + b];
+    totals[q] = s;
+}
+
+extern "C" thor_status thor_dyn_globdiag_noalloc(thor_dyn *h, double out[5]) {
+    using namespace thor;
+    if (!h || !out) return fail(THOR_ERR_BAD_ARG, "thor_dyn_globdiag_noalloc: bad argument");
+    if (h->npart > kGdMaxBlocks)
+        return fail(THOR_ERR_UNSUPPORTED, "thor_dyn_globdiag_noalloc: " + std::to_string((long)h->P * h->nv)
+                                              + " cells exceed the static scratch's limit of "
+                                              + std::to_string((long)kGdMaxBlocks * 2048)
+                                              + " (use thor_dyn_globdiag)");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != h->device)
+        return fail(THOR_ERR_BAD_ARG, "thor_dyn_globdiag_noalloc: the current device is not the handle's");
+    // every partial and total is overwritten below, so nothing is cleared
+    Fields f = stage_fields(*h, 0);
+    const long n = (long)h->P * h->nv;
+    double *part = h->gd_scratch, *totals = h->gd_scratch + kGdTotals;
+    k_globdiag<<<h->npart, 1024, 0, h->stream>>>(h->g, f, part, n);
+    k_globdiag_totals<<<1, 64, 0, h->stream>>>(part, h->npart, totals);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, totals, 5 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
     return THOR_OK;
 }
 
