@@ -413,10 +413,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                    [&] { k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc); });
         } else {
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
-                   [&] {
-                       if (ml) k_slow_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f, divdamp, rk);
-                       else k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk);
-                   });
+                   [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
         }
         if (rk > 0) rk_arrays(0);
         auto divm0 = [&] {

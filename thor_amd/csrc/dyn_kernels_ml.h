@@ -21,6 +21,20 @@
 
 namespace thor {
 
+// Minimum resident blocks per CU requested for the substep kernels (momentum,
+// divm0, prepare) and for the once-per-stage kernels (adv_diff1, diff2_divm0):
+// bounds their register use against their LDS use. (The slow modes stay one
+// level per block: over two levels their register need spills, 463 against
+// 314 us at g6 L48.)
+#ifndef THOR_ML_MINB_SUB
+#define THOR_ML_MINB_SUB 2
+#endif
+#ifndef THOR_ML_MINB_STAGE
+#define THOR_ML_MINB_STAGE 2
+#endif
+#define ML_SUB_BOUNDS __launch_bounds__(256, THOR_ML_MINB_SUB)
+#define ML_STAGE_BOUNDS __launch_bounds__(256, THOR_ML_MINB_STAGE)
+
 // Block -> (tile, first level): XCD-major like block_tile_lc.
 template <int NL> __device__ __forceinline__ bool block_tile_ml(const Geo &g, int &r, int &l0) {
     const int i = blockIdx.x, xcd = i & 7, idx = i >> 3;
@@ -248,7 +262,7 @@ __device__ __forceinline__ void divm_ml(const Geo &g, int p, int l0, int nf, Get
 // Grid: 8 * tiles_per_xcd * ceil(nv / NL) blocks of 256 threads.
 
 // Divergence-damping pass 0 on the deviations (k_divm0, FP64).
-template <int NL> __global__ STENCIL_BOUNDS void k_divm0_ml(Geo g, Fields f) {
+template <int NL> __global__ ML_SUB_BOUNDS void k_divm0_ml(Geo g, Fields f) {
     int r, l0;
     if (!block_tile_ml<NL>(g, r, l0)) return;
     const size_t S = g.S;
@@ -293,7 +307,7 @@ template <int NL> __global__ STENCIL_BOUNDS void k_divm0_ml(Geo g, Fields f) {
 }
 
 // Divergence-damping pass 1 + Momentum_Eq (k_momentum, FP64).
-template <int NL> __global__ STENCIL_BOUNDS void k_momentum_ml(Geo g, Fields f, int divdamp, double dt) {
+template <int NL> __global__ ML_SUB_BOUNDS void k_momentum_ml(Geo g, Fields f, int divdamp, double dt) {
     int r, l0;
     if (!block_tile_ml<NL>(g, r, l0)) return;
     const size_t S = g.S;
@@ -352,7 +366,7 @@ template <int NL> __global__ STENCIL_BOUNDS void k_momentum_ml(Geo g, Fields f, 
 
 // Prepare_Implicit_Vertical + the horizontal flux divergences of
 // Density_Pressure_Eqs (k_prepare, FP64): prepare_point_w's sums at NL levels.
-template <int NL> __global__ STENCIL_BOUNDS void k_prepare_ml(Geo g, Fields f) {
+template <int NL> __global__ ML_SUB_BOUNDS void k_prepare_ml(Geo g, Fields f) {
     int r, l0;
     if (!block_tile_ml<NL>(g, r, l0)) return;
     const size_t S = g.S;
@@ -434,7 +448,7 @@ template <int NL> __global__ STENCIL_BOUNDS void k_prepare_ml(Geo g, Fields f) {
 
 // Horizontal advection (Compute_Advec_Cori1) + the first hyperdiffusion pass
 // (k_adv_diff1, FP64).
-template <int NL> __global__ STENCIL_BOUNDS void k_adv_diff1_ml(Geo g, Fields f, int hydiff) {
+template <int NL> __global__ ML_STAGE_BOUNDS void k_adv_diff1_ml(Geo g, Fields f, int hydiff) {
     int r, l0;
     if (!block_tile_ml<NL>(g, r, l0)) return;
     const size_t S = g.S;
@@ -528,7 +542,7 @@ template <int NL> __global__ STENCIL_BOUNDS void k_adv_diff1_ml(Geo g, Fields f,
 // Last hyperdiffusion pass + Correct_Horizontal, and divergence-damping pass 0
 // on the stage values (k_diff2_divm0, FP64).
 template <int NL>
-__global__ STENCIL_BOUNDS void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, int divdamp, double sign) {
+__global__ ML_STAGE_BOUNDS void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, int divdamp, double sign) {
     int r, l0;
     if (!block_tile_ml<NL>(g, r, l0)) return;
     const size_t S = g.S;
@@ -598,91 +612,6 @@ __global__ STENCIL_BOUNDS void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, int
                 for (int k = 0; k < 3; k++) VEC(divg, k, lev, p) = o[l][k];
             }
         }
-    }
-}
-
-// Compute_Slow_Modes + UpdateRK (k_slow, FP64).
-template <int NL> __global__ STENCIL_BOUNDS void k_slow_ml(Geo g, Fields f, int divdamp, int rk) {
-    int r, l0;
-    if (!block_tile_ml<NL>(g, r, l0)) return;
-    const size_t S = g.S;
-    const int nv = g.nv;
-    const double *divg = f.divg;
-    if (r == g.nr) {
-        PoleCtx c;
-        int lev;
-        if (!pole_ctx_ml<NL>(g, l0, c, lev)) return;
-        slow_point<false>(
-            g, f, c.p, lev, 5, true, divdamp, rk, [&](int s, int k) { return VEC(divg, k, lev, c.gi(s)); },
-            [&](int s) { return LVP(f.pk, lev, c.gi(s)); }, [&](int s) { return LVP(f.h, lev, c.gi(s)); },
-            [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); });
-        return;
-    }
-    __shared__ double sG[NL][3][NHL2], sM[NL][3][NHL2], sP[NL][NHL2], sH[NL][NHL2];
-    const TileThread T = tile_thread(g, r);
-    stage_tile_ml<NL>(g, r, T, l0, [&](int l, int pos, int q, int lev) {
-        if (divdamp) {
-#pragma unroll
-            for (int k = 0; k < 3; k++) sG[l][k][pos] = q < 0 ? 0.0 : VEC(divg, k, lev, q);
-        }
-#pragma unroll
-        for (int k = 0; k < 3; k++) sM[l][k][pos] = q < 0 ? 0.0 : VEC(f.Mhk, k, lev, q);
-        sP[l][pos] = q < 0 ? 0.0 : LVP(f.pk, lev, q);
-        sH[l][pos] = q < 0 ? 0.0 : LVP(f.h, lev, q);
-    });
-    __syncthreads();
-    const int p = T.p;
-    double DivM[NL][3];
-#pragma unroll
-    for (int l = 0; l < NL; l++) DivM[l][0] = DivM[l][1] = DivM[l][2] = 0.0;
-    if (divdamp) {
-        auto none = [&](int, int) { return 0.0; };
-        divm_ml<true, NL>(
-            g, p, l0, T.nf, [&](int l, int s, int k) { return sG[l][k][T.pos(s)]; }, none, none, DivM);
-    }
-    double rs[NL], nflxv[NL][3], nflxr[NL], nflxp[NL];
-#pragma unroll
-    for (int l = 0; l < NL; l++) {
-        rs[l] = rscale_at(g, min(l0 + l, nv - 1));
-        nflxr[l] = nflxp[l] = 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-        double sg[NL], sr[NL], sp[NL];
-        const double g0 = g.grad[(size_t)(0 * 3 + k) * S + p], d0 = g.div[(size_t)(0 * 3 + k) * S + p];
-        const int q0 = T.pos(0);
-#pragma unroll
-        for (int l = 0; l < NL; l++) {
-            sg[l] = g0 * sP[l][q0];
-            sr[l] = d0 * sM[l][k][q0];
-            sp[l] = d0 * sM[l][k][q0] * sH[l][q0];
-        }
-#pragma unroll
-        for (int s = 1; s < 7; s++) {
-            const double gd = g.grad[(size_t)(s * 3 + k) * S + p], dd = g.div[(size_t)(s * 3 + k) * S + p];
-            const int qs = T.pos(s);
-#pragma unroll
-            for (int l = 0; l < NL; l++) {
-                sg[l] = sg[l] + gd * sP[l][qs];
-                sr[l] = sr[l] + dd * sM[l][k][qs];
-                sp[l] = sp[l] + dd * sM[l][k][qs] * sH[l][qs];
-            }
-        }
-#pragma unroll
-        for (int l = 0; l < NL; l++) {
-            nflxv[l][k] = rs[l] * sg[l];
-            nflxr[l] += rs[l] * sr[l];
-            nflxp[l] += rs[l] * sp[l];
-        }
-    }
-    const int q0 = T.pos(0);
-    // one level's tail after the other (interleaving them costs registers)
-#pragma unroll
-    for (int l = 0; l < NL; l++) {
-        if (l0 + l < nv)
-            slow_tail<false>(g, f, p, l0 + l, false, rk, DivM[l], nflxv[l], nflxr[l], nflxp[l], sP[l][q0],
-                             sM[l][0][q0], sM[l][1][q0], sM[l][2][q0]);
-        __builtin_amdgcn_sched_barrier(0);
     }
 }
 
