@@ -2,7 +2,8 @@
 
 CPU restatement of the per-column surrogate network of thor_mlp.h in numpy
 FP64: n_in -> n_hidden GELU -> n_hidden GELU -> n_out, mean-squared loss and
-its gradient. With bf16=True every tensor the GPU keeps in BF16 (inputs,
+its gradient. GELU in its tanh form, x sigma(2 c (x + 0.044715 x^3)) with
+c = sqrt(2 / pi) (= 0.5 x (1 + tanh(c (x + 0.044715 x^3)))). With bf16=True every tensor the GPU keeps in BF16 (inputs,
 weight copies, hidden activations, backward operands) is rounded to BF16 at
 the same point (round to nearest even), and the arithmetic is exact FP64, so
 the GPU kernels (FP32 accumulation) must agree to FP32 summation error. With
@@ -13,7 +14,6 @@ Only tests/ load this module.
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
 
 __all__ = ["bf16", "split", "forward", "loss_and_grad"]
 
@@ -38,12 +38,19 @@ def split(p: np.ndarray, n_in: int, nh: int, n_out: int):
     return out
 
 
+_C2 = 2.0 * np.sqrt(2.0 / np.pi)  # 1.5957691216057308
+_A = 0.044715
+
+
 def _gelu(z):
-    return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    u = _C2 * z * (1.0 + _A * z * z)
+    return z / (1.0 + np.exp(-u))
 
 
 def _gelu_grad(z):
-    return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
+    u = _C2 * z * (1.0 + _A * z * z)
+    s = 1.0 / (1.0 + np.exp(-u))
+    return s + z * s * (1.0 - s) * _C2 * (1.0 + 3.0 * _A * z * z)
 
 
 def forward(p, X, dims, bf=True):

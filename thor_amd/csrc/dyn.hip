@@ -129,6 +129,13 @@ struct Dyn {
     SurPar sp{};
     int judge_every = 0;
     int *sur_mask = nullptr, *sur_nflag = nullptr;
+    // out-of-envelope columns of a surrogate call, and the two per-call
+    // counters of that list (call k appends with counter k % 2 and clears the other)
+    int *sur_list = nullptr, *sur_cnt = nullptr;
+    int ncu = 0;  // compute units of the device (persistent surrogate kernel)
+    int rt_list_cb = 4;  // columns per block of the exact RT over the surrogate's fallback list
+    int rt_cb = 32;      // columns per block of the exact RT over all columns (nv <= 63)
+    bool sur_fused = true;  // surrogate ProfX as one kernel (else features, network, heating apart)
     double *qsur = nullptr, *err_part = nullptr;  // [nv][S] surrogate Qheat on judge steps; 3 x nblk partials
     int err_nblk = 0;
     long sur_calls = 0, sur_judged = 0, sur_fallback = 0;
@@ -493,15 +500,21 @@ template <bool MX> static int thor_step_t(Dyn &d) {
 static int thor_step(Dyn &d) { return d.mixed ? thor_step_t<true>(d) : thor_step_t<false>(d); }
 
 
-// Double-gray ProfX over all columns (only = nullptr) or the flagged ones:
-// 32 columns per block up to 63 levels, 16 above (4 LDS planes of
-// (nv + 1) x columns doubles within 64 KiB).
-static void launch_rt(Dyn &d, const Geo &g, const Fields &f, double dt, const int *only) {
-    const int cb = d.nv <= 63 ? 32 : 16;
+// Double-gray ProfX over all columns (list = nullptr) or over the *count
+// columns of list: 32 columns per block up to 63 levels, 16 above (4 LDS
+// planes of (nv + 1) x columns doubles within 64 KiB). The list's length is
+// known only on the device, so its grid covers every column and the blocks
+// past the list's end return at once.
+static void launch_rt(Dyn &d, const Geo &g, const Fields &f, double dt, const int *list = nullptr,
+                      const int *count = nullptr) {
+    const int cb = list ? d.rt_list_cb : (d.nv <= 63 ? d.rt_cb : 16);
     const size_t lds = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
     const dim3 grid((d.P + cb - 1) / cb);
-    if (cb == 32) k_profx_rt<32><<<grid, 256, lds, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, only);
-    else k_profx_rt<16><<<grid, 256, lds, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, only);
+    auto go = [&](auto kern) { kern<<<grid, 256, lds, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, list, count); };
+    if (cb == 32) go(k_profx_rt<32>);
+    else if (cb == 16) go(k_profx_rt<16>);
+    else if (cb == 8) go(k_profx_rt<8>);
+    else go(k_profx_rt<4>);
 }
 
 static int profx(Dyn &d, int forcing) {
@@ -545,10 +558,11 @@ static int profx(Dyn &d, int forcing) {
     } else if (forcing == THOR_FORCING_DOUBLE_GRAY) {
         if (!d.rt_set) return fail(THOR_ERR_BAD_ARG, "profx: RT parameters not set");
         // Wh 2, p, rho in; W, T, Qheat, p out; two flux planes written and read twice
-        launch(d, "profx_double_gray", 14 * B, [&] { launch_rt(d, g, f, dt, nullptr); });
+        launch(d, "profx_double_gray", 14 * B, [&] { launch_rt(d, g, f, dt); });
     } else if (forcing == THOR_FORCING_RT_SURROGATE) {
         if (!d.rt_set || !d.sur) return fail(THOR_ERR_BAD_ARG, "profx: RT surrogate not set");
         const bool judge = d.judge_every > 0 && d.sur_calls % d.judge_every == 0;
+        int *cnt = d.sur_cnt + (d.sur_calls & 1), *cnt_next = d.sur_cnt + ((d.sur_calls & 1) ^ 1);
         d.sur_calls++;
         hipError_t e = mlp::prepare(d.sur, d.P);
         // the network's queued work (training, new parameters) comes first
@@ -558,28 +572,44 @@ static int profx(Dyn &d, int forcing) {
         uint16_t *X = mlp::input(d.sur);
         const float *Y = mlp::output(d.sur);
         mlp::InferWeights iw;
-        if (d.nv <= 64 && mlp::fused_weights(d.sur, &iw)) {
-            // features, network and heating in one kernel
+        if (d.sur_fused && d.nv <= 63 && mlp::fused_weights(d.sur, &iw)) {
+            // features, network and heating in one kernel, one 8-wave block per CU
             launch(d, "surrogate_profx", 10 * B, [&] {
-                // 32 columns per block (16 or 64 measured slower at g7)
-                k_sur_fused<256, 2><<<(d.P + 31) / 32, 256, 0, d.stream>>>(g, f, d.sp, iw, d.sur_mask, d.sur_nflag, dt,
-                                                                          judge ? 0 : 1);
+                const unsigned grid = (unsigned)std::max(1, std::min((d.P + 31) / 32, d.ncu));
+                const int set_p = judge ? 0 : 1;
+                // (n_in tiles, levels per thread) = (kpad / 32, ceil(nv / 16))
+                auto go = [&](auto kern) {
+                    kern<<<grid, mlp::kInferThreads, 0, d.stream>>>(g, f, d.sp, iw, d.sur_mask, d.sur_list, cnt,
+                                                                    cnt_next, d.sur_nflag, dt, set_p);
+                };
+                const int nl = (d.nv + 15) / 16;
+                switch (iw.kpad / 32) {
+                case 1: go(k_sur_persist<1, 1>); break;
+                case 2: go(k_sur_persist<2, 2>); break;
+                case 3: go(k_sur_persist<3, 3>); break;
+                default:
+                    if (nl <= 3) go(k_sur_persist<4, 3>);
+                    else go(k_sur_persist<4, 4>);
+                    break;
+                }
             });
         } else {
         launch(d, "surrogate_features", 6 * B, [&] {
             const size_t lds = (size_t)64 * (2 * d.nv + 1) * sizeof(uint16_t);
-            k_sur_features<<<(d.P + 63) / 64, kTile, lds, d.stream>>>(g, f, d.sp, X, d.sur_mask, d.sur_nflag);
+            k_sur_features<<<(d.P + 63) / 64, kTile, lds, d.stream>>>(g, f, d.sp, X, d.sur_mask, d.sur_list, cnt,
+                                                                       cnt_next, d.sur_nflag);
         });
         // BF16 inputs and weights, 193 kFLOP per column for 81 -> 256 -> 256 -> 40
         launch(d, "surrogate_mlp", 2.0 * d.P * (2 * d.nv + 1 + d.nv) * 2, [&] { mlp::forward_on(d.sur, d.P, d.stream); });
         launch(d, "surrogate_apply", 4 * B, [&] {
-            k_sur_apply<<<gpl, kTile, 0, d.stream>>>(g, f, d.sp, Y, d.sur_mask, dt, judge ? 0 : 1);
+            const size_t lds = (size_t)64 * (d.nv + 1) * sizeof(float);
+            k_sur_apply<<<(d.P + 63) / 64, kTile, lds, d.stream>>>(g, f, d.sp, Y, d.sur_mask, dt, judge ? 0 : 1);
         });
         }
         if (judge) {
             const size_t n1 = (size_t)d.nv * d.S;
             hipMemcpyAsync(d.qsur, f.Qheat, n1 * sizeof(double), hipMemcpyDeviceToDevice, d.stream);
-            launch(d, "profx_double_gray", 14 * B, [&] { launch_rt(d, g, f, dt, nullptr); });
+            launch(d, "profx_double_gray", 14 * B, [&] { launch_rt(d, g, f, dt); });
             k_sur_error<<<d.err_nblk, 256, 0, d.stream>>>(g, f.Qheat, d.qsur, d.sur_mask, (long)n1, d.err_part,
                                                           d.err_part + d.err_nblk, d.err_part + 2 * d.err_nblk);
             std::vector<double> part((size_t)3 * d.err_nblk);
@@ -593,7 +623,8 @@ static int profx(Dyn &d, int forcing) {
             }
             d.sur_judged++;
         } else {
-            launch(d, "profx_double_gray_fallback", 14 * B, [&] { launch_rt(d, g, f, dt, d.sur_mask); });
+            // the exact RT for the columns outside the envelope only
+            launch(d, "profx_double_gray_fallback", 14 * B, [&] { launch_rt(d, g, f, dt, d.sur_list, cnt); });
         }
     } else {
         return fail(THOR_ERR_BAD_ARG, "unknown forcing");
@@ -776,8 +807,9 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         || hipEventCreateWithFlags(&d.ev_in, hipEventDisableTiming) != hipSuccess
         || hipEventCreateWithFlags(&d.ev_out, hipEventDisableTiming) != hipSuccess)
         return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: stream"));
-    if (hipGetDevice(&d.device) != hipSuccess)
-        return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: hipGetDevice"));
+    if (hipGetDevice(&d.device) != hipSuccess
+        || hipDeviceGetAttribute(&d.ncu, hipDeviceAttributeMultiprocessorCount, d.device) != hipSuccess)
+        return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: device attributes"));
     if (const hipError_t e = gd_scratch_address(d.device, &d.gd_scratch); e != hipSuccess)
         return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: hipGetSymbolAddress(gd_scratch): ")
                                                    + hipGetErrorString(e)));
@@ -878,6 +910,15 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     }
     if (const char *e = std::getenv("THOR_PREPARE_SPLIT")) d.prep_split = d.prep_split && std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_FP64_MULTILEVEL")) d.fp64_ml = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_RT_LIST_COLUMNS")) {
+        const int v = std::atoi(e);
+        if (v == 4 || v == 8 || v == 16 || v == 32) d.rt_list_cb = v;
+    }
+    if (const char *e = std::getenv("THOR_RT_COLUMNS")) {
+        const int v = std::atoi(e);
+        if (v == 4 || v == 8 || v == 16 || v == 32) d.rt_cb = v;
+    }
+    if (const char *e = std::getenv("THOR_SURROGATE_FUSED")) d.sur_fused = std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
               && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)
@@ -1132,6 +1173,11 @@ extern "C" thor_status thor_dyn_set_rt_surrogate(thor_dyn *h, thor_mlp *m, const
         if (e == hipSuccess) e = hipMalloc((void **)&h->sur_nflag, sizeof(int));
         if (e == hipSuccess) h->allocs.push_back(h->sur_nflag);
         if (e == hipSuccess) e = hipMemset(h->sur_nflag, 0, sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void **)&h->sur_list, (size_t)h->S * sizeof(int));
+        if (e == hipSuccess) h->allocs.push_back(h->sur_list);
+        if (e == hipSuccess) e = hipMalloc((void **)&h->sur_cnt, 2 * sizeof(int));
+        if (e == hipSuccess) h->allocs.push_back(h->sur_cnt);
+        if (e == hipSuccess) e = hipMemset(h->sur_cnt, 0, 2 * sizeof(int));
         if (e == hipSuccess) e = dalloc(*h, &h->qsur, (size_t)nv * h->S);
         h->err_nblk = (int)(((long)nv * h->S + 255) / 256);
         if (e == hipSuccess) e = dalloc(*h, &h->err_part, (size_t)3 * h->err_nblk);

@@ -3369,15 +3369,31 @@ __device__ __forceinline__ double rt_source(double bb, double bl, double bt, dou
 // arithmetic, so the results are those of the one-thread-per-column form.
 // LDS: 4 planes of (nv + 1) x CB doubles (A: SW transmission -> SW flux,
 // B: LW transmission, C: LW down source -> LW down flux, D: LW up source ->
-// LW up flux). only != nullptr: just the flagged columns (surrogate fallback).
+// LW up flux). list != nullptr: just the *count columns list[0 .. *count)
+// (the surrogate's fallback), block b taking entries b CB .. b CB + CB - 1.
+// The optical-depth exponents n - 1 of the RT are 0 or 1 in the usual set-ups
+// (n_sw = 1, n_lw = 2): those powers are 1 and x exactly, as pow returns them,
+// without the call.
+__device__ __forceinline__ double rt_pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
+    return MPOW(x, y);
+}
+
 template <int CB>
 __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, double time_step, double *insol,
-                                                  double *asr, double *olr, const int *only) {
+                                                  double *asr, double *olr, const int *list, const int *count) {
     extern __shared__ double rts[];
     constexpr int NROW = 256 / CB;
     const int c = threadIdx.x % CB, row = threadIdx.x / CB;
-    const int p = blockIdx.x * CB + c;
-    const bool act = p < g.P && (!only || only[p]);
+    int p = blockIdx.x * CB + c;
+    bool act = p < g.P;
+    if (list) {
+        const int n = *count;
+        if ((int)blockIdx.x * CB >= n) return;  // the whole block: before any barrier
+        act = p < n;
+        p = act ? list[p] : 0;
+    }
     const size_t S = g.S;
     const int nv = g.nv, L = nv + 1;
     double *sA = rts, *sB = rts + L * CB, *sC = rts + 2 * L * CB, *sD = rts + 3 * L * CB;
@@ -3408,11 +3424,11 @@ __global__ __launch_bounds__(256) void k_profx_rt(Geo g, Fields f, RTPar q, doub
         return Tl(lev - 1) * a + Tl(lev) * b;
     };
     auto tau_sw = [&](int lev) {
-        return q.n_sw * q.kappa_sw * MPOW(LV(f.p, lev) / q.ps0, q.n_sw - 1) * LV(f.Rho, lev)
+        return q.n_sw * q.kappa_sw * rt_pow(LV(f.p, lev) / q.ps0, q.n_sw - 1) * LV(f.Rho, lev)
                * (Alth[lev + 1] - Alth[lev]);
     };
     auto tau_lw = [&](int lev) {
-        const double t = q.kappa_lw * (1 + q.n_lw * (1 - q.f_lw) / q.f_lw * MPOW(LV(f.p, lev) / q.ps0, q.n_lw - 1))
+        const double t = q.kappa_lw * (1 + q.n_lw * (1 - q.f_lw) / q.f_lw * rt_pow(LV(f.p, lev) / q.ps0, q.n_lw - 1))
                          * LV(f.Rho, lev) * (Alth[lev + 1] - Alth[lev]);
         return t < 0.0 ? 0.0 : t;
     };
@@ -3507,8 +3523,22 @@ struct SurPar {
 // ProfX does), the normalised inputs [T_k, ln p_k, cos z] of each column staged
 // in LDS and written out as contiguous BF16 rows; a column is outside the
 // training envelope if any of its normalised inputs is.
+// Normalised input of the surrogate, rounded to BF16: the difference from the
+// mean formed in FP64 and rounded to FP32, times the FP32 reciprocal of the
+// scale (the network reads 8 significant bits; FP64 keeps the difference
+// exact where the scale is small against the mean). `out` is raised when the
+// value leaves the training envelope [lo, hi]. ln p is the FP64 logarithm
+// (an FP32 one, an ulp or more from the host's, would move a few per cent of
+// the normalised inputs across a BF16 rounding boundary).
+__device__ __forceinline__ uint16_t sur_input(double x, float mean, float inv_std, float lo, float hi, bool &out) {
+    const float xn = (float)(x - (double)mean) * inv_std;
+    out |= !(xn >= lo && xn <= hi);
+    return mlp::f2bf_dev(xn);
+}
+__device__ __forceinline__ double sur_lnp(double p) { return MLOG(p); }
+
 __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q, uint16_t *X, int *mask,
-                                                      int *nflag) {
+                                                      int *list, int *cnt, int *cnt_next, int *nflag) {
     constexpr int CB = 64, NROW = 4;
     extern __shared__ uint16_t sx[];  // [CB][nin]
     __shared__ int sout[CB];
@@ -3521,9 +3551,7 @@ __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q,
     __syncthreads();
     bool out = false;
     auto put = [&](int j, double x) {
-        const float xn = (float)((x - q.in_mean[j]) / q.in_std[j]);
-        out |= !(xn >= q.in_lo[j] && xn <= q.in_hi[j]);
-        sx[c * nin + j] = mlp::f2bf(xn);
+        sx[c * nin + j] = sur_input(x, q.in_mean[j], 1.0f / q.in_std[j], q.in_lo[j], q.in_hi[j], out);
     };
     if (p < g.P) {
         for (int lev = row; lev < nv; lev += NROW) {
@@ -3535,7 +3563,7 @@ __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q,
             const double T = pr / (g.Rd * LV(f.Rho, lev));
             LV(f.T, lev) = T;
             put(lev, T);
-            put(nv + lev, MLOG(pr));
+            put(nv + lev, sur_lnp(pr));
         }
         if (row == 0) {
             double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
@@ -3547,119 +3575,190 @@ __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q,
     __syncthreads();
     const int ncols = min(CB, g.P - p0);
     for (int i = threadIdx.x; i < ncols * nin; i += 256) X[(size_t)p0 * nin + i] = sx[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cnt_next = 0;
     if ((int)threadIdx.x < ncols) {
         const int o = sout[threadIdx.x];
         mask[p0 + threadIdx.x] = o;
-        if (o) atomicAdd(nflag, 1);
+        if (o) {
+            atomicAdd(nflag, 1);
+            list[atomicAdd(cnt, 1)] = p0 + threadIdx.x;
+        }
     }
 }
 
 // Qheat from the network's outputs Y[p][lev] (with the RT's positivity clip),
 // for the columns inside the envelope; with set_p, also Compute_pressure
 // p = T Rd rho for them (the exact RT does it for the others).
-// The whole surrogate ProfX for 32 columns per 256-thread block (32 columns x
-// 8 level groups): ProfX's W from Wh and T, the normalised inputs [T_k, ln p_k,
-// cos z] staged in LDS as BF16 with the envelope flag of each column, the
-// three MFMA layers (mlp3_rows), and Qheat = y out_std + out_mean with the
-// reference's pressure clip for the columns inside the envelope; set_p: also
-// p = rho Rd T, which closes ProfX (not on judge calls, where the exact RT
-// follows). The same values as k_sur_features + the inference kernel +
-// k_sur_apply. Needs nv <= 64 (the network's 64 padded outputs).
-template <int NH, int RT = 2>
-__global__ __launch_bounds__(256) void k_sur_fused(Geo g, Fields f, SurPar q, mlp::InferWeights w, int *mask,
-                                                   int *nflag, double time_step, int set_p) {
-    constexpr int R = 16 * RT, LDH = NH + 8;  // R columns per block
+// The whole surrogate ProfX, one block of 8 waves per CU looping over groups
+// of 32 columns, with the network's weights in the waves' registers for the
+// whole kernel (mlp::load_wave_weights). Per group (32 columns x 16 level rows
+// of threads, up to KT1 levels each, loaded one group ahead): ProfX's W from Wh
+// and T, the normalised inputs [T_k, ln p_k, cos z] staged in LDS as BF16
+// with the envelope flag of each column, the three MFMA layers (mlp3_rows),
+// and Qheat = y out_std + out_mean with the reference's pressure clip for the
+// columns inside the envelope; set_p: also p = rho Rd T, which closes ProfX
+// (not on judge calls, where the exact RT follows). Columns outside the
+// envelope are flagged in mask and appended to list (count in *cnt), for the
+// exact RT to take them; *cnt_next, the next call's counter, is cleared. The
+// normalisation and the level interpolation weights are staged in LDS once.
+// The same values as k_sur_features + the inference kernel + k_sur_apply.
+// Needs nv <= 63 (n_in <= 128, the network's 64 padded outputs). KT1 = kpad / 32;
+// NL = ceil(nv / 16) levels per thread (NL <= KT1).
+template <int KT1, int NL>
+__global__ __launch_bounds__(mlp::kInferThreads, 1) void k_sur_persist(Geo g, Fields f, SurPar q,
+                                                                       mlp::InferWeights w, int *mask, int *list,
+                                                                       int *cnt, int *cnt_next, int *nflag,
+                                                                       double time_step, int set_p) {
+    constexpr int R = 32, LDH = 256 + 8, NROW = mlp::kInferThreads / R;
     __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
     __shared__ int sout[R];
-    const int tid = threadIdx.x, r = tid & (R - 1), grp = tid / R;
-    const int p = blockIdx.x * R + r;
+    __shared__ float sin_[4][128], sout_[2][64];  // in_mean, 1 / in_std, in_lo, in_hi; out_mean, out_std
+    __shared__ double sint[2][64];                // ProfX's W interpolation weights intt, intl per level
+    const int tid = threadIdx.x, r = tid & (R - 1), row = tid / R;
     const size_t S = g.S;
     const int nv = g.nv, nin = 2 * nv + 1, ldx = w.kpad + 8;
-    const double *Alt = g.Alt, *Alth = g.Alth;
-    if (tid < R) sout[tid] = 0;
-    __syncthreads();
-    const bool valid = p < g.P;
-    bool out = false;
-    auto put = [&](int j, double x) {
-        const float xn = (float)((x - q.in_mean[j]) / q.in_std[j]);
-        out |= !(xn >= q.in_lo[j] && xn <= q.in_hi[j]);
-        bufA[r * ldx + j] = mlp::f2bf(xn);
+    const double Rd = g.Rd, Cp = g.Cp;
+    if (blockIdx.x == 0 && tid == 0) *cnt_next = 0;
+    for (int j = tid; j < nin; j += mlp::kInferThreads) {
+        sin_[0][j] = q.in_mean[j];
+        sin_[1][j] = 1.0f / q.in_std[j];
+        sin_[2][j] = q.in_lo[j];
+        sin_[3][j] = q.in_hi[j];
+    }
+    for (int lev = tid; lev < nv; lev += mlp::kInferThreads) {
+        sout_[0][lev] = q.out_mean[lev];
+        sout_[1][lev] = q.out_std[lev];
+        const double xi = g.Alt[lev], xim = g.Alth[lev], xip = g.Alth[lev + 1];
+        sint[0][lev] = (xi - xip) / (xim - xip);
+        sint[1][lev] = (xi - xim) / (xip - xim);
+    }
+    mlp::WaveWeights<KT1> wr;
+    mlp::load_wave_weights<KT1>(w, wr);
+    const int ngroups = (g.P + R - 1) / R;
+    // this thread's levels row, row + NROW, ... of group gi: Wh at both
+    // interfaces, p and rho, loaded one group ahead (the next group's loads
+    // are in flight while this group's network runs)
+    double whl[NL], wht[NL], pr[NL], rho[NL], T[NL];
+    auto load_group = [&](int gi) {
+        const int p = gi * R + r;
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            const int lev = row + k * NROW;
+            const bool on = gi < ngroups && p < g.P && lev < nv;
+            const int lv = on ? lev : 0, pp = on ? p : 0;
+            whl[k] = f.Wh[(size_t)lv * S + pp];
+            wht[k] = f.Wh[(size_t)(lv + 1) * S + pp];
+            pr[k] = f.p[(size_t)lv * S + pp];
+            rho[k] = f.Rho[(size_t)lv * S + pp];
+        }
     };
+    load_group(blockIdx.x);
+    for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
+        const int p = gi * R + r;
+        const bool valid = p < g.P;
+        if (tid < R) sout[tid] = 0;
+        __syncthreads();  // sout cleared (and, first time round, the LDS tables written)
+        bool out = false;
+        auto put = [&](int j, double x) {
+            bufA[r * ldx + j] = sur_input(x, sin_[0][j], sin_[1][j], sin_[2][j], sin_[3][j], out);
+        };
 #pragma unroll
-    for (int j = 0; j < 64 / (256 / R); j++) {  // nv <= 64: at most 8 levels per thread
-        const int lev = grp + j * (256 / R);
-        if (lev >= nv) break;
+        for (int k = 0; k < NL; k++) {
+            const int lev = row + k * NROW;
+            T[k] = 0.0;
+            if (lev >= nv) continue;
+            if (valid) {
+                if (!f.skip_w) LV(f.W, lev) = whl[k] * sint[0][lev] + wht[k] * sint[1][lev];
+                T[k] = pr[k] / (Rd * rho[k]);
+                LV(f.T, lev) = T[k];
+                put(lev, T[k]);
+                put(nv + lev, sur_lnp(pr[k]));
+            } else {
+                bufA[r * ldx + lev] = bufA[r * ldx + nv + lev] = 0;
+            }
+        }
+        if (row == 0) {
+            if (valid) {
+                double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
+                if (coszrs < 0.0) coszrs = 0.0;
+                put(2 * nv, coszrs);
+            } else {
+                bufA[r * ldx + 2 * nv] = 0;
+            }
+            for (int k = nin; k < w.kpad; k++) bufA[r * ldx + k] = 0;
+        }
+        if (out) atomicOr(&sout[r], 1);
+        __syncthreads();
+        // this group's epilogue keeps p, rho and T; the next group's inputs go in flight
+        double pr_c[NL], rho_c[NL];
+#pragma unroll
+        for (int k = 0; k < NL; k++) {
+            pr_c[k] = pr[k];
+            rho_c[k] = rho[k];
+        }
+        load_group(gi + gridDim.x);
+        float y[2][4];
+        mlp::mlp3_rows<KT1>(bufA, bufB, w.kpad, wr, y);
+        // bufB is free once layer 2 is done: stage the outputs there as [R][64] FP32
+        float *sY = reinterpret_cast<float *>(bufB);
+        {
+            const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lq = (lane >> 4) * 4;
+            if (wave < 4) {
+#pragma unroll
+                for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+                    for (int e = 0; e < 4; e++) sY[(16 * ti + li) * 64 + 16 * wave + lq + e] = y[ti][e];
+            }
+        }
+        __syncthreads();
         if (valid) {
-            const double whl = LV(f.Wh, lev), wht = LV(f.Wh, lev + 1);
-            const double xi = Alt[lev], xim = Alth[lev], xip = Alth[lev + 1];
-            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
-            if (!f.skip_w) LV(f.W, lev) = whl * intt + wht * intl;
-            const double pr = LV(f.p, lev);
-            const double T = pr / (g.Rd * LV(f.Rho, lev));
-            LV(f.T, lev) = T;
-            put(lev, T);
-            put(nv + lev, MLOG(pr));
-        } else {
-            bufA[r * ldx + lev] = bufA[r * ldx + nv + lev] = 0;
-        }
-    }
-    if (grp == 0) {
-        if (valid) {
-            double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
-            if (coszrs < 0.0) coszrs = 0.0;
-            put(2 * nv, coszrs);
-        } else {
-            bufA[r * ldx + 2 * nv] = 0;
-        }
-        for (int k = nin; k < w.kpad; k++) bufA[r * ldx + k] = 0;
-    }
-    if (out) atomicOr(&sout[r], 1);
-    __syncthreads();
-    float y[RT][4];
-    mlp::mlp3_rows<NH, RT>(bufA, bufB, w, y);
-    // bufB is free once layer 2 is done: stage the outputs there as [R][64] FP32
-    float *sY = reinterpret_cast<float *>(bufB);
-    {
-        const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = (lane >> 4) * 4;
+            const int o = sout[r];
+            if (!o) {
 #pragma unroll
-        for (int ti = 0; ti < RT; ti++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) sY[(16 * ti + lk + e) * 64 + 16 * wave + li] = y[ti][e];
-    }
-    __syncthreads();
-    if (!valid) return;
-    const int o = sout[r];
-    if (!o) {
-        const double Rd = g.Rd, Cp = g.Cp;
-#pragma unroll
-        for (int j = 0; j < 64 / (256 / R); j++) {
-            const int lev = grp + j * (256 / R);
-            if (lev >= nv) break;
-            double Q = (double)sY[r * 64 + lev] * q.out_std[lev] + q.out_mean[lev];
-            const double pr = LV(f.p, lev);
-            if (pr + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr / time_step;
-            LV(f.Qheat, lev) = Q;
-            if (set_p) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
+                for (int k = 0; k < NL; k++) {
+                    const int lev = row + k * NROW;
+                    if (lev >= nv) continue;
+                    double Q = (double)sY[r * 64 + lev] * sout_[1][lev] + sout_[0][lev];
+                    if (pr_c[k] + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr_c[k] / time_step;
+                    LV(f.Qheat, lev) = Q;
+                    if (set_p) LV(f.p, lev) = T[k] * Rd * rho_c[k];
+                }
+            }
+            if (row == 0) {
+                mask[p] = o;
+                if (o) {
+                    atomicAdd(nflag, 1);
+                    list[atomicAdd(cnt, 1)] = p;
+                }
+            }
         }
-    }
-    if (grp == 0) {
-        mask[p] = o;
-        if (o) atomicAdd(nflag, 1);
+        __syncthreads();  // sY and bufA are read before the next group writes them
     }
 }
 
+// 64 columns x 4 level rows per block: the block's rows of Y (64 nv
+// contiguous floats) are read coalesced into LDS (row stride nv + 1 against
+// bank conflicts), then each thread forms Qheat (and p) for its column at
+// every fourth level, with the level-planar stores coalesced across columns.
 __global__ __launch_bounds__(256) void k_sur_apply(Geo g, Fields f, SurPar q, const float *Y, const int *mask,
                                                    double time_step, int set_p) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lev = blockIdx.y;
-    if (p >= g.P || mask[p]) return;
+    constexpr int CB = 64, NROW = 4;
+    extern __shared__ float sYa[];  // [CB][nv + 1]
+    const int nv = g.nv, ld = nv + 1;
+    const int p0 = blockIdx.x * CB, ncols = min(CB, g.P - p0);
+    for (int i = threadIdx.x; i < ncols * nv; i += 256) sYa[(i / nv) * ld + i % nv] = Y[(size_t)p0 * nv + i];
+    __syncthreads();
+    const int c = threadIdx.x % CB, row = threadIdx.x / CB, p = p0 + c;
+    if (c >= ncols || mask[p]) return;
     const size_t S = g.S;
-    const int nv = g.nv;
-    double Q = (double)Y[(size_t)p * nv + lev] * q.out_std[lev] + q.out_mean[lev];
-    const double pr = LV(f.p, lev), Rd = g.Rd, Cp = g.Cp;
-    if (pr + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr / time_step;
-    LV(f.Qheat, lev) = Q;
-    if (set_p) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
+    const double Rd = g.Rd, Cp = g.Cp;
+    for (int lev = row; lev < nv; lev += NROW) {
+        double Q = (double)sYa[c * ld + lev] * q.out_std[lev] + q.out_mean[lev];
+        const double pr = LV(f.p, lev);
+        if (pr + Rd / (Cp - Rd) * Q * time_step < 0) Q = -pr / time_step;
+        LV(f.Qheat, lev) = Q;
+        if (set_p) LV(f.p, lev) = LV(f.T, lev) * Rd * LV(f.Rho, lev);
+    }
 }
 
 // Judge: squared and absolute differences between the exact RT heating and

@@ -1,5 +1,5 @@
 // Per-column MLP of the radiative-heating surrogate: BF16 GEMMs on the gfx950
-// matrix cores (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, GELU,
+// matrix cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, GELU,
 // mean-squared loss, backward pass and Adam, all as HIP kernels. Training is
 // driven from the host one mini-batch at a time.
 #include "../../include/thor_mlp.h"
@@ -27,11 +27,12 @@ struct Net {
     float *gm = nullptr, *gv = nullptr;  // Adam moments
     float *grad = nullptr;             // gradients, same layout
     uint16_t *wb = nullptr;            // BF16 copies of W1, W2, W3 (biases stay FP32)
-    // Transposed BF16 copies for the fused inference kernel (n_hidden = 256,
-    // n_out <= 64, n_in <= 256 only): W1^T [256][kpad], W2^T [256][256],
-    // W3^T [64][256], zero-padded.
+    // BF16 copies in MFMA operand order for the inference kernels (n_hidden =
+    // 256, n_out <= 64, n_in <= 128 only): W1 [kpad][256], W2 [256][256],
+    // W3 [256][64], zero-padded (k_frag_bf16).
     bool fused = false;
     int kpad = 0;
+    int ncu = 0;  // compute units of the device: inference blocks
     uint16_t *wt1 = nullptr, *wt2 = nullptr, *wt3 = nullptr;
     long step = 0;
     // activations for up to cap rows
@@ -176,7 +177,13 @@ static void infer(Net &n, long rows) {
         return;
     }
     InferArgs a{n.X0, rows, infer_weights(n), n.Y};
-    k_mlp_infer<256><<<(unsigned)((rows + 31) / 32), 256, 0, n.stream>>>(a);
+    const unsigned grid = (unsigned)std::max(1L, std::min((rows + 31) / 32, (long)n.ncu));
+    switch (n.kpad / 32) {
+    case 1: k_mlp_infer<1><<<grid, kInferThreads, 0, n.stream>>>(a); break;
+    case 2: k_mlp_infer<2><<<grid, kInferThreads, 0, n.stream>>>(a); break;
+    case 3: k_mlp_infer<3><<<grid, kInferThreads, 0, n.stream>>>(a); break;
+    default: k_mlp_infer<4><<<grid, kInferThreads, 0, n.stream>>>(a); break;
+    }
 }
 
 static double sum_loss(Net &n, int slots) {
@@ -235,9 +242,14 @@ extern "C" thor_status thor_mlp_create(int n_in, int n_hidden, int n_out, unsign
     if (e == hipSuccess) e = alloc(n, &n.gv, n.np);
     if (e == hipSuccess) e = alloc(n, &n.grad, n.np);
     if (e == hipSuccess) e = alloc(n, &n.wb, n.nwb());
-    n.fused = n_hidden == 256 && n_out <= 64 && n_in <= 256;
+    n.fused = n_hidden == 256 && n_out <= 64 && n_in <= 128;
+    if (e == hipSuccess) {
+        int dev = 0;
+        e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&n.ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    }
     if (n.fused) {
-        n.kpad = (n_in + 15) / 16 * 16;
+        n.kpad = (n_in + 31) / 32 * 32;
         if (e == hipSuccess) e = alloc(n, &n.wt1, (size_t)256 * n.kpad);
         if (e == hipSuccess) e = alloc(n, &n.wt2, (size_t)256 * 256);
         if (e == hipSuccess) e = alloc(n, &n.wt3, (size_t)64 * 256);

@@ -30,20 +30,22 @@ struct GemmArgs {
     const float *bias;
 };
 
+// d/dz of gelu: s + z s (1 - s) u'(z), s = sigma(u), u' = 2 c (1 + 3 0.044715 z^2).
 __device__ inline float gelu_grad(float z) {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) + z * 0.39894228040143268f * expf(-0.5f * z * z);
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-gelu_u(z)));
+    return s + z * s * (1.0f - s) * 1.5957691216057308f * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
 // 64 x 64 output tile per 256-thread block; each of the 4 waves owns a 32 x 32
-// quarter made of 2 x 2 MFMA 16x16x16 tiles. K advances 16 at a time through
-// LDS tiles of A (64 x 16) and B (16 x 64); out-of-range elements load as 0.
-// MFMA 16x16x16 bf16 operand layout: lane l holds A[l % 16][4 (l / 16) + e]
-// and B[4 (l / 16) + e][l % 16], e = 0..3; the result element e of lane l is
-// D[4 (l / 16) + e][l % 16].
+// quarter made of 2 x 2 MFMA 16x16x32 tiles. K advances 32 at a time through
+// LDS tiles of A (64 x 32) and B (32 x 64); out-of-range elements load as 0.
+// MFMA 16x16x32 bf16 operand layout (gfx950): lane l holds A[l % 16][8 (l / 16)
+// + e] and B[8 (l / 16) + e][l % 16], e = 0..7; the result element e of lane l
+// is D[4 (l / 16) + e][l % 16] (e = 0..3).
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-    __shared__ uint16_t sA[64][16 + 4];
-    __shared__ uint16_t sB[16][64 + 4];
+    __shared__ __align__(16) uint16_t sA[64][32 + 8];
+    __shared__ __align__(16) uint16_t sBt[64][32 + 8];  // B transposed: [n][k]
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -52,40 +54,37 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-    const int ar = tid >> 2, ak = (tid & 3) * 4;   // A staging: row, first k
-    const int bk = tid >> 4, bc = (tid & 15) * 4;  // B staging: k, first column
-    const int li = lane & 15, lk = (lane >> 4) * 4;
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
+    const int ar = tid >> 2, ak = (tid & 3) * 8;  // A staging: row, first k (8 per thread)
+    const int bn = tid >> 2, bk = (tid & 3) * 8;  // B staging: column, first k
+    const int li = lane & 15, lk = (lane >> 4) * 8;
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
         {
             const long i = m0 + ar;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
+            for (int e = 0; e < 8; e++) {
                 const long k = k0 + ak + e;
                 sA[ar][ak + e] = (i < g.M && k < g.K) ? g.A[i * g.as_i + k * g.as_k] : (uint16_t)0;
             }
         }
         {
-            const long k = k0 + bk;
+            const long n = n0 + bn;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const long n = n0 + bc + e;
-                sB[bk][bc + e] = (k < g.K && n < g.N) ? g.B[k * g.bs_k + n * g.bs_n] : (uint16_t)0;
+            for (int e = 0; e < 8; e++) {
+                const long k = k0 + bk + e;
+                sBt[bn][bk + e] = (k < g.K && n < g.N) ? g.B[k * g.bs_k + n * g.bs_n] : (uint16_t)0;
             }
         }
         __syncthreads();
-        v4s a[2], b[2];
+        v8s a[2], b[2];
 #pragma unroll
-        for (int t = 0; t < 2; t++)
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                a[t][e] = (short)sA[wm + t * 16 + li][lk + e];
-                b[t][e] = (short)sB[lk + e][wn + t * 16 + li];
-            }
+        for (int t = 0; t < 2; t++) {
+            a[t] = *(const v8s *)&sA[wm + t * 16 + li][lk];
+            b[t] = *(const v8s *)&sBt[wn + t * 16 + li][lk];
+        }
 #pragma unroll
         for (int ti = 0; ti < 2; ti++)
 #pragma unroll
-            for (int tj = 0; tj < 2; tj++)
-                acc[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+            for (int tj = 0; tj < 2; tj++) acc[ti][tj] = mfma32(a[ti], b[tj], acc[ti][tj]);
         __syncthreads();
     }
 #pragma unroll
@@ -100,14 +99,15 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
                 float v = acc[ti][tj][e];
                 if (EPI != EPI_F32) v += g.bias[n];
                 g.C[i * g.ldc + n] = v;
-                if (EPI == EPI_BIAS_GELU) g.H[i * g.ldh + n] = f2bf(gelu(v));
+                if (EPI == EPI_BIAS_GELU) g.H[i * g.ldh + n] = f2bf_dev(gelu(v));
             }
 }
 
-// Fused inference: 32 rows per 256-thread block, all three layers in one
-// kernel with the hidden activations in LDS (mlp3_rows). The block's input
-// rows are one contiguous run of X; each thread loads its share of one row
-// (all loads issued before the first store).
+// Inference over row groups of 32 (mlp3_rows), one 256-thread block per CU
+// looping over the groups: each wave loads its weight slice into registers
+// once, so the layers' B operands come from registers and only the
+// activations pass through LDS. The group's input rows are one contiguous run
+// of X.
 struct InferArgs {
     const uint16_t *X;  // [rows][nin]
     long rows;
@@ -115,54 +115,60 @@ struct InferArgs {
     float *Y;  // [rows][nout]
 };
 
-template <int NH> __global__ __launch_bounds__(256) void k_mlp_infer(InferArgs g) {
-    constexpr int R = 32, LDH = NH + 8;
+template <int KT1> __global__ __launch_bounds__(kInferThreads, 1) void k_mlp_infer(InferArgs g) {
+    constexpr int R = 32, LDH = 256 + 8;
     __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = (lane >> 4) * 4;
-    const long r0 = (long)blockIdx.x * R;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, li = lane & 15, lq = (lane >> 4) * 4;
     const int nin = g.w.nin, kpad = g.w.kpad, ldx = kpad + 8;
-    {
-        const int r = tid >> 3, part = tid & 7, kper = kpad / 8;  // 32 rows x 8 parts, kper <= 32
-        const bool rv = r0 + r < g.rows;
-        uint16_t v[32];
+    WaveWeights<KT1> wr;
+    load_wave_weights<KT1>(g.w, wr);
+    const long ngroups = (g.rows + R - 1) / R;
+    for (long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const long r0 = grp * R;
+        {
+            const int r = tid >> 4, part = tid & 15, kper = kpad / 16;  // 32 rows x 16 parts, kper <= 8
+            const bool rv = r0 + r < g.rows;
+            uint16_t v[8];
 #pragma unroll
-        for (int j = 0; j < 32; j++) {
-            const int k = part * kper + j;
-            v[j] = (j < kper && rv && k < nin) ? g.X[(r0 + r) * nin + k] : (uint16_t)0;
+            for (int j = 0; j < 8; j++) {
+                const int k = part * kper + j;
+                v[j] = (j < kper && rv && k < nin) ? g.X[(r0 + r) * nin + k] : (uint16_t)0;
+            }
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (j < kper) bufA[r * ldx + part * kper + j] = v[j];
         }
+        __syncthreads();
+        float y[2][4];
+        mlp3_rows<KT1>(bufA, bufB, kpad, wr, y);
+        if (wave < 4) {
 #pragma unroll
-        for (int j = 0; j < 32; j++)
-            if (j < kper) bufA[r * ldx + part * kper + j] = v[j];
+            for (int ti = 0; ti < 2; ti++) {
+                const long row = r0 + 16 * ti + li;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int col = 16 * wave + lq + e;
+                    if (row < g.rows && col < g.w.nout) g.Y[row * g.w.nout + col] = y[ti][e];
+                }
+            }
+        }
+        __syncthreads();  // layer 3 has read bufA before the next group's inputs go there
     }
-    __syncthreads();
-    float y[2][4];
-    mlp3_rows<NH>(bufA, bufB, g.w, y);
-    const int col = 16 * wave + li;
-    if (col >= g.w.nout) return;
-#pragma unroll
-    for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const long row = r0 + 16 * ti + lk + e;
-            if (row < g.rows) g.Y[row * g.w.nout + col] = y[ti][e];
-        }
 }
 
-// dst[n][k] = bf16(src[k][n]) for k < K, n < N; zero in the padding up to
-// [Npad][Kpad].
 // BF16 copy of W [K][N] (FP32, row-major) in the order the inference kernels
-// read their MFMA B operands: for each 16 x 16 tile (nt, kt) of W^T, lane l's
-// four values W[16 kt + 4 (l / 16) + e][16 nt + l % 16], e = 0..3, are at
-// ((nt * Kpad / 16 + kt) * 64 + l) * 4 + e, so a wave's operand load is 512
+// read their MFMA B operands: for each 32 x 16 tile (kt, nt) of W, lane l's
+// eight values W[32 kt + 8 (l / 16) + e][16 nt + l % 16], e = 0..7, are at
+// ((nt * Kpad / 32 + kt) * 64 + l) * 8 + e, so a wave's operand load is 1024
 // contiguous bytes. Zero beyond K and N.
 __global__ void k_frag_bf16(const float *src, int K, int N, uint16_t *dst, int Kpad, int Npad) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long)Npad * Kpad) return;
-    const int KT = Kpad / 16;
-    const int e = (int)(i & 3), l = (int)((i >> 2) & 63);
-    const long tile = i >> 8;
+    const int KT = Kpad / 32;
+    const int e = (int)(i & 7), l = (int)((i >> 3) & 63);
+    const long tile = i >> 9;
     const int kt = (int)(tile % KT), nt = (int)(tile / KT);
-    const int n = 16 * nt + (l & 15), k = 16 * kt + 4 * (l >> 4) + e;
+    const int n = 16 * nt + (l & 15), k = 32 * kt + 8 * (l >> 4) + e;
     dst[i] = (n < N && k < K) ? f2bf(src[(size_t)k * N + n]) : (uint16_t)0;
 }
 

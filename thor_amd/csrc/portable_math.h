@@ -111,8 +111,11 @@ PM_FN double exp(double x) {
     return sum * from_bits((uint64_t)(ni + 1023) << 52);
 }
 
-// x^y for x > 0; small integer exponents by repeated multiplication.
+// x^y for x > 0; y = 0 and 1 exactly (as pow does), small integer exponents
+// by repeated multiplication.
 PM_FN double pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    if (y == 1.0) return x;
     if (y == 2.0) return x * x;
     if (y == 3.0) return x * x * x;
     if (y == 4.0) {

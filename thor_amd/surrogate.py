@@ -66,6 +66,14 @@ def features(T: np.ndarray, p: np.ndarray, mu: np.ndarray) -> np.ndarray:
     return np.concatenate([T, np.log(p), mu[:, None]], axis=1)
 
 
+def normalise_inputs(X: np.ndarray, norm: dict) -> np.ndarray:
+    """The normalised inputs as the surrogate kernels form them: the
+    difference from the mean in FP64 rounded to FP32, times the FP32
+    reciprocal of the scale."""
+    inv = np.float32(1.0) / norm["in_std"].astype(np.float32)
+    return (X - norm["in_mean"].astype(np.float64)).astype(np.float32) * inv
+
+
 def collect(dyn: "capi.Dyn", grid, nsteps: int, every: int, rng=None, max_columns=None, lon_sub=0.0):
     """Runs nsteps of (double-gray ProfX + Thor) and samples (inputs, heating)
     of the columns after every `every`-th ProfX. Returns X [n, 2 nv + 1],
@@ -91,10 +99,11 @@ def collect(dyn: "capi.Dyn", grid, nsteps: int, every: int, rng=None, max_column
 
 def normalisation(X: np.ndarray, Y: np.ndarray, margin: float = 0.5) -> dict:
     """Per-feature mean / std of the inputs and outputs, and the envelope
-    [min - margin, max + margin] of the normalised training inputs."""
+    [min - margin, max + margin] of the normalised training inputs. The input
+    scale is at least 1e-3 of the feature's typical magnitude, so that the
+    FP32 rounding of a normalised input stays far below its BF16 resolution."""
     in_mean = X.mean(axis=0)
-    in_std = X.std(axis=0)
-    in_std[in_std == 0] = 1.0
+    in_std = np.maximum(X.std(axis=0), 1e-3 * np.maximum(np.abs(in_mean), 1.0))
     Xn = (X - in_mean) / in_std
     out_mean = Y.mean(axis=0)
     out_std = Y.std(axis=0)
@@ -106,8 +115,7 @@ def normalisation(X: np.ndarray, Y: np.ndarray, margin: float = 0.5) -> dict:
 
 
 def apply_norm(X: np.ndarray, Y: np.ndarray, norm: dict):
-    return ((X - norm["in_mean"]) / norm["in_std"]).astype(np.float32), \
-        ((Y - norm["out_mean"]) / norm["out_std"]).astype(np.float32)
+    return normalise_inputs(X, norm), ((Y - norm["out_mean"]) / norm["out_std"]).astype(np.float32)
 
 
 def train(mlp: "capi.MLP", Xn: np.ndarray, Yn: np.ndarray, epochs: int, batch: int = 512, lr: float = 1e-3,
