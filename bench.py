@@ -12,7 +12,7 @@ precision path (FP32 hyperdiffusion / divergence-damping operators and
 momentum deviation, FP64 elsewhere). `value` is the mixed-precision form's
 throughput; `fp64` and `speedup_vs_fp64` give the FP64 form beside it.
 Held-Suarez has no radiative transfer, so the RT surrogate is not on this
-workload's path.
+workload's path; the learned_rt object below times it on its own workload.
 
 Prints ONE JSON line. Extra objects:
   roofline       - dominant kernel of the headline form: algorithmic bytes per
@@ -22,11 +22,19 @@ Prints ONE JSON line. Extra objects:
                    "traffic" = HBM bytes per launch from the committed
                    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (profiles/hbm_traffic_*.json);
                    "step" = the whole large step's algorithmic bytes / wall time;
-  error_vs_fp64  - both forms run --error-days simulated days from the same
-                   state; RMS / max differences of T, u, v, w and surface
-                   pressure between them;
-  cpu_baseline   - the serial FP64 oracle (THOR's kernels as a host loop) timed
-                   on one host core for one step of the same workload.
+  error_vs_fp64  - FP64, mixed and a one-ulp-perturbed FP64 twin run
+                   --error-days simulated days from the same state; each day
+                   the RMS / max differences of T, u, v, w and surface
+                   pressure (point-wise, and of the zonal means of T and u)
+                   of the mixed form from FP64, beside the perturbed twin's
+                   (the chaotic spread precision cannot be told apart from);
+  cpu_baseline   - the FP64 oracle (THOR's kernels as a host loop) timed on
+                   the host's threads (and on one) for steps of the same
+                   workload;
+  learned_rt     - the learned-RT workload (hot Jupiter g5 L40, double-gray
+                   RT): 50-day exact-RT spin-up, GPU training of the MLP,
+                   ProfX and step timing of FP64 + exact RT, mixed + exact RT
+                   and mixed + MLP (judge every 10 calls), judge error.
 """
 from __future__ import annotations
 
@@ -128,6 +136,16 @@ def roofline(run, workload):
         "step": {"alg_bytes": round(step_bytes), "achieved": round(step_gbs, 1),
                  "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
     }
+
+
+def _host_cpu() -> str:
+    """Model name and logical CPU count of the host (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            names = [ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")]
+        return f"{names[0]} x {len(names)} logical CPUs (affinity: {len(os.sched_getaffinity(0))})"
+    except (OSError, IndexError):
+        return "unknown"
 
 
 def diagnostics(g, p, st):
@@ -370,35 +388,68 @@ def at_scale_run(capi, args):
             **errs}
 
 
+def _zonal_means(g, x, nbands=36):
+    """Zonal means of a [P, nv] (or [P]) field in latitude bands of equal width."""
+    lat = np.asarray(g.lonlat).reshape(g.P, 2)[:, 1]
+    band = np.minimum(((lat + np.pi / 2) / np.pi * nbands).astype(int), nbands - 1)
+    cnt = np.bincount(band, minlength=nbands).astype(np.float64)
+    if x.ndim == 1:
+        return np.bincount(band, weights=x, minlength=nbands) / np.maximum(cnt, 1)
+    return np.stack([np.bincount(band, weights=x[:, k], minlength=nbands) for k in range(x.shape[1])], axis=1) \
+        / np.maximum(cnt, 1)[:, None]
+
+
+def _diffs(g, a, b):
+    """RMS and max differences of T, u, v, w and ps between two diagnostic
+    sets, point-wise and of the zonal means of T and u."""
+    out = {}
+    for k in ("T", "u", "v", "w", "ps"):
+        d = b[k] - a[k]
+        out[k] = {"rms": float(f"{np.sqrt((d ** 2).mean()):.4e}"), "max": float(f"{np.abs(d).max():.4e}")}
+    for k in ("T", "u"):
+        d = _zonal_means(g, b[k]) - _zonal_means(g, a[k])
+        out[k + "_zonal"] = {"rms": float(f"{np.sqrt((d ** 2).mean()):.4e}"), "max": float(f"{np.abs(d).max():.4e}")}
+    return out
+
+
 def error_run(capi, g, p64, pmx, st, args):
+    """Three runs of --error-days from the same state: FP64, the mixed form,
+    and FP64 from the state with every pressure one ulp up. Each simulated
+    day, the mixed form's distance from FP64 is set beside the perturbed FP64
+    run's (the flow's own sensitivity to a last-bit change: what precision
+    alone can be told apart from)."""
     nsteps = int(round(args.error_days * 86400.0 / args.dt))
-    ds = [capi.Dyn(g, q, variant=args.variant) for q in (p64, pmx)]
-    for d in ds:
-        d.set_state(st)
-    chunk = 240
+    per_day = int(round(86400.0 / args.dt))
+    st_p = dict(st)
+    st_p["pressure"] = np.nextafter(st["pressure"], np.inf)
+    ds = [capi.Dyn(g, q, variant=args.variant) for q in (p64, pmx, p64)]
+    for d, s0 in zip(ds, (st, st, st_p)):
+        d.set_state(s0)
     t0 = time.perf_counter()
-    done = 0
+    done, days = 0, []
     while done < nsteps:
-        n = min(chunk, nsteps - done)
+        n = min(per_day, nsteps - done)
         for d in ds:
             d.run(n)
         for d in ds:
             d.sync()
         done += n
-        log(f"error run: {done}/{nsteps} steps ({done * args.dt / 86400:.1f} days, "
-            f"{time.perf_counter() - t0:.1f} s)")
+        a, b, c = (diagnostics(g, q, d.get_state()) for q, d in zip((p64, pmx, p64), ds))
+        days.append({"day": round(done * args.dt / 86400.0, 3), "mixed": _diffs(g, a, b), "fp64_ulp": _diffs(g, a, c)})
+        log(f"error run: day {days[-1]['day']:g}: T rms mixed {days[-1]['mixed']['T']['rms']:.3e}, "
+            f"1-ulp FP64 {days[-1]['fp64_ulp']['T']['rms']:.3e} K ({time.perf_counter() - t0:.1f} s)")
     a = diagnostics(g, p64, ds[0].get_state())
-    b = diagnostics(g, pmx, ds[1].get_state())
     gd = [d.globdiag() for d in ds]
     for d in ds:
         d.close()
-    out = {"days": args.error_days, "steps": nsteps}
-    for k in ("T", "u", "v", "w", "ps"):
-        diff = b[k] - a[k]
-        out[k] = {"rms": float(f"{np.sqrt((diff ** 2).mean()):.4e}"), "max": float(f"{np.abs(diff).max():.4e}"),
-                  "fp64_rms": float(f"{np.sqrt((a[k] ** 2).mean()):.4e}")}
-    out["units"] = {"T": "K", "u": "m/s", "v": "m/s", "w": "m/s", "ps": "Pa"}
-    out["mass_rel_diff"] = float(f"{(gd[1][0] - gd[0][0]) / gd[0][0]:.3e}")
+    last = days[-1]
+    out = {"days": args.error_days, "steps": nsteps, "final": last,
+           "fp64_rms": {k: float(f"{np.sqrt((a[k] ** 2).mean()):.4e}") for k in ("T", "u", "v", "w", "ps")},
+           "mixed_within_ulp_spread": {k: bool(last["mixed"][k]["rms"] <= last["fp64_ulp"][k]["rms"])
+                                       for k in ("T", "u", "v", "w", "ps", "T_zonal", "u_zonal")},
+           "per_day": days,
+           "units": {"T": "K", "u": "m/s", "v": "m/s", "w": "m/s", "ps": "Pa"},
+           "mass_rel_diff": float(f"{(gd[1][0] - gd[0][0]) / gd[0][0]:.3e}")}
     return out
 
 
@@ -417,6 +468,8 @@ def main():
     ap.add_argument("--error-days", type=float, default=10.0,
                     help="simulated days of the FP64-vs-mixed error run (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-rt", action="store_true",
+                    help="skip the learned-RT leg (hot Jupiter g5 L40: exact RT vs the trained surrogate)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="first run a plain device-to-device copy of this many MiB (three times); "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE against known bytes")
@@ -477,15 +530,28 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        from oracle import oracle as orc
         from oracle.oracle import Oracle
+        # the oracle's block and column loops on every host thread (same bits
+        # as the serial loop), then one thread for the serial figure
+        nthr = orc.set_threads(0, args.variant)
         o = Oracle(g, p64, variant=args.variant)
         o.set_state(st)
         tc = time.perf_counter()
-        o.steps(1)
-        cpu_s = time.perf_counter() - tc
-        cpu = {"value": round(args.dt / cpu_s, 3), "unit": "sim-days/wall-day", "cores": 1, "kind": "port",
-               "sample": f"1 step (ProfX + Thor) of the same g{args.glevel} L{args.nv} workload, serial FP64 "
-                         f"oracle on 1 host core: {cpu_s:.2f} s"}
+        o.steps(2)
+        cpu_s = (time.perf_counter() - tc) / 2
+        orc.set_threads(1, args.variant)
+        o1 = Oracle(g, p64, variant=args.variant)
+        o1.set_state(st)
+        tc = time.perf_counter()
+        o1.steps(1)
+        cpu1_s = time.perf_counter() - tc
+        orc.set_threads(nthr, args.variant)
+        cpu = {"value": round(args.dt / cpu_s, 3), "unit": "sim-days/wall-day", "cores": nthr, "kind": "port",
+               "sample": f"2 steps (ProfX + Thor) of the same g{args.glevel} L{args.nv} workload, FP64 oracle "
+                         f"(THOR's kernels as a host loop) on {nthr} host threads: {cpu_s:.2f} s per step; "
+                         f"1 thread: {cpu1_s:.2f} s per step",
+               "value_1_thread": round(args.dt / cpu1_s, 3), "host_cpu": _host_cpu()}
         d2 = capi.Dyn(g, p64, variant=args.variant)
         d2.set_state(st)
         d2.run(1)
@@ -518,6 +584,9 @@ def main():
             out["speedup_vs_fp64"] = round(runs["fp64"]["ms_step"] / hr["ms_step"], 3)
     out["error_vs_fp64"] = err
     out["cpu_baseline"] = cpu
+    if not args.no_rt:
+        log("learned-RT leg (hot Jupiter g5 L40):")
+        out["learned_rt"] = rt_surrogate_run(capi, args)
     print(json.dumps(out), flush=True)
 
 
