@@ -125,3 +125,51 @@ def test_spring_dynamics_converged(built):
     # pentagons stay fixed at the icosahedron's vertices
     zp = np.abs(g4.xyz[g4.pent_ind[:10], 2]) / A
     assert np.allclose(zp, 1 / np.sqrt(5), atol=1e-9)
+
+
+def _rotation(angle, flip=False):
+    c, s = np.cos(angle), np.sin(angle)
+    return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, -1.0 if flip else 1.0]])
+
+
+@pytest.mark.parametrize("angle,flip", [(2 * np.pi / 5, False), (np.pi / 5, True)])
+def test_icosahedral_symmetry(g, angle, flip):
+    """The construction (lattice bisection, spring dynamics, centroid
+    relocation, control areas) commutes with the icosahedron's symmetries
+    about the polar axis: a 72-degree rotation and a 36-degree rotation
+    combined with z -> -z map the point set onto itself and carry every
+    control area with it, to round-off. A wrong neighbour, lattice index or
+    force term breaks the symmetry at the size of the grid spacing."""
+    from scipy.spatial import cKDTree
+
+    x = g.xyz / A
+    dist, idx = cKDTree(x).query(x @ _rotation(angle, flip).T)
+    assert len(set(idx.tolist())) == g.P
+    assert dist.max() < 1e-12
+    assert np.abs(g.areasT[idx] - g.areasT).max() < 1e-10 * g.areasT.max()
+
+
+def _normproj(v1, v2):
+    """Great-circle arc from v2 to v1 as (unit normal) x (angle)."""
+    c = np.cross(v1, v2)
+    l2 = np.linalg.norm(c, axis=-1, keepdims=True)
+    return c / l2 * np.arctan2(l2[..., 0], np.einsum("...k,...k->...", v1, v2))[..., None]
+
+
+def test_centres_are_control_volume_centroids(g):
+    """After spring dynamics every point is moved to the spherical centroid of
+    its control volume (grid.cu:1838-1900): the normalised sum of the arcs
+    between consecutive corners xyzq, in corner order."""
+    q = g.xyzq / A
+    pent = np.zeros(g.P, bool)
+    pent[g.pent_ind] = True
+    s = np.zeros((g.P, 3))
+    for j in range(1, 6):
+        arc = _normproj(q[:, j], q[:, j - 1])
+        s += np.where((pent & (j == 5))[:, None], 0.0, arc)
+    last = np.where(pent, 4, 5)
+    s += _normproj(q[:, 0], q[np.arange(g.P), last])
+    want = s / np.linalg.norm(s, axis=1, keepdims=True)
+    # (both arrays were scaled by A and back, and the short arcs cancel in
+    # their cross products: round-off; a wrong corner moves it by ~1e-3)
+    assert np.abs(g.xyz / A - want).max() < 1e-12

@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -25,7 +26,7 @@ HOST_SRC = ["host.cpp", "grid.cpp"]
 IO_LIB = os.path.join(HERE, "libthor_io.so")
 ESP_EXE = os.path.join(HERE, "thor_esp")
 HDF5 = os.environ.get("HDF5_ROOT", "/opt/conda")
-HIP_SRC = ["dyn.hip", "mlp.hip"]
+HIP_SRC = ["dyn.hip", "mlp.hip", "grid_gpu.hip"]
 
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 # The FP64 path must round every product and sum as written (no FMA
@@ -48,21 +49,25 @@ def _link(objs, out):
 
 
 def build(verbose: bool = True) -> str:
-    """Compiles both variants of the library (production and parity build)."""
+    """Compiles both variants of the library (production and parity build);
+    the translation units compile in parallel."""
     os.makedirs(OBJ, exist_ok=True)
     hipcc = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
-    host = []
+    jobs, host, objs = [], [], {LIB: [], LIB_PM: []}
     for src in HOST_SRC:
         o = os.path.join(OBJ, src + ".o")
-        _run(["g++", *CXXFLAGS, "-I", os.path.join(ROCM, "include"), "-c", os.path.join(CSRC, src), "-o", o])
+        jobs.append(["g++", *CXXFLAGS, "-I", os.path.join(ROCM, "include"), "-c", os.path.join(CSRC, src), "-o", o])
         host.append(o)
     for lib, extra, tag in ((LIB, [], ""), (LIB_PM, ["-DTHOR_PORTABLE_MATH"], "_pm")):
-        objs = list(host)
+        objs[lib] = list(host)
         for src in HIP_SRC:
             o = os.path.join(OBJ, src + tag + ".o")
-            _run([hipcc, *HIPFLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", o])
-            objs.append(o)
-        _link(objs, lib)
+            jobs.append([hipcc, *HIPFLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", o])
+            objs[lib].append(o)
+    with ThreadPoolExecutor(max_workers=max(1, min(len(jobs), os.cpu_count() or 1, 8))) as ex:
+        list(ex.map(_run, jobs))
+    for lib in (LIB, LIB_PM):
+        _link(objs[lib], lib)
     build_io()
     return LIB
 

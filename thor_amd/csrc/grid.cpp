@@ -7,6 +7,8 @@
 // refinement, OpenMP over points) is this project's own.
 #include "grid.h"
 
+#include <cstdlib>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -479,7 +481,10 @@ void build_horizontal(IcoGrid &g, int glevel, bool spring, double beta) {
     g.xyzq.assign((size_t)18 * P, 0.0);
     g.spring_iterations = 0;
     if (spring) {
-        spring_dynamics(g);
+        // on the GPU when there is one (THOR_GRID_HOST=1: the host loop)
+        const char *host = std::getenv("THOR_GRID_HOST");
+        std::string err;
+        if ((host && std::atoi(host)) || !spring_dynamics_device(g, &err)) spring_dynamics(g);
         find_qpoints(g);
         relocate_centres(g);
     } else {

@@ -56,6 +56,10 @@ void build_horizontal(IcoGrid &g, int glevel, bool spring, double beta);
 // lon/lat and the uniform vertical grid (grid.cu:193-302).
 void finish_grid(IcoGrid &g, double A, double top, int nv);
 
+// Spring dynamics of build_horizontal on the current HIP device (grid_gpu.hip):
+// false (and the reason in *err) when there is no device or a HIP call fails.
+bool spring_dynamics_device(IcoGrid &g, std::string *err);
+
 bool save_horizontal(const IcoGrid &g, const std::string &path);
 bool load_horizontal(IcoGrid &g, const std::string &path, int glevel, bool spring, double beta);
 
