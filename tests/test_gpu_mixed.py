@@ -236,9 +236,11 @@ def test_chunked_slow_modes_equal_per_level_kernel(built, deep, nonhydro, ghv):
           f"slow modes", flush=True)
     st = capi.ic_rest(g, p, seed=1011, temperature_noise=1.0, wind_noise=2.0)
     a = _two_steps(capi, g, p, st, {"THOR_SLOW_CHUNKED": "0"})
-    b = _two_steps(capi, g, p, st, {"THOR_PREPARE_SPLIT": "0"})
-    for k in a:
-        assert np.array_equal(a[k], b[k]), k
+    # both chunked kernels: k_slow_lc and k_slow_lc2 (two blocks per CU)
+    for v2 in ("0", "1"):
+        b = _two_steps(capi, g, p, st, {"THOR_PREPARE_SPLIT": "0", "THOR_SLOW_V2": v2})
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (v2, k)
     # Default: the substep pressure flux divergence adds the stage part
     # (formed by the slow-mode kernel) to the deviation part, whose h and pt
     # factors are read as FP32 copies (they multiply the FP32 momentum
@@ -323,3 +325,28 @@ def test_row_parallel_vertical_sweeps_stay_close(built):
             err = float(np.abs(b[k] - a[k]).max() / np.abs(a[k]).max())
             print(f"  L{nv} {k}: max rel diff {err:.2e}")
             assert err < 1e-9, (nv, k)
+
+
+@pytest.mark.parametrize("nv,deep,nonhydro,ghv", [(20, 1, 1, 0), (33, 0, 0, 1), (48, 1, 1, 1), (64, 1, 0, 0)])
+def test_segment_vertical_solve_stays_close(built, nv, deep, nonhydro, ghv):
+    """The segment form of the mixed vertical solve (each row of the block
+    owns a contiguous run of interfaces in all three phases and keeps its
+    cells' values in registers for the density/pressure update), with 16 and
+    32 columns per block, and the segment form of the stage's factorisation
+    (Moebius maps of the pivot recurrence chained over the rows), against the
+    one-lane sweeps and recurrence: two steps agree to FP64 rounding,
+    including ragged last segments and rows without interfaces."""
+    capi = built
+    g = grid(capi, 4, nv)
+    p = earth_params(capi, 4, dt=600.0, precision=1, HyDiff=1, DivDampP=1, DeepModel=deep, NonHydro=nonhydro,
+                     GravHeightVar=ghv)
+    st = capi.ic_rest(g, p, seed=1018, temperature_noise=1.0, wind_noise=2.0)
+    a = _two_steps(capi, g, p, st, {"THOR_THOMAS_ROWS": "0", "THOR_THOMAS_SEG": "0", "THOR_FACTOR_SEG": "0"})
+    for cb in ("16", "32"):
+        print(f"case: mixed g4 L{nv}, DeepModel {deep} NonHydro {nonhydro} GravHeightVar {ghv}: segment vertical "
+              f"solve, {cb} columns per block, vs one-lane sweeps", flush=True)
+        b = _two_steps(capi, g, p, st, {"THOR_THOMAS_SEG": cb, "THOR_FACTOR_SEG": "1"})
+        for k in a:
+            err = float(np.abs(b[k] - a[k]).max() / np.abs(a[k]).max())
+            print(f"  L{nv} cb {cb} {k}: max rel diff {err:.2e}")
+            assert err < 1e-9, (nv, cb, k)

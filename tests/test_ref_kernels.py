@@ -130,7 +130,10 @@ def test_tracer_kernels_match_thor(built, deep):
     the first diffusion pass (THOR writes it into diff_d), the last pass
     (difftr; the pole kernel writes nothing there, which the oracle keeps) and
     Tracer_Eq (tracers), from the same random tracers on an oracle state with
-    winds, bit for bit (deep: r^2 by pow on both sides, 1e-14)."""
+    winds, bit for bit (deep: r^2 by pow on both sides, glibc's in the oracle
+    and the device's in THOR's kernel; their 1-ulp differences pass through
+    the cancelling flux sums, a few 1e-14 of the largest tracer tendency
+    depending on the grid's last bits, so 1e-13)."""
     from oracle import oracle as om
     if not om.ref_available():
         pytest.skip("THOR's kernels are not built (oracle/_ref/libthor_ref.so missing)")
@@ -158,7 +161,7 @@ def test_tracer_kernels_match_thor(built, deep):
         ref.set(n, a)
     o.set("tracerk", trk)
     ref.set("tracerk", trk)
-    tol = 1e-14 if deep else 0.0
+    tol = 1e-13 if deep else 0.0
 
     def cmp(label, got, want):
         err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-300)

@@ -99,8 +99,12 @@ struct Dyn {
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
     bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
+    bool slow_v2 = true;   // mixed path: k_slow_lc2 (two blocks per CU) in place of k_slow_lc
+    bool slow_lc64 = false;  // FP64 path: k_divm1_ml + k_slow_lc2<false> in place of k_slow<false>
     bool fp64_ml = true;   // FP64 path: horizontal stencils over several levels per block (*_ml kernels)
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
+    int tseg = 16;         // mixed path: columns per block of the segment Thomas kernel (16 or 32; 0: off)
+    bool fseg = true;      // mixed path: segment form of the stage's Thomas factorisation
     bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
     bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
     int cp_lv = 2;            // levels per thread of the column-preparation kernel (1, 2 or 4)
@@ -417,7 +421,17 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             if (divdamp)
                 launch(d, "divm1", W(0, 6), [&] { k_divm1_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, lc); });
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
-                   [&] { k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc); });
+                   [&] {
+                       if (d.slow_v2) k_slow_lc2<true><<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc);
+                       else k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc);
+                   });
+        } else if (!MX && d.slow_lc64) {
+            // FP64: DivM over several levels per block into the idle
+            // hyperdiffusion scratch, then the slow modes over level chunks
+            if (divdamp)
+                launch(d, "divm1", W(6, 3), [&] { k_divm1_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f, f.diff); });
+            launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 3 : 0) + 21 + (rk ? 3 : 0)),
+                   [&] { k_slow_lc2<false><<<grid_lc, blk, 0, st>>>(g, f, f.diff, divdamp, rk, lc); });
         } else {
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                    [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
@@ -432,7 +446,12 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         };
         if (divdamp) divm0();
         launch(d, "thomas_factor", 5 * B, [&] {
-            if (cb == 32) k_thomas_factor<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
+            const int fseg = (d.nv - 1 + 7) / 8;  // interfaces per row of the segment form (32 columns)
+            if (MX && d.fseg && d.nv > 1 && fseg <= 8) {
+                const dim3 gs((d.P + 31) / 32);
+                if (fseg <= 6) k_thomas_factor_seg<32, 6><<<gs, 256, 0, st>>>(g, f, times);
+                else k_thomas_factor_seg<32, 8><<<gs, 256, 0, st>>>(g, f, times);
+            } else if (cb == 32) k_thomas_factor<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
             else k_thomas_factor<16, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
         });
         for (int ns = 0; ns < ns_it; ns++) {
@@ -456,7 +475,19 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             // the solve's plane plus the row maps of the mixed path's sweeps
             const size_t lds = ((size_t)(d.nv + 1) * d.tcb + 2 * 256) * sizeof(double);
             const dim3 grid_th((d.P + d.tcb - 1) / d.tcb);
-            if (!d.hooks.dyn_core_loop_fast_modes) {
+            // segment kernel: rows of (nv - 1 + NROW - 1) / NROW interfaces
+            const int seg16 = (d.nv - 1 + 15) / 16, seg32 = (d.nv - 1 + 7) / 8;
+            const bool tseg = MX && d.f.thomas_rows && d.nv > 1 && (d.tseg == 16 ? seg16 <= 4 : d.tseg == 32 && seg32 <= 8);
+            const size_t lds_seg = ((size_t)(d.nv + 1) * d.tseg + 2 * 256) * sizeof(double);
+            if (!d.hooks.dyn_core_loop_fast_modes && tseg) {
+                launch(d, "thomas_dp", W(12 + 18, 4), [&] {
+                    const dim3 gs((d.P + d.tseg - 1) / d.tseg);
+                    if (d.tseg == 16 && seg16 <= 3) k_thomas_dp_seg<16, 3><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else if (d.tseg == 16) k_thomas_dp_seg<16, 4><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else if (seg32 <= 6) k_thomas_dp_seg<32, 6><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else k_thomas_dp_seg<32, 8><<<gs, 256, lds_seg, st>>>(g, f, times);
+                });
+            } else if (!d.hooks.dyn_core_loop_fast_modes) {
                 launch(d, "thomas_dp", W(12 + 18, 4), [&] {
                     if (d.tcb == 64 && d.tur == 2) k_thomas_dp<64, MX, true, 2><<<grid_th, 256, lds, st>>>(g, f, times);
                     else if (d.tcb == 64) k_thomas_dp<64, MX><<<grid_th, 256, lds, st>>>(g, f, times);
@@ -466,7 +497,13 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             } else {
                 // the module runs between the vertical solve and the density/pressure update
                 launch(d, "thomas", W(12, 0), [&] {
-                    if (d.tcb == 64) k_thomas_dp<64, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
+                    const dim3 gs((d.P + d.tseg - 1) / d.tseg);
+                    if (tseg && d.tseg == 16 && seg16 <= 3)
+                        k_thomas_dp_seg<16, 3, false><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else if (tseg && d.tseg == 16) k_thomas_dp_seg<16, 4, false><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else if (tseg && seg32 <= 6) k_thomas_dp_seg<32, 6, false><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else if (tseg) k_thomas_dp_seg<32, 8, false><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    else if (d.tcb == 64) k_thomas_dp<64, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
                     else if (d.tcb == 32) k_thomas_dp<32, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
                     else k_thomas_dp<16, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
                 });
@@ -891,19 +928,30 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     // The level-chunked slow-mode kernel uses the gradient weights for the
     // neighbours of the divergence too; THOR's operators share them, checked
     // here bit for bit.
-    d.slow_lc = d.mixed;
-    for (int p = 0; p < P && d.slow_lc; p++)
+    bool shared_weights = true;
+    for (int p = 0; p < P && shared_weights; p++)
         if (std::memcmp(gd->div + (size_t)p * 21 + 3, gd->grad + (size_t)p * 21 + 3, 18 * sizeof(double)) != 0)
-            d.slow_lc = false;
+            shared_weights = false;
+    d.slow_lc = d.mixed && shared_weights;
+    d.slow_lc64 = !d.mixed && shared_weights;
     if (const char *e = std::getenv("THOR_THOMAS_UNROLL")) {
         const int v = std::atoi(e);
         if (v == 1 || v == 2) d.tur = v;
     }
-    if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) d.slow_lc = d.slow_lc && std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_SLOW_CHUNKED")) {
+        d.slow_lc = d.slow_lc && std::atoi(e) != 0;
+        d.slow_lc64 = d.slow_lc64 && std::atoi(e) != 0;
+    }
+    if (const char *e = std::getenv("THOR_SLOW_V2")) d.slow_v2 = std::atoi(e) != 0;
     d.prep_split = d.slow_lc;
     if (const char *e = std::getenv("THOR_DIFF2_CHUNKED")) d.diff2_lc = std::atoi(e) != 0;
     d.f.thomas_rows = 1;
     if (const char *e = std::getenv("THOR_THOMAS_ROWS")) d.f.thomas_rows = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_FACTOR_SEG")) d.fseg = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_THOMAS_SEG")) {
+        const int v = std::atoi(e);
+        if (v == 0 || v == 16 || v == 32) d.tseg = v;
+    }
     if (const char *e = std::getenv("THOR_COLPREP_LEVELS")) {
         const int v = std::atoi(e);
         if (v == 1 || v == 2 || v == 4) d.cp_lv = v;
@@ -962,6 +1010,18 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         }
     }
     if (e != hipSuccess) return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: alloc: ") + hipGetErrorString(e)));
+    {
+        // per-level constants of the chunked slow modes (both forms)
+        double *slv = nullptr;
+        if (e == hipSuccess) e = dalloc(d, &slv, (size_t)6 * nv);
+        if (e == hipSuccess) {
+            k_slow_levels<<<(nv + 63) / 64, 64, 0, d.stream>>>(g, slv);
+            e = hipGetLastError();
+            g.slv = slv;
+        }
+        if (e != hipSuccess)
+            return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: level table: ") + hipGetErrorString(e)));
+    }
     if (d.mixed) {
         auto F = [&](float **ptr, size_t n) {
             if (e == hipSuccess) e = falloc(d, ptr, n);

@@ -47,6 +47,9 @@ struct Geo {
     // Mixed precision only: per-level FP32 constants [4][nv]: rscale, rscale^2,
     // 1 / (Alth[lev] - Alth[lev + 1]) and Kdhz.
     const float *lvc;
+    // Mixed precision only: per-level FP64 constants of the chunked slow modes
+    // [6][nv]: r2p, r2m, r2l, rscale, dr2dz, (A / (A + Alt))^2 (k_slow_levels).
+    const double *slv;
     double A, Omega, Gravit, P_Ref, Rd, Cp;
     int deep, nonhydro, ghv;
     int energy;  // total-energy thermodynamic equation (FP64 form only)

@@ -2620,6 +2620,338 @@ __global__ __launch_bounds__(256, 1) void k_slow_lc(Geo g, Fields f, const float
     }
 }
 
+// k_slow_lc for both precision forms, the mixed one with two blocks per CU:
+// the per-level constants come from a table in HBM (k_slow_levels, the same
+// expressions), the staged tile values are loaded one level ahead instead of
+// two, and the values used late in a level are read at its start. The
+// arithmetic is slow_point's, so the results equal k_slow<MX>'s bit for bit.
+// dm: DivM (divergence-damping pass 1) of the tile points, from k_divm1_lc
+// (mixed) or k_divm1_ml (FP64).
+template <bool MX>
+__global__ __launch_bounds__(256, MX ? 2 : 1) void k_slow_lc2(Geo g, Fields f, const typename FS<MX>::R *dm,
+                                                              int divdamp, int rk, int lc) {
+    using R = typename FS<MX>::R;
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev)) {
+            const auto *divg = FS<MX>::divg(f);
+            slow_point<MX>(
+                g, f, c.p, lev, 5, true, divdamp, rk, [&](int s, int k) { return VEC(divg, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.pk, lev, c.gi(s)); }, [&](int s) { return LVP(f.h, lev, c.gi(s)); },
+                [&](int s, int k) { return VEC(f.Mhk, k, lev, c.gi(s)); });
+        }
+        return;
+    }
+    __shared__ double sM[2][3][NHL2], sP[2][NHL2], sH[2][NHL2], sT[2][NHL2];
+    __shared__ double sW[24][kTile];  // the thread's 21 gradient weights and 3 divergence centre weights
+    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cv = g.Cp - g.Rd;
+    const double *slv = g.slv;  // per-level constants (k_slow_levels)
+    const int nvl = nv;
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
+    {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int i = 0; i < 21; i++) sW[i][t] = g.grad[(size_t)i * S + p];
+#pragma unroll
+        for (int k = 0; k < 3; k++) sW[21 + k][t] = g.div[(size_t)k * S + p];
+    }
+    auto wg = [&](int i) { return sW[i][threadIdx.x]; };
+    auto wd0 = [&](int k) { return sW[21 + k][threadIdx.x]; };
+    const double fx = g.fr[p], fy = g.fr[S + p], fz = g.fr[2 * S + p];
+    const size_t pl4 = S * 4, pl8 = S * 8, plR = S * sizeof(R);
+    // a value of an array of the form's precision at the thread's point
+    auto ldR = [&](__amdgpu_buffer_rsrc_t rs, unsigned off) -> R {
+        if constexpr (MX) return ldb_f(rs, T.o4, off);
+        else return ldb_d(rs, T.o8, off);
+    };
+    const auto rMk = rsrc(f.Mhk, 3 * nv * pl8), rPk = rsrc(f.pk, nv * pl8), rH = rsrc(f.h, nv * pl8),
+               rPt = rsrc(f.pt, nv * pl8),
+               rAdv = rsrc(FS<MX>::Adv(f), 3 * nv * plR), rRk = rsrc(f.Rhok, nv * pl8),
+               rWk = rsrc(f.Whk, (nv + 1) * pl8), rHh = rsrc(f.hh, (nv + 1) * pl8),
+               rDmh = rsrc(FS<MX>::diffmh(f), 3 * nv * plR), rdM = rsrc(f.dMh, 3 * nv * pl8),
+               rDw = rsrc(FS<MX>::diffw(f), nv * plR), rdW = rsrc(f.dWh, (nv + 1) * pl8),
+               rDr = rsrc(FS<MX>::diffrh(f), nv * plR), rDp = rsrc(FS<MX>::diffpr(f), nv * plR),
+               rGt = rsrc(f.gtil, nv * pl8), rQ = rsrc(f.Qheat, nv * pl8), rDM = rsrc(dm, 3 * nv * plR),
+               rM = rsrc(f.Mh, 3 * nv * pl8),
+               rR = rsrc(f.Rho, nv * pl8), rPr = rsrc(f.p, nv * pl8), rW = rsrc(f.Wh, (nv + 1) * pl8);
+    struct Stg {  // staged tile values: own point and halo point, two levels ahead
+        double om[3], op, oh, ot, hm[3], hp, hh, ht;
+    };
+    struct Pt {  // the point's own values, one level ahead
+        R adv[3], dmh[3], dM[3], dw;
+        double rhok, whk1, hh1, dmv[3];
+    };
+    auto load_stg = [&](Stg &v, int lev) {
+        const unsigned l8 = (unsigned)(lev * pl8);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.om[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.hm[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
+        }
+        v.op = ldb_d(rPk, T.o8, l8);
+        v.hp = ldb_d(rPk, T.h8, l8);
+        v.oh = ldb_d(rH, T.o8, l8);
+        v.hh = ldb_d(rH, T.h8, l8);
+        if constexpr (MX) {  // pt: the stage part of the substeps' pressure flux divergence
+            v.ot = ldb_d(rPt, T.o8, l8);
+            v.ht = ldb_d(rPt, T.h8, l8);
+        } else {
+            v.ot = v.ht = 0.0;
+        }
+    };
+    auto load_pt = [&](Pt &v, int lev) {
+        const unsigned l8 = (unsigned)(lev * pl8), u8 = (unsigned)((lev + 1) * pl8);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.adv[k] = ldR(rAdv, (unsigned)((k * nv + lev) * plR));
+            v.dmh[k] = ldR(rDmh, (unsigned)((k * nv + lev) * plR));
+            v.dM[k] = divdamp ? ldR(rDM, (unsigned)((k * nv + lev) * plR)) : (R)0;
+            v.dmv[k] = ldb_d(rdM, T.o8, (unsigned)((k * nv + lev) * pl8));
+        }
+        v.dw = ldR(rDw, (unsigned)(lev * plR));
+        v.rhok = ldb_d(rRk, T.o8, l8);
+        v.whk1 = ldb_d(rWk, T.o8, u8);
+        v.hh1 = ldb_d(rHh, T.o8, u8);
+    };
+    auto put = [&](const Stg &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) sM[b][k][T.ir] = v.om[k];
+        sP[b][T.ir] = v.op;
+        sH[b][T.ir] = v.oh;
+        sT[b][T.ir] = v.ot;
+        if (T.hp >= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) sM[b][k][T.hp] = v.hm[k];
+            sP[b][T.hp] = v.hp;
+            sH[b][T.hp] = v.hh;
+            sT[b][T.hp] = v.ht;
+        }
+    };
+    // values of level lev - 1 carried to level lev
+    R cadv[3] = {0, 0, 0}, cdw = 0;
+    double crho = 0.0, cpk = 0.0, cm[3] = {0.0, 0.0, 0.0}, cwh, chh;
+    if (l0 > 0) {
+        const int lm = l0 - 1;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            cadv[k] = ldR(rAdv, (unsigned)((k * nv + lm) * plR));
+            cm[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lm) * pl8));
+        }
+        cdw = ldR(rDw, (unsigned)(lm * plR));
+        crho = ldb_d(rRk, T.o8, (unsigned)(lm * pl8));
+        cpk = ldb_d(rPk, T.o8, (unsigned)(lm * pl8));
+    }
+    cwh = ldb_d(rWk, T.o8, (unsigned)(l0 * pl8));  // Whk and hh at the chunk's lowest interface
+    chh = ldb_d(rHh, T.o8, (unsigned)(l0 * pl8));
+    auto *SlowMh = FS<MX>::SlowMh(f);
+    auto *Mhs = FS<MX>::Mhs(f);
+    const auto rSm = rsrc(SlowMh, 3 * nv * plR);
+    auto body = [&](int lev, Stg &Rs, const Pt &v, Pt &nxt) {
+        load_pt(nxt, min(lev + 1, l1 - 1));
+        load_stg(Rs, min(lev + 1, l1 - 1));
+        const int b = (lev - l0) & 1;
+        // values used late in the level: read at its start
+        const R v_drh = ldR(rDr, (unsigned)(lev * plR)), v_dpr = ldR(rDp, (unsigned)(lev * plR));
+        const double v_dwh = ldb_d(rdW, T.o8, (unsigned)(lev * pl8)), v_gtil = ldb_d(rGt, T.o8, (unsigned)(lev * pl8)),
+                     v_q = ldb_d(rQ, T.o8, (unsigned)(lev * pl8));
+        double advx = v.adv[0], advy = v.adv[1], advz = v.adv[2];
+        const double rho = v.rhok;
+        double advrl = 0, rhol = 0, altl = 0, pressurel = 0;
+        if (lev != 0) {
+            const double ax = cadv[0], ay = cadv[1], az = cadv[2];
+            advrl = ax * fx + ay * fy + az * fz;
+            rhol = crho;
+            altl = g.Alt[lev - 1];
+            pressurel = cpk;
+        }
+        double advr = advx * fx + advy * fy + advz * fz;
+        advx += -advr * fx;
+        advy += -advr * fy;
+        advz += -advr * fz;
+        const double altht = g.Alth[lev + 1], althl = g.Alth[lev], alt = g.Alt[lev];
+        const double M0x = sM[b][0][T.pos[0]], M0y = sM[b][1][T.pos[0]], M0z = sM[b][2][T.pos[0]];
+        if (!g.nonhydro) {
+            advr = (XPOW(M0x, 2) + XPOW(M0y, 2) + XPOW(M0z, 2)) / (rho * (alt + A));
+            if (lev != 0) advrl = (XPOW(cm[0], 2) + XPOW(cm[1], 2) + XPOW(cm[2], 2)) / (rhol * (altl + A));
+        }
+        const double dz = altht - althl;
+        const double r2p = slv[lev], r2m = slv[nvl + lev], r2l = slv[2 * nvl + lev], rscale = slv[3 * nvl + lev],
+                     dr2dz = slv[4 * nvl + lev];
+        // nflxv, nflxr, nflxp as slow_point; dptk: the stage part of the
+        // pressure flux divergence of the substeps (prepare_point_w's st with
+        // the stage momentum), formed here once per stage. The stencil slots
+        // are walked once for all three components (each sum still adds its
+        // slots in order 0..6).
+        const double p0 = sP[b][T.pos[0]];
+        double sg[3], sr[3], sp3[3], st3[3];
+        {
+            const double h0 = sH[b][T.pos[0]], t0 = sT[b][T.pos[0]];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double m0 = sM[b][k][T.pos[0]];
+                sg[k] = wg(k) * p0;
+                sr[k] = wd0(k) * m0;
+                sp3[k] = wd0(k) * m0 * h0;
+                st3[k] = wd0(k) * m0 * t0;
+            }
+        }
+#pragma unroll
+        for (int s = 1; s < 7; s++) {
+            const double ps_ = sP[b][T.pos[s]], hs_ = sH[b][T.pos[s]], ts_ = sT[b][T.pos[s]];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double w = wg(s * 3 + k), m = sM[b][k][T.pos[s]];
+                sg[k] = sg[k] + w * ps_;
+                sr[k] = sr[k] + w * m;
+                sp3[k] = sp3[k] + w * m * hs_;
+                st3[k] = st3[k] + w * m * ts_;
+            }
+        }
+        double nflxv[3], nflxr = 0.0, nflxp = 0.0, dptk = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            nflxv[k] = rscale * sg[k];
+            nflxr += rscale * sr[k];
+            nflxp += rscale * sp3[k];
+            dptk += rscale * st3[k];
+        }
+        if constexpr (MX) LV(f.hflx_ptk, lev) = dptk;
+        double whl, wht, hhl, hht;
+        if (lev == 0) {
+            whl = 0.0;
+            wht = v.whk1;
+            hhl = 0.0;
+            hht = v.hh1;
+        } else {
+            whl = cwh;
+            wht = v.whk1;
+            hhl = chh;
+            hht = v.hh1;
+        }
+        const double dwdz = (wht * r2p - whl * r2l) / (dr2dz);
+        const double dwhdz = (wht * r2p * hht - whl * r2l * hhl) / (dz * r2m);
+        const double dpr = nflxv[0] * fx + nflxv[1] * fy + nflxv[2] * fz;
+        const double fv[3] = {fx, fy, fz}, ad[3] = {advx, advy, advz};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double DivM = v.dM[k];
+            // the vertical momentum hyperdiffusion has its own array in the FP64
+            // form only (the mixed form adds it to diffmh)
+            double dmv = 0.0;
+            if constexpr (!MX) dmv = f.diffmv ? VEC(f.diffmv, k, lev, p) : 0.0;
+            const double o = -(nflxv[k] - dpr * fv[k]) - ad[k] + DivM + v.dmh[k] + dmv + v.dmv[k];
+            if constexpr (MX) stb_f(rSm, T.o4, (unsigned)((k * nv + lev) * plR), (float)o);
+            else VEC(SlowMh, k, lev, p) = o;
+        }
+        if (lev == 0) {
+            LV(f.SlowWh, 0) = 0.0;
+            LV(f.SlowWh, nv) = 0.0;
+        } else {
+            const double xi = althl, xim = altl, xip = alt;
+            const double intt = (xi - xip) / (xim - xip), intl = (xi - xim) / (xip - xim);
+            double swr;
+            if (g.deep) {
+                if (g.ghv)
+                    swr = (advrl + rhol * Gravit * slv[5 * nvl + lev - 1]) * intt
+                          + (advr + rho * Gravit * slv[5 * nvl + lev]) * intl;
+                else
+                    swr = (advrl + rhol * Gravit) * intt + (advr + rho * Gravit) * intl;
+            } else {
+                if (g.ghv)
+                    swr = (rhol * Gravit * slv[5 * nvl + lev - 1]) * intt + (rho * Gravit * slv[5 * nvl + lev]) * intl;
+                else
+                    swr = (rhol * Gravit) * intt + (rho * Gravit) * intl;
+            }
+            if (g.nonhydro) swr += (-cdw - 0.0) * intt + (-v.dw - 0.0) * intl;
+            const double dzl = alt - altl;
+            const double dpdz = (p0 - pressurel) / dzl;
+            swr += dpdz;
+            swr -= v_dwh;
+            LV(f.SlowWh, lev) = -swr;
+        }
+        nflxr += dwdz;
+        LV(f.SlowRho, lev) = -nflxr + v_drh + 0.0;
+        const double rr = 1.0 / rho;
+        const double vgp = (nflxv[0] - dpr * fx) * M0x * rr + (nflxv[1] - dpr * fy) * M0y * rr
+                           + (nflxv[2] - dpr * fz) * M0z * rr;
+        double sp = (Rd / Cv) * (-nflxp - v_gtil - dwhdz + vgp + v_q);
+        if (!MX && g.energy) sp += Rd / Cv * (v_dpr + 0.0);  // thor_slowmodes.h:375-377
+        else sp += v_dpr + 0.0;
+        LV(f.Slowp, lev) = sp;
+        if (rk == 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(Mhs, k, lev, p) = 0.0;
+            LV(f.Rhos, lev) = 0.0;
+            LV(f.ps, lev) = 0.0;
+            LV(f.Whs, lev) = 0.0;
+            if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+        } else {
+            const unsigned l8 = (unsigned)(lev * pl8);
+            const double Mx = ldb_d(rM, T.o8, (unsigned)((0 * nv + lev) * pl8)) - M0x,
+                         My = ldb_d(rM, T.o8, (unsigned)((1 * nv + lev) * pl8)) - M0y,
+                         Mz = ldb_d(rM, T.o8, (unsigned)((2 * nv + lev) * pl8)) - M0z;
+            const double aux = Mx * fx + My * fy + Mz * fz;
+            VEC(Mhs, 0, lev, p) = Mx - aux * fx;
+            VEC(Mhs, 1, lev, p) = My - aux * fy;
+            VEC(Mhs, 2, lev, p) = Mz - aux * fz;
+            LV(f.Rhos, lev) = ldb_d(rR, T.o8, l8) - rho;
+            LV(f.ps, lev) = ldb_d(rPr, T.o8, l8) - p0;
+            LV(f.Whs, lev) = lev == 0 ? 0.0 : ldb_d(rW, T.o8, l8) - cwh;
+            if (lev == nv - 1) LV(f.Whs, nv) = 0.0;
+        }
+        // carry this level's values to the next
+#pragma unroll
+        for (int k = 0; k < 3; k++) cadv[k] = v.adv[k];
+        cm[0] = M0x;
+        cm[1] = M0y;
+        cm[2] = M0z;
+        cdw = v.dw;
+        crho = rho;
+        cpk = p0;
+        cwh = v.whk1;
+        chh = v.hh1;
+        if (lev + 1 < l1) put(Rs, b ^ 1);
+        __syncthreads();
+    };
+    Stg S0;
+    Pt P0, P1;
+    load_pt(P0, l0);
+    load_stg(S0, l0);
+    put(S0, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, S0, P0, P1);
+        if (lev + 1 < l1) body(lev + 1, S0, P1, P0);
+    }
+}
+
+// Per-level constants of k_slow_lc2, with k_slow_lc's expressions: [6][nv]
+// r2p, r2m, r2l, rscale, dr2dz and (ghv) (A / (A + Alt))^2.
+__global__ void k_slow_levels(Geo g, double *out) {
+    const int lev = blockIdx.x * blockDim.x + threadIdx.x, nv = g.nv;
+    if (lev >= nv) return;
+    const double A = g.A;
+    const double altht = g.Alth[lev + 1], althl = g.Alth[lev], alt = g.Alt[lev];
+    if (g.deep) {
+        out[lev] = XPOW(altht + A, 2.0);
+        out[nv + lev] = XPOW(alt + A, 2.0);
+        out[2 * nv + lev] = XPOW(althl + A, 2.0);
+        out[3 * nv + lev] = A / (alt + A);
+        out[4 * nv + lev] = (XPOW(altht + A, 3.0) - XPOW(althl + A, 3.0)) / 3;
+    } else {
+        out[lev] = out[nv + lev] = out[2 * nv + lev] = 1.0;
+        out[3 * nv + lev] = 1.0;
+        out[4 * nv + lev] = 1.0 * (altht - althl);
+    }
+    out[5 * nv + lev] = g.ghv ? XPOW(A / (A + alt), 2) : 0.0;
+}
+
 // Compute_Advec_Cori1 (thor_adv_cor.h:51-235) and the first Diffusion_Op pass
 // (thor_diff.h:54-522) of the mixed-precision path, in FP32 over level
 // chunks. Each staged point carries its momentum, the advecting velocity
@@ -2851,15 +3183,129 @@ __global__ __launch_bounds__(256) void k_thomas_factor(Geo g, Fields f, double d
     }
 }
 
+// Mixed path, segment form of k_thomas_factor: row r of the block owns the
+// contiguous interfaces [lo, hi) of its column (as in thomas_sweeps_rows),
+// forms their aa, bb, cc in registers with k_thomas_factor's arithmetic, and
+// the recurrence of the modified upper diagonal,
+//   c'_1 = cc_1 / bb_1,  c'_l = cc_l / (bb_l - aa_l c'_{l-1}),
+// a Moebius map of c'_{l-1} with matrix [[0, cc_l], [-aa_l, bb_l]], is
+// composed over the segment (normalised after each product), the rows' maps
+// are chained in order through LDS (sX: 4 x NROW x CB doubles), and each row
+// then runs the recurrence over its segment from its true start value. The
+// pivots and c' agree with the one-lane recurrence to FP64 rounding.
+template <int CB, int SEG>
+__global__ __launch_bounds__(256) void k_thomas_factor_seg(Geo g, Fields f, double deltat) {
+    constexpr int NROW = 256 / CB;
+    __shared__ double sX[4][NROW][CB];
+    const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
+    const int p = blockIdx.x * CB + c;
+    const bool valid = p < g.P;
+    const size_t S = g.S;
+    const int seg = (nv - 1 + NROW - 1) / NROW;
+    const int lo = 1 + row * seg, hi = min(nv, lo + seg);
+    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const bool ghv = g.ghv, nonh = g.nonhydro;
+    const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
+    auto gcor = [&](double altx) {
+        return ghv ? Gravit * XPOW(A / (A + altx), 2) * (Cp - Rd) / Rd : Gravit * (Cp - Rd) / Rd;
+    };
+    double ra[SEG], rb[SEG], rc[SEG];
+#pragma unroll
+    for (int i = 0; i < SEG; i++) {
+        const int lev = lo + i;
+        ra[i] = 0.0;
+        rb[i] = 1.0;
+        rc[i] = 0.0;
+        if (valid && lev < hi) {
+            const double altht = Alth[lev + 1], alth = Alth[lev], althl = Alth[lev - 1];
+            const double alt = Alt[lev], altl = Alt[lev - 1];
+            double hp = LV(f.hh, lev + 1), h = LV(f.hh, lev), hm = LV(f.hh, lev - 1);
+            const double gp = LV(f.gtilh, lev + 1), gg = LV(f.gtilh, lev), gm = LV(f.gtilh, lev - 1);
+            const double GCoRl = gcor(altl), GCoRu = gcor(alt);
+            const double dzp = 1.0 / (altht - alth), dzh = 1.0 / (alt - altl), dzm = 1.0 / (alth - althl);
+            const double dzph = dzp * dzh, dzmh = dzm * dzh;
+            const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
+            const double GCoR = (GCoRl * (alt - alth) + GCoRu * (alth - altl)) / (alt - altl);
+            const double intl = (alt - alth) * dzp * dzh;
+            const double inttm = -(altl - alth) * dzm * dzh;
+            double aa, bb, cc;
+            if (g.deep) {
+                hp = hp * XPOW(altht + A, 2);
+                h = h * XPOW(alth + A, 2);
+                hm = hm * XPOW(althl + A, 2);
+                const double or2 = 1 / XPOW(alth + A, 2);
+                const double tor3 = 2 / XPOW(alth + A, 3);
+                cc = -dzph * or2 * hp - inttm * (gp + GCoR - tor3 * hp);
+                if (nonh) bb = CRdd + (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
+                else bb = (dzph + dzmh) * or2 * h + (intl - inttm) * (gg + GCoR - tor3 * h);
+                aa = -dzmh * or2 * hm + intl * (gm + GCoR - tor3 * hm);
+            } else {
+                cc = -dzph * hp - inttm * (gp + GCoR);
+                if (nonh) bb = CRdd + (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
+                else bb = (dzph + dzmh) * h + (intl - inttm) * (gg + GCoR);
+                aa = -dzmh * hm + intl * (gm + GCoR);
+            }
+            ra[i] = aa;
+            rb[i] = bb;
+            rc[i] = cc;
+        }
+    }
+    // the segment's map x_{hi-1} = (m00 x + m01) / (m10 x + m11) of x_{lo-1}
+    double m00 = 1.0, m01 = 0.0, m10 = 0.0, m11 = 1.0;
+#pragma unroll
+    for (int i = 0; i < SEG; i++) {
+        if (lo + i < hi) {
+            const double a = lo + i == 1 ? 0.0 : -ra[i];  // c'_0 does not enter the first interface
+            // [[0, cc], [a, bb]] x M
+            const double n00 = rc[i] * m10, n01 = rc[i] * m11;
+            const double n10 = a * m00 + rb[i] * m10, n11 = a * m01 + rb[i] * m11;
+            const double sc = 1.0 / fmax(fmax(fabs(n00), fabs(n01)), fmax(fabs(n10), fabs(n11)));
+            m00 = n00 * sc;
+            m01 = n01 * sc;
+            m10 = n10 * sc;
+            m11 = n11 * sc;
+        }
+    }
+    sX[0][row][c] = m00;
+    sX[1][row][c] = m01;
+    sX[2][row][c] = m10;
+    sX[3][row][c] = m11;
+    __syncthreads();
+    double x = 0.0;  // c' at interface lo - 1
+    for (int r = 0; r < row; r++) x = (sX[0][r][c] * x + sX[1][r][c]) / (sX[2][r][c] * x + sX[3][r][c]);
+    if (!valid) return;
+    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+#pragma unroll
+    for (int i = 0; i < SEG; i++) {
+        const int lev = lo + i;
+        if (lev < hi) {
+            if (lev == 1) {
+                x = rc[i] / rb[i];
+                TT[1] = rb[i];
+            } else {
+                const double tt = 1.0 / (rb[i] - x * ra[i]);
+                x = rc[i] * tt;
+                TT[lev] = tt;
+                AA[lev] = ra[i];
+            }
+            CC[lev] = x;
+        }
+    }
+}
+
 // ===================================================================== K_G2
 // Density_Pressure_Eqs / _Poles (thor_fastmodes.h:255-847, entropy form),
 // column part: the horizontal flux divergences nflxr = rscale sum div M' and
 // nflxpt = rscale sum div (Mk + M') pt come from k_prepare (same sums, same
 // order), the vertical fluxes from the freshly solved Whs of the column
 // (whs(k) = Whs at interface k).
+// rhos, pss, slowrho: Rhos, ps and SlowRho of the cell, read by the caller
+// (the segment kernel keeps them from its right-hand-side phase).
 template <bool MX, class GetW>
-__device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
-                                          double nflxr, double nflxpt, GetW whs) {
+__device__ __forceinline__ void dp_column_in(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
+                                             double nflxr, double nflxpt, GetW whs, double rhos, double pss,
+                                             double slowrho) {
     const size_t S = g.S;
     const double A = g.A, P_Ref = g.P_Ref, Rd = g.Rd, Cp = g.Cp, Cv = Cp - Rd;
     const double altht = g.Alth[lev + 1], althl = g.Alth[lev], dz = altht - althl;
@@ -2893,10 +3339,10 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
     const double dwdz = pole ? (wht * r2p - whl * r2l) / (dz * r2m) : (wht * r2p - whl * r2l) / (dr2dz);
     const double dwptdz = (wht2 * pht * r2p - whl2 * phl * r2l) / (dz * r2m);
     double aux = -(nflxpt + dwptdz) * dt;
-    const double rhos = LV(f.Rhos, lev), pss = LV(f.ps, lev), pkv = LV(f.pk, lev);
+    const double pkv = LV(f.pk, lev);
     const double r = LV(f.Rhok, lev) + rhos;
     nflxr += dwdz;
-    const double rhosn = rhos + (LV(f.SlowRho, lev) - nflxr) * dt;
+    const double rhosn = rhos + (slowrho - nflxr) * dt;
     LV(f.Rhos, lev) = rhosn;
     double psn;
     if constexpr (!MX) {
@@ -2956,6 +3402,14 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
     if (isnan(psn)) atomicOr(f.flags, FLAG_NAN_P);
 }
 
+template <bool MX, class GetW>
+__device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, int lev, bool pole, double dt,
+                                          double nflxr, double nflxpt, GetW whs) {
+    const size_t S = g.S;
+    dp_column_in<MX>(g, f, p, lev, pole, dt, nflxr, nflxpt, whs, LV(f.Rhos, lev), LV(f.ps, lev),
+                     LV(f.SlowRho, lev));
+}
+
 // Vertical_Eq (thor_vertical_int.h:52-563) with the stage's factorisation
 // (right-hand side C0 from the substep's sources, forward elimination into
 // Whs as the reference's dd, back substitution Whs_k = -cc'_k Whs_{k+1} + dd_k,
@@ -2974,7 +3428,7 @@ __device__ __forceinline__ void dp_column(const Geo &g, const Fields &f, int p, 
 // arithmetic. The loads of the factorisation are spread over all rows and the
 // serial chain is a quarter of the one-lane sweep's. All threads call it.
 constexpr int kSeg = 12;
-template <int CB>
+template <int CB, int KS = kSeg>
 __device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const Plane &AA, const Plane &CC,
                                                    const Plane &TT, int nv, int c, int row, bool valid) {
     constexpr int NROW = 256 / CB;
@@ -2985,11 +3439,11 @@ __device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const
     // back substitution's coefficients are loaded here too, so both sweeps
     // wait for memory once
     double a = 1.0, b = 0.0;
-    double ccl[kSeg];
+    double ccl[KS];
     {
-        double al[kSeg], tl[kSeg], cl[kSeg];
+        double al[KS], tl[KS], cl[KS];
 #pragma unroll
-        for (int i = 0; i < kSeg; i++) {
+        for (int i = 0; i < KS; i++) {
             const int l = lo + i;
             const bool in = valid && l < hi;
             al[i] = in ? AA[l] : 0.0;
@@ -2998,7 +3452,7 @@ __device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const
             cl[i] = in ? sD[l * CB + c] : 0.0;
         }
 #pragma unroll
-        for (int i = 0; i < kSeg; i++) {
+        for (int i = 0; i < KS; i++) {
             const int l = lo + i;
             if (l < hi) {
                 if (l == 1) {
@@ -3018,16 +3472,16 @@ __device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const
     double x = 0.0;  // x at interface lo - 1
     for (int r = 0; r < row; r++) x = sa[r * CB + c] * x + sb[r * CB + c];
     {
-        double al[kSeg], tl[kSeg];
+        double al[KS], tl[KS];
 #pragma unroll
-        for (int i = 0; i < kSeg; i++) {
+        for (int i = 0; i < KS; i++) {
             const int l = lo + i;
             const bool in = valid && l < hi;
             al[i] = in ? AA[l] : 0.0;
             tl[i] = in ? TT[l] : 1.0;
         }
 #pragma unroll
-        for (int i = 0; i < kSeg; i++) {
+        for (int i = 0; i < KS; i++) {
             const int l = lo + i;
             if (l < hi) {
                 x = l == 1 ? (-sD[CB + c]) / tl[i] : (-sD[l * CB + c] - x * al[i]) * tl[i];
@@ -3040,7 +3494,7 @@ __device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const
     double e = 1.0, fo = 0.0;
     {
 #pragma unroll
-        for (int i = kSeg - 1; i >= 0; i--) {
+        for (int i = KS - 1; i >= 0; i--) {
             const int l = lo + i;
             if (l < hi) {
                 e = -ccl[i] * e;
@@ -3053,7 +3507,7 @@ __device__ __forceinline__ void thomas_sweeps_rows(double *sD, double *sX, const
         double w = 0.0;  // w at interface hi (0 at the top, interface nv)
         for (int r = NROW - 1; r > row; r--) w = sa[r * CB + c] * w + sb[r * CB + c];
 #pragma unroll
-        for (int i = kSeg - 1; i >= 0; i--) {
+        for (int i = KS - 1; i >= 0; i--) {
             const int l = lo + i;
             if (l < hi) {
                 w = (-ccl[i] * w + sD[l * CB + c]);
@@ -3175,6 +3629,109 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
     for (int lev = row; lev < nv; lev += NROW)
         dp_column<MX>(g, f, p, lev, pole, deltat, LV(f.hflx_r, lev), LV(f.hflx_pt, lev),
                       [&](int k) { return sD[k * CB + c]; });
+}
+
+// Mixed path, segment form of k_thomas_dp: each row of the block owns a
+// contiguous segment of interfaces [lo, hi) of its column in all three phases
+// (the segments of thomas_sweeps_rows) and the cells lo - 1 .. hi - 1 under and
+// above them. The right-hand side of interface l reads the cells l - 1 and l,
+// all in the row's segment, and the cells' Rhos, ps, SlowRho and hflx_r stay
+// in registers for the density/pressure update, which the row then applies to
+// the cells lo - 1 .. hi - 2 (and nv - 1 on the top row): no cell value is
+// read twice from HBM. C0, the sweeps and the update are k_thomas_dp's
+// arithmetic; the results equal k_thomas_dp<CB, true>'s with the same rows.
+// SEG: interfaces per row at most; the host launches it with
+// (nv - 1 + NROW - 1) / NROW <= SEG. DP = false: the vertical solve alone
+// (Whs written; k_dp applies the update after a physics hook).
+template <int CB, int SEG, bool DP = true>
+__global__ __launch_bounds__(256) void k_thomas_dp_seg(Geo g, Fields f, double deltat) {
+    constexpr int NROW = 256 / CB;
+    extern __shared__ double sm[];
+    double *sD = sm;
+    const int nv = g.nv, t = threadIdx.x, c = t % CB, row = t / CB;
+    const int p = blockIdx.x * CB + c;
+    const bool valid = p < g.P;
+    const size_t S = g.S;
+    const int seg = (nv - 1 + NROW - 1) / NROW;
+    const int lo = 1 + row * seg, hi = min(nv, lo + seg);
+    const int ncell = hi >= lo ? hi - lo + 1 : 0;  // cells lo - 1 .. hi - 1
+    const double A = g.A, Gravit = g.Gravit, Rd = g.Rd, Cp = g.Cp;
+    const double *Alt = g.Alt, *Alth = g.Alth;
+    const bool ghv = g.ghv, nonh = g.nonhydro;
+    const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
+    const double dt2 = XPOW(deltat, 2.0);
+    double cR[SEG + 1], cP[SEG + 1], cS[SEG + 1], cF[SEG + 1], cSp[SEG + 1];
+#pragma unroll
+    for (int i = 0; i <= SEG; i++) {
+        const int lev = lo - 1 + i;
+        const bool in = valid && i < ncell;
+        cR[i] = in ? LV(f.Rhos, lev) : 0.0;
+        cP[i] = in ? LV(f.ps, lev) : 0.0;
+        cS[i] = in ? LV(f.SlowRho, lev) : 0.0;
+        cF[i] = in ? LV(f.hflx_r, lev) : 0.0;
+        cSp[i] = in ? LV(f.Sp, lev) : 0.0;
+    }
+#pragma unroll
+    for (int i = 1; i <= SEG; i++) {
+        const int lev = lo - 1 + i;  // interface
+        if (valid && i < ncell) {
+            const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
+            const double Sp = cSp[i], Sd = -cF[i] + cS[i], pr = cP[i], r = cR[i];
+            const double Spl = cSp[i - 1], Sdl = -cF[i - 1] + cS[i - 1], pl = cP[i - 1], rl = cR[i - 1];
+            const double dzh = 1.0 / (alt - altl);
+            const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
+            const double dSpdz = (Sp - Spl) * dzh, dPdz = (pr - pl) * dzh;
+            const double intt = (alth - alt) / (altl - alt), intl = (alth - altl) / (alt - altl);
+            const double rhohs = rl * intt + r * intl;
+            const double Sdh = Sdl * intt + Sd * intl;
+            const double srh = LV(f.SlowWh, lev);
+            double C0;
+            if (!nonh) {
+                if (ghv)
+                    C0 = (dt2 * dSpdz + dt2 * Gravit * XPOW(A / (A + alt), 2) * Sdh
+                          + Gravit * XPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
+                         * (CRdd);
+                else
+                    C0 = (dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                          - deltat * srh)
+                         * (CRdd);
+            } else {
+                const double whs = LV(f.Whs, lev);
+                if (ghv)
+                    C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * XPOW(A / (A + alt), 2) * Sdh
+                          + Gravit * XPOW(A / (A + alt), 2) * deltat * rhohs + deltat * dPdz - deltat * srh)
+                         * (CRdd);
+                else
+                    C0 = (-whs + dt2 * dSpdz + dt2 * Gravit * Sdh + Gravit * deltat * rhohs + deltat * dPdz
+                          - deltat * srh)
+                         * (CRdd);
+            }
+            sD[lev * CB + c] = C0;
+        }
+    }
+    __syncthreads();
+    const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
+    thomas_sweeps_rows<CB, SEG>(sD, sm + (size_t)(nv + 1) * CB, AA, CC, TT, nv, c, row, valid);
+    if (row == 0) {
+        sD[c] = 0.0;
+        sD[nv * CB + c] = 0.0;
+    }
+    __syncthreads();
+    if (!valid) return;
+    {
+        const Plane W{f.Whs, S, p};
+        for (int lev = row; lev <= nv; lev += NROW) W[lev] = sD[lev * CB + c];
+    }
+    if (!DP) return;
+    const bool pole = p >= g.P - 2;
+    const int nlast = hi == nv ? ncell : ncell - 1;  // the top row also updates cell nv - 1
+#pragma unroll
+    for (int i = 0; i < SEG + 1; i++) {
+        const int lev = lo - 1 + i;
+        if (i < nlast)
+            dp_column_in<true>(g, f, p, lev, pole, deltat, cF[i], LV(f.hflx_pt, lev),
+                               [&](int k) { return sD[k * CB + c]; }, cR[i], cP[i], cS[i]);
+    }
 }
 
 // Column part of Density_Pressure_Eqs after a separate vertical solve

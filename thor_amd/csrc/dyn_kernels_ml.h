@@ -364,6 +364,34 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_momentum_ml(Geo g, Fields f, i
         if (l0 + l < nv) momentum_tail<false>(g, f, p, l0 + l, nf3[l], DivM[l], dt);
 }
 
+// Divergence-damping pass 1 of the slow modes (DivM of slow_point, FP64) at
+// the tile points, into dm [3][nv][S], for k_slow_lc2<false> (whose pole
+// blocks form the poles' DivM themselves): the same sums as momentum's pass 1.
+template <int NL> __global__ ML_SUB_BOUNDS void k_divm1_ml(Geo g, Fields f, double *dm) {
+    int r, l0;
+    if (!block_tile_ml<NL>(g, r, l0) || r == g.nr) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double *divg = f.divg;
+    __shared__ double sG[NL][3][NHL2];
+    const TileThread T = tile_thread(g, r);
+    stage_tile_ml<NL>(g, r, T, l0, [&](int l, int pos, int q, int lev) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) sG[l][k][pos] = q < 0 ? 0.0 : VEC(divg, k, lev, q);
+    });
+    __syncthreads();
+    double DivM[NL][3];
+    auto none = [&](int, int) { return 0.0; };
+    divm_ml<true, NL>(
+        g, T.p, l0, T.nf, [&](int l, int s, int k) { return sG[l][k][T.pos(s)]; }, none, none, DivM);
+#pragma unroll
+    for (int l = 0; l < NL; l++)
+        if (l0 + l < nv) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) VEC(dm, k, l0 + l, T.p) = DivM[l][k];
+        }
+}
+
 // Prepare_Implicit_Vertical + the horizontal flux divergences of
 // Density_Pressure_Eqs (k_prepare, FP64): prepare_point_w's sums at NL levels.
 template <int NL> __global__ ML_SUB_BOUNDS void k_prepare_ml(Geo g, Fields f) {
