@@ -89,6 +89,70 @@ def measured_traffic(kernel: str, workload: str):
     return k["hbm_bytes_per_launch"]
 
 
+# kernel symbol (demangled, rocprofv3) -> launch name of the library's per-kernel timing
+TRAFFIC_NAMES = [
+    (r"k_profx_relax<1>", "profx_held_suarez"),
+    (r"k_column_prep", "column_prep"),
+    (r"k_adv_diff1", "adv_diff1"),
+    (r"k_diff2_divm0", "diff2_divm0"),
+    (r"k_diff2_lc", "diff2"),
+    (r"k_divm0_lc<true>", "divm0_stage"),
+    (r"k_divm0", "divm0"),
+    (r"k_divm1", "divm1"),
+    (r"k_slow_levels", None),  # the slow modes' level table, once per handle
+    (r"k_slow", "slow_modes"),
+    (r"k_thomas_factor", "thomas_factor"),
+    (r"k_momentum", "momentum"),
+    (r"k_prepare", "prepare"),
+    (r"k_thomas_dp", "thomas_dp"),
+    (r"k_update_rk2", "update_rk2"),
+]
+
+
+def traffic_summary(fetch_dir: str, write_dir: str, workload: str, mib: int, commands: list) -> dict:
+    """profiles/hbm_traffic_<workload>.json from two rocprofv3 counter runs of
+    the same program (FETCH_SIZE, then WRITE_SIZE, in KiB per dispatch, csv
+    output): each kernel's mean per dispatch, each counter divided by what it
+    read for the program's plain copy of `mib` MiB (the largest copy
+    dispatch, --copy-probe-mib)."""
+    import csv
+    import glob
+    import re
+
+    def load(d, counter):
+        rows = []
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            for r in csv.DictReader(open(path)):
+                if r["Counter_Name"] == counter:
+                    rows.append((r["Kernel_Name"], float(r["Counter_Value"])))
+        return rows
+
+    true_kib = mib * 1024.0
+    out = {"workload": workload, "copy_probe_MiB": mib, "commands": commands, "kernels": {}}
+    per = {}
+    for counter, d, key in (("FETCH_SIZE", fetch_dir, "fetch"), ("WRITE_SIZE", write_dir, "write")):
+        rows = load(d, counter)
+        copies = [v for n, v in rows if "copyBuffer" in n or "elementwise" in n or "copy" in n.lower()]
+        if not copies:
+            raise SystemExit(f"no copy dispatch in {d}: run the program with --copy-probe-mib")
+        factor = max(copies) / true_kib
+        out[f"{key}_counter_per_true_byte"] = round(factor, 5)
+        for n, v in rows:
+            name = next((t for rx, t in TRAFFIC_NAMES if re.search(rx, n.replace(" ", ""))), "")
+            if not name:
+                continue
+            e = per.setdefault(name, {"fetch": [], "write": []})
+            e[key].append(v / factor)
+    for name, e in per.items():
+        if not e["fetch"] or not e["write"]:
+            continue
+        rd = sum(e["fetch"]) / len(e["fetch"]) * 1024.0
+        wr = sum(e["write"]) / len(e["write"]) * 1024.0
+        out["kernels"][name] = {"hbm_read_bytes": round(rd), "hbm_write_bytes": round(wr),
+                                "hbm_bytes_per_launch": round(rd + wr), "dispatches": len(e["fetch"])}
+    return out
+
+
 def timed_run(capi, g, p, st, args):
     """K timed steps after W warm-up steps, then per-kernel HIP-event timing."""
     d = capi.Dyn(g, p, variant=args.variant)
@@ -485,7 +549,14 @@ def main():
                          "FP64 + exact RT over --scale-days (surrogate trained on a g5 L48 spin-up)")
     ap.add_argument("--scale-days", type=float, default=5.0)
     ap.add_argument("--scale-glevel", type=int, default=7)
+    ap.add_argument("--traffic-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "WORKLOAD"),
+                    help="instead of running: print the profiles/hbm_traffic_WORKLOAD.json table from two "
+                         "rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of a run with --copy-probe-mib")
     args = ap.parse_args()
+    if args.traffic_summary:
+        fd, wd, wl = args.traffic_summary
+        print(json.dumps(traffic_summary(fd, wd, wl, args.copy_probe_mib or 512, []), indent=1))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the multi-GPU (rhombus-sharded) "
                                    "path is a design only, so --gpus must be 1"}))

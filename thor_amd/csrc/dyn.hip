@@ -146,11 +146,16 @@ struct Dyn {
     double err_sq = 0.0, err_max = 0.0, err_cnt = 0.0;
 };
 
+// Zero-filled device allocations. hipMemset runs on the null stream, which
+// the handle's non-blocking stream does not wait for: the fill is waited for
+// here, so a kernel the handle queues next cannot run before it (and have its
+// output cleared by it).
 static hipError_t dalloc(Dyn &d, double **ptr, size_t n) {
     hipError_t e = hipMalloc((void **)ptr, n * sizeof(double));
     if (e == hipSuccess) {
         d.allocs.push_back(*ptr);
         e = hipMemset(*ptr, 0, n * sizeof(double));
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     }
     return e;
 }
@@ -160,6 +165,7 @@ static hipError_t falloc(Dyn &d, float **ptr, size_t n) {
     if (e == hipSuccess) {
         d.allocs.push_back(*ptr);
         e = hipMemset(*ptr, 0, n * sizeof(float));
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     }
     return e;
 }
@@ -429,8 +435,9 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             // FP64: DivM over several levels per block into the idle
             // hyperdiffusion scratch, then the slow modes over level chunks
             if (divdamp)
-                launch(d, "divm1", W(6, 3), [&] { k_divm1_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f, f.diff); });
-            launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 3 : 0) + 21 + (rk ? 3 : 0)),
+                launch(d, "divm1", W(0, 6), [&] { k_divm1_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f, f.diff); });
+            // booked as the reference's slow modes with DivM pass 1 (as the per-level kernel)
+            launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                    [&] { k_slow_lc2<false><<<grid_lc, blk, 0, st>>>(g, f, f.diff, divdamp, rk, lc); });
         } else {
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
@@ -1007,6 +1014,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         if (e == hipSuccess) {
             d.allocs.push_back(f.flags);
             e = hipMemset(f.flags, 0, sizeof(unsigned int));
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         }
     }
     if (e != hipSuccess) return cleanup_fail(fail(THOR_ERR_HIP, std::string("thor_dyn_create: alloc: ") + hipGetErrorString(e)));
@@ -1238,6 +1246,7 @@ extern "C" thor_status thor_dyn_set_rt_surrogate(thor_dyn *h, thor_mlp *m, const
         if (e == hipSuccess) e = hipMalloc((void **)&h->sur_cnt, 2 * sizeof(int));
         if (e == hipSuccess) h->allocs.push_back(h->sur_cnt);
         if (e == hipSuccess) e = hipMemset(h->sur_cnt, 0, 2 * sizeof(int));
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);  // the fills before the handle's stream uses them
         if (e == hipSuccess) e = dalloc(*h, &h->qsur, (size_t)nv * h->S);
         h->err_nblk = (int)(((long)nv * h->S + 255) / 256);
         if (e == hipSuccess) e = dalloc(*h, &h->err_part, (size_t)3 * h->err_nblk);
@@ -1277,6 +1286,7 @@ extern "C" thor_status thor_dyn_surrogate_stats(thor_dyn *h, long *calls, long *
     HIP_TRY(hipStreamSynchronize(h->stream));
     HIP_TRY(hipMemcpy(&nflag, h->sur_nflag, sizeof(int), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemset(h->sur_nflag, 0, sizeof(int)));
+    HIP_TRY(hipStreamSynchronize(nullptr));  // cleared before the handle's stream counts again
     if (calls) *calls = h->sur_calls;
     if (judged) *judged = h->sur_judged;
     if (fallback_columns) *fallback_columns = nflag;
@@ -1296,6 +1306,7 @@ extern "C" thor_status thor_dyn_sync(thor_dyn *h) {
     HIP_TRY(hipMemcpy(&flags, h->f.flags, sizeof(unsigned int), hipMemcpyDeviceToHost));
     if (flags) {
         HIP_TRY(hipMemset(h->f.flags, 0, sizeof(unsigned int)));
+        HIP_TRY(hipStreamSynchronize(nullptr));  // cleared before the handle's stream can set it again
         if (flags & FLAG_NAN_T) return fail(THOR_ERR_NAN, "NaN in temperature after ProfX (profx_driver.cu:389-396)");
         return fail(THOR_ERR_NAN, "NaN in pressure in Density_Pressure_Eqs (thor_fastmodes.h:562-569)");
     }
@@ -1652,6 +1663,7 @@ extern "C" thor_status thor_tracers_create(thor_dyn *h, int ntr, thor_tracers **
         if (e == hipSuccess) {
             t->allocs.push_back(*p);
             e = hipMemset(*p, 0, n * sizeof(double));
+            if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         }
     }
     if (e != hipSuccess) {

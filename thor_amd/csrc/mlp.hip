@@ -71,6 +71,8 @@ template <class T> static hipError_t alloc(Net &n, T **p, size_t count) {
     if (e == hipSuccess) {
         n.allocs.push_back(*p);
         e = hipMemset(*p, 0, std::max<size_t>(count, 1) * sizeof(T));
+        // the null-stream fill finishes before the network's own stream uses the buffer
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     }
     return e;
 }
