@@ -29,6 +29,9 @@
  *                             ESP::initial_values (src/ESP/esp_initial.cu:68-483, 1292-1392)
  *   thor_dyn_set_sponge    <- RayleighSponge (raysp_calc_mode = imp) of ESP::ProfX,
  *                             src/ESP/profx_driver.cu:92-195, src/ESP/profx_sponge.cu:50-437
+ *   thor_dyn_set_boundary_layer
+ *                          <- the boundary_layer physics module, bl_type = RayleighHS
+ *                             src/physics/modules/src/boundary_layer.cu:186-212,441-490
  *   thor_dyn_set_hooks     <- phy_modules_dyn_core_loop_init / _slow_modes / _fast_modes / _end
  *                             src/headers/phy_modules.h:76-86, called at thor_driver.cu:116,641,1290,1465
  *   thor_dyn_register_rk_array
@@ -365,6 +368,21 @@ thor_status thor_dyn_set_hooks(thor_dyn *h, const thor_hooks *hooks);
 thor_status thor_dyn_register_rk_array(thor_dyn *h, double *array_d, double *arrayk_d, double *arrayi_d,
                                        int dimensions);
 thor_status thor_dyn_clear_rk_arrays(thor_dyn *h);
+
+/* ------------------------------------------------ boundary-layer module */
+
+/* THOR's boundary_layer module with bl_type = RayleighHS
+ * (src/physics/modules/src/boundary_layer.cu:186-212, kernel rayleighHS
+ * :441-490; keys surf_drag, bl_sigma): in every ProfX, after the forcing and
+ * before the new pressure (phy_modules_phy_loop, src/ESP/profx_driver.cu:403-416),
+ *   Mh <- Mh / (1 + kv dt),  kv = surf_drag * max(0, (p/ps - bl_sigma) / (1 - bl_sigma)),
+ * with ps extrapolated from the two lowest layers (GravHeightVar as in the
+ * step). NULL switches it off. */
+typedef struct thor_bl_params {
+    double surf_drag; /* [1/s], THOR's default 1/86400 */
+    double bl_sigma;  /* THOR's default 0.7 */
+} thor_bl_params;
+thor_status thor_dyn_set_boundary_layer(thor_dyn *h, const thor_bl_params *p);
 
 /* ------------------------------------------------- tracer module (chemistry) */
 

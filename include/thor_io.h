@@ -60,6 +60,9 @@ typedef struct thor_run_config {
     /* double-gray radiative transfer (radiative_transfer = true) */
     thor_rt_params rt;
     double Tstar, radius_star, planet_star_dist; /* [K], [R_sun], [au] */
+    /* boundary-layer module (boundary_layer = true, bl_type = RayleighHS) */
+    int boundary_layer;
+    thor_bl_params bl;
 } thor_run_config;
 
 /* Fills r from c; keys with values this build does not run (another

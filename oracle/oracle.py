@@ -83,6 +83,8 @@ def lib(variant: str = ""):
             f.restype = None
         L.oracle_set_sponge.argtypes = [C.c_void_p, _dp]
         L.oracle_set_sponge.restype = None
+        L.oracle_set_boundary_layer.argtypes = [C.c_void_p, _dp]
+        L.oracle_set_boundary_layer.restype = None
         L.oracle_zonal_table.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.c_long, C.POINTER(C.c_int)]
         L.oracle_zonal_table.restype = C.c_int
         L.oracle_profx.argtypes = [C.c_void_p, C.c_int]
@@ -201,6 +203,14 @@ class Oracle:
                      dtype=np.float64)
         a = np.concatenate([a, np.zeros(11 - a.size)])
         self.L.oracle_set_sponge(self._h, a.ctypes.data_as(_dp))
+
+    def set_boundary_layer(self, q):
+        """q: a ctypes BLParams (or surf_drag, bl_sigma); None: off."""
+        if q is None:
+            self.L.oracle_set_boundary_layer(self._h, None)
+            return
+        a = np.array([q.surf_drag, q.bl_sigma] if hasattr(q, "_fields_") else q, dtype=np.float64)
+        self.L.oracle_set_boundary_layer(self._h, a.ctypes.data_as(_dp))
 
     def zonal_table(self):
         """(table [P][3] = bin, points in bin, index in bin; max_count)."""

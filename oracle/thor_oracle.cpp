@@ -58,6 +58,8 @@ struct Oracle {
     double nrq[8] = {0}, rtq[11] = {0};  // forcing parameters
     // Rayleigh sponge: ns, Ruv, Rw, nlat_bins, damp_uv_to_mean, damp_w_to_mean,
     // temp_sponge, RT_sponge
+    bool bl_on = false;
+    double blq[2] = {0};  // boundary layer (RayleighHS): surf_drag, bl_sigma
     bool sponge_on = false;
     double spq[11] = {0};  // ..., [8] raysp_calc_mode: 0 imp, 1 exp1, 3 exp3; [9] shrink_sponge, [10] t_shrink
     long nstep = 0;        // Thor steps completed (the current step is nstep + 1)
@@ -1859,6 +1861,32 @@ void held_suarez(Oracle &o) {
         }
 }
 
+// ------------------------------------------------ boundary-layer module
+// rayleighHS (boundary_layer.cu:441-490), run by phy_modules_phy_loop after
+// the benchmark forcing: implicit drag on Mh below sigma = bl_sigma, with the
+// surface pressure extrapolated from the two lowest layers of p (the pressure
+// before ProfX's Compute_pressure).
+void boundary_layer_rayleigh(Oracle &o, const double *p) {
+    const int nv = o.nv;
+    double *Mh = o.A_("Mh"), *Rho = o.A_("Rho");
+    const double *Alt = o.g.Altitude;
+    const double Gravit = o.p.Gravit, A = o.p.A, dt = o.p.timestep, kf = o.blq[0], sigmab = o.blq[1];
+    ORACLE_PAR
+    for (int id = 0; id < o.P; id++)
+        for (int lev = 0; lev < nv; lev++) {
+            double psm1;
+            if (o.p.GravHeightVar)
+                psm1 = p[id * nv + 1] - Rho[id * nv + 0] * Gravit * MPOW(A / (A + Alt[0]), 2) * (-Alt[0] - Alt[1]);
+            else
+                psm1 = p[id * nv + 1] - Rho[id * nv + 0] * Gravit * (-Alt[0] - Alt[1]);
+            const double ps = 0.5 * (p[id * nv + 0] + psm1);
+            const double sigma = (p[id * nv + lev] / ps);
+            const double kv_hs = kf * std::max(0.0, (sigma - sigmab) / (1.0 - sigmab));
+            for (int k = 0; k < 3; k++)
+                Mh[(id * nv + lev) * 3 + k] = Mh[(id * nv + lev) * 3 + k] / (1.0 + kv_hs * dt);
+        }
+}
+
 // --------------------------------------------------- Newtonian relaxation
 // Hot-Jupiter forcing (no reference counterpart; structure of shallowHJ,
 // profx_shallowHJ.h:51-117): T relaxes implicitly towards
@@ -2419,6 +2447,10 @@ int oracle_kernel(oracle *o, const char *name, int a0, int a1) {
         slow_modes(*o);
     } else if (n == "held_suarez") {
         held_suarez(*o);
+    } else if (n == "boundary_layer") {  // on the current pressure
+        if (!o->bl_on) return -1;
+        std::vector<double> p0 = o->arr["pressure"];
+        boundary_layer_rayleigh(*o, p0.data());
     } else if (n == "vertical_eq") {  // a0: the substep length in milliseconds
         o->capture_tri = a1 != 0;
         vertical_eq(*o, a0 * 1e-3);
@@ -2475,13 +2507,22 @@ int oracle_zonal_table(oracle *o, int *dst, long n, int *max_count) {
     *max_count = o->max_count;
     return 0;
 }
+void oracle_set_boundary_layer(oracle *o, const double q[2]) {
+    o->bl_on = q != nullptr;
+    if (q) std::memcpy(o->blq, q, sizeof(o->blq));
+}
 int oracle_profx(oracle *o, int forcing) {
     if (forcing == 0) return 0;
     if (o->sponge_imp()) sponge(*o);
+    // the physics modules see the pressure ProfX started from (THOR computes
+    // the new one after them, profx_driver.cu:403-445)
+    std::vector<double> p0;
+    if (o->bl_on) p0 = o->arr["pressure"];
     if (forcing == 1) profx_hs(*o);
     else if (forcing == 2) profx_nr(*o);
     else if (forcing == 3) profx_rt(*o);
     else return -1;
+    if (o->bl_on) boundary_layer_rayleigh(*o, p0.data());
     return 0;
 }
 int oracle_run(oracle *o, int n, int forcing) {

@@ -99,6 +99,9 @@ void oracle_set_rt(oracle *o, const double q[11]);
 void oracle_set_sponge(oracle *o, const double q[6]);
 /* The sponge's latitude-bin table [P][3] (bin, points in bin, index in bin). */
 int oracle_zonal_table(oracle *o, int *dst, long n, int *max_count);
+/* Boundary-layer module (RayleighHS) after the ProfX forcing: q = {surf_drag,
+ * bl_sigma}; NULL switches it off. */
+void oracle_set_boundary_layer(oracle *o, const double q[2]);
 int oracle_profx(oracle *o, int forcing);
 // n x (ProfX with forcing + Thor).
 int oracle_run(oracle *o, int n, int forcing);

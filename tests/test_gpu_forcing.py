@@ -343,3 +343,52 @@ def test_sponge_many_slot_blocks_bitwise(built):
     print("sponge, 2 bands", {k: f"{v:.2e}" for k, v in errs.items()})
     assert max(errs.values()) == 0.0
     d.close()
+
+
+@pytest.mark.parametrize("forcing,ghv", [(1, 0), (1, 1), (2, 0), (3, 0)])
+def test_boundary_layer_steps_bitwise(built, forcing, ghv):
+    """THOR's boundary-layer module (RayleighHS) after the ProfX forcing, on
+    the pressure ProfX started from: fused into the Held-Suarez and Newtonian
+    ProfX kernels, a kernel of its own before the double-gray RT. The parity
+    build equals the oracle bit for bit over two ProfX + Thor steps."""
+    from oracle.oracle import Oracle
+    from common import earth_params, grid
+    capi = built
+    if forcing == 1:
+        g = grid(capi, 4, 20)
+        p = earth_params(capi, 4, dt=600.0, GravHeightVar=ghv)
+        st = capi.ic_rest(g, p, seed=1021, temperature_noise=1.0, wind_noise=1.0)
+    else:
+        g = hj_grid(capi, 4, 20)
+        p = hj_params(capi, 4, GravHeightVar=ghv)
+        st = capi.ic_rest(g, p, seed=1022, temperature_noise=1.0, wind_noise=0.1)
+    print(f"case: parity build g4 L20, forcing {forcing}, GravHeightVar {ghv}, boundary layer, 2 steps", flush=True)
+    bl = capi.BLParams(surf_drag=1.0 / 86400.0, bl_sigma=0.7)
+    d = capi.Dyn(g, p, variant="pm")
+    o = Oracle(g, p, variant="pm")
+    for x in (d, o):
+        x.set_state(st)
+        x.set_boundary_layer(bl)
+        if forcing == 2:
+            x.set_newtonian(nr_params(capi))
+        if forcing == 3:
+            x.set_rt(rt_params(capi))
+    d.run(2, forcing)
+    d.sync()
+    o.run(2, forcing)
+    gs, os_ = d.get_state(), o.get_state()
+    errs = {k: _max_rel(gs[k], os_[k]) for k in gs}
+    print("boundary layer", {k: f"{v:.2e}" for k, v in errs.items()})
+    assert max(errs.values()) == 0.0
+    # the module acted: the same steps without it differ
+    d2 = capi.Dyn(g, p, variant="pm")
+    d2.set_state(st)
+    if forcing == 2:
+        d2.set_newtonian(nr_params(capi))
+    if forcing == 3:
+        d2.set_rt(rt_params(capi))
+    d2.run(2, forcing)
+    d2.sync()
+    assert _max_rel(d2.get_state()["Mh"], gs["Mh"]) > 1e-8
+    d.close()
+    d2.close()

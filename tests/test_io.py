@@ -46,7 +46,7 @@ def _run_config(built):
             ("simulation_ID", C.c_char * 160), ("results_path", C.c_char * 512), ("initial", C.c_char * 512),
             ("GPU_ID_N", C.c_int), ("RayleighSponge", C.c_int), ("sponge", capi.SpongeParams),
             ("rt", capi.RTParams), ("Tstar", C.c_double), ("radius_star", C.c_double),
-            ("planet_star_dist", C.c_double)]
+            ("planet_star_dist", C.c_double), ("boundary_layer", C.c_int), ("bl", capi.BLParams)]
     return RunConfig
 
 
@@ -77,7 +77,14 @@ def test_config_values_defaults_and_unknown_keys(io, built, tmp_path):
     ce = io.Config(str(bad))
     io._check(io.lib().thor_config_run(ce._h, C.byref(rc)))
     assert rc.params.thermo_equation == built.THERMO_ENERGY
-    for text, word in (("thermo_equation = enthalpy\n", "thermo_equation"), ("core_benchmark = JetSteady\n",
+    # boundary layer (boundary_layer.cu:339-362): RayleighHS with THOR's defaults
+    bad.write_text("boundary_layer = true\nbl_type = RayleighHS\nsurf_drag = 2e-5\n")
+    cl = io.Config(str(bad))
+    assert cl.unknown() == []
+    io._check(io.lib().thor_config_run(cl._h, C.byref(rc)))
+    assert rc.boundary_layer == 1 and rc.bl.surf_drag == 2e-5 and rc.bl.bl_sigma == 0.7
+    for text, word in (("boundary_layer = true\nbl_type = LocalMixL\n", "bl_type"),
+                       ("thermo_equation = enthalpy\n", "thermo_equation"), ("core_benchmark = JetSteady\n",
                                                                              "JetSteady"),
                        ("vert_refined = true\n", "vert_refined"), ("glevel = four\n", "glevel")):
         bad.write_text(text)

@@ -70,6 +70,11 @@ class SpongeParams(C.Structure):
                 ("t_shrink", C.c_int)]
 
 
+class BLParams(C.Structure):
+    """thor_bl_params: boundary-layer module, RayleighHS type."""
+    _fields_ = [("surf_drag", C.c_double), ("bl_sigma", C.c_double)]
+
+
 class StepView(C.Structure):
     """thor_step_view: device pointers of the step handed to physics hooks."""
     _fields_ = [("plane", C.c_long), ("point_num", C.c_int), ("nv", C.c_int), ("mhs_fp32", C.c_int)] + \
@@ -184,6 +189,7 @@ def lib(variant: str = "") -> C.CDLL:
                                                C.POINTER(C.c_long), _dp, _dp]
             L.thor_dyn_time_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
             L.thor_dyn_set_sponge.argtypes = [C.c_void_p, C.POINTER(SpongeParams)]
+            L.thor_dyn_set_boundary_layer.argtypes = [C.c_void_p, C.POINTER(BLParams)]
             L.thor_dyn_set_hooks.argtypes = [C.c_void_p, C.POINTER(Hooks)]
             L.thor_tracers_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]
             L.thor_tracers_destroy.argtypes = [C.c_void_p]
@@ -434,6 +440,10 @@ class Dyn:
 
     def set_sponge(self, q: "SpongeParams | None"):
         self._chk(self.L.thor_dyn_set_sponge(self._h, C.byref(q) if q is not None else None))
+
+    def set_boundary_layer(self, q: "BLParams | None"):
+        """THOR's boundary_layer module (RayleighHS) in every ProfX; None: off."""
+        self._chk(self.L.thor_dyn_set_boundary_layer(self._h, C.byref(q) if q is not None else None))
 
     def set_hooks(self, init=None, slow_modes=None, fast_modes=None, end=None):
         """Installs Python physics hooks: init(stream, nstep, view), slow_modes /

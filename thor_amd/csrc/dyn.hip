@@ -126,6 +126,7 @@ struct Dyn {
     // forcings
     bool nr_set = false, rt_set = false;
     NRPar nr{};
+    BLPar bl{};  // boundary-layer module (RayleighHS), off by default
     RTPar rt{};
     double *insol = nullptr, *asr = nullptr, *olr = nullptr;  // [P] RT column diagnostics
     // RT surrogate
@@ -589,15 +590,20 @@ static int profx(Dyn &d, int forcing) {
         });
         f.skip_w = 1;
     }
+    // The boundary-layer module runs after the forcing, on the pressure ProfX
+    // started from: fused into the relaxation forcings; before the RT forcings,
+    // which leave Mh alone (the same numbers as after them).
+    if (d.bl.on && forcing != THOR_FORCING_HELD_SUAREZ && forcing != THOR_FORCING_NEWTONIAN)
+        launch(d, "boundary_layer", 8 * B, [&] { k_bl_rayleigh<<<grid_c, blk, 0, d.stream>>>(g, f, d.bl, dt); });
     if (forcing == THOR_FORCING_HELD_SUAREZ) {
         launch(d, "profx_held_suarez", 24 * B, [&] {
-            k_profx_relax<THOR_FORCING_HELD_SUAREZ><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr);
+            k_profx_relax<THOR_FORCING_HELD_SUAREZ><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr, d.bl);
         });
     } else if (forcing == THOR_FORCING_NEWTONIAN) {
         if (!d.nr_set) return fail(THOR_ERR_BAD_ARG, "profx: Newtonian forcing parameters not set");
         // W, T, relaxation and pressure per cell: Wh 2, p, rho, Mh untouched -> 4 in, W, p, Qheat out
         launch(d, "profx_newtonian", 7 * B, [&] {
-            k_profx_relax<THOR_FORCING_NEWTONIAN><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr);
+            k_profx_relax<THOR_FORCING_NEWTONIAN><<<grid_c, blk, 0, d.stream>>>(g, f, dt, 1, d.nr, d.bl);
         });
     } else if (forcing == THOR_FORCING_DOUBLE_GRAY) {
         if (!d.rt_set) return fail(THOR_ERR_BAD_ARG, "profx: RT parameters not set");
@@ -1190,6 +1196,21 @@ extern "C" thor_status thor_dyn_set_newtonian(thor_dyn *h, const thor_newtonian_
     n.lntau_top = std::log(q->tau_top);
     n.lntau_bot = std::log(q->tau_bottom);
     h->nr_set = true;
+    return THOR_OK;
+}
+
+extern "C" thor_status thor_dyn_set_boundary_layer(thor_dyn *h, const thor_bl_params *q) {
+    using namespace thor;
+    if (!h) return fail(THOR_ERR_BAD_ARG, "set_boundary_layer: null handle");
+    if (!q) {
+        h->bl.on = 0;
+        return THOR_OK;
+    }
+    if (!(q->surf_drag >= 0 && q->bl_sigma >= 0 && q->bl_sigma < 1))
+        return fail(THOR_ERR_BAD_ARG, "set_boundary_layer: need surf_drag >= 0 and 0 <= bl_sigma < 1");
+    h->bl.on = 1;
+    h->bl.surf_drag = q->surf_drag;
+    h->bl.bl_sigma = q->bl_sigma;
     return THOR_OK;
 }
 

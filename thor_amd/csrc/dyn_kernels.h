@@ -3828,8 +3828,47 @@ __device__ __forceinline__ void nr_profile(const NRPar &q, double lon, double la
 // Block of 64 columns x 4 level rows; the surface pressure of each column is
 // read before any level of it is overwritten (one barrier), then every
 // (column, level) is independent.
+// Boundary-layer module, RayleighHS type (boundary_layer.cu:441-490): an
+// implicit Rayleigh drag on the momentum of the layers near the surface,
+// kv = surf_drag max(0, (sigma - bl_sigma) / (1 - bl_sigma)) with sigma = p / ps,
+// ps extrapolated from the two lowest layers as the reference does (with the
+// gravity factor under GravHeightVar). Runs in ProfX after the forcing and
+// before the new pressure (phy_modules_phy_loop, profx_driver.cu:403-416), so
+// p is the pressure ProfX started from.
+struct BLPar {
+    int on;
+    double surf_drag, bl_sigma;
+};
+__device__ __forceinline__ double bl_surface_pressure(const Geo &g, const Fields &f, int p) {
+    const size_t S = g.S;
+    const double *Alt = g.Alt, Gravit = g.Gravit, A = g.A;
+    const double psm1 = g.ghv ? LV(f.p, 1) - LV(f.Rho, 0) * Gravit * XPOW(A / (A + Alt[0]), 2) * (-Alt[0] - Alt[1])
+                              : LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
+    return 0.5 * (LV(f.p, 0) + psm1);
+}
+__device__ __forceinline__ void bl_drag(const Geo &g, const Fields &f, const BLPar &bl, int p, int lev, double pre,
+                                        double ps, double time_step) {
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double sigma = (pre / ps);
+    const double kv_hs = bl.surf_drag * fmax(0.0, (sigma - bl.bl_sigma) / (1.0 - bl.bl_sigma));
+#pragma unroll
+    for (int k = 0; k < 3; k++) VEC(f.Mh, k, lev, p) = VEC(f.Mh, k, lev, p) / (1.0 + kv_hs * time_step);
+}
+// The module alone (forcings that leave Mh alone run after it: the same numbers).
+__global__ __launch_bounds__(256) void k_bl_rayleigh(Geo g, Fields f, BLPar bl, double time_step) {
+    constexpr int CB = 64, NROW = 4;
+    const int c = threadIdx.x % CB, row = threadIdx.x / CB;
+    const int p = blockIdx.x * CB + c;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const double ps = bl_surface_pressure(g, f, p);
+    for (int lev = row; lev < g.nv; lev += NROW) bl_drag(g, f, bl, p, lev, LV(f.p, lev), ps, time_step);
+}
+
 template <int F>
-__global__ __launch_bounds__(256) void k_profx_relax(Geo g, Fields f, double time_step, int zero_q, NRPar nr) {
+__global__ __launch_bounds__(256) void k_profx_relax(Geo g, Fields f, double time_step, int zero_q, NRPar nr,
+                                                     BLPar bl) {
     constexpr int CB = 64, NROW = 4;
     const int c = threadIdx.x % CB, row = threadIdx.x / CB;
     const int p = blockIdx.x * CB + c;
@@ -3841,12 +3880,13 @@ __global__ __launch_bounds__(256) void k_profx_relax(Geo g, Fields f, double tim
     const double kappa = Rd / Cp;
     const double sigmab = 0.7, ka = (1.0 / 40.0) * (1.0 / 86400.0), kf = 1.0 / 86400.0,
                  ks = (1.0 / 4.0) * (1.0 / 86400.0), dTy = 60.0, dthetaz = 10.0, p0 = 100000.0;
-    double ps = 0.0, lat = 0.0, lon = 0.0;
+    double ps = 0.0, lat = 0.0, lon = 0.0, ps_bl = 0.0;
     if (valid) {
         lon = g.lonlat[p];
         lat = g.lonlat[S + p];
         const double psm1 = LV(f.p, 1) - LV(f.Rho, 0) * Gravit * (-Alt[0] - Alt[1]);
         ps = 0.5 * (LV(f.p, 0) + psm1);
+        if (bl.on) ps_bl = bl_surface_pressure(g, f, p);
     }
     __syncthreads();
     if (!valid) return;
@@ -3874,6 +3914,7 @@ __global__ __launch_bounds__(256) void k_profx_relax(Geo g, Fields f, double tim
             nr_profile(nr, lon, lat, pre, Teq, tau);
             T = (T + time_step * Teq / tau) / (1.0 + time_step / tau);
         }
+        if (bl.on) bl_drag(g, f, bl, p, lev, pre, ps_bl, time_step);  // the module after the forcing
         nan_t |= isnan(T);
         LV(f.p, lev) = T * Rd * rho;
         if (zero_q) LV(f.Qheat, lev) = 0.0;

@@ -50,7 +50,7 @@ const std::set<std::string> &known_keys() {
         "raysp_calc_mode", "shrink_sponge", "t_shrink", "DiffSponge", "Dv_sponge", "ns_diff_sponge",
         "order_diff_sponge", "radiative_transfer", "rt_type", "Tstar", "planet_star_dist", "radius_star", "albedo",
         "diff_ang", "n_lw", "n_sw", "kappa_lw", "kappa_sw", "f_lw", "Tint", "latf_lw", "kappa_lw_pole", "surface",
-        "Csurf", "rt1Dmode", "sync_rot", "boundary_layer", "bl_type", "GPU_ID_N", "output_diffusion",
+        "Csurf", "rt1Dmode", "sync_rot", "boundary_layer", "bl_type", "surf_drag", "bl_sigma", "GPU_ID_N", "output_diffusion",
         "out_interm_momentum", "single_column", "MetStar", "bv_freq", "a_dense", "b_dense", "tausw", "taulw",
         "dgrt_spinup_start", "dgrt_spinup_stop", "dgrt_spindown_stop", "n_substeps", "precision"};
     return k;
@@ -407,8 +407,17 @@ extern "C" thor_status thor_config_run(const thor_config *c, thor_run_config *r)
         return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: vert_refined must be false (uniform vertical grid)");
     if (!must("conv_adj", "false")) return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: conv_adj must be false");
     if (!must("surface", "false")) return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: surface must be false");
-    if (!must("boundary_layer", "false"))
-        return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: boundary_layer must be false");
+    // boundary-layer module (boundary_layer.cu:339-362, defaults boundary_layer.h:104-105)
+    B("boundary_layer", r->boundary_layer);
+    r->bl.surf_drag = 1.0 / 86400.0;
+    r->bl.bl_sigma = 0.7;
+    if (r->boundary_layer) {
+        if (!must("bl_type", "RayleighHS"))
+            return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: bl_type must be RayleighHS (LocalMixL is not run)");
+        D("surf_drag", r->bl.surf_drag);
+        D("bl_sigma", r->bl.bl_sigma);
+        if (!bad.empty()) return fail(THOR_ERR_BAD_ARG, "thor_config_run: bad value for key " + bad);
+    }
     if (!must("ultrahot_thermo", "none") || !must("ultrahot_heating", "none"))
         return fail(THOR_ERR_UNSUPPORTED, "thor_config_run: ultrahot_thermo / ultrahot_heating must be none");
     if (!must("output_mean", "false") && get("output_mean"))

@@ -151,6 +151,7 @@ int main(int argc, char **argv) {
     thor_state st = {0, Mh.data(), Wh.data(), W.data(), Rho.data(), pr.data()};
     if (thor_dyn_set_state(dyn, &st) != THOR_OK) return die("state", 1);
     if (r.RayleighSponge && thor_dyn_set_sponge(dyn, &r.sponge) != THOR_OK) return die("sponge", 1);
+    if (r.boundary_layer && thor_dyn_set_boundary_layer(dyn, &r.bl) != THOR_OK) return die("boundary layer", 1);
     if (r.forcing == THOR_FORCING_DOUBLE_GRAY && thor_dyn_set_rt(dyn, &r.rt) != THOR_OK) return die("RT", 1);
     const int forcing = r.forcing;
 
