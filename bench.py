@@ -364,12 +364,16 @@ def at_scale_run(capi, args):
     substeps, Rayleigh sponge, seeded +-0.1 m/s wind noise: mixed-precision
     dynamics with the MLP surrogate (exact RT for out-of-envelope columns, judge
     every 10 calls) against the FP64 form with the exact RT, from the same state
-    over the same simulated days, and the mixed form with the exact RT beside
-    them to separate the two sources of difference. The network is trained on a
-    g5 L48 exact-RT spin-up: its inputs (T, ln p, cos z per column) and outputs
-    do not depend on the horizontal grid. Reports SDPD, the step's algorithmic
-    HBM rate and the differences from the FP64 run at the end, point by point
-    and of the zonal means (5-degree bands, per level)."""
+    over the same simulated days, with the mixed form with the exact RT beside
+    them to separate the two sources of difference, and an FP64 twin started
+    from the state with every pressure one ulp up: the flow's own sensitivity to
+    a last-bit change, which the other differences are set against. The network
+    is trained on a g5 L48 exact-RT spin-up: its inputs (T, ln p, cos z per
+    column) and outputs do not depend on the horizontal grid. The four runs
+    advance a simulated day at a time (each timed on its own); each day the
+    differences from the FP64 run are reported point by point and of the zonal
+    means (5-degree bands, per level). Reports SDPD and the step's algorithmic
+    HBM rate per form."""
     from thor_amd import surrogate as sg
     nv, gl, dt = 48, args.scale_glevel, 120.0
     m, norm, train = _spinup_and_train(capi, args, nv)
@@ -378,27 +382,63 @@ def at_scale_run(capi, args):
     pmx = sg.hot_jupiter_params(gl, dt=dt, ns=6, precision=1)
     log(f"grid g{gl} L{nv}: P={g.P} in {time.perf_counter() - t0:.1f} s")
     st = capi.ic_rest(g, p64, seed=1010, temperature_noise=0.0, wind_noise=0.1)
+    st_p = dict(st)
+    st_p["pressure"] = np.nextafter(st["pressure"], np.inf)
     nsteps = int(round(args.scale_days * 86400.0 / dt))
-    forms = {"fp64_exact_rt": (p64, capi.FORCING_DOUBLE_GRAY), "mixed_exact_rt": (pmx, capi.FORCING_DOUBLE_GRAY),
-             "mixed_surrogate": (pmx, capi.FORCING_RT_SURROGATE)}
-    out, final = {}, {}
-    for name, (prm, forcing) in forms.items():
+    per_day = int(round(86400.0 / dt))
+    forms = {"fp64_exact_rt": (p64, capi.FORCING_DOUBLE_GRAY, st),
+             "fp64_ulp_exact_rt": (p64, capi.FORCING_DOUBLE_GRAY, st_p),
+             "mixed_exact_rt": (pmx, capi.FORCING_DOUBLE_GRAY, st),
+             "mixed_surrogate": (pmx, capi.FORCING_RT_SURROGATE, st)}
+    ds, walls = {}, {}
+    for name, (prm, forcing, s0) in forms.items():
         d = capi.Dyn(g, prm, variant=args.variant)
-        d.set_state(st)
+        d.set_state(s0)
         d.set_rt(sg.double_gray())
         d.set_sponge(sg.hot_jupiter_sponge())
         if forcing == capi.FORCING_RT_SURROGATE:
             d.set_rt_surrogate(m, norm, judge_every=10)
-        chunk, done, wall = 720, 0, 0.0
-        while done < nsteps:
-            n = min(chunk, nsteps - done)
+        ds[name], walls[name] = d, 0.0
+    lat = np.asarray(g.lonlat).reshape(g.P, 2)[:, 1]
+    band = np.minimum(((lat + np.pi / 2) / np.pi * 36).astype(int), 35)
+    cnt = np.bincount(band, minlength=36)
+
+    def zonal(x):  # zonal mean per (5-degree latitude band, level)
+        return np.stack([np.bincount(band, weights=x[:, l], minlength=36) / cnt for l in range(x.shape[1])], 1)
+
+    def diffs(a, b):
+        err = {}
+        for k in ("T", "u", "v", "w"):
+            diff = b[k] - a[k]
+            zd = zonal(b[k]) - zonal(a[k])
+            err[k] = {"rms": float(f"{np.sqrt((diff ** 2).mean()):.4e}"), "max": float(f"{np.abs(diff).max():.4e}"),
+                      "zonal_mean_rms": float(f"{np.sqrt((zd ** 2).mean()):.4e}")}
+        return err
+
+    done, days = 0, []
+    while done < nsteps:
+        n = min(per_day, nsteps - done)
+        for name, (prm, forcing, _) in forms.items():
             t1 = time.perf_counter()
-            d.run(n, forcing)
-            d.sync()
-            wall += time.perf_counter() - t1
-            done += n
-            log(f"{name}: {done}/{nsteps} steps ({done * dt / 86400:.1f} days), {wall:.1f} s")
-        final[name] = d.get_state()
+            ds[name].run(n, forcing)
+            ds[name].sync()
+            walls[name] += time.perf_counter() - t1
+        done += n
+        diag = {name: diagnostics(g, p64, d.get_state()) for name, d in ds.items()}
+        a = diag["fp64_exact_rt"]
+        days.append({"day": round(done * dt / 86400.0, 3),
+                     **{f"{o}_vs_fp64": diffs(a, diag[o]) for o in ("fp64_ulp_exact_rt", "mixed_exact_rt",
+                                                                    "mixed_surrogate")}})
+        last = days[-1]
+        log(f"day {last['day']:g}: T rms vs FP64: 1-ulp FP64 {last['fp64_ulp_exact_rt_vs_fp64']['T']['rms']:.3e}, "
+            f"mixed {last['mixed_exact_rt_vs_fp64']['T']['rms']:.3e}, mixed + surrogate "
+            f"{last['mixed_surrogate_vs_fp64']['T']['rms']:.3e} K")
+    a = diagnostics(g, p64, ds["fp64_exact_rt"].get_state())
+    fp64_scale = {k: {"rms": float(f"{np.sqrt((a[k] ** 2).mean()):.4e}"), "max_abs": float(f"{np.abs(a[k]).max():.4e}")}
+                  for k in ("T", "u", "v", "w")}
+    out = {}
+    for name, (prm, forcing, _) in forms.items():
+        d = ds[name]
         sstats = d.surrogate_stats() if forcing == capi.FORCING_RT_SURROGATE else None
         d.kernel_timing(True)
         d.run(10, forcing)
@@ -407,13 +447,13 @@ def at_scale_run(capi, args):
         d.kernel_timing(False)
         d.close()
         step_bytes = sum(k["bytes_per_launch"] * k["launches"] for k in ks if k["name"] not in NOT_REFERENCE) / 10
-        ms = 1e3 * wall / nsteps
+        ms = 1e3 * walls[name] / nsteps
         gbs = step_bytes / (ms * 1e-3) / 1e9
         tot = sum(k["total_ms"] for k in ks)
         top = sorted(ks, key=lambda k: -k["total_ms"])[:6]
         rt_names = ("profx_double_gray", "profx_double_gray_fallback", "surrogate_profx", "sponge_zonal",
                     "sponge_mean", "sponge_apply")
-        out[name] = {"ms_per_step": round(ms, 3), "sdpd": round(dt * nsteps / wall, 1),
+        out[name] = {"ms_per_step": round(ms, 3), "sdpd": round(dt * nsteps / walls[name], 1),
                      "step_alg_bytes": round(step_bytes), "hbm_gbs_alg": round(gbs, 1),
                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
                      "top_kernels_us": {k["name"]: round(1e3 * k["total_ms"] / 10, 1) for k in top},
@@ -425,29 +465,12 @@ def at_scale_run(capi, args):
                                   "fallback_columns_per_call": round(sstats["fallback_columns"] / max(1, sstats["calls"]), 1),
                                   "rms_err_rel_to_rms_heating": round(sstats["rms_err"] / train["rms_heating_W_m3"], 4)}
         log(f"{name}: {out[name]}")
-    lat = np.asarray(g.lonlat).reshape(g.P, 2)[:, 1]
-    band = np.minimum(((lat + np.pi / 2) / np.pi * 36).astype(int), 35)
-    cnt = np.bincount(band, minlength=36)
-
-    def zonal(x):  # zonal mean per (5-degree latitude band, level)
-        return np.stack([np.bincount(band, weights=x[:, l], minlength=36) / cnt for l in range(x.shape[1])], 1)
-
-    a = diagnostics(g, p64, final["fp64_exact_rt"])
-    errs = {}
-    for other in ("mixed_exact_rt", "mixed_surrogate"):
-        b = diagnostics(g, p64, final[other])
-        err = {}
-        for k in ("T", "u", "v", "w"):
-            diff = b[k] - a[k]
-            zd = zonal(b[k]) - zonal(a[k])
-            err[k] = {"rms": float(f"{np.sqrt((diff ** 2).mean()):.4e}"), "max": float(f"{np.abs(diff).max():.4e}"),
-                      "zonal_mean_rms": float(f"{np.sqrt((zd ** 2).mean()):.4e}"),
-                      "fp64_rms": float(f"{np.sqrt((a[k] ** 2).mean()):.4e}"),
-                      "fp64_max_abs": float(f"{np.abs(a[k]).max():.4e}"),
-                      "max_abs": float(f"{np.abs(b[k]).max():.4e}")}
-        err["units"] = {"T": "K", "u": "m/s", "v": "m/s", "w": "m/s"}
-        errs[f"diff_{other}_vs_fp64_exact"] = err
     m.close()
+    last = days[-1]
+    within = {o: {k: {"point": bool(last[f"{o}_vs_fp64"][k]["rms"] <= last["fp64_ulp_exact_rt_vs_fp64"][k]["rms"]),
+                      "zonal": bool(last[f"{o}_vs_fp64"][k]["zonal_mean_rms"]
+                                    <= last["fp64_ulp_exact_rt_vs_fp64"][k]["zonal_mean_rms"])}
+                  for k in ("T", "u", "v", "w")} for o in ("mixed_exact_rt", "mixed_surrogate")}
     return {"metric": "SDPD and algorithmic HBM rate, hot Jupiter g7 L48: mixed-precision dynamics + MLP RT "
                       "surrogate vs FP64 dynamics + exact double-gray RT",
             "workload": f"hot-Jupiter g{gl} L{nv} ({g.P} columns), dt {dt:g} s, 6 substeps, double-gray RT, Rayleigh "
@@ -455,7 +478,8 @@ def at_scale_run(capi, args):
             "steps": nsteps, "surrogate_training": {"spinup": f"g5 L{nv}, {args.rt_spinup_steps} steps of 300 s",
                                                     **train},
             **out, "speedup": round(out["fp64_exact_rt"]["ms_per_step"] / out["mixed_surrogate"]["ms_per_step"], 3),
-            **errs}
+            "fp64_scale": fp64_scale, "within_ulp_spread_at_end": within, "per_day": days,
+            "units": {"T": "K", "u": "m/s", "v": "m/s", "w": "m/s"}}
 
 
 def _zonal_means(g, x, nbands=36):
