@@ -3642,9 +3642,13 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
 // arithmetic; the results equal k_thomas_dp<CB, true>'s with the same rows.
 // SEG: interfaces per row at most; the host launches it with
 // (nv - 1 + NROW - 1) / NROW <= SEG. DP = false: the vertical solve alone
-// (Whs written; k_dp applies the update after a physics hook).
+// (Whs written; k_dp applies the update after a physics hook). Up to three
+// interfaces per row at 16 columns: five waves per SIMD (96 VGPRs, no spills;
+// 79 against 83 us at g6 L48 with four, and six spill); longer rows spill at
+// five and keep the compiler's choice.
 template <int CB, int SEG, bool DP = true>
-__global__ __launch_bounds__(256) void k_thomas_dp_seg(Geo g, Fields f, double deltat) {
+__global__ __launch_bounds__(256, (CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_dp_seg(Geo g, Fields f,
+                                                                                       double deltat) {
     constexpr int NROW = 256 / CB;
     extern __shared__ double sm[];
     double *sD = sm;
