@@ -114,3 +114,44 @@ def test_chunked_slow_modes_equal_per_level_kernel(built, nv, extra):
     for k in STATE:
         assert np.isfinite(a[k]).all()
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("nv,extra", [
+    (20, dict(DeepModel=1, NonHydro=1)),
+    (23, dict(DeepModel=0, NonHydro=0, GravHeightVar=1)),
+    (48, dict(DeepModel=1, NonHydro=1, thermo_equation=1)),
+])
+def test_segment_vertical_solve_equals_fp64_kernel(built, nv, extra):
+    """FP64 vertical solve in segment form (16 columns per block, each row a
+    run of interfaces and the cells below them, values kept for the
+    density/pressure update, one-lane sweeps in the reference's order)
+    against the 64-column kernel: two steps equal bit for bit, with a hook
+    splitting the solve from the update as well."""
+    capi = built
+    print(f"case: FP64 g4 L{nv} {extra}: segment vs 64-column vertical solve, 2 steps", flush=True)
+    g = grid(capi, 4, nv)
+    p = earth_params(capi, 4, dt=600.0, ns=6, **extra)
+    st = capi.ic_rest(g, p, seed=1080 + nv, temperature_noise=1e-3, wind_noise=1.0)
+    a = _run_env(capi, g, p, st, 2, {"THOR_THOMAS_SEG64": "16"})
+    b = _run_env(capi, g, p, st, 2, {"THOR_THOMAS_SEG64": "0"})
+    for k in STATE:
+        assert np.isfinite(a[k]).all()
+        assert np.array_equal(a[k], b[k]), k
+    # with a fast-mode hook the solve runs without the update, then k_dp
+    old = os.environ.get("THOR_THOMAS_SEG64")
+    os.environ["THOR_THOMAS_SEG64"] = "16"
+    try:
+        d = capi.Dyn(g, p)
+    finally:
+        if old is None:
+            del os.environ["THOR_THOMAS_SEG64"]
+        else:
+            os.environ["THOR_THOMAS_SEG64"] = old
+    d.set_hooks(fast_modes=lambda stream, nstep, times, view: 0)
+    d.set_state(st)
+    d.run(2)
+    d.sync()
+    c = d.get_state()
+    d.close()
+    for k in STATE:
+        assert np.array_equal(a[k], c[k]), ("hook split", k)

@@ -19,6 +19,7 @@
 #include <mutex>
 #include <string>
 #include <utility>
+#include <type_traits>
 #include <vector>
 
 namespace thor {
@@ -105,6 +106,9 @@ struct Dyn {
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
     int tseg = 16;         // mixed path: columns per block of the segment Thomas kernel (16 or 32; 0: off)
     bool fseg = true;      // mixed path: segment form of the stage's Thomas factorisation
+    int tseg64 = 0;        // FP64 path: segment Thomas kernel, 16 columns per block (0: the 64-column kernel;
+                           // the segment form is bit for bit the same but slower: its one-lane sweeps run
+                           // on 16 lanes per block, 118 against 97 us at g6 L48)
     bool prep_split = false;  // mixed path: k_prepare_lc takes the stage part of hflx_pt from k_slow_lc
     bool diff2_lc = true;     // mixed path: k_diff2_lc + k_divm0_lc<true> instead of k_diff2_divm0<true>
     int cp_lv = 2;            // levels per thread of the column-preparation kernel (1, 2 or 4)
@@ -485,16 +489,21 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             const dim3 grid_th((d.P + d.tcb - 1) / d.tcb);
             // segment kernel: rows of (nv - 1 + NROW - 1) / NROW interfaces
             const int seg16 = (d.nv - 1 + 15) / 16, seg32 = (d.nv - 1 + 7) / 8;
-            const bool tseg = MX && d.f.thomas_rows && d.nv > 1 && (d.tseg == 16 ? seg16 <= 4 : d.tseg == 32 && seg32 <= 8);
-            const size_t lds_seg = ((size_t)(d.nv + 1) * d.tseg + 2 * 256) * sizeof(double);
+            // mixed: 16 or 32 columns (rows chained); FP64: 16 columns (one-lane sweeps)
+            const int cbs = MX ? d.tseg : d.tseg64;
+            const bool tseg = d.nv > 1 && (MX ? d.f.thomas_rows && (cbs == 16 ? seg16 <= 4 : cbs == 32 && seg32 <= 8)
+                                              : cbs == 16 && seg16 <= 4);
+            const size_t lds_seg = ((size_t)(d.nv + 1) * cbs + 2 * 256) * sizeof(double);
+            auto seg_solve = [&](auto dp) {
+                constexpr bool DP = decltype(dp)::value;
+                const dim3 gs((d.P + cbs - 1) / cbs);
+                if (cbs == 16 && seg16 <= 3) k_thomas_dp_seg<16, 3, DP, MX><<<gs, 256, lds_seg, st>>>(g, f, times);
+                else if (cbs == 16) k_thomas_dp_seg<16, 4, DP, MX><<<gs, 256, lds_seg, st>>>(g, f, times);
+                else if (seg32 <= 6) k_thomas_dp_seg<32, 6, DP, MX><<<gs, 256, lds_seg, st>>>(g, f, times);
+                else k_thomas_dp_seg<32, 8, DP, MX><<<gs, 256, lds_seg, st>>>(g, f, times);
+            };
             if (!d.hooks.dyn_core_loop_fast_modes && tseg) {
-                launch(d, "thomas_dp", W(12 + 18, 4), [&] {
-                    const dim3 gs((d.P + d.tseg - 1) / d.tseg);
-                    if (d.tseg == 16 && seg16 <= 3) k_thomas_dp_seg<16, 3><<<gs, 256, lds_seg, st>>>(g, f, times);
-                    else if (d.tseg == 16) k_thomas_dp_seg<16, 4><<<gs, 256, lds_seg, st>>>(g, f, times);
-                    else if (seg32 <= 6) k_thomas_dp_seg<32, 6><<<gs, 256, lds_seg, st>>>(g, f, times);
-                    else k_thomas_dp_seg<32, 8><<<gs, 256, lds_seg, st>>>(g, f, times);
-                });
+                launch(d, "thomas_dp", W(12 + 18, 4), [&] { seg_solve(std::true_type{}); });
             } else if (!d.hooks.dyn_core_loop_fast_modes) {
                 launch(d, "thomas_dp", W(12 + 18, 4), [&] {
                     if (d.tcb == 64 && d.tur == 2) k_thomas_dp<64, MX, true, 2><<<grid_th, 256, lds, st>>>(g, f, times);
@@ -505,12 +514,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             } else {
                 // the module runs between the vertical solve and the density/pressure update
                 launch(d, "thomas", W(12, 0), [&] {
-                    const dim3 gs((d.P + d.tseg - 1) / d.tseg);
-                    if (tseg && d.tseg == 16 && seg16 <= 3)
-                        k_thomas_dp_seg<16, 3, false><<<gs, 256, lds_seg, st>>>(g, f, times);
-                    else if (tseg && d.tseg == 16) k_thomas_dp_seg<16, 4, false><<<gs, 256, lds_seg, st>>>(g, f, times);
-                    else if (tseg && seg32 <= 6) k_thomas_dp_seg<32, 6, false><<<gs, 256, lds_seg, st>>>(g, f, times);
-                    else if (tseg) k_thomas_dp_seg<32, 8, false><<<gs, 256, lds_seg, st>>>(g, f, times);
+                    if (tseg) seg_solve(std::false_type{});
                     else if (d.tcb == 64) k_thomas_dp<64, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
                     else if (d.tcb == 32) k_thomas_dp<32, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
                     else k_thomas_dp<16, MX, false><<<grid_th, 256, lds, st>>>(g, f, times);
@@ -961,6 +965,10 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     d.f.thomas_rows = 1;
     if (const char *e = std::getenv("THOR_THOMAS_ROWS")) d.f.thomas_rows = std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_FACTOR_SEG")) d.fseg = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_THOMAS_SEG64")) {
+        const int v = std::atoi(e);
+        if (v == 0 || v == 16) d.tseg64 = v;
+    }
     if (const char *e = std::getenv("THOR_THOMAS_SEG")) {
         const int v = std::atoi(e);
         if (v == 0 || v == 16 || v == 32) d.tseg = v;

@@ -3631,7 +3631,7 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
                       [&](int k) { return sD[k * CB + c]; });
 }
 
-// Mixed path, segment form of k_thomas_dp: each row of the block owns a
+// Segment form of k_thomas_dp: each row of the block owns a
 // contiguous segment of interfaces [lo, hi) of its column in all three phases
 // (the segments of thomas_sweeps_rows) and the cells lo - 1 .. hi - 1 under and
 // above them. The right-hand side of interface l reads the cells l - 1 and l,
@@ -3639,16 +3639,19 @@ __global__ __launch_bounds__(256) void k_thomas_dp(Geo g, Fields f, double delta
 // in registers for the density/pressure update, which the row then applies to
 // the cells lo - 1 .. hi - 2 (and nv - 1 on the top row): no cell value is
 // read twice from HBM. C0, the sweeps and the update are k_thomas_dp's
-// arithmetic; the results equal k_thomas_dp<CB, true>'s with the same rows.
+// arithmetic: the mixed form (MX) runs both sweeps on all rows
+// (thomas_sweeps_rows) and equals k_thomas_dp<CB, true> with the same rows;
+// the FP64 form runs them on one lane per column in the reference's order and
+// equals k_thomas_dp<.., false> bit for bit.
 // SEG: interfaces per row at most; the host launches it with
 // (nv - 1 + NROW - 1) / NROW <= SEG. DP = false: the vertical solve alone
 // (Whs written; k_dp applies the update after a physics hook). Up to three
 // interfaces per row at 16 columns: five waves per SIMD (96 VGPRs, no spills;
 // 79 against 83 us at g6 L48 with four, and six spill); longer rows spill at
 // five and keep the compiler's choice.
-template <int CB, int SEG, bool DP = true>
-__global__ __launch_bounds__(256, (CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_dp_seg(Geo g, Fields f,
-                                                                                       double deltat) {
+template <int CB, int SEG, bool DP = true, bool MX = true>
+__global__ __launch_bounds__(256, (MX && CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_dp_seg(Geo g, Fields f,
+                                                                                             double deltat) {
     constexpr int NROW = 256 / CB;
     extern __shared__ double sm[];
     double *sD = sm;
@@ -3664,7 +3667,7 @@ __global__ __launch_bounds__(256, (CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_
     const bool ghv = g.ghv, nonh = g.nonhydro;
     const double CRdd_c = (Cp - Rd) / (Rd * deltat * deltat);
     const double dt2 = XPOW(deltat, 2.0);
-    double cR[SEG + 1], cP[SEG + 1], cS[SEG + 1], cF[SEG + 1], cSp[SEG + 1];
+    double cR[SEG + 1], cP[SEG + 1], cS[SEG + 1], cF[SEG + 1], cSp[SEG + 1], cD[SEG + 1];
 #pragma unroll
     for (int i = 0; i <= SEG; i++) {
         const int lev = lo - 1 + i;
@@ -3674,14 +3677,16 @@ __global__ __launch_bounds__(256, (CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_
         cS[i] = in ? LV(f.SlowRho, lev) : 0.0;
         cF[i] = in ? LV(f.hflx_r, lev) : 0.0;
         cSp[i] = in ? LV(f.Sp, lev) : 0.0;
+        if constexpr (!MX) cD[i] = in ? LV(f.Sd, lev) : 0.0;  // the FP64 prepare writes Sd
     }
 #pragma unroll
     for (int i = 1; i <= SEG; i++) {
         const int lev = lo - 1 + i;  // interface
         if (valid && i < ncell) {
             const double alth = Alth[lev], alt = Alt[lev], altl = Alt[lev - 1];
-            const double Sp = cSp[i], Sd = -cF[i] + cS[i], pr = cP[i], r = cR[i];
-            const double Spl = cSp[i - 1], Sdl = -cF[i - 1] + cS[i - 1], pl = cP[i - 1], rl = cR[i - 1];
+            const double Sp = cSp[i], Sd = MX ? -cF[i] + cS[i] : cD[i], pr = cP[i], r = cR[i];
+            const double Spl = cSp[i - 1], Sdl = MX ? -cF[i - 1] + cS[i - 1] : cD[i - 1], pl = cP[i - 1],
+                         rl = cR[i - 1];
             const double dzh = 1.0 / (alt - altl);
             const double CRdd = (CRdd_c * (alt - alth) + CRdd_c * (alth - altl)) / (alt - altl);
             const double dSpdz = (Sp - Spl) * dzh, dPdz = (pr - pl) * dzh;
@@ -3715,7 +3720,26 @@ __global__ __launch_bounds__(256, (CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_
     }
     __syncthreads();
     const Plane AA{f.tAA, S, p}, CC{f.tCC, S, p}, TT{f.tTT, S, p};
-    thomas_sweeps_rows<CB, SEG>(sD, sm + (size_t)(nv + 1) * CB, AA, CC, TT, nv, c, row, valid);
+    if constexpr (MX) {
+        thomas_sweeps_rows<CB, SEG>(sD, sm + (size_t)(nv + 1) * CB, AA, CC, TT, nv, c, row, valid);
+    } else if (row == 0 && valid) {
+        // one lane per column: forward elimination and back substitution in
+        // the reference's order (the factorisation's loads ahead of the chain)
+        double ddp = (-sD[CB + c]) / TT[1];
+        sD[CB + c] = ddp;
+#pragma unroll THOR_SWEEP_UNROLL
+        for (int lev = 2; lev < nv; lev++) {
+            ddp = (-sD[lev * CB + c] - ddp * AA[lev]) * TT[lev];
+            sD[lev * CB + c] = ddp;
+        }
+        double wup = 0.0;
+#pragma unroll THOR_SWEEP_UNROLL
+        for (int lev = nv - 1; lev > 0; lev--) {
+            const double w = (-CC[lev] * wup + sD[lev * CB + c]);
+            sD[lev * CB + c] = w;
+            wup = w;
+        }
+    }
     if (row == 0) {
         sD[c] = 0.0;
         sD[nv * CB + c] = 0.0;
@@ -3733,7 +3757,7 @@ __global__ __launch_bounds__(256, (CB == 16 && SEG <= 3) ? 5 : 1) void k_thomas_
     for (int i = 0; i < SEG + 1; i++) {
         const int lev = lo - 1 + i;
         if (i < nlast)
-            dp_column_in<true>(g, f, p, lev, pole, deltat, cF[i], LV(f.hflx_pt, lev),
+            dp_column_in<MX>(g, f, p, lev, pole, deltat, cF[i], LV(f.hflx_pt, lev),
                                [&](int k) { return sD[k * CB + c]; }, cR[i], cP[i], cS[i]);
     }
 }
