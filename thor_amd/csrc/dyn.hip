@@ -101,6 +101,17 @@ struct Dyn {
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
     bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
     bool slow_v2 = true;   // mixed path: k_slow_lc2 (two blocks per CU) in place of k_slow_lc
+    // Stage kernels that do not depend on the hyperdiffusion chain can run on
+    // a second stream beside it (no physics hooks installed): the stage
+    // factorisation after the column preparation, and in the mixed form the
+    // stage's divergence-damping passes (into dmbuf) from the stage start.
+    // Bit for bit the same states, but slower on MI355X (3.53 -> 3.68 ms mixed,
+    // 8.46 -> 8.52 ms FP64 at g6 L48: the concurrent kernels slow each other
+    // more than they fill the gaps), so off unless THOR_STAGE_OVERLAP=1.
+    bool overlap = false;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_s[4] = {};
+    float *dmbuf = nullptr;  // mixed form: DivM of the slow modes, [3][nv][S], when overlapped
     bool slow_lc64 = false;  // FP64 path: k_divm1_ml + k_slow_lc2<false> in place of k_slow<false>
     bool fp64_ml = true;   // FP64 path: horizontal stencils over several levels per block (*_ml kernels)
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
@@ -201,7 +212,9 @@ static void drain_timing(Dyn &d) {
 
 // Launches a kernel through fn(); when timing is on, brackets it with events
 // on the handle's stream and books bytes_per_launch of algorithmic traffic.
-template <class Fn> static void launch(Dyn &d, const char *name, double bytes, Fn fn) {
+// fn enqueues its kernels on `on` (the handle's stream when null); with
+// per-kernel timing the events bracket them on that stream.
+template <class Fn> static void launch(Dyn &d, const char *name, double bytes, Fn fn, hipStream_t on = nullptr) {
     if (d.timing) {
         auto &st = d.stats[name];
         if (st.launches == 0 && std::find(d.stat_order.begin(), d.stat_order.end(), name) == d.stat_order.end())
@@ -209,9 +222,9 @@ template <class Fn> static void launch(Dyn &d, const char *name, double bytes, F
         st.launches++;
         st.bytes += bytes;
         Pending pd{name, get_event(d), get_event(d)};
-        hipEventRecord(pd.a, d.stream);
+        hipEventRecord(pd.a, on ? on : d.stream);
         fn();
-        hipEventRecord(pd.b, d.stream);
+        hipEventRecord(pd.b, on ? on : d.stream);
         d.pending.push_back(pd);
         if (d.pending.size() > 2048) drain_timing(d);
     } else {
@@ -297,8 +310,11 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     const int cb = kThomasCB(d.nv);
     const dim3 grid_t((d.P + cb - 1) / cb);
     const size_t lds_t = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
-    hipStream_t st = d.stream;
+    hipStream_t st = d.stream, s2 = d.stream2;
     const int nstep = (int)(d.nstep + 1);
+    // physics hooks may read or write any array of the step: no second stream then
+    const bool overlap_ok = d.overlap && s2 && !d.hooks.dyn_core_loop_init && !d.hooks.dyn_core_loop_slow_modes
+                            && !d.hooks.dyn_core_loop_fast_modes && !d.hooks.dyn_core_loop_end;
     auto view = [&](const Fields &f) {
         thor_step_view v{};
         v.plane = d.S;
@@ -341,11 +357,41 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         const double times = rk == 0 ? d.prm.timestep / 3.0 : d.prm.timestep / (double)ns_total;
         Fields f = stage_fields(d, rk);
         const Geo g = d.g;
+        // second stream: the divergence-damping passes of the slow modes need
+        // only the stage values, the factorisation only the column preparation
+        const bool ovl = overlap_ok;
+        const bool ovl_divm = ovl && MX && d.slow_lc && d.diff2_lc && divdamp;
+        if (ovl) {
+            hipEventRecord(d.ev_s[0], st);  // the stage values are ready
+            hipStreamWaitEvent(s2, d.ev_s[0], 0);
+        }
+        if (ovl_divm) {
+            launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_lc<true><<<grid_lc, blk, 0, s2>>>(g, f, lc); }, s2);
+            launch(d, "divm1", W(0, 6), [&] { k_divm1_lc<<<grid_lc, blk, 0, s2>>>(g, f, d.dmbuf, lc); }, s2);
+            hipEventRecord(d.ev_s[1], s2);  // DivM ready
+        }
         launch(d, "column_prep", W(12 + 5, 6), [&] {
             if (d.cp_lv == 4) k_column_prep_lv<MX, 4><<<grid_c4, blk, 0, st>>>(g, f);
             else if (d.cp_lv == 2) k_column_prep_lv<MX, 2><<<grid_c2, blk, 0, st>>>(g, f);
             else k_column_prep<MX><<<grid_c, blk, 0, st>>>(g, f);
         });
+        auto factor = [&](hipStream_t on) {
+            launch(d, "thomas_factor", 5 * B, [&] {
+                const int fseg = (d.nv - 1 + 7) / 8;  // interfaces per row of the segment form (32 columns)
+                if (MX && d.fseg && d.nv > 1 && fseg <= 8) {
+                    const dim3 gs((d.P + 31) / 32);
+                    if (fseg <= 6) k_thomas_factor_seg<32, 6><<<gs, 256, 0, on>>>(g, f, times);
+                    else k_thomas_factor_seg<32, 8><<<gs, 256, 0, on>>>(g, f, times);
+                } else if (cb == 32) k_thomas_factor<32, MX><<<grid_t, 256, lds_t, on>>>(g, f, times);
+                else k_thomas_factor<16, MX><<<grid_t, 256, lds_t, on>>>(g, f, times);
+            }, on);
+        };
+        if (ovl) {
+            hipEventRecord(d.ev_s[2], st);  // hh and gtilh ready
+            hipStreamWaitEvent(s2, d.ev_s[2], 0);
+            factor(s2);
+            hipEventRecord(d.ev_s[3], s2);  // factorisation ready
+        }
         if (g.energy)
             launch(d, "energy_prep", W(10, 3), [&] {
                 k_energy_prep<<<dim3((d.P + 255) / 256, d.nv + 1), blk, 0, st>>>(g, f);
@@ -380,7 +426,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             // pass 0, each over level chunks
             if (hydiff)
                 launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, fd, last_sign, lc); });
-            if (divdamp)
+            if (divdamp && !ovl_divm)
                 launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_lc<true><<<grid_lc, blk, 0, st>>>(g, f, lc); });
         } else if (hydiff || divdamp) {
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
@@ -428,13 +474,16 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             });
         }
         if (MX && d.slow_lc) {
-            // DivM into the idle hyperdiffusion scratch (its first three planes)
-            if (divdamp)
-                launch(d, "divm1", W(0, 6), [&] { k_divm1_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, lc); });
+            // DivM from the second stream, or into the idle hyperdiffusion
+            // scratch (its first three planes)
+            float *dm = ovl_divm ? d.dmbuf : f.mdiff;
+            if (ovl_divm) hipStreamWaitEvent(st, d.ev_s[1], 0);
+            else if (divdamp)
+                launch(d, "divm1", W(0, 6), [&] { k_divm1_lc<<<grid_lc, blk, 0, st>>>(g, f, dm, lc); });
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                    [&] {
-                       if (d.slow_v2) k_slow_lc2<true><<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc);
-                       else k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, f.mdiff, divdamp, rk, lc);
+                       if (d.slow_v2) k_slow_lc2<true><<<grid_lc, blk, 0, st>>>(g, f, dm, divdamp, rk, lc);
+                       else k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, dm, divdamp, rk, lc);
                    });
         } else if (!MX && d.slow_lc64) {
             // FP64: DivM over several levels per block into the idle
@@ -457,15 +506,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             });
         };
         if (divdamp) divm0();
-        launch(d, "thomas_factor", 5 * B, [&] {
-            const int fseg = (d.nv - 1 + 7) / 8;  // interfaces per row of the segment form (32 columns)
-            if (MX && d.fseg && d.nv > 1 && fseg <= 8) {
-                const dim3 gs((d.P + 31) / 32);
-                if (fseg <= 6) k_thomas_factor_seg<32, 6><<<gs, 256, 0, st>>>(g, f, times);
-                else k_thomas_factor_seg<32, 8><<<gs, 256, 0, st>>>(g, f, times);
-            } else if (cb == 32) k_thomas_factor<32, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
-            else k_thomas_factor<16, MX><<<grid_t, 256, lds_t, st>>>(g, f, times);
-        });
+        if (ovl) hipStreamWaitEvent(st, d.ev_s[3], 0);
+        else factor(st);
         for (int ns = 0; ns < ns_it; ns++) {
             const int next = divdamp && ns < ns_it - 1;
             launch(d, "momentum", W(1, (divdamp ? 6 : 0) + 12), [&] {
@@ -861,6 +903,13 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         || hipEventCreateWithFlags(&d.ev_in, hipEventDisableTiming) != hipSuccess
         || hipEventCreateWithFlags(&d.ev_out, hipEventDisableTiming) != hipSuccess)
         return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: stream"));
+    // the second stream, default priority like the first
+    if (hipStreamCreateWithFlags(&d.stream2, hipStreamNonBlocking) != hipSuccess)
+        return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: second stream"));
+    for (auto &ev : d.ev_s)
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+            return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: stream events"));
+    if (const char *e = std::getenv("THOR_STAGE_OVERLAP")) d.overlap = std::atoi(e) != 0;
     if (hipGetDevice(&d.device) != hipSuccess
         || hipDeviceGetAttribute(&d.ncu, hipDeviceAttributeMultiprocessorCount, d.device) != hipSuccess)
         return cleanup_fail(fail(THOR_ERR_HIP, "thor_dyn_create: device attributes"));
@@ -1051,6 +1100,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         F(&f.mMhs, n3); F(&f.mAdv, n3); F(&f.mdiff, 6 * n1); F(&f.mdiffmh, n3);
         F(&f.mdiffw, n1); F(&f.mdiffrh, n1); F(&f.mdiffpr, n1); F(&f.mdivg, n3); F(&f.mSlowMh, n3);
         F(&f.mhpt, 2 * n1);
+        F(&d.dmbuf, n3);
         if (diff_mid) F(&f.mdiffx, 6 * n1);
         float *wlap = nullptr, *wdd = nullptr, *udd = nullptr, *lvc = nullptr;
         F(&wlap, (size_t)7 * S); F(&wdd, (size_t)63 * S); F(&udd, (size_t)21 * S); F(&lvc, (size_t)4 * nv);
@@ -1096,6 +1146,12 @@ extern "C" void thor_dyn_destroy(thor_dyn *h) {
     for (void *ptr : h->allocs) hipFree(ptr);
     if (h->ev_in) hipEventDestroy(h->ev_in);
     if (h->ev_out) hipEventDestroy(h->ev_out);
+    for (auto ev : h->ev_s)
+        if (ev) hipEventDestroy(ev);
+    if (h->stream2) {
+        hipStreamSynchronize(h->stream2);
+        hipStreamDestroy(h->stream2);
+    }
     if (h->stream) hipStreamDestroy(h->stream);
     delete h;
 }
