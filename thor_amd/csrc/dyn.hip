@@ -98,6 +98,7 @@ struct Dyn {
     bool mixed = false;  // THOR_PRECISION_MIXED
     int lc = 16;         // levels per block of the mixed-precision momentum and divergence stencils
     int lc_prep = 16;    // levels per block of the mixed-precision prepare stencil
+    int lc_slow = 16;    // levels per block of the chunked slow modes
     int tcb = 64;        // columns per block of the substep Thomas kernel (16, 32 or 64)
     bool slow_lc = false;  // mixed path: level-chunked slow modes (k_divm1_lc + k_slow_lc)
     bool slow_v2 = true;   // mixed path: k_slow_lc2 (two blocks per CU) in place of k_slow_lc
@@ -114,6 +115,12 @@ struct Dyn {
     float *dmbuf = nullptr;  // mixed form: DivM of the slow modes, [3][nv][S], when overlapped
     bool slow_lc64 = false;  // FP64 path: k_divm1_ml + k_slow_lc2<false> in place of k_slow<false>
     bool fp64_ml = true;   // FP64 path: horizontal stencils over several levels per block (*_ml kernels)
+    int ml_nl = 4;         // FP64 path: levels per block of the divergence-damping pass 0 and momentum (2 or 4)
+    bool diff2_split64 = false; // FP64 path: the last hyperdiffusion pass and the stage divergence-damping pass 0
+                                // as two kernels (false: k_diff2_divm0_ml, both in one)
+    bool lap_v3 = false;        // FP64 path, split form: the last pass's six Laplacians as two passes of three
+    bool prep64 = true;    // FP64 path: level-chunked prepare k_prepare_lc64 (false: k_prepare_ml; 107.7 -> 63.6 us
+                           // at g6 L48; at three blocks per CU it spills, 95 us)
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
     int tseg = 16;         // mixed path: columns per block of the segment Thomas kernel (16 or 32; 0: off)
     bool fseg = true;      // mixed path: segment form of the stage's Thomas factorisation
@@ -299,7 +306,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
     // mixed-precision substep stencils: one tile x lc levels per block
     const dim3 grid_lc(8 * d.g.tiles_per_xcd * ((d.nv + d.lc - 1) / d.lc));
     const dim3 grid_lp(8 * d.g.tiles_per_xcd * ((d.nv + d.lc_prep - 1) / d.lc_prep));
-    const int lc = d.lc, lcp = d.lc_prep;
+    const dim3 grid_ls(8 * d.g.tiles_per_xcd * ((d.nv + d.lc_slow - 1) / d.lc_slow));
+    const int lc = d.lc, lcp = d.lc_prep, lcs = d.lc_slow;
     // FP64 multi-level stencils: one tile x NL levels per block
     const bool ml = !MX && d.fp64_ml;
     auto grid_ml = [&](int nl) { return dim3(8 * d.g.tiles_per_xcd * ((d.nv + nl - 1) / nl)); };
@@ -428,6 +436,20 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 launch(d, "diff2", W(1, 12 + 12), [&] { k_diff2_lc<<<grid_lc, blk, 0, st>>>(g, fd, last_sign, lc); });
             if (divdamp && !ovl_divm)
                 launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_lc<true><<<grid_lc, blk, 0, st>>>(g, f, lc); });
+        } else if (ml && d.diff2_split64) {
+            // FP64: the last hyperdiffusion pass, then divergence-damping pass 0
+            // on the stage values, each kernel with its own register budget
+            if (hydiff)
+                launch(d, "diff2", W(1, 12 + 12), [&] {
+                    if (d.lap_v3) k_diff2_divm0_ml<2, false, true><<<grid_ml(2), blk, 0, st>>>(g, fd, hydiff, 0, last_sign);
+                    else k_diff2_divm0_ml<2, false><<<grid_ml(2), blk, 0, st>>>(g, fd, hydiff, 0, last_sign);
+                });
+            if (divdamp) {
+                Fields fk = f;
+                fk.Mhs = f.Mhk;
+                fk.Whs = f.Whk;
+                launch(d, "divm0_stage", W(4, 3), [&] { k_divm0_ml<4><<<grid_ml(4), blk, 0, st>>>(g, fk); });
+            }
         } else if (hydiff || divdamp) {
             launch(d, "diff2_divm0", W((hydiff ? 1 : 0) + (divdamp ? 4 : 0), (hydiff ? 12 + 12 : 0) + (divdamp ? 3 : 0)),
                    [&] {
@@ -482,8 +504,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 launch(d, "divm1", W(0, 6), [&] { k_divm1_lc<<<grid_lc, blk, 0, st>>>(g, f, dm, lc); });
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                    [&] {
-                       if (d.slow_v2) k_slow_lc2<true><<<grid_lc, blk, 0, st>>>(g, f, dm, divdamp, rk, lc);
-                       else k_slow_lc<<<grid_lc, blk, 0, st>>>(g, f, dm, divdamp, rk, lc);
+                       if (d.slow_v2) k_slow_lc2<true><<<grid_ls, blk, 0, st>>>(g, f, dm, divdamp, rk, lcs);
+                       else k_slow_lc<<<grid_ls, blk, 0, st>>>(g, f, dm, divdamp, rk, lcs);
                    });
         } else if (!MX && d.slow_lc64) {
             // FP64: DivM over several levels per block into the idle
@@ -492,7 +514,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
                 launch(d, "divm1", W(0, 6), [&] { k_divm1_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f, f.diff); });
             // booked as the reference's slow modes with DivM pass 1 (as the per-level kernel)
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
-                   [&] { k_slow_lc2<false><<<grid_lc, blk, 0, st>>>(g, f, f.diff, divdamp, rk, lc); });
+                   [&] { k_slow_lc2<false><<<grid_ls, blk, 0, st>>>(g, f, f.diff, divdamp, rk, lcs); });
         } else {
             launch(d, "slow_modes", W(19 + (rk ? 16 : 0), (divdamp ? 6 : 0) + 21 + (rk ? 3 : 0)),
                    [&] { k_slow<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, rk); });
@@ -501,6 +523,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
         auto divm0 = [&] {
             launch(d, "divm0", W(1, 6), [&] {
                 if (MX) k_divm0_lc<false><<<grid_lc, blk, 0, st>>>(g, f, lc);
+                else if (ml && d.ml_nl == 2) k_divm0_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f);
                 else if (ml) k_divm0_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f);
                 else k_divm0<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
@@ -512,6 +535,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             const int next = divdamp && ns < ns_it - 1;
             launch(d, "momentum", W(1, (divdamp ? 6 : 0) + 12), [&] {
                 if (MX) k_momentum_lc<<<grid_lc, blk, 0, st>>>(g, f, divdamp, times, lc);
+                else if (ml && d.ml_nl == 2) k_momentum_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f, divdamp, times);
                 else if (ml) k_momentum_ml<4><<<grid_ml(4), blk, 0, st>>>(g, f, divdamp, times);
                 else k_momentum<MX><<<grid_t2, blk, 0, st>>>(g, f, divdamp, times);
             });
@@ -522,6 +546,7 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             launch(d, "prepare", MX && d.prep_split ? W(4, 5) : W(7, 3), [&] {
                 if (MX && d.prep_split) k_prepare_lc<true><<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else if (MX) k_prepare_lc<false><<<grid_lp, blk, 0, st>>>(g, f, lcp);
+                else if (ml && d.prep64) k_prepare_lc64<<<grid_lp, blk, 0, st>>>(g, f, lcp);
                 else if (ml) k_prepare_ml<2><<<grid_ml(2), blk, 0, st>>>(g, f);
                 else k_prepare<MX><<<grid_t2, blk, 0, st>>>(g, f);
             });
@@ -884,7 +909,11 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     // Levels per block of the mixed-precision substep stencils (tuning knob).
     if (const char *e = std::getenv("THOR_STENCIL_LEVELS")) {
         const int v = std::atoi(e);
-        if (v >= 1 && v <= kLcMax) d.lc = d.lc_prep = v;
+        if (v >= 1 && v <= kLcMax) d.lc = d.lc_prep = d.lc_slow = v;
+    }
+    if (const char *e = std::getenv("THOR_SLOW_LEVELS")) {
+        const int v = std::atoi(e);
+        if (v >= 1 && v <= kLcMax) d.lc_slow = v;
     }
     // Columns per block of the substep Thomas kernel (tuning knob).
     if (const char *e = std::getenv("THOR_THOMAS_COLUMNS")) {
@@ -1028,6 +1057,13 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     }
     if (const char *e = std::getenv("THOR_PREPARE_SPLIT")) d.prep_split = d.prep_split && std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_FP64_MULTILEVEL")) d.fp64_ml = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_ML_LEVELS")) {
+        const int v = std::atoi(e);
+        if (v == 2 || v == 4) d.ml_nl = v;
+    }
+    if (const char *e = std::getenv("THOR_PREPARE_LC64")) d.prep64 = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_DIFF2_SPLIT64")) d.diff2_split64 = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_LAP_SPLIT64")) d.lap_v3 = std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_RT_LIST_COLUMNS")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16 || v == 32) d.rt_list_cb = v;

@@ -2018,6 +2018,125 @@ __global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields
     }
 }
 
+// Prepare_Implicit_Vertical (thor_vertical_int.h:566-771) and the horizontal
+// flux divergences of Density_Pressure_Eqs of the FP64 form over level
+// chunks: prepare_point's sums with FP64 inputs, in its operation order and
+// without contraction (bit for bit k_prepare and k_prepare_ml). The divergence
+// weights stay in registers for the whole chunk and the next level's tile is
+// loaded while the current one is summed, as in k_prepare_lc.
+__global__ __launch_bounds__(256, 2) void k_prepare_lc64(Geo g, Fields f, int lc) {
+    int r, l0, l1;
+    if (!block_tile_lc(g, lc, r, l0, l1)) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    const double *Mhs = f.Mhs;
+    if (r == g.nr) {
+        PoleCtx c;
+        int lev;
+        if (pole_ctx_lc(g, l0, l1, c, lev))
+            prepare_point(
+                g, f, c.p, lev, [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.h, lev, c.gi(s)); },
+                [&](int s, int k) { return VEC(Mhs, k, lev, c.gi(s)) + VEC(f.Mhk, k, lev, c.gi(s)); },
+                [&](int s) { return LVP(f.pt, lev, c.gi(s)); });
+        return;
+    }
+    __shared__ double sM[2][3][NHL2], sV1[2][3][NHL2], sRs[kLcMax];
+    __shared__ double2 sHT[2][NHL2];
+    if ((int)threadIdx.x < l1 - l0) sRs[threadIdx.x] = rscale_at(g, l0 + threadIdx.x);
+    const LcThread T = lc_thread(g, r);
+    const int p = T.p;
+    double wv[21];
+#pragma unroll
+    for (int i = 0; i < 21; i++) wv[i] = g.div[(size_t)i * S + p];
+    const size_t pl8 = S * 8;
+    const auto rM = rsrc(Mhs, 3 * nv * pl8), rMk = rsrc(f.Mhk, 3 * nv * pl8), rH = rsrc(f.h, nv * pl8),
+               rPt = rsrc(f.pt, nv * pl8), rSp = rsrc(f.Slowp, nv * pl8), rSr = rsrc(f.SlowRho, nv * pl8);
+    struct In {
+        double slowp, slowrho, om[3], hm[3], omk[3], hmk[3], oh, hh, opt, hpt;
+    };
+    auto load = [&](In &v, int lev) {
+        v.slowp = ldb_d(rSp, T.o8, (unsigned)(lev * pl8));
+        v.slowrho = ldb_d(rSr, T.o8, (unsigned)(lev * pl8));
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v.om[k] = ldb_d(rM, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.hm[k] = ldb_d(rM, T.h8, (unsigned)((k * nv + lev) * pl8));
+            v.omk[k] = ldb_d(rMk, T.o8, (unsigned)((k * nv + lev) * pl8));
+            v.hmk[k] = ldb_d(rMk, T.h8, (unsigned)((k * nv + lev) * pl8));
+        }
+        v.oh = ldb_d(rH, T.o8, (unsigned)(lev * pl8));
+        v.opt = ldb_d(rPt, T.o8, (unsigned)(lev * pl8));
+        v.hh = ldb_d(rH, T.h8, (unsigned)(lev * pl8));
+        v.hpt = ldb_d(rPt, T.h8, (unsigned)(lev * pl8));
+    };
+    auto put = [&](const In &v, int b) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sM[b][k][T.ir] = v.om[k];
+            sV1[b][k][T.ir] = v.om[k] + v.omk[k];
+        }
+        sHT[b][T.ir] = make_double2(v.oh, v.opt);
+        if (T.hp >= 0) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                sM[b][k][T.hp] = v.hm[k];
+                sV1[b][k][T.hp] = v.hm[k] + v.hmk[k];
+            }
+            sHT[b][T.hp] = make_double2(v.hh, v.hpt);
+        }
+    };
+    auto body = [&](int lev, In &R, In &L) {
+        const double slowp = L.slowp, slowrho = L.slowrho;
+        load(L, min(lev + 2, l1 - 1));
+        const int b = (lev - l0) & 1;
+        double sr[3], sp[3], st[3];
+#pragma unroll
+        for (int s = 0; s < 7; s++) {
+            const double2 hq = sHT[b][T.pos[s]];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double dd = wv[s * 3 + k], m = sM[b][k][T.pos[s]], v1 = sV1[b][k][T.pos[s]];
+                if (s == 0) {
+                    sr[k] = dd * m;
+                    sp[k] = dd * m * hq.x;
+                    st[k] = dd * v1 * hq.y;
+                } else {
+                    sr[k] = sr[k] + dd * m;
+                    sp[k] = sp[k] + dd * m * hq.x;
+                    st[k] = st[k] + dd * v1 * hq.y;
+                }
+            }
+        }
+        {
+            const double rs = sRs[lev - l0], RoC = g.Rd / (g.Cp - g.Rd);
+            double nflxr = 0.0, nflxp = 0.0, dr = 0.0, dpt = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                nflxr -= rs * sr[k];
+                nflxp -= (RoC)*rs * sp[k];
+                dr += rs * sr[k];
+                dpt += rs * st[k];
+            }
+            LV(f.Sp, lev) = nflxp + slowp;
+            LV(f.Sd, lev) = nflxr + slowrho;
+            LV(f.hflx_r, lev) = dr;
+            LV(f.hflx_pt, lev) = dpt;
+        }
+        if (lev + 1 < l1) put(R, b ^ 1);
+        __syncthreads();
+    };
+    In A, B;
+    load(A, l0);
+    load(B, min(l0 + 1, l1 - 1));
+    put(A, 0);
+    __syncthreads();
+    for (int lev = l0; lev < l1; lev += 2) {
+        body(lev, B, A);
+        if (lev + 1 < l1) body(lev + 1, A, B);
+    }
+}
+
 // Last Diffusion_Op pass with Correct_Horizontal (thor_diff.h:524-915) of
 // the mixed-precision path over level chunks: the FP32 Laplacian of the first
 // pass's output (variables 1..5 weighted by the stage density, as
