@@ -179,3 +179,23 @@ def test_chunked_prepare_equals_multilevel_kernel(built, nv, extra, env):
     for k in STATE:
         assert np.isfinite(a[k]).all()
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("nv,extra,env", [
+    (20, dict(DeepModel=1, NonHydro=1), {"THOR_DIFF2_SPLIT64": "1"}),
+    (23, dict(DeepModel=0, NonHydro=0, HyDiffOrder=6), {"THOR_DIFF2_SPLIT64": "1"}),
+    (21, dict(DeepModel=1, NonHydro=1, DivDampP=0), {"THOR_DIFF2_SPLIT64": "1"}),
+])
+def test_split_last_diffusion_pass_equals_fused_kernel(built, nv, extra, env):
+    """FP64 last hyperdiffusion pass and stage divergence-damping pass 0 as
+    two kernels against the fused kernel: two steps equal bit for bit."""
+    capi = built
+    print(f"case: FP64 g4 L{nv} {extra} {env}: split vs fused last diffusion pass, 2 steps", flush=True)
+    g = grid(capi, 4, nv)
+    p = earth_params(capi, 4, dt=600.0, ns=6, **extra)
+    st = capi.ic_rest(g, p, seed=1100 + nv, temperature_noise=1e-3, wind_noise=1.0)
+    a = _run_env(capi, g, p, st, 2, env)
+    b = _run_env(capi, g, p, st, 2, {"THOR_DIFF2_SPLIT64": "0"})
+    for k in STATE:
+        assert np.isfinite(a[k]).all()
+        assert np.array_equal(a[k], b[k]), k

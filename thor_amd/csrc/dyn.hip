@@ -116,9 +116,8 @@ struct Dyn {
     bool slow_lc64 = false;  // FP64 path: k_divm1_ml + k_slow_lc2<false> in place of k_slow<false>
     bool fp64_ml = true;   // FP64 path: horizontal stencils over several levels per block (*_ml kernels)
     int ml_nl = 4;         // FP64 path: levels per block of the divergence-damping pass 0 and momentum (2 or 4)
-    bool diff2_split64 = false; // FP64 path: the last hyperdiffusion pass and the stage divergence-damping pass 0
-                                // as two kernels (false: k_diff2_divm0_ml, both in one)
-    bool lap_v3 = false;        // FP64 path, split form: the last pass's six Laplacians as two passes of three
+    bool diff2_split64 = true;  // FP64 path: the last hyperdiffusion pass and the stage divergence-damping pass 0
+                                // as two kernels (false: k_diff2_divm0_ml, both in one; 7.86 -> 7.82 ms)
     bool prep64 = true;    // FP64 path: level-chunked prepare k_prepare_lc64 (false: k_prepare_ml; 107.7 -> 63.6 us
                            // at g6 L48; at three blocks per CU it spills, 95 us)
     int tur = 2;           // unroll of the level loops of the fused Thomas kernel (1 or 2; 64 columns)
@@ -440,10 +439,8 @@ template <bool MX> static int thor_step_t(Dyn &d) {
             // FP64: the last hyperdiffusion pass, then divergence-damping pass 0
             // on the stage values, each kernel with its own register budget
             if (hydiff)
-                launch(d, "diff2", W(1, 12 + 12), [&] {
-                    if (d.lap_v3) k_diff2_divm0_ml<2, false, true><<<grid_ml(2), blk, 0, st>>>(g, fd, hydiff, 0, last_sign);
-                    else k_diff2_divm0_ml<2, false><<<grid_ml(2), blk, 0, st>>>(g, fd, hydiff, 0, last_sign);
-                });
+                launch(d, "diff2", W(1, 12 + 12),
+                       [&] { k_diff2_divm0_ml<2, false><<<grid_ml(2), blk, 0, st>>>(g, fd, hydiff, 0, last_sign); });
             if (divdamp) {
                 Fields fk = f;
                 fk.Mhs = f.Mhk;
@@ -1063,7 +1060,6 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
     }
     if (const char *e = std::getenv("THOR_PREPARE_LC64")) d.prep64 = std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_DIFF2_SPLIT64")) d.diff2_split64 = std::atoi(e) != 0;
-    if (const char *e = std::getenv("THOR_LAP_SPLIT64")) d.lap_v3 = std::atoi(e) != 0;
     if (const char *e = std::getenv("THOR_RT_LIST_COLUMNS")) {
         const int v = std::atoi(e);
         if (v == 4 || v == 8 || v == 16 || v == 32) d.rt_list_cb = v;

@@ -149,28 +149,6 @@ __device__ __forceinline__ void laplacians_ml(const Geo &g, int p, int nf, const
         for (int l = 0; l < NL; l++) out[v][l] = lap[v][l];
 }
 
-// The six Laplacians of laplacians_ml<6> as two passes over three variables
-// each (V3): every variable's arithmetic is unchanged, the triangle geometry
-// is read twice and half as many partial sums are live.
-template <bool V3, int NL, class GetA>
-__device__ __forceinline__ void laplacians6_ml(const Geo &g, int p, int nf, const double rs[NL], GetA a,
-                                               double out[6][NL]) {
-    if constexpr (!V3) {
-        laplacians_ml<6, NL>(g, p, nf, rs, a, out);
-    } else {
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            double o3[3][NL];
-            laplacians_ml<3, NL>(
-                g, p, nf, rs, [&](int v, int l, int s) { return a(3 * h + v, l, s); }, o3);
-#pragma unroll
-            for (int v = 0; v < 3; v++)
-#pragma unroll
-                for (int l = 0; l < NL; l++) out[3 * h + v][l] = o3[v][l];
-        }
-    }
-}
-
 // divm_nf at NL levels l0 .. l0 + NL - 1 (clamped to nv - 1 like the staging):
 // a(l, s, k), wl(l, s), wt(l, s) (LDS reads, called again wherever divm_nf
 // reuses a value); out[l][3]. Tile points only (pole = false).
@@ -592,10 +570,10 @@ template <int NL> __global__ ML_STAGE_BOUNDS void k_adv_diff1_ml(Geo g, Fields f
 // Last hyperdiffusion pass + Correct_Horizontal, and divergence-damping pass 0
 // on the stage values (k_diff2_divm0, FP64). DD = false: the hyperdiffusion
 // pass alone (divergence-damping pass 0 then runs as k_divm0_ml on the stage
-// values), so its registers are sized for the Laplacians only; with V3 (two
-// passes of three Laplacians) it fits three blocks per CU.
-template <int NL, bool DD = true, bool V3 = false>
-__global__ __launch_bounds__(256, V3 ? 3 : THOR_ML_MINB_STAGE) void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, int divdamp, double sign) {
+// values). 323 -> 218 + 104 us at g6 L48, and the following divergence-damping
+// pass 1 runs faster after it (76 -> 69 us).
+template <int NL, bool DD = true>
+__global__ ML_STAGE_BOUNDS void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, int divdamp, double sign) {
     int r, l0;
     if (!block_tile_ml<NL>(g, r, l0)) return;
     if (!DD) divdamp = 0;
@@ -641,7 +619,7 @@ __global__ __launch_bounds__(256, V3 ? 3 : THOR_ML_MINB_STAGE) void k_diff2_divm
             return var == 0 ? sD[l][0][q] : sD[l][var][q] * sR[l][q];
         };
         double lap[6][NL];
-        laplacians6_ml<V3, NL>(g, p, T.nf, rs, a, lap);
+        laplacians_ml<6, NL>(g, p, T.nf, rs, a, lap);
 #pragma unroll
         for (int l = 0; l < NL; l++) {
             if (l0 + l < nv) {
