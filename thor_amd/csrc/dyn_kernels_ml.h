@@ -149,12 +149,26 @@ __device__ __forceinline__ void laplacians_ml(const Geo &g, int p, int nf, const
         for (int l = 0; l < NL; l++) out[v][l] = lap[v][l];
 }
 
+// The per-level factors of divm_nf for the block's NL levels, formed once per
+// block into LDS (sLv[0][l] = rscale, sLv[1][l] = 1 / (Alth[lev] -
+// Alth[lev + 1])) by threads 0 .. NL - 1, before the block's barrier: the
+// triangle loop reads them instead of dividing again for every triangle.
+template <int NL> __device__ __forceinline__ void level_factors_ml(const Geo &g, int l0, double (*sLv)[NL]) {
+    const int t = threadIdx.x;
+    if (t < NL) {
+        const int lev = min(l0 + t, g.nv - 1);
+        sLv[0][t] = rscale_at(g, lev);
+        sLv[1][t] = 1.0 / (g.Alth[lev] - g.Alth[lev + 1]);
+    }
+}
+
 // divm_nf at NL levels l0 .. l0 + NL - 1 (clamped to nv - 1 like the staging):
 // a(l, s, k), wl(l, s), wt(l, s) (LDS reads, called again wherever divm_nf
-// reuses a value); out[l][3]. Tile points only (pole = false).
+// reuses a value); sLv from level_factors_ml; out[l][3]. Tile points only
+// (pole = false).
 template <bool LAST, int NL, class GetA, class GetWl, class GetWt>
 __device__ __forceinline__ void divm_ml(const Geo &g, int p, int l0, int nf, GetA a, GetWl wl, GetWt wt,
-                                        double out[NL][3]) {
+                                        const double (*sLv)[NL], double out[NL][3]) {
     const size_t S = g.S;
     double lapx[NL], lapy[NL], lapz[NL], lap2[NL], dwdz2[NL];
 #pragma unroll
@@ -183,10 +197,9 @@ __device__ __forceinline__ void divm_ml(const Geo &g, int p, int l0, int nf, Get
         }
 #pragma unroll
         for (int l = 0; l < NL; l++) {
-            const int lev = min(l0 + l, g.nv - 1);
-            const double rs = rscale_at(g, lev);
+            const double rs = sLv[0][l];
             const double AT0 = 0.5 * rs;
-            const double dz = LAST ? 0.0 : 1.0 / (g.Alth[lev] - g.Alth[lev + 1]);
+            const double dz = LAST ? 0.0 : sLv[1][l];
             const double a0x = a(l, 0, 0), a0y = a(l, 0, 1), a0z = a(l, 0, 2);
             // slot 1 + j: the previous triangle's third corner
             const double a1x = a(l, s1, 0), a1y = a(l, s1, 1), a1z = a(l, s1, 2);
@@ -281,8 +294,9 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_divm0_ml(Geo g, Fields f) {
         for (int k = 0; k < 3; k++) VEC(divg, k, lev, c.p) = o3[k];
         return;
     }
-    __shared__ double sM[NL][3][NHL2], sW[NL + 1][NHL2];
+    __shared__ double sM[NL][3][NHL2], sW[NL + 1][NHL2], sLv[2][NL];
     const TileThread T = tile_thread(g, r);
+    level_factors_ml<NL>(g, l0, sLv);
     stage_tile_ml<NL>(g, r, T, l0, [&](int l, int pos, int q, int lev) {
 #pragma unroll
         for (int k = 0; k < 3; k++) sM[l][k][pos] = q < 0 ? 0.0 : VEC(Mhs, k, lev, q);
@@ -295,7 +309,8 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_divm0_ml(Geo g, Fields f) {
     double o[NL][3];
     divm_ml<false, NL>(
         g, T.p, l0, T.nf, [&](int l, int s, int k) { return sM[l][k][T.pos(s)]; },
-        [&](int l, int s) { return sW[l][T.pos(s)]; }, [&](int l, int s) { return sW[l + 1][T.pos(s)]; }, o);
+        [&](int l, int s) { return sW[l][T.pos(s)]; }, [&](int l, int s) { return sW[l + 1][T.pos(s)]; }, sLv,
+        o);
 #pragma unroll
     for (int l = 0; l < NL; l++) {
         const int lev = l0 + l;
@@ -322,8 +337,9 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_momentum_ml(Geo g, Fields f, i
             [&](int s) { return LVP(f.ps, lev, c.gi(s)); });
         return;
     }
-    __shared__ double sG[NL][3][NHL2], sP[NL][NHL2];
+    __shared__ double sG[NL][3][NHL2], sP[NL][NHL2], sLv[2][NL];
     const TileThread T = tile_thread(g, r);
+    level_factors_ml<NL>(g, l0, sLv);
     stage_tile_ml<NL>(g, r, T, l0, [&](int l, int pos, int q, int lev) {
         if (divdamp) {
 #pragma unroll
@@ -339,7 +355,7 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_momentum_ml(Geo g, Fields f, i
     if (divdamp) {
         auto none = [&](int, int) { return 0.0; };
         divm_ml<true, NL>(
-            g, p, l0, T.nf, [&](int l, int s, int k) { return sG[l][k][T.pos(s)]; }, none, none, DivM);
+            g, p, l0, T.nf, [&](int l, int s, int k) { return sG[l][k][T.pos(s)]; }, none, none, sLv, DivM);
     }
     double rs[NL], nf3[NL][3];
 #pragma unroll
@@ -373,8 +389,9 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_divm1_ml(Geo g, Fields f, doub
     const size_t S = g.S;
     const int nv = g.nv;
     const double *divg = f.divg;
-    __shared__ double sG[NL][3][NHL2];
+    __shared__ double sG[NL][3][NHL2], sLv[2][NL];
     const TileThread T = tile_thread(g, r);
+    level_factors_ml<NL>(g, l0, sLv);
     stage_tile_ml<NL>(g, r, T, l0, [&](int l, int pos, int q, int lev) {
 #pragma unroll
         for (int k = 0; k < 3; k++) sG[l][k][pos] = q < 0 ? 0.0 : VEC(divg, k, lev, q);
@@ -383,7 +400,7 @@ template <int NL> __global__ ML_SUB_BOUNDS void k_divm1_ml(Geo g, Fields f, doub
     double DivM[NL][3];
     auto none = [&](int, int) { return 0.0; };
     divm_ml<true, NL>(
-        g, T.p, l0, T.nf, [&](int l, int s, int k) { return sG[l][k][T.pos(s)]; }, none, none, DivM);
+        g, T.p, l0, T.nf, [&](int l, int s, int k) { return sG[l][k][T.pos(s)]; }, none, none, sLv, DivM);
 #pragma unroll
     for (int l = 0; l < NL; l++)
         if (l0 + l < nv) {
@@ -592,8 +609,9 @@ __global__ ML_STAGE_BOUNDS void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, in
             [&](int s) { return LVP(f.Whk, lev, c.gi(s)); }, [&](int s) { return LVP(f.Whk, lev + 1, c.gi(s)); });
         return;
     }
-    __shared__ double sD[NL][6][NHL2], sR[NL][NHL2], sM[NL][3][NHL2], sW[NL + 1][NHL2];
+    __shared__ double sD[NL][6][NHL2], sR[NL][NHL2], sM[NL][3][NHL2], sW[NL + 1][NHL2], sLv[2][NL];
     const TileThread T = tile_thread(g, r);
+    if (divdamp) level_factors_ml<NL>(g, l0, sLv);
     stage_tile_ml<NL>(g, r, T, l0, [&](int l, int pos, int q, int lev) {
         if (hydiff) {
 #pragma unroll
@@ -634,7 +652,8 @@ __global__ ML_STAGE_BOUNDS void k_diff2_divm0_ml(Geo g, Fields f, int hydiff, in
         double o[NL][3];
         divm_ml<false, NL>(
             g, p, l0, T.nf, [&](int l, int s, int k) { return sM[l][k][T.pos(s)]; },
-            [&](int l, int s) { return sW[l][T.pos(s)]; }, [&](int l, int s) { return sW[l + 1][T.pos(s)]; }, o);
+            [&](int l, int s) { return sW[l][T.pos(s)]; }, [&](int l, int s) { return sW[l + 1][T.pos(s)]; }, sLv,
+            o);
         double *divg = f.divg;
 #pragma unroll
         for (int l = 0; l < NL; l++) {
