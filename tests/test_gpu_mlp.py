@@ -9,6 +9,8 @@ therefore agree to ~1e-3 relative, and the tests allow 1e-2 (outputs) and
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -203,3 +205,52 @@ def test_surrogate_profx_is_the_network_on_column_features(built, glevel, nv):
     d.close()
     m.close()
 
+
+
+def test_concurrent_fallback_equals_sequential(built):
+    """Surrogate ProfX with the exact RT of the out-of-envelope columns on a
+    second stream beside the network (flags formed first) against the
+    sequential order (network, then the fallback list): six steps with a
+    judge every third call give the same states bit for bit and the same
+    fallback count, and some columns did fall back."""
+    from thor_amd import surrogate as sg
+    capi = built
+    print("case: hot Jupiter g4 L20, surrogate ProfX, concurrent vs sequential fallback, 6 steps", flush=True)
+    g = hj_grid(capi, 4, 20)
+    p = hj_params(capi, 4)
+    st = capi.ic_rest(g, p, seed=1007, temperature_noise=1.0, wind_noise=0.1)
+    d0 = capi.Dyn(g, p)
+    d0.set_state(st)
+    d0.set_rt(rt_params(capi))
+    X, Y = sg.collect(d0, g, nsteps=6, every=2)
+    start = d0.get_state()
+    d0.close()
+    norm = sg.normalisation(X, Y)
+    m = capi.MLP(2 * g.nv + 1, 256, g.nv, seed=9)
+    sg.train(m, *sg.apply_norm(X, Y, norm), epochs=3, batch=256, lr=2e-3)
+    out = {}
+    for conc in ("1", "0"):
+        old = os.environ.get("THOR_SURROGATE_CONCURRENT")
+        os.environ["THOR_SURROGATE_CONCURRENT"] = conc
+        try:
+            d = capi.Dyn(g, p)
+        finally:
+            if old is None:
+                del os.environ["THOR_SURROGATE_CONCURRENT"]
+            else:
+                os.environ["THOR_SURROGATE_CONCURRENT"] = old
+        d.set_state(start)
+        d.set_rt(rt_params(capi))
+        d.set_rt_surrogate(m, norm, judge_every=3)
+        d.run(6, capi.FORCING_RT_SURROGATE)
+        d.sync()
+        out[conc] = (d.get_state(), d.surrogate_stats(), d.get_field("profx_Qheat"))
+        d.close()
+    (a, sa, qa), (b, sb, qb) = out["1"], out["0"]
+    print(f"fallback columns: concurrent {sa['fallback_columns']}, sequential {sb['fallback_columns']}")
+    assert sa["fallback_columns"] == sb["fallback_columns"] > 0
+    for k in a:
+        assert np.isfinite(a[k]).all()
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(qa, qb)
+    m.close()

@@ -162,6 +162,9 @@ struct Dyn {
     int rt_list_cb = 4;  // columns per block of the exact RT over the surrogate's fallback list
     int rt_cb = 32;      // columns per block of the exact RT over all columns (nv <= 63)
     bool sur_fused = true;  // surrogate ProfX as one kernel (else features, network, heating apart)
+    bool sur_conc = false;  // fused surrogate ProfX: the exact RT of the flagged columns on the second stream,
+                            // beside the network (flags formed first by k_sur_flags); bit for bit the same, but
+                            // slower on MI355X (hot Jupiter g5 L40: 1.408 -> 1.491 ms per step)
     double *qsur = nullptr, *err_part = nullptr;  // [nv][S] surrogate Qheat on judge steps; 3 x nblk partials
     int err_nblk = 0;
     long sur_calls = 0, sur_judged = 0, sur_fallback = 0;
@@ -619,11 +622,12 @@ static int thor_step(Dyn &d) { return d.mixed ? thor_step_t<true>(d) : thor_step
 // known only on the device, so its grid covers every column and the blocks
 // past the list's end return at once.
 static void launch_rt(Dyn &d, const Geo &g, const Fields &f, double dt, const int *list = nullptr,
-                      const int *count = nullptr) {
+                      const int *count = nullptr, hipStream_t on = nullptr) {
     const int cb = list ? d.rt_list_cb : (d.nv <= 63 ? d.rt_cb : 16);
     const size_t lds = 4 * (size_t)(d.nv + 1) * cb * sizeof(double);
     const dim3 grid((d.P + cb - 1) / cb);
-    auto go = [&](auto kern) { kern<<<grid, 256, lds, d.stream>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, list, count); };
+    hipStream_t st = on ? on : d.stream;
+    auto go = [&](auto kern) { kern<<<grid, 256, lds, st>>>(g, f, d.rt, dt, d.insol, d.asr, d.olr, list, count); };
     if (cb == 32) go(k_profx_rt<32>);
     else if (cb == 16) go(k_profx_rt<16>);
     else if (cb == 8) go(k_profx_rt<8>);
@@ -690,7 +694,23 @@ static int profx(Dyn &d, int forcing) {
         uint16_t *X = mlp::input(d.sur);
         const float *Y = mlp::output(d.sur);
         mlp::InferWeights iw;
-        if (d.sur_fused && d.nv <= 63 && mlp::fused_weights(d.sur, &iw)) {
+        const bool fused = d.sur_fused && d.nv <= 63 && mlp::fused_weights(d.sur, &iw);
+        // concurrent fallback: the flags first, then the exact RT for the flagged
+        // columns on the second stream while the network takes the others
+        const bool conc = fused && d.sur_conc && !judge;
+        if (conc) {
+            launch(d, "surrogate_flags", 2 * B, [&] {
+                k_sur_flags<<<(d.P + 255) / 256, 256, 0, d.stream>>>(g, f, d.sp, d.sur_mask, d.sur_list, cnt, cnt_next,
+                                                                    d.sur_nflag);
+            });
+            hipEventRecord(d.ev_s[0], d.stream);
+            hipStreamWaitEvent(d.stream2, d.ev_s[0], 0);
+            launch(
+                d, "profx_double_gray_fallback", 14 * B, [&] { launch_rt(d, g, f, dt, d.sur_list, cnt, d.stream2); },
+                d.stream2);
+            hipEventRecord(d.ev_s[1], d.stream2);
+        }
+        if (fused) {
             // features, network and heating in one kernel, one 8-wave block per CU
             launch(d, "surrogate_profx", 10 * B, [&] {
                 const unsigned grid = (unsigned)std::max(1, std::min((d.P + 31) / 32, d.ncu));
@@ -698,7 +718,8 @@ static int profx(Dyn &d, int forcing) {
                 // (n_in tiles, levels per thread) = (kpad / 32, ceil(nv / 16))
                 auto go = [&](auto kern) {
                     kern<<<grid, mlp::kInferThreads, 0, d.stream>>>(g, f, d.sp, iw, d.sur_mask, d.sur_list, cnt,
-                                                                    cnt_next, d.sur_nflag, dt, set_p);
+                                                                    cnt_next, d.sur_nflag, dt, set_p,
+                                                                    conc ? d.sur_mask : nullptr);
                 };
                 const int nl = (d.nv + 15) / 16;
                 switch (iw.kpad / 32) {
@@ -740,6 +761,8 @@ static int profx(Dyn &d, int forcing) {
                 d.err_cnt += part[2 * d.err_nblk + b];
             }
             d.sur_judged++;
+        } else if (conc) {
+            hipStreamWaitEvent(d.stream, d.ev_s[1], 0);  // the fallback columns are done
         } else {
             // the exact RT for the columns outside the envelope only
             launch(d, "profx_double_gray_fallback", 14 * B, [&] { launch_rt(d, g, f, dt, d.sur_list, cnt); });
@@ -1069,6 +1092,7 @@ extern "C" thor_status thor_dyn_create(const thor_grid_desc *gd, const thor_para
         if (v == 4 || v == 8 || v == 16 || v == 32) d.rt_cb = v;
     }
     if (const char *e = std::getenv("THOR_SURROGATE_FUSED")) d.sur_fused = std::atoi(e) != 0;
+    if (const char *e = std::getenv("THOR_SURROGATE_CONCURRENT")) d.sur_conc = std::atoi(e) != 0;
     bool ok = up_i(ring, &g.ring) && up_i(nfc, &g.nfc) && up_i(halo, &g.halo) && up_d(planar(gd->div, 21), &g.div)
               && up_d(planar(gd->grad, 21), &g.grad) && up_d(planar(gd->areas, 18), &g.areas)
               && up_d(planar(gd->nvecoa, 18), &g.noa) && up_d(planar(gd->nvecti, 18), &g.nti)

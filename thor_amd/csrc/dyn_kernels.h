@@ -4282,6 +4282,36 @@ __device__ __forceinline__ uint16_t sur_input(double x, float mean, float inv_st
 }
 __device__ __forceinline__ double sur_lnp(double p) { return MLOG(p); }
 
+// The envelope flags of a surrogate call alone, one thread per column, with
+// the inputs formed as k_sur_persist forms them (T = p / (Rd rho), ln p,
+// cos z): out-of-envelope columns are marked in mask and appended to list, so
+// the exact RT can take them on another stream while k_sur_persist (given
+// the mask) runs the network on the others.
+__global__ __launch_bounds__(256) void k_sur_flags(Geo g, Fields f, SurPar q, int *mask, int *list, int *cnt,
+                                                   int *cnt_next, int *nflag) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *cnt_next = 0;
+    if (p >= g.P) return;
+    const size_t S = g.S;
+    const int nv = g.nv;
+    bool out = false;
+    for (int lev = 0; lev < nv; lev++) {
+        const double pr = LV(f.p, lev);
+        const double T = pr / (g.Rd * LV(f.Rho, lev));
+        sur_input(T, q.in_mean[lev], 1.0f / q.in_std[lev], q.in_lo[lev], q.in_hi[lev], out);
+        sur_input(sur_lnp(pr), q.in_mean[nv + lev], 1.0f / q.in_std[nv + lev], q.in_lo[nv + lev], q.in_hi[nv + lev],
+                  out);
+    }
+    double coszrs = MCOS(g.lonlat[S + p]) * MCOS(g.lonlat[p] - q.lon_sub);
+    if (coszrs < 0.0) coszrs = 0.0;
+    sur_input(coszrs, q.in_mean[2 * nv], 1.0f / q.in_std[2 * nv], q.in_lo[2 * nv], q.in_hi[2 * nv], out);
+    mask[p] = out;
+    if (out) {
+        atomicAdd(nflag, 1);
+        list[atomicAdd(cnt, 1)] = p;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sur_features(Geo g, Fields f, SurPar q, uint16_t *X, int *mask,
                                                       int *list, int *cnt, int *cnt_next, int *nflag) {
     constexpr int CB = 64, NROW = 4;
@@ -4353,7 +4383,8 @@ template <int KT1, int NL>
 __global__ __launch_bounds__(mlp::kInferThreads, 1) void k_sur_persist(Geo g, Fields f, SurPar q,
                                                                        mlp::InferWeights w, int *mask, int *list,
                                                                        int *cnt, int *cnt_next, int *nflag,
-                                                                       double time_step, int set_p) {
+                                                                       double time_step, int set_p,
+                                                                       const int *pre) {
     constexpr int R = 32, LDH = 256 + 8, NROW = mlp::kInferThreads / R;
     __shared__ __align__(16) uint16_t bufA[R * LDH], bufB[R * LDH];
     __shared__ int sout[R];
@@ -4400,7 +4431,9 @@ __global__ __launch_bounds__(mlp::kInferThreads, 1) void k_sur_persist(Geo g, Fi
     load_group(blockIdx.x);
     for (int gi = blockIdx.x; gi < ngroups; gi += gridDim.x) {
         const int p = gi * R + r;
-        const bool valid = p < g.P;
+        // pre: the flags of k_sur_flags; a flagged column is left to the exact RT
+        // entirely (no reads used, no writes)
+        const bool valid = p < g.P && !(pre && pre[p]);
         if (tid < R) sout[tid] = 0;
         __syncthreads();  // sout cleared (and, first time round, the LDS tables written)
         bool out = false;
@@ -4469,7 +4502,7 @@ __global__ __launch_bounds__(mlp::kInferThreads, 1) void k_sur_persist(Geo g, Fi
                     if (set_p) LV(f.p, lev) = T[k] * Rd * rho_c[k];
                 }
             }
-            if (row == 0) {
+            if (row == 0 && !pre) {
                 mask[p] = o;
                 if (o) {
                     atomicAdd(nflag, 1);
