@@ -600,7 +600,10 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
 // (one power per level instead of three). Block i: XCD i % 8, a contiguous
 // range of column blocks per XCD, level groups of one column block
 // consecutive.
-template <bool MX, int LV> __global__ __launch_bounds__(256) void k_column_prep_lv(Geo g, Fields f) {
+#ifndef THOR_COLPREP_MINB
+#define THOR_COLPREP_MINB 1
+#endif
+template <bool MX, int LV> __global__ __launch_bounds__(256, THOR_COLPREP_MINB) void k_column_prep_lv(Geo g, Fields f) {
     const int nv = g.nv, ngrp = (nv + 1 + LV - 1) / LV;
     const int i = blockIdx.x, idx = i >> 3;
     const int pbx = ((g.P + 255) / 256 + 7) / 8;  // column blocks per XCD
@@ -1882,8 +1885,11 @@ __global__ __launch_bounds__(256, 3) void k_momentum_lc(Geo g, Fields f, int div
 // prepare_point (the momentum deviation is read from its FP32 storage).
 // SPLIT: the stage part of the pressure flux divergence comes from
 // f.hflx_ptk (formed by k_slow_lc), so the stage momentum Mhk is not staged.
+#ifndef THOR_PREP_LC_MINB
+#define THOR_PREP_LC_MINB 3
+#endif
 template <bool SPLIT>
-__global__ __launch_bounds__(256, SPLIT ? 3 : 2) void k_prepare_lc(Geo g, Fields f, int lc) {
+__global__ __launch_bounds__(256, SPLIT ? THOR_PREP_LC_MINB : 2) void k_prepare_lc(Geo g, Fields f, int lc) {
 #pragma clang fp contract(fast)
     int r, l0, l1;
     if (!block_tile_lc(g, lc, r, l0, l1)) return;
@@ -2241,8 +2247,11 @@ __global__ __launch_bounds__(256, 3) void k_diff2_lc(Geo g, Fields f, double sig
 // the bottom value of lev + 1, so each level loads one new interface.
 // STAGE: pass 0 of the slow part, on the stage values Mhk / Whk (FP64 storage,
 // rounded to FP32 as diff2_divm0_point does).
+#ifndef THOR_DIVM0_LC_MINB
+#define THOR_DIVM0_LC_MINB 3
+#endif
 template <bool STAGE>
-__global__ __launch_bounds__(256, 3) void k_divm0_lc(Geo g, Fields f, int lc) {
+__global__ __launch_bounds__(256, THOR_DIVM0_LC_MINB) void k_divm0_lc(Geo g, Fields f, int lc) {
 #pragma clang fp contract(fast)
     int r, l0, l1;
     if (!block_tile_lc(g, lc, r, l0, l1)) return;
