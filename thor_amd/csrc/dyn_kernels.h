@@ -600,6 +600,7 @@ template <bool MX> __global__ __launch_bounds__(256) void k_column_prep(Geo g, F
 // (one power per level instead of three). Block i: XCD i % 8, a contiguous
 // range of column blocks per XCD, level groups of one column block
 // consecutive.
+// blocks per CU of the column preparation (5 spills 18 VGPRs: 78 -> 99 us at g6 L48)
 #ifndef THOR_COLPREP_MINB
 #define THOR_COLPREP_MINB 1
 #endif
@@ -1885,6 +1886,7 @@ __global__ __launch_bounds__(256, 3) void k_momentum_lc(Geo g, Fields f, int div
 // prepare_point (the momentum deviation is read from its FP32 storage).
 // SPLIT: the stage part of the pressure flux divergence comes from
 // f.hflx_ptk (formed by k_slow_lc), so the stage momentum Mhk is not staged.
+// blocks per CU of the split mixed prepare (4 spills: 33.7 -> 38.8 us at g6 L48)
 #ifndef THOR_PREP_LC_MINB
 #define THOR_PREP_LC_MINB 3
 #endif
@@ -2247,6 +2249,7 @@ __global__ __launch_bounds__(256, 3) void k_diff2_lc(Geo g, Fields f, double sig
 // the bottom value of lev + 1, so each level loads one new interface.
 // STAGE: pass 0 of the slow part, on the stage values Mhk / Whk (FP64 storage,
 // rounded to FP32 as diff2_divm0_point does).
+// blocks per CU of the mixed divm0 (4 spills: 24 -> 36 us at g6 L48)
 #ifndef THOR_DIVM0_LC_MINB
 #define THOR_DIVM0_LC_MINB 3
 #endif
